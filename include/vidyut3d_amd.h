@@ -1,0 +1,359 @@
+/*
+ * vidyut3d_amd.h -- C ABI of the MI355X (gfx950) implementation of Vidyut's
+ * per-timestep hot path: drift-diffusion WENO face fluxes, species and
+ * electron-energy sources, the variable-coefficient Poisson / implicit
+ * species (Helmholtz) multigrid solves, the CFL reduction and the
+ * refinement tags.
+ *
+ * Every hot-path entry point works on ONE box of a single-level MultiFab,
+ * passed as an Array4-equivalent POD (vd_array4) whose pointer is DEVICE
+ * memory.  The C++ host layer (vidyut3d_amd/host/Vidyut.H) keeps the
+ * reference's Vidyut member signatures and calls these per box.
+ *
+ * All calls are asynchronous on the context's HIP stream unless stated.
+ * No call aborts the process: failures come back as a VD_* status and
+ * vd_last_error() holds a message.
+ */
+#ifndef VIDYUT3D_AMD_H
+#define VIDYUT3D_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- status */
+enum {
+    VD_OK = 0,
+    VD_EINVAL = 1,   /* bad argument: hyp_order, weno_scheme, bc code, shape */
+    VD_ENOCONV = 2,  /* linear solver hit maxiter above its target        */
+    VD_ENOMEM = 3,   /* device allocation failed / serving admission full   */
+    VD_EHIP = 4,     /* HIP runtime error                                   */
+    VD_ENOTFOUND = 5 /* named species / variable not found                 */
+};
+
+/* --------------------------------------------- state component layout
+ * Source/VarDefines.H:5-16 : species 0..NS-1, then the plasma variables. */
+#define VD_NUM_PLASMAVARS 10
+#define VD_EEN(ns) ((ns) + 0)
+#define VD_ETEMP(ns) ((ns) + 1)
+#define VD_POT(ns) ((ns) + 2)
+#define VD_EFX(ns) ((ns) + 3)
+#define VD_EFY(ns) ((ns) + 4)
+#define VD_EFZ(ns) ((ns) + 5)
+#define VD_EJH(ns) ((ns) + 6)
+#define VD_EIH(ns) ((ns) + 7)
+#define VD_EEH(ns) ((ns) + 8)
+#define VD_REF(ns) ((ns) + 9)
+#define VD_NVAR(ns) ((ns) + VD_NUM_PLASMAVARS)
+
+/* boundary-condition codes, Source/VarDefines.H:18-22 */
+enum { VD_PERBC = 0, VD_DIRCBC = 1, VD_HNEUBC = 2, VD_IHNEUBC = 3, VD_ROBINBC = 4 };
+
+/* ------------------------------------------------------------ data model
+ * Array4 semantics: element (i,j,k,n) lives at
+ *   p[(i-begin[0]) + (j-begin[1])*jstride + (k-begin[2])*kstride + n*nstride]
+ * begin/end (exclusive) bound the allocated box INCLUDING ghost cells.
+ * In 2-D, begin[2] = 0 and end[2] = 1. */
+typedef struct vd_array4 {
+    double* p;
+    int begin[3];
+    int end[3];
+    int64_t jstride, kstride, nstride;
+    int ncomp;
+} vd_array4;
+
+/* same layout for the char TagBox (Source/Tagging.H) */
+typedef struct vd_tagarray4 {
+    char* p;
+    int begin[3];
+    int end[3];
+    int64_t jstride, kstride;
+} vd_tagarray4;
+
+/* index box (cell-centred, inclusive) */
+typedef struct vd_box {
+    int lo[3];
+    int hi[3];
+} vd_box;
+
+/* amrex::Geometry of one level */
+typedef struct vd_geom {
+    int dim;            /* AMREX_SPACEDIM: 2 or 3 */
+    int dom_lo[3];
+    int dom_hi[3];
+    double prob_lo[3];
+    double prob_hi[3];
+    double dx[3];
+    int is_periodic[3];
+} vd_geom;
+
+/* The vidyut.* ParmParse keys of the hot path (Source/Vidyut.H:149-272) */
+typedef struct vd_params {
+    double gas_temperature; /* K  (vidyut.gas_temperature) */
+    double gas_pressure;    /* Pa (vidyut.gas_pressure)    */
+    int hyp_order;          /* 1 upwind, 2 WAF-MINBEE, 5 WENO */
+    int weno_scheme;        /* 1 JS, 2 M, 3 Z */
+    int elecenergy_solve;
+    int do_reactions;
+    int do_transport;
+    int do_spacechrg;
+    int do_bg_reactions;
+    int bound_specden;
+    double min_species_density;
+    double min_electron_density;
+    double min_electron_temp;
+    int floor_jh;
+    int const_ele_trans;
+    double ele_mob;
+    double ele_diff;
+    int voltage_profile;    /* 0 DC, 1 sine, 2 triangle (HelperFuncs.H:75-92) */
+    double voltage_amp_1, voltage_amp_2, voltage_freq, voltage_dur, voltage_center;
+    double linsolve_reltol;
+    double linsolve_abstol;
+    int linsolve_maxiter;
+    int linsolve_max_coarsening_level;
+    int mg_pre_smooth;      /* GSRB sweeps before restriction  */
+    int mg_post_smooth;     /* GSRB sweeps after prolongation  */
+    int mg_precision;       /* 0: all-FP64 V-cycles; 1: FP32 V-cycles in FP64 IR */
+    int pot_bc_lo[3], pot_bc_hi[3];
+    int eden_bc_lo[3], eden_bc_hi[3];
+    int eenrg_bc_lo[3], eenrg_bc_hi[3];
+    int ion_bc_lo[3], ion_bc_hi[3];
+    int neutral_bc_lo[3], neutral_bc_hi[3];
+} vd_params;
+
+/* fills the reference defaults of Source/Vidyut.H:149-272 */
+void vd_params_default(vd_params* p);
+
+/* ----------------------------------------------------- case plug-in ABI
+ * Replaces the compile-time Chemistry.H / UserFunctions.H of a case:
+ * species table (CKSYMS_STR, CKCHRG, mw), transport (specMob, specDiff)
+ * and a table-driven reaction mechanism. */
+#define VD_MAX_SPECIES 48
+#define VD_MAX_REACTIONS 256
+#define VD_MAX_REACTANTS 3
+#define VD_MAX_PRODUCTS 4
+
+/* chemistry forms */
+enum {
+    VD_CHEM_TABLE = 0, /* Source/reaction_source.H:10-64: number densities,
+                          rate = k*prod n^nu, energy += e_r*rate          */
+    VD_CHEM_CKWC = 1   /* generated Chemistry.H CKWC/productionRate form:
+                          mol units round trip + comp_ener_exch           */
+};
+
+/* diffusion models per species (specDiff) */
+enum {
+    VD_DIFF_EINSTEIN_TE = 0, /* D = (Te/eV)*|mu|   (electrons)          */
+    VD_DIFF_EINSTEIN_TG = 1, /* D = (Tg/eV)*|mu|   (ions / neutrals)    */
+    VD_DIFF_CONST = 2        /* D = diff[s]                             */
+};
+
+typedef struct vd_reaction {
+    double A;          /* k = A * T^beta * exp(-Ea/T), SI units */
+    double beta;
+    double Ea;         /* K */
+    int temp_is_te;    /* T = Te (1) or Tg (0) */
+    int nreact, nprod;
+    int react_sp[VD_MAX_REACTANTS];
+    double react_nu[VD_MAX_REACTANTS];
+    int prod_sp[VD_MAX_PRODUCTS];
+    double prod_nu[VD_MAX_PRODUCTS];
+    double energy;     /* VD_CHEM_TABLE: J per event into the electron energy */
+    int rxntype;       /* VD_CHEM_CKWC: 0 none, 1 excitation, 2 ionization,
+                          3 recombination, 5 elastic exchange               */
+    double eexci;      /* VD_CHEM_CKWC: eV */
+    int elidx;         /* VD_CHEM_CKWC: collision partner for rxntype 5 */
+} vd_reaction;
+
+typedef struct vd_mech_desc {
+    int nspec;                       /* NUM_SPECIES            */
+    int nreac;                       /* NUM_REACTIONS          */
+    int e_idx;                       /* electron index (E_IDX) */
+    char names[VD_MAX_SPECIES][16];  /* CKSYMS_STR            */
+    int charge[VD_MAX_SPECIES];      /* CKCHRG                */
+    double molwt[VD_MAX_SPECIES];    /* mw(), g/mol           */
+    int is_bg[VD_MAX_SPECIES];       /* vidyut.bg_species_ids */
+    double mob[VD_MAX_SPECIES];      /* signed mobility m^2/V/s (specMob) */
+    int diff_kind[VD_MAX_SPECIES];   /* VD_DIFF_*             */
+    double diff[VD_MAX_SPECIES];     /* for VD_DIFF_CONST     */
+    int chem_form;                   /* VD_CHEM_*             */
+    int elastic_loss;                /* VD_CHEM_CKWC: add the e-neutral elastic loss */
+    double elastic_partner_amu;      /* ... with this partner mass */
+    vd_reaction reac[VD_MAX_REACTIONS];
+} vd_mech_desc;
+
+/* ---------------------------------------------------------- handles */
+typedef struct vd_ctx vd_ctx;
+typedef struct vd_mech vd_mech;
+
+const char* vd_last_error(void);
+const char* vd_version(void);
+
+/* one context per host thread; owns a HIP stream, solver workspaces and
+ * a device-memory pool reused across calls (no allocation after warm-up) */
+int vd_ctx_create(int device, vd_ctx** out);
+int vd_ctx_destroy(vd_ctx* ctx);
+int vd_ctx_sync(vd_ctx* ctx);
+void* vd_ctx_stream(vd_ctx* ctx); /* the hipStream_t, as void* */
+
+int vd_malloc(vd_ctx* ctx, size_t bytes, void** out);
+int vd_free(vd_ctx* ctx, void* p);
+int vd_memcpy_h2d(vd_ctx* ctx, void* dst, const void* src, size_t bytes);
+int vd_memcpy_d2h(vd_ctx* ctx, void* dst, const void* src, size_t bytes);
+int vd_memcpy_d2d(vd_ctx* ctx, void* dst, const void* src, size_t bytes);
+int vd_memset_zero(vd_ctx* ctx, void* dst, size_t bytes);
+
+/* replaces plasmachem::init (Source/PlasmaChem.cpp:12-19) and the case
+ * Chemistry.H tables; copies the descriptor to device memory */
+int vd_mech_create(vd_ctx* ctx, const vd_mech_desc* desc, vd_mech** out);
+int vd_mech_destroy(vd_mech* m);
+/* plasmachem::find_id (Source/PlasmaChem.cpp:21-30); -1 when absent */
+int vd_mech_find_id(const vd_mech* m, const char* name);
+
+/* ---------------------------------------------- bulk MultiFab operations
+ * MultiFab::setVal / MultiFab::Copy / MultiFab::LinComb on the valid box
+ * (Source/Evolve.cpp:105-171,290-300; Source/ScalarSolve.cpp:443-465). */
+int vd_setval(vd_ctx* ctx, vd_array4* a, const vd_box* box, int scomp, int ncomp, double v);
+int vd_copy(vd_ctx* ctx, const vd_array4* src, vd_array4* dst, const vd_box* box, int scomp,
+            int dcomp, int ncomp);
+/* dst[dcomp+n] = a*x[xcomp+n] + b*y[ycomp+n] */
+int vd_lincomb(vd_ctx* ctx, vd_array4* dst, double a, const vd_array4* x, int xcomp, double b,
+               const vd_array4* y, int ycomp, int dcomp, int ncomp, const vd_box* box);
+/* a[comp] = max(a[comp], lo) on the valid box */
+int vd_clamp_min(vd_ctx* ctx, vd_array4* a, const vd_box* box, int comp, double lo);
+
+/* ------------------------------------------------------ hot path
+ * `box` is the valid (cell-centred) box being processed; array views
+ * must cover box grown by the ghost width each call needs. */
+
+/* Vidyut::FillPatch single level (Source/GridWork.cpp:112-124 with the
+ * foextrap / int_dir bcspec of Source/Vidyut.cpp:93-104): copies comps
+ * [scomp, scomp+ncomp) of src (valid data) into dst and fills dst's
+ * ghost cells by periodic wrap or first-order extrapolation. */
+int vd_fill_patch(vd_ctx* ctx, const vd_array4* src, vd_array4* dst, const vd_box* box,
+                  const vd_geom* geom, int scomp, int ncomp);
+
+/* Vidyut::compute_scalar_transport_flux (Source/ScalarSolve.cpp:170-255)
+ * -> compute_flux (Source/compute_explicit_flux.H:200-326) on every face
+ * of `box` in each direction.  sborder needs hyp_order-dependent ghosts
+ * (1, 2, 3 for hyp_order 1, 2, 5).  flux[d] is the face box. */
+int vd_compute_scalar_transport_flux(vd_ctx* ctx, const vd_mech* m, const vd_array4* sborder,
+                                     vd_array4 flux[3], const vd_box* box, const vd_geom* geom,
+                                     const int bc_lo[3], const int bc_hi[3], const vd_params* prm,
+                                     double time, double dt, int specid);
+
+/* Vidyut::compute_dsdt (Source/ScalarSolve.cpp:17-55) */
+int vd_compute_dsdt(vd_ctx* ctx, const vd_array4 flux[3], const vd_array4* rxn_src,
+                    vd_array4* dsdt, const vd_box* box, const vd_geom* geom, int specid);
+
+/* compute_scalar_transport_flux + compute_dsdt fused (Source/ScalarSolve.cpp
+ * :57-111 update_explsrc_at_all_levels for one level): face fluxes stay
+ * on chip; only dsdt is written.  Same results as the two calls above. */
+int vd_transport_rhs(vd_ctx* ctx, const vd_mech* m, const vd_array4* sborder,
+                     const vd_array4* rxn_src, vd_array4* dsdt, const vd_box* box,
+                     const vd_geom* geom, const int bc_lo[3], const int bc_hi[3],
+                     const vd_params* prm, double time, double dt, int specid);
+
+/* Vidyut::update_rxnsrc_at_all_levels (Source/ScalarSolve.cpp:113-168)
+ * with the mechanism's CKWC / reaction_source.H source: rxn_src gets
+ * NS species rates (1/m^3/s) and, at comp NS, the electron energy
+ * source (J/m^3/s). */
+int vd_update_rxnsrc(vd_ctx* ctx, const vd_mech* m, const vd_array4* sborder, vd_array4* rxn_src,
+                     const vd_box* box, const vd_params* prm, double time);
+
+/* Vidyut::compute_elecenergy_source (Source/ElecEnergySource.cpp:16-173):
+ * Joule heating from face E and face grad(n_e); dsdt += heat; writes
+ * phi_new EJH and EIH. */
+int vd_compute_elecenergy_source(vd_ctx* ctx, const vd_mech* m, const vd_array4* sborder,
+                                 const vd_array4* rxn_src, const vd_array4 efield[3],
+                                 const vd_array4 gradne[3], vd_array4* dsdt, vd_array4* phi_new,
+                                 const vd_box* box, const vd_geom* geom, const vd_params* prm);
+
+/* solver report */
+typedef struct vd_solve_info {
+    int iterations;       /* outer (V-cycle / IR) iterations */
+    double resnorm0;      /* max-norm of the initial residual */
+    double resnorm;       /* max-norm of the final residual   */
+    double target;        /* max(abstol, reltol*resnorm0)     */
+} vd_solve_info;
+
+/* Vidyut::solve_potential (Source/PotentialSolve.cpp:19-393): rhs
+ * -e/eps0*sum z n, Dirichlet/Neumann/periodic/Robin BCs with applied
+ * voltage, multigrid solve, E = -grad(phi) on faces (efield), cell
+ * averages into phi_new EFX..EFZ, potential into POT and E/N into REF. */
+int vd_solve_potential(vd_ctx* ctx, const vd_mech* m, const vd_array4* sborder, vd_array4* phi_new,
+                       vd_array4 efield[3], const vd_box* box, const vd_geom* geom,
+                       const vd_params* prm, double time, vd_solve_info* info);
+
+/* Vidyut::implicit_solve_scalar (Source/ScalarSolve.cpp:289-680):
+ * (1/dt) n - div(D grad n) = n_old/dt + dsdt_expl with harmonic face D,
+ * species boundary conditions, floors, grad n_e on faces (electrons) and
+ * the Te update (electron energy).  Writes phi_new comp `spec`. */
+int vd_implicit_solve_scalar(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sborder,
+                             const vd_array4* phi_old, const vd_array4* dsdt_expl,
+                             vd_array4* phi_new, vd_array4 grad_fc[3], const vd_box* box,
+                             const vd_geom* geom, const int bc_lo[3], const int bc_hi[3],
+                             const vd_params* prm, double time, double dt, vd_solve_info* info);
+
+/* Vidyut::find_time_scales (Source/TimeStep.cpp:34-116): wave64
+ * max-reductions; out = {dt_edrift, dt_ediff, dt_diel_relax}.  Blocks
+ * until the three scalars are on the host. */
+int vd_find_time_scales(vd_ctx* ctx, const vd_mech* m, const vd_array4* phi_new, const vd_box* box,
+                        const vd_geom* geom, const vd_params* prm, double out[3]);
+
+/* Vidyut::ErrorEst kernels (Source/Tagging.H:8-69, Source/Vidyut.cpp:
+ * 221-253): state needs 1 ghost; tags[cell] = tagval where
+ * |q| > refine_phi[c] or the one-sided gradient exceeds refine_phigrad[c]. */
+int vd_error_est(vd_ctx* ctx, const vd_array4* state, vd_tagarray4* tags, const vd_box* box,
+                 int ntagvars, const int* comps, const double* refine_phi,
+                 const double* refine_phigrad, char tagval);
+
+/* General cell-centred ABecLaplacian solve, the operator of
+ * MLABecLaplacian as used by PotentialSolve.cpp / ScalarSolve.cpp:
+ *   alpha_a*phi - beta*div(bface grad phi) = rhs
+ * bface[d] are face arrays (NULL => constant 1), bc per side with values
+ * in bcval (NULL => homogeneous): Dirichlet value, inhomogeneous-Neumann
+ * d(phi)/dx_d, or Robin (robin_a, robin_b, robin_f; NULL => not Robin).
+ * phi holds the initial guess on entry (valid region). */
+typedef struct vd_mlmg_problem {
+    double alpha_a;
+    double beta;
+    const vd_array4* bface[3];
+    int bc_lo[3], bc_hi[3];
+    const vd_array4* bcval;     /* ghost cells of a 1-ghost array hold BC data */
+    const vd_array4* robin_a;
+    const vd_array4* robin_b;
+    const vd_array4* robin_f;
+    double reltol, abstol;
+    int maxiter;
+    int precision;              /* 0 FP64, 1 FP32 V-cycles in FP64 IR */
+    int pre_smooth, post_smooth;
+    int max_coarsening_level;
+} vd_mlmg_problem;
+
+int vd_mlmg_solve(vd_ctx* ctx, const vd_mlmg_problem* prob, vd_array4* phi, const vd_array4* rhs,
+                  const vd_box* box, const vd_geom* geom, vd_solve_info* info);
+
+/* ------------------------------------------------- kernel timing (bench)
+ * When enabled for a kernel family, every launch of that family is
+ * bracketed by hipEvents on the context's stream; the totals come back
+ * with vd_ctx_kernel_timing (synchronises). */
+enum {
+    VD_KT_NONE = 0,
+    VD_KT_GSRB = 1,       /* multigrid red-black smoother, finest level */
+    VD_KT_TRANSPORT = 2,  /* fused WENO transport rhs */
+    VD_KT_CHEM = 3,       /* per-cell chemistry source */
+    VD_KT_RESID = 4,      /* multigrid residual, finest level */
+    VD_KT_FILL = 5        /* ghost fill */
+};
+int vd_ctx_set_kernel_timing(vd_ctx* ctx, int family);
+int vd_ctx_kernel_timing(vd_ctx* ctx, double* total_ms, long long* launches, double* cells);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* VIDYUT3D_AMD_H */

@@ -1,0 +1,41 @@
+/*
+ * vidyut3d_amd_host.h -- C handle over the C++ host driver (class
+ * vidyut_amd::Vidyut in vidyut3d_amd/host/Vidyut.H), for callers that drive
+ * whole steps (Source/main.cpp -> Vidyut::InitData / Evolve) instead of the
+ * per-kernel entry points of vidyut3d_amd.h.
+ */
+#ifndef VIDYUT3D_AMD_HOST_H
+#define VIDYUT3D_AMD_HOST_H
+
+#include "vidyut3d_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct vd_sim vd_sim;
+
+/* Vidyut::Vidyut + the level-0 MultiFabs (single box covering the domain) */
+int vd_sim_create(vd_ctx* ctx, const vd_mech_desc* mech, const vd_geom* geom, const vd_params* prm,
+                  vd_sim** out);
+int vd_sim_destroy(vd_sim* s);
+/* whole state (NVAR comps, valid box, component-major, i fastest) */
+int vd_sim_set_state(vd_sim* s, const double* host, double time);
+int vd_sim_get_state(vd_sim* s, double* host);
+/* device view of phi_new (no ghost cells) */
+int vd_sim_state_view(vd_sim* s, vd_array4* out);
+/* Vidyut::Evolve for nsteps at fixed dt with ncorr correctors */
+int vd_sim_evolve(vd_sim* s, int nsteps, double dt, int ncorr);
+double vd_sim_time(const vd_sim* s);
+long long vd_sim_mg_iterations(const vd_sim* s);
+/* Vidyut::WritePlotFile (Source/Plot.cpp:29-38): AMReX HyperCLaw-V1.1 */
+int vd_sim_write_plotfile(vd_sim* s, const char* dir);
+/* ErrorEst on phi_new (1-ghost FillPatch then the Tagging.H kernels);
+ * tags is a host array over the valid box */
+int vd_sim_error_est(vd_sim* s, int ntag, const int* comps, const double* thr, const double* gthr,
+                     char* host_tags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,196 @@
+"""ctypes binding of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+the cpu_baseline leg of bench.py, never by the product package.
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from vidyut3d_amd import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "liboracle.so")
+_lib = None
+
+P = C.POINTER
+A4 = P(abi.vd_array4)
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.or_weno.restype = C.c_double
+        L.or_weno.argtypes = [C.c_double] * 5 + [C.c_int]
+        L.or_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _ck(st, what, allow=(abi.VD_OK,)):
+    if st not in allow:
+        raise RuntimeError("%s failed: %s" % (what, abi.STATUS_NAMES.get(st, st)))
+    return st
+
+
+class Grid:
+    """Host arrays of one single-box level for the oracle."""
+
+    def __init__(self, geom):
+        self.geom = geom
+        self.box = abi.domain_box(geom)
+        self.dim = geom.dim
+
+    def extent(self, ng=0, face=None):
+        return abi.grown_extent(self.box, ng, self.dim, face)
+
+    def zeros(self, ncomp, ng=0, face=None):
+        b, e = self.extent(ng, face)
+        return np.zeros(abi.shape_of(b, e, ncomp))
+
+    def view(self, arr, ng=0, face=None):
+        b, e = self.extent(ng, face)
+        return abi.host_array4(arr, b, e)
+
+    def faces(self, fill=0.0):
+        return [self.zeros(1, 0, d) + fill for d in range(3)] if self.dim == 3 else \
+            [self.zeros(1, 0, 0) + fill, self.zeros(1, 0, 1) + fill, self.zeros(1, 0, 0) + fill]
+
+    def face_views(self, arrs):
+        dims = [0, 1, 2] if self.dim == 3 else [0, 1, 0]
+        return (abi.vd_array4 * 3)(*[self.view(a, 0, dims[q]) for q, a in enumerate(arrs)])
+
+
+def weno(u, scheme):
+    return lib().or_weno(*[float(x) for x in u], int(scheme))
+
+
+def fill_patch(grid, src, ng):
+    dst = grid.zeros(src.shape[0], ng)
+    s, d = grid.view(src, 0), grid.view(dst, ng)
+    _ck(lib().or_fill_patch(C.byref(s), C.byref(d), C.byref(grid.box), C.byref(grid.geom), 0,
+                            src.shape[0]), "or_fill_patch")
+    return dst
+
+
+def transport_flux(case_mech, prm, grid, sb, ng, specid, bclo, bchi, dt=1e-12):
+    fl = grid.faces()
+    fv = grid.face_views(fl)
+    st = lib().or_compute_scalar_transport_flux(
+        C.byref(case_mech), C.byref(grid.view(sb, ng)), fv, C.byref(grid.box),
+        C.byref(grid.geom), abi.ivec(bclo), abi.ivec(bchi), C.byref(prm), C.c_double(0.0),
+        C.c_double(dt), int(specid))
+    _ck(st, "or_compute_scalar_transport_flux")
+    return fl
+
+
+def dsdt(grid, fl, rxn, specid):
+    out = grid.zeros(1)
+    _ck(lib().or_compute_dsdt(grid.face_views(fl), C.byref(grid.view(rxn)),
+                              C.byref(grid.view(out)), C.byref(grid.box), C.byref(grid.geom),
+                              int(specid)), "or_compute_dsdt")
+    return out
+
+
+def rxn_source(mech, prm, grid, sb, ng):
+    out = grid.zeros(mech.nspec + 1)
+    _ck(lib().or_update_rxnsrc(C.byref(mech), C.byref(grid.view(sb, ng)),
+                               C.byref(grid.view(out)), C.byref(grid.box), C.byref(prm),
+                               C.c_double(0.0)), "or_update_rxnsrc")
+    return out
+
+
+def elec_energy_source(mech, prm, grid, sb, ng, rxn, efield, gradne, dsdt_in, phi_new):
+    ds = dsdt_in.copy()
+    pn = phi_new.copy()
+    _ck(lib().or_compute_elecenergy_source(
+        C.byref(mech), C.byref(grid.view(sb, ng)), C.byref(grid.view(rxn)),
+        grid.face_views(efield), grid.face_views(gradne), C.byref(grid.view(ds)),
+        C.byref(grid.view(pn)), C.byref(grid.box), C.byref(grid.geom), C.byref(prm)),
+        "or_compute_elecenergy_source")
+    return ds, pn
+
+
+def solve_potential(mech, prm, grid, sb, ng, phi_new, time=0.0):
+    pn = phi_new.copy()
+    ef = grid.faces()
+    info = abi.vd_solve_info()
+    st = lib().or_solve_potential(C.byref(mech), C.byref(grid.view(sb, ng)),
+                                  C.byref(grid.view(pn)), grid.face_views(ef),
+                                  C.byref(grid.box), C.byref(grid.geom), C.byref(prm),
+                                  C.c_double(time), C.byref(info))
+    _ck(st, "or_solve_potential", (abi.VD_OK, abi.VD_ENOCONV))
+    return pn, ef, info
+
+
+def implicit_solve(mech, prm, grid, spec, sb, ng, phi_old, dsdt_expl, phi_new, bclo, bchi, dt):
+    pn = phi_new.copy()
+    gr = grid.faces()
+    info = abi.vd_solve_info()
+    st = lib().or_implicit_solve_scalar(
+        C.byref(mech), int(spec), C.byref(grid.view(sb, ng)), C.byref(grid.view(phi_old)),
+        C.byref(grid.view(dsdt_expl)), C.byref(grid.view(pn)), grid.face_views(gr),
+        C.byref(grid.box), C.byref(grid.geom), abi.ivec(bclo), abi.ivec(bchi), C.byref(prm),
+        C.c_double(0.0), C.c_double(dt), C.byref(info))
+    _ck(st, "or_implicit_solve_scalar", (abi.VD_OK, abi.VD_ENOCONV))
+    return pn, gr, info
+
+
+def time_scales(mech, prm, grid, phi):
+    out = (C.c_double * 3)()
+    _ck(lib().or_find_time_scales(C.byref(mech), C.byref(grid.view(phi)), C.byref(grid.box),
+                                  C.byref(grid.geom), C.byref(prm), out), "or_find_time_scales")
+    return list(out)
+
+
+def error_est(grid, state1, comps, thr, gthr, tagval=1):
+    b, e = grid.extent(0)
+    tags = np.zeros((e[2] - b[2], e[1] - b[1], e[0] - b[0]), dtype=np.int8)
+    tv = abi.tag_view(tags.ctypes.data, b, e)
+    n = len(comps)
+    _ck(lib().or_error_est(C.byref(grid.view(state1, 1)), C.byref(tv), C.byref(grid.box), n,
+                           (C.c_int * n)(*comps), (C.c_double * n)(*thr),
+                           (C.c_double * n)(*gthr), C.c_char(tagval)), "or_error_est")
+    return tags
+
+
+def mlmg_solve(grid, prob, phi0, rhs):
+    phi = phi0.copy()
+    info = abi.vd_solve_info()
+    st = lib().or_mlmg_solve(C.byref(prob), C.byref(grid.view(phi)), C.byref(grid.view(rhs)),
+                             C.byref(grid.box), C.byref(grid.geom), C.byref(info))
+    _ck(st, "or_mlmg_solve", (abi.VD_OK, abi.VD_ENOCONV))
+    return phi, info
+
+
+def mlmg_assemble(grid, prob):
+    n = int(np.prod([grid.box.hi[d] - grid.box.lo[d] + 1 for d in range(3)]))
+    A = np.zeros((n, n))
+    bs = np.zeros(n)
+    _ck(lib().or_mlmg_assemble(C.byref(prob), C.byref(grid.box), C.byref(grid.geom),
+                               A.ctypes.data_as(P(C.c_double)),
+                               bs.ctypes.data_as(P(C.c_double))), "or_mlmg_assemble")
+    return A, bs
+
+
+def evolve(case, nsteps=None, ncorr=1, time=0.0):
+    """Runs Vidyut::Evolve on a copy of case.state; returns (state, time)."""
+    grid = Grid(case.geom)
+    pn = case.state.copy()
+    po = np.zeros_like(pn)
+    t = C.c_double(time)
+    n = case.nsteps if nsteps is None else nsteps
+    st = lib().or_evolve(C.byref(case.mech), C.byref(grid.view(pn)), C.byref(grid.view(po)),
+                         C.byref(grid.box), C.byref(case.geom), C.byref(case.params), int(n),
+                         C.c_double(case.dt), int(ncorr), C.byref(t))
+    _ck(st, "or_evolve", (abi.VD_OK, abi.VD_ENOCONV))
+    return pn, t.value

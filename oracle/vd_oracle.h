@@ -1,0 +1,74 @@
+/*
+ * vd_oracle.h -- CPU FP64 restatement of Vidyut's per-timestep hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (vidyut3d_amd/) links,
+ * loads or calls this code.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py use it, as the checker and the timed CPU
+ * baseline.
+ *
+ * Each or_* function mirrors the vd_* entry point of the same name in
+ * include/vidyut3d_amd.h on host arrays (vd_array4 with host pointers).
+ */
+#ifndef VD_ORACLE_H
+#define VD_ORACLE_H
+#include "../include/vidyut3d_amd.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+void or_params_default(vd_params* p);
+
+int or_fill_patch(const vd_array4* src, vd_array4* dst, const vd_box* box, const vd_geom* g,
+                  int scomp, int ncomp);
+
+/* single-call WENO (compute_explicit_flux.H:25-112), exposed for the
+ * closed-form pins */
+double or_weno(double im2, double im1, double i0, double ip1, double ip2, int scheme);
+
+int or_compute_scalar_transport_flux(const vd_mech_desc* m, const vd_array4* sb, vd_array4 flux[3],
+                                     const vd_box* box, const vd_geom* g, const int bclo[3],
+                                     const int bchi[3], const vd_params* prm, double time,
+                                     double dt, int specid);
+int or_compute_dsdt(const vd_array4 flux[3], const vd_array4* rxn, vd_array4* dsdt,
+                    const vd_box* box, const vd_geom* g, int specid);
+int or_update_rxnsrc(const vd_mech_desc* m, const vd_array4* sb, vd_array4* rxn,
+                     const vd_box* box, const vd_params* prm, double time);
+int or_compute_elecenergy_source(const vd_mech_desc* m, const vd_array4* sb, const vd_array4* rxn,
+                                 const vd_array4 efield[3], const vd_array4 gradne[3],
+                                 vd_array4* dsdt, vd_array4* phi_new, const vd_box* box,
+                                 const vd_geom* g, const vd_params* prm);
+int or_solve_potential(const vd_mech_desc* m, const vd_array4* sb, vd_array4* phi_new,
+                       vd_array4 efield[3], const vd_box* box, const vd_geom* g,
+                       const vd_params* prm, double time, vd_solve_info* info);
+int or_implicit_solve_scalar(const vd_mech_desc* m, int spec, const vd_array4* sb,
+                             const vd_array4* phi_old, const vd_array4* dsdt_expl,
+                             vd_array4* phi_new, vd_array4 grad[3], const vd_box* box,
+                             const vd_geom* g, const int bclo[3], const int bchi[3],
+                             const vd_params* prm, double time, double dt, vd_solve_info* info);
+int or_find_time_scales(const vd_mech_desc* m, const vd_array4* phi, const vd_box* box,
+                        const vd_geom* g, const vd_params* prm, double out[3]);
+int or_error_est(const vd_array4* state, vd_tagarray4* tags, const vd_box* box, int ntag,
+                 const int* comps, const double* thr, const double* gthr, char tagval);
+int or_mlmg_solve(const vd_mlmg_problem* prob, vd_array4* phi, const vd_array4* rhs,
+                  const vd_box* box, const vd_geom* g, vd_solve_info* info);
+
+/* Vidyut::Evolve (Source/Evolve.cpp:17-376) for `nsteps` steps at fixed dt
+ * on a single-level, single-box domain; phi_new/phi_old have no ghosts.
+ * ncorr = num_timestep_correctors.  time is advanced in place. */
+int or_evolve(const vd_mech_desc* m, vd_array4* phi_new, vd_array4* phi_old, const vd_box* box,
+              const vd_geom* g, const vd_params* prm, int nsteps, double dt, int ncorr,
+              double* time);
+
+/* dense assembled operator of the multigrid level-0 problem (tiny grids):
+ * writes the n x n matrix A (row-major) and the rhs shift due to the
+ * inhomogeneous boundary data, for the brute-force pin of the solver. */
+int or_mlmg_assemble(const vd_mlmg_problem* prob, const vd_box* box, const vd_geom* g,
+                     double* A, double* bshift);
+
+int or_num_threads(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,322 @@
+// vd_capi.hip -- context, memory, case plug-in upload and kernel timing.
+#include <cstdarg>
+#include <cstring>
+
+#include "vd_internal.hpp"
+
+namespace vd {
+
+static thread_local char g_err[1024] = "";
+
+void set_error(const char* fmt, ...) {
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+int scratch(vd_ctx* ctx, const std::string& name, size_t bytes, void** out) {
+    Buf& b = ctx->scratch[name];
+    if (b.bytes < bytes) {
+        if (b.p) VD_HIP(hipFree(b.p));
+        b.p = nullptr;
+        b.bytes = 0;
+        size_t sz = bytes < 256 ? 256 : bytes;
+        hipError_t e = hipMalloc(&b.p, sz);
+        if (e != hipSuccess) {
+            set_error("scratch '%s' %zu bytes: %s", name.c_str(), sz, hipGetErrorString(e));
+            b.p = nullptr;
+            return VD_ENOMEM;
+        }
+        b.bytes = sz;
+    }
+    *out = b.p;
+    return VD_OK;
+}
+
+static hipEvent_t kt_event(vd_ctx* ctx) {
+    if (ctx->kt_next < ctx->kt_pool.size()) return ctx->kt_pool[ctx->kt_next++];
+    hipEvent_t e;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    ctx->kt_pool.push_back(e);
+    ctx->kt_next++;
+    return e;
+}
+
+void kt_begin(vd_ctx* ctx, int family) {
+    if (ctx->kt_family == VD_KT_NONE || ctx->kt_family != family) return;
+    hipEvent_t a = kt_event(ctx), b = kt_event(ctx);
+    if (!a || !b) return;
+    (void)hipEventRecord(a, ctx->stream);
+    ctx->kt_events.push_back({a, b});
+}
+
+void kt_end(vd_ctx* ctx, int family, double cells) {
+    if (ctx->kt_family == VD_KT_NONE || ctx->kt_family != family || ctx->kt_events.empty()) return;
+    (void)hipEventRecord(ctx->kt_events.back().second, ctx->stream);
+    ctx->kt_cells += cells;
+}
+
+}  // namespace vd
+
+using namespace vd;
+
+extern "C" {
+
+const char* vd_last_error(void) { return g_err; }
+const char* vd_version(void) { return "vidyut3d_amd 0.1.0 gfx950"; }
+
+void vd_params_default(vd_params* p) {
+    memset(p, 0, sizeof(*p));
+    p->gas_temperature = 300.0;
+    p->gas_pressure = 101325.0;
+    p->hyp_order = 1;
+    p->weno_scheme = 3;
+    p->do_reactions = 1;
+    p->do_transport = 1;
+    p->do_spacechrg = 1;
+    p->min_species_density = 1e-20;
+    p->min_electron_density = 1e12;
+    p->min_electron_temp = 300.0;
+    p->ele_mob = -1.0e24;
+    p->ele_diff = 1.0e24;
+    p->voltage_freq = 1.0;
+    p->linsolve_reltol = 1e-10;
+    p->linsolve_abstol = 1e-10;
+    p->linsolve_maxiter = 100;
+    p->linsolve_max_coarsening_level = 10;
+    p->mg_pre_smooth = 2;
+    p->mg_post_smooth = 2;
+    for (int d = 0; d < 3; d++) {
+        p->pot_bc_lo[d] = p->pot_bc_hi[d] = VD_ROBINBC;
+        p->eden_bc_lo[d] = p->eden_bc_hi[d] = VD_ROBINBC;
+        p->eenrg_bc_lo[d] = p->eenrg_bc_hi[d] = VD_ROBINBC;
+        p->ion_bc_lo[d] = p->ion_bc_hi[d] = VD_ROBINBC;
+        p->neutral_bc_lo[d] = p->neutral_bc_hi[d] = VD_ROBINBC;
+    }
+}
+
+int vd_ctx_create(int device, vd_ctx** out) {
+    if (!out) return VD_EINVAL;
+    int n = 0;
+    VD_HIP(hipGetDeviceCount(&n));
+    if (device < 0 || device >= n) {
+        set_error("device %d not present (%d devices)", device, n);
+        return VD_EINVAL;
+    }
+    VD_HIP(hipSetDevice(device));
+    vd_ctx* c = new vd_ctx();
+    c->device = device;
+    hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete c;
+        set_error("hipStreamCreate: %s", hipGetErrorString(e));
+        return VD_EHIP;
+    }
+    e = hipHostMalloc((void**)&c->pinned, 64 * sizeof(double), hipHostMallocDefault);
+    if (e != hipSuccess) {
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        set_error("hipHostMalloc: %s", hipGetErrorString(e));
+        return VD_EHIP;
+    }
+    *out = c;
+    return VD_OK;
+}
+
+int vd_ctx_destroy(vd_ctx* ctx) {
+    if (!ctx) return VD_EINVAL;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->scratch)
+        if (kv.second.p) (void)hipFree(kv.second.p);
+    for (auto& kv : ctx->mg) mgwork_free(kv.second);
+    for (auto e : ctx->kt_pool) (void)hipEventDestroy(e);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return VD_OK;
+}
+
+int vd_ctx_sync(vd_ctx* ctx) {
+    VD_HIP(hipStreamSynchronize(ctx->stream));
+    return VD_OK;
+}
+
+void* vd_ctx_stream(vd_ctx* ctx) { return (void*)ctx->stream; }
+
+int vd_malloc(vd_ctx* ctx, size_t bytes, void** out) {
+    VD_HIP(hipSetDevice(ctx->device));
+    hipError_t e = hipMalloc(out, bytes ? bytes : 8);
+    if (e != hipSuccess) {
+        set_error("hipMalloc %zu: %s", bytes, hipGetErrorString(e));
+        return VD_ENOMEM;
+    }
+    return VD_OK;
+}
+
+int vd_free(vd_ctx* ctx, void* p) {
+    VD_HIP(hipStreamSynchronize(ctx->stream));
+    VD_HIP(hipFree(p));
+    return VD_OK;
+}
+
+int vd_memcpy_h2d(vd_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    VD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    VD_HIP(hipStreamSynchronize(ctx->stream));
+    return VD_OK;
+}
+
+int vd_memcpy_d2h(vd_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    VD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    VD_HIP(hipStreamSynchronize(ctx->stream));
+    return VD_OK;
+}
+
+int vd_memcpy_d2d(vd_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    VD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    return VD_OK;
+}
+
+int vd_memset_zero_kernel(vd_ctx* ctx, void* dst, size_t bytes);
+
+// a fill kernel, not hipMemsetAsync, so captured steps replay correctly
+int vd_memset_zero(vd_ctx* ctx, void* dst, size_t bytes) { return vd_memset_zero_kernel(ctx, dst, bytes); }
+
+int vd_mech_create(vd_ctx* ctx, const vd_mech_desc* d, vd_mech** out) {
+    if (!ctx || !d || !out) return VD_EINVAL;
+    if (d->nspec < 1 || d->nspec > VD_MAX_SPECIES || d->nreac < 0 || d->nreac > VD_MAX_REACTIONS ||
+        d->e_idx < 0 || d->e_idx >= d->nspec) {
+        set_error("mechanism: bad sizes nspec=%d nreac=%d e_idx=%d", d->nspec, d->nreac, d->e_idx);
+        return VD_EINVAL;
+    }
+    if (d->chem_form != VD_CHEM_TABLE && d->chem_form != VD_CHEM_CKWC) {
+        set_error("mechanism: unknown chem_form %d", d->chem_form);
+        return VD_EINVAL;
+    }
+    vd_mech* m = new vd_mech();
+    m->ctx = ctx;
+    m->host = *d;
+    DevMech& h = m->hdev;
+    memset(&h, 0, sizeof(h));
+    h.nspec = d->nspec;
+    h.nreac = d->nreac;
+    h.e_idx = d->e_idx;
+    h.chem_form = d->chem_form;
+    h.elastic_loss = d->elastic_loss;
+    h.elastic_partner_amu = d->elastic_partner_amu;
+    for (int s = 0; s < d->nspec; s++) {
+        h.charge[s] = d->charge[s];
+        h.molwt[s] = d->molwt[s];
+        h.mob[s] = d->mob[s];
+        h.diff_kind[s] = d->diff_kind[s];
+        h.diff[s] = d->diff[s];
+    }
+    for (int r = 0; r < d->nreac; r++) {
+        const vd_reaction& R = d->reac[r];
+        if (R.nreact < 0 || R.nreact > VD_MAX_REACTANTS || R.nprod < 0 || R.nprod > VD_MAX_PRODUCTS) {
+            delete m;
+            set_error("reaction %d: bad reactant/product count", r);
+            return VD_EINVAL;
+        }
+        h.A[r] = R.A;
+        h.beta[r] = R.beta;
+        h.Ea[r] = R.Ea;
+        h.te[r] = R.temp_is_te;
+        h.energy[r] = R.energy;
+        h.rxntype[r] = R.rxntype;
+        h.eexci[r] = R.eexci;
+        h.elidx[r] = R.elidx;
+        h.nreact[r] = R.nreact;
+        h.nprod[r] = R.nprod;
+        int order = 0;
+        for (int q = 0; q < R.nreact; q++) {
+            if (R.react_sp[q] < 0 || R.react_sp[q] >= d->nspec) {
+                delete m;
+                set_error("reaction %d: reactant index %d", r, R.react_sp[q]);
+                return VD_EINVAL;
+            }
+            h.rsp[q][r] = R.react_sp[q];
+            h.rnu[q][r] = R.react_nu[q];
+            order += (int)R.react_nu[q];
+        }
+        h.order[r] = order;
+        for (int q = 0; q < R.nprod; q++) {
+            if (R.prod_sp[q] < 0 || R.prod_sp[q] >= d->nspec) {
+                delete m;
+                set_error("reaction %d: product index %d", r, R.prod_sp[q]);
+                return VD_EINVAL;
+            }
+            h.psp[q][r] = R.prod_sp[q];
+            h.pnu[q][r] = R.prod_nu[q];
+        }
+        // net stoichiometry (ascending species), generated-code form
+        int nn = 0;
+        for (int sp = 0; sp < d->nspec; sp++) {
+            double net = 0.0;
+            int hit = 0;
+            for (int q = 0; q < R.nreact; q++)
+                if (R.react_sp[q] == sp) {
+                    net -= R.react_nu[q];
+                    hit = 1;
+                }
+            for (int q = 0; q < R.nprod; q++)
+                if (R.prod_sp[q] == sp) {
+                    net += R.prod_nu[q];
+                    hit = 1;
+                }
+            if (!hit || net == 0.0) continue;
+            h.netsp[nn][r] = sp;
+            h.netnu[nn][r] = net;
+            nn++;
+        }
+        h.nnet[r] = nn;
+    }
+    hipError_t e = hipMalloc((void**)&m->dev, sizeof(DevMech));
+    if (e == hipSuccess) e = hipMemcpy(m->dev, &h, sizeof(DevMech), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        delete m;
+        set_error("mechanism upload: %s", hipGetErrorString(e));
+        return VD_EHIP;
+    }
+    *out = m;
+    return VD_OK;
+}
+
+int vd_mech_destroy(vd_mech* m) {
+    if (!m) return VD_EINVAL;
+    (void)hipFree(m->dev);
+    delete m;
+    return VD_OK;
+}
+
+int vd_mech_find_id(const vd_mech* m, const char* name) {
+    for (int s = 0; s < m->host.nspec; s++)
+        if (strncmp(m->host.names[s], name, 16) == 0) return s;
+    return -1;
+}
+
+int vd_ctx_set_kernel_timing(vd_ctx* ctx, int family) {
+    VD_HIP(hipStreamSynchronize(ctx->stream));
+    ctx->kt_family = family;
+    ctx->kt_events.clear();
+    ctx->kt_next = 0;
+    ctx->kt_cells = 0.0;
+    return VD_OK;
+}
+
+int vd_ctx_kernel_timing(vd_ctx* ctx, double* total_ms, long long* launches, double* cells) {
+    VD_HIP(hipStreamSynchronize(ctx->stream));
+    double tot = 0.0;
+    for (auto& pr : ctx->kt_events) {
+        float ms = 0.f;
+        VD_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
+        tot += ms;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = (long long)ctx->kt_events.size();
+    if (cells) *cells = ctx->kt_cells;
+    return VD_OK;
+}
+
+}  // extern "C"

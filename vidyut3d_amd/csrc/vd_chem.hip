@@ -1,0 +1,296 @@
+// vd_chem.hip -- fused per-cell species + electron-energy sources and the
+// Joule-heating source.  One thread per cell, FP64 throughout (rate
+// exponentials included).  Per-thread source accumulators live in LDS
+// columns (species-major, thread-minor: conflict-free, indexable by the
+// table's species ids); densities are re-read through L1/L2.
+#include "vd_internal.hpp"
+
+namespace vd {
+
+struct ChemArgs {
+    const DevMech* __restrict__ M;
+    Arr<double> sb, rxn;
+    double Tg;
+    int3 lo;
+    int nx, ny, nz;
+};
+
+constexpr int CHEM_BLOCK = 128;
+
+__device__ __forceinline__ double powi_nu(double n, double nu) {
+    if (nu == 1.0) return n;
+    if (nu == 2.0) return n * n;
+    return pow(n, nu);
+}
+
+// Source/reaction_source.H:10-64 (table form)
+__global__ void __launch_bounds__(CHEM_BLOCK) k_rxn_table(ChemArgs a) {
+    extern __shared__ double w[];
+    const DevMech& M = *a.M;
+    const int tid = threadIdx.x;
+    const int ns = M.nspec;
+    int64_t t = (int64_t)blockIdx.x * CHEM_BLOCK + tid;
+    int64_t tot = (int64_t)a.nx * a.ny * a.nz;
+    if (t >= tot) return;
+    int i = (int)(t % a.nx);
+    int64_t r0 = t / a.nx;
+    int j = (int)(r0 % a.ny);
+    int k = (int)(r0 / a.ny);
+    i += a.lo.x;
+    j += a.lo.y;
+    k += a.lo.z;
+    for (int sp = 0; sp <= ns; sp++) w[sp * CHEM_BLOCK + tid] = 0.0;
+    const int o = a.sb.off(i, j, k);
+    const double* __restrict__ sbp = a.sb.p;
+    const int64_t nst = a.sb.ns;
+    const double Te = sbp[(int64_t)VD_ETEMP(ns) * nst + o];
+    const double lTe = log(Te), iTe = 1.0 / Te;
+    const double lTg = log(a.Tg), iTg = 1.0 / a.Tg;
+    const int nr = M.nreac;
+    for (int r = 0; r < nr; r++) {
+        const bool te = M.te[r] != 0;
+        double kr = M.A[r] * exp(M.beta[r] * (te ? lTe : lTg) - M.Ea[r] * (te ? iTe : iTg));
+        double conc = 1.0;
+        const int nq = M.nreact[r];
+        for (int q = 0; q < nq; q++) conc *= powi_nu(sbp[(int64_t)M.rsp[q][r] * nst + o], M.rnu[q][r]);
+        const double rate = kr * conc;
+        for (int q = 0; q < nq; q++) w[M.rsp[q][r] * CHEM_BLOCK + tid] -= M.rnu[q][r] * rate;
+        const int np = M.nprod[r];
+        for (int q = 0; q < np; q++) w[M.psp[q][r] * CHEM_BLOCK + tid] += M.pnu[q][r] * rate;
+        w[ns * CHEM_BLOCK + tid] += M.energy[r] * rate;
+    }
+    const int orx = a.rxn.off(i, j, k);
+    for (int sp = 0; sp <= ns; sp++) a.rxn.p[(int64_t)sp * a.rxn.ns + orx] = w[sp * CHEM_BLOCK + tid];
+}
+
+// CKWC / productionRate / comp_ener_exch of a generated Chemistry.H
+// (test/models/ionelec_DC_1d/Chemistry.H:240-357) behind
+// update_rxnsrc_at_all_levels (Source/ScalarSolve.cpp:143-165).
+__global__ void __launch_bounds__(CHEM_BLOCK) k_rxn_ckwc(ChemArgs a) {
+    extern __shared__ double w[];
+    const DevMech& M = *a.M;
+    const int tid = threadIdx.x;
+    const int ns = M.nspec, E = M.e_idx;
+    int64_t t = (int64_t)blockIdx.x * CHEM_BLOCK + tid;
+    int64_t tot = (int64_t)a.nx * a.ny * a.nz;
+    if (t >= tot) return;
+    int i = (int)(t % a.nx);
+    int64_t r0 = t / a.nx;
+    int j = (int)(r0 % a.ny);
+    int k = (int)(r0 / a.ny);
+    i += a.lo.x;
+    j += a.lo.y;
+    k += a.lo.z;
+    for (int sp = 0; sp < ns; sp++) w[sp * CHEM_BLOCK + tid] = 0.0;
+    const int o = a.sb.off(i, j, k);
+    const double* __restrict__ sbp = a.sb.p;
+    const int64_t nst = a.sb.ns;
+    // C in mol/m^3: (n * 1e-6 / N_A) * 1e6
+#define CONC(sp) ((sbp[(int64_t)(sp) * nst + o] * 1.0e-6 / VD_N_A) * 1.0e6)
+    const double Te = sbp[(int64_t)VD_ETEMP(ns) * nst + o];
+    const double T = a.Tg;
+    const double lTe = log(Te), iTe = 1.0 / Te, lTg = log(T), iTg = 1.0 / T;
+    const double CE = CONC(E);
+    const double ne_pr = CE * 6.02214085774e23;
+    const double Ue = 1.5 * Te * ne_pr * 1.380649e-23;
+    const double eV2J = 1.602176620800000e-19, NA = 6.02214085774e23, kB = 1.380649e-23;
+    const double mee = Ue / (CE * NA);
+    const double sefact = (tanh((Te - T) / 0.5) + 1.0) / 2.0;
+    double ener = 0.0;
+    const int nr = M.nreac;
+    for (int r = 0; r < nr; r++) {
+        const bool te = M.te[r] != 0;
+        double k_f = M.A[r] * exp(M.beta[r] * (te ? lTe : lTg) - M.Ea[r] * (te ? iTe : iTg));
+        const int ord = M.order[r];
+        for (int q = 1; q < ord; q++) k_f *= 6.02214076e23;
+        double cprod = 1.0;
+        bool first = true;
+        const int nq = M.nreact[r];
+        for (int q = 0; q < nq; q++) {
+            const double c = CONC(M.rsp[q][r]);
+            const int nu = (int)M.rnu[q][r];
+            for (int e = 0; e < nu; e++) {
+                cprod = first ? c : cprod * c;
+                first = false;
+            }
+        }
+        const double qf = k_f * cprod;
+        const double qdot = qf - 0.0;
+        const int nn = M.nnet[r];
+        for (int q = 0; q < nn; q++) {
+            const double net = M.netnu[q][r];
+            double* wp = &w[M.netsp[q][r] * CHEM_BLOCK + tid];
+            if (net == 1.0) *wp += qdot;
+            else if (net == -1.0) *wp -= qdot;
+            else *wp += net * qdot;
+        }
+        const int rt = M.rxntype[r];
+        if (rt == 1 || rt == 2) ener -= (M.eexci[r] * eV2J) * NA * (qf - 0.0) * sefact;
+        if (rt == 3) ener -= mee * NA * (qf - 0.0);
+        if (rt == 5) {
+            const int el = M.elidx[r];
+            double qq = CE * CONC(el) * k_f;
+            ener -= 3.0 * kB * (M.molwt[E] / M.molwt[el]) * (Te - T) * qq * NA;
+        }
+    }
+    if (M.elastic_loss) {
+        // test/models/ionelec_DC_1d/Chemistry.H:334-341
+        double mu = spec_mob(M, E);
+        double nu = -VD_ECHARGE / VD_ME / mu;
+        double term = 3.0 / 2.0 * VD_K_B * CE * VD_N_A * (Te - T) * nu *
+                      (2.0 * VD_ME / (M.elastic_partner_amu * VD_M_AMU));
+        ener -= term;
+    }
+#undef CONC
+    const int orx = a.rxn.off(i, j, k);
+    for (int sp = 0; sp < ns; sp++)
+        a.rxn.p[(int64_t)sp * a.rxn.ns + orx] = ((w[sp * CHEM_BLOCK + tid] * 1.0e-6) * VD_N_A) * 1.0e6;
+    a.rxn.p[(int64_t)ns * a.rxn.ns + orx] = ener;
+}
+
+int update_rxnsrc(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4* rxn,
+                  const vd_box* box, const vd_params* prm) {
+    if (rxn->ncomp < m->hdev.nspec + 1) {
+        set_error("rxn_src needs NUM_SPECIES+1 components");
+        return VD_EINVAL;
+    }
+    ChemArgs a;
+    a.M = m->dev;
+    a.sb = arr(sb);
+    a.rxn = arr(rxn);
+    a.Tg = prm->gas_temperature;
+    a.lo = make_int3(box->lo[0], box->lo[1], box->lo[2]);
+    a.nx = box->hi[0] - box->lo[0] + 1;
+    a.ny = box->hi[1] - box->lo[1] + 1;
+    a.nz = box->hi[2] - box->lo[2] + 1;
+    int64_t tot = (int64_t)a.nx * a.ny * a.nz;
+    size_t lds = sizeof(double) * (m->hdev.nspec + 1) * CHEM_BLOCK;
+    kt_begin(ctx, VD_KT_CHEM);
+    if (m->hdev.chem_form == VD_CHEM_TABLE)
+        hipLaunchKernelGGL(k_rxn_table, grid1d(tot, CHEM_BLOCK), dim3(CHEM_BLOCK), lds, ctx->stream, a);
+    else
+        hipLaunchKernelGGL(k_rxn_ckwc, grid1d(tot, CHEM_BLOCK), dim3(CHEM_BLOCK), lds, ctx->stream, a);
+    kt_end(ctx, VD_KT_CHEM, (double)tot);
+    VD_CHECK_LAUNCH();
+    return VD_OK;
+}
+
+struct JouleArgs {
+    const DevMech* __restrict__ M;
+    Arr<double> sb, rxn, ef0, ef1, ef2, gn0, gn1, gn2, dsdt, phi;
+    double Tg, ele_mob, ele_diff;
+    int const_ele_trans, floor_jh, dim;
+    int3 dlo, dhi, lo;
+    int nx, ny, nz;
+};
+
+// Vidyut::compute_elecenergy_source (Source/ElecEnergySource.cpp:65-171)
+__global__ void __launch_bounds__(256) k_joule(JouleArgs a) {
+    const DevMech& M = *a.M;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t tot = (int64_t)a.nx * a.ny * a.nz;
+    if (t >= tot) return;
+    int i = (int)(t % a.nx);
+    int64_t r0 = t / a.nx;
+    int j = (int)(r0 % a.ny);
+    int k = (int)(r0 / a.ny);
+    i += a.lo.x;
+    j += a.lo.y;
+    k += a.lo.z;
+    const int ns = M.nspec, eidx = M.e_idx;
+    const double charge = M.charge[eidx] * VD_ECHARGE;
+    const int dlo[3] = {a.dlo.x, a.dlo.y, a.dlo.z}, dhi[3] = {a.dhi.x, a.dhi.y, a.dhi.z};
+    double jheat = 0.0;
+    for (int idim = 0; idim < a.dim; idim++) {
+        const Arr<double>& EF = idim == 0 ? a.ef0 : (idim == 1 ? a.ef1 : a.ef2);
+        const Arr<double>& GN = idim == 0 ? a.gn0 : (idim == 1 ? a.gn1 : a.gn2);
+        for (int f = 0; f < 2; f++) {
+            int face[3] = {i, j, k};
+            face[idim] += f;
+            if (face[idim] == dlo[idim]) face[idim] += 1;
+            if (face[idim] == dhi[idim] + 1) face[idim] -= 1;
+            int lc[3] = {face[0], face[1], face[2]};
+            lc[idim] -= 1;
+            const int ol = a.sb.off(lc[0], lc[1], lc[2]), orr = a.sb.off(face[0], face[1], face[2]);
+            const double* S = a.sb.p;
+            const int64_t N = a.sb.ns;
+            double etemp = 0.5 * (S[VD_ETEMP(ns) * N + ol] + S[VD_ETEMP(ns) * N + orr]);
+            double ne = 0.5 * (S[(int64_t)eidx * N + ol] + S[(int64_t)eidx * N + orr]);
+            double efield_face = EF(face[0], face[1], face[2]);
+            double gradne_face = GN(face[0], face[1], face[2]);
+            double mu, dcoeff;
+            if (a.const_ele_trans) {
+                double ndens = 0.0;
+                for (int sp = 0; sp < ns; sp++) ndens += 0.5 * (S[(int64_t)sp * N + ol] + S[(int64_t)sp * N + orr]);
+                mu = a.ele_mob / ndens;
+                dcoeff = a.ele_diff / ndens;
+            } else {
+                mu = spec_mob(M, eidx);
+                dcoeff = spec_diff(M, eidx, etemp, a.Tg);
+            }
+            double cur = charge * (mu * ne * efield_face - dcoeff * gradne_face);
+            jheat += cur * efield_face;
+        }
+    }
+    jheat *= 0.5;
+    if (a.floor_jh && jheat < 0.0) jheat = 0.0;
+    a.dsdt(i, j, k) += jheat;
+    a.phi(i, j, k, VD_EJH(ns)) = jheat;
+    a.phi(i, j, k, VD_EIH(ns)) = a.rxn(i, j, k, ns);
+}
+
+int elecenergy_source(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, const vd_array4* rxn,
+                      const vd_array4 ef[3], const vd_array4 gn[3], vd_array4* ds, vd_array4* phi,
+                      const vd_box* box, const vd_geom* g, const vd_params* prm) {
+    JouleArgs a;
+    a.M = m->dev;
+    a.sb = arr(sb);
+    a.rxn = arr(rxn);
+    a.ef0 = arr(&ef[0]);
+    a.ef1 = g->dim > 1 ? arr(&ef[1]) : a.ef0;
+    a.ef2 = g->dim > 2 ? arr(&ef[2]) : a.ef0;
+    a.gn0 = arr(&gn[0]);
+    a.gn1 = g->dim > 1 ? arr(&gn[1]) : a.gn0;
+    a.gn2 = g->dim > 2 ? arr(&gn[2]) : a.gn0;
+    a.dsdt = arr(ds);
+    a.phi = arr(phi);
+    a.Tg = prm->gas_temperature;
+    a.ele_mob = prm->ele_mob;
+    a.ele_diff = prm->ele_diff;
+    a.const_ele_trans = prm->const_ele_trans;
+    a.floor_jh = prm->floor_jh;
+    a.dim = g->dim;
+    a.dlo = make_int3(g->dom_lo[0], g->dom_lo[1], g->dom_lo[2]);
+    a.dhi = make_int3(g->dom_hi[0], g->dom_hi[1], g->dom_hi[2]);
+    a.lo = make_int3(box->lo[0], box->lo[1], box->lo[2]);
+    a.nx = box->hi[0] - box->lo[0] + 1;
+    a.ny = box->hi[1] - box->lo[1] + 1;
+    a.nz = box->hi[2] - box->lo[2] + 1;
+    int64_t tot = (int64_t)a.nx * a.ny * a.nz;
+    hipLaunchKernelGGL(k_joule, grid1d(tot, 256), dim3(256), 0, ctx->stream, a);
+    VD_CHECK_LAUNCH();
+    return VD_OK;
+}
+
+}  // namespace vd
+
+using namespace vd;
+
+extern "C" int vd_update_rxnsrc(vd_ctx* ctx, const vd_mech* m, const vd_array4* sborder,
+                                vd_array4* rxn_src, const vd_box* box, const vd_params* prm,
+                                double time) {
+    (void)time;
+    if (!ctx || !m || !sborder || !rxn_src || !box || !prm) return VD_EINVAL;
+    return update_rxnsrc(ctx, m, sborder, rxn_src, box, prm);
+}
+
+extern "C" int vd_compute_elecenergy_source(vd_ctx* ctx, const vd_mech* m, const vd_array4* sborder,
+                                            const vd_array4* rxn_src, const vd_array4 efield[3],
+                                            const vd_array4 gradne[3], vd_array4* dsdt,
+                                            vd_array4* phi_new, const vd_box* box,
+                                            const vd_geom* geom, const vd_params* prm) {
+    if (!ctx || !m || !sborder || !rxn_src || !efield || !gradne || !dsdt || !phi_new || !box ||
+        !geom || !prm)
+        return VD_EINVAL;
+    return elecenergy_source(ctx, m, sborder, rxn_src, efield, gradne, dsdt, phi_new, box, geom, prm);
+}

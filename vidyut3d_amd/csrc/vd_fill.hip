@@ -1,0 +1,120 @@
+// vd_fill.hip -- ghost fill (Vidyut::FillPatch, single level) and the
+// small fill/copy kernels used by the other entry points (no hipMemset in
+// the step path, so a captured step replays correctly).
+#include "vd_internal.hpp"
+
+namespace vd {
+
+__global__ void k_fill_value(double* __restrict__ p, int64_t n, double v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_fill_value_f(float* __restrict__ p, int64_t n, float v) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+__global__ void k_fill_bytes(unsigned char* __restrict__ p, int64_t n) {
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0;
+}
+
+static inline dim3 grid_stride(int64_t n, int block) {
+    int64_t g = (n + block - 1) / block;
+    if (g > 65536) g = 65536;
+    if (g < 1) g = 1;
+    return dim3((unsigned)g);
+}
+
+int fill_value(vd_ctx* ctx, double* p, int64_t n, double v) {
+    if (n <= 0) return VD_OK;
+    hipLaunchKernelGGL(k_fill_value, grid_stride(n, 256), dim3(256), 0, ctx->stream, p, n, v);
+    VD_CHECK_LAUNCH();
+    return VD_OK;
+}
+
+int fill_value_f(vd_ctx* ctx, float* p, int64_t n, float v) {
+    if (n <= 0) return VD_OK;
+    hipLaunchKernelGGL(k_fill_value_f, grid_stride(n, 256), dim3(256), 0, ctx->stream, p, n, v);
+    VD_CHECK_LAUNCH();
+    return VD_OK;
+}
+
+// FillPatchSingleLevel with foextrap / int_dir bcspec: every destination
+// cell (valid or ghost) copies the source cell found by wrapping periodic
+// indices and clamping the others into the domain.  One thread per
+// destination cell, blockIdx.y = component.
+__global__ void k_fill_patch(Arr<double> src, Arr<double> dst, int dim, int3 lo, int3 n,
+                             int3 per, int nx, int ny, int nz, int scomp) {
+    int q = blockIdx.y + scomp;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t tot = (int64_t)nx * ny * nz;
+    if (t >= tot) return;
+    int i = (int)(t % nx);
+    int64_t r = t / nx;
+    int j = (int)(r % ny);
+    int k = (int)(r / ny);
+    i += dst.bx;
+    j += dst.by;
+    k += dst.bz;
+    int c[3] = {i, j, k};
+    int l[3] = {lo.x, lo.y, lo.z}, nn[3] = {n.x, n.y, n.z}, pp[3] = {per.x, per.y, per.z};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (d >= dim) break;
+        int v = c[d] - l[d];
+        if (v < 0 || v >= nn[d]) {
+            if (pp[d]) {
+                v %= nn[d];
+                if (v < 0) v += nn[d];
+            } else {
+                v = v < 0 ? 0 : nn[d] - 1;
+            }
+        }
+        c[d] = v + l[d];
+    }
+    dst.comp(q)[dst.off(i, j, k)] = src.comp(q)[src.off(c[0], c[1], c[2])];
+}
+
+int fill_patch(vd_ctx* ctx, const vd_array4* src, vd_array4* dst, const vd_box* box,
+               const vd_geom* g, int scomp, int ncomp) {
+    for (int d = 0; d < g->dim; d++)
+        if (box->lo[d] != g->dom_lo[d] || box->hi[d] != g->dom_hi[d]) {
+            set_error("fill_patch: box must cover the domain (single box per level)");
+            return VD_EINVAL;
+        }
+    if (ncomp <= 0) return VD_OK;
+    int nx = dst->end[0] - dst->begin[0], ny = dst->end[1] - dst->begin[1],
+        nz = dst->end[2] - dst->begin[2];
+    int64_t tot = (int64_t)nx * ny * nz;
+    int3 lo = make_int3(box->lo[0], box->lo[1], box->lo[2]);
+    int3 n = make_int3(box->hi[0] - box->lo[0] + 1, box->hi[1] - box->lo[1] + 1,
+                       box->hi[2] - box->lo[2] + 1);
+    int3 per = make_int3(g->is_periodic[0], g->is_periodic[1], g->dim > 2 ? g->is_periodic[2] : 0);
+    kt_begin(ctx, VD_KT_FILL);
+    hipLaunchKernelGGL(k_fill_patch, dim3((unsigned)((tot + 255) / 256), ncomp), dim3(256), 0,
+                       ctx->stream, arr(src), arr(dst), g->dim, lo, n, per, nx, ny, nz, scomp);
+    kt_end(ctx, VD_KT_FILL, (double)tot * ncomp);
+    VD_CHECK_LAUNCH();
+    return VD_OK;
+}
+
+}  // namespace vd
+
+extern "C" int vd_fill_patch(vd_ctx* ctx, const vd_array4* src, vd_array4* dst, const vd_box* box,
+                             const vd_geom* geom, int scomp, int ncomp) {
+    if (!ctx || !src || !dst || !box || !geom) return VD_EINVAL;
+    return vd::fill_patch(ctx, src, dst, box, geom, scomp, ncomp);
+}
+
+extern "C" int vd_memset_zero_kernel(vd_ctx* ctx, void* dst, size_t bytes) {
+    if (!bytes) return VD_OK;
+    hipLaunchKernelGGL(vd::k_fill_bytes, vd::grid_stride((int64_t)bytes, 256), dim3(256), 0,
+                       ctx->stream, (unsigned char*)dst, (int64_t)bytes);
+    VD_CHECK_LAUNCH();
+    return VD_OK;
+}

@@ -1,0 +1,171 @@
+// vd_internal.hpp -- shared device/host helpers of the gfx950 library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/vidyut3d_amd.h"
+
+// Source/UnivConstants.H:4-19
+#define VD_N_A 6.02214076e23
+#define VD_K_B 1.380649e-23
+#define VD_PI 3.14159265358979323846264338327950288
+#define VD_ECHARGE 1.60217662e-19
+#define VD_EPS0 8.854187817e-12
+#define VD_EV (VD_ECHARGE / VD_K_B)
+#define VD_ME 9.10938188e-31
+#define VD_M_AMU 1.66054e-27
+#define VD_FIVEBYTHREE (5.0 / 3.0)
+#define VD_TWOTHIRD (2.0 / 3.0)
+
+namespace vd {
+
+void set_error(const char* fmt, ...);
+
+#define VD_HIP(call)                                                                      \
+    do {                                                                                  \
+        hipError_t e_ = (call);                                                           \
+        if (e_ != hipSuccess) {                                                           \
+            ::vd::set_error("%s:%d %s: %s", __FILE__, __LINE__, #call, hipGetErrorString(e_)); \
+            return VD_EHIP;                                                               \
+        }                                                                                 \
+    } while (0)
+
+#define VD_CHECK_LAUNCH()                                                                 \
+    do {                                                                                  \
+        hipError_t e_ = hipGetLastError();                                                \
+        if (e_ != hipSuccess) {                                                           \
+            ::vd::set_error("%s:%d launch: %s", __FILE__, __LINE__, hipGetErrorString(e_)); \
+            return VD_EHIP;                                                               \
+        }                                                                                 \
+    } while (0)
+
+#define VD_TRY(x)                  \
+    do {                           \
+        int s_ = (x);              \
+        if (s_ != VD_OK) return s_;\
+    } while (0)
+
+// Array4 view for kernels: component base pointer + 32-bit cell offsets.
+template <class T>
+struct Arr {
+    T* p;
+    int bx, by, bz;
+    int js, ks;
+    int64_t ns;
+    __host__ __device__ inline T* comp(int n) const { return p + (int64_t)n * ns; }
+    __host__ __device__ inline int off(int i, int j, int k) const {
+        return (i - bx) + (j - by) * js + (k - bz) * ks;
+    }
+    __host__ __device__ inline T& operator()(int i, int j, int k, int n = 0) const {
+        return comp(n)[off(i, j, k)];
+    }
+};
+
+inline Arr<double> arr(const vd_array4* a) {
+    Arr<double> r;
+    r.p = a->p;
+    r.bx = a->begin[0];
+    r.by = a->begin[1];
+    r.bz = a->begin[2];
+    r.js = (int)a->jstride;
+    r.ks = (int)a->kstride;
+    r.ns = a->nstride;
+    return r;
+}
+
+// Device copy of the case plug-in descriptor (vd_mech_desc), reaction
+// table in structure-of-arrays form for uniform (scalar) loads.
+struct DevMech {
+    int nspec, nreac, e_idx, chem_form, elastic_loss;
+    double elastic_partner_amu;
+    int charge[VD_MAX_SPECIES];
+    double molwt[VD_MAX_SPECIES];
+    double mob[VD_MAX_SPECIES];
+    int diff_kind[VD_MAX_SPECIES];
+    double diff[VD_MAX_SPECIES];
+    // reactions
+    double A[VD_MAX_REACTIONS], beta[VD_MAX_REACTIONS], Ea[VD_MAX_REACTIONS];
+    double energy[VD_MAX_REACTIONS], eexci[VD_MAX_REACTIONS];
+    int te[VD_MAX_REACTIONS], rxntype[VD_MAX_REACTIONS], elidx[VD_MAX_REACTIONS];
+    int nreact[VD_MAX_REACTIONS], nprod[VD_MAX_REACTIONS];
+    int rsp[VD_MAX_REACTANTS][VD_MAX_REACTIONS];
+    double rnu[VD_MAX_REACTANTS][VD_MAX_REACTIONS];
+    int psp[VD_MAX_PRODUCTS][VD_MAX_REACTIONS];
+    double pnu[VD_MAX_PRODUCTS][VD_MAX_REACTIONS];
+    // CKWC form: net stoichiometry per reaction as (species, net) lists
+    int nnet[VD_MAX_REACTIONS];
+    int netsp[VD_MAX_REACTANTS + VD_MAX_PRODUCTS][VD_MAX_REACTIONS];
+    double netnu[VD_MAX_REACTANTS + VD_MAX_PRODUCTS][VD_MAX_REACTIONS];
+    int order[VD_MAX_REACTIONS];
+};
+
+// specMob / specDiff of the case plug-in (constant mobilities, Einstein
+// diffusion; index nspec = electron energy with 5/3 of the electron values)
+__host__ __device__ inline double spec_mob(const DevMech& m, int s) {
+    return s == m.nspec ? VD_FIVEBYTHREE * m.mob[m.e_idx] : m.mob[s];
+}
+__host__ __device__ inline double spec_diff(const DevMech& m, int s, double Te, double Tg) {
+    if (s == m.nspec) {
+        double te_ev = Te / VD_EV;
+        return VD_FIVEBYTHREE * te_ev * fabs(m.mob[m.e_idx]);
+    }
+    int k = m.diff_kind[s];
+    if (k == VD_DIFF_EINSTEIN_TE) return (Te / VD_EV) * fabs(m.mob[s]);
+    if (k == VD_DIFF_EINSTEIN_TG) return (Tg / VD_EV) * fabs(m.mob[s]);
+    return m.diff[s];
+}
+
+struct Buf {
+    void* p = nullptr;
+    size_t bytes = 0;
+};
+
+}  // namespace vd
+
+struct vd_mech {
+    vd_ctx* ctx;
+    vd_mech_desc host;
+    vd::DevMech hdev;     // host copy of the device struct
+    vd::DevMech* dev;     // device memory
+};
+
+namespace vd {
+struct MGWork;  // multigrid workspace (vd_mg.hip)
+void mgwork_free(MGWork* w);
+}
+
+struct vd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    // named grow-only scratch buffers (device memory), reused across calls
+    std::map<std::string, vd::Buf> scratch;
+    // multigrid workspaces keyed by shape + precision
+    std::map<std::string, vd::MGWork*> mg;
+    // small pinned host buffer for scalar read-backs
+    double* pinned = nullptr;
+    // kernel timing
+    int kt_family = VD_KT_NONE;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> kt_events;
+    std::vector<hipEvent_t> kt_pool;
+    double kt_cells = 0.0;
+    size_t kt_next = 0;
+};
+
+namespace vd {
+
+// returns a device buffer of at least `bytes` registered under `name`
+int scratch(vd_ctx* ctx, const std::string& name, size_t bytes, void** out);
+
+// kernel-timing brackets (no-ops unless the family is selected)
+void kt_begin(vd_ctx* ctx, int family);
+void kt_end(vd_ctx* ctx, int family, double cells);
+
+inline dim3 grid1d(int64_t n, int block) { return dim3((unsigned)((n + block - 1) / block)); }
+
+}  // namespace vd

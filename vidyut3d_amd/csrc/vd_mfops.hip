@@ -1,0 +1,110 @@
+// vd_mfops.hip -- the bulk MultiFab operations of the step
+// (MultiFab::setVal / Copy / LinComb, Source/Evolve.cpp:105-171,290-300).
+#include "vd_internal.hpp"
+
+namespace vd {
+
+struct BoxIter {
+    int3 lo;
+    int nx, ny, nz;
+};
+
+static BoxIter box_iter(const vd_box* b, const vd_array4* a) {
+    BoxIter it;
+    it.lo = make_int3(b->lo[0], b->lo[1], b->lo[2]);
+    it.nx = b->hi[0] - b->lo[0] + 1;
+    it.ny = b->hi[1] - b->lo[1] + 1;
+    it.nz = b->hi[2] - b->lo[2] + 1;
+    (void)a;
+    return it;
+}
+
+__global__ void k_setval(Arr<double> a, BoxIter it, int scomp, double v) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)it.nx * it.ny * it.nz) return;
+    int i = (int)(t % it.nx);
+    int64_t q = t / it.nx;
+    int j = (int)(q % it.ny), k = (int)(q / it.ny);
+    a(i + it.lo.x, j + it.lo.y, k + it.lo.z, scomp + (int)blockIdx.y) = v;
+}
+
+// dst = a*x + b*y  (b == 0 and y == x gives a scaled copy)
+__global__ void k_lincomb(Arr<double> dst, Arr<double> x, Arr<double> y, BoxIter it, int xc, int yc,
+                          int dc, double a, double b, int copy) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)it.nx * it.ny * it.nz) return;
+    int i = (int)(t % it.nx);
+    int64_t q = t / it.nx;
+    int j = (int)(q % it.ny), k = (int)(q / it.ny);
+    i += it.lo.x;
+    j += it.lo.y;
+    k += it.lo.z;
+    int n = (int)blockIdx.y;
+    if (copy)
+        dst(i, j, k, dc + n) = x(i, j, k, xc + n);
+    else
+        dst(i, j, k, dc + n) = a * x(i, j, k, xc + n) + b * y(i, j, k, yc + n);
+}
+
+}  // namespace vd
+
+using namespace vd;
+
+extern "C" int vd_setval(vd_ctx* ctx, vd_array4* a, const vd_box* box, int scomp, int ncomp, double v) {
+    if (!ctx || !a || !box || ncomp < 0 || scomp < 0 || scomp + ncomp > a->ncomp) return VD_EINVAL;
+    if (!ncomp) return VD_OK;
+    BoxIter it = box_iter(box, a);
+    int64_t tot = (int64_t)it.nx * it.ny * it.nz;
+    hipLaunchKernelGGL(k_setval, dim3((unsigned)((tot + 255) / 256), ncomp), dim3(256), 0, ctx->stream,
+                       arr(a), it, scomp, v);
+    VD_CHECK_LAUNCH();
+    return VD_OK;
+}
+
+extern "C" int vd_copy(vd_ctx* ctx, const vd_array4* src, vd_array4* dst, const vd_box* box, int scomp,
+                       int dcomp, int ncomp) {
+    if (!ctx || !src || !dst || !box || ncomp < 0) return VD_EINVAL;
+    if (scomp + ncomp > src->ncomp || dcomp + ncomp > dst->ncomp) return VD_EINVAL;
+    if (!ncomp) return VD_OK;
+    BoxIter it = box_iter(box, dst);
+    int64_t tot = (int64_t)it.nx * it.ny * it.nz;
+    hipLaunchKernelGGL(k_lincomb, dim3((unsigned)((tot + 255) / 256), ncomp), dim3(256), 0, ctx->stream,
+                       arr(dst), arr(src), arr(src), it, scomp, scomp, dcomp, 1.0, 0.0, 1);
+    VD_CHECK_LAUNCH();
+    return VD_OK;
+}
+
+extern "C" int vd_lincomb(vd_ctx* ctx, vd_array4* dst, double a, const vd_array4* x, int xcomp, double b,
+                          const vd_array4* y, int ycomp, int dcomp, int ncomp, const vd_box* box) {
+    if (!ctx || !dst || !x || !y || !box || ncomp < 0) return VD_EINVAL;
+    if (!ncomp) return VD_OK;
+    BoxIter it = box_iter(box, dst);
+    int64_t tot = (int64_t)it.nx * it.ny * it.nz;
+    hipLaunchKernelGGL(k_lincomb, dim3((unsigned)((tot + 255) / 256), ncomp), dim3(256), 0, ctx->stream,
+                       arr(dst), arr(x), arr(y), it, xcomp, ycomp, dcomp, a, b, 0);
+    VD_CHECK_LAUNCH();
+    return VD_OK;
+}
+
+namespace vd {
+__global__ void k_clamp_min(Arr<double> a, BoxIter it, int comp, double lo) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)it.nx * it.ny * it.nz) return;
+    int i = (int)(t % it.nx);
+    int64_t q = t / it.nx;
+    int j = (int)(q % it.ny), k = (int)(q / it.ny);
+    double& v = a(i + it.lo.x, j + it.lo.y, k + it.lo.z, comp);
+    if (v < lo) v = lo;
+}
+}  // namespace vd
+
+// bounded background-species update floor (Source/Evolve.cpp:278-283)
+extern "C" int vd_clamp_min(vd_ctx* ctx, vd_array4* a, const vd_box* box, int comp, double lo) {
+    if (!ctx || !a || !box || comp < 0 || comp >= a->ncomp) return VD_EINVAL;
+    BoxIter it = box_iter(box, a);
+    int64_t tot = (int64_t)it.nx * it.ny * it.nz;
+    hipLaunchKernelGGL(k_clamp_min, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, arr(a),
+                       it, comp, lo);
+    VD_CHECK_LAUNCH();
+    return VD_OK;
+}

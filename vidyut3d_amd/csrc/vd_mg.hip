@@ -1,0 +1,1511 @@
+// vd_mg.hip -- geometric multigrid for the cell-centred ABecLaplacian
+//   alpha_a*phi - beta*div(b grad phi) = f
+// used by the potential solve (Source/PotentialSolve.cpp:148-336) and the
+// implicit species solves (Source/ScalarSolve.cpp:420-596).
+//
+// Conventions (identical in the CPU oracle):
+//  * stencil: dg = alpha_a + sum_d s_d*dd_d, of = sum_d s_d*oo_d,
+//    s_d = beta/dx_d^2, (L x)_c = dg*x_c - of; boundary neighbours are
+//    ghosts c0*x_c (+cinh on the finest level, folded into the rhs)
+//  * Dirichlet (maxorder 2) c0=-1 cinh=2v; Neumann c0=1; inhomogeneous
+//    Neumann c0=1 cinh=-/+dx*g; Robin c0=-(a/2-b/dx)/(a/2+b/dx),
+//    cinh=f/(a/2+b/dx)
+//  * correction-form V-cycle: red-black Gauss-Seidel (colour (i+j+k)&1,
+//    red first), residual fused with cell-average restriction, linear
+//    prolongation, face b and boundary c0 averaged to the coarse level,
+//    CG bottom solve in one workgroup
+//  * stop when ||r||_inf <= max(abstol, reltol*||r0||_inf)
+//  * precision 1: the V-cycle runs in FP32 on FP32 copies of the
+//    hierarchy; the finest-level residual, the update x += e and the
+//    stopping test stay FP64 (iterative refinement).
+#include <algorithm>
+#include <cmath>
+
+#include "vd_internal.hpp"
+
+namespace vd {
+
+int fill_value(vd_ctx* ctx, double* p, int64_t n, double v);
+int fill_value_f(vd_ctx* ctx, float* p, int64_t n, float v);
+
+// ------------------------------------------------------------ data
+template <class T>
+struct LevV {  // kernel view of one level
+    int nx, ny, nz;
+    T s0, s1, s2;        // beta/dx^2
+    T alpha, bconst;
+    const T* bf0;
+    const T* bf1;
+    const T* bf2;
+    const T* c0[3][2];   // per boundary face c0 (non-periodic sides)
+    int per0, per1, per2;
+};
+
+template <class T>
+struct Lev {
+    int n[3];
+    double dx[3];
+    T* bf[3] = {nullptr, nullptr, nullptr};
+    T* c0[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    T* e = nullptr;
+    T* r = nullptr;
+    T* p = nullptr;  // CG scratch (bottom level)
+    T* q = nullptr;
+    T* w = nullptr;
+};
+
+struct MGWork {
+    int dim = 3, nlev = 0, prec = 0, constb = 0;
+    int n[3] = {1, 1, 1};
+    int per[3] = {0, 0, 0};
+    double dx[3] = {1, 1, 1};
+    std::vector<void*> allocs;
+    // FP64 finest-level data
+    double* x = nullptr;
+    double* f = nullptr;
+    double* res = nullptr;
+    double* bf0[3] = {nullptr, nullptr, nullptr};
+    double* c00[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    double* bcv[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    double* rob[3][2][3];  // robin a, b, f per side
+    unsigned long long* dnorm = nullptr;
+    std::vector<Lev<double>> L64;
+    std::vector<Lev<float>> L32;
+};
+
+void mgwork_free(MGWork* w) {
+    if (!w) return;
+    for (void* p : w->allocs) (void)hipFree(p);
+    delete w;
+}
+
+static int dalloc(MGWork* w, size_t bytes, void** out) {
+    hipError_t e = hipMalloc(out, bytes < 64 ? 64 : bytes);
+    if (e != hipSuccess) {
+        set_error("multigrid workspace: hipMalloc %zu: %s", bytes, hipGetErrorString(e));
+        return VD_ENOMEM;
+    }
+    w->allocs.push_back(*out);
+    return VD_OK;
+}
+
+static inline int64_t ncell(const int n[3]) { return (int64_t)n[0] * n[1] * n[2]; }
+static inline int64_t nfacef(const int n[3], int d) {
+    return (int64_t)(n[0] + (d == 0)) * (n[1] + (d == 1)) * (n[2] + (d == 2));
+}
+static inline int64_t ntanf(const int n[3], int d) {
+    return d == 0 ? (int64_t)n[1] * n[2] : (d == 1 ? (int64_t)n[0] * n[2] : (int64_t)n[0] * n[1]);
+}
+
+// ------------------------------------------------------------ stencil
+template <class T>
+__device__ __forceinline__ int fidx(const LevV<T>& L, int d, int i, int j, int k) {
+    int n0 = L.nx + (d == 0), n1 = L.ny + (d == 1);
+    return i + n0 * (j + n1 * k);
+}
+template <class T>
+__device__ __forceinline__ int tidx(const LevV<T>& L, int d, int i, int j, int k) {
+    return d == 0 ? j + L.ny * k : (d == 1 ? i + L.nx * k : i + L.nx * j);
+}
+
+template <class T, bool CB, int DIM>
+__device__ __forceinline__ void stencil(const LevV<T>& L, const T* __restrict__ x, int i, int j,
+                                        int k, T& dgo, T& ofo) {
+    const int c = i + L.nx * (j + L.ny * k);
+    T dg = L.alpha, of = T(0);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        const int cd = d == 0 ? i : (d == 1 ? j : k);
+        const int nd = d == 0 ? L.nx : (d == 1 ? L.ny : L.nz);
+        const int st = d == 0 ? 1 : (d == 1 ? L.nx : L.nx * L.ny);
+        const int per = d == 0 ? L.per0 : (d == 1 ? L.per1 : L.per2);
+        const T s = d == 0 ? L.s0 : (d == 1 ? L.s1 : L.s2);
+        T blo, bhi;
+        if (CB) {
+            blo = bhi = L.bconst;
+        } else {
+            const T* bf = d == 0 ? L.bf0 : (d == 1 ? L.bf1 : L.bf2);
+            const int f = fidx(L, d, i, j, k);
+            const int fs = d == 0 ? 1 : (d == 1 ? (L.nx) : (L.nx * L.ny));
+            blo = bf[f];
+            bhi = bf[f + fs];
+        }
+        T dd = T(0), oo = T(0);
+        if (cd > 0) {
+            dd += blo;
+            oo += blo * x[c - st];
+        } else if (per) {
+            dd += blo;
+            oo += blo * x[c + (nd - 1) * st];
+        } else {
+            dd += blo * (T(1) - L.c0[d][0][tidx(L, d, i, j, k)]);
+        }
+        if (cd < nd - 1) {
+            dd += bhi;
+            oo += bhi * x[c + st];
+        } else if (per) {
+            dd += bhi;
+            oo += bhi * x[c - (nd - 1) * st];
+        } else {
+            dd += bhi * (T(1) - L.c0[d][1][tidx(L, d, i, j, k)]);
+        }
+        dg += s * dd;
+        of += s * oo;
+    }
+    dgo = dg;
+    ofo = of;
+}
+
+// ------------------------------------------------------------ kernels
+template <class T, bool CB, int DIM>
+__global__ void __launch_bounds__(256) k_gsrb(LevV<T> L, T* __restrict__ x, const T* __restrict__ rhs,
+                                              int color) {
+    const int nh = (L.nx + 1) >> 1;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nh * L.ny * L.nz) return;
+    int ii = (int)(t % nh);
+    int row = (int)(t / nh);
+    int j = row % L.ny, k = row / L.ny;
+    int i = 2 * ii + ((color + j + k) & 1);
+    if (i >= L.nx) return;
+    T dg, of;
+    stencil<T, CB, DIM>(L, x, i, j, k, dg, of);
+    int c = i + L.nx * row;
+    x[c] = (rhs[c] + of) / dg;
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v) {
+    if (v != v) v = __builtin_huge_val();
+    atomicMax(p, (unsigned long long)__double_as_longlong(v));
+}
+
+// r = f - L x with the max-norm (finest level, FP64)
+template <bool CB, int DIM>
+__global__ void __launch_bounds__(256) k_resid0(LevV<double> L, const double* __restrict__ x,
+                                                const double* __restrict__ f, double* __restrict__ r,
+                                                unsigned long long* __restrict__ nrm) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t tot = (int64_t)L.nx * L.ny * L.nz;
+    double a = 0.0;
+    if (t < tot) {
+        int i = (int)(t % L.nx);
+        int row = (int)(t / L.nx);
+        int j = row % L.ny, k = row / L.ny;
+        double dg, of;
+        stencil<double, CB, DIM>(L, x, i, j, k, dg, of);
+        double v = f[t] - (dg * x[t] - of);
+        r[t] = v;
+        a = fabs(v);
+        if (a != a) a = __builtin_huge_val();
+    }
+    // wave64 max, then one atomic per wave
+    for (int o = 32; o > 0; o >>= 1) a = fmax(a, __shfl_xor(a, o, 64));
+    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(nrm, a);
+}
+
+// coarse r = average of the 2^DIM fine residuals rhs - L e (fused)
+template <class T, bool CB, int DIM>
+__global__ void __launch_bounds__(256) k_resid_restrict(LevV<T> F, const T* __restrict__ e,
+                                                        const T* __restrict__ rhs, int cnx, int cny,
+                                                        int cnz, T* __restrict__ rc) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)cnx * cny * cnz) return;
+    int I = (int)(t % cnx);
+    int row = (int)(t / cnx);
+    int J = row % cny, K = row / cny;
+    T s = T(0);
+    const int kz = DIM == 3 ? 2 : 1;
+    for (int kk = 0; kk < kz; kk++)
+        for (int jj = 0; jj < 2; jj++)
+            for (int ii = 0; ii < 2; ii++) {
+                int i = 2 * I + ii, j = 2 * J + jj, k = DIM == 3 ? 2 * K + kk : 0;
+                T dg, of;
+                stencil<T, CB, DIM>(F, e, i, j, k, dg, of);
+                int c = i + F.nx * (j + F.ny * k);
+                s += rhs[c] - (dg * e[c] - of);
+            }
+    rc[t] = s * (DIM == 3 ? T(0.125) : T(0.25));
+}
+
+// e_f += P e_c (linear, tensor 3/4-1/4; ghosts of the coarse correction
+// are c0 * interior, periodic wraps)
+template <class T, int DIM>
+__global__ void __launch_bounds__(256) k_prolong(LevV<T> C, const T* __restrict__ ec, int fnx, int fny,
+                                                 int fnz, T* __restrict__ ef) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)fnx * fny * fnz) return;
+    int i = (int)(t % fnx);
+    int row = (int)(t / fnx);
+    int j = row % fny, k = row / fny;
+    int f[3] = {i, j, k};
+    int cn[3] = {C.nx, C.ny, C.nz};
+    int pr[3] = {C.per0, C.per1, C.per2};
+    int ci[3][2];
+    bool ghost[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (d >= DIM) {
+            ci[d][0] = ci[d][1] = 0;
+            ghost[d] = false;
+            continue;
+        }
+        int I = f[d] >> 1, o = f[d] & 1;
+        int nb = o ? I + 1 : I - 1;
+        ghost[d] = false;
+        if (nb < 0 || nb >= cn[d]) {
+            if (pr[d]) nb = nb < 0 ? cn[d] - 1 : 0;
+            else {
+                ghost[d] = true;
+                nb = I;
+            }
+        }
+        ci[d][0] = I;
+        ci[d][1] = nb;
+    }
+    T val = T(0);
+    const int kn = DIM == 3 ? 2 : 1;
+    for (int kz = 0; kz < kn; kz++)
+        for (int jy = 0; jy < 2; jy++)
+            for (int ix = 0; ix < 2; ix++) {
+                int cc0 = ci[0][ix], cc1 = ci[1][jy], cc2 = ci[2][kz];
+                int sel[3] = {ix, jy, kz};
+                T v = ec[cc0 + C.nx * (cc1 + C.ny * cc2)];
+#pragma unroll
+                for (int d = 0; d < DIM; d++)
+                    if (sel[d] == 1 && ghost[d]) {
+                        int side = f[d] & 1;
+                        v = v * C.c0[d][side][tidx(C, d, cc0, cc1, cc2)];
+                    }
+                T w = (ix == 0 ? T(0.75) : T(0.25)) * (jy == 0 ? T(0.75) : T(0.25)) *
+                      (DIM == 3 ? (kz == 0 ? T(0.75) : T(0.25)) : T(1));
+                val += w * v;
+            }
+    ef[t] += val;
+}
+
+// CG bottom solve, one workgroup; dots accumulated in double
+template <class T, bool CB, int DIM>
+__global__ void __launch_bounds__(1024) k_bottom_cg(LevV<T> L, T* __restrict__ e, const T* __restrict__ b,
+                                                    T* __restrict__ r, T* __restrict__ p,
+                                                    T* __restrict__ q) {
+    __shared__ double red[1024 / 64];
+    __shared__ double bc;
+    const int n = L.nx * L.ny * L.nz;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    auto bdot = [&](const T* u, const T* v) -> double {
+        double s = 0.0;
+        for (int c = tid; c < n; c += nt) s += (double)u[c] * (double)v[c];
+        for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+        __syncthreads();
+        if ((tid & 63) == 0) red[tid >> 6] = s;
+        __syncthreads();
+        if (tid == 0) {
+            double z = 0.0;
+            for (int w = 0; w < (nt + 63) / 64; w++) z += red[w];
+            bc = z;
+        }
+        __syncthreads();
+        return bc;
+    };
+    for (int c = tid; c < n; c += nt) {
+        e[c] = T(0);
+        r[c] = b[c];
+        p[c] = b[c];
+    }
+    __syncthreads();
+    double rr = bdot(r, r), rr0 = rr;
+    const int maxit = 4 * n + 20;
+    for (int it = 0; it < maxit; it++) {
+        if (rr <= 1e-28 * rr0 || rr == 0.0) break;
+        for (int c = tid; c < n; c += nt) {
+            int i = c % L.nx, row = c / L.nx;
+            int j = row % L.ny, k = row / L.ny;
+            T dg, of;
+            stencil<T, CB, DIM>(L, p, i, j, k, dg, of);
+            q[c] = dg * p[c] - of;
+        }
+        __syncthreads();
+        double pq = bdot(p, q);
+        if (pq == 0.0) break;
+        double alpha = rr / pq;
+        for (int c = tid; c < n; c += nt) {
+            e[c] += T(alpha) * p[c];
+            r[c] -= T(alpha) * q[c];
+        }
+        __syncthreads();
+        double rn = bdot(r, r);
+        double beta = rn / rr;
+        for (int c = tid; c < n; c += nt) p[c] = r[c] + T(beta) * p[c];
+        __syncthreads();
+        rr = rn;
+    }
+}
+
+// coarse face b = average of the covering fine faces
+template <class T>
+__global__ void k_coarsen_faces(const T* __restrict__ fb, int fnx, int fny, int d, int dim, int cnx,
+                                int cny, int cnz, T* __restrict__ cb) {
+    int n0 = cnx + (d == 0), n1 = cny + (d == 1), n2 = cnz + (d == 2);
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n0 * n1 * n2) return;
+    int i = (int)(t % n0);
+    int row = (int)(t / n0);
+    int j = row % n1, k = row / n1;
+    int b[3] = {2 * i, 2 * j, dim == 3 ? 2 * k : 0};
+    int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+    int nt2 = dim == 3 ? 2 : 1;
+    int f0 = fnx + (d == 0), f1 = fny + (d == 1);
+    T s = T(0);
+    for (int b2 = 0; b2 < nt2; b2++)
+        for (int b1 = 0; b1 < 2; b1++) {
+            int f[3] = {b[0], b[1], b[2]};
+            f[t1] += b1;
+            if (dim == 3) f[t2] += b2;
+            s += fb[f[0] + f0 * (f[1] + f1 * f[2])];
+        }
+    cb[t] = s * (dim == 3 ? T(0.25) : T(0.5));
+}
+
+template <class T>
+__global__ void k_coarsen_c0(const T* __restrict__ fc, int fnx, int fny, int d, int dim, int cnx,
+                             int cny, int cnz, T* __restrict__ cc) {
+    int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
+    int cn[3] = {cnx, cny, cnz}, fn[3] = {fnx, fny, 0};
+    int nt1 = cn[t1], nt2 = dim == 3 ? cn[t2] : 1;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nt1 * nt2) return;
+    int b1 = (int)(t % nt1), b2 = (int)(t / nt1);
+    T s = T(0);
+    int nk = dim == 3 ? 2 : 1;
+    for (int q2 = 0; q2 < nk; q2++)
+        for (int q1 = 0; q1 < 2; q1++) {
+            int f[3] = {0, 0, 0};
+            f[t1] = 2 * b1 + q1;
+            if (dim == 3) f[t2] = 2 * b2 + q2;
+            int ti = d == 0 ? f[1] + fn[1] * f[2] : (d == 1 ? f[0] + fn[0] * f[2] : f[0] + fn[0] * f[1]);
+            s += fc[ti];
+        }
+    int c[3] = {0, 0, 0};
+    c[t1] = b1;
+    if (dim == 3) c[t2] = b2;
+    int ci = d == 0 ? c[1] + cny * c[2] : (d == 1 ? c[0] + cnx * c[2] : c[0] + cnx * c[1]);
+    cc[ci] = s * (dim == 3 ? T(0.25) : T(0.5));
+}
+
+template <class A, class B>
+__global__ void k_convert(const A* __restrict__ a, B* __restrict__ b, int64_t n) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) b[t] = (B)a[t];
+}
+
+template <class T>
+__global__ void k_zero(T* __restrict__ p, int64_t n) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = T(0);
+}
+
+template <class T>
+__global__ void k_axpy_to_double(double* __restrict__ x, const T* __restrict__ e, int64_t n) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) x[t] += (double)e[t];
+}
+
+__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+
+// BC coefficient for one face (ghost = c0*x + cinh)
+__device__ __host__ inline void bc_coef(int type, int side, double dx, double val, double ra,
+                                        double rb, double rf, double& c0, double& cinh) {
+    if (type == VD_DIRCBC) {
+        c0 = -1.0;
+        cinh = 2.0 * val;
+    } else if (type == VD_HNEUBC) {
+        c0 = 1.0;
+        cinh = 0.0;
+    } else if (type == VD_IHNEUBC) {
+        c0 = 1.0;
+        cinh = (side == 0 ? -dx : dx) * val;
+    } else {
+        double den = ra / 2.0 + rb / dx;
+        c0 = -(ra / 2.0 - rb / dx) / den;
+        cinh = rf / den;
+    }
+}
+
+struct SideArgs {
+    int d, side, type, dim;
+    int nx, ny, nz;
+    double dx, s;      // dx_d, beta/dx_d^2
+    const double* bcv;
+    const double* ra;
+    const double* rb;
+    const double* rf;
+    const double* bf;  // finest faces of dir d (null => bconst)
+    double bconst;
+    double* c0;
+    double* f;         // rhs to shift
+};
+
+// finest-level c0 and rhs shift s*b*cinh for one boundary side
+__global__ void k_side_setup(SideArgs a) {
+    int t1 = a.d == 0 ? 1 : 0, t2 = a.d == 2 ? 1 : 2;
+    int n[3] = {a.nx, a.ny, a.nz};
+    int nt1 = n[t1], nt2 = a.dim == 3 ? n[t2] : 1;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nt1 * nt2) return;
+    int b1 = (int)(t % nt1), b2 = (int)(t / nt1);
+    int c[3] = {0, 0, 0};
+    c[t1] = b1;
+    if (a.dim == 3) c[t2] = b2;
+    c[a.d] = a.side ? n[a.d] - 1 : 0;
+    int ti = a.d == 0 ? c[1] + a.ny * c[2] : (a.d == 1 ? c[0] + a.nx * c[2] : c[0] + a.nx * c[1]);
+    double val = a.bcv ? a.bcv[ti] : 0.0;
+    double ra = 0.0, rb = 1.0, rf = 0.0;
+    if (a.type == VD_ROBINBC && a.ra) {
+        ra = a.ra[ti];
+        rb = a.rb[ti];
+        rf = a.rf[ti];
+    }
+    double c0, cinh;
+    bc_coef(a.type, a.side, a.dx, val, ra, rb, rf, c0, cinh);
+    a.c0[ti] = c0;
+    int fc[3] = {c[0], c[1], c[2]};
+    fc[a.d] += a.side;
+    int n0 = a.nx + (a.d == 0), n1 = a.ny + (a.d == 1);
+    double b = a.bf ? a.bf[fc[0] + n0 * (fc[1] + n1 * fc[2])] : a.bconst;
+    int cell = c[0] + a.nx * (c[1] + a.ny * c[2]);
+    a.f[cell] += a.s * b * cinh;
+}
+
+// ------------------------------------------------------------ host side
+template <class T>
+static LevV<T> view(const MGWork& W, const Lev<T>& L, double alpha, double beta, double bconst) {
+    LevV<T> v;
+    v.nx = L.n[0];
+    v.ny = L.n[1];
+    v.nz = L.n[2];
+    double s[3];
+    for (int d = 0; d < 3; d++) s[d] = d < W.dim ? beta / (L.dx[d] * L.dx[d]) : 0.0;
+    v.s0 = (T)s[0];
+    v.s1 = (T)s[1];
+    v.s2 = (T)s[2];
+    v.alpha = (T)alpha;
+    v.bconst = (T)bconst;
+    v.bf0 = L.bf[0];
+    v.bf1 = L.bf[1];
+    v.bf2 = L.bf[2];
+    for (int d = 0; d < 3; d++)
+        for (int sd = 0; sd < 2; sd++) v.c0[d][sd] = L.c0[d][sd];
+    v.per0 = W.per[0];
+    v.per1 = W.per[1];
+    v.per2 = W.per[2];
+    return v;
+}
+
+template <class T>
+static int alloc_levels(MGWork* W, std::vector<Lev<T>>& L, bool share0) {
+    L.resize(W->nlev);
+    int n[3] = {W->n[0], W->n[1], W->n[2]};
+    double dx[3] = {W->dx[0], W->dx[1], W->dx[2]};
+    for (int l = 0; l < W->nlev; l++) {
+        Lev<T>& v = L[l];
+        for (int d = 0; d < 3; d++) {
+            v.n[d] = n[d];
+            v.dx[d] = dx[d];
+        }
+        int64_t nc = ncell(n);
+        void* p;
+        VD_TRY(dalloc(W, nc * sizeof(T), &p));
+        v.e = (T*)p;
+        if (!(share0 && l == 0)) {
+            VD_TRY(dalloc(W, nc * sizeof(T), &p));
+            v.r = (T*)p;
+        }
+        if (l == W->nlev - 1) {
+            VD_TRY(dalloc(W, nc * sizeof(T), &p));
+            v.p = (T*)p;
+            VD_TRY(dalloc(W, nc * sizeof(T), &p));
+            v.q = (T*)p;
+            VD_TRY(dalloc(W, nc * sizeof(T), &p));
+            v.w = (T*)p;
+        }
+        for (int d = 0; d < W->dim; d++) {
+            if (!W->constb && !(share0 && l == 0)) {
+                VD_TRY(dalloc(W, nfacef(n, d) * sizeof(T), &p));
+                v.bf[d] = (T*)p;
+            }
+            if (!W->per[d] && !(share0 && l == 0))
+                for (int s = 0; s < 2; s++) {
+                    VD_TRY(dalloc(W, ntanf(n, d) * sizeof(T), &p));
+                    v.c0[d][s] = (T*)p;
+                }
+        }
+        for (int d = 0; d < W->dim; d++) {
+            n[d] /= 2;
+            dx[d] *= 2.0;
+        }
+    }
+    return VD_OK;
+}
+
+// workspace for a shape / precision / coefficient kind, cached in the ctx
+static int get_work(vd_ctx* ctx, const vd_geom* g, const vd_box* box, int prec, int constb,
+                    int maxc, MGWork** out) {
+    int n[3], per[3];
+    for (int d = 0; d < 3; d++) {
+        n[d] = d < g->dim ? box->hi[d] - box->lo[d] + 1 : 1;
+        per[d] = d < g->dim ? g->is_periodic[d] : 0;
+    }
+    char key[160];
+    snprintf(key, sizeof(key), "%d:%dx%dx%d:p%d%d%d:prec%d:cb%d:mc%d:dx%.17g,%.17g,%.17g", g->dim, n[0],
+             n[1], n[2], per[0], per[1], per[2], prec, constb, maxc, g->dx[0], g->dx[1], g->dx[2]);
+    auto it = ctx->mg.find(key);
+    if (it != ctx->mg.end()) {
+        *out = it->second;
+        return VD_OK;
+    }
+    MGWork* W = new MGWork();
+    W->dim = g->dim;
+    W->prec = prec;
+    W->constb = constb;
+    for (int d = 0; d < 3; d++) {
+        W->n[d] = n[d];
+        W->per[d] = per[d];
+        W->dx[d] = d < g->dim ? g->dx[d] : 1.0;
+        for (int s = 0; s < 2; s++)
+            for (int q = 0; q < 3; q++) W->rob[d][s][q] = nullptr;
+    }
+    // coarsening: every dimension even and >= 4
+    int nl = 1, m[3] = {n[0], n[1], n[2]};
+    while (nl - 1 < maxc) {
+        bool ok = true;
+        for (int d = 0; d < g->dim; d++)
+            if (m[d] % 2 != 0 || m[d] < 4) ok = false;
+        if (!ok) break;
+        for (int d = 0; d < g->dim; d++) m[d] /= 2;
+        nl++;
+    }
+    W->nlev = nl;
+    int64_t nc = ncell(n);
+    void* p;
+    int st = VD_OK;
+    if ((st = dalloc(W, nc * 8, &p))) goto fail;
+    W->x = (double*)p;
+    if ((st = dalloc(W, nc * 8, &p))) goto fail;
+    W->f = (double*)p;
+    if ((st = dalloc(W, nc * 8, &p))) goto fail;
+    W->res = (double*)p;
+    if ((st = dalloc(W, 64, &p))) goto fail;
+    W->dnorm = (unsigned long long*)p;
+    for (int d = 0; d < g->dim; d++) {
+        if (!constb) {
+            if ((st = dalloc(W, nfacef(n, d) * 8, &p))) goto fail;
+            W->bf0[d] = (double*)p;
+        }
+        if (!per[d])
+            for (int s = 0; s < 2; s++) {
+                if ((st = dalloc(W, ntanf(n, d) * 8, &p))) goto fail;
+                W->c00[d][s] = (double*)p;
+                if ((st = dalloc(W, ntanf(n, d) * 8, &p))) goto fail;
+                W->bcv[d][s] = (double*)p;
+                for (int q = 0; q < 3; q++) {
+                    if ((st = dalloc(W, ntanf(n, d) * 8, &p))) goto fail;
+                    W->rob[d][s][q] = (double*)p;
+                }
+            }
+    }
+    if (prec == 0) {
+        if ((st = alloc_levels<double>(W, W->L64, true))) goto fail;
+        W->L64[0].r = W->res;
+        for (int d = 0; d < g->dim; d++) {
+            W->L64[0].bf[d] = W->bf0[d];
+            for (int s = 0; s < 2; s++) W->L64[0].c0[d][s] = W->c00[d][s];
+        }
+    } else {
+        if ((st = alloc_levels<float>(W, W->L32, false))) goto fail;
+    }
+    ctx->mg[key] = W;
+    *out = W;
+    return VD_OK;
+fail:
+    mgwork_free(W);
+    return st;
+}
+
+template <class T>
+static void launch_gsrb(vd_ctx* ctx, const MGWork& W, const LevV<T>& v, T* x, const T* rhs, int color,
+                        bool fine) {
+    int64_t nt = (int64_t)((v.nx + 1) / 2) * v.ny * v.nz;
+    dim3 g = grid1d(nt, 256), b(256);
+    if (fine) kt_begin(ctx, VD_KT_GSRB);
+    if (W.constb) {
+        if (W.dim == 3) hipLaunchKernelGGL((k_gsrb<T, true, 3>), g, b, 0, ctx->stream, v, x, rhs, color);
+        else hipLaunchKernelGGL((k_gsrb<T, true, 2>), g, b, 0, ctx->stream, v, x, rhs, color);
+    } else {
+        if (W.dim == 3) hipLaunchKernelGGL((k_gsrb<T, false, 3>), g, b, 0, ctx->stream, v, x, rhs, color);
+        else hipLaunchKernelGGL((k_gsrb<T, false, 2>), g, b, 0, ctx->stream, v, x, rhs, color);
+    }
+    if (fine) kt_end(ctx, VD_KT_GSRB, (double)v.nx * v.ny * v.nz * 0.5);
+}
+
+template <class T>
+static void vcycle(vd_ctx* ctx, const MGWork& W, std::vector<Lev<T>>& L, double alpha, double beta,
+                   double bconst, int nu1, int nu2) {
+    const int nl = W.nlev;
+    // down
+    for (int l = 0; l < nl - 1; l++) {
+        Lev<T>& F = L[l];
+        LevV<T> v = view<T>(W, F, alpha, beta, bconst);
+        int64_t nc = ncell(F.n);
+        hipLaunchKernelGGL(k_zero<T>, grid1d(nc, 256), dim3(256), 0, ctx->stream, F.e, nc);
+        for (int s = 0; s < nu1; s++) {
+            launch_gsrb<T>(ctx, W, v, F.e, F.r, 0, l == 0);
+            launch_gsrb<T>(ctx, W, v, F.e, F.r, 1, l == 0);
+        }
+        Lev<T>& Cc = L[l + 1];
+        int64_t ncc = ncell(Cc.n);
+        dim3 g = grid1d(ncc, 256), b(256);
+        if (W.constb) {
+            if (W.dim == 3)
+                hipLaunchKernelGGL((k_resid_restrict<T, true, 3>), g, b, 0, ctx->stream, v, F.e, F.r,
+                                   Cc.n[0], Cc.n[1], Cc.n[2], Cc.r);
+            else
+                hipLaunchKernelGGL((k_resid_restrict<T, true, 2>), g, b, 0, ctx->stream, v, F.e, F.r,
+                                   Cc.n[0], Cc.n[1], Cc.n[2], Cc.r);
+        } else {
+            if (W.dim == 3)
+                hipLaunchKernelGGL((k_resid_restrict<T, false, 3>), g, b, 0, ctx->stream, v, F.e, F.r,
+                                   Cc.n[0], Cc.n[1], Cc.n[2], Cc.r);
+            else
+                hipLaunchKernelGGL((k_resid_restrict<T, false, 2>), g, b, 0, ctx->stream, v, F.e, F.r,
+                                   Cc.n[0], Cc.n[1], Cc.n[2], Cc.r);
+        }
+    }
+    // bottom
+    {
+        Lev<T>& B = L[nl - 1];
+        LevV<T> v = view<T>(W, B, alpha, beta, bconst);
+        int threads = 1024;
+        if (W.constb) {
+            if (W.dim == 3)
+                hipLaunchKernelGGL((k_bottom_cg<T, true, 3>), dim3(1), dim3(threads), 0, ctx->stream, v,
+                                   B.e, B.r, B.w, B.p, B.q);
+            else
+                hipLaunchKernelGGL((k_bottom_cg<T, true, 2>), dim3(1), dim3(threads), 0, ctx->stream, v,
+                                   B.e, B.r, B.w, B.p, B.q);
+        } else {
+            if (W.dim == 3)
+                hipLaunchKernelGGL((k_bottom_cg<T, false, 3>), dim3(1), dim3(threads), 0, ctx->stream,
+                                   v, B.e, B.r, B.w, B.p, B.q);
+            else
+                hipLaunchKernelGGL((k_bottom_cg<T, false, 2>), dim3(1), dim3(threads), 0, ctx->stream,
+                                   v, B.e, B.r, B.w, B.p, B.q);
+        }
+    }
+    // up
+    for (int l = nl - 2; l >= 0; l--) {
+        Lev<T>& F = L[l];
+        Lev<T>& Cc = L[l + 1];
+        LevV<T> vc = view<T>(W, Cc, alpha, beta, bconst);
+        LevV<T> v = view<T>(W, F, alpha, beta, bconst);
+        int64_t nf = ncell(F.n);
+        if (W.dim == 3)
+            hipLaunchKernelGGL((k_prolong<T, 3>), grid1d(nf, 256), dim3(256), 0, ctx->stream, vc, Cc.e,
+                               F.n[0], F.n[1], F.n[2], F.e);
+        else
+            hipLaunchKernelGGL((k_prolong<T, 2>), grid1d(nf, 256), dim3(256), 0, ctx->stream, vc, Cc.e,
+                               F.n[0], F.n[1], F.n[2], F.e);
+        for (int s = 0; s < nu2; s++) {
+            launch_gsrb<T>(ctx, W, v, F.e, F.r, 0, l == 0);
+            launch_gsrb<T>(ctx, W, v, F.e, F.r, 1, l == 0);
+        }
+    }
+}
+
+template <class T>
+static void coarsen_hierarchy(vd_ctx* ctx, MGWork& W, std::vector<Lev<T>>& L) {
+    for (int l = 1; l < W.nlev; l++) {
+        Lev<T>& F = L[l - 1];
+        Lev<T>& Cc = L[l];
+        for (int d = 0; d < W.dim; d++) {
+            if (!W.constb) {
+                int64_t nf = nfacef(Cc.n, d);
+                hipLaunchKernelGGL(k_coarsen_faces<T>, grid1d(nf, 256), dim3(256), 0, ctx->stream,
+                                   F.bf[d], F.n[0], F.n[1], d, W.dim, Cc.n[0], Cc.n[1], Cc.n[2], Cc.bf[d]);
+            }
+            if (!W.per[d])
+                for (int s = 0; s < 2; s++) {
+                    int64_t nt = ntanf(Cc.n, d);
+                    hipLaunchKernelGGL(k_coarsen_c0<T>, grid1d(nt, 256), dim3(256), 0, ctx->stream,
+                                       F.c0[d][s], F.n[0], F.n[1], d, W.dim, Cc.n[0], Cc.n[1], Cc.n[2],
+                                       Cc.c0[d][s]);
+                }
+        }
+    }
+}
+
+static int resid0(vd_ctx* ctx, MGWork& W, const LevV<double>& v0, double* nrm_out) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, ctx->stream, W.dnorm);
+    int64_t nc = ncell(W.n);
+    dim3 g = grid1d(nc, 256), b(256);
+    kt_begin(ctx, VD_KT_RESID);
+    if (W.constb) {
+        if (W.dim == 3) hipLaunchKernelGGL((k_resid0<true, 3>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, W.dnorm);
+        else hipLaunchKernelGGL((k_resid0<true, 2>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, W.dnorm);
+    } else {
+        if (W.dim == 3) hipLaunchKernelGGL((k_resid0<false, 3>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, W.dnorm);
+        else hipLaunchKernelGGL((k_resid0<false, 2>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, W.dnorm);
+    }
+    kt_end(ctx, VD_KT_RESID, (double)nc);
+    VD_CHECK_LAUNCH();
+    VD_HIP(hipMemcpyAsync(ctx->pinned, W.dnorm, 8, hipMemcpyDeviceToHost, ctx->stream));
+    VD_HIP(hipStreamSynchronize(ctx->stream));
+    *nrm_out = ctx->pinned[0];
+    return VD_OK;
+}
+
+// Solve with the finest-level data already in W (x, f with BC shift,
+// bf0 / bconst, c00).  Builds the coarse hierarchy, iterates.
+static int mg_iterate(vd_ctx* ctx, MGWork& W, double alpha, double beta, double bconst, double reltol,
+                      double abstol, int maxiter, int nu1, int nu2, vd_solve_info* info) {
+    int64_t nc = ncell(W.n);
+    if (W.prec == 0) {
+        coarsen_hierarchy<double>(ctx, W, W.L64);
+    } else {
+        Lev<float>& L0 = W.L32[0];
+        for (int d = 0; d < W.dim; d++) {
+            if (!W.constb) {
+                int64_t nf = nfacef(W.n, d);
+                hipLaunchKernelGGL((k_convert<double, float>), grid1d(nf, 256), dim3(256), 0, ctx->stream,
+                                   W.bf0[d], L0.bf[d], nf);
+            }
+            if (!W.per[d])
+                for (int s = 0; s < 2; s++) {
+                    int64_t nt = ntanf(W.n, d);
+                    hipLaunchKernelGGL((k_convert<double, float>), grid1d(nt, 256), dim3(256), 0,
+                                       ctx->stream, W.c00[d][s], L0.c0[d][s], nt);
+                }
+        }
+        coarsen_hierarchy<float>(ctx, W, W.L32);
+    }
+    VD_CHECK_LAUNCH();
+    Lev<double> fine;
+    for (int d = 0; d < 3; d++) {
+        fine.n[d] = W.n[d];
+        fine.dx[d] = W.dx[d];
+        fine.bf[d] = W.bf0[d];
+        for (int s = 0; s < 2; s++) fine.c0[d][s] = W.c00[d][s];
+    }
+    LevV<double> v0 = view<double>(W, fine, alpha, beta, bconst);
+    double r0, rn;
+    VD_TRY(resid0(ctx, W, v0, &r0));
+    double target = std::max(abstol, reltol * r0);
+    rn = r0;
+    int it = 0, st = VD_OK;
+    if (!(r0 <= target)) {
+        for (it = 1; it <= maxiter; it++) {
+            if (W.prec == 0) {
+                vcycle<double>(ctx, W, W.L64, alpha, beta, bconst, nu1, nu2);
+                hipLaunchKernelGGL(k_axpy_to_double<double>, grid1d(nc, 256), dim3(256), 0, ctx->stream,
+                                   W.x, W.L64[0].e, nc);
+            } else {
+                hipLaunchKernelGGL((k_convert<double, float>), grid1d(nc, 256), dim3(256), 0, ctx->stream,
+                                   W.res, W.L32[0].r, nc);
+                vcycle<float>(ctx, W, W.L32, alpha, beta, bconst, nu1, nu2);
+                hipLaunchKernelGGL(k_axpy_to_double<float>, grid1d(nc, 256), dim3(256), 0, ctx->stream,
+                                   W.x, W.L32[0].e, nc);
+            }
+            VD_CHECK_LAUNCH();
+            VD_TRY(resid0(ctx, W, v0, &rn));
+            if (rn <= target) break;
+        }
+        if (it > maxiter) {
+            it = maxiter;
+            st = VD_ENOCONV;
+            set_error("multigrid: no convergence in %d iterations (|r|=%g target=%g)", maxiter, rn,
+                      target);
+        }
+    }
+    if (info) {
+        info->iterations = it;
+        info->resnorm0 = r0;
+        info->resnorm = rn;
+        info->target = target;
+    }
+    return st;
+}
+
+// ------------------------------------------------ array4 <-> flat helpers
+struct FlatArgs {
+    Arr<double> a;
+    int comp;
+    int3 lo;
+    int nx, ny, nz;
+};
+
+__global__ void k_a4_to_flat(FlatArgs f, double* __restrict__ out) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)f.nx * f.ny * f.nz) return;
+    int i = (int)(t % f.nx);
+    int64_t r = t / f.nx;
+    int j = (int)(r % f.ny), k = (int)(r / f.ny);
+    out[t] = f.a(i + f.lo.x, j + f.lo.y, k + f.lo.z, f.comp);
+}
+
+__global__ void k_flat_to_a4(FlatArgs f, const double* __restrict__ in) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)f.nx * f.ny * f.nz) return;
+    int i = (int)(t % f.nx);
+    int64_t r = t / f.nx;
+    int j = (int)(r % f.ny), k = (int)(r / f.ny);
+    f.a(i + f.lo.x, j + f.lo.y, k + f.lo.z, f.comp) = in[t];
+}
+
+static FlatArgs flat_args(const vd_array4* a, int comp, const int lo[3], const int n[3]) {
+    FlatArgs f;
+    f.a = arr(a);
+    f.comp = comp;
+    f.lo = make_int3(lo[0], lo[1], lo[2]);
+    f.nx = n[0];
+    f.ny = n[1];
+    f.nz = n[2];
+    return f;
+}
+
+// ghost-cell values of a 1-ghost array on one boundary side -> bcv (tangential)
+__global__ void k_side_from_ghost(Arr<double> a, int d, int side, int3 lo, int nx, int ny, int nz, int dim,
+                                  double* __restrict__ out) {
+    int n[3] = {nx, ny, nz};
+    int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+    int nt1 = n[t1], nt2 = dim == 3 ? n[t2] : 1;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nt1 * nt2) return;
+    int c[3] = {0, 0, 0};
+    c[t1] = (int)(t % nt1);
+    if (dim == 3) c[t2] = (int)(t / nt1);
+    int g[3] = {c[0] + lo.x, c[1] + lo.y, c[2] + lo.z};
+    g[d] = side ? (d == 0 ? lo.x : (d == 1 ? lo.y : lo.z)) + n[d] : (d == 0 ? lo.x : (d == 1 ? lo.y : lo.z)) - 1;
+    int ti = d == 0 ? c[1] + ny * c[2] : (d == 1 ? c[0] + nx * c[2] : c[0] + nx * c[1]);
+    out[ti] = a(g[0], g[1], g[2]);
+}
+
+// face gradient (MLMG::getGradSolution) with the solver's BC conventions,
+// scaled by `sign` (-1 turns grad phi into E)
+struct GradArgs {
+    const double* x;
+    Arr<double> out;
+    int d, dim, nx, ny, nz;
+    int3 lo;
+    int per, tlo, thi;
+    double dx, sign;
+    const double* bcv[2];
+    const double* ra[2];
+    const double* rb[2];
+    const double* rf[2];
+};
+
+__global__ void k_grad_faces(GradArgs a) {
+    int n0 = a.nx + (a.d == 0), n1 = a.ny + (a.d == 1), n2 = a.nz + (a.d == 2);
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n0 * n1 * n2) return;
+    int i = (int)(t % n0);
+    int64_t r = t / n0;
+    int j = (int)(r % n1), k = (int)(r / n1);
+    int f[3] = {i, j, k};
+    int n[3] = {a.nx, a.ny, a.nz};
+    const int st = a.d == 0 ? 1 : (a.d == 1 ? a.nx : a.nx * a.ny);
+    int fd = f[a.d];
+    double gv;
+    if (fd > 0 && fd < n[a.d]) {
+        int c = i + a.nx * (j + a.ny * k);
+        gv = (a.x[c] - a.x[c - st]) / a.dx;
+    } else if (a.per) {
+        // faces 0 and n both separate cell n-1 (left) from cell 0 (right)
+        int cr[3] = {i, j, k};
+        cr[a.d] = 0;
+        int c0i = cr[0] + a.nx * (cr[1] + a.ny * cr[2]);
+        gv = (a.x[c0i] - a.x[c0i + (n[a.d] - 1) * st]) / a.dx;
+    } else {
+        int side = fd == 0 ? 0 : 1;
+        int type = side ? a.thi : a.tlo;
+        int ci[3] = {i, j, k};
+        if (side) ci[a.d] -= 1;
+        int c = ci[0] + a.nx * (ci[1] + a.ny * ci[2]);
+        int ti = a.d == 0 ? ci[1] + a.ny * ci[2] : (a.d == 1 ? ci[0] + a.nx * ci[2] : ci[0] + a.nx * ci[1]);
+        double val = a.bcv[side] ? a.bcv[side][ti] : 0.0;
+        double xin = a.x[c];
+        if (type == VD_DIRCBC) {
+            gv = side ? 2.0 * (val - xin) / a.dx : 2.0 * (xin - val) / a.dx;
+        } else if (type == VD_HNEUBC) {
+            gv = 0.0;
+        } else if (type == VD_IHNEUBC) {
+            gv = val;
+        } else {
+            double ra = 0.0, rb = 1.0, rf = 0.0, c0, cinh;
+            if (a.ra[side]) {
+                ra = a.ra[side][ti];
+                rb = a.rb[side][ti];
+                rf = a.rf[side][ti];
+            }
+            bc_coef(VD_ROBINBC, side, a.dx, val, ra, rb, rf, c0, cinh);
+            double xg = c0 * xin + cinh;
+            gv = side ? (xg - xin) / a.dx : (xin - xg) / a.dx;
+        }
+    }
+    if (a.sign != 1.0) gv *= a.sign;
+    a.out(i + a.lo.x, j + a.lo.y, k + a.lo.z) = gv;
+}
+
+static int grad_faces(vd_ctx* ctx, MGWork& W, const int tlo[3], const int thi[3], vd_array4 out[3],
+                      const vd_box* box, double sign) {
+    for (int d = 0; d < W.dim; d++) {
+        GradArgs a;
+        a.x = W.x;
+        a.out = arr(&out[d]);
+        a.d = d;
+        a.dim = W.dim;
+        a.nx = W.n[0];
+        a.ny = W.n[1];
+        a.nz = W.n[2];
+        a.lo = make_int3(box->lo[0], box->lo[1], box->lo[2]);
+        a.per = W.per[d];
+        a.tlo = tlo[d];
+        a.thi = thi[d];
+        a.dx = W.dx[d];
+        a.sign = sign;
+        for (int s = 0; s < 2; s++) {
+            a.bcv[s] = W.bcv[d][s];
+            bool rob = (s ? thi[d] : tlo[d]) == VD_ROBINBC;
+            a.ra[s] = rob ? W.rob[d][s][0] : nullptr;
+            a.rb[s] = rob ? W.rob[d][s][1] : nullptr;
+            a.rf[s] = rob ? W.rob[d][s][2] : nullptr;
+        }
+        int64_t nf = nfacef(W.n, d);
+        hipLaunchKernelGGL(k_grad_faces, grid1d(nf, 256), dim3(256), 0, ctx->stream, a);
+    }
+    VD_CHECK_LAUNCH();
+    return VD_OK;
+}
+
+// c0 and rhs shift for every non-periodic side (sequential sides: no races
+// on corner cells).  bcv / rob must be filled.
+static int side_setup(vd_ctx* ctx, MGWork& W, const int tlo[3], const int thi[3], double beta,
+                      double bconst) {
+    for (int d = 0; d < W.dim; d++) {
+        if (W.per[d]) continue;
+        for (int s = 0; s < 2; s++) {
+            SideArgs a;
+            a.d = d;
+            a.side = s;
+            a.type = s ? thi[d] : tlo[d];
+            a.dim = W.dim;
+            a.nx = W.n[0];
+            a.ny = W.n[1];
+            a.nz = W.n[2];
+            a.dx = W.dx[d];
+            a.s = beta / (W.dx[d] * W.dx[d]);
+            a.bcv = W.bcv[d][s];
+            bool rob = a.type == VD_ROBINBC;
+            a.ra = rob ? W.rob[d][s][0] : nullptr;
+            a.rb = rob ? W.rob[d][s][1] : nullptr;
+            a.rf = rob ? W.rob[d][s][2] : nullptr;
+            a.bf = W.constb ? nullptr : W.bf0[d];
+            a.bconst = bconst;
+            a.c0 = W.c00[d][s];
+            a.f = W.f;
+            int64_t nt = ntanf(W.n, d);
+            hipLaunchKernelGGL(k_side_setup, grid1d(nt, 256), dim3(256), 0, ctx->stream, a);
+        }
+    }
+    VD_CHECK_LAUNCH();
+    return VD_OK;
+}
+
+static int check_bc(const vd_geom* g, const int lo[3], const int hi[3]) {
+    for (int d = 0; d < g->dim; d++) {
+        if (g->is_periodic[d]) continue;
+        for (int v : {lo[d], hi[d]})
+            if (v < VD_DIRCBC || v > VD_ROBINBC) {
+                set_error("bc code %d invalid for non-periodic direction %d", v, d);
+                return VD_EINVAL;
+            }
+    }
+    return VD_OK;
+}
+
+// ------------------------------------------------ potential solve kernels
+struct PotRhsArgs {
+    const DevMech* __restrict__ M;
+    Arr<double> sb;
+    int3 lo;
+    int nx, ny, nz, spacechrg;
+};
+
+// rhs = -e/eps0 * sum z n (PotentialSolve.cpp:203-223)
+__global__ void k_pot_rhs(PotRhsArgs a, double* __restrict__ f) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)a.nx * a.ny * a.nz) return;
+    double r = 0.0;
+    if (a.spacechrg) {
+        int i = (int)(t % a.nx);
+        int64_t q = t / a.nx;
+        int j = (int)(q % a.ny), k = (int)(q / a.ny);
+        int o = a.sb.off(i + a.lo.x, j + a.lo.y, k + a.lo.z);
+        const DevMech& M = *a.M;
+        for (int sp = 0; sp < M.nspec; sp++)
+            if (abs(M.charge[sp]) > 0) r += M.charge[sp] * a.sb.p[(int64_t)sp * a.sb.ns + o];
+        r *= (-VD_ECHARGE / VD_EPS0);
+    }
+    f[t] = r;
+}
+
+struct PotPostArgs {
+    const double* x;
+    Arr<double> ef0, ef1, ef2, phi;
+    int ns, dim;
+    int3 lo;
+    int nx, ny, nz;
+};
+
+// POT copy, E cell averages (average_face_to_cellcenter) and E/N in Td
+// (PotentialSolve.cpp:340-379)
+__global__ void k_pot_post(PotPostArgs a) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)a.nx * a.ny * a.nz) return;
+    int i = (int)(t % a.nx);
+    int64_t q = t / a.nx;
+    int j = (int)(q % a.ny), k = (int)(q / a.ny);
+    i += a.lo.x;
+    j += a.lo.y;
+    k += a.lo.z;
+    const int ns = a.ns;
+    a.phi(i, j, k, VD_POT(ns)) = a.x[t];
+    double e[3] = {0.0, 0.0, 0.0};
+    e[0] = 0.5 * (a.ef0(i, j, k) + a.ef0(i + 1, j, k));
+    if (a.dim > 1) e[1] = 0.5 * (a.ef1(i, j, k) + a.ef1(i, j + 1, k));
+    if (a.dim > 2) e[2] = 0.5 * (a.ef2(i, j, k) + a.ef2(i, j, k + 1));
+    for (int d = 0; d < a.dim; d++) a.phi(i, j, k, VD_EFX(ns) + d) = e[d];
+    double nd = 0.0;
+    for (int sp = 0; sp < ns; sp++) nd += a.phi(i, j, k, sp);
+    double es = 0.0;
+    for (int d = 0; d < a.dim; d++) es += e[d] * e[d];
+    a.phi(i, j, k, VD_REF(ns)) = (sqrt(es) / nd) / 1.0e-21;
+}
+
+// applied voltage (Source/HelperFuncs.H:75-92)
+static double applied_potential(double t, int domend, const vd_params* p) {
+    double amp = domend == -1 ? p->voltage_amp_1 : p->voltage_amp_2;
+    if (p->voltage_profile == 1) return sin(2.0 * VD_PI * p->voltage_freq * t) * amp;
+    if (p->voltage_profile == 2) {
+        double vc = p->voltage_center, vd = p->voltage_dur;
+        if (t <= vc) return (vc - t < vd / 2.0) ? (1.0 - (vc - t) / (vd / 2.0)) * amp : 0.0;
+        return (t - vc < vd / 2.0) ? (1.0 - (t - vc) / (vd / 2.0)) * amp : 0.0;
+    }
+    return amp;
+}
+
+static void dims_of(const vd_box* box, const vd_geom* g, int n[3], int lo[3]) {
+    for (int d = 0; d < 3; d++) {
+        n[d] = d < g->dim ? box->hi[d] - box->lo[d] + 1 : 1;
+        lo[d] = box->lo[d];
+    }
+}
+
+int solve_potential(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4* phi,
+                    vd_array4 ef[3], const vd_box* box, const vd_geom* g, const vd_params* prm,
+                    double time, vd_solve_info* info) {
+    VD_TRY(check_bc(g, prm->pot_bc_lo, prm->pot_bc_hi));
+    MGWork* W;
+    VD_TRY(get_work(ctx, g, box, prm->mg_precision, 1, prm->linsolve_max_coarsening_level, &W));
+    int n[3], lo[3];
+    dims_of(box, g, n, lo);
+    int64_t nc = ncell(n);
+    PotRhsArgs ra;
+    ra.M = m->dev;
+    ra.sb = arr(sb);
+    ra.lo = make_int3(lo[0], lo[1], lo[2]);
+    ra.nx = n[0];
+    ra.ny = n[1];
+    ra.nz = n[2];
+    ra.spacechrg = prm->do_spacechrg;
+    hipLaunchKernelGGL(k_pot_rhs, grid1d(nc, 256), dim3(256), 0, ctx->stream, ra, W->f);
+    // potential_bc (PlasmaChem.H:18-72): Dirichlet / Robin(1,0,V) value V
+    for (int d = 0; d < g->dim; d++) {
+        if (W->per[d]) continue;
+        for (int s = 0; s < 2; s++) {
+            double V = applied_potential(time, s ? 1 : -1, prm);
+            int64_t nt = ntanf(n, d);
+            VD_TRY(fill_value(ctx, W->bcv[d][s], nt, V));
+            VD_TRY(fill_value(ctx, W->rob[d][s][0], nt, 1.0));
+            VD_TRY(fill_value(ctx, W->rob[d][s][1], nt, 0.0));
+            VD_TRY(fill_value(ctx, W->rob[d][s][2], nt, V));
+        }
+    }
+    const double bconst = -1.0, alpha = 0.0, beta = 1.0;
+    VD_TRY(side_setup(ctx, *W, prm->pot_bc_lo, prm->pot_bc_hi, beta, bconst));
+    VD_TRY(fill_value(ctx, W->x, nc, 0.0));
+    int st = mg_iterate(ctx, *W, alpha, beta, bconst, prm->linsolve_reltol, prm->linsolve_abstol,
+                        prm->linsolve_maxiter, prm->mg_pre_smooth, prm->mg_post_smooth, info);
+    if (st != VD_OK && st != VD_ENOCONV) return st;
+    VD_TRY(grad_faces(ctx, *W, prm->pot_bc_lo, prm->pot_bc_hi, ef, box, -1.0));
+    PotPostArgs pa;
+    pa.x = W->x;
+    pa.ef0 = arr(&ef[0]);
+    pa.ef1 = g->dim > 1 ? arr(&ef[1]) : pa.ef0;
+    pa.ef2 = g->dim > 2 ? arr(&ef[2]) : pa.ef0;
+    pa.phi = arr(phi);
+    pa.ns = m->hdev.nspec;
+    pa.dim = g->dim;
+    pa.lo = make_int3(lo[0], lo[1], lo[2]);
+    pa.nx = n[0];
+    pa.ny = n[1];
+    pa.nz = n[2];
+    hipLaunchKernelGGL(k_pot_post, grid1d(nc, 256), dim3(256), 0, ctx->stream, pa);
+    VD_CHECK_LAUNCH();
+    return st;
+}
+
+// ------------------------------------------------ implicit species solve
+struct HarmArgs {
+    const DevMech* __restrict__ M;
+    Arr<double> sb;
+    int spec, d, ns;
+    int3 lo;
+    int nx, ny, nz;
+    double Tg;
+};
+
+// harmonic face D from cell specDiff (ScalarSolve.cpp:483-512)
+__global__ void k_harmonic_faces(HarmArgs a, double* __restrict__ bf) {
+    int n0 = a.nx + (a.d == 0), n1 = a.ny + (a.d == 1), n2 = a.nz + (a.d == 2);
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n0 * n1 * n2) return;
+    int i = (int)(t % n0);
+    int64_t r = t / n0;
+    int j = (int)(r % n1), k = (int)(r / n1);
+    i += a.lo.x;
+    j += a.lo.y;
+    k += a.lo.z;
+    int l[3] = {i, j, k};
+    l[a.d] -= 1;
+    const DevMech& M = *a.M;
+    const double* te = a.sb.comp(VD_ETEMP(a.ns));
+    double A = spec_diff(M, a.spec, te[a.sb.off(l[0], l[1], l[2])], a.Tg);
+    double B = spec_diff(M, a.spec, te[a.sb.off(i, j, k)], a.Tg);
+    bf[t] = (A + B == 0.0) ? 0.0 : 2.0 * A * B / (A + B);
+}
+
+struct SpecBcArgs {
+    const DevMech* __restrict__ M;
+    Arr<double> sb;
+    int spec, d, side, ns, dim;
+    int3 lo;
+    int nx, ny, nz;
+    double Tg;
+    double* out;
+};
+
+// plasmachem_transport::species_bc (Source/PlasmaChem.H:74-184)
+__global__ void k_species_bc(SpecBcArgs a) {
+    int n[3] = {a.nx, a.ny, a.nz};
+    int t1 = a.d == 0 ? 1 : 0, t2 = a.d == 2 ? 1 : 2;
+    int nt1 = n[t1], nt2 = a.dim == 3 ? n[t2] : 1;
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)nt1 * nt2) return;
+    int c[3] = {0, 0, 0};
+    c[t1] = (int)(t % nt1);
+    if (a.dim == 3) c[t2] = (int)(t / nt1);
+    int ti = a.d == 0 ? c[1] + a.ny * c[2] : (a.d == 1 ? c[0] + a.nx * c[2] : c[0] + a.nx * c[1]);
+    int lo[3] = {a.lo.x, a.lo.y, a.lo.z};
+    int face[3] = {c[0] + lo[0], c[1] + lo[1], c[2] + lo[2]};
+    face[a.d] = a.side ? lo[a.d] + n[a.d] : lo[a.d];
+    int cell[3] = {face[0], face[1], face[2]};
+    if (a.side) cell[a.d] -= 1;
+    const DevMech& M = *a.M;
+    const int ns = a.ns, E = M.e_idx;
+    const int oc = a.sb.off(cell[0], cell[1], cell[2]);
+    const double* S = a.sb.p;
+    const int64_t N = a.sb.ns;
+    double sgn = a.side ? 1.0 : -1.0;
+    double Te = S[VD_ETEMP(ns) * N + oc];
+    double v;
+    if (a.spec == E) {
+        double ne = S[(int64_t)E * N + oc];
+        double D = spec_diff(M, a.spec, Te, a.Tg);
+        double cbar = sqrt(8.0 * VD_K_B * Te / VD_PI / VD_ME);
+        double flx = ne * cbar / 4.0 * sgn;
+        v = -flx / D;
+    } else if (a.spec == ns) {
+        double ne = S[(int64_t)E * N + oc];
+        double D = spec_diff(M, a.spec, Te, a.Tg);
+        double cbar = sqrt(8.0 * VD_K_B * Te / VD_PI / VD_ME);
+        double flx = (2.0 * VD_K_B * Te) * ne * cbar / 4.0 * sgn;
+        v = -flx / D;
+    } else if (M.charge[a.spec] != 0) {
+        double mu = spec_mob(M, a.spec);
+        double D = spec_diff(M, a.spec, Te, a.Tg);
+        double efield = S[(int64_t)(VD_EFX(ns) + a.d) * N + oc];
+        double idf = mu * S[(int64_t)a.spec * N + oc] * efield;
+        double dotn = idf * sgn;
+        v = (dotn >= 0.0) ? (-idf / D) : 0.0;
+    } else {
+        double D = spec_diff(M, a.spec, Te, a.Tg);
+        double cbar = sqrt(8.0 * VD_K_B * a.Tg / VD_PI / (M.molwt[a.spec] * VD_M_AMU));
+        double flx = S[(int64_t)a.spec * N + a.sb.off(face[0], face[1], face[2])] * cbar / 4.0 * sgn;
+        v = -flx / D;
+    }
+    a.out[ti] = v;
+}
+
+struct SpecRhsArgs {
+    Arr<double> old, ds, sb;
+    int spec;
+    int3 lo;
+    int nx, ny, nz;
+    double idt;
+};
+
+// rhs = (1/dt) n_old + dsdt; initial guess = Sborder (ScalarSolve.cpp:443-465)
+__global__ void k_spec_rhs(SpecRhsArgs a, double* __restrict__ f, double* __restrict__ x) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)a.nx * a.ny * a.nz) return;
+    int i = (int)(t % a.nx);
+    int64_t q = t / a.nx;
+    int j = (int)(q % a.ny), k = (int)(q / a.ny);
+    i += a.lo.x;
+    j += a.lo.y;
+    k += a.lo.z;
+    f[t] = a.idt * a.old(i, j, k, a.spec) + 1.0 * a.ds(i, j, k);
+    x[t] = a.sb(i, j, k, a.spec);
+}
+
+struct SpecPostArgs {
+    double* x;
+    Arr<double> phi, sb;
+    int spec, ns, E, floor_on, energy;
+    double floor_val, min_te;
+    int3 lo;
+    int nx, ny, nz;
+};
+
+__global__ void k_spec_floor(SpecPostArgs a) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)a.nx * a.ny * a.nz) return;
+    if (a.floor_on && a.x[t] < a.floor_val) a.x[t] = a.floor_val;
+}
+
+// copy to phi_new and the Te update (ScalarSolve.cpp:632-667)
+__global__ void k_spec_post(SpecPostArgs a) {
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)a.nx * a.ny * a.nz) return;
+    int i = (int)(t % a.nx);
+    int64_t q = t / a.nx;
+    int j = (int)(q % a.ny), k = (int)(q / a.ny);
+    i += a.lo.x;
+    j += a.lo.y;
+    k += a.lo.z;
+    a.phi(i, j, k, a.spec) = a.x[t];
+    if (a.energy) {
+        const int ns = a.ns;
+        double te = VD_TWOTHIRD / VD_K_B * a.phi(i, j, k, VD_EEN(ns)) / a.sb(i, j, k, a.E);
+        a.phi(i, j, k, VD_ETEMP(ns)) = te;
+        if (te < a.min_te) {
+            a.phi(i, j, k, VD_ETEMP(ns)) = a.min_te;
+            a.phi(i, j, k, VD_EEN(ns)) = 1.5 * VD_K_B * a.phi(i, j, k, a.E) * a.min_te;
+        }
+    }
+}
+
+int implicit_solve(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sb, const vd_array4* old,
+                   const vd_array4* ds, vd_array4* phi, vd_array4 grad[3], const vd_box* box,
+                   const vd_geom* g, const int bclo[3], const int bchi[3], const vd_params* prm,
+                   double dt, vd_solve_info* info) {
+    const int ns = m->hdev.nspec, E = m->hdev.e_idx;
+    if (spec < 0 || spec > ns) {
+        set_error("implicit solve: species %d out of range", spec);
+        return VD_EINVAL;
+    }
+    VD_TRY(check_bc(g, bclo, bchi));
+    MGWork* W;
+    VD_TRY(get_work(ctx, g, box, prm->mg_precision, 0, prm->linsolve_max_coarsening_level, &W));
+    int n[3], lo[3];
+    dims_of(box, g, n, lo);
+    int64_t nc = ncell(n);
+    const double Tg = prm->gas_temperature;
+    for (int d = 0; d < g->dim; d++) {
+        HarmArgs h;
+        h.M = m->dev;
+        h.sb = arr(sb);
+        h.spec = spec;
+        h.d = d;
+        h.ns = ns;
+        h.lo = make_int3(lo[0], lo[1], lo[2]);
+        h.nx = n[0];
+        h.ny = n[1];
+        h.nz = n[2];
+        h.Tg = Tg;
+        hipLaunchKernelGGL(k_harmonic_faces, grid1d(nfacef(n, d), 256), dim3(256), 0, ctx->stream, h,
+                           W->bf0[d]);
+        if (W->per[d]) continue;
+        for (int s = 0; s < 2; s++) {
+            SpecBcArgs b;
+            b.M = m->dev;
+            b.sb = arr(sb);
+            b.spec = spec;
+            b.d = d;
+            b.side = s;
+            b.ns = ns;
+            b.dim = g->dim;
+            b.lo = h.lo;
+            b.nx = n[0];
+            b.ny = n[1];
+            b.nz = n[2];
+            b.Tg = Tg;
+            b.out = W->bcv[d][s];
+            hipLaunchKernelGGL(k_species_bc, grid1d(ntanf(n, d), 256), dim3(256), 0, ctx->stream, b);
+        }
+    }
+    // Robin species sides keep the reference default (a, b, f) = (0, 1, 0)
+    for (int d = 0; d < g->dim; d++) {
+        if (W->per[d]) continue;
+        for (int s = 0; s < 2; s++)
+            if ((s ? bchi[d] : bclo[d]) == VD_ROBINBC) {
+                int64_t nt = ntanf(n, d);
+                VD_TRY(fill_value(ctx, W->rob[d][s][0], nt, 0.0));
+                VD_TRY(fill_value(ctx, W->rob[d][s][1], nt, 1.0));
+                VD_TRY(fill_value(ctx, W->rob[d][s][2], nt, 0.0));
+            }
+    }
+    SpecRhsArgs r;
+    r.old = arr(old);
+    r.ds = arr(ds);
+    r.sb = arr(sb);
+    r.spec = spec;
+    r.lo = make_int3(lo[0], lo[1], lo[2]);
+    r.nx = n[0];
+    r.ny = n[1];
+    r.nz = n[2];
+    r.idt = 1.0 / dt;
+    hipLaunchKernelGGL(k_spec_rhs, grid1d(nc, 256), dim3(256), 0, ctx->stream, r, W->f, W->x);
+    VD_CHECK_LAUNCH();
+    const double alpha = 1.0 / dt, beta = 1.0;
+    VD_TRY(side_setup(ctx, *W, bclo, bchi, beta, 0.0));
+    int st = mg_iterate(ctx, *W, alpha, beta, 0.0, prm->linsolve_reltol, prm->linsolve_abstol,
+                        prm->linsolve_maxiter, prm->mg_pre_smooth, prm->mg_post_smooth, info);
+    if (st != VD_OK && st != VD_ENOCONV) return st;
+    SpecPostArgs p;
+    p.x = W->x;
+    p.phi = arr(phi);
+    p.sb = arr(sb);
+    p.spec = spec;
+    p.ns = ns;
+    p.E = E;
+    p.floor_on = prm->bound_specden && spec != ns;
+    p.floor_val = spec == E ? prm->min_electron_density : prm->min_species_density;
+    p.energy = spec == ns;
+    p.min_te = prm->min_electron_temp;
+    p.lo = make_int3(lo[0], lo[1], lo[2]);
+    p.nx = n[0];
+    p.ny = n[1];
+    p.nz = n[2];
+    hipLaunchKernelGGL(k_spec_floor, grid1d(nc, 256), dim3(256), 0, ctx->stream, p);
+    if (spec == E && grad) VD_TRY(grad_faces(ctx, *W, bclo, bchi, grad, box, 1.0));
+    hipLaunchKernelGGL(k_spec_post, grid1d(nc, 256), dim3(256), 0, ctx->stream, p);
+    VD_CHECK_LAUNCH();
+    return st;
+}
+
+// ------------------------------------------------ generic solve
+int mlmg_solve(vd_ctx* ctx, const vd_mlmg_problem* P, vd_array4* phi, const vd_array4* rhs,
+               const vd_box* box, const vd_geom* g, vd_solve_info* info) {
+    VD_TRY(check_bc(g, P->bc_lo, P->bc_hi));
+    int constb = (P->bface[0] == nullptr);
+    MGWork* W;
+    VD_TRY(get_work(ctx, g, box, P->precision, constb, P->max_coarsening_level, &W));
+    int n[3], lo[3];
+    dims_of(box, g, n, lo);
+    int64_t nc = ncell(n);
+    FlatArgs fa = flat_args(rhs, 0, lo, n);
+    hipLaunchKernelGGL(k_a4_to_flat, grid1d(nc, 256), dim3(256), 0, ctx->stream, fa, W->f);
+    fa = flat_args(phi, 0, lo, n);
+    hipLaunchKernelGGL(k_a4_to_flat, grid1d(nc, 256), dim3(256), 0, ctx->stream, fa, W->x);
+    if (!constb)
+        for (int d = 0; d < g->dim; d++) {
+            int fn[3] = {n[0] + (d == 0), n[1] + (d == 1), n[2] + (d == 2)};
+            FlatArgs fb = flat_args(P->bface[d], 0, lo, fn);
+            hipLaunchKernelGGL(k_a4_to_flat, grid1d(ncell(fn), 256), dim3(256), 0, ctx->stream, fb,
+                               W->bf0[d]);
+        }
+    for (int d = 0; d < g->dim; d++) {
+        if (W->per[d]) continue;
+        for (int s = 0; s < 2; s++) {
+            int64_t nt = ntanf(n, d);
+            dim3 gg = grid1d(nt, 256);
+            int3 l3 = make_int3(lo[0], lo[1], lo[2]);
+            if (P->bcval)
+                hipLaunchKernelGGL(k_side_from_ghost, gg, dim3(256), 0, ctx->stream, arr(P->bcval), d, s,
+                                   l3, n[0], n[1], n[2], g->dim, W->bcv[d][s]);
+            else
+                VD_TRY(fill_value(ctx, W->bcv[d][s], nt, 0.0));
+            int type = s ? P->bc_hi[d] : P->bc_lo[d];
+            if (type == VD_ROBINBC) {
+                const vd_array4* rr[3] = {P->robin_a, P->robin_b, P->robin_f};
+                const double dflt[3] = {0.0, 1.0, 0.0};
+                for (int q = 0; q < 3; q++) {
+                    if (rr[q])
+                        hipLaunchKernelGGL(k_side_from_ghost, gg, dim3(256), 0, ctx->stream, arr(rr[q]), d,
+                                           s, l3, n[0], n[1], n[2], g->dim, W->rob[d][s][q]);
+                    else
+                        VD_TRY(fill_value(ctx, W->rob[d][s][q], nt, dflt[q]));
+                }
+            }
+        }
+    }
+    VD_CHECK_LAUNCH();
+    VD_TRY(side_setup(ctx, *W, P->bc_lo, P->bc_hi, P->beta, 1.0));
+    int st = mg_iterate(ctx, *W, P->alpha_a, P->beta, 1.0, P->reltol, P->abstol, P->maxiter,
+                        P->pre_smooth, P->post_smooth, info);
+    if (st != VD_OK && st != VD_ENOCONV) return st;
+    fa = flat_args(phi, 0, lo, n);
+    hipLaunchKernelGGL(k_flat_to_a4, grid1d(nc, 256), dim3(256), 0, ctx->stream, fa, W->x);
+    VD_CHECK_LAUNCH();
+    return st;
+}
+
+}  // namespace vd
+
+using namespace vd;
+
+extern "C" int vd_solve_potential(vd_ctx* ctx, const vd_mech* m, const vd_array4* sborder,
+                                  vd_array4* phi_new, vd_array4 efield[3], const vd_box* box,
+                                  const vd_geom* geom, const vd_params* prm, double time,
+                                  vd_solve_info* info) {
+    if (!ctx || !m || !sborder || !phi_new || !efield || !box || !geom || !prm) return VD_EINVAL;
+    if (prm->mg_precision != 0 && prm->mg_precision != 1) return VD_EINVAL;
+    return solve_potential(ctx, m, sborder, phi_new, efield, box, geom, prm, time, info);
+}
+
+extern "C" int vd_implicit_solve_scalar(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sborder,
+                                        const vd_array4* phi_old, const vd_array4* dsdt_expl,
+                                        vd_array4* phi_new, vd_array4 grad_fc[3], const vd_box* box,
+                                        const vd_geom* geom, const int bc_lo[3], const int bc_hi[3],
+                                        const vd_params* prm, double time, double dt,
+                                        vd_solve_info* info) {
+    (void)time;
+    if (!ctx || !m || !sborder || !phi_old || !dsdt_expl || !phi_new || !box || !geom || !prm)
+        return VD_EINVAL;
+    if (prm->mg_precision != 0 && prm->mg_precision != 1) return VD_EINVAL;
+    if (!(dt > 0.0)) {
+        set_error("implicit solve: dt must be positive");
+        return VD_EINVAL;
+    }
+    return implicit_solve(ctx, m, spec, sborder, phi_old, dsdt_expl, phi_new, grad_fc, box, geom, bc_lo,
+                          bc_hi, prm, dt, info);
+}
+
+extern "C" int vd_mlmg_solve(vd_ctx* ctx, const vd_mlmg_problem* prob, vd_array4* phi,
+                             const vd_array4* rhs, const vd_box* box, const vd_geom* geom,
+                             vd_solve_info* info) {
+    if (!ctx || !prob || !phi || !rhs || !box || !geom) return VD_EINVAL;
+    if (prob->precision != 0 && prob->precision != 1) return VD_EINVAL;
+    return mlmg_solve(ctx, prob, phi, rhs, box, geom, info);
+}

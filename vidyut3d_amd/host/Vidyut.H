@@ -1,0 +1,118 @@
+// Vidyut.H -- host C++ layer over the C ABI.  Keeps the member-function
+// boundary of the reference's Vidyut class (Source/Vidyut.H:72-119) with
+// MultiFab-in / MultiFab-out calls, single level, one box per level, all
+// data resident in device memory.
+#pragma once
+
+#include <array>
+#include <string>
+#include <vector>
+
+#include "../../include/vidyut3d_amd.h"
+
+namespace vidyut_amd {
+
+using Real = double;
+
+// One-box, device-resident MultiFab (component-major Array4 layout).
+// face_dir >= 0 makes it face-centred in that direction (surroundingNodes).
+class MultiFab {
+   public:
+    MultiFab() = default;
+    MultiFab(vd_ctx* ctx, const vd_box& valid, int dim, int ncomp, int ngrow, int face_dir = -1);
+    ~MultiFab();
+    MultiFab(const MultiFab&) = delete;
+    MultiFab& operator=(const MultiFab&) = delete;
+    MultiFab(MultiFab&& o) noexcept;
+    MultiFab& operator=(MultiFab&& o) noexcept;
+
+    void define(vd_ctx* ctx, const vd_box& valid, int dim, int ncomp, int ngrow, int face_dir = -1);
+    bool ok() const { return a_.p != nullptr; }
+    vd_array4& array() { return a_; }
+    const vd_array4& array() const { return a_; }
+    const vd_box& box() const { return valid_; }
+    int nComp() const { return a_.ncomp; }
+    int nGrow() const { return ngrow_; }
+    size_t bytes() const { return (size_t)a_.nstride * a_.ncomp * sizeof(double); }
+    double* data() { return a_.p; }
+    int setVal(double v);
+    // copies comps [scomp, scomp+n) of src to [dcomp, ...) on the valid box
+    int copy_from(const MultiFab& src, int scomp, int dcomp, int n);
+
+   private:
+    void release();
+    vd_ctx* ctx_ = nullptr;
+    vd_array4 a_{};
+    vd_box valid_{};
+    int ngrow_ = 0;
+};
+
+using FaceMF = std::array<MultiFab, 3>;
+
+class Vidyut {
+   public:
+    Vidyut(vd_ctx* ctx, const vd_mech_desc& mech, const vd_geom& geom, const vd_params& prm);
+    ~Vidyut();
+    int status() const { return status_; }
+
+    // ---- the reference's member-function boundary ----------------------
+    int compute_scalar_transport_flux(int lev, MultiFab& Sborder, FaceMF& flux, const int* bc_lo,
+                                      const int* bc_hi, Real time, int specid);
+    int compute_dsdt(int lev, int specid, FaceMF& flux, MultiFab& rxn_src, MultiFab& dsdt, Real time,
+                     Real dt);
+    int update_explsrc_at_all_levels(int specid, std::vector<MultiFab>& Sborder,
+                                     std::vector<FaceMF>& flux, std::vector<MultiFab>& rxn_src,
+                                     std::vector<MultiFab>& expl_src, const int* bc_lo,
+                                     const int* bc_hi, Real cur_time);
+    int update_rxnsrc_at_all_levels(std::vector<MultiFab>& Sborder, std::vector<MultiFab>& rxn_src,
+                                    Real cur_time);
+    int compute_elecenergy_source(int lev, MultiFab& Sborder, MultiFab& rxnsrc, FaceMF& efield,
+                                  FaceMF& gradne, MultiFab& dsdt, Real time, Real dt, int floor_jh);
+    int solve_potential(Real current_time, std::vector<MultiFab>& Sborder, const int* bc_lo,
+                        const int* bc_hi, std::vector<FaceMF>& efield_ec);
+    int implicit_solve_scalar(Real current_time, Real dt, int spec_id, std::vector<MultiFab>& Sborder,
+                              std::vector<MultiFab>& Sborder_old, std::vector<MultiFab>& dsdt_expl,
+                              const int* bc_lo, const int* bc_hi, std::vector<FaceMF>& grad_fc);
+    int find_time_scales(int lev, Real& dt_edrift, Real& dt_ediff, Real& dt_diel_relax);
+    int ErrorEst(int lev, vd_tagarray4& tags, Real time, int ngrow);
+    int FillPatch(int lev, Real time, MultiFab& mf, int icomp, int ncomp);
+
+    // ---- driver ---------------------------------------------------------
+    // Evolve (Source/Evolve.cpp:33-369) for nsteps at fixed dt
+    int Evolve(int nsteps, Real dt, int ncorr);
+    int WritePlotFile(const std::string& dir) const;
+    int set_tagging(const std::vector<int>& comps, const std::vector<double>& thr,
+                    const std::vector<double>& gthr);
+
+    MultiFab& phi_new() { return phi_new_[0]; }
+    MultiFab& phi_old() { return phi_old_[0]; }
+    Real time() const { return t_new_; }
+    void set_time(Real t) { t_new_ = t; }
+    int step() const { return istep_; }
+    const vd_solve_info& last_potential_info() const { return pot_info_; }
+    long long mg_iterations() const { return mg_iters_; }
+    std::vector<std::string> varnames() const;
+
+   private:
+    int alloc_step_data();
+    vd_ctx* ctx_;
+    vd_mech* mech_ = nullptr;
+    vd_mech_desc desc_;
+    vd_geom geom_;
+    vd_params prm_;
+    vd_box dom_;
+    int ns_, nvar_, dim_, ngrow_fp_;
+    int status_ = VD_OK;
+    Real t_new_ = 0.0;
+    int istep_ = 0;
+    std::vector<MultiFab> phi_new_, phi_old_;
+    // persistent step data (the reference allocates these every step)
+    std::vector<MultiFab> Sborder_, Sborder_old_, expl_src_, rxn_src_;
+    std::vector<FaceMF> flux_, efield_ec_, gradne_fc_, grad_fc_;
+    vd_solve_info pot_info_{};
+    long long mg_iters_ = 0;
+    std::vector<int> tag_comps_;
+    std::vector<double> tag_thr_, tag_gthr_;
+};
+
+}  // namespace vidyut_amd

@@ -1,0 +1,478 @@
+// Vidyut.cpp -- host driver over the C ABI: the reference's member-function
+// boundary and step loop (Source/Evolve.cpp, ScalarSolve.cpp,
+// PotentialSolve.cpp, ElecEnergySource.cpp, TimeStep.cpp, Vidyut.cpp,
+// GridWork.cpp, Plot.cpp) on device-resident single-box MultiFabs.
+#include "Vidyut.H"
+
+#include <sys/stat.h>
+
+#include <algorithm>
+#include <cerrno>
+#include <cfloat>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iomanip>
+#include <sstream>
+
+#include "../../include/vidyut3d_amd_host.h"
+
+namespace vidyut_amd {
+
+#define TRY(x)                 \
+    do {                       \
+        int s_ = (x);          \
+        if (s_ != VD_OK) return s_; \
+    } while (0)
+
+// ------------------------------------------------------------- MultiFab
+MultiFab::MultiFab(vd_ctx* ctx, const vd_box& valid, int dim, int ncomp, int ngrow, int face_dir) {
+    define(ctx, valid, dim, ncomp, ngrow, face_dir);
+}
+
+MultiFab::~MultiFab() { release(); }
+
+void MultiFab::release() {
+    if (a_.p && ctx_) vd_free(ctx_, a_.p);
+    a_.p = nullptr;
+}
+
+MultiFab::MultiFab(MultiFab&& o) noexcept {
+    ctx_ = o.ctx_;
+    a_ = o.a_;
+    valid_ = o.valid_;
+    ngrow_ = o.ngrow_;
+    o.a_.p = nullptr;
+}
+
+MultiFab& MultiFab::operator=(MultiFab&& o) noexcept {
+    if (this != &o) {
+        release();
+        ctx_ = o.ctx_;
+        a_ = o.a_;
+        valid_ = o.valid_;
+        ngrow_ = o.ngrow_;
+        o.a_.p = nullptr;
+    }
+    return *this;
+}
+
+void MultiFab::define(vd_ctx* ctx, const vd_box& valid, int dim, int ncomp, int ngrow, int face_dir) {
+    release();
+    ctx_ = ctx;
+    valid_ = valid;
+    ngrow_ = ngrow;
+    if (face_dir >= 0) valid_.hi[face_dir] += 1;
+    for (int d = 0; d < 3; d++) {
+        int g = d < dim ? ngrow : 0;
+        a_.begin[d] = valid_.lo[d] - g;
+        a_.end[d] = valid_.hi[d] + 1 + g;
+    }
+    a_.jstride = a_.end[0] - a_.begin[0];
+    a_.kstride = a_.jstride * (a_.end[1] - a_.begin[1]);
+    a_.nstride = a_.kstride * (a_.end[2] - a_.begin[2]);
+    a_.ncomp = ncomp;
+    void* p = nullptr;
+    if (vd_malloc(ctx, (size_t)a_.nstride * ncomp * sizeof(double), &p) == VD_OK) {
+        a_.p = (double*)p;
+        vd_memset_zero(ctx, p, (size_t)a_.nstride * ncomp * sizeof(double));
+    }
+}
+
+int MultiFab::setVal(double v) {
+    vd_box whole;
+    for (int d = 0; d < 3; d++) {
+        whole.lo[d] = a_.begin[d];
+        whole.hi[d] = a_.end[d] - 1;
+    }
+    return vd_setval(ctx_, &a_, &whole, 0, a_.ncomp, v);
+}
+
+int MultiFab::copy_from(const MultiFab& src, int scomp, int dcomp, int n) {
+    return vd_copy(ctx_, &src.array(), &a_, &valid_, scomp, dcomp, n);
+}
+
+// ---------------------------------------------------------------- Vidyut
+Vidyut::Vidyut(vd_ctx* ctx, const vd_mech_desc& mech, const vd_geom& geom, const vd_params& prm)
+    : ctx_(ctx), desc_(mech), geom_(geom), prm_(prm) {
+    ns_ = mech.nspec;
+    nvar_ = VD_NVAR(ns_);
+    dim_ = geom.dim;
+    for (int d = 0; d < 3; d++) {
+        dom_.lo[d] = d < dim_ ? geom.dom_lo[d] : 0;
+        dom_.hi[d] = d < dim_ ? geom.dom_hi[d] : 0;
+    }
+    // ngrow_for_fillpatch by hyp_order (Source/Vidyut.cpp:318-335)
+    if (prm.hyp_order == 1) ngrow_fp_ = 1;
+    else if (prm.hyp_order == 2) ngrow_fp_ = 2;
+    else if (prm.hyp_order == 5) ngrow_fp_ = 3;
+    else {
+        status_ = VD_EINVAL;
+        return;
+    }
+    status_ = vd_mech_create(ctx, &mech, &mech_);
+    if (status_ != VD_OK) return;
+    phi_new_.emplace_back(ctx, dom_, dim_, nvar_, 0);
+    phi_old_.emplace_back(ctx, dom_, dim_, nvar_, 0);
+    status_ = alloc_step_data();
+}
+
+Vidyut::~Vidyut() {
+    if (mech_) vd_mech_destroy(mech_);
+}
+
+int Vidyut::alloc_step_data() {
+    Sborder_.emplace_back(ctx_, dom_, dim_, nvar_, ngrow_fp_);
+    expl_src_.emplace_back(ctx_, dom_, dim_, 1, 0);
+    rxn_src_.emplace_back(ctx_, dom_, dim_, ns_ + 1, 0);
+    for (auto* v : {&flux_, &efield_ec_, &gradne_fc_, &grad_fc_}) {
+        v->emplace_back();
+        for (int d = 0; d < 3; d++) (*v)[0][d].define(ctx_, dom_, dim_, 1, 0, d < dim_ ? d : 0);
+    }
+    if (!phi_new_[0].ok() || !Sborder_[0].ok()) return VD_ENOMEM;
+    return VD_OK;
+}
+
+std::vector<std::string> Vidyut::varnames() const {
+    static const char* pv[VD_NUM_PLASMAVARS] = {"Electron_energy",   "Electron_Temp", "Potential",
+                                                "Efieldx",           "Efieldy",       "Efieldz",
+                                                "Electron_Jheat",    "Electron_inelasticHeat",
+                                                "Electron_elasticHeat", "ReducedEF"};
+    std::vector<std::string> v;
+    for (int s = 0; s < ns_; s++) v.emplace_back(desc_.names[s]);
+    for (int q = 0; q < VD_NUM_PLASMAVARS; q++) v.emplace_back(pv[q]);
+    return v;
+}
+
+int Vidyut::FillPatch(int lev, Real time, MultiFab& mf, int icomp, int ncomp) {
+    (void)lev;
+    (void)time;
+    return vd_fill_patch(ctx_, &phi_new_[0].array(), &mf.array(), &dom_, &geom_, icomp, ncomp);
+}
+
+int Vidyut::compute_scalar_transport_flux(int lev, MultiFab& Sborder, FaceMF& flux, const int* bc_lo,
+                                          const int* bc_hi, Real time, int specid) {
+    (void)lev;
+    vd_array4 f[3] = {flux[0].array(), flux[1].array(), flux[2].array()};
+    return vd_compute_scalar_transport_flux(ctx_, mech_, &Sborder.array(), f, &dom_, &geom_, bc_lo, bc_hi,
+                                            &prm_, time, 0.0, specid);
+}
+
+int Vidyut::compute_dsdt(int lev, int specid, FaceMF& flux, MultiFab& rxn_src, MultiFab& dsdt, Real time,
+                         Real dt) {
+    (void)lev;
+    (void)time;
+    (void)dt;
+    vd_array4 f[3] = {flux[0].array(), flux[1].array(), flux[2].array()};
+    return vd_compute_dsdt(ctx_, f, &rxn_src.array(), &dsdt.array(), &dom_, &geom_, specid);
+}
+
+int Vidyut::update_explsrc_at_all_levels(int specid, std::vector<MultiFab>& Sborder,
+                                         std::vector<FaceMF>& flux, std::vector<MultiFab>& rxn_src,
+                                         std::vector<MultiFab>& expl_src, const int* bc_lo,
+                                         const int* bc_hi, Real cur_time) {
+    (void)flux;
+    // fluxes stay on chip (vd_transport_rhs == flux + dsdt)
+    return vd_transport_rhs(ctx_, mech_, &Sborder[0].array(), &rxn_src[0].array(), &expl_src[0].array(),
+                            &dom_, &geom_, bc_lo, bc_hi, &prm_, cur_time, 0.0, specid);
+}
+
+int Vidyut::update_rxnsrc_at_all_levels(std::vector<MultiFab>& Sborder, std::vector<MultiFab>& rxn_src,
+                                        Real cur_time) {
+    return vd_update_rxnsrc(ctx_, mech_, &Sborder[0].array(), &rxn_src[0].array(), &dom_, &prm_, cur_time);
+}
+
+int Vidyut::compute_elecenergy_source(int lev, MultiFab& Sborder, MultiFab& rxnsrc, FaceMF& efield,
+                                      FaceMF& gradne, MultiFab& dsdt, Real time, Real dt, int floor_jh) {
+    (void)lev;
+    (void)time;
+    (void)dt;
+    vd_params p = prm_;
+    p.floor_jh = floor_jh;
+    vd_array4 ef[3] = {efield[0].array(), efield[1].array(), efield[2].array()};
+    vd_array4 gn[3] = {gradne[0].array(), gradne[1].array(), gradne[2].array()};
+    return vd_compute_elecenergy_source(ctx_, mech_, &Sborder.array(), &rxnsrc.array(), ef, gn,
+                                        &dsdt.array(), &phi_new_[0].array(), &dom_, &geom_, &p);
+}
+
+int Vidyut::solve_potential(Real current_time, std::vector<MultiFab>& Sborder, const int* bc_lo,
+                            const int* bc_hi, std::vector<FaceMF>& efield_ec) {
+    vd_params p = prm_;
+    for (int d = 0; d < 3; d++) {
+        p.pot_bc_lo[d] = bc_lo[d];
+        p.pot_bc_hi[d] = bc_hi[d];
+    }
+    vd_array4 ef[3] = {efield_ec[0][0].array(), efield_ec[0][1].array(), efield_ec[0][2].array()};
+    int st = vd_solve_potential(ctx_, mech_, &Sborder[0].array(), &phi_new_[0].array(), ef, &dom_, &geom_,
+                                &p, current_time, &pot_info_);
+    mg_iters_ += pot_info_.iterations;
+    return st;
+}
+
+int Vidyut::implicit_solve_scalar(Real current_time, Real dt, int spec_id, std::vector<MultiFab>& Sborder,
+                                  std::vector<MultiFab>& Sborder_old, std::vector<MultiFab>& dsdt_expl,
+                                  const int* bc_lo, const int* bc_hi, std::vector<FaceMF>& grad_fc) {
+    vd_array4 g[3] = {grad_fc[0][0].array(), grad_fc[0][1].array(), grad_fc[0][2].array()};
+    vd_solve_info info;
+    int st = vd_implicit_solve_scalar(ctx_, mech_, spec_id, &Sborder[0].array(), &Sborder_old[0].array(),
+                                      &dsdt_expl[0].array(), &phi_new_[0].array(), g, &dom_, &geom_, bc_lo,
+                                      bc_hi, &prm_, current_time, dt, &info);
+    mg_iters_ += info.iterations;
+    return st;
+}
+
+int Vidyut::find_time_scales(int lev, Real& dt_edrift, Real& dt_ediff, Real& dt_diel_relax) {
+    (void)lev;
+    double out[3];
+    TRY(vd_find_time_scales(ctx_, mech_, &phi_new_[0].array(), &dom_, &geom_, &prm_, out));
+    dt_edrift = out[0];
+    dt_ediff = out[1];
+    dt_diel_relax = out[2];
+    return VD_OK;
+}
+
+int Vidyut::set_tagging(const std::vector<int>& comps, const std::vector<double>& thr,
+                        const std::vector<double>& gthr) {
+    if (comps.size() != thr.size() || comps.size() != gthr.size()) return VD_EINVAL;
+    for (int c : comps)
+        if (c < 0 || c >= nvar_) return VD_ENOTFOUND;
+    tag_comps_ = comps;
+    tag_thr_ = thr;
+    tag_gthr_ = gthr;
+    return VD_OK;
+}
+
+int Vidyut::ErrorEst(int lev, vd_tagarray4& tags, Real time, int ngrow) {
+    (void)ngrow;
+    if (tag_comps_.empty()) return VD_OK;
+    MultiFab S(ctx_, dom_, dim_, nvar_, 1);
+    TRY(FillPatch(lev, time, S, 0, nvar_));
+    return vd_error_est(ctx_, &S.array(), &tags, &dom_, (int)tag_comps_.size(), tag_comps_.data(),
+                        tag_thr_.data(), tag_gthr_.data(), 1);
+}
+
+// Source/Evolve.cpp:33-369 on one level with fixed dt
+int Vidyut::Evolve(int nsteps, Real dt, int ncorr) {
+    const int E = desc_.e_idx;
+    for (int step = 0; step < nsteps; step++) {
+        Real dt_edrift, dt_ediff, dt_diel;
+        TRY(find_time_scales(0, dt_edrift, dt_ediff, dt_diel));
+        const Real cur_time = t_new_;
+        TRY(phi_old_[0].copy_from(phi_new_[0], 0, 0, nvar_));
+        // Sborder_old is read only in its valid region: phi_old stands in.
+        for (int niter = 0; niter < ncorr; niter++) {
+            TRY(FillPatch(0, cur_time + dt, Sborder_[0], 0, nvar_));
+            TRY(solve_potential(cur_time, Sborder_, prm_.pot_bc_lo, prm_.pot_bc_hi, efield_ec_));
+            // refill what solve_potential changed: POT, EFX..EFZ, REF
+            TRY(FillPatch(0, cur_time + dt, Sborder_[0], VD_POT(ns_), 4));
+            TRY(FillPatch(0, cur_time + dt, Sborder_[0], VD_REF(ns_), 1));
+            if (prm_.do_reactions)
+                TRY(update_rxnsrc_at_all_levels(Sborder_, rxn_src_, cur_time));
+            else
+                TRY(rxn_src_[0].setVal(0.0));
+            TRY(update_explsrc_at_all_levels(E, Sborder_, flux_, rxn_src_, expl_src_, prm_.eden_bc_lo,
+                                             prm_.eden_bc_hi, cur_time));
+            int st = implicit_solve_scalar(cur_time, dt, E, Sborder_, phi_old_, expl_src_, prm_.eden_bc_lo,
+                                           prm_.eden_bc_hi, gradne_fc_);
+            if (st != VD_OK && st != VD_ENOCONV) return st;
+            if (prm_.elecenergy_solve) {
+                TRY(update_explsrc_at_all_levels(ns_, Sborder_, flux_, rxn_src_, expl_src_, prm_.eenrg_bc_lo,
+                                                 prm_.eenrg_bc_hi, cur_time));
+                TRY(compute_elecenergy_source(0, Sborder_[0], rxn_src_[0], efield_ec_[0], gradne_fc_[0],
+                                              expl_src_[0], cur_time, dt, prm_.floor_jh));
+                st = implicit_solve_scalar(cur_time, dt, ns_, Sborder_, phi_old_, expl_src_, prm_.eenrg_bc_lo,
+                                           prm_.eenrg_bc_hi, grad_fc_);
+                if (st != VD_OK && st != VD_ENOCONV) return st;
+            }
+            for (int ind = 0; ind < ns_; ind++) {
+                bool solve = !(desc_.is_bg[ind] || ind == E);
+                if (solve) {
+                    const int* lo = desc_.charge[ind] != 0 ? prm_.ion_bc_lo : prm_.neutral_bc_lo;
+                    const int* hi = desc_.charge[ind] != 0 ? prm_.ion_bc_hi : prm_.neutral_bc_hi;
+                    TRY(update_explsrc_at_all_levels(ind, Sborder_, flux_, rxn_src_, expl_src_, lo, hi,
+                                                     cur_time));
+                    st = implicit_solve_scalar(cur_time, dt, ind, Sborder_, phi_old_, expl_src_, lo, hi,
+                                               grad_fc_);
+                    if (st != VD_OK && st != VD_ENOCONV) return st;
+                } else if (prm_.do_bg_reactions) {
+                    // Source/Evolve.cpp:267-287 (also reached for the electron index)
+                    TRY(vd_lincomb(ctx_, &phi_new_[0].array(), 1.0, &phi_new_[0].array(), ind, dt,
+                                   &rxn_src_[0].array(), ind, ind, 1, &dom_));
+                    if (prm_.bound_specden)
+                        TRY(vd_clamp_min(ctx_, &phi_new_[0].array(), &dom_, ind, prm_.min_species_density));
+                }
+            }
+            if (niter < ncorr - 1)
+                TRY(vd_lincomb(ctx_, &phi_new_[0].array(), 0.5, &phi_old_[0].array(), 0, 0.5,
+                               &phi_new_[0].array(), 0, 0, nvar_, &dom_));
+        }
+        istep_++;
+        t_new_ = cur_time + dt;
+    }
+    return vd_ctx_sync(ctx_);
+}
+
+// --------------------------------------------------------------- plotfile
+static int mkdir_p(const std::string& d) {
+    if (mkdir(d.c_str(), 0755) != 0 && errno != EEXIST) return VD_EINVAL;
+    return VD_OK;
+}
+
+static std::string box_str(const vd_box& b, int dim) {
+    std::ostringstream o;
+    o << "((";
+    for (int d = 0; d < dim; d++) o << b.lo[d] << (d + 1 < dim ? "," : "");
+    o << ") (";
+    for (int d = 0; d < dim; d++) o << b.hi[d] << (d + 1 < dim ? "," : "");
+    o << ") (";
+    for (int d = 0; d < dim; d++) o << 0 << (d + 1 < dim ? "," : "");
+    o << "))";
+    return o.str();
+}
+
+// WritePlotFile -> amrex::WriteMultiLevelPlotfile (Source/Plot.cpp:29-38):
+// HyperCLaw-V1.1 Header, Level_0/Cell_H (VisMF v1) and one Cell_D file
+// holding the FAB (little-endian IEEE doubles, Fortran order per comp).
+int Vidyut::WritePlotFile(const std::string& dir) const {
+    const vd_array4& a = phi_new_[0].array();
+    size_t n = (size_t)a.nstride * nvar_;
+    std::vector<double> h(n);
+    TRY(vd_memcpy_d2h(ctx_, h.data(), a.p, n * sizeof(double)));
+    TRY(mkdir_p(dir));
+    TRY(mkdir_p(dir + "/Level_0"));
+    std::vector<std::string> names = varnames();
+    {
+        std::ofstream H(dir + "/Header");
+        H.precision(17);
+        H << "HyperCLaw-V1.1\n" << nvar_ << "\n";
+        for (auto& s : names) H << s << "\n";
+        H << dim_ << "\n" << t_new_ << "\n" << 0 << "\n";
+        for (int d = 0; d < dim_; d++) H << geom_.prob_lo[d] << " ";
+        H << "\n";
+        for (int d = 0; d < dim_; d++) H << geom_.prob_hi[d] << " ";
+        H << "\n\n" << box_str(dom_, dim_) << " \n" << istep_ << " \n";
+        for (int d = 0; d < dim_; d++) H << geom_.dx[d] << " ";
+        H << "\n0\n0\n";
+        H << 0 << " " << 1 << " " << t_new_ << "\n" << istep_ << "\n";
+        for (int d = 0; d < dim_; d++) {
+            double lo = geom_.prob_lo[d] + geom_.dx[d] * (dom_.lo[d] - geom_.dom_lo[d]);
+            double hi = geom_.prob_lo[d] + geom_.dx[d] * (dom_.hi[d] + 1 - geom_.dom_lo[d]);
+            H << lo << " " << hi << "\n";
+        }
+        H << "Level_0/Cell\n";
+    }
+    std::string hdr;
+    {
+        std::ostringstream f;
+        f << "FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))" << box_str(dom_, dim_) << " "
+          << nvar_ << "\n";
+        hdr = f.str();
+        std::ofstream D(dir + "/Level_0/Cell_D_00000", std::ios::binary);
+        D.write(hdr.data(), hdr.size());
+        D.write((const char*)h.data(), n * sizeof(double));
+    }
+    std::vector<double> mn(nvar_, DBL_MAX), mx(nvar_, -DBL_MAX);
+    for (int q = 0; q < nvar_; q++)
+        for (int64_t c = 0; c < a.nstride; c++) {
+            double v = h[(size_t)q * a.nstride + c];
+            mn[q] = std::min(mn[q], v);
+            mx[q] = std::max(mx[q], v);
+        }
+    std::ofstream C(dir + "/Level_0/Cell_H");
+    C.precision(16);
+    C << std::scientific;
+    C << "1\n1\n" << nvar_ << "\n0\n(1 0\n" << box_str(dom_, dim_) << "\n)\n1\nFabOnDisk: Cell_D_00000 0\n\n";
+    C << "1," << nvar_ << "\n";
+    for (int q = 0; q < nvar_; q++) C << mn[q] << ",";
+    C << "\n\n1," << nvar_ << "\n";
+    for (int q = 0; q < nvar_; q++) C << mx[q] << ",";
+    C << "\n";
+    return VD_OK;
+}
+
+}  // namespace vidyut_amd
+
+// ------------------------------------------------------------ C handle
+struct vd_sim {
+    vidyut_amd::Vidyut* v;
+    vd_ctx* ctx;
+};
+
+using vidyut_amd::Vidyut;
+
+extern "C" {
+
+int vd_sim_create(vd_ctx* ctx, const vd_mech_desc* mech, const vd_geom* geom, const vd_params* prm,
+                  vd_sim** out) {
+    if (!ctx || !mech || !geom || !prm || !out) return VD_EINVAL;
+    if (geom->dim != 2 && geom->dim != 3) return VD_EINVAL;
+    Vidyut* v = new Vidyut(ctx, *mech, *geom, *prm);
+    if (v->status() != VD_OK) {
+        int st = v->status();
+        delete v;
+        return st;
+    }
+    *out = new vd_sim{v, ctx};
+    return VD_OK;
+}
+
+int vd_sim_destroy(vd_sim* s) {
+    if (!s) return VD_EINVAL;
+    delete s->v;
+    delete s;
+    return VD_OK;
+}
+
+int vd_sim_set_state(vd_sim* s, const double* host, double time) {
+    auto& mf = s->v->phi_new();
+    s->v->set_time(time);
+    return vd_memcpy_h2d(s->ctx, mf.data(), host, mf.bytes());
+}
+
+int vd_sim_get_state(vd_sim* s, double* host) {
+    auto& mf = s->v->phi_new();
+    return vd_memcpy_d2h(s->ctx, host, mf.data(), mf.bytes());
+}
+
+int vd_sim_state_view(vd_sim* s, vd_array4* out) {
+    *out = s->v->phi_new().array();
+    return VD_OK;
+}
+
+int vd_sim_evolve(vd_sim* s, int nsteps, double dt, int ncorr) {
+    if (nsteps < 0 || !(dt > 0) || ncorr < 1) return VD_EINVAL;
+    return s->v->Evolve(nsteps, dt, ncorr);
+}
+
+double vd_sim_time(const vd_sim* s) { return s->v->time(); }
+long long vd_sim_mg_iterations(const vd_sim* s) { return s->v->mg_iterations(); }
+
+int vd_sim_write_plotfile(vd_sim* s, const char* dir) { return s->v->WritePlotFile(dir); }
+
+int vd_sim_error_est(vd_sim* s, int ntag, const int* comps, const double* thr, const double* gthr,
+                     char* host_tags) {
+    std::vector<int> c(comps, comps + ntag);
+    std::vector<double> t(thr, thr + ntag), g(gthr, gthr + ntag);
+    int st = s->v->set_tagging(c, t, g);
+    if (st) return st;
+    const vd_box& b = s->v->phi_new().box();
+    size_t n = (size_t)(b.hi[0] - b.lo[0] + 1) * (b.hi[1] - b.lo[1] + 1) * (b.hi[2] - b.lo[2] + 1);
+    void* d;
+    st = vd_malloc(s->ctx, n, &d);
+    if (st) return st;
+    vd_memset_zero(s->ctx, d, n);
+    vd_tagarray4 tags;
+    tags.p = (char*)d;
+    for (int q = 0; q < 3; q++) {
+        tags.begin[q] = b.lo[q];
+        tags.end[q] = b.hi[q] + 1;
+    }
+    tags.jstride = tags.end[0] - tags.begin[0];
+    tags.kstride = tags.jstride * (tags.end[1] - tags.begin[1]);
+    st = s->v->ErrorEst(0, tags, s->v->time(), 0);
+    if (st == VD_OK) st = vd_memcpy_d2h(s->ctx, host_tags, d, n);
+    vd_free(s->ctx, d);
+    return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,327 @@
+"""ctypes front end of libvidyut3d_amd.so (the C ABI of include/
+vidyut3d_amd.h and include/vidyut3d_amd_host.h).
+
+Device memory is owned by small Python handles (``DeviceArray``) that carry
+their Array4 extent; every call goes straight to the HIP library.  There is
+no CPU fallback: if the library is missing or no GPU is visible the calls
+raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBPATH = os.path.join(HERE, "libvidyut3d_amd.so")
+_lib = None
+
+P = C.POINTER
+A4P = P(abi.vd_array4)
+
+
+class VidyutError(RuntimeError):
+    def __init__(self, what, status, msg):
+        super().__init__("%s failed: %s (%s)" % (what, abi.STATUS_NAMES.get(status, status), msg))
+        self.status = status
+
+
+def lib():
+    """Loads the in-tree HIP library (built by vidyut3d_amd.build)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIBPATH):
+            raise RuntimeError("libvidyut3d_amd.so is not built: run __graft_entry__.build() or "
+                               "python vidyut3d_amd/build.py")
+        L = C.CDLL(LIBPATH)
+        L.vd_last_error.restype = C.c_char_p
+        L.vd_version.restype = C.c_char_p
+        L.vd_ctx_stream.restype = C.c_void_p
+        L.vd_sim_time.restype = C.c_double
+        L.vd_sim_time.argtypes = [C.c_void_p]
+        L.vd_sim_mg_iterations.restype = C.c_longlong
+        L.vd_sim_mg_iterations.argtypes = [C.c_void_p]
+        L.vd_mech_find_id.argtypes = [C.c_void_p, C.c_char_p]
+        _lib = L
+    return _lib
+
+
+def check(st, what, allow=(abi.VD_OK,)):
+    if st not in allow:
+        raise VidyutError(what, st, lib().vd_last_error().decode(errors="replace"))
+    return st
+
+
+class Device:
+    """A vd_ctx (one HIP stream + workspaces) on one GPU."""
+
+    def __init__(self, device=0):
+        self.ptr = C.c_void_p()
+        check(lib().vd_ctx_create(int(device), C.byref(self.ptr)), "vd_ctx_create")
+
+    def close(self):
+        if self.ptr:
+            lib().vd_ctx_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def sync(self):
+        check(lib().vd_ctx_sync(self.ptr), "vd_ctx_sync")
+
+    def alloc(self, nbytes):
+        p = C.c_void_p()
+        check(lib().vd_malloc(self.ptr, C.c_size_t(int(nbytes)), C.byref(p)), "vd_malloc")
+        return p.value
+
+    def free(self, p):
+        lib().vd_free(self.ptr, C.c_void_p(p))
+
+    def upload(self, dptr, arr):
+        arr = np.ascontiguousarray(arr)
+        check(lib().vd_memcpy_h2d(self.ptr, C.c_void_p(dptr), arr.ctypes.data_as(C.c_void_p),
+                                  C.c_size_t(arr.nbytes)), "vd_memcpy_h2d")
+
+    def download(self, dptr, out):
+        check(lib().vd_memcpy_d2h(self.ptr, out.ctypes.data_as(C.c_void_p), C.c_void_p(dptr),
+                                  C.c_size_t(out.nbytes)), "vd_memcpy_d2h")
+        return out
+
+    def set_kernel_timing(self, family):
+        check(lib().vd_ctx_set_kernel_timing(self.ptr, int(family)), "vd_ctx_set_kernel_timing")
+
+    def kernel_timing(self):
+        ms, n, cells = C.c_double(), C.c_longlong(), C.c_double()
+        check(lib().vd_ctx_kernel_timing(self.ptr, C.byref(ms), C.byref(n), C.byref(cells)),
+              "vd_ctx_kernel_timing")
+        return ms.value, n.value, cells.value
+
+
+class DeviceArray:
+    """Device Array4 of shape (ncomp, nz, ny, nx) over begin/end."""
+
+    def __init__(self, dev, begin, end, ncomp, host=None, dtype=np.float64):
+        self.dev, self.begin, self.end, self.ncomp = dev, list(begin), list(end), ncomp
+        self.shape = abi.shape_of(begin, end, ncomp)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        self.ptr = dev.alloc(self.nbytes)
+        if host is not None:
+            self.upload(host)
+        else:
+            check(lib().vd_memset_zero(dev.ptr, C.c_void_p(self.ptr), C.c_size_t(self.nbytes)),
+                  "vd_memset_zero")
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, dtype=self.dtype)
+        assert host.shape == self.shape, (host.shape, self.shape)
+        self.dev.upload(self.ptr, host)
+
+    def get(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        return self.dev.download(self.ptr, out)
+
+    def view(self):
+        return abi.array4_view(self.ptr, self.begin, self.end, self.ncomp)
+
+    def tag_view(self):
+        return abi.tag_view(self.ptr, self.begin, self.end)
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                self.dev.free(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class Mech:
+    def __init__(self, dev, desc):
+        self.dev, self.desc = dev, desc
+        self.ptr = C.c_void_p()
+        check(lib().vd_mech_create(dev.ptr, C.byref(desc), C.byref(self.ptr)), "vd_mech_create")
+
+    def find_id(self, name):
+        return lib().vd_mech_find_id(self.ptr, name.encode())
+
+    def __del__(self):
+        try:
+            if self.ptr:
+                lib().vd_mech_destroy(self.ptr)
+                self.ptr = None
+        except Exception:
+            pass
+
+
+class Grid:
+    """Device arrays of one single-box level."""
+
+    def __init__(self, dev, geom):
+        self.dev, self.geom = dev, geom
+        self.box = abi.domain_box(geom)
+        self.dim = geom.dim
+
+    def extent(self, ng=0, face=None):
+        return abi.grown_extent(self.box, ng, self.dim, face)
+
+    def array(self, ncomp, ng=0, face=None, host=None):
+        b, e = self.extent(ng, face)
+        return DeviceArray(self.dev, b, e, ncomp, host)
+
+    def faces(self):
+        dims = [0, 1, 2] if self.dim == 3 else [0, 1, 0]
+        return [self.array(1, 0, d) for d in dims]
+
+    @staticmethod
+    def face_views(fs):
+        return (abi.vd_array4 * 3)(*[f.view() for f in fs])
+
+
+# --------------------------------------------------------- entry points
+def fill_patch(grid, src, dst, scomp=0, ncomp=None):
+    ncomp = src.ncomp if ncomp is None else ncomp
+    s, d = src.view(), dst.view()
+    check(lib().vd_fill_patch(grid.dev.ptr, C.byref(s), C.byref(d), C.byref(grid.box),
+                              C.byref(grid.geom), int(scomp), int(ncomp)), "vd_fill_patch")
+
+
+def transport_flux(mech, prm, grid, sb, specid, bclo, bchi, flux, dt=1e-12):
+    check(lib().vd_compute_scalar_transport_flux(
+        grid.dev.ptr, mech.ptr, C.byref(sb.view()), Grid.face_views(flux), C.byref(grid.box),
+        C.byref(grid.geom), abi.ivec(bclo), abi.ivec(bchi), C.byref(prm), C.c_double(0.0),
+        C.c_double(dt), int(specid)), "vd_compute_scalar_transport_flux")
+
+
+def dsdt(grid, flux, rxn, out, specid):
+    check(lib().vd_compute_dsdt(grid.dev.ptr, Grid.face_views(flux), C.byref(rxn.view()),
+                                C.byref(out.view()), C.byref(grid.box), C.byref(grid.geom),
+                                int(specid)), "vd_compute_dsdt")
+
+
+def transport_rhs(mech, prm, grid, sb, rxn, out, specid, bclo, bchi, dt=1e-12):
+    check(lib().vd_transport_rhs(grid.dev.ptr, mech.ptr, C.byref(sb.view()), C.byref(rxn.view()),
+                                 C.byref(out.view()), C.byref(grid.box), C.byref(grid.geom),
+                                 abi.ivec(bclo), abi.ivec(bchi), C.byref(prm), C.c_double(0.0),
+                                 C.c_double(dt), int(specid)), "vd_transport_rhs")
+
+
+def rxn_source(mech, prm, grid, sb, out):
+    check(lib().vd_update_rxnsrc(grid.dev.ptr, mech.ptr, C.byref(sb.view()), C.byref(out.view()),
+                                 C.byref(grid.box), C.byref(prm), C.c_double(0.0)),
+          "vd_update_rxnsrc")
+
+
+def elec_energy_source(mech, prm, grid, sb, rxn, efield, gradne, ds, phi_new):
+    check(lib().vd_compute_elecenergy_source(
+        grid.dev.ptr, mech.ptr, C.byref(sb.view()), C.byref(rxn.view()), Grid.face_views(efield),
+        Grid.face_views(gradne), C.byref(ds.view()), C.byref(phi_new.view()), C.byref(grid.box),
+        C.byref(grid.geom), C.byref(prm)), "vd_compute_elecenergy_source")
+
+
+def solve_potential(mech, prm, grid, sb, phi_new, efield, time=0.0):
+    info = abi.vd_solve_info()
+    check(lib().vd_solve_potential(grid.dev.ptr, mech.ptr, C.byref(sb.view()),
+                                   C.byref(phi_new.view()), Grid.face_views(efield),
+                                   C.byref(grid.box), C.byref(grid.geom), C.byref(prm),
+                                   C.c_double(time), C.byref(info)),
+          "vd_solve_potential", (abi.VD_OK, abi.VD_ENOCONV))
+    return info
+
+
+def implicit_solve(mech, prm, grid, spec, sb, phi_old, dsdt_expl, phi_new, grad, bclo, bchi, dt):
+    info = abi.vd_solve_info()
+    check(lib().vd_implicit_solve_scalar(
+        grid.dev.ptr, mech.ptr, int(spec), C.byref(sb.view()), C.byref(phi_old.view()),
+        C.byref(dsdt_expl.view()), C.byref(phi_new.view()), Grid.face_views(grad),
+        C.byref(grid.box), C.byref(grid.geom), abi.ivec(bclo), abi.ivec(bchi), C.byref(prm),
+        C.c_double(0.0), C.c_double(dt), C.byref(info)),
+        "vd_implicit_solve_scalar", (abi.VD_OK, abi.VD_ENOCONV))
+    return info
+
+
+def time_scales(mech, prm, grid, phi):
+    out = (C.c_double * 3)()
+    check(lib().vd_find_time_scales(grid.dev.ptr, mech.ptr, C.byref(phi.view()),
+                                    C.byref(grid.box), C.byref(grid.geom), C.byref(prm), out),
+          "vd_find_time_scales")
+    return list(out)
+
+
+def error_est(grid, state1, tags, comps, thr, gthr, tagval=1):
+    n = len(comps)
+    tv = tags.tag_view()
+    check(lib().vd_error_est(grid.dev.ptr, C.byref(state1.view()), C.byref(tv), C.byref(grid.box),
+                             n, (C.c_int * n)(*comps), (C.c_double * n)(*thr),
+                             (C.c_double * n)(*gthr), C.c_char(tagval)), "vd_error_est")
+
+
+def mlmg_solve(grid, prob, phi, rhs):
+    info = abi.vd_solve_info()
+    check(lib().vd_mlmg_solve(grid.dev.ptr, C.byref(prob), C.byref(phi.view()),
+                              C.byref(rhs.view()), C.byref(grid.box), C.byref(grid.geom),
+                              C.byref(info)), "vd_mlmg_solve", (abi.VD_OK, abi.VD_ENOCONV))
+    return info
+
+
+class Sim:
+    """vd_sim: the C++ Vidyut driver (device-resident state)."""
+
+    def __init__(self, dev, case):
+        self.dev, self.case = dev, case
+        self.ptr = C.c_void_p()
+        check(lib().vd_sim_create(dev.ptr, C.byref(case.mech), C.byref(case.geom),
+                                  C.byref(case.params), C.byref(self.ptr)), "vd_sim_create")
+        self.shape = case.state.shape
+        self.set_state(case.state, 0.0)
+
+    def set_state(self, st, time=0.0):
+        st = np.ascontiguousarray(st, dtype=np.float64)
+        assert st.shape == self.shape
+        check(lib().vd_sim_set_state(self.ptr, st.ctypes.data_as(C.c_void_p), C.c_double(time)),
+              "vd_sim_set_state")
+
+    def get_state(self):
+        out = np.empty(self.shape)
+        check(lib().vd_sim_get_state(self.ptr, out.ctypes.data_as(C.c_void_p)), "vd_sim_get_state")
+        return out
+
+    def evolve(self, nsteps, dt=None, ncorr=1):
+        dt = self.case.dt if dt is None else dt
+        check(lib().vd_sim_evolve(self.ptr, int(nsteps), C.c_double(dt), int(ncorr)),
+              "vd_sim_evolve", (abi.VD_OK, abi.VD_ENOCONV))
+
+    @property
+    def time(self):
+        return lib().vd_sim_time(self.ptr)
+
+    @property
+    def mg_iterations(self):
+        return lib().vd_sim_mg_iterations(self.ptr)
+
+    def write_plotfile(self, path):
+        check(lib().vd_sim_write_plotfile(self.ptr, path.encode()), "vd_sim_write_plotfile")
+
+    def error_est(self, comps, thr, gthr):
+        n = len(comps)
+        out = np.zeros(self.shape[1:], dtype=np.int8)
+        check(lib().vd_sim_error_est(self.ptr, n, (C.c_int * n)(*comps), (C.c_double * n)(*thr),
+                                     (C.c_double * n)(*gthr), out.ctypes.data_as(C.c_char_p)),
+              "vd_sim_error_est")
+        return out
+
+    def close(self):
+        if self.ptr:
+            lib().vd_sim_destroy(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
