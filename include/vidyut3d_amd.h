@@ -340,8 +340,9 @@ int vd_mlmg_solve(vd_ctx* ctx, const vd_mlmg_problem* prob, vd_array4* phi, cons
 
 /* ------------------------------------------------- kernel timing (bench)
  * When enabled for a kernel family, every launch of that family is
- * bracketed by hipEvents on the context's stream; the totals come back
- * with vd_ctx_kernel_timing (synchronises). */
+ * bracketed by hipEvents on the context's stream; the totals (time,
+ * launches and the canonical algorithmic bytes of those launches) come
+ * back with vd_ctx_kernel_timing (synchronises). */
 enum {
     VD_KT_NONE = 0,
     VD_KT_GSRB = 1,       /* multigrid red-black smoother, finest level */
@@ -351,7 +352,7 @@ enum {
     VD_KT_FILL = 5        /* ghost fill */
 };
 int vd_ctx_set_kernel_timing(vd_ctx* ctx, int family);
-int vd_ctx_kernel_timing(vd_ctx* ctx, double* total_ms, long long* launches, double* cells);
+int vd_ctx_kernel_timing(vd_ctx* ctx, double* total_ms, long long* launches, double* bytes);
 
 #ifdef __cplusplus
 }

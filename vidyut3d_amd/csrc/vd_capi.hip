@@ -51,10 +51,10 @@ void kt_begin(vd_ctx* ctx, int family) {
     ctx->kt_events.push_back({a, b});
 }
 
-void kt_end(vd_ctx* ctx, int family, double cells) {
+void kt_end(vd_ctx* ctx, int family, double bytes) {
     if (ctx->kt_family == VD_KT_NONE || ctx->kt_family != family || ctx->kt_events.empty()) return;
     (void)hipEventRecord(ctx->kt_events.back().second, ctx->stream);
-    ctx->kt_cells += cells;
+    ctx->kt_bytes += bytes;
 }
 
 }  // namespace vd
@@ -301,11 +301,11 @@ int vd_ctx_set_kernel_timing(vd_ctx* ctx, int family) {
     ctx->kt_family = family;
     ctx->kt_events.clear();
     ctx->kt_next = 0;
-    ctx->kt_cells = 0.0;
+    ctx->kt_bytes = 0.0;
     return VD_OK;
 }
 
-int vd_ctx_kernel_timing(vd_ctx* ctx, double* total_ms, long long* launches, double* cells) {
+int vd_ctx_kernel_timing(vd_ctx* ctx, double* total_ms, long long* launches, double* bytes) {
     VD_HIP(hipStreamSynchronize(ctx->stream));
     double tot = 0.0;
     for (auto& pr : ctx->kt_events) {
@@ -315,7 +315,7 @@ int vd_ctx_kernel_timing(vd_ctx* ctx, double* total_ms, long long* launches, dou
     }
     if (total_ms) *total_ms = tot;
     if (launches) *launches = (long long)ctx->kt_events.size();
-    if (cells) *cells = ctx->kt_cells;
+    if (bytes) *bytes = ctx->kt_bytes;
     return VD_OK;
 }
 

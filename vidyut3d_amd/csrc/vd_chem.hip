@@ -170,7 +170,7 @@ int update_rxnsrc(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4*
         hipLaunchKernelGGL(k_rxn_table, grid1d(tot, CHEM_BLOCK), dim3(CHEM_BLOCK), lds, ctx->stream, a);
     else
         hipLaunchKernelGGL(k_rxn_ckwc, grid1d(tot, CHEM_BLOCK), dim3(CHEM_BLOCK), lds, ctx->stream, a);
-    kt_end(ctx, VD_KT_CHEM, (double)tot);
+    kt_end(ctx, VD_KT_CHEM, 8.0 * (double)tot * (2 * m->hdev.nspec + 3));
     VD_CHECK_LAUNCH();
     return VD_OK;
 }

@@ -98,7 +98,7 @@ int fill_patch(vd_ctx* ctx, const vd_array4* src, vd_array4* dst, const vd_box* 
     kt_begin(ctx, VD_KT_FILL);
     hipLaunchKernelGGL(k_fill_patch, dim3((unsigned)((tot + 255) / 256), ncomp), dim3(256), 0,
                        ctx->stream, arr(src), arr(dst), g->dim, lo, n, per, nx, ny, nz, scomp);
-    kt_end(ctx, VD_KT_FILL, (double)tot * ncomp);
+    kt_end(ctx, VD_KT_FILL, 16.0 * (double)tot * ncomp);
     VD_CHECK_LAUNCH();
     return VD_OK;
 }

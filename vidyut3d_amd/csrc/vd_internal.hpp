@@ -153,7 +153,7 @@ struct vd_ctx {
     int kt_family = VD_KT_NONE;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> kt_events;
     std::vector<hipEvent_t> kt_pool;
-    double kt_cells = 0.0;
+    double kt_bytes = 0.0;
     size_t kt_next = 0;
 };
 
@@ -164,7 +164,7 @@ int scratch(vd_ctx* ctx, const std::string& name, size_t bytes, void** out);
 
 // kernel-timing brackets (no-ops unless the family is selected)
 void kt_begin(vd_ctx* ctx, int family);
-void kt_end(vd_ctx* ctx, int family, double cells);
+void kt_end(vd_ctx* ctx, int family, double bytes);
 
 inline dim3 grid1d(int64_t n, int block) { return dim3((unsigned)((n + block - 1) / block)); }
 

@@ -644,7 +644,8 @@ static void launch_gsrb(vd_ctx* ctx, const MGWork& W, const LevV<T>& v, T* x, co
         if (W.dim == 3) hipLaunchKernelGGL((k_gsrb<T, false, 3>), g, b, 0, ctx->stream, v, x, rhs, color);
         else hipLaunchKernelGGL((k_gsrb<T, false, 2>), g, b, 0, ctx->stream, v, x, rhs, color);
     }
-    if (fine) kt_end(ctx, VD_KT_GSRB, (double)v.nx * v.ny * v.nz * 0.5);
+    // one colour = half of a sweep that reads phi, rhs, n_b faces and writes phi
+    if (fine) kt_end(ctx, VD_KT_GSRB, 8.0 * (double)v.nx * v.ny * v.nz * (3 + (W.constb ? 0 : W.dim)) / 2.0);
 }
 
 template <class T>
@@ -755,7 +756,7 @@ static int resid0(vd_ctx* ctx, MGWork& W, const LevV<double>& v0, double* nrm_ou
         if (W.dim == 3) hipLaunchKernelGGL((k_resid0<false, 3>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, W.dnorm);
         else hipLaunchKernelGGL((k_resid0<false, 2>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, W.dnorm);
     }
-    kt_end(ctx, VD_KT_RESID, (double)nc);
+    kt_end(ctx, VD_KT_RESID, 8.0 * (double)nc * (4 + (W.constb ? 0 : W.dim)));
     VD_CHECK_LAUNCH();
     VD_HIP(hipMemcpyAsync(ctx->pinned, W.dnorm, 8, hipMemcpyDeviceToHost, ctx->stream));
     VD_HIP(hipStreamSynchronize(ctx->stream));

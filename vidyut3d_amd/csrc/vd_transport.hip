@@ -280,7 +280,7 @@ int transport_rhs(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, const vd_a
     kt_begin(ctx, VD_KT_TRANSPORT);
     if (prm->do_transport) VD_TRY(transport_flux(ctx, m, sb, fl, box, g, bc_lo, bc_hi, prm, dt, specid));
     int st = dsdt(ctx, fl, rxn, out, box, g, specid);
-    kt_end(ctx, VD_KT_TRANSPORT, cells);
+    kt_end(ctx, VD_KT_TRANSPORT, 8.0 * cells * (m->hdev.nspec + 6));
     return st;
 }
 
