@@ -278,7 +278,7 @@ typedef struct vd_solve_info {
     int iterations;       /* outer (V-cycle / IR) iterations */
     double resnorm0;      /* max-norm of the initial residual */
     double resnorm;       /* max-norm of the final residual   */
-    double target;        /* max(abstol, reltol*resnorm0)     */
+    double target;        /* max(abstol, reltol*max(|b|, resnorm0)) */
 } vd_solve_info;
 
 /* Vidyut::solve_potential (Source/PotentialSolve.cpp:19-393): rhs

@@ -614,7 +614,9 @@ int or_error_est(const vd_array4* st, vd_tagarray4* tags, const vd_box* box, int
  *              (i+j+k)&1, red first), cell-average restriction, linear
  *              (tensor 3/4-1/4) prolongation, face coefficients averaged to
  *              coarse faces, CG bottom solve on the coarsest level
- *   stop       ||r||_inf <= max(abstol, reltol*||r0||_inf)
+ *   stop       ||r||_inf <= max(abstol, reltol*max(||b||_inf, ||r0||_inf)),
+ *              b = rhs with the boundary data folded in (MLMG::solve's
+ *              bnorm / resid0 choice)
  * ==================================================================== */
 
 typedef struct {
@@ -1072,7 +1074,13 @@ int or_mlmg_solve(const vd_mlmg_problem* P, vd_array4* phi, const vd_array4* rhs
                 x[c] = AT(phi, i + box->lo[0], j + box->lo[1], k + box->lo[2], 0);
             }
     double r0 = residual(M, L, x, f, L->r);
-    double target = fmax(P->abstol, P->reltol * r0);
+    double bnorm = 0.0;
+    for (int64_t c = 0; c < n; c++) {
+        double a = fabs(f[c]);
+        if (a != a) a = INFINITY;
+        if (a > bnorm) bnorm = a;
+    }
+    double target = fmax(P->abstol, P->reltol * fmax(bnorm, r0));
     double rn = r0;
     int it = 0, st = VD_OK;
     if (!(r0 <= target)) {

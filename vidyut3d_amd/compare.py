@@ -22,3 +22,29 @@ def field_errors(f1, f2):
 def compare_states(ref, test, names):
     """Per-field error dict for two (NVAR, nz, ny, nx) states."""
     return {n: field_errors(ref[q], test[q]) for q, n in enumerate(names)}
+
+
+def state_errors(ref, test, ns, dim):
+    """Per-field relative L2 errors of two states (NVAR, nz, ny, nx).
+
+    The E-field components are normalised by the norm of the whole field
+    vector: a component that is ~0 everywhere (e.g. E_y of a symmetric
+    discharge at t ~ 0) has no scale of its own, and its relative error
+    would only measure rounding noise of the potential.  Fields that are
+    identically zero in the reference must be identically zero in `test`.
+    Returns {component index: rel_l2}."""
+    ref = np.asarray(ref, dtype=np.float64)
+    test = np.asarray(test, dtype=np.float64)
+    efx = ns + 3
+    evec = float(np.sqrt(sum(np.sum(ref[efx + d] ** 2) for d in range(dim))))
+    out = {}
+    for q in range(ref.shape[0]):
+        d = float(np.linalg.norm((ref[q] - test[q]).ravel()))
+        nrm = float(np.linalg.norm(ref[q].ravel()))
+        if efx <= q < efx + dim:
+            nrm = evec
+        if nrm == 0.0:
+            out[q] = 0.0 if d == 0.0 else float("inf")
+        else:
+            out[q] = d / nrm
+    return out

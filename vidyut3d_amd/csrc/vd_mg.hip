@@ -14,7 +14,8 @@
 //    red first), residual fused with cell-average restriction, linear
 //    prolongation, face b and boundary c0 averaged to the coarse level,
 //    CG bottom solve in one workgroup
-//  * stop when ||r||_inf <= max(abstol, reltol*||r0||_inf)
+//  * stop when ||r||_inf <= max(abstol, reltol*max(||b||_inf, ||r0||_inf)),
+//    b = rhs with the boundary data folded in
 //  * precision 1: the V-cycle runs in FP32 on FP32 copies of the
 //    hierarchy; the finest-level residual, the update x += e and the
 //    stopping test stay FP64 (iterative refinement).
@@ -179,15 +180,30 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double 
     atomicMax(p, (unsigned long long)__double_as_longlong(v));
 }
 
+// block max (256 threads: wave64 shuffles, then LDS) and one atomic per block
+__device__ __forceinline__ void block_max_atomic(double a, unsigned long long* nrm) {
+    __shared__ double red[4];
+    for (int o = 32; o > 0; o >>= 1) a = fmax(a, __shfl_xor(a, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, red[w]);
+        atomic_max_nonneg(nrm, m);
+    }
+}
+
+constexpr int NORM_BLOCKS = 2048;  // grid-stride: 8 blocks per CU, 2048 atomics per norm
+
 // r = f - L x with the max-norm (finest level, FP64)
 template <bool CB, int DIM>
 __global__ void __launch_bounds__(256) k_resid0(LevV<double> L, const double* __restrict__ x,
                                                 const double* __restrict__ f, double* __restrict__ r,
                                                 unsigned long long* __restrict__ nrm) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    int64_t tot = (int64_t)L.nx * L.ny * L.nz;
+    const int64_t tot = (int64_t)L.nx * L.ny * L.nz;
     double a = 0.0;
-    if (t < tot) {
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot;
+         t += (int64_t)gridDim.x * blockDim.x) {
         int i = (int)(t % L.nx);
         int row = (int)(t / L.nx);
         int j = row % L.ny, k = row / L.ny;
@@ -195,12 +211,23 @@ __global__ void __launch_bounds__(256) k_resid0(LevV<double> L, const double* __
         stencil<double, CB, DIM>(L, x, i, j, k, dg, of);
         double v = f[t] - (dg * x[t] - of);
         r[t] = v;
-        a = fabs(v);
-        if (a != a) a = __builtin_huge_val();
+        double b = fabs(v);
+        if (b != b) b = __builtin_huge_val();
+        a = fmax(a, b);
     }
-    // wave64 max, then one atomic per wave
-    for (int o = 32; o > 0; o >>= 1) a = fmax(a, __shfl_xor(a, o, 64));
-    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(nrm, a);
+    block_max_atomic(a, nrm);
+}
+
+__global__ void __launch_bounds__(256) k_maxabs(const double* __restrict__ f, int64_t n,
+                                                unsigned long long* __restrict__ nrm) {
+    double a = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        double b = fabs(f[t]);
+        if (b != b) b = __builtin_huge_val();
+        a = fmax(a, b);
+    }
+    block_max_atomic(a, nrm);
 }
 
 // coarse r = average of the 2^DIM fine residuals rhs - L e (fused)
@@ -410,7 +437,7 @@ __global__ void k_axpy_to_double(double* __restrict__ x, const T* __restrict__ e
     if (t < n) x[t] += (double)e[t];
 }
 
-__global__ void k_zero_u64(unsigned long long* p) { *p = 0ull; }
+__global__ void k_zero_u64(unsigned long long* p) { p[threadIdx.x] = 0ull; }
 
 // BC coefficient for one face (ghost = c0*x + cinh)
 __device__ __host__ inline void bc_coef(int type, int side, double dx, double val, double ra,
@@ -744,10 +771,19 @@ static void coarsen_hierarchy(vd_ctx* ctx, MGWork& W, std::vector<Lev<T>>& L) {
     }
 }
 
-static int resid0(vd_ctx* ctx, MGWork& W, const LevV<double>& v0, double* nrm_out) {
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(1), 0, ctx->stream, W.dnorm);
+static inline dim3 norm_grid(int64_t n) {
+    int64_t g = (n + 255) / 256;
+    return dim3((unsigned)(g < NORM_BLOCKS ? (g < 1 ? 1 : g) : NORM_BLOCKS));
+}
+
+// residual of the finest level into W.res and its max-norm; with bnorm
+// also the max-norm of f (the rhs with boundary data)
+static int resid0(vd_ctx* ctx, MGWork& W, const LevV<double>& v0, double* nrm_out,
+                  double* bnorm = nullptr) {
+    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(2), 0, ctx->stream, W.dnorm);
     int64_t nc = ncell(W.n);
-    dim3 g = grid1d(nc, 256), b(256);
+    dim3 g = norm_grid(nc), b(256);
+    if (bnorm) hipLaunchKernelGGL(k_maxabs, g, b, 0, ctx->stream, W.f, nc, W.dnorm + 1);
     kt_begin(ctx, VD_KT_RESID);
     if (W.constb) {
         if (W.dim == 3) hipLaunchKernelGGL((k_resid0<true, 3>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, W.dnorm);
@@ -758,9 +794,10 @@ static int resid0(vd_ctx* ctx, MGWork& W, const LevV<double>& v0, double* nrm_ou
     }
     kt_end(ctx, VD_KT_RESID, 8.0 * (double)nc * (4 + (W.constb ? 0 : W.dim)));
     VD_CHECK_LAUNCH();
-    VD_HIP(hipMemcpyAsync(ctx->pinned, W.dnorm, 8, hipMemcpyDeviceToHost, ctx->stream));
+    VD_HIP(hipMemcpyAsync(ctx->pinned, W.dnorm, 16, hipMemcpyDeviceToHost, ctx->stream));
     VD_HIP(hipStreamSynchronize(ctx->stream));
     *nrm_out = ctx->pinned[0];
+    if (bnorm) *bnorm = ctx->pinned[1];
     return VD_OK;
 }
 
@@ -797,9 +834,9 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, double alpha, double beta, double 
         for (int s = 0; s < 2; s++) fine.c0[d][s] = W.c00[d][s];
     }
     LevV<double> v0 = view<double>(W, fine, alpha, beta, bconst);
-    double r0, rn;
-    VD_TRY(resid0(ctx, W, v0, &r0));
-    double target = std::max(abstol, reltol * r0);
+    double r0, rn, bnorm;
+    VD_TRY(resid0(ctx, W, v0, &r0, &bnorm));
+    double target = std::max(abstol, reltol * std::max(bnorm, r0));
     rn = r0;
     int it = 0, st = VD_OK;
     if (!(r0 <= target)) {
