@@ -1,19 +1,29 @@
 """Benchmark: cell-steps/s of the Vidyut per-timestep hot path on one MI355X.
 
-Workload (default): 3-D 256^3 single-level discharge, FP64, 10 species /
-30 seeded reactions (air-like), WENO-Z transport of electrons, electron
-energy and 4 ions with implicit multigrid diffusion, chemistry, Joule
-heating, the potential multigrid solve and the CFL reduction every step.
-One "step" = Vidyut::Evolve's step loop body (Source/Evolve.cpp:33-369).
+Headline workload (the `value` of the JSON line): 3-D 256^3 single-level
+discharge, FP64, 10 species / 30 seeded reactions (air-like), WENO-Z
+transport of electrons, electron energy and 4 ions with implicit
+multigrid diffusion, chemistry, Joule heating, the potential multigrid
+solve and the CFL reduction every step.  One "step" = the body of
+Vidyut::Evolve's step loop (Source/Evolve.cpp:33-369).
 
-Prints ONE JSON line (rank 0).  --gpus above 1 is refused: this build runs
-on one GPU.
+The same run also measures (under "configs"): the 2-D 128^2 discharge for
+100 steps against the CPU oracle, the headline case with FP32 V-cycles in
+FP64 iterative refinement, the 512^3 dielectric Poisson solve FP64 vs
+mixed precision, one 40-species / 200-reaction source evaluation at 256^3
+(FP64 FLOP/s) and the serving loop (p50 / p99 time-to-solution).
+
+Prints ONE JSON line.  --gpus above 1 is refused: this build runs on one
+GPU.
 """
 import argparse
 import json
+import math
 import os
 import sys
 import time
+
+import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -21,28 +31,180 @@ sys.path.insert(0, ROOT)
 # Canonical algorithmic bytes per launch (FP64 = 8 B per value, N = cells
 # of the level the launch runs on), accumulated by the library per launch:
 #  gsrb      one colour of a red-black sweep on the finest level: a full
-#            sweep reads phi, rhs and n_b face arrays and writes phi,
+#            sweep reads phi, rhs and the coefficient data and writes phi,
 #            8*N*(3 + n_b); one colour is half (n_b = 0 for the constant-
-#            coefficient potential solve, 3 for the implicit species solves)
+#            coefficient potential solve, 1 for the cell coefficient of the
+#            implicit species solves, 3 for face arrays)
 #  transport flux + divergence of one species: 8*N*(NS + 6)
 #  chem      one source evaluation: 8*N*(2*NS + 3)
 #  resid     finest residual + norm: 8*N*(4 + n_b)
+HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6
 
 
-def cpu_baseline(case_fn, sample_n, sample_steps):
-    """The CPU oracle (OpenMP on this host's cores) on a bounded sample of
-    the same workload: `sample_steps` steps of the same physics at
-    sample_n^3 cells."""
+def log(*a):
+    print("bench:", *a, flush=True)
+
+
+def roofline_from_timing(dev, kernel):
+    ms_total, launches, kbytes = dev.kernel_timing()
+    if launches == 0:
+        return None
+    avg_s = ms_total / 1e3 / launches
+    byts = kbytes / launches
+    ach = byts / avg_s / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
+            "launches": launches, "avg_launch_us": round(avg_s * 1e6, 2),
+            "algorithmic_bytes_per_launch": round(byts)}
+
+
+def run_steps(native, abi, dev, case, steps, warmup, family=None):
+    sim = native.Sim(dev, case)
+    sim.evolve(warmup)
+    dev.sync()
+    if family is not None:
+        dev.set_kernel_timing(family)
+    it0 = sim.mg_iterations
+    t0 = time.perf_counter()
+    sim.evolve(steps)
+    dev.sync()
+    wall = time.perf_counter() - t0
+    roof = None
+    if family is not None:
+        roof = dev.kernel_timing()
+        dev.set_kernel_timing(abi.KT_NONE)
+    st = sim.get_state()
+    iters = sim.mg_iterations - it0
+    sim.close()
+    return wall, st, iters, roof
+
+
+def cpu_baseline_steps(case_fn, sample_n, sample_steps):
     from oracle import oracle as O
     c = case_fn(sample_n)
     t0 = time.perf_counter()
     O.evolve(c, sample_steps)
     dt = time.perf_counter() - t0
-    cells = c.ncells
-    return {"value": cells * sample_steps / dt, "unit": "cell-steps/s",
+    return {"value": c.ncells * sample_steps / dt, "unit": "cell-steps/s",
             "cores": int(O.lib().or_num_threads()), "kind": "port",
             "sample": "oracle (C, OpenMP) %d step(s) of the same case at %d^3 cells, %.1f s"
                       % (sample_steps, sample_n, dt)}
+
+
+# ------------------------------------------------------------ sub-benchmarks
+def bench_discharge_2d(native, abi, cases, compare, dev, with_cpu):
+    """Config 0: 2-D 128^2, 100 steps of dt = 1e-11 s, FP64, against the
+    oracle run on this host (the CPU reference path)."""
+    c = cases.discharge_2d(n=128, nsteps=100)
+    wall, st, iters, _ = run_steps(native, abi, dev, c, 100, 0)
+    out = {"cells": c.ncells, "steps": 100, "gpu_s": round(wall, 4),
+           "cell_steps_per_s": c.ncells * 100 / wall, "mg_iterations": iters}
+    if with_cpu:
+        from oracle import oracle as O
+        t0 = time.perf_counter()
+        ref, _ = O.evolve(c, 100)
+        tcpu = time.perf_counter() - t0
+        errs = compare.state_errors(ref, st, c.mech.nspec, c.geom.dim)
+        names = [c.mech.names[s].value.decode() for s in range(c.mech.nspec)] + abi.PLASMA_VAR_NAMES
+        worst = max(errs, key=errs.get)
+        rms = {names[q]: compare.field_errors(ref[q], st[q])["rel_rms_over_meanabs"]
+               for q in range(ref.shape[0]) if np.max(np.abs(ref[q])) > 0}
+        out.update({"cpu_s": round(tcpu, 3), "cpu_cell_steps_per_s": c.ncells * 100 / tcpu,
+                    "cpu_cores": int(O.lib().or_num_threads()),
+                    "max_rel_l2_vs_oracle": errs[worst], "worst_field": names[worst],
+                    "max_rms_over_meanabs_vs_oracle": max(rms.values())})
+    return out
+
+
+def bench_dielectric(native, abi, cases, compare, dev, n):
+    """Config 2: n^3 variable-permittivity Poisson, FP64 multigrid vs FP32
+    V-cycles in FP64 iterative refinement, both to 1e-10 relative residual
+    from a zero initial guess."""
+    import ctypes as C
+    g, faces, rhs = cases.dielectric_poisson(n=n)
+    grid = native.Grid(dev, g)
+    df = [grid.array(1, 0, d, host=np.ascontiguousarray(faces[d][None])) for d in range(3)]
+    drhs = grid.array(1, host=rhs[None])
+    del faces
+    views = [f.view() for f in df]
+    res = {}
+    sols = {}
+    for prec in (0, 1):
+        p = abi.vd_mlmg_problem()
+        p.alpha_a, p.beta = 0.0, 1.0
+        for d in range(3):
+            p.bface[d] = C.pointer(views[d])
+        p.bc_lo[:] = [0, 0, abi.DIRCBC]
+        p.bc_hi[:] = [0, 0, abi.DIRCBC]
+        p.reltol, p.abstol, p.maxiter = 1e-10, 0.0, 100
+        p.precision = prec
+        p.pre_smooth = p.post_smooth = 2
+        p.max_coarsening_level = 10
+        phi = grid.array(1)
+        native.mlmg_solve(grid, p, phi, drhs)  # warm-up (workspace allocation)
+        best = None
+        for _ in range(2):
+            phi = grid.array(1)
+            dev.sync()
+            t0 = time.perf_counter()
+            info = native.mlmg_solve(grid, p, phi, drhs)
+            dev.sync()
+            t = time.perf_counter() - t0
+            best = t if best is None else min(best, t)
+        sols[prec] = phi.get()
+        res["fp64" if prec == 0 else "mixed"] = {"solve_s": round(best, 4),
+                                                  "iterations": info.iterations,
+                                                  "final_rel_residual": info.resnorm / max(info.resnorm0, 1e-300)}
+    err = compare.field_errors(sols[0], sols[1])
+    res.update({"cells": n ** 3, "speedup_mixed_over_fp64": res["fp64"]["solve_s"] / res["mixed"]["solve_s"],
+                "mixed_rel_l2_vs_fp64": err["rel_l2"],
+                "mixed_rms_over_meanabs_vs_fp64": err["rel_rms_over_meanabs"]})
+    return res
+
+
+def chem_flops_per_cell(m):
+    """Canonical FP64 flop count of one cell's source evaluation: per
+    reaction 24 (rate: exp = 20 + 4) + 2*order + 2*(reactants + products),
+    +6 per electron-impact reaction, plus 2 log + 2 div per cell (20 each)."""
+    f = 80
+    for r in range(m.nreac):
+        R = m.reac[r]
+        order = int(sum(R.react_nu[q] for q in range(R.nreact)))
+        f += 24 + 2 * order + 2 * (R.nreact + R.nprod) + (6 if R.temp_is_te else 0)
+    return f
+
+
+def bench_chemistry(native, abi, cases, dev, n):
+    """Config 3: one fused 40-species / 200-reaction source evaluation
+    (species rates + electron-energy source) over n^3 cells."""
+    m = cases.co2_mechanism(seed=3)
+    blk = min(n, 128)
+    st = cases.co2_state(blk, seed=4)
+    reps = n // blk
+    st = np.ascontiguousarray(np.tile(st, (1, reps, reps, reps)))
+    g = abi.make_geom([n] * 3, [0] * 3, [1] * 3, [0] * 3)
+    grid = native.Grid(dev, g)
+    mech = native.Mech(dev, m)
+    dst = grid.array(st.shape[0], host=st)
+    del st
+    out = grid.array(m.nspec + 1)
+    prm = cases.default_params()
+    native.rxn_source(mech, prm, grid, dst, out)
+    dev.sync()
+    dev.set_kernel_timing(abi.KT_CHEM)
+    for _ in range(5):
+        native.rxn_source(mech, prm, grid, dst, out)
+    ms, launches, kbytes = dev.kernel_timing()
+    dev.set_kernel_timing(abi.KT_NONE)
+    t = ms / 1e3 / launches
+    fpc = chem_flops_per_cell(m)
+    tfl = fpc * n ** 3 / t / 1e12
+    return {"cells": n ** 3, "nspec": m.nspec, "nreac": m.nreac, "launch_ms": round(t * 1e3, 3),
+            "cells_per_s": n ** 3 / t, "flop_per_cell": fpc,
+            "roofline": {"bound": "fp64", "achieved": round(tfl, 2), "peak": FP64_PEAK_TFLOPS,
+                         "unit": "TFLOP/s", "frac": round(tfl / FP64_PEAK_TFLOPS, 4),
+                         "hbm_GBs": round(kbytes / launches / t / 1e9, 1)}}
 
 
 def main():
@@ -51,69 +213,75 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--precision", type=int, default=0, help="0 FP64 multigrid, 1 FP32 V-cycles in FP64 IR")
+    ap.add_argument("--precision", type=int, default=0, help="headline: 0 FP64 MG, 1 FP32 V-cycles in FP64 IR")
     ap.add_argument("--kernel", default="gsrb", choices=["gsrb", "transport", "chem", "resid", "none"])
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--headline-only", action="store_true", help="skip the other configurations")
+    ap.add_argument("--poisson-n", type=int, default=512)
+    ap.add_argument("--chem-n", type=int, default=256)
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus,
               file=sys.stderr)
         sys.exit(2)
 
-    from vidyut3d_amd import abi, cases, native
+    from vidyut3d_amd import abi, cases, compare, native
 
-    def case_fn(n):
+    def case_fn(n, precision=args.precision):
         c = cases.air_3d(n=n)
-        c.params.mg_precision = args.precision
+        c.params.mg_precision = precision
         return c
 
-    t_setup = time.perf_counter()
-    case = case_fn(args.n)
     dev = native.Device(0)
-    sim = native.Sim(dev, case)
-    print("bench: setup %.1f s, %d^3 cells, warmup %d steps" % (time.perf_counter() - t_setup,
-                                                               args.n, args.warmup), flush=True)
-    sim.evolve(args.warmup)
-    dev.sync()
     fam = {"gsrb": abi.KT_GSRB, "transport": abi.KT_TRANSPORT, "chem": abi.KT_CHEM,
-           "resid": abi.KT_RESID, "none": abi.KT_NONE}[args.kernel]
-    dev.set_kernel_timing(fam)
-    it0 = sim.mg_iterations
-    dev.sync()
-    t0 = time.perf_counter()
-    sim.evolve(args.steps)
-    dev.sync()
-    t1 = time.perf_counter()
-    ms_total, launches, kbytes = dev.kernel_timing()
-    dev.set_kernel_timing(abi.KT_NONE)
-    mg_iters = sim.mg_iterations - it0
-    wall = t1 - t0
-    ncells = case.ncells
-    value = ncells * args.steps / wall
-    st = sim.get_state()
-    import numpy as np
+           "resid": abi.KT_RESID, "none": None}[args.kernel]
+    case = case_fn(args.n)
+    log("headline: %d^3 air-like discharge, %d warmup + %d timed steps" % (args.n, args.warmup, args.steps))
+    wall, st, iters, kt = run_steps(native, abi, dev, case, args.steps, args.warmup, fam)
     finite = bool(np.isfinite(st).all())
-    print("bench: %d steps in %.3f s, %d multigrid iterations, state finite: %s"
-          % (args.steps, wall, mg_iters, finite), flush=True)
-
+    del st
+    value = case.ncells * args.steps / wall
+    log("%d steps in %.3f s (%.2f ms/step), %d multigrid iterations, finite %s"
+        % (args.steps, wall, wall / args.steps * 1e3, iters, finite))
     roof = None
-    if fam != abi.KT_NONE and launches > 0:
+    if kt is not None and kt[1] > 0:
+        ms_total, launches, kbytes = kt
         avg_s = ms_total / 1e3 / launches
         byts = kbytes / launches
         ach = byts / avg_s / 1e9
-        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": 8000.0, "unit": "GB/s",
-                "frac": round(ach / 8000.0, 4), "traffic": None,
-                "kernel": args.kernel, "launches": launches,
-                "avg_launch_us": round(avg_s * 1e6, 2),
+        roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": args.kernel,
+                "launches": launches, "avg_launch_us": round(avg_s * 1e6, 2),
                 "algorithmic_bytes_per_launch": round(byts)}
+
+    configs = {}
+    if not args.headline_only:
+        log("headline case with FP32 V-cycles in FP64 iterative refinement")
+        cmp_case = case_fn(args.n, precision=1)
+        wall_mp, st_mp, iters_mp, _ = run_steps(native, abi, dev, cmp_case, args.steps, args.warmup)
+        del st_mp
+        configs["air_3d_mixed_precision"] = {
+            "cell_steps_per_s": case.ncells * args.steps / wall_mp,
+            "ms_per_step": wall_mp / args.steps * 1e3, "mg_iterations": iters_mp,
+            "speedup_over_fp64": wall / wall_mp}
+        log("2-D 128^2 discharge, 100 steps (GPU and oracle)")
+        configs["discharge_2d_128"] = bench_discharge_2d(native, abi, cases, compare, dev, not args.no_cpu)
+        log("%d^3 dielectric Poisson FP64 vs mixed" % args.poisson_n)
+        configs["dielectric_poisson"] = bench_dielectric(native, abi, cases, compare, dev, args.poisson_n)
+        log("%d^3 40-species chemistry" % args.chem_n)
+        configs["chemistry_40sp_200rx"] = bench_chemistry(native, abi, cases, dev, args.chem_n)
+        if args.serve_seconds > 0:
+            from vidyut3d_amd import serve
+            log("serving loop for %.0f s of arrivals" % args.serve_seconds)
+            configs["serving"] = serve.bench(dev_index=0, duration_s=args.serve_seconds)
 
     cpu = None
     if not args.no_cpu:
-        print("bench: cpu baseline (oracle) %d step(s) at %d^3 ..." % (args.cpu_steps, args.cpu_n),
-              flush=True)
-        cpu = cpu_baseline(case_fn, args.cpu_n, args.cpu_steps)
+        log("cpu baseline (oracle) %d step(s) at %d^3" % (args.cpu_steps, args.cpu_n))
+        cpu = cpu_baseline_steps(lambda n: case_fn(n, 0), args.cpu_n, args.cpu_steps)
 
     out = {
         "metric": "cell-steps/s FP64 vs mixed-precision+batched; p50/p99 time-to-solution",
@@ -122,15 +290,13 @@ def main():
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f64" if args.precision == 0 else "f64+f32mg", "data": "synthetic",
         "config": {"workload": "3-D %d^3 single-level discharge, 10 species / 30 seeded reactions, "
-                               "WENO-Z + implicit MG diffusion + chemistry + Poisson, FP64"
-                               % args.n,
-                   "cells": ncells, "nspec": case.mech.nspec, "nreac": case.mech.nreac,
+                               "WENO-Z + implicit MG diffusion + chemistry + Poisson, FP64" % args.n,
+                   "cells": case.ncells, "nspec": case.mech.nspec, "nreac": case.mech.nreac,
                    "dt": case.dt, "mg_precision": args.precision,
-                   "mg_iterations": mg_iters, "state_finite": finite},
-        "roofline": roof, "cpu_baseline": cpu,
+                   "mg_iterations": iters, "state_finite": finite},
+        "roofline": roof, "cpu_baseline": cpu, "configs": configs,
     }
     print(json.dumps(out))
-    sim.close()
     dev.close()
 
 

@@ -198,6 +198,10 @@ const char* vd_version(void);
 int vd_ctx_create(int device, vd_ctx** out);
 int vd_ctx_destroy(vd_ctx* ctx);
 int vd_ctx_sync(vd_ctx* ctx);
+/* frees the cached solver workspaces and scratch buffers (they are
+ * re-created on demand); synchronises.  For long-lived contexts that
+ * serve cases of many grid sizes. */
+int vd_ctx_release_workspaces(vd_ctx* ctx);
 void* vd_ctx_stream(vd_ctx* ctx); /* the hipStream_t, as void* */
 
 int vd_malloc(vd_ctx* ctx, size_t bytes, void** out);

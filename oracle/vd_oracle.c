@@ -614,6 +614,8 @@ int or_error_est(const vd_array4* st, vd_tagarray4* tags, const vd_box* box, int
  *              (i+j+k)&1, red first), cell-average restriction, linear
  *              (tensor 3/4-1/4) prolongation, face coefficients averaged to
  *              coarse faces, CG bottom solve on the coarsest level
+ *   shortcut   when rho = sum_d 2|beta/dx_d^2| max|b_d| / |alpha| <= 1e-2
+ *              an iteration is one finest-level red-black sweep on phi
  *   stop       ||r||_inf <= max(abstol, reltol*max(||b||_inf, ||r0||_inf)),
  *              b = rhs with the boundary data folded in (MLMG::solve's
  *              bnorm / resid0 choice)
@@ -1081,12 +1083,29 @@ int or_mlmg_solve(const vd_mlmg_problem* P, vd_array4* phi, const vd_array4* rhs
         if (a > bnorm) bnorm = a;
     }
     double target = fmax(P->abstol, P->reltol * fmax(bnorm, r0));
+    /* Jacobi bound rho = sum_d 2|beta/dx_d^2| max|b_d| / |alpha|: at
+     * rho <= 1e-2 each red-black sweep contracts the error by rho and the
+     * solve uses finest-level sweeps only */
+    int smooth_only = 0;
+    if (P->alpha_a != 0.0) {
+        double rho = 0.0;
+        for (int d = 0; d < g->dim; d++) {
+            double bm = 0.0;
+            for (int64_t q = 0; q < nface(L, d); q++) bm = fmax(bm, fabs(L->bf[d][q]));
+            rho += 2.0 * fabs(P->beta / (L->dx[d] * L->dx[d])) * bm;
+        }
+        smooth_only = rho / fabs(P->alpha_a) <= 1e-2;
+    }
     double rn = r0;
     int it = 0, st = VD_OK;
     if (!(r0 <= target)) {
         for (it = 1; it <= P->maxiter; it++) {
-            vcycle(M, 0, P->pre_smooth, P->post_smooth);
-            for (int64_t c = 0; c < n; c++) x[c] += L->e[c];
+            if (smooth_only) {
+                gsrb(M, L, x, f);
+            } else {
+                vcycle(M, 0, P->pre_smooth, P->post_smooth);
+                for (int64_t c = 0; c < n; c++) x[c] += L->e[c];
+            }
             rn = residual(M, L, x, f, L->r);
             if (rn <= target) break;
         }

@@ -351,3 +351,31 @@ def test_sim_error_est_matches_oracle_tags(dev):
     got = sim.error_est(comps, thr, gthr)
     assert np.array_equal(got, ref)
     assert ref.any()
+
+
+def test_discharge_2d_full_config_matches_oracle(dev):
+    """The 2-D config at its full size: 128^2, 100 steps of 1e-11 s, the
+    reference's default solver tolerances (1e-10 relative and absolute)."""
+    c = cases.discharge_2d(n=128, nsteps=100)
+    ref, _ = O.evolve(c, 100)
+    sim = native.Sim(dev, c)
+    sim.evolve(100)
+    got = sim.get_state()
+    errs = compare.state_errors(ref, got, c.mech.nspec, c.geom.dim)
+    print({q: "%.2e" % e for q, e in errs.items()})
+    bad = {q: e for q, e in errs.items() if not e <= 1e-10}
+    assert not bad, bad
+
+
+def test_mixed_precision_step_within_1e8_of_fp64_oracle(dev):
+    c = cases.air_3d(n=24, nsteps=4)
+    c.params.mg_precision = 1
+    ref_case = cases.air_3d(n=24, nsteps=4)
+    ref, _ = O.evolve(ref_case, 4)
+    sim = native.Sim(dev, c)
+    sim.evolve(4)
+    got = sim.get_state()
+    errs = compare.state_errors(ref, got, c.mech.nspec, c.geom.dim)
+    print({q: "%.2e" % e for q, e in errs.items()})
+    bad = {q: e for q, e in errs.items() if not e <= 1e-8}
+    assert not bad, bad

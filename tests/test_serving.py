@@ -1,0 +1,55 @@
+"""Serving loop: arrivals, admission and per-case invariance."""
+import numpy as np
+import pytest
+
+from vidyut3d_amd import serve
+
+
+def test_draw_requests_sizes_and_rates():
+    reqs = serve.draw_requests(rate_hz=50.0, duration_s=20.0, seed=1, nsteps=500)
+    assert 800 < len(reqs) < 1200
+    arr = np.array([r.arrival for r in reqs])
+    assert np.all(np.diff(arr) > 0) and arr[-1] <= 20.0
+    cells = np.array([r.cells for r in reqs])
+    assert cells.min() >= 64 ** 2 and cells.max() <= 256 ** 3
+    assert all(r.n % 16 == 0 for r in reqs)
+    # log-uniform: about half of the cases below the geometric mean size
+    gm = np.sqrt(64.0 ** 2 * 256.0 ** 3)
+    frac = np.mean(cells < gm)
+    assert 0.35 < frac < 0.65
+    v = np.array([r.voltage for r in reqs])
+    assert v.min() >= 500.0 and v.max() <= 1e4
+
+
+def test_footprint_grows_with_size():
+    a = serve.footprint_bytes(2, 64, 3)
+    b = serve.footprint_bytes(3, 64, 10)
+    c = serve.footprint_bytes(3, 256, 10)
+    assert a < b < c
+    assert 8e9 < c < 40e9
+
+
+@pytest.mark.gpu
+def test_results_are_the_same_alone_and_in_the_mix():
+    reqs = [serve.Request(0, 0.00, 2, 64, 800.0, 30, 11), serve.Request(1, 0.01, 2, 96, 3000.0, 30, 12),
+            serve.Request(2, 0.02, 3, 32, 5000.0, 20, 13), serve.Request(3, 0.03, 2, 64, 9000.0, 30, 14),
+            serve.Request(4, 0.04, 3, 48, 2000.0, 20, 15)]
+    srv = serve.Server(0, slots=3, keep_state=True)
+    try:
+        srv.run(reqs)
+    finally:
+        srv.close()
+    assert all(r.error is None for r in reqs), [r.error for r in reqs]
+    # requests overlapped on the device
+    assert any(reqs[q + 1].start < reqs[q].done for q in range(len(reqs) - 1))
+    for r in reqs:
+        alone = serve.Request(r.rid, 0.0, r.dim, r.n, r.voltage, r.nsteps, r.seed)
+        s1 = serve.Server(0, slots=1, keep_state=True)
+        try:
+            s1.run([alone])
+        finally:
+            s1.close()
+        assert np.isfinite(r.state).all()
+        assert np.array_equal(alone.state, r.state), r.rid
+    out = serve.summarize(reqs)
+    assert out["p99_time_to_solution_s"] >= out["p50_time_to_solution_s"] > 0

@@ -113,7 +113,7 @@ int vd_ctx_create(int device, vd_ctx** out) {
         set_error("hipStreamCreate: %s", hipGetErrorString(e));
         return VD_EHIP;
     }
-    e = hipHostMalloc((void**)&c->pinned, 64 * sizeof(double), hipHostMallocDefault);
+    e = hipHostMalloc((void**)&c->pinned, 8192 * sizeof(double), hipHostMallocDefault);
     if (e != hipSuccess) {
         (void)hipStreamDestroy(c->stream);
         delete c;
@@ -140,6 +140,16 @@ int vd_ctx_destroy(vd_ctx* ctx) {
 
 int vd_ctx_sync(vd_ctx* ctx) {
     VD_HIP(hipStreamSynchronize(ctx->stream));
+    return VD_OK;
+}
+
+int vd_ctx_release_workspaces(vd_ctx* ctx) {
+    VD_HIP(hipStreamSynchronize(ctx->stream));
+    for (auto& kv : ctx->scratch)
+        if (kv.second.p) VD_HIP(hipFree(kv.second.p));
+    ctx->scratch.clear();
+    for (auto& kv : ctx->mg) mgwork_free(kv.second);
+    ctx->mg.clear();
     return VD_OK;
 }
 

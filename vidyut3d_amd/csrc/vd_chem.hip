@@ -187,7 +187,7 @@ struct JouleArgs {
 // Vidyut::compute_elecenergy_source (Source/ElecEnergySource.cpp:65-171)
 __global__ void __launch_bounds__(256) k_joule(JouleArgs a) {
     const DevMech& M = *a.M;
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     int64_t tot = (int64_t)a.nx * a.ny * a.nz;
     if (t >= tot) return;
     int i = (int)(t % a.nx);

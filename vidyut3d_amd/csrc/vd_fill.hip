@@ -6,19 +6,19 @@
 namespace vd {
 
 __global__ void k_fill_value(double* __restrict__ p, int64_t n, double v) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = (int64_t)lblock() * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = v;
 }
 
 __global__ void k_fill_value_f(float* __restrict__ p, int64_t n, float v) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = (int64_t)lblock() * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = v;
 }
 
 __global__ void k_fill_bytes(unsigned char* __restrict__ p, int64_t n) {
-    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t i = (int64_t)lblock() * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = 0;
 }
@@ -51,7 +51,7 @@ int fill_value_f(vd_ctx* ctx, float* p, int64_t n, float v) {
 __global__ void k_fill_patch(Arr<double> src, Arr<double> dst, int dim, int3 lo, int3 n,
                              int3 per, int nx, int ny, int nz, int scomp) {
     int q = blockIdx.y + scomp;
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     int64_t tot = (int64_t)nx * ny * nz;
     if (t >= tot) return;
     int i = (int)(t % nx);

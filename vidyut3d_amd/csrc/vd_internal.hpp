@@ -168,4 +168,18 @@ void kt_end(vd_ctx* ctx, int family, double bytes);
 
 inline dim3 grid1d(int64_t n, int block) { return dim3((unsigned)((n + block - 1) / block)); }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2): hardware block b runs on XCD b % 8.  Stencil kernels read the rows
+// next to their own, so give every XCD a contiguous range of logical
+// blocks: neighbouring rows then share an L2 instead of being fetched from
+// HBM by up to three XCDs.
+constexpr unsigned NUM_XCD = 8;
+__device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
+    const unsigned q = nb / NUM_XCD, r = nb % NUM_XCD;
+    const unsigned x = b % NUM_XCD, idx = b / NUM_XCD;
+    return x * q + (x < r ? x : r) + idx;
+}
+// logical (remapped) block index of a 1-D grid
+__device__ __forceinline__ unsigned lblock() { return xcd_remap(blockIdx.x, gridDim.x); }
+
 }  // namespace vd

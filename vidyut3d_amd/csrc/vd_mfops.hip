@@ -20,7 +20,7 @@ static BoxIter box_iter(const vd_box* b, const vd_array4* a) {
 }
 
 __global__ void k_setval(Arr<double> a, BoxIter it, int scomp, double v) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)it.nx * it.ny * it.nz) return;
     int i = (int)(t % it.nx);
     int64_t q = t / it.nx;
@@ -31,7 +31,7 @@ __global__ void k_setval(Arr<double> a, BoxIter it, int scomp, double v) {
 // dst = a*x + b*y  (b == 0 and y == x gives a scaled copy)
 __global__ void k_lincomb(Arr<double> dst, Arr<double> x, Arr<double> y, BoxIter it, int xc, int yc,
                           int dc, double a, double b, int copy) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)it.nx * it.ny * it.nz) return;
     int i = (int)(t % it.nx);
     int64_t q = t / it.nx;
@@ -88,7 +88,7 @@ extern "C" int vd_lincomb(vd_ctx* ctx, vd_array4* dst, double a, const vd_array4
 
 namespace vd {
 __global__ void k_clamp_min(Arr<double> a, BoxIter it, int comp, double lo) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)it.nx * it.ny * it.nz) return;
     int i = (int)(t % it.nx);
     int64_t q = t / it.nx;

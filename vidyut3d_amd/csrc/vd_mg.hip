@@ -38,6 +38,7 @@ struct LevV {  // kernel view of one level
     const T* bf0;
     const T* bf1;
     const T* bf2;
+    const T* dcell;      // BM 2: cell coefficient on the grown box
     const T* c0[3][2];   // per boundary face c0 (non-periodic sides)
     int per0, per1, per2;
 };
@@ -69,7 +70,11 @@ struct MGWork {
     double* c00[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     double* bcv[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     double* rob[3][2][3];  // robin a, b, f per side
-    unsigned long long* dnorm = nullptr;
+    unsigned long long* dnorm = nullptr;  // 4 groups of NSLOT norm slots
+    double* dcell = nullptr;              // cell coefficient on the grown box
+    double* part = nullptr;               // per-block partial norms
+    int c0_types[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};
+    bool c0_valid = false;                // coarse c0 match c0_types
     std::vector<Lev<double>> L64;
     std::vector<Lev<float>> L32;
 };
@@ -99,6 +104,9 @@ static inline int64_t ntanf(const int n[3], int d) {
 }
 
 // ------------------------------------------------------------ stencil
+// BM (coefficient mode): 0 face arrays, 1 constant b, 2 harmonic face
+// value of a cell coefficient on the grown box (finest level of the
+// implicit species solves: 8 B/cell read instead of 24 B/cell of faces)
 template <class T>
 __device__ __forceinline__ int fidx(const LevV<T>& L, int d, int i, int j, int k) {
     int n0 = L.nx + (d == 0), n1 = L.ny + (d == 1);
@@ -109,10 +117,22 @@ __device__ __forceinline__ int tidx(const LevV<T>& L, int d, int i, int j, int k
     return d == 0 ? j + L.ny * k : (d == 1 ? i + L.nx * k : i + L.nx * j);
 }
 
-template <class T, bool CB, int DIM>
+// harmonic face average, the formula of the face-array setup kernel
+template <class T>
+__device__ __forceinline__ T harm(T a, T b) {
+    return (a + b == T(0)) ? T(0) : T(2) * a * b / (a + b);
+}
+
+template <class T, int BM, int DIM>
 __device__ __forceinline__ void stencil(const LevV<T>& L, const T* __restrict__ x, int i, int j,
                                         int k, T& dgo, T& ofo) {
     const int c = i + L.nx * (j + L.ny * k);
+    int gc = 0, gx = 0, gxy = 0;
+    if (BM == 2) {
+        gx = L.nx + 2;
+        gxy = gx * (L.ny + 2);
+        gc = (i + 1) + gx * (j + 1) + (DIM == 3 ? gxy * (k + 1) : 0);
+    }
     T dg = L.alpha, of = T(0);
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
@@ -122,14 +142,19 @@ __device__ __forceinline__ void stencil(const LevV<T>& L, const T* __restrict__ 
         const int per = d == 0 ? L.per0 : (d == 1 ? L.per1 : L.per2);
         const T s = d == 0 ? L.s0 : (d == 1 ? L.s1 : L.s2);
         T blo, bhi;
-        if (CB) {
+        if (BM == 1) {
             blo = bhi = L.bconst;
-        } else {
+        } else if (BM == 0) {
             const T* bf = d == 0 ? L.bf0 : (d == 1 ? L.bf1 : L.bf2);
             const int f = fidx(L, d, i, j, k);
             const int fs = d == 0 ? 1 : (d == 1 ? (L.nx) : (L.nx * L.ny));
             blo = bf[f];
             bhi = bf[f + fs];
+        } else {
+            const int gs = d == 0 ? 1 : (d == 1 ? gx : gxy);
+            const T dc = L.dcell[gc];
+            blo = harm(L.dcell[gc - gs], dc);
+            bhi = harm(dc, L.dcell[gc + gs]);
         }
         T dd = T(0), oo = T(0);
         if (cd > 0) {
@@ -158,20 +183,20 @@ __device__ __forceinline__ void stencil(const LevV<T>& L, const T* __restrict__ 
 }
 
 // ------------------------------------------------------------ kernels
-template <class T, bool CB, int DIM>
-__global__ void __launch_bounds__(256) k_gsrb(LevV<T> L, T* __restrict__ x, const T* __restrict__ rhs,
-                                              int color) {
-    const int nh = (L.nx + 1) >> 1;
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)nh * L.ny * L.nz) return;
-    int ii = (int)(t % nh);
-    int row = (int)(t / nh);
-    int j = row % L.ny, k = row / L.ny;
-    int i = 2 * ii + ((color + j + k) & 1);
+// one colour of a red-black sweep: 1-D grid of gx blocks per row, rows
+// ordered (j, k) and XCD-remapped
+template <class T, int BM, int DIM>
+__global__ void __launch_bounds__(128) k_gsrb(LevV<T> L, T* __restrict__ x, const T* __restrict__ rhs,
+                                              int color, int gx) {
+    const unsigned lb = lblock();
+    const unsigned row = lb / gx;
+    const int j = (int)(row % L.ny), k = (int)(row / L.ny);
+    const int ii = (int)(lb % gx) * blockDim.x + threadIdx.x;
+    const int i = 2 * ii + ((color + j + k) & 1);
     if (i >= L.nx) return;
     T dg, of;
-    stencil<T, CB, DIM>(L, x, i, j, k, dg, of);
-    int c = i + L.nx * row;
+    stencil<T, BM, DIM>(L, x, i, j, k, dg, of);
+    const int c = i + L.nx * (j + L.ny * k);
     x[c] = (rhs[c] + of) / dg;
 }
 
@@ -180,66 +205,140 @@ __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double 
     atomicMax(p, (unsigned long long)__double_as_longlong(v));
 }
 
-// block max (256 threads: wave64 shuffles, then LDS) and one atomic per block
-__device__ __forceinline__ void block_max_atomic(double a, unsigned long long* nrm) {
-    __shared__ double red[4];
+// norm accumulators: NSLOT slots, one per 128-byte line, blocks spread over
+// them; the host takes the max of the slots
+constexpr int NSLOT = 64;
+constexpr int SLOT_STRIDE = 16;
+constexpr int NORM_BLOCKS = 2048;
+
+// block max (wave64 shuffles, then LDS), one atomic per block
+__device__ __forceinline__ void block_max_atomic(double a, unsigned long long* slots) {
+    __shared__ double red[16];
     for (int o = 32; o > 0; o >>= 1) a = fmax(a, __shfl_xor(a, o, 64));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
     __syncthreads();
     if (threadIdx.x == 0) {
         double m = red[0];
         for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, red[w]);
-        atomic_max_nonneg(nrm, m);
+        atomic_max_nonneg(slots + (blockIdx.x % NSLOT) * SLOT_STRIDE, m);
     }
 }
 
-constexpr int NORM_BLOCKS = 2048;  // grid-stride: 8 blocks per CU, 2048 atomics per norm
-
-// r = f - L x with the max-norm (finest level, FP64)
-template <bool CB, int DIM>
+// r = f - L x (finest level, FP64), one block per gx-chunk of a row
+// (XCD-remapped); each block writes the max |r| of its cells to part[lb]
+template <int BM, int DIM>
 __global__ void __launch_bounds__(256) k_resid0(LevV<double> L, const double* __restrict__ x,
                                                 const double* __restrict__ f, double* __restrict__ r,
-                                                unsigned long long* __restrict__ nrm) {
-    const int64_t tot = (int64_t)L.nx * L.ny * L.nz;
+                                                double* __restrict__ part, int gx) {
+    __shared__ double red[4];
+    const unsigned lb = lblock();
+    const unsigned row = lb / gx;
+    const int j = (int)(row % L.ny), k = (int)(row / L.ny);
+    const int i = (int)(lb % gx) * blockDim.x + threadIdx.x;
     double a = 0.0;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < tot;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        int i = (int)(t % L.nx);
-        int row = (int)(t / L.nx);
-        int j = row % L.ny, k = row / L.ny;
+    if (i < L.nx) {
         double dg, of;
-        stencil<double, CB, DIM>(L, x, i, j, k, dg, of);
-        double v = f[t] - (dg * x[t] - of);
-        r[t] = v;
-        double b = fabs(v);
-        if (b != b) b = __builtin_huge_val();
-        a = fmax(a, b);
+        stencil<double, BM, DIM>(L, x, i, j, k, dg, of);
+        const int c = i + L.nx * (int)row;
+        double v = f[c] - (dg * x[c] - of);
+        r[c] = v;
+        a = fabs(v);
+        if (a != a) a = __builtin_huge_val();
     }
-    block_max_atomic(a, nrm);
+    for (int o = 32; o > 0; o >>= 1) a = fmax(a, __shfl_xor(a, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, red[w]);
+        part[lb] = m;
+    }
 }
 
-__global__ void __launch_bounds__(256) k_maxabs(const double* __restrict__ f, int64_t n,
-                                                unsigned long long* __restrict__ nrm) {
+// max of n per-block partials into slot 0 of a slot group (one workgroup)
+__global__ void __launch_bounds__(1024) k_reduce_partials(const double* __restrict__ part, int n,
+                                                          unsigned long long* __restrict__ slots) {
+    __shared__ double red[16];
     double a = 0.0;
-    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-         t += (int64_t)gridDim.x * blockDim.x) {
+    for (int t = threadIdx.x; t < n; t += blockDim.x) a = fmax(a, part[t]);
+    for (int o = 32; o > 0; o >>= 1) a = fmax(a, __shfl_xor(a, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, red[w]);
+        atomic_max_nonneg(slots, m);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_maxabs(const double* __restrict__ f, unsigned n,
+                                                unsigned long long* __restrict__ slots) {
+    double a = 0.0;
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         double b = fabs(f[t]);
         if (b != b) b = __builtin_huge_val();
         a = fmax(a, b);
     }
-    block_max_atomic(a, nrm);
+    block_max_atomic(a, slots);
 }
 
-// coarse r = average of the 2^DIM fine residuals rhs - L e (fused)
-template <class T, bool CB, int DIM>
-__global__ void __launch_bounds__(256) k_resid_restrict(LevV<T> F, const T* __restrict__ e,
+// per direction, the max over all faces of the harmonic face value of a
+// cell coefficient on the grown box, in one pass over the face-index box
+// (nx+1, ny+1, nz+1): point (i,j,k) owns face (i,j,k) of every direction
+// whose face box contains it; results go to slot groups 0..DIM-1
+template <int DIM>
+__global__ void __launch_bounds__(256) k_face_max_cd(const double* __restrict__ dcell, int nx, int ny,
+                                                     int nz, unsigned long long* __restrict__ slots) {
+    __shared__ double red[3][4];
+    const int p0 = nx + 1, p1 = ny + 1, p2 = DIM == 3 ? nz + 1 : 1;
+    const unsigned n = (unsigned)p0 * p1 * p2;
+    const int gx = nx + 2, gxy = gx * (ny + 2);
+    double a[3] = {0.0, 0.0, 0.0};
+    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
+        int i = (int)(t % p0);
+        unsigned q = t / p0;
+        int j = (int)(q % p1), k = (int)(q / p1);
+        const int gc = (i + 1) + gx * (j + 1) + (DIM == 3 ? gxy * (k + 1) : 0);  // right cell
+        const int ext[3] = {nx, ny, nz};
+        const int ijk[3] = {i, j, k};
+#pragma unroll
+        for (int d = 0; d < DIM; d++) {
+            bool ok = true;
+#pragma unroll
+            for (int e = 0; e < DIM; e++)
+                if (e != d && ijk[e] >= ext[e]) ok = false;
+            if (!ok) continue;
+            const int gs = d == 0 ? 1 : (d == 1 ? gx : gxy);
+            double b = fabs(harm(dcell[gc - gs], dcell[gc]));
+            if (b != b) b = __builtin_huge_val();
+            a[d] = fmax(a[d], b);
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        double v = a[d];
+        for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+        if ((threadIdx.x & 63) == 0) red[d][threadIdx.x >> 6] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < DIM) {
+        double m = red[threadIdx.x][0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, red[threadIdx.x][w]);
+        atomic_max_nonneg(slots + (size_t)threadIdx.x * NSLOT * SLOT_STRIDE + (blockIdx.x % NSLOT) * SLOT_STRIDE, m);
+    }
+}
+
+// coarse r = average of the 2^DIM fine residuals rhs - L e (fused);
+// 1-D grid of gx blocks per coarse row, XCD-remapped
+template <class T, int BM, int DIM>
+__global__ void __launch_bounds__(128) k_resid_restrict(LevV<T> F, const T* __restrict__ e,
                                                         const T* __restrict__ rhs, int cnx, int cny,
-                                                        int cnz, T* __restrict__ rc) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)cnx * cny * cnz) return;
-    int I = (int)(t % cnx);
-    int row = (int)(t / cnx);
-    int J = row % cny, K = row / cny;
+                                                        int cnz, T* __restrict__ rc, int gx) {
+    const unsigned lb = lblock();
+    const unsigned row = lb / gx;
+    const int J = (int)(row % cny), K = (int)(row / cny);
+    const int I = (int)(lb % gx) * blockDim.x + threadIdx.x;
+    if (I >= cnx) return;
     T s = T(0);
     const int kz = DIM == 3 ? 2 : 1;
     for (int kk = 0; kk < kz; kk++)
@@ -247,23 +346,23 @@ __global__ void __launch_bounds__(256) k_resid_restrict(LevV<T> F, const T* __re
             for (int ii = 0; ii < 2; ii++) {
                 int i = 2 * I + ii, j = 2 * J + jj, k = DIM == 3 ? 2 * K + kk : 0;
                 T dg, of;
-                stencil<T, CB, DIM>(F, e, i, j, k, dg, of);
+                stencil<T, BM, DIM>(F, e, i, j, k, dg, of);
                 int c = i + F.nx * (j + F.ny * k);
                 s += rhs[c] - (dg * e[c] - of);
             }
-    rc[t] = s * (DIM == 3 ? T(0.125) : T(0.25));
+    rc[I + cnx * (J + cny * K)] = s * (DIM == 3 ? T(0.125) : T(0.25));
 }
 
 // e_f += P e_c (linear, tensor 3/4-1/4; ghosts of the coarse correction
-// are c0 * interior, periodic wraps)
+// are c0 * interior, periodic wraps); 1-D grid of gx blocks per fine row
 template <class T, int DIM>
-__global__ void __launch_bounds__(256) k_prolong(LevV<T> C, const T* __restrict__ ec, int fnx, int fny,
-                                                 int fnz, T* __restrict__ ef) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)fnx * fny * fnz) return;
-    int i = (int)(t % fnx);
-    int row = (int)(t / fnx);
-    int j = row % fny, k = row / fny;
+__global__ void __launch_bounds__(128) k_prolong(LevV<T> C, const T* __restrict__ ec, int fnx, int fny,
+                                                 int fnz, T* __restrict__ ef, int gx) {
+    const unsigned lb = lblock();
+    const unsigned row = lb / gx;
+    const int j = (int)(row % fny), k = (int)(row / fny);
+    const int i = (int)(lb % gx) * blockDim.x + threadIdx.x;
+    if (i >= fnx) return;
     int f[3] = {i, j, k};
     int cn[3] = {C.nx, C.ny, C.nz};
     int pr[3] = {C.per0, C.per1, C.per2};
@@ -307,11 +406,11 @@ __global__ void __launch_bounds__(256) k_prolong(LevV<T> C, const T* __restrict_
                       (DIM == 3 ? (kz == 0 ? T(0.75) : T(0.25)) : T(1));
                 val += w * v;
             }
-    ef[t] += val;
+    ef[i + fnx * (j + fny * k)] += val;
 }
 
 // CG bottom solve, one workgroup; dots accumulated in double
-template <class T, bool CB, int DIM>
+template <class T, int BM, int DIM>
 __global__ void __launch_bounds__(1024) k_bottom_cg(LevV<T> L, T* __restrict__ e, const T* __restrict__ b,
                                                     T* __restrict__ r, T* __restrict__ p,
                                                     T* __restrict__ q) {
@@ -348,7 +447,7 @@ __global__ void __launch_bounds__(1024) k_bottom_cg(LevV<T> L, T* __restrict__ e
             int i = c % L.nx, row = c / L.nx;
             int j = row % L.ny, k = row / L.ny;
             T dg, of;
-            stencil<T, CB, DIM>(L, p, i, j, k, dg, of);
+            stencil<T, BM, DIM>(L, p, i, j, k, dg, of);
             q[c] = dg * p[c] - of;
         }
         __syncthreads();
@@ -373,7 +472,7 @@ template <class T>
 __global__ void k_coarsen_faces(const T* __restrict__ fb, int fnx, int fny, int d, int dim, int cnx,
                                 int cny, int cnz, T* __restrict__ cb) {
     int n0 = cnx + (d == 0), n1 = cny + (d == 1), n2 = cnz + (d == 2);
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)n0 * n1 * n2) return;
     int i = (int)(t % n0);
     int row = (int)(t / n0);
@@ -399,7 +498,7 @@ __global__ void k_coarsen_c0(const T* __restrict__ fc, int fnx, int fny, int d, 
     int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
     int cn[3] = {cnx, cny, cnz}, fn[3] = {fnx, fny, 0};
     int nt1 = cn[t1], nt2 = dim == 3 ? cn[t2] : 1;
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nt1 * nt2) return;
     int b1 = (int)(t % nt1), b2 = (int)(t / nt1);
     T s = T(0);
@@ -421,23 +520,22 @@ __global__ void k_coarsen_c0(const T* __restrict__ fc, int fnx, int fny, int d, 
 
 template <class A, class B>
 __global__ void k_convert(const A* __restrict__ a, B* __restrict__ b, int64_t n) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t < n) b[t] = (B)a[t];
 }
 
 template <class T>
 __global__ void k_zero(T* __restrict__ p, int64_t n) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t < n) p[t] = T(0);
 }
 
 template <class T>
 __global__ void k_axpy_to_double(double* __restrict__ x, const T* __restrict__ e, int64_t n) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t < n) x[t] += (double)e[t];
 }
 
-__global__ void k_zero_u64(unsigned long long* p) { p[threadIdx.x] = 0ull; }
 
 // BC coefficient for one face (ghost = c0*x + cinh)
 __device__ __host__ inline void bc_coef(int type, int side, double dx, double val, double ra,
@@ -466,7 +564,8 @@ struct SideArgs {
     const double* ra;
     const double* rb;
     const double* rf;
-    const double* bf;  // finest faces of dir d (null => bconst)
+    const double* bf;  // finest faces of dir d (null => dcell or bconst)
+    const double* dcell;  // cell coefficient on the grown box (harmonic face)
     double bconst;
     double* c0;
     double* f;         // rhs to shift
@@ -477,7 +576,7 @@ __global__ void k_side_setup(SideArgs a) {
     int t1 = a.d == 0 ? 1 : 0, t2 = a.d == 2 ? 1 : 2;
     int n[3] = {a.nx, a.ny, a.nz};
     int nt1 = n[t1], nt2 = a.dim == 3 ? n[t2] : 1;
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nt1 * nt2) return;
     int b1 = (int)(t % nt1), b2 = (int)(t / nt1);
     int c[3] = {0, 0, 0};
@@ -498,7 +597,17 @@ __global__ void k_side_setup(SideArgs a) {
     int fc[3] = {c[0], c[1], c[2]};
     fc[a.d] += a.side;
     int n0 = a.nx + (a.d == 0), n1 = a.ny + (a.d == 1);
-    double b = a.bf ? a.bf[fc[0] + n0 * (fc[1] + n1 * fc[2])] : a.bconst;
+    double b;
+    if (a.bf) {
+        b = a.bf[fc[0] + n0 * (fc[1] + n1 * fc[2])];
+    } else if (a.dcell) {
+        const int gx = a.nx + 2, gxy = gx * (a.ny + 2);
+        const int gs = a.d == 0 ? 1 : (a.d == 1 ? gx : gxy);
+        const int gc = (fc[0] + 1) + gx * (fc[1] + 1) + (a.dim == 3 ? gxy * (fc[2] + 1) : 0);
+        b = harm(a.dcell[gc - gs], a.dcell[gc]);
+    } else {
+        b = a.bconst;
+    }
     int cell = c[0] + a.nx * (c[1] + a.ny * c[2]);
     a.f[cell] += a.s * b * cinh;
 }
@@ -520,6 +629,7 @@ static LevV<T> view(const MGWork& W, const Lev<T>& L, double alpha, double beta,
     v.bf0 = L.bf[0];
     v.bf1 = L.bf[1];
     v.bf2 = L.bf[2];
+    v.dcell = nullptr;
     for (int d = 0; d < 3; d++)
         for (int sd = 0; sd < 2; sd++) v.c0[d][sd] = L.c0[d][sd];
     v.per0 = W.per[0];
@@ -621,8 +731,15 @@ static int get_work(vd_ctx* ctx, const vd_geom* g, const vd_box* box, int prec, 
     W->f = (double*)p;
     if ((st = dalloc(W, nc * 8, &p))) goto fail;
     W->res = (double*)p;
-    if ((st = dalloc(W, 64, &p))) goto fail;
+    if ((st = dalloc(W, 4 * NSLOT * SLOT_STRIDE * sizeof(unsigned long long), &p))) goto fail;
     W->dnorm = (unsigned long long*)p;
+    if ((st = dalloc(W, ((n[0] + 255) / 256) * (int64_t)n[1] * n[2] * 8, &p))) goto fail;
+    W->part = (double*)p;
+    if (!constb) {
+        int64_t ng = (int64_t)(n[0] + 2) * (n[1] + 2) * (g->dim == 3 ? n[2] + 2 : 1);
+        if ((st = dalloc(W, ng * 8, &p))) goto fail;
+        W->dcell = (double*)p;
+    }
     for (int d = 0; d < g->dim; d++) {
         if (!constb) {
             if ((st = dalloc(W, nfacef(n, d) * 8, &p))) goto fail;
@@ -658,26 +775,56 @@ fail:
     return st;
 }
 
+template <class T, int BM>
+static void launch_gsrb_bm(vd_ctx* ctx, int dim, const LevV<T>& v, T* x, const T* rhs, int color) {
+    const int nh = (v.nx + 1) / 2;
+    const int bx = nh > 64 ? 128 : 64;
+    const int gx = (nh + bx - 1) / bx;
+    dim3 g((unsigned)gx * v.ny * v.nz), b(bx);
+    if (dim == 3) hipLaunchKernelGGL((k_gsrb<T, BM, 3>), g, b, 0, ctx->stream, v, x, rhs, color, gx);
+    else hipLaunchKernelGGL((k_gsrb<T, BM, 2>), g, b, 0, ctx->stream, v, x, rhs, color, gx);
+}
+
+// bm: 0 face arrays, 1 constant, 2 cell coefficient (harmonic on the fly)
 template <class T>
-static void launch_gsrb(vd_ctx* ctx, const MGWork& W, const LevV<T>& v, T* x, const T* rhs, int color,
-                        bool fine) {
-    int64_t nt = (int64_t)((v.nx + 1) / 2) * v.ny * v.nz;
-    dim3 g = grid1d(nt, 256), b(256);
+static void launch_gsrb(vd_ctx* ctx, const MGWork& W, int bm, const LevV<T>& v, T* x, const T* rhs,
+                        int color, bool fine) {
     if (fine) kt_begin(ctx, VD_KT_GSRB);
-    if (W.constb) {
-        if (W.dim == 3) hipLaunchKernelGGL((k_gsrb<T, true, 3>), g, b, 0, ctx->stream, v, x, rhs, color);
-        else hipLaunchKernelGGL((k_gsrb<T, true, 2>), g, b, 0, ctx->stream, v, x, rhs, color);
-    } else {
-        if (W.dim == 3) hipLaunchKernelGGL((k_gsrb<T, false, 3>), g, b, 0, ctx->stream, v, x, rhs, color);
-        else hipLaunchKernelGGL((k_gsrb<T, false, 2>), g, b, 0, ctx->stream, v, x, rhs, color);
-    }
-    // one colour = half of a sweep that reads phi, rhs, n_b faces and writes phi
-    if (fine) kt_end(ctx, VD_KT_GSRB, 8.0 * (double)v.nx * v.ny * v.nz * (3 + (W.constb ? 0 : W.dim)) / 2.0);
+    if (bm == 1) launch_gsrb_bm<T, 1>(ctx, W.dim, v, x, rhs, color);
+    else if (bm == 0) launch_gsrb_bm<T, 0>(ctx, W.dim, v, x, rhs, color);
+    else launch_gsrb_bm<T, 2>(ctx, W.dim, v, x, rhs, color);
+    // one colour = half of a sweep that reads phi, rhs and the coefficient
+    // data (none, DIM face arrays, or one cell array) and writes phi
+    const int nb = bm == 1 ? 0 : (bm == 0 ? W.dim : 1);
+    if (fine) kt_end(ctx, VD_KT_GSRB, 8.0 * (double)v.nx * v.ny * v.nz * (3 + nb) / 2.0);
+}
+
+template <class T, int BM>
+static void launch_resid_restrict(vd_ctx* ctx, int dim, const LevV<T>& v, const Lev<T>& F, Lev<T>& Cc) {
+    const int bx = Cc.n[0] > 64 ? 128 : 64;
+    const int gx = (Cc.n[0] + bx - 1) / bx;
+    dim3 g((unsigned)gx * Cc.n[1] * Cc.n[2]), b(bx);
+    if (dim == 3)
+        hipLaunchKernelGGL((k_resid_restrict<T, BM, 3>), g, b, 0, ctx->stream, v, F.e, F.r, Cc.n[0],
+                           Cc.n[1], Cc.n[2], Cc.r, gx);
+    else
+        hipLaunchKernelGGL((k_resid_restrict<T, BM, 2>), g, b, 0, ctx->stream, v, F.e, F.r, Cc.n[0],
+                           Cc.n[1], Cc.n[2], Cc.r, gx);
+}
+
+template <class T, int BM>
+static void launch_bottom(vd_ctx* ctx, int dim, const LevV<T>& v, Lev<T>& B) {
+    if (dim == 3)
+        hipLaunchKernelGGL((k_bottom_cg<T, BM, 3>), dim3(1), dim3(1024), 0, ctx->stream, v, B.e, B.r, B.w,
+                           B.p, B.q);
+    else
+        hipLaunchKernelGGL((k_bottom_cg<T, BM, 2>), dim3(1), dim3(1024), 0, ctx->stream, v, B.e, B.r, B.w,
+                           B.p, B.q);
 }
 
 template <class T>
-static void vcycle(vd_ctx* ctx, const MGWork& W, std::vector<Lev<T>>& L, double alpha, double beta,
-                   double bconst, int nu1, int nu2) {
+static void vcycle(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L, double alpha,
+                   double beta, double bconst, int nu1, int nu2) {
     const int nl = W.nlev;
     // down
     for (int l = 0; l < nl - 1; l++) {
@@ -686,48 +833,18 @@ static void vcycle(vd_ctx* ctx, const MGWork& W, std::vector<Lev<T>>& L, double 
         int64_t nc = ncell(F.n);
         hipLaunchKernelGGL(k_zero<T>, grid1d(nc, 256), dim3(256), 0, ctx->stream, F.e, nc);
         for (int s = 0; s < nu1; s++) {
-            launch_gsrb<T>(ctx, W, v, F.e, F.r, 0, l == 0);
-            launch_gsrb<T>(ctx, W, v, F.e, F.r, 1, l == 0);
+            launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 0, l == 0);
+            launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 1, l == 0);
         }
-        Lev<T>& Cc = L[l + 1];
-        int64_t ncc = ncell(Cc.n);
-        dim3 g = grid1d(ncc, 256), b(256);
-        if (W.constb) {
-            if (W.dim == 3)
-                hipLaunchKernelGGL((k_resid_restrict<T, true, 3>), g, b, 0, ctx->stream, v, F.e, F.r,
-                                   Cc.n[0], Cc.n[1], Cc.n[2], Cc.r);
-            else
-                hipLaunchKernelGGL((k_resid_restrict<T, true, 2>), g, b, 0, ctx->stream, v, F.e, F.r,
-                                   Cc.n[0], Cc.n[1], Cc.n[2], Cc.r);
-        } else {
-            if (W.dim == 3)
-                hipLaunchKernelGGL((k_resid_restrict<T, false, 3>), g, b, 0, ctx->stream, v, F.e, F.r,
-                                   Cc.n[0], Cc.n[1], Cc.n[2], Cc.r);
-            else
-                hipLaunchKernelGGL((k_resid_restrict<T, false, 2>), g, b, 0, ctx->stream, v, F.e, F.r,
-                                   Cc.n[0], Cc.n[1], Cc.n[2], Cc.r);
-        }
+        if (bm == 1) launch_resid_restrict<T, 1>(ctx, W.dim, v, F, L[l + 1]);
+        else launch_resid_restrict<T, 0>(ctx, W.dim, v, F, L[l + 1]);
     }
     // bottom
     {
         Lev<T>& B = L[nl - 1];
         LevV<T> v = view<T>(W, B, alpha, beta, bconst);
-        int threads = 1024;
-        if (W.constb) {
-            if (W.dim == 3)
-                hipLaunchKernelGGL((k_bottom_cg<T, true, 3>), dim3(1), dim3(threads), 0, ctx->stream, v,
-                                   B.e, B.r, B.w, B.p, B.q);
-            else
-                hipLaunchKernelGGL((k_bottom_cg<T, true, 2>), dim3(1), dim3(threads), 0, ctx->stream, v,
-                                   B.e, B.r, B.w, B.p, B.q);
-        } else {
-            if (W.dim == 3)
-                hipLaunchKernelGGL((k_bottom_cg<T, false, 3>), dim3(1), dim3(threads), 0, ctx->stream,
-                                   v, B.e, B.r, B.w, B.p, B.q);
-            else
-                hipLaunchKernelGGL((k_bottom_cg<T, false, 2>), dim3(1), dim3(threads), 0, ctx->stream,
-                                   v, B.e, B.r, B.w, B.p, B.q);
-        }
+        if (bm == 1) launch_bottom<T, 1>(ctx, W.dim, v, B);
+        else launch_bottom<T, 0>(ctx, W.dim, v, B);
     }
     // up
     for (int l = nl - 2; l >= 0; l--) {
@@ -735,32 +852,32 @@ static void vcycle(vd_ctx* ctx, const MGWork& W, std::vector<Lev<T>>& L, double 
         Lev<T>& Cc = L[l + 1];
         LevV<T> vc = view<T>(W, Cc, alpha, beta, bconst);
         LevV<T> v = view<T>(W, F, alpha, beta, bconst);
-        int64_t nf = ncell(F.n);
+        const int bx = F.n[0] > 64 ? 128 : 64;
+        const int gx = (F.n[0] + bx - 1) / bx;
+        dim3 g((unsigned)gx * F.n[1] * F.n[2]), b(bx);
         if (W.dim == 3)
-            hipLaunchKernelGGL((k_prolong<T, 3>), grid1d(nf, 256), dim3(256), 0, ctx->stream, vc, Cc.e,
-                               F.n[0], F.n[1], F.n[2], F.e);
+            hipLaunchKernelGGL((k_prolong<T, 3>), g, b, 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.n[2], F.e, gx);
         else
-            hipLaunchKernelGGL((k_prolong<T, 2>), grid1d(nf, 256), dim3(256), 0, ctx->stream, vc, Cc.e,
-                               F.n[0], F.n[1], F.n[2], F.e);
+            hipLaunchKernelGGL((k_prolong<T, 2>), g, b, 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.n[2], F.e, gx);
         for (int s = 0; s < nu2; s++) {
-            launch_gsrb<T>(ctx, W, v, F.e, F.r, 0, l == 0);
-            launch_gsrb<T>(ctx, W, v, F.e, F.r, 1, l == 0);
+            launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 0, l == 0);
+            launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 1, l == 0);
         }
     }
 }
 
 template <class T>
-static void coarsen_hierarchy(vd_ctx* ctx, MGWork& W, std::vector<Lev<T>>& L) {
+static void coarsen_hierarchy(vd_ctx* ctx, MGWork& W, std::vector<Lev<T>>& L, bool faces, bool c0) {
     for (int l = 1; l < W.nlev; l++) {
         Lev<T>& F = L[l - 1];
         Lev<T>& Cc = L[l];
         for (int d = 0; d < W.dim; d++) {
-            if (!W.constb) {
+            if (faces && !W.constb) {
                 int64_t nf = nfacef(Cc.n, d);
                 hipLaunchKernelGGL(k_coarsen_faces<T>, grid1d(nf, 256), dim3(256), 0, ctx->stream,
                                    F.bf[d], F.n[0], F.n[1], d, W.dim, Cc.n[0], Cc.n[1], Cc.n[2], Cc.bf[d]);
             }
-            if (!W.per[d])
+            if (c0 && !W.per[d])
                 for (int s = 0; s < 2; s++) {
                     int64_t nt = ntanf(Cc.n, d);
                     hipLaunchKernelGGL(k_coarsen_c0<T>, grid1d(nt, 256), dim3(256), 0, ctx->stream,
@@ -776,56 +893,148 @@ static inline dim3 norm_grid(int64_t n) {
     return dim3((unsigned)(g < NORM_BLOCKS ? (g < 1 ? 1 : g) : NORM_BLOCKS));
 }
 
+// slot group q of the norm buffer
+static inline unsigned long long* slot_group(MGWork& W, int q) { return W.dnorm + (size_t)q * NSLOT * SLOT_STRIDE; }
+
+static void zero_groups(vd_ctx* ctx, MGWork& W, int ngroups) {
+    int64_t n = (int64_t)ngroups * NSLOT * SLOT_STRIDE;
+    hipLaunchKernelGGL(k_zero<unsigned long long>, grid1d(n, 256), dim3(256), 0, ctx->stream, W.dnorm, n);
+}
+
+// copies the first ngroups slot groups to the host and reduces each
+static int read_groups(vd_ctx* ctx, MGWork& W, int ngroups, double* out) {
+    size_t bytes = (size_t)ngroups * NSLOT * SLOT_STRIDE * sizeof(unsigned long long);
+    VD_HIP(hipMemcpyAsync(ctx->pinned, W.dnorm, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    VD_HIP(hipStreamSynchronize(ctx->stream));
+    for (int q = 0; q < ngroups; q++) {
+        double m = 0.0;
+        for (int s = 0; s < NSLOT; s++) {
+            double v = ctx->pinned[(size_t)q * NSLOT * SLOT_STRIDE + (size_t)s * SLOT_STRIDE];
+            if (v != v || v > m) m = (v != v) ? v : m > v ? m : v;
+            if (m != m) break;
+        }
+        out[q] = m;
+    }
+    return VD_OK;
+}
+
 // residual of the finest level into W.res and its max-norm; with bnorm
 // also the max-norm of f (the rhs with boundary data)
-static int resid0(vd_ctx* ctx, MGWork& W, const LevV<double>& v0, double* nrm_out,
+static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double* nrm_out,
                   double* bnorm = nullptr) {
-    hipLaunchKernelGGL(k_zero_u64, dim3(1), dim3(2), 0, ctx->stream, W.dnorm);
+    zero_groups(ctx, W, 2);
     int64_t nc = ncell(W.n);
-    dim3 g = norm_grid(nc), b(256);
-    if (bnorm) hipLaunchKernelGGL(k_maxabs, g, b, 0, ctx->stream, W.f, nc, W.dnorm + 1);
+    if (bnorm)
+        hipLaunchKernelGGL(k_maxabs, norm_grid(nc), dim3(256), 0, ctx->stream, W.f, (unsigned)nc,
+                           slot_group(W, 1));
+    const int gx = (W.n[0] + 255) / 256;
+    const int nblk = gx * W.n[1] * W.n[2];
+    dim3 g((unsigned)nblk), b(256);
+    unsigned long long* sl = slot_group(W, 0);
+    double* part = W.part;
     kt_begin(ctx, VD_KT_RESID);
-    if (W.constb) {
-        if (W.dim == 3) hipLaunchKernelGGL((k_resid0<true, 3>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, W.dnorm);
-        else hipLaunchKernelGGL((k_resid0<true, 2>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, W.dnorm);
+    if (bm == 1) {
+        if (W.dim == 3) hipLaunchKernelGGL((k_resid0<1, 3>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, part, gx);
+        else hipLaunchKernelGGL((k_resid0<1, 2>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, part, gx);
+    } else if (bm == 0) {
+        if (W.dim == 3) hipLaunchKernelGGL((k_resid0<0, 3>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, part, gx);
+        else hipLaunchKernelGGL((k_resid0<0, 2>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, part, gx);
     } else {
-        if (W.dim == 3) hipLaunchKernelGGL((k_resid0<false, 3>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, W.dnorm);
-        else hipLaunchKernelGGL((k_resid0<false, 2>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, W.dnorm);
+        if (W.dim == 3) hipLaunchKernelGGL((k_resid0<2, 3>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, part, gx);
+        else hipLaunchKernelGGL((k_resid0<2, 2>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, part, gx);
     }
-    kt_end(ctx, VD_KT_RESID, 8.0 * (double)nc * (4 + (W.constb ? 0 : W.dim)));
+    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, ctx->stream, part, nblk, sl);
+    const int nb = bm == 1 ? 0 : (bm == 0 ? W.dim : 1);
+    kt_end(ctx, VD_KT_RESID, 8.0 * (double)nc * (4 + nb));
     VD_CHECK_LAUNCH();
-    VD_HIP(hipMemcpyAsync(ctx->pinned, W.dnorm, 16, hipMemcpyDeviceToHost, ctx->stream));
-    VD_HIP(hipStreamSynchronize(ctx->stream));
-    *nrm_out = ctx->pinned[0];
-    if (bnorm) *bnorm = ctx->pinned[1];
+    double m[2];
+    VD_TRY(read_groups(ctx, W, bnorm ? 2 : 1, m));
+    *nrm_out = m[0];
+    if (bnorm) *bnorm = m[1];
+    return VD_OK;
+}
+
+// Jacobi bound of the finest operator, rho = sum_d 2|beta/dx_d^2| bmax_d / |alpha|.
+// With rho <= SMOOTH_ONLY_RHO every red-black sweep contracts the error by
+// at least rho, the coarse-grid correction buys nothing, and the solve
+// runs finest-level sweeps only (the implicit species solves at small dt).
+constexpr double SMOOTH_ONLY_RHO = 1e-2;
+
+static int jacobi_rho(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta, double bconst,
+                      const double* dcell, double* rho) {
+    *rho = INFINITY;
+    if (alpha == 0.0) return VD_OK;
+    double bmax[3] = {0.0, 0.0, 0.0};
+    if (bm == 1) {
+        for (int d = 0; d < W.dim; d++) bmax[d] = fabs(bconst);
+    } else {
+        zero_groups(ctx, W, W.dim);
+        if (bm == 0) {
+            for (int d = 0; d < W.dim; d++) {
+                int64_t nf = nfacef(W.n, d);
+                hipLaunchKernelGGL(k_maxabs, norm_grid(nf), dim3(256), 0, ctx->stream, W.bf0[d], (unsigned)nf,
+                                   slot_group(W, d));
+            }
+        } else {
+            int64_t np = (int64_t)(W.n[0] + 1) * (W.n[1] + 1) * (W.dim == 3 ? W.n[2] + 1 : 1);
+            if (W.dim == 3)
+                hipLaunchKernelGGL((k_face_max_cd<3>), norm_grid(np), dim3(256), 0, ctx->stream, dcell, W.n[0],
+                                   W.n[1], W.n[2], slot_group(W, 0));
+            else
+                hipLaunchKernelGGL((k_face_max_cd<2>), norm_grid(np), dim3(256), 0, ctx->stream, dcell, W.n[0],
+                                   W.n[1], W.n[2], slot_group(W, 0));
+        }
+        VD_CHECK_LAUNCH();
+        VD_TRY(read_groups(ctx, W, W.dim, bmax));
+    }
+    double r = 0.0;
+    for (int d = 0; d < W.dim; d++) r += 2.0 * fabs(beta / (W.dx[d] * W.dx[d])) * bmax[d];
+    *rho = r / fabs(alpha);
     return VD_OK;
 }
 
 // Solve with the finest-level data already in W (x, f with BC shift,
-// bf0 / bconst, c00).  Builds the coarse hierarchy, iterates.
-static int mg_iterate(vd_ctx* ctx, MGWork& W, double alpha, double beta, double bconst, double reltol,
-                      double abstol, int maxiter, int nu1, int nu2, vd_solve_info* info) {
+// bf0 / bconst / dcell, c00).  Builds the coarse hierarchy unless the
+// operator is diagonally dominant enough for sweeps alone, then iterates.
+static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta, double bconst,
+                      const double* dcell, const int tlo[3], const int thi[3], bool smooth_only,
+                      double reltol, double abstol, int maxiter, int nu1, int nu2, vd_solve_info* info) {
     int64_t nc = ncell(W.n);
-    if (W.prec == 0) {
-        coarsen_hierarchy<double>(ctx, W, W.L64);
-    } else {
-        Lev<float>& L0 = W.L32[0];
+    if (!smooth_only) {
+        // coarse c0 depends only on the BC types unless a side is Robin
+        bool robin = false, same = W.c0_valid;
         for (int d = 0; d < W.dim; d++) {
-            if (!W.constb) {
-                int64_t nf = nfacef(W.n, d);
-                hipLaunchKernelGGL((k_convert<double, float>), grid1d(nf, 256), dim3(256), 0, ctx->stream,
-                                   W.bf0[d], L0.bf[d], nf);
-            }
-            if (!W.per[d])
-                for (int s = 0; s < 2; s++) {
-                    int64_t nt = ntanf(W.n, d);
-                    hipLaunchKernelGGL((k_convert<double, float>), grid1d(nt, 256), dim3(256), 0,
-                                       ctx->stream, W.c00[d][s], L0.c0[d][s], nt);
-                }
+            if (W.per[d]) continue;
+            robin |= tlo[d] == VD_ROBINBC || thi[d] == VD_ROBINBC;
+            same &= W.c0_types[d][0] == tlo[d] && W.c0_types[d][1] == thi[d];
         }
-        coarsen_hierarchy<float>(ctx, W, W.L32);
+        const bool c0 = robin || !same;
+        if (W.prec == 0) {
+            coarsen_hierarchy<double>(ctx, W, W.L64, true, c0);
+        } else {
+            Lev<float>& L0 = W.L32[0];
+            for (int d = 0; d < W.dim; d++) {
+                if (!W.constb) {
+                    int64_t nf = nfacef(W.n, d);
+                    hipLaunchKernelGGL((k_convert<double, float>), grid1d(nf, 256), dim3(256), 0, ctx->stream,
+                                       W.bf0[d], L0.bf[d], nf);
+                }
+                if (c0 && !W.per[d])
+                    for (int s = 0; s < 2; s++) {
+                        int64_t nt = ntanf(W.n, d);
+                        hipLaunchKernelGGL((k_convert<double, float>), grid1d(nt, 256), dim3(256), 0,
+                                           ctx->stream, W.c00[d][s], L0.c0[d][s], nt);
+                    }
+            }
+            coarsen_hierarchy<float>(ctx, W, W.L32, true, c0);
+        }
+        for (int d = 0; d < 3; d++) {
+            W.c0_types[d][0] = tlo[d];
+            W.c0_types[d][1] = thi[d];
+        }
+        W.c0_valid = !robin;
+        VD_CHECK_LAUNCH();
     }
-    VD_CHECK_LAUNCH();
     Lev<double> fine;
     for (int d = 0; d < 3; d++) {
         fine.n[d] = W.n[d];
@@ -834,26 +1043,30 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, double alpha, double beta, double 
         for (int s = 0; s < 2; s++) fine.c0[d][s] = W.c00[d][s];
     }
     LevV<double> v0 = view<double>(W, fine, alpha, beta, bconst);
+    v0.dcell = dcell;
     double r0, rn, bnorm;
-    VD_TRY(resid0(ctx, W, v0, &r0, &bnorm));
+    VD_TRY(resid0(ctx, W, bm, v0, &r0, &bnorm));
     double target = std::max(abstol, reltol * std::max(bnorm, r0));
     rn = r0;
     int it = 0, st = VD_OK;
     if (!(r0 <= target)) {
         for (it = 1; it <= maxiter; it++) {
-            if (W.prec == 0) {
-                vcycle<double>(ctx, W, W.L64, alpha, beta, bconst, nu1, nu2);
+            if (smooth_only) {
+                launch_gsrb<double>(ctx, W, bm, v0, W.x, W.f, 0, true);
+                launch_gsrb<double>(ctx, W, bm, v0, W.x, W.f, 1, true);
+            } else if (W.prec == 0) {
+                vcycle<double>(ctx, W, bm, W.L64, alpha, beta, bconst, nu1, nu2);
                 hipLaunchKernelGGL(k_axpy_to_double<double>, grid1d(nc, 256), dim3(256), 0, ctx->stream,
                                    W.x, W.L64[0].e, nc);
             } else {
                 hipLaunchKernelGGL((k_convert<double, float>), grid1d(nc, 256), dim3(256), 0, ctx->stream,
                                    W.res, W.L32[0].r, nc);
-                vcycle<float>(ctx, W, W.L32, alpha, beta, bconst, nu1, nu2);
+                vcycle<float>(ctx, W, bm, W.L32, alpha, beta, bconst, nu1, nu2);
                 hipLaunchKernelGGL(k_axpy_to_double<float>, grid1d(nc, 256), dim3(256), 0, ctx->stream,
                                    W.x, W.L32[0].e, nc);
             }
             VD_CHECK_LAUNCH();
-            VD_TRY(resid0(ctx, W, v0, &rn));
+            VD_TRY(resid0(ctx, W, bm, v0, &rn));
             if (rn <= target) break;
         }
         if (it > maxiter) {
@@ -881,7 +1094,7 @@ struct FlatArgs {
 };
 
 __global__ void k_a4_to_flat(FlatArgs f, double* __restrict__ out) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)f.nx * f.ny * f.nz) return;
     int i = (int)(t % f.nx);
     int64_t r = t / f.nx;
@@ -890,7 +1103,7 @@ __global__ void k_a4_to_flat(FlatArgs f, double* __restrict__ out) {
 }
 
 __global__ void k_flat_to_a4(FlatArgs f, const double* __restrict__ in) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)f.nx * f.ny * f.nz) return;
     int i = (int)(t % f.nx);
     int64_t r = t / f.nx;
@@ -915,7 +1128,7 @@ __global__ void k_side_from_ghost(Arr<double> a, int d, int side, int3 lo, int n
     int n[3] = {nx, ny, nz};
     int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
     int nt1 = n[t1], nt2 = dim == 3 ? n[t2] : 1;
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nt1 * nt2) return;
     int c[3] = {0, 0, 0};
     c[t1] = (int)(t % nt1);
@@ -943,7 +1156,7 @@ struct GradArgs {
 
 __global__ void k_grad_faces(GradArgs a) {
     int n0 = a.nx + (a.d == 0), n1 = a.ny + (a.d == 1), n2 = a.nz + (a.d == 2);
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)n0 * n1 * n2) return;
     int i = (int)(t % n0);
     int64_t r = t / n0;
@@ -1027,7 +1240,7 @@ static int grad_faces(vd_ctx* ctx, MGWork& W, const int tlo[3], const int thi[3]
 // c0 and rhs shift for every non-periodic side (sequential sides: no races
 // on corner cells).  bcv / rob must be filled.
 static int side_setup(vd_ctx* ctx, MGWork& W, const int tlo[3], const int thi[3], double beta,
-                      double bconst) {
+                      double bconst, const double* dcell = nullptr) {
     for (int d = 0; d < W.dim; d++) {
         if (W.per[d]) continue;
         for (int s = 0; s < 2; s++) {
@@ -1046,7 +1259,8 @@ static int side_setup(vd_ctx* ctx, MGWork& W, const int tlo[3], const int thi[3]
             a.ra = rob ? W.rob[d][s][0] : nullptr;
             a.rb = rob ? W.rob[d][s][1] : nullptr;
             a.rf = rob ? W.rob[d][s][2] : nullptr;
-            a.bf = W.constb ? nullptr : W.bf0[d];
+            a.bf = (W.constb || dcell) ? nullptr : W.bf0[d];
+            a.dcell = dcell;
             a.bconst = bconst;
             a.c0 = W.c00[d][s];
             a.f = W.f;
@@ -1080,7 +1294,7 @@ struct PotRhsArgs {
 
 // rhs = -e/eps0 * sum z n (PotentialSolve.cpp:203-223)
 __global__ void k_pot_rhs(PotRhsArgs a, double* __restrict__ f) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.nx * a.ny * a.nz) return;
     double r = 0.0;
     if (a.spacechrg) {
@@ -1107,7 +1321,7 @@ struct PotPostArgs {
 // POT copy, E cell averages (average_face_to_cellcenter) and E/N in Td
 // (PotentialSolve.cpp:340-379)
 __global__ void k_pot_post(PotPostArgs a) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.nx * a.ny * a.nz) return;
     int i = (int)(t % a.nx);
     int64_t q = t / a.nx;
@@ -1181,8 +1395,9 @@ int solve_potential(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array
     const double bconst = -1.0, alpha = 0.0, beta = 1.0;
     VD_TRY(side_setup(ctx, *W, prm->pot_bc_lo, prm->pot_bc_hi, beta, bconst));
     VD_TRY(fill_value(ctx, W->x, nc, 0.0));
-    int st = mg_iterate(ctx, *W, alpha, beta, bconst, prm->linsolve_reltol, prm->linsolve_abstol,
-                        prm->linsolve_maxiter, prm->mg_pre_smooth, prm->mg_post_smooth, info);
+    int st = mg_iterate(ctx, *W, 1, alpha, beta, bconst, nullptr, prm->pot_bc_lo, prm->pot_bc_hi, false,
+                        prm->linsolve_reltol, prm->linsolve_abstol, prm->linsolve_maxiter,
+                        prm->mg_pre_smooth, prm->mg_post_smooth, info);
     if (st != VD_OK && st != VD_ENOCONV) return st;
     VD_TRY(grad_faces(ctx, *W, prm->pot_bc_lo, prm->pot_bc_hi, ef, box, -1.0));
     PotPostArgs pa;
@@ -1203,33 +1418,38 @@ int solve_potential(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array
 }
 
 // ------------------------------------------------ implicit species solve
-struct HarmArgs {
+struct CellDArgs {
     const DevMech* __restrict__ M;
     Arr<double> sb;
-    int spec, d, ns;
+    int spec, ns, dim;
     int3 lo;
-    int nx, ny, nz;
+    int gnx, gny, gnz;  // grown box extent
     double Tg;
 };
 
-// harmonic face D from cell specDiff (ScalarSolve.cpp:483-512)
-__global__ void k_harmonic_faces(HarmArgs a, double* __restrict__ bf) {
-    int n0 = a.nx + (a.d == 0), n1 = a.ny + (a.d == 1), n2 = a.nz + (a.d == 2);
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)n0 * n1 * n2) return;
-    int i = (int)(t % n0);
-    int64_t r = t / n0;
-    int j = (int)(r % n1), k = (int)(r / n1);
-    i += a.lo.x;
-    j += a.lo.y;
-    k += a.lo.z;
-    int l[3] = {i, j, k};
-    l[a.d] -= 1;
-    const DevMech& M = *a.M;
+// cell specDiff on the box grown by one (ScalarSolve.cpp:483-497)
+__global__ void k_cell_diff(CellDArgs a, double* __restrict__ dcell) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
+    if (i >= a.gnx) return;
+    const int gz = a.dim == 3 ? 1 : 0;
     const double* te = a.sb.comp(VD_ETEMP(a.ns));
-    double A = spec_diff(M, a.spec, te[a.sb.off(l[0], l[1], l[2])], a.Tg);
-    double B = spec_diff(M, a.spec, te[a.sb.off(i, j, k)], a.Tg);
-    bf[t] = (A + B == 0.0) ? 0.0 : 2.0 * A * B / (A + B);
+    dcell[i + a.gnx * (j + a.gny * k)] =
+        spec_diff(*a.M, a.spec, te[a.sb.off(a.lo.x + i - 1, a.lo.y + j - 1, a.lo.z + k - gz)], a.Tg);
+}
+
+// harmonic face average of the cell coefficient (ScalarSolve.cpp:511-512)
+__global__ void k_harmonic_faces(const double* __restrict__ dcell, int nx, int ny, int nz, int d, int dim,
+                                 double* __restrict__ bf) {
+    int n0 = nx + (d == 0), n1 = ny + (d == 1), n2 = nz + (d == 2);
+    unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (unsigned)n0 * n1 * n2) return;
+    int i = (int)(t % n0);
+    unsigned r = t / n0;
+    int j = (int)(r % n1), k = (int)(r / n1);
+    const int gx = nx + 2, gxy = gx * (ny + 2);
+    const int gs = d == 0 ? 1 : (d == 1 ? gx : gxy);
+    const int gc = (i + 1) + gx * (j + 1) + (dim == 3 ? gxy * (k + 1) : 0);
+    bf[t] = harm(dcell[gc - gs], dcell[gc]);
 }
 
 struct SpecBcArgs {
@@ -1247,7 +1467,7 @@ __global__ void k_species_bc(SpecBcArgs a) {
     int n[3] = {a.nx, a.ny, a.nz};
     int t1 = a.d == 0 ? 1 : 0, t2 = a.d == 2 ? 1 : 2;
     int nt1 = n[t1], nt2 = a.dim == 3 ? n[t2] : 1;
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)nt1 * nt2) return;
     int c[3] = {0, 0, 0};
     c[t1] = (int)(t % nt1);
@@ -1304,7 +1524,7 @@ struct SpecRhsArgs {
 
 // rhs = (1/dt) n_old + dsdt; initial guess = Sborder (ScalarSolve.cpp:443-465)
 __global__ void k_spec_rhs(SpecRhsArgs a, double* __restrict__ f, double* __restrict__ x) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.nx * a.ny * a.nz) return;
     int i = (int)(t % a.nx);
     int64_t q = t / a.nx;
@@ -1326,14 +1546,14 @@ struct SpecPostArgs {
 };
 
 __global__ void k_spec_floor(SpecPostArgs a) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.nx * a.ny * a.nz) return;
     if (a.floor_on && a.x[t] < a.floor_val) a.x[t] = a.floor_val;
 }
 
 // copy to phi_new and the Te update (ScalarSolve.cpp:632-667)
 __global__ void k_spec_post(SpecPostArgs a) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.nx * a.ny * a.nz) return;
     int i = (int)(t % a.nx);
     int64_t q = t / a.nx;
@@ -1369,20 +1589,23 @@ int implicit_solve(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sb,
     dims_of(box, g, n, lo);
     int64_t nc = ncell(n);
     const double Tg = prm->gas_temperature;
+    {
+        CellDArgs cd;
+        cd.M = m->dev;
+        cd.sb = arr(sb);
+        cd.spec = spec;
+        cd.ns = ns;
+        cd.dim = g->dim;
+        cd.lo = make_int3(lo[0], lo[1], lo[2]);
+        cd.gnx = n[0] + 2;
+        cd.gny = n[1] + 2;
+        cd.gnz = g->dim == 3 ? n[2] + 2 : 1;
+        cd.Tg = Tg;
+        const int bx = cd.gnx > 64 ? 128 : 64;
+        hipLaunchKernelGGL(k_cell_diff, dim3((cd.gnx + bx - 1) / bx, cd.gny, cd.gnz), dim3(bx), 0, ctx->stream,
+                           cd, W->dcell);
+    }
     for (int d = 0; d < g->dim; d++) {
-        HarmArgs h;
-        h.M = m->dev;
-        h.sb = arr(sb);
-        h.spec = spec;
-        h.d = d;
-        h.ns = ns;
-        h.lo = make_int3(lo[0], lo[1], lo[2]);
-        h.nx = n[0];
-        h.ny = n[1];
-        h.nz = n[2];
-        h.Tg = Tg;
-        hipLaunchKernelGGL(k_harmonic_faces, grid1d(nfacef(n, d), 256), dim3(256), 0, ctx->stream, h,
-                           W->bf0[d]);
         if (W->per[d]) continue;
         for (int s = 0; s < 2; s++) {
             SpecBcArgs b;
@@ -1393,7 +1616,7 @@ int implicit_solve(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sb,
             b.side = s;
             b.ns = ns;
             b.dim = g->dim;
-            b.lo = h.lo;
+            b.lo = make_int3(lo[0], lo[1], lo[2]);
             b.nx = n[0];
             b.ny = n[1];
             b.nz = n[2];
@@ -1426,9 +1649,17 @@ int implicit_solve(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sb,
     hipLaunchKernelGGL(k_spec_rhs, grid1d(nc, 256), dim3(256), 0, ctx->stream, r, W->f, W->x);
     VD_CHECK_LAUNCH();
     const double alpha = 1.0 / dt, beta = 1.0;
-    VD_TRY(side_setup(ctx, *W, bclo, bchi, beta, 0.0));
-    int st = mg_iterate(ctx, *W, alpha, beta, 0.0, prm->linsolve_reltol, prm->linsolve_abstol,
-                        prm->linsolve_maxiter, prm->mg_pre_smooth, prm->mg_post_smooth, info);
+    double rho;
+    VD_TRY(jacobi_rho(ctx, *W, 2, alpha, beta, 0.0, W->dcell, &rho));
+    const bool smooth = rho <= SMOOTH_ONLY_RHO;
+    if (!smooth)
+        for (int d = 0; d < g->dim; d++)
+            hipLaunchKernelGGL(k_harmonic_faces, grid1d(nfacef(n, d), 256), dim3(256), 0, ctx->stream, W->dcell,
+                               n[0], n[1], n[2], d, g->dim, W->bf0[d]);
+    VD_TRY(side_setup(ctx, *W, bclo, bchi, beta, 0.0, W->dcell));
+    int st = mg_iterate(ctx, *W, smooth ? 2 : 0, alpha, beta, 0.0, W->dcell, bclo, bchi, smooth,
+                        prm->linsolve_reltol, prm->linsolve_abstol, prm->linsolve_maxiter,
+                        prm->mg_pre_smooth, prm->mg_post_smooth, info);
     if (st != VD_OK && st != VD_ENOCONV) return st;
     SpecPostArgs p;
     p.x = W->x;
@@ -1500,8 +1731,12 @@ int mlmg_solve(vd_ctx* ctx, const vd_mlmg_problem* P, vd_array4* phi, const vd_a
     }
     VD_CHECK_LAUNCH();
     VD_TRY(side_setup(ctx, *W, P->bc_lo, P->bc_hi, P->beta, 1.0));
-    int st = mg_iterate(ctx, *W, P->alpha_a, P->beta, 1.0, P->reltol, P->abstol, P->maxiter,
-                        P->pre_smooth, P->post_smooth, info);
+    const int bm = constb ? 1 : 0;
+    double rho;
+    VD_TRY(jacobi_rho(ctx, *W, bm, P->alpha_a, P->beta, 1.0, nullptr, &rho));
+    int st = mg_iterate(ctx, *W, bm, P->alpha_a, P->beta, 1.0, nullptr, P->bc_lo, P->bc_hi,
+                        rho <= SMOOTH_ONLY_RHO, P->reltol, P->abstol, P->maxiter, P->pre_smooth,
+                        P->post_smooth, info);
     if (st != VD_OK && st != VD_ENOCONV) return st;
     fa = flat_args(phi, 0, lo, n);
     hipLaunchKernelGGL(k_flat_to_a4, grid1d(nc, 256), dim3(256), 0, ctx->stream, fa, W->x);

@@ -29,7 +29,7 @@ __device__ __forceinline__ unsigned long long bits(double v) {
 
 __global__ void __launch_bounds__(256) k_cfl(CflArgs a) {
     __shared__ double red[3][4];
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     int64_t tot = (int64_t)a.nx * a.ny * a.nz;
     double m0 = 0.0, m1 = 0.0, m2 = 0.0;
     const DevMech& M = *a.M;
@@ -138,7 +138,7 @@ struct TagArgs {
 
 // state_based_refinement + stategrad_based_refinement
 __global__ void __launch_bounds__(256) k_tags(TagArgs a) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.nx * a.ny * a.nz) return;
     int i = (int)(t % a.nx);
     int64_t q = t / a.nx;

@@ -81,7 +81,7 @@ struct FluxArgs {
 // one thread per face of direction a.dir
 template <int HYP, int S>
 __global__ void __launch_bounds__(256) k_face_flux(FluxArgs a) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     int64_t tot = (int64_t)a.fnx * a.fny * a.fnz;
     if (t >= tot) return;
     int i = (int)(t % a.fnx);
@@ -141,7 +141,7 @@ struct DsdtArgs {
 
 // compute_dsdt (Source/ScalarSolve.cpp:44-53)
 __global__ void __launch_bounds__(256) k_dsdt(DsdtArgs a) {
-    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     int64_t tot = (int64_t)a.nx * a.ny * a.nz;
     if (t >= tot) return;
     int i = (int)(t % a.nx);

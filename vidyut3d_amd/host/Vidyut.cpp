@@ -258,14 +258,17 @@ int Vidyut::Evolve(int nsteps, Real dt, int ncorr) {
         Real dt_edrift, dt_ediff, dt_diel;
         TRY(find_time_scales(0, dt_edrift, dt_ediff, dt_diel));
         const Real cur_time = t_new_;
-        TRY(phi_old_[0].copy_from(phi_new_[0], 0, 0, nvar_));
-        // Sborder_old is read only in its valid region: phi_old stands in.
+        // Sborder_old is read only in its valid region by the implicit
+        // solves (species and energy), so phi_old stands in for it; with one
+        // corrector nothing else of phi_old is read.
+        TRY(phi_old_[0].copy_from(phi_new_[0], 0, 0, ncorr > 1 ? nvar_ : ns_ + 1));
         for (int niter = 0; niter < ncorr; niter++) {
-            TRY(FillPatch(0, cur_time + dt, Sborder_[0], 0, nvar_));
+            // Sborder components read by the step: species, EEN, ETEMP (here)
+            // and POT, EFX..EFZ (after the potential solve); EJH, EIH, EEH and
+            // REF of Sborder are never read.
+            TRY(FillPatch(0, cur_time + dt, Sborder_[0], 0, ns_ + 2));
             TRY(solve_potential(cur_time, Sborder_, prm_.pot_bc_lo, prm_.pot_bc_hi, efield_ec_));
-            // refill what solve_potential changed: POT, EFX..EFZ, REF
             TRY(FillPatch(0, cur_time + dt, Sborder_[0], VD_POT(ns_), 4));
-            TRY(FillPatch(0, cur_time + dt, Sborder_[0], VD_REF(ns_), 1));
             if (prm_.do_reactions)
                 TRY(update_rxnsrc_at_all_levels(Sborder_, rxn_src_, cur_time));
             else

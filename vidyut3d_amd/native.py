@@ -74,6 +74,9 @@ class Device:
     def sync(self):
         check(lib().vd_ctx_sync(self.ptr), "vd_ctx_sync")
 
+    def release_workspaces(self):
+        check(lib().vd_ctx_release_workspaces(self.ptr), "vd_ctx_release_workspaces")
+
     def alloc(self, nbytes):
         p = C.c_void_p()
         check(lib().vd_malloc(self.ptr, C.c_size_t(int(nbytes)), C.byref(p)), "vd_malloc")
