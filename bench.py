@@ -219,6 +219,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--headline-only", action="store_true", help="skip the other configurations")
+    ap.add_argument("--only", default="", help="comma-separated subset of: mixed,discharge,poisson,chemistry,serving")
     ap.add_argument("--poisson-n", type=int, default=512)
     ap.add_argument("--chem-n", type=int, default=256)
     ap.add_argument("--serve-seconds", type=float, default=20.0)
@@ -258,7 +259,12 @@ def main():
                 "algorithmic_bytes_per_launch": round(byts)}
 
     configs = {}
-    if not args.headline_only:
+    only = set(x for x in args.only.split(",") if x)
+
+    def want(name):
+        return not args.headline_only and (not only or name in only)
+
+    if want("mixed"):
         log("headline case with FP32 V-cycles in FP64 iterative refinement")
         cmp_case = case_fn(args.n, precision=1)
         wall_mp, st_mp, iters_mp, _ = run_steps(native, abi, dev, cmp_case, args.steps, args.warmup)
@@ -267,16 +273,19 @@ def main():
             "cell_steps_per_s": case.ncells * args.steps / wall_mp,
             "ms_per_step": wall_mp / args.steps * 1e3, "mg_iterations": iters_mp,
             "speedup_over_fp64": wall / wall_mp}
+    if want("discharge"):
         log("2-D 128^2 discharge, 100 steps (GPU and oracle)")
         configs["discharge_2d_128"] = bench_discharge_2d(native, abi, cases, compare, dev, not args.no_cpu)
+    if want("poisson"):
         log("%d^3 dielectric Poisson FP64 vs mixed" % args.poisson_n)
         configs["dielectric_poisson"] = bench_dielectric(native, abi, cases, compare, dev, args.poisson_n)
+    if want("chemistry"):
         log("%d^3 40-species chemistry" % args.chem_n)
         configs["chemistry_40sp_200rx"] = bench_chemistry(native, abi, cases, dev, args.chem_n)
-        if args.serve_seconds > 0:
-            from vidyut3d_amd import serve
-            log("serving loop for %.0f s of arrivals" % args.serve_seconds)
-            configs["serving"] = serve.bench(dev_index=0, duration_s=args.serve_seconds)
+    if want("serving") and args.serve_seconds > 0:
+        from vidyut3d_amd import serve
+        log("serving loop for %.0f s of arrivals" % args.serve_seconds)
+        configs["serving"] = serve.bench(dev_index=0, duration_s=args.serve_seconds)
 
     cpu = None
     if not args.no_cpu:
