@@ -217,6 +217,15 @@ int vd_mech_create(vd_ctx* ctx, const vd_mech_desc* desc, vd_mech** out);
 int vd_mech_destroy(vd_mech* m);
 /* plasmachem::find_id (Source/PlasmaChem.cpp:21-30); -1 when absent */
 int vd_mech_find_id(const vd_mech* m, const char* name);
+/* Mechanism-specialised source kernel for table mechanisms: the reaction
+ * table is compiled into straight-line gfx950 code, the run-time
+ * counterpart of the reference compiling a case's generated Chemistry.H
+ * into the executable (test/models/ionelec_DC_1d/Chemistry.H:240-357).
+ * vd_mech_create builds it by default (environment VD_CHEM_SPECIALISE=0
+ * turns that off); enable = 1 builds and uses it, 0 returns to the
+ * generic table kernel.  VD_EINVAL for CKWC mechanisms. */
+int vd_mech_specialise(vd_mech* m, int enable);
+int vd_mech_is_specialised(const vd_mech* m);
 
 /* ---------------------------------------------- bulk MultiFab operations
  * MultiFab::setVal / MultiFab::Copy / MultiFab::LinComb on the valid box

@@ -132,7 +132,10 @@ def test_rxn_ckwc_2d(dev):
         assert relmax(ref[q], got[q]) < 1e-12, q
 
 
-def test_rxn_table_40_species(dev):
+@pytest.mark.parametrize("specialised", [True, False])
+def test_rxn_table_40_species(dev, specialised):
+    """Table chemistry, 40 species / 200 reactions, through the
+    mechanism-specialised kernel and through the generic table kernel."""
     m = cases.co2_mechanism(seed=3)
     n = 8
     g = abi.make_geom([n] * 3, [0] * 3, [1] * 3, [0] * 3)
@@ -141,6 +144,8 @@ def test_rxn_table_40_species(dev):
     prm = cases.default_params()
     ref = O.rxn_source(m, prm, og, st, 0)
     mech = native.Mech(dev, m)
+    mech.specialise(specialised)
+    assert mech.specialised == specialised
     out = dg.array(m.nspec + 1)
     native.rxn_source(mech, prm, dg, dg.array(st.shape[0], host=st), out)
     got = out.get()

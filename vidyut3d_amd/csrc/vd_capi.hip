@@ -1,5 +1,6 @@
 // vd_capi.hip -- context, memory, case plug-in upload and kernel timing.
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 
 #include "vd_internal.hpp"
@@ -289,16 +290,38 @@ int vd_mech_create(vd_ctx* ctx, const vd_mech_desc* d, vd_mech** out) {
         set_error("mechanism upload: %s", hipGetErrorString(e));
         return VD_EHIP;
     }
+    if (h.chem_form == VD_CHEM_TABLE && h.nreac > 0) {
+        const char* env = getenv("VD_CHEM_SPECIALISE");
+        if (!(env && strcmp(env, "0") == 0)) {
+            if (chem_jit_load(m) == VD_OK) m->use_spec = 1;
+            else fprintf(stderr, "vidyut3d_amd: specialised source kernel unavailable (%s); "
+                                 "using the table kernel\n", vd_last_error());
+        }
+    }
     *out = m;
     return VD_OK;
 }
 
 int vd_mech_destroy(vd_mech* m) {
     if (!m) return VD_EINVAL;
+    chem_jit_unload(m);
     (void)hipFree(m->dev);
     delete m;
     return VD_OK;
 }
+
+int vd_mech_specialise(vd_mech* m, int enable) {
+    if (!m) return VD_EINVAL;
+    if (!enable) {
+        m->use_spec = 0;
+        return VD_OK;
+    }
+    VD_TRY(chem_jit_load(m));
+    m->use_spec = 1;
+    return VD_OK;
+}
+
+int vd_mech_is_specialised(const vd_mech* m) { return m && m->use_spec && m->fn ? 1 : 0; }
 
 int vd_mech_find_id(const vd_mech* m, const char* name) {
     for (int s = 0; s < m->host.nspec; s++)

@@ -166,6 +166,11 @@ int update_rxnsrc(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4*
     int64_t tot = (int64_t)a.nx * a.ny * a.nz;
     size_t lds = sizeof(double) * (m->hdev.nspec + 1) * CHEM_BLOCK;
     kt_begin(ctx, VD_KT_CHEM);
+    if (m->hdev.chem_form == VD_CHEM_TABLE && m->use_spec && m->fn) {
+        int st = chem_jit_launch(ctx, m, sb, rxn, box, prm->gas_temperature);
+        kt_end(ctx, VD_KT_CHEM, 8.0 * (double)tot * (2 * m->hdev.nspec + 3));
+        return st;
+    }
     if (m->hdev.chem_form == VD_CHEM_TABLE)
         hipLaunchKernelGGL(k_rxn_table, grid1d(tot, CHEM_BLOCK), dim3(CHEM_BLOCK), lds, ctx->stream, a);
     else

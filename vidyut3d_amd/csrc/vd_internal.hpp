@@ -133,7 +133,18 @@ struct vd_mech {
     vd_mech_desc host;
     vd::DevMech hdev;     // host copy of the device struct
     vd::DevMech* dev;     // device memory
+    // mechanism-specialised source kernel (table form), loaded on ctx's device
+    void* mod = nullptr;  // hipModule_t
+    void* fn = nullptr;   // hipFunction_t
+    int use_spec = 0;
 };
+
+namespace vd {
+int chem_jit_load(vd_mech* m);
+void chem_jit_unload(vd_mech* m);
+int chem_jit_launch(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4* rxn, const vd_box* box,
+                    double Tg);
+}
 
 namespace vd {
 struct MGWork;  // multigrid workspace (vd_mg.hip)

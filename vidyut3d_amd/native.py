@@ -153,6 +153,15 @@ class Mech:
     def find_id(self, name):
         return lib().vd_mech_find_id(self.ptr, name.encode())
 
+    def specialise(self, enable=True):
+        """Use (True) the mechanism-specialised source kernel or (False) the
+        generic table kernel."""
+        check(lib().vd_mech_specialise(self.ptr, 1 if enable else 0), "vd_mech_specialise")
+
+    @property
+    def specialised(self):
+        return bool(lib().vd_mech_is_specialised(self.ptr))
+
     def __del__(self):
         try:
             if self.ptr:
