@@ -21,6 +21,9 @@
 //    stopping test stay FP64 (iterative refinement).
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <type_traits>
 
 #include "vd_internal.hpp"
 
@@ -40,6 +43,7 @@ struct LevV {  // kernel view of one level
     const T* bf2;
     const T* dcell;      // BM 2: cell coefficient on the grown box
     const T* c0[3][2];   // per boundary face c0 (non-periodic sides)
+    T c0u[3][2];         // c0 of sides whose c0 is uniform (no Robin side)
     int per0, per1, per2;
 };
 
@@ -50,6 +54,7 @@ struct Lev {
     T* bf[3] = {nullptr, nullptr, nullptr};
     T* c0[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     T* e = nullptr;
+    T* e2 = nullptr;  // second correction buffer of the fused smoother (finest level)
     T* r = nullptr;
     T* p = nullptr;  // CG scratch (bottom level)
     T* q = nullptr;
@@ -64,6 +69,7 @@ struct MGWork {
     std::vector<void*> allocs;
     // FP64 finest-level data
     double* x = nullptr;
+    double* x2 = nullptr;  // second solution buffer of the fused smoother
     double* f = nullptr;
     double* res = nullptr;
     double* bf0[3] = {nullptr, nullptr, nullptr};
@@ -75,6 +81,10 @@ struct MGWork {
     double* part = nullptr;               // per-block partial norms
     int c0_types[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};
     bool c0_valid = false;                // coarse c0 match c0_types
+    // no Robin side: c0 is -1 (Dirichlet) or 1 (Neumann) on every boundary
+    // face of every level, and the fused smoother takes it as a constant
+    int c0uni = 0;
+    double c0u[3][2] = {{0, 0}, {0, 0}, {0, 0}};
     std::vector<Lev<double>> L64;
     std::vector<Lev<float>> L32;
 };
@@ -123,63 +133,73 @@ __device__ __forceinline__ T harm(T a, T b) {
     return (a + b == T(0)) ? T(0) : T(2) * a * b / (a + b);
 }
 
+// face coefficients of cell (i,j,k) in direction d
 template <class T, int BM, int DIM>
-__device__ __forceinline__ void stencil(const LevV<T>& L, const T* __restrict__ x, int i, int j,
-                                        int k, T& dgo, T& ofo) {
-    const int c = i + L.nx * (j + L.ny * k);
-    int gc = 0, gx = 0, gxy = 0;
-    if (BM == 2) {
-        gx = L.nx + 2;
-        gxy = gx * (L.ny + 2);
-        gc = (i + 1) + gx * (j + 1) + (DIM == 3 ? gxy * (k + 1) : 0);
+__device__ __forceinline__ void face_coef(const LevV<T>& L, int d, int i, int j, int k, T& blo, T& bhi) {
+    if (BM == 1) {
+        blo = bhi = L.bconst;
+    } else if (BM == 0) {
+        const T* bf = d == 0 ? L.bf0 : (d == 1 ? L.bf1 : L.bf2);
+        const int f = fidx(L, d, i, j, k);
+        const int fs = d == 0 ? 1 : (d == 1 ? (L.nx) : (L.nx * L.ny));
+        blo = bf[f];
+        bhi = bf[f + fs];
+    } else {
+        const int gx = L.nx + 2, gxy = gx * (L.ny + 2);
+        const int gc = (i + 1) + gx * (j + 1) + (DIM == 3 ? gxy * (k + 1) : 0);
+        const int gs = d == 0 ? 1 : (d == 1 ? gx : gxy);
+        const T dc = L.dcell[gc];
+        blo = harm(L.dcell[gc - gs], dc);
+        bhi = harm(dc, L.dcell[gc + gs]);
     }
+}
+
+// dg and of of cell (i,j,k); nb(d, side) returns the neighbour value on
+// side 0 (low) / 1 (high) of direction d when that neighbour is a cell of
+// the level (interior or periodic image)
+template <class T, int BM, int DIM, class NB, bool C0U = false>
+__device__ __forceinline__ void stencil_nb(const LevV<T>& L, int i, int j, int k, NB nb, T& dgo, T& ofo) {
     T dg = L.alpha, of = T(0);
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
         const int cd = d == 0 ? i : (d == 1 ? j : k);
         const int nd = d == 0 ? L.nx : (d == 1 ? L.ny : L.nz);
-        const int st = d == 0 ? 1 : (d == 1 ? L.nx : L.nx * L.ny);
         const int per = d == 0 ? L.per0 : (d == 1 ? L.per1 : L.per2);
         const T s = d == 0 ? L.s0 : (d == 1 ? L.s1 : L.s2);
         T blo, bhi;
-        if (BM == 1) {
-            blo = bhi = L.bconst;
-        } else if (BM == 0) {
-            const T* bf = d == 0 ? L.bf0 : (d == 1 ? L.bf1 : L.bf2);
-            const int f = fidx(L, d, i, j, k);
-            const int fs = d == 0 ? 1 : (d == 1 ? (L.nx) : (L.nx * L.ny));
-            blo = bf[f];
-            bhi = bf[f + fs];
-        } else {
-            const int gs = d == 0 ? 1 : (d == 1 ? gx : gxy);
-            const T dc = L.dcell[gc];
-            blo = harm(L.dcell[gc - gs], dc);
-            bhi = harm(dc, L.dcell[gc + gs]);
-        }
+        face_coef<T, BM, DIM>(L, d, i, j, k, blo, bhi);
         T dd = T(0), oo = T(0);
-        if (cd > 0) {
+        if (cd > 0 || per) {
             dd += blo;
-            oo += blo * x[c - st];
-        } else if (per) {
-            dd += blo;
-            oo += blo * x[c + (nd - 1) * st];
+            oo += blo * nb(d, 0);
         } else {
-            dd += blo * (T(1) - L.c0[d][0][tidx(L, d, i, j, k)]);
+            dd += blo * (T(1) - (C0U ? L.c0u[d][0] : L.c0[d][0][tidx(L, d, i, j, k)]));
         }
-        if (cd < nd - 1) {
+        if (cd < nd - 1 || per) {
             dd += bhi;
-            oo += bhi * x[c + st];
-        } else if (per) {
-            dd += bhi;
-            oo += bhi * x[c - (nd - 1) * st];
+            oo += bhi * nb(d, 1);
         } else {
-            dd += bhi * (T(1) - L.c0[d][1][tidx(L, d, i, j, k)]);
+            dd += bhi * (T(1) - (C0U ? L.c0u[d][1] : L.c0[d][1][tidx(L, d, i, j, k)]));
         }
         dg += s * dd;
         of += s * oo;
     }
     dgo = dg;
     ofo = of;
+}
+
+template <class T, int BM, int DIM>
+__device__ __forceinline__ void stencil(const LevV<T>& L, const T* __restrict__ x, int i, int j,
+                                        int k, T& dgo, T& ofo) {
+    const int c = i + L.nx * (j + L.ny * k);
+    auto nb = [&](int d, int side) -> T {
+        const int cd = d == 0 ? i : (d == 1 ? j : k);
+        const int nd = d == 0 ? L.nx : (d == 1 ? L.ny : L.nz);
+        const int st = d == 0 ? 1 : (d == 1 ? L.nx : L.nx * L.ny);
+        if (side == 0) return cd > 0 ? x[c - st] : x[c + (nd - 1) * st];
+        return cd < nd - 1 ? x[c + st] : x[c - (nd - 1) * st];
+    };
+    stencil_nb<T, BM, DIM>(L, i, j, k, nb, dgo, ofo);
 }
 
 // ------------------------------------------------------------ kernels
@@ -198,6 +218,267 @@ __global__ void __launch_bounds__(128) k_gsrb(LevV<T> L, T* __restrict__ x, cons
     stencil<T, BM, DIM>(L, x, i, j, k, dg, of);
     const int c = i + L.nx * (j + L.ny * k);
     x[c] = (rhs[c] + of) / dg;
+}
+
+// Fused red-black smoother for 3-D levels whose z direction is not
+// periodic.  One launch applies S colour sweeps (S = 2: one red + black
+// sweep, S = 4: two), optionally followed (RES) by the max-norm of the
+// residual of the result, in a single pass over HBM instead of S launches
+// that each re-read the level.
+//  * A block owns an interior tile of FZ_IX x FZ_IY cells and zc planes;
+//    it loads a halo of FZ_H cells / S+RES planes around it and streams up
+//    the planes.  At pipeline step t, stage s updates plane t-s+1, so every
+//    stage sees its z-neighbours after stage s-1 and before stage s+1 --
+//    the values S separate colour launches would give, bit for bit.
+//  * Each thread owns a red/black pair of x-adjacent cells of one row and
+//    keeps its column of planes (and their rhs) in registers; in-plane
+//    neighbours of the opposite colour go through LDS, one buffer per
+//    stage, published once per step.
+//  * Halo values go wrong by one cell per stage from the tile edge inward;
+//    the halo is as wide as the stage count, so the interior is exact.
+//  * Reads xin (zeros when xin is null), writes xout.  Halos overlap the
+//    neighbouring blocks' interiors, so xin and xout must differ.
+// Requires even nx, ny (colour parity of periodic images).
+constexpr int FZ_IX = 64, FZ_IY = 16, FZ_H = 4;
+constexpr int FZ_PX = (FZ_IX + 2 * FZ_H) / 2;  // 36 thread columns, 2 cells each
+constexpr int FZ_RY = FZ_IY + 2 * FZ_H;        // 24 thread rows
+constexpr int FZ_THREADS = FZ_PX * FZ_RY;      // 864
+
+// raw buffer resource over `bytes` of memory (gfx9 family config word);
+// accesses at offsets >= bytes return 0 (loads) or are dropped (stores)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, unsigned bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, (int)bytes, 0x00020000);
+}
+template <class T>
+__device__ __forceinline__ T buf_ld(__amdgpu_buffer_rsrc_t r, unsigned off);
+template <>
+__device__ __forceinline__ double buf_ld<double>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, 0, 0));
+}
+template <>
+__device__ __forceinline__ float buf_ld<float>(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)off, 0, 0));
+}
+template <class T>
+__device__ __forceinline__ void buf_st(__amdgpu_buffer_rsrc_t r, unsigned off, T v);
+template <>
+__device__ __forceinline__ void buf_st<double>(__amdgpu_buffer_rsrc_t r, unsigned off, double v) {
+    using U = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U, v), r, (int)off, 0, 0);
+}
+template <>
+__device__ __forceinline__ void buf_st<float>(__amdgpu_buffer_rsrc_t r, unsigned off, float v) {
+    using U = decltype(__builtin_amdgcn_raw_buffer_load_b32(r, 0, 0, 0));
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(U, v), r, (int)off, 0, 0);
+}
+
+// f(integral_constant<K>, t + K) for K = K0 .. N-1 while t + K < tend
+template <int K, int N, class F>
+__device__ __forceinline__ void fused_unroll(F& f, int t, int tend) {
+    if constexpr (K < N) {
+        if (t + K < tend) {
+            f(std::integral_constant<int, K>{}, t + K);
+            fused_unroll<K + 1, N>(f, t, tend);
+        }
+    }
+}
+
+template <class T, int BM, int S, int RES, int ZI>
+__global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* __restrict__ xin,
+                                                           T* __restrict__ xout, const T* __restrict__ rhs,
+                                                           int zc, int ntx, int nty,
+                                                           double* __restrict__ part) {
+    // Register ring indexed by plane: plane p lives in slot (p - zlo) mod NV
+    // (NV even), and the step loop is unrolled NV times, so every slot index
+    // is a compile-time constant and no values move between registers.
+    // Stage s of step t updates plane t-s+1 (slot k-s of unrolled step k);
+    // values are kept by colour (red = (i+j+k) even) so that the colour a
+    // stage updates is a compile-time constant too.
+    constexpr int NV = ((S + 2 + RES) + 1) & ~1;
+    __shared__ T nbuf[S][FZ_RY][FZ_PX];
+    __shared__ T fbuf[RES ? 2 : 1][FZ_RY][FZ_PX];
+    __shared__ double red[(FZ_THREADS + 63) / 64];
+
+    const unsigned lb = lblock();
+    const int ntile = ntx * nty;
+    const int tile = (int)(lb % (unsigned)ntile), zb = (int)(lb / (unsigned)ntile);
+    const int x0 = (tile % ntx) * FZ_IX, y0 = (tile / ntx) * FZ_IY, z0 = zb * zc;
+    const int tx = (int)threadIdx.x % FZ_PX, ty = (int)threadIdx.x / FZ_PX;
+    // cells ia, ia+1 of row ja; global (periodically wrapped) indices
+    const int ia = x0 - FZ_H + 2 * tx, ja = y0 - FZ_H + ty;
+    int ic0 = ia, ic1 = ia + 1, jc = ja;
+    bool ok0, ok1, okj;
+    if (L.per0) {
+        ic0 %= L.nx;
+        if (ic0 < 0) ic0 += L.nx;
+        ic1 %= L.nx;
+        if (ic1 < 0) ic1 += L.nx;
+        ok0 = ok1 = true;
+    } else {
+        ok0 = ic0 >= 0 && ic0 < L.nx;
+        ok1 = ic1 >= 0 && ic1 < L.nx;
+    }
+    if (L.per1) {
+        jc %= L.ny;
+        if (jc < 0) jc += L.ny;
+        okj = true;
+    } else {
+        okj = jc >= 0 && jc < L.ny;
+    }
+    // interior cells of this tile that lie in the level (written / normed)
+    const bool orow = tx >= FZ_H / 2 && tx < FZ_PX - FZ_H / 2 && ty >= FZ_H && ty < FZ_RY - FZ_H &&
+                      ja < L.ny;
+    const bool oc0 = orow && ia < L.nx, oc1 = orow && ia + 1 < L.nx;
+
+    const int zlo = max(z0 - S - RES, 0), zhi = min(z0 + zc - 1 + S + RES, L.nz - 1);
+    const int zout = min(z0 + zc, L.nz);
+    const int pstr = L.nx * L.ny;
+    const int par = ia + ja;  // colour of cell ia on plane p is (par + p) & 1
+    // clamped indices: every lane evaluates the stencil (results of lanes
+    // outside the level are discarded), so no memory access depends on a
+    // branch and the compiler can count loads in flight exactly
+    const int ic0c = min(max(ic0, 0), L.nx - 1), ic1c = min(max(ic1, 0), L.nx - 1);
+    const int jcc = min(max(jc, 0), L.ny - 1);
+    const int pc0 = ic0c + L.nx * jcc, pc1 = ic1c + L.nx * jcc;
+    // range-checked buffer accesses: an offset with bit 31 set is outside
+    // the buffer, so the load returns 0 and the store is dropped
+    constexpr unsigned OOB = 0x80000000u;
+    const unsigned lbytes = (unsigned)((int64_t)pstr * L.nz * (int64_t)sizeof(T));
+    const __amdgpu_buffer_rsrc_t rx = buf_rsrc(xin, lbytes), rf = buf_rsrc(rhs, lbytes),
+                                 ro = buf_rsrc(xout, lbytes);
+    auto offs = [&](int pc, bool ok, int p) -> unsigned {
+        return ok ? (unsigned)(pc + pstr * p) * (unsigned)sizeof(T) : OOB;
+    };
+
+    T vr[NV], vb[NV], fr[NV], fb[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) vr[q] = vb[q] = fr[q] = fb[q] = T(0);
+    // planes are prefetched two steps ahead into two alternating buffers
+    // (no register moves of in-flight loads), issued at the end of a step
+    // after that step's own loads
+    struct PF {
+        T x0, x1, f0, f1;  // by position
+    };
+    auto load = [&](PF& q, int p) {
+        const bool okp = p >= zlo && p <= zhi && okj;
+        const unsigned o0 = offs(pc0, okp && ok0, p), o1 = offs(pc1, okp && ok1, p);
+        q.f0 = buf_ld<T>(rf, o0);
+        q.f1 = buf_ld<T>(rf, o1);
+        if constexpr (ZI) {
+            q.x0 = q.x1 = T(0);
+        } else {
+            q.x0 = buf_ld<T>(rx, o0);
+            q.x1 = buf_ld<T>(rx, o1);
+        }
+    };
+    double amax = 0.0;
+    const int txm = tx > 0 ? tx - 1 : tx, txp = tx < FZ_PX - 1 ? tx + 1 : tx;
+    const int tym = ty > 0 ? ty - 1 : ty, typ = ty < FZ_RY - 1 ? ty + 1 : ty;
+    const int tend = zout + S;
+    PF qa, qb;
+    auto step = [&](auto kc, int t) {
+        constexpr int k = decltype(kc)::value;  // (t - zlo + 1) mod NV
+        PF& q = (k & 1) ? qb : qa;
+        {
+            const bool c0red = ((par + t + 1) & 1) == 0;
+            vr[k] = c0red ? q.x0 : q.x1;
+            vb[k] = c0red ? q.x1 : q.x0;
+            fr[k] = c0red ? q.f0 : q.f1;
+            fb[k] = c0red ? q.f1 : q.f0;
+        }
+        // publish the opposite-colour value of each stage's plane
+#pragma unroll
+        for (int s = 1; s <= S; s++) {
+            const int q_s = (k - s + 2 * NV) % NV;  // plane t+1-s
+            nbuf[s - 1][ty][tx] = ((s - 1) & 1) ? vr[q_s] : vb[q_s];
+        }
+        constexpr int qo = (k - S - 1 + 2 * NV) % NV;  // slot of plane t-S
+        if constexpr (RES) {
+            fbuf[0][ty][tx] = vr[qo];
+            fbuf[1][ty][tx] = vb[qo];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 1; s <= S; s++) {
+            const int p = t - s + 1;
+            const int q_s = (k - s + 2 * NV) % NV;          // plane p
+            const int q_lo = (k - s - 1 + 2 * NV) % NV;     // plane p-1
+            const int q_hi = (k - s + 1 + 2 * NV) % NV;     // plane p+1
+            const bool cs = ((s - 1) & 1) != 0;              // colour updated by this stage
+            const int u = (par + p + (cs ? 1 : 0)) & 1;      // its position in the pair
+            const bool valid = p >= zlo && p <= zhi && okj && (u ? ok1 : ok0);
+            const T other = cs ? vr[q_s] : vb[q_s];
+            const T zlo_v = cs ? vr[q_lo] : vb[q_lo];
+            const T zhi_v = cs ? vr[q_hi] : vb[q_hi];
+            const T xm = u ? other : nbuf[s - 1][ty][txm];
+            const T xp = u ? nbuf[s - 1][ty][txp] : other;
+            const T ym = nbuf[s - 1][tym][tx], yp = nbuf[s - 1][typ][tx];
+            auto nb = [&](int d, int side) -> T {
+                if (d == 0) return side == 0 ? xm : xp;
+                if (d == 1) return side == 0 ? ym : yp;
+                return side == 0 ? zlo_v : zhi_v;
+            };
+            T dg, of;
+            stencil_nb<T, BM, 3, decltype(nb), true>(L, u ? ic1c : ic0c, jcc, min(max(p, 0), L.nz - 1), nb,
+                                                      dg, of);
+            if (cs) {
+                const T nv = (fb[q_s] + of) / dg;
+                vb[q_s] = valid ? nv : vb[q_s];
+            } else {
+                const T nv = (fr[q_s] + of) / dg;
+                vr[q_s] = valid ? nv : vr[q_s];
+            }
+        }
+        const int po = t - S;  // final from here on
+        const bool outp = po >= z0 && po < zout;
+        if constexpr (RES) {
+            constexpr int qlo = (qo - 1 + NV) % NV, qhi = (qo + 1) % NV;
+            const int poc = min(max(po, 0), L.nz - 1);
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const int u = (par + po + c) & 1;
+                const bool valid = outp && (u ? oc1 : oc0);
+                const T own = c ? vb[qo] : vr[qo];
+                const T other = c ? vr[qo] : vb[qo];
+                const T zlo_v = c ? vr[qlo] : vb[qlo];
+                const T zhi_v = c ? vr[qhi] : vb[qhi];
+                const T xm = u ? other : fbuf[1 - c][ty][txm];
+                const T xp = u ? fbuf[1 - c][ty][txp] : other;
+                const T ym = fbuf[1 - c][tym][tx], yp = fbuf[1 - c][typ][tx];
+                auto nb = [&](int d, int side) -> T {
+                    if (d == 0) return side == 0 ? xm : xp;
+                    if (d == 1) return side == 0 ? ym : yp;
+                    return side == 0 ? zlo_v : zhi_v;
+                };
+                T dg, of;
+                stencil_nb<T, BM, 3, decltype(nb), true>(L, u ? ic1c : ic0c, jcc, poc, nb, dg, of);
+                const double r = (c ? fb[qo] : fr[qo]) - (dg * own - of);
+                double a = fabs(r);
+                if (a != a) a = __builtin_huge_val();
+                amax = valid ? fmax(amax, a) : amax;
+            }
+        }
+        {
+            const bool c0red = ((par + po) & 1) == 0;
+            buf_st<T>(ro, offs(pc0, outp && oc0, po), c0red ? vr[qo] : vb[qo]);
+            buf_st<T>(ro, offs(pc1, outp && oc1, po), c0red ? vb[qo] : vr[qo]);
+        }
+        load(q, t + 3);
+        __syncthreads();
+    };
+    load(qa, zlo);
+    load(qb, zlo + 1);
+    for (int t = zlo - 1; t < tend; t += NV) fused_unroll<0, NV>(step, t, tend);
+    if constexpr (RES) {
+        for (int o = 32; o > 0; o >>= 1) amax = fmax(amax, __shfl_xor(amax, o, 64));
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = amax;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = red[0];
+            for (int w = 1; w < (int)((blockDim.x + 63) >> 6); w++) m = fmax(m, red[w]);
+            part[lb] = m;
+        }
+    }
 }
 
 __device__ __forceinline__ void atomic_max_nonneg(unsigned long long* p, double v) {
@@ -241,7 +522,7 @@ __global__ void __launch_bounds__(256) k_resid0(LevV<double> L, const double* __
         stencil<double, BM, DIM>(L, x, i, j, k, dg, of);
         const int c = i + L.nx * (int)row;
         double v = f[c] - (dg * x[c] - of);
-        r[c] = v;
+        if (r) r[c] = v;
         a = fabs(v);
         if (a != a) a = __builtin_huge_val();
     }
@@ -631,11 +912,75 @@ static LevV<T> view(const MGWork& W, const Lev<T>& L, double alpha, double beta,
     v.bf2 = L.bf[2];
     v.dcell = nullptr;
     for (int d = 0; d < 3; d++)
-        for (int sd = 0; sd < 2; sd++) v.c0[d][sd] = L.c0[d][sd];
+        for (int sd = 0; sd < 2; sd++) {
+            v.c0[d][sd] = L.c0[d][sd];
+            v.c0u[d][sd] = (T)W.c0u[d][sd];
+        }
     v.per0 = W.per[0];
     v.per1 = W.per[1];
     v.per2 = W.per[2];
     return v;
+}
+
+// finest levels the fused smoother handles: 3-D, z not periodic, even
+// nx and ny, 32-bit cell offsets; VD_FUSED_SMOOTHER=0 turns it off
+static bool fused_capable(const MGWork& W) {
+    const char* e = getenv("VD_FUSED_SMOOTHER");  // read when a workspace is created
+    const bool env = !(e && strcmp(e, "0") == 0);
+    return env && W.dim == 3 && !W.per[2] && W.n[0] % 2 == 0 && W.n[1] % 2 == 0 && W.n[0] >= 8 &&
+           W.n[1] >= 8 && W.n[2] >= 2 && ncell(W.n) * 8 < ((int64_t)1 << 31);
+}
+
+// planes per block of the fused smoother (VD_FUSED_ZC overrides)
+static int fused_zc(int nz) {
+    static int env = -1;
+    if (env < 0) {
+        const char* e = getenv("VD_FUSED_ZC");
+        env = e ? atoi(e) : 0;
+    }
+    int zc = env > 0 ? env : 32;
+    return zc > nz ? nz : zc;
+}
+
+template <class T, int BM, int S, int RES>
+static void launch_fused_k(vd_ctx* ctx, const LevV<T>& v, const T* xin, T* xout, const T* rhs,
+                           double* part, int* nblk) {
+    // the kernel reads xin unless it starts from zero (ZI)
+    const int ntx = (v.nx + FZ_IX - 1) / FZ_IX, nty = (v.ny + FZ_IY - 1) / FZ_IY;
+    const int zc = fused_zc(v.nz), nzc = (v.nz + zc - 1) / zc;
+    const int nb = ntx * nty * nzc;
+    if (nblk) *nblk = nb;
+    if (xin)
+        hipLaunchKernelGGL((k_gsrb_fused<T, BM, S, RES, 0>), dim3((unsigned)nb), dim3(FZ_THREADS), 0, ctx->stream,
+                           v, xin, xout, rhs, zc, ntx, nty, part);
+    else if constexpr (RES == 0)
+        hipLaunchKernelGGL((k_gsrb_fused<T, BM, S, RES, 1>), dim3((unsigned)nb), dim3(FZ_THREADS), 0, ctx->stream,
+                           v, xin, xout, rhs, zc, ntx, nty, part);
+}
+
+// nsweep (1 or 2) red-black sweeps of x (zeros when xin is null) into
+// xout in one launch, for face-array (bm 0) or constant (bm 1) coefficients;
+// with res (FP64, one sweep) also the per-block
+// max-norm of the residual of the result into part[0..*nblk)
+template <class T>
+static void launch_fused(vd_ctx* ctx, const MGWork& W, int bm, const LevV<T>& v, const T* xin, T* xout,
+                         const T* rhs, int nsweep, int res, double* part, int* nblk) {
+    kt_begin(ctx, VD_KT_GSRB);
+#define VD_FUSED_BM(BMV)                                                                      \
+    if (nsweep == 2) launch_fused_k<T, BMV, 4, 0>(ctx, v, xin, xout, rhs, part, nblk);     \
+    else if constexpr (std::is_same<T, double>::value) {                                      \
+        if (res) launch_fused_k<T, BMV, 2, 1>(ctx, v, xin, xout, rhs, part, nblk);          \
+        else launch_fused_k<T, BMV, 2, 0>(ctx, v, xin, xout, rhs, part, nblk);              \
+    } else launch_fused_k<T, BMV, 2, 0>(ctx, v, xin, xout, rhs, part, nblk);
+    if (bm == 1) {
+        VD_FUSED_BM(1)
+    } else {
+        VD_FUSED_BM(0)
+    }
+#undef VD_FUSED_BM
+    // one pass: read x (unless zero), rhs and the coefficient data, write x
+    const int nb = bm == 1 ? 0 : (bm == 0 ? W.dim : 1);
+    kt_end(ctx, VD_KT_GSRB, 8.0 * (double)v.nx * v.ny * v.nz * ((xin ? 3 : 2) + nb));
 }
 
 template <class T>
@@ -653,6 +998,10 @@ static int alloc_levels(MGWork* W, std::vector<Lev<T>>& L, bool share0) {
         void* p;
         VD_TRY(dalloc(W, nc * sizeof(T), &p));
         v.e = (T*)p;
+        if (l == 0 && fused_capable(*W)) {
+            VD_TRY(dalloc(W, nc * sizeof(T), &p));
+            v.e2 = (T*)p;
+        }
         if (!(share0 && l == 0)) {
             VD_TRY(dalloc(W, nc * sizeof(T), &p));
             v.r = (T*)p;
@@ -727,13 +1076,22 @@ static int get_work(vd_ctx* ctx, const vd_geom* g, const vd_box* box, int prec, 
     int st = VD_OK;
     if ((st = dalloc(W, nc * 8, &p))) goto fail;
     W->x = (double*)p;
+    if (fused_capable(*W)) {
+        if ((st = dalloc(W, nc * 8, &p))) goto fail;
+        W->x2 = (double*)p;
+    }
     if ((st = dalloc(W, nc * 8, &p))) goto fail;
     W->f = (double*)p;
     if ((st = dalloc(W, nc * 8, &p))) goto fail;
     W->res = (double*)p;
     if ((st = dalloc(W, 4 * NSLOT * SLOT_STRIDE * sizeof(unsigned long long), &p))) goto fail;
     W->dnorm = (unsigned long long*)p;
-    if ((st = dalloc(W, ((n[0] + 255) / 256) * (int64_t)n[1] * n[2] * 8, &p))) goto fail;
+    {
+        // per-block partial norms of k_resid0 and of the fused smoother
+        int64_t np = ((n[0] + 255) / 256) * (int64_t)n[1] * n[2];
+        int64_t nf = (int64_t)((n[0] + FZ_IX - 1) / FZ_IX) * ((n[1] + FZ_IY - 1) / FZ_IY) * n[2];
+        if ((st = dalloc(W, std::max(np, nf) * 8, &p))) goto fail;
+    }
     W->part = (double*)p;
     if (!constb) {
         int64_t ng = (int64_t)(n[0] + 2) * (n[1] + 2) * (g->dim == 3 ? n[2] + 2 : 1);
@@ -831,10 +1189,25 @@ static void vcycle(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L,
         Lev<T>& F = L[l];
         LevV<T> v = view<T>(W, F, alpha, beta, bconst);
         int64_t nc = ncell(F.n);
-        hipLaunchKernelGGL(k_zero<T>, grid1d(nc, 256), dim3(256), 0, ctx->stream, F.e, nc);
-        for (int s = 0; s < nu1; s++) {
-            launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 0, l == 0);
-            launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 1, l == 0);
+        if (F.e2 && W.c0uni && bm != 2) {
+            // from a zero correction: the first launch writes e without reading it
+            for (int s = 0; s < nu1;) {
+                const int k = nu1 - s >= 2 ? 2 : 1;
+                if (s == 0) {
+                    launch_fused<T>(ctx, W, bm, v, nullptr, F.e, F.r, k, 0, nullptr, nullptr);
+                } else {
+                    launch_fused<T>(ctx, W, bm, v, F.e, F.e2, F.r, k, 0, nullptr, nullptr);
+                    std::swap(F.e, F.e2);
+                }
+                s += k;
+            }
+            if (nu1 == 0) hipLaunchKernelGGL(k_zero<T>, grid1d(nc, 256), dim3(256), 0, ctx->stream, F.e, nc);
+        } else {
+            hipLaunchKernelGGL(k_zero<T>, grid1d(nc, 256), dim3(256), 0, ctx->stream, F.e, nc);
+            for (int s = 0; s < nu1; s++) {
+                launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 0, l == 0);
+                launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 1, l == 0);
+            }
         }
         if (bm == 1) launch_resid_restrict<T, 1>(ctx, W.dim, v, F, L[l + 1]);
         else launch_resid_restrict<T, 0>(ctx, W.dim, v, F, L[l + 1]);
@@ -859,9 +1232,18 @@ static void vcycle(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L,
             hipLaunchKernelGGL((k_prolong<T, 3>), g, b, 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.n[2], F.e, gx);
         else
             hipLaunchKernelGGL((k_prolong<T, 2>), g, b, 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.n[2], F.e, gx);
-        for (int s = 0; s < nu2; s++) {
-            launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 0, l == 0);
-            launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 1, l == 0);
+        if (F.e2 && W.c0uni && bm != 2) {
+            for (int s = 0; s < nu2;) {
+                const int k = nu2 - s >= 2 ? 2 : 1;
+                launch_fused<T>(ctx, W, bm, v, F.e, F.e2, F.r, k, 0, nullptr, nullptr);
+                std::swap(F.e, F.e2);
+                s += k;
+            }
+        } else {
+            for (int s = 0; s < nu2; s++) {
+                launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 0, l == 0);
+                launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 1, l == 0);
+            }
         }
     }
 }
@@ -921,7 +1303,7 @@ static int read_groups(vd_ctx* ctx, MGWork& W, int ngroups, double* out) {
 // residual of the finest level into W.res and its max-norm; with bnorm
 // also the max-norm of f (the rhs with boundary data)
 static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double* nrm_out,
-                  double* bnorm = nullptr) {
+                  double* bnorm = nullptr, bool write_res = true) {
     zero_groups(ctx, W, 2);
     int64_t nc = ncell(W.n);
     if (bnorm)
@@ -932,20 +1314,21 @@ static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double
     dim3 g((unsigned)nblk), b(256);
     unsigned long long* sl = slot_group(W, 0);
     double* part = W.part;
+    double* res = write_res ? W.res : nullptr;  // the norm alone when no V-cycle follows
     kt_begin(ctx, VD_KT_RESID);
     if (bm == 1) {
-        if (W.dim == 3) hipLaunchKernelGGL((k_resid0<1, 3>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, part, gx);
-        else hipLaunchKernelGGL((k_resid0<1, 2>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, part, gx);
+        if (W.dim == 3) hipLaunchKernelGGL((k_resid0<1, 3>), g, b, 0, ctx->stream, v0, W.x, W.f, res, part, gx);
+        else hipLaunchKernelGGL((k_resid0<1, 2>), g, b, 0, ctx->stream, v0, W.x, W.f, res, part, gx);
     } else if (bm == 0) {
-        if (W.dim == 3) hipLaunchKernelGGL((k_resid0<0, 3>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, part, gx);
-        else hipLaunchKernelGGL((k_resid0<0, 2>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, part, gx);
+        if (W.dim == 3) hipLaunchKernelGGL((k_resid0<0, 3>), g, b, 0, ctx->stream, v0, W.x, W.f, res, part, gx);
+        else hipLaunchKernelGGL((k_resid0<0, 2>), g, b, 0, ctx->stream, v0, W.x, W.f, res, part, gx);
     } else {
-        if (W.dim == 3) hipLaunchKernelGGL((k_resid0<2, 3>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, part, gx);
-        else hipLaunchKernelGGL((k_resid0<2, 2>), g, b, 0, ctx->stream, v0, W.x, W.f, W.res, part, gx);
+        if (W.dim == 3) hipLaunchKernelGGL((k_resid0<2, 3>), g, b, 0, ctx->stream, v0, W.x, W.f, res, part, gx);
+        else hipLaunchKernelGGL((k_resid0<2, 2>), g, b, 0, ctx->stream, v0, W.x, W.f, res, part, gx);
     }
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, ctx->stream, part, nblk, sl);
     const int nb = bm == 1 ? 0 : (bm == 0 ? W.dim : 1);
-    kt_end(ctx, VD_KT_RESID, 8.0 * (double)nc * (4 + nb));
+    kt_end(ctx, VD_KT_RESID, 8.0 * (double)nc * ((write_res ? 4 : 3) + nb));
     VD_CHECK_LAUNCH();
     double m[2];
     VD_TRY(read_groups(ctx, W, bnorm ? 2 : 1, m));
@@ -1000,6 +1383,13 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
                       const double* dcell, const int tlo[3], const int thi[3], bool smooth_only,
                       double reltol, double abstol, int maxiter, int nu1, int nu2, vd_solve_info* info) {
     int64_t nc = ncell(W.n);
+    W.c0uni = 1;
+    for (int d = 0; d < 3; d++)
+        for (int sd = 0; sd < 2; sd++) {
+            const int ty = sd ? thi[d] : tlo[d];
+            W.c0u[d][sd] = ty == VD_DIRCBC ? -1.0 : 1.0;
+            if (d < W.dim && !W.per[d] && ty == VD_ROBINBC) W.c0uni = 0;
+        }
     if (!smooth_only) {
         // coarse c0 depends only on the BC types unless a side is Robin
         bool robin = false, same = W.c0_valid;
@@ -1045,12 +1435,25 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
     LevV<double> v0 = view<double>(W, fine, alpha, beta, bconst);
     v0.dcell = dcell;
     double r0, rn, bnorm;
-    VD_TRY(resid0(ctx, W, bm, v0, &r0, &bnorm));
+    VD_TRY(resid0(ctx, W, bm, v0, &r0, &bnorm, !smooth_only));
     double target = std::max(abstol, reltol * std::max(bnorm, r0));
     rn = r0;
     int it = 0, st = VD_OK;
     if (!(r0 <= target)) {
         for (it = 1; it <= maxiter; it++) {
+            if (smooth_only && W.x2 && W.c0uni && bm != 2) {
+                // one red-black sweep and the residual norm of its result in one pass
+                zero_groups(ctx, W, 1);
+                int nblk = 0;
+                launch_fused<double>(ctx, W, bm, v0, W.x, W.x2, W.f, 1, 1, W.part, &nblk);
+                std::swap(W.x, W.x2);
+                hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, ctx->stream, W.part, nblk,
+                                   slot_group(W, 0));
+                VD_CHECK_LAUNCH();
+                VD_TRY(read_groups(ctx, W, 1, &rn));
+                if (rn <= target) break;
+                continue;
+            }
             if (smooth_only) {
                 launch_gsrb<double>(ctx, W, bm, v0, W.x, W.f, 0, true);
                 launch_gsrb<double>(ctx, W, bm, v0, W.x, W.f, 1, true);
@@ -1066,7 +1469,7 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
                                    W.x, W.L32[0].e, nc);
             }
             VD_CHECK_LAUNCH();
-            VD_TRY(resid0(ctx, W, bm, v0, &rn));
+            VD_TRY(resid0(ctx, W, bm, v0, &rn, nullptr, !smooth_only));
             if (rn <= target) break;
         }
         if (it > maxiter) {
