@@ -30,6 +30,10 @@ sys.path.insert(0, ROOT)
 
 # Canonical algorithmic bytes per launch (FP64 = 8 B per value, N = cells
 # of the level the launch runs on), accumulated by the library per launch:
+#  fused     the fused red-black smoother (two sweeps, or one sweep plus
+#            the residual norm, in one pass): it must read x (unless the
+#            correction starts from zero), rhs and the coefficient data
+#            once and write x once, 8*N*(3 + n_b) (8*N*(2 + n_b) from zero)
 #  gsrb      one colour of a red-black sweep on the finest level: a full
 #            sweep reads phi, rhs and the coefficient data and writes phi,
 #            8*N*(3 + n_b); one colour is half (n_b = 0 for the constant-
@@ -46,17 +50,18 @@ def log(*a):
     print("bench:", *a, flush=True)
 
 
-def roofline_from_timing(dev, kernel):
-    ms_total, launches, kbytes = dev.kernel_timing()
-    if launches == 0:
+def pmc_traffic(kernel, n, precision):
+    """HBM bytes per launch of the roofline kernel from the committed
+    rocprofv3 --pmc summary of the same workload (FETCH_SIZE and
+    WRITE_SIZE passes, scaled by a copy kernel of known size), or None."""
+    path = os.path.join(ROOT, "profiles", "%s_%d_fp%d_pmc_traffic.md" % (kernel, n, 64 if precision == 0 else 32))
+    if not os.path.exists(path):
         return None
-    avg_s = ms_total / 1e3 / launches
-    byts = kbytes / launches
-    ach = byts / avg_s / 1e9
-    return {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": kernel,
-            "launches": launches, "avg_launch_us": round(avg_s * 1e6, 2),
-            "algorithmic_bytes_per_launch": round(byts)}
+    with open(path) as f:
+        for line in f:
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    return None
 
 
 def run_steps(native, abi, dev, case, steps, warmup, family=None):
@@ -214,7 +219,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--precision", type=int, default=0, help="headline: 0 FP64 MG, 1 FP32 V-cycles in FP64 IR")
-    ap.add_argument("--kernel", default="gsrb", choices=["gsrb", "transport", "chem", "resid", "none"])
+    ap.add_argument("--kernel", default="fused", choices=["fused", "gsrb", "transport", "chem", "resid", "none"])
     ap.add_argument("--cpu-n", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
@@ -237,7 +242,7 @@ def main():
         return c
 
     dev = native.Device(0)
-    fam = {"gsrb": abi.KT_GSRB, "transport": abi.KT_TRANSPORT, "chem": abi.KT_CHEM,
+    fam = {"fused": abi.KT_FUSED, "gsrb": abi.KT_GSRB, "transport": abi.KT_TRANSPORT, "chem": abi.KT_CHEM,
            "resid": abi.KT_RESID, "none": None}[args.kernel]
     case = case_fn(args.n)
     log("headline: %d^3 air-like discharge, %d warmup + %d timed steps" % (args.n, args.warmup, args.steps))
@@ -254,7 +259,8 @@ def main():
         byts = kbytes / launches
         ach = byts / avg_s / 1e9
         roof = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None, "kernel": args.kernel,
+                "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": pmc_traffic(args.kernel, args.n, args.precision), "kernel": args.kernel,
                 "launches": launches, "avg_launch_us": round(avg_s * 1e6, 2),
                 "algorithmic_bytes_per_launch": round(byts)}
 

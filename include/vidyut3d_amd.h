@@ -358,11 +358,12 @@ int vd_mlmg_solve(vd_ctx* ctx, const vd_mlmg_problem* prob, vd_array4* phi, cons
  * back with vd_ctx_kernel_timing (synchronises). */
 enum {
     VD_KT_NONE = 0,
-    VD_KT_GSRB = 1,       /* multigrid red-black smoother, finest level */
+    VD_KT_GSRB = 1,       /* multigrid red-black colour sweep, finest level */
     VD_KT_TRANSPORT = 2,  /* fused WENO transport rhs */
     VD_KT_CHEM = 3,       /* per-cell chemistry source */
     VD_KT_RESID = 4,      /* multigrid residual, finest level */
-    VD_KT_FILL = 5        /* ghost fill */
+    VD_KT_FILL = 5,       /* ghost fill */
+    VD_KT_FUSED = 6       /* fused multi-sweep red-black smoother, finest level */
 };
 int vd_ctx_set_kernel_timing(vd_ctx* ctx, int family);
 int vd_ctx_kernel_timing(vd_ctx* ctx, double* total_ms, long long* launches, double* bytes);

@@ -384,3 +384,34 @@ def test_mixed_precision_step_within_1e8_of_fp64_oracle(dev):
     print({q: "%.2e" % e for q, e in errs.items()})
     bad = {q: e for q, e in errs.items() if not e <= 1e-8}
     assert not bad, bad
+
+
+# ---------------------------------------------------------- plotfile output
+def test_plotfile_from_sim_reads_back_and_matches_oracle(dev, tmp_path):
+    """WritePlotFile (Source/Plot.cpp:29-38) of the device state reads back
+    bit-exact, and the reference comparator metric
+    (py_utils/compare_fields_from_twoplotfiles.py:44-46) against the
+    oracle's state written as a plotfile stays within 1e-10."""
+    from vidyut3d_amd import plotfile
+    c = cases.discharge_2d(n=32, nsteps=5)
+    tight(c.params, 1e-12)
+    ref, tref = O.evolve(c, 5)
+    sim = native.Sim(dev, c)
+    sim.evolve(5)
+    p = str(tmp_path / "plt00005")
+    sim.write_plotfile(p)
+    pf = plotfile.read(p)
+    assert np.array_equal(pf.data, sim.get_state())
+    names = [c.mech.names[s].value.decode() for s in range(c.mech.nspec)] + abi.PLASMA_VAR_NAMES
+    assert pf.names == names and pf.step == 5 and pf.dim == 2
+    assert pf.time == pytest.approx(tref, rel=1e-15)
+    assert pf.prob_lo == [c.geom.prob_lo[0], c.geom.prob_lo[1]]
+    q = str(tmp_path / "oracle00005")
+    plotfile.write(q, names, ref, 2, tref, 5, pf.prob_lo, pf.prob_hi)
+    errs = plotfile.compare(q, p)
+    vec = np.sqrt(np.mean(ref[abi.comp(3)["EFX"]] ** 2 + ref[abi.comp(3)["EFY"]] ** 2))
+    for name, e in errs.items():
+        if name in ("Efieldx", "Efieldy"):
+            assert e["err"] <= 1e-10 * vec, (name, e)  # component of a vector: scale by |E|
+        else:
+            assert e["rel"] <= 1e-10, (name, e)

@@ -18,7 +18,7 @@ VD_MAX_REACTIONS = 256
 VD_MAX_REACTANTS = 3
 VD_MAX_PRODUCTS = 4
 NUM_PLASMAVARS = 10
-KT_NONE, KT_GSRB, KT_TRANSPORT, KT_CHEM, KT_RESID, KT_FILL = range(6)
+KT_NONE, KT_GSRB, KT_TRANSPORT, KT_CHEM, KT_RESID, KT_FILL, KT_FUSED = range(7)
 
 # Source/UnivConstants.H:4-19
 N_A = 6.02214076e23

@@ -965,7 +965,7 @@ static void launch_fused_k(vd_ctx* ctx, const LevV<T>& v, const T* xin, T* xout,
 template <class T>
 static void launch_fused(vd_ctx* ctx, const MGWork& W, int bm, const LevV<T>& v, const T* xin, T* xout,
                          const T* rhs, int nsweep, int res, double* part, int* nblk) {
-    kt_begin(ctx, VD_KT_GSRB);
+    kt_begin(ctx, VD_KT_FUSED);
 #define VD_FUSED_BM(BMV)                                                                      \
     if (nsweep == 2) launch_fused_k<T, BMV, 4, 0>(ctx, v, xin, xout, rhs, part, nblk);     \
     else if constexpr (std::is_same<T, double>::value) {                                      \
@@ -980,7 +980,7 @@ static void launch_fused(vd_ctx* ctx, const MGWork& W, int bm, const LevV<T>& v,
 #undef VD_FUSED_BM
     // one pass: read x (unless zero), rhs and the coefficient data, write x
     const int nb = bm == 1 ? 0 : (bm == 0 ? W.dim : 1);
-    kt_end(ctx, VD_KT_GSRB, 8.0 * (double)v.nx * v.ny * v.nz * ((xin ? 3 : 2) + nb));
+    kt_end(ctx, VD_KT_FUSED, (double)sizeof(T) * v.nx * v.ny * v.nz * ((xin ? 3 : 2) + nb));
 }
 
 template <class T>
