@@ -44,6 +44,7 @@ struct LevV {  // kernel view of one level
     const T* dcell;      // BM 2: cell coefficient on the grown box
     const T* c0[3][2];   // per boundary face c0 (non-periodic sides)
     T c0u[3][2];         // c0 of sides whose c0 is uniform (no Robin side)
+    int c0uni;           // 1: c0u holds every boundary c0 (no c0 loads)
     int per0, per1, per2;
 };
 
@@ -127,10 +128,12 @@ __device__ __forceinline__ int tidx(const LevV<T>& L, int d, int i, int j, int k
     return d == 0 ? j + L.ny * k : (d == 1 ? i + L.nx * k : i + L.nx * j);
 }
 
-// harmonic face average, the formula of the face-array setup kernel
+// harmonic face average 2ab/(a+b) (ScalarSolve.cpp:511-512), 0 when
+// a+b = 0; one formula for the face arrays and the on-the-fly faces
 template <class T>
 __device__ __forceinline__ T harm(T a, T b) {
-    return (a + b == T(0)) ? T(0) : T(2) * a * b / (a + b);
+    const T s = a + b;
+    return (s == T(0)) ? T(0) : fdiv(T(2) * a * b, s);
 }
 
 // face coefficients of cell (i,j,k) in direction d
@@ -188,18 +191,59 @@ __device__ __forceinline__ void stencil_nb(const LevV<T>& L, int i, int j, int k
     ofo = of;
 }
 
+// dg and of of cell (i,j,k) from x in memory.  Every load is issued
+// unconditionally (neighbour index clamped to the cell itself on a
+// non-periodic boundary, where the value is not used, wrapped on a periodic
+// one) and the boundary terms are selected afterwards: loads under
+// divergent branches would be waited for branch by branch.  The arithmetic
+// is that of stencil_nb term for term.
 template <class T, int BM, int DIM>
 __device__ __forceinline__ void stencil(const LevV<T>& L, const T* __restrict__ x, int i, int j,
                                         int k, T& dgo, T& ofo) {
     const int c = i + L.nx * (j + L.ny * k);
-    auto nb = [&](int d, int side) -> T {
+    T xl[DIM], xh[DIM];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
         const int cd = d == 0 ? i : (d == 1 ? j : k);
         const int nd = d == 0 ? L.nx : (d == 1 ? L.ny : L.nz);
         const int st = d == 0 ? 1 : (d == 1 ? L.nx : L.nx * L.ny);
-        if (side == 0) return cd > 0 ? x[c - st] : x[c + (nd - 1) * st];
-        return cd < nd - 1 ? x[c + st] : x[c - (nd - 1) * st];
-    };
-    stencil_nb<T, BM, DIM>(L, i, j, k, nb, dgo, ofo);
+        const int per = d == 0 ? L.per0 : (d == 1 ? L.per1 : L.per2);
+        const int il = cd > 0 ? c - st : (per ? c + (nd - 1) * st : c);
+        const int ih = cd < nd - 1 ? c + st : (per ? c - (nd - 1) * st : c);
+        xl[d] = x[il];
+        xh[d] = x[ih];
+    }
+    T dg = L.alpha, of = T(0);
+#pragma unroll
+    for (int d = 0; d < DIM; d++) {
+        const int cd = d == 0 ? i : (d == 1 ? j : k);
+        const int nd = d == 0 ? L.nx : (d == 1 ? L.ny : L.nz);
+        const int per = d == 0 ? L.per0 : (d == 1 ? L.per1 : L.per2);
+        const T s = d == 0 ? L.s0 : (d == 1 ? L.s1 : L.s2);
+        T blo, bhi;
+        face_coef<T, BM, DIM>(L, d, i, j, k, blo, bhi);
+        T c0l = T(1), c0h = T(1);
+        if (!per) {
+            if (L.c0uni) {
+                c0l = L.c0u[d][0];
+                c0h = L.c0u[d][1];
+            } else {
+                const int t = tidx(L, d, i, j, k);
+                c0l = L.c0[d][0][t];
+                c0h = L.c0[d][1][t];
+            }
+        }
+        const bool lin = cd > 0 || per, hin = cd < nd - 1 || per;
+        T dd = T(0), oo = T(0);
+        dd = lin ? dd + blo : dd + blo * (T(1) - c0l);
+        oo = lin ? oo + blo * xl[d] : oo;
+        dd = hin ? dd + bhi : dd + bhi * (T(1) - c0h);
+        oo = hin ? oo + bhi * xh[d] : oo;
+        dg += s * dd;
+        of += s * oo;
+    }
+    dgo = dg;
+    ofo = of;
 }
 
 // ------------------------------------------------------------ kernels
@@ -681,7 +725,7 @@ __global__ void __launch_bounds__(128) k_prolong(LevV<T> C, const T* __restrict_
                 for (int d = 0; d < DIM; d++)
                     if (sel[d] == 1 && ghost[d]) {
                         int side = f[d] & 1;
-                        v = v * C.c0[d][side][tidx(C, d, cc0, cc1, cc2)];
+                        v = v * (C.c0uni ? C.c0u[d][side] : C.c0[d][side][tidx(C, d, cc0, cc1, cc2)]);
                     }
                 T w = (ix == 0 ? T(0.75) : T(0.25)) * (jy == 0 ? T(0.75) : T(0.25)) *
                       (DIM == 3 ? (kz == 0 ? T(0.75) : T(0.25)) : T(1));
@@ -916,6 +960,7 @@ static LevV<T> view(const MGWork& W, const Lev<T>& L, double alpha, double beta,
             v.c0[d][sd] = L.c0[d][sd];
             v.c0u[d][sd] = (T)W.c0u[d][sd];
         }
+    v.c0uni = W.c0uni;
     v.per0 = W.per[0];
     v.per1 = W.per[1];
     v.per2 = W.per[2];
@@ -1841,18 +1886,28 @@ __global__ void k_cell_diff(CellDArgs a, double* __restrict__ dcell) {
 }
 
 // harmonic face average of the cell coefficient (ScalarSolve.cpp:511-512)
-__global__ void k_harmonic_faces(const double* __restrict__ dcell, int nx, int ny, int nz, int d, int dim,
-                                 double* __restrict__ bf) {
+// harmonic face coefficient of direction d from the cell coefficient on the
+// grown box (ScalarSolve.cpp:511-512), plus the block max of the faces into
+// a norm slot group (Jacobi bound of the implicit operator)
+__global__ void __launch_bounds__(256) k_harmonic_faces(const double* __restrict__ dcell, int nx, int ny,
+                                                        int nz, int d, int dim, double* __restrict__ bf,
+                                                        unsigned long long* __restrict__ slots) {
     int n0 = nx + (d == 0), n1 = ny + (d == 1), n2 = nz + (d == 2);
     unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (unsigned)n0 * n1 * n2) return;
-    int i = (int)(t % n0);
-    unsigned r = t / n0;
-    int j = (int)(r % n1), k = (int)(r / n1);
-    const int gx = nx + 2, gxy = gx * (ny + 2);
-    const int gs = d == 0 ? 1 : (d == 1 ? gx : gxy);
-    const int gc = (i + 1) + gx * (j + 1) + (dim == 3 ? gxy * (k + 1) : 0);
-    bf[t] = harm(dcell[gc - gs], dcell[gc]);
+    double m = 0.0;
+    if (t < (unsigned)n0 * n1 * n2) {
+        int i = (int)(t % n0);
+        unsigned r = t / n0;
+        int j = (int)(r % n1), k = (int)(r / n1);
+        const int gx = nx + 2, gxy = gx * (ny + 2);
+        const int gs = d == 0 ? 1 : (d == 1 ? gx : gxy);
+        const int gc = (i + 1) + gx * (j + 1) + (dim == 3 ? gxy * (k + 1) : 0);
+        const double b = harm(dcell[gc - gs], dcell[gc]);
+        bf[t] = b;
+        m = fabs(b);
+        if (m != m) m = __builtin_huge_val();
+    }
+    if (slots) block_max_atomic(m, slots);
 }
 
 struct SpecBcArgs {
@@ -2052,13 +2107,16 @@ int implicit_solve(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sb,
     hipLaunchKernelGGL(k_spec_rhs, grid1d(nc, 256), dim3(256), 0, ctx->stream, r, W->f, W->x);
     VD_CHECK_LAUNCH();
     const double alpha = 1.0 / dt, beta = 1.0;
+    // the Jacobi bound rho of the operator (harmonic face maxima) decides
+    // between smoother-only iterations on the cell coefficient and full
+    // V-cycles on face arrays
     double rho;
     VD_TRY(jacobi_rho(ctx, *W, 2, alpha, beta, 0.0, W->dcell, &rho));
     const bool smooth = rho <= SMOOTH_ONLY_RHO;
     if (!smooth)
         for (int d = 0; d < g->dim; d++)
             hipLaunchKernelGGL(k_harmonic_faces, grid1d(nfacef(n, d), 256), dim3(256), 0, ctx->stream, W->dcell,
-                               n[0], n[1], n[2], d, g->dim, W->bf0[d]);
+                               n[0], n[1], n[2], d, g->dim, W->bf0[d], (unsigned long long*)nullptr);
     VD_TRY(side_setup(ctx, *W, bclo, bchi, beta, 0.0, W->dcell));
     int st = mg_iterate(ctx, *W, smooth ? 2 : 0, alpha, beta, 0.0, W->dcell, bclo, bchi, smooth,
                         prm->linsolve_reltol, prm->linsolve_abstol, prm->linsolve_maxiter,

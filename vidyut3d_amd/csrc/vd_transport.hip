@@ -13,18 +13,6 @@ int fill_value(vd_ctx* ctx, double* p, int64_t n, double v);
 
 __device__ __forceinline__ double sq(double x) { return x * x; }
 
-// a/b from the hardware reciprocal with two Newton steps and a final
-// residual correction (within an ulp of the rounded quotient for the
-// positive, normal denominators of the WENO weights); a quarter of the
-// instructions of the IEEE division sequence
-__device__ __forceinline__ double fdiv(double a, double b) {
-    double r = __builtin_amdgcn_rcp(b);
-    r = fma(fma(-b, r, 1.0), r, r);
-    r = fma(fma(-b, r, 1.0), r, r);
-    const double q = a * r;
-    return fma(fma(-b, q, a), r, q);
-}
-
 // gk (compute_explicit_flux.H:16-23)
 __device__ __forceinline__ double gk(double wk, double g) {
     return fdiv(wk * (g + g * g - 3.0 * g * wk + wk * wk), g * g + wk * (1 - 2.0 * g));
