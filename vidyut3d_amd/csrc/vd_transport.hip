@@ -53,6 +53,58 @@ __device__ __forceinline__ double weno5(double im2, double im1, double i0, doubl
     return fdiv(a1 * p1 + a2 * p2 + a3 * p3, a1 + a2 + a3);
 }
 
+// Both WENO reconstructions centred on cell c from (a, b, c, d, e) =
+// u[c-2 .. c+2]: L = weno(a,b,c,d,e), the value at face c+1/2 from the
+// left, and R = weno(e,d,c,b,a), the value at face c-1/2 from the right.
+// The mirrored call has the same smoothness indicators in reverse order
+// (beta'_1 = beta_3, beta'_2 = beta_2, beta'_3 = beta_1) and the same tau,
+// so they and the weight ratios are computed once for both.
+template <int S>
+__device__ __forceinline__ void weno5_lr(double a, double b, double c, double d, double e, double& L,
+                                         double& R) {
+    const double eps = 1e-20;
+    const double p1 = (1.0 / 3.0) * a - (7.0 / 6.0) * b + (11.0 / 6.0) * c;
+    const double p2 = -(1.0 / 6.0) * b + (5.0 / 6.0) * c + (1.0 / 3.0) * d;
+    const double p3 = (1.0 / 3.0) * c + (5.0 / 6.0) * d - (1.0 / 6.0) * e;
+    const double q1 = (1.0 / 3.0) * e - (7.0 / 6.0) * d + (11.0 / 6.0) * c;
+    const double q2 = -(1.0 / 6.0) * d + (5.0 / 6.0) * c + (1.0 / 3.0) * b;
+    const double q3 = (1.0 / 3.0) * c + (5.0 / 6.0) * b - (1.0 / 6.0) * a;
+    const double b1 = (13.0 / 12.0) * sq(a - 2.0 * b + c) + (1.0 / 4.0) * sq(a - 4.0 * b + 3.0 * c);
+    const double b2 = (13.0 / 12.0) * sq(b - 2.0 * c + d) + (1.0 / 4.0) * sq(b - d);
+    const double b3 = (13.0 / 12.0) * sq(c - 2.0 * d + e) + (1.0 / 4.0) * sq(3.0 * c - 4.0 * d + e);
+    const double g1 = 1.0 / 10.0, g2 = 6.0 / 10.0, g3 = 3.0 / 10.0;
+    double x1, x2, x3, y1, y2, y3;
+    if (S == 3) {
+        const double tau = fabs(b1 - b3);
+        const double r1 = sq(fdiv(tau, b1 + eps)), r2 = sq(fdiv(tau, b2 + eps)), r3 = sq(fdiv(tau, b3 + eps));
+        x1 = g1 * (1.0 + r1);
+        x2 = g2 * (1.0 + r2);
+        x3 = g3 * (1.0 + r3);
+        y1 = g1 * (1.0 + r3);
+        y2 = g2 * (1.0 + r2);
+        y3 = g3 * (1.0 + r1);
+    } else {
+        const double i1 = fdiv(1.0, sq(b1 + eps)), i2 = fdiv(1.0, sq(b2 + eps)), i3 = fdiv(1.0, sq(b3 + eps));
+        x1 = g1 * i1;
+        x2 = g2 * i2;
+        x3 = g3 * i3;
+        y1 = g1 * i3;
+        y2 = g2 * i2;
+        y3 = g3 * i1;
+        if (S == 2) {
+            const double sx = fdiv(1.0, x1 + x2 + x3), sy = fdiv(1.0, y1 + y2 + y3);
+            x1 = gk(x1 * sx, g1);
+            x2 = gk(x2 * sx, g2);
+            x3 = gk(x3 * sx, g3);
+            y1 = gk(y1 * sy, g1);
+            y2 = gk(y2 * sy, g2);
+            y3 = gk(y3 * sy, g3);
+        }
+    }
+    L = fdiv(x1 * p1 + x2 * p2 + x3 * p3, x1 + x2 + x3);
+    R = fdiv(y1 * q1 + y2 * q2 + y3 * q3, y1 + y2 + y3);
+}
+
 __device__ __forceinline__ double slope_lim(double c, double dl, double dm, double dr) {
     double r = (c >= 0.0) ? dl / (dm + VD_HEPS) : dr / (dm + VD_HEPS);
     double lim;
@@ -129,6 +181,134 @@ __global__ void __launch_bounds__(256) k_face_flux(FluxArgs a) {
     a.flux.p[a.flux.off(i, j, k)] = f;
 }
 
+// WENO-5 fluxes of direction D (1 = y, 2 = z) streamed along pencils: one
+// thread per (i, other) column and chunk of faces, x fastest across lanes
+// (coalesced).  Every cell of the pencil is reconstructed once (both
+// one-sided values of u and of f = u v), and each face flux combines the
+// left value of the cell below with the right value of the cell above
+// (compute_explicit_flux.H:262-312), so the per-face work of k_face_flux is
+// shared between neighbouring faces.
+struct PencilArgs {
+    Arr<double> sb, flux;
+    int specid, efcomp;
+    double mob;
+    int lo_zero, hi_zero;  // face index zeroed (IHNEU) or INT_MIN
+    int3 flo;              // first face index (box lo)
+    int nx, no, nd;        // cells along x, the other direction, and D
+    int nchunk, chunk;     // face chunks along D
+};
+
+template <int S, int D>
+__global__ void __launch_bounds__(256) k_pencil_flux(PencilArgs a) {
+    const unsigned t = lblock() * blockDim.x + threadIdx.x;
+    if (t >= (unsigned)a.nx * a.no * a.nchunk) return;
+    const int i = (int)(t % a.nx);
+    const unsigned r = t / a.nx;
+    const int o = (int)(r % a.no), ch = (int)(r / a.no);
+    const int f0 = ch * a.chunk;
+    const int f1 = ch == a.nchunk - 1 ? a.nd + 1 : f0 + a.chunk;  // faces [f0, f1)
+    // cell (i, o, m) along D -> Array4 offset
+    const int ig = a.flo.x + i;
+    const int jg = D == 1 ? 0 : a.flo.y + o;
+    const int kg = D == 1 ? a.flo.z + o : 0;
+    const int st = D == 1 ? a.sb.js : a.sb.ks;
+    const int mlo = D == 1 ? a.flo.y : a.flo.z;
+    const int base = D == 1 ? a.sb.off(ig, mlo, kg) : a.sb.off(ig, jg, mlo);
+    const double* __restrict__ u = a.sb.comp(a.specid);
+    const double* __restrict__ E = a.sb.comp(a.efcomp + D);
+    // window of cells c-2 .. c+2 (u, v and f = u v) for centre c
+    double w_u[5], w_f[5], w_v[5];
+    auto cell = [&](int m, double& uu, double& ff, double& vv) {
+        const int off = base + m * st;
+        uu = u[off];
+        vv = a.mob * E[off];
+        ff = uu * vv;
+    };
+#pragma unroll
+    for (int q = 0; q < 5; q++) cell(f0 - 3 + q, w_u[q], w_f[q], w_v[q]);
+    double Lu_prev = 0.0, Lf_prev = 0.0, v_prev = 0.0;
+    for (int c = f0 - 1; c < f1; c++) {
+        double Lu, Ru, Lf, Rf;
+        weno5_lr<S>(w_u[0], w_u[1], w_u[2], w_u[3], w_u[4], Lu, Ru);
+        weno5_lr<S>(w_f[0], w_f[1], w_f[2], w_f[3], w_f[4], Lf, Rf);
+        const double vc = w_v[2];
+        if (c >= f0) {
+            // face c between cells c-1 and c
+            const double lam = fabs(0.5 * (v_prev + vc));
+            double fl = 0.5 * (Lf_prev + lam * Lu_prev) + 0.5 * (Rf - lam * Ru);
+            const int fd = mlo + c;
+            if (fd == a.lo_zero || fd == a.hi_zero) fl = 0.0;
+            if (D == 1) a.flux.p[a.flux.off(ig, fd, kg)] = fl;
+            else a.flux.p[a.flux.off(ig, jg, fd)] = fl;
+        }
+        Lu_prev = Lu;
+        Lf_prev = Lf;
+        v_prev = vc;
+        if (c + 1 < f1) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                w_u[q] = w_u[q + 1];
+                w_f[q] = w_f[q + 1];
+                w_v[q] = w_v[q + 1];
+            }
+            cell(c + 3, w_u[4], w_f[4], w_v[4]);
+        }
+    }
+}
+
+// WENO-5 fluxes of the x direction: each wave reconstructs 64 consecutive
+// cells of one row once (both one-sided values of u and f) and forms the
+// fluxes of the 63 faces between them, taking the right-hand values of the
+// next cell from the neighbouring lane (wave shuffle).  Waves of a row
+// overlap by one cell.
+struct XLineArgs {
+    Arr<double> sb, flux;
+    int specid, efx;
+    double mob;
+    int lo_zero, hi_zero;
+    int3 flo;
+    int nx, nrow, nseg;  // cells along x, rows (y, z), 63-face segments per row
+    int ny;
+};
+
+template <int S>
+__global__ void __launch_bounds__(256) k_xline_flux(XLineArgs a) {
+    const unsigned wv = (lblock() * blockDim.x + threadIdx.x) >> 6;
+    const int lane = threadIdx.x & 63;
+    if (wv >= (unsigned)a.nrow * a.nseg) return;  // whole waves exit together
+    const int row = (int)(wv / a.nseg), seg = (int)(wv % a.nseg);
+    const int jg = a.flo.y + row % a.ny, kg = a.flo.z + row / a.ny;
+    const int c = seg * 63 - 1 + lane;  // centre cell, relative to box lo
+    const bool live = c <= a.nx;        // cells -1 .. nx are reconstructed
+    const double* __restrict__ u = a.sb.comp(a.specid);
+    const double* __restrict__ E = a.sb.comp(a.efx);
+    const int base = a.sb.off(a.flo.x, jg, kg);
+    double wu[5], wf[5], vc = 0.0;
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        const int m = min(c - 2 + q, a.nx + 2);  // clamped for idle lanes
+        const double uu = u[base + m], vv = a.mob * E[base + m];
+        wu[q] = uu;
+        wf[q] = uu * vv;
+        if (q == 2) vc = vv;
+    }
+    double Lu = 0.0, Ru = 0.0, Lf = 0.0, Rf = 0.0;
+    if (live) {
+        weno5_lr<S>(wu[0], wu[1], wu[2], wu[3], wu[4], Lu, Ru);
+        weno5_lr<S>(wf[0], wf[1], wf[2], wf[3], wf[4], Lf, Rf);
+    }
+    // face c+1 between this lane's cell and the next lane's
+    const double Ru1 = __shfl_down(Ru, 1, 64), Rf1 = __shfl_down(Rf, 1, 64), v1 = __shfl_down(vc, 1, 64);
+    const int f = c + 1;
+    if (lane < 63 && f >= 0 && f <= a.nx) {
+        const double lam = fabs(0.5 * (vc + v1));
+        double fl = 0.5 * (Lf + lam * Lu) + 0.5 * (Rf1 - lam * Ru1);
+        const int fd = a.flo.x + f;
+        if (fd == a.lo_zero || fd == a.hi_zero) fl = 0.0;
+        a.flux.p[a.flux.off(fd, jg, kg)] = fl;
+    }
+}
+
 struct DsdtArgs {
     Arr<double> f0, f1, f2, rxn, dsdt;
     int specid, dim;
@@ -159,6 +339,72 @@ template <int HYP, int S>
 static void launch_flux(vd_ctx* ctx, const FluxArgs& a) {
     int64_t tot = (int64_t)a.fnx * a.fny * a.fnz;
     hipLaunchKernelGGL((k_face_flux<HYP, S>), grid1d(tot, 256), dim3(256), 0, ctx->stream, a);
+}
+
+// chunk of faces per pencil thread along D: long enough to amortise the
+// 4-cell ramp, short enough to give the device >= ~8 waves per CU
+static int launch_pencil(vd_ctx* ctx, const FluxArgs& f, int d, int scheme, const vd_box* box) {
+    PencilArgs a;
+    a.sb = f.sb;
+    a.flux = f.flux;
+    a.specid = f.specid;
+    a.efcomp = f.efcomp;
+    a.mob = f.mob;
+    a.lo_zero = f.lo_zero;
+    a.hi_zero = f.hi_zero;
+    a.flo = f.flo;
+    a.nx = box->hi[0] - box->lo[0] + 1;
+    a.nd = box->hi[d] - box->lo[d] + 1;
+    a.no = d == 1 ? box->hi[2] - box->lo[2] + 1 : box->hi[1] - box->lo[1] + 1;
+    const int64_t cols = (int64_t)a.nx * a.no;
+    int chunk = 32;
+    while (chunk > 8 && cols * ((a.nd + 1) / chunk) < 256 * 64 * 8) chunk /= 2;
+    a.nchunk = (a.nd + 1) / chunk > 0 ? (a.nd + 1) / chunk : 1;
+    a.chunk = chunk;
+    const int64_t tot = cols * a.nchunk;
+    if (tot >= ((int64_t)1 << 32)) {
+        set_error("transport: box too large for one pencil launch");
+        return VD_EINVAL;
+    }
+    dim3 g = grid1d(tot, 256), b(256);
+    if (d == 1) {
+        if (scheme == 1) hipLaunchKernelGGL((k_pencil_flux<1, 1>), g, b, 0, ctx->stream, a);
+        else if (scheme == 2) hipLaunchKernelGGL((k_pencil_flux<2, 1>), g, b, 0, ctx->stream, a);
+        else hipLaunchKernelGGL((k_pencil_flux<3, 1>), g, b, 0, ctx->stream, a);
+    } else {
+        if (scheme == 1) hipLaunchKernelGGL((k_pencil_flux<1, 2>), g, b, 0, ctx->stream, a);
+        else if (scheme == 2) hipLaunchKernelGGL((k_pencil_flux<2, 2>), g, b, 0, ctx->stream, a);
+        else hipLaunchKernelGGL((k_pencil_flux<3, 2>), g, b, 0, ctx->stream, a);
+    }
+    VD_CHECK_LAUNCH();
+    return VD_OK;
+}
+
+static int launch_xline(vd_ctx* ctx, const FluxArgs& f, int scheme, const vd_box* box) {
+    XLineArgs a;
+    a.sb = f.sb;
+    a.flux = f.flux;
+    a.specid = f.specid;
+    a.efx = f.efcomp;
+    a.mob = f.mob;
+    a.lo_zero = f.lo_zero;
+    a.hi_zero = f.hi_zero;
+    a.flo = f.flo;
+    a.nx = box->hi[0] - box->lo[0] + 1;
+    a.ny = box->hi[1] - box->lo[1] + 1;
+    a.nrow = a.ny * (box->hi[2] - box->lo[2] + 1);
+    a.nseg = (a.nx + 1 + 62) / 63;
+    const int64_t threads = (int64_t)a.nrow * a.nseg * 64;
+    if (threads >= ((int64_t)1 << 32)) {
+        set_error("transport: box too large for one flux launch");
+        return VD_EINVAL;
+    }
+    dim3 g = grid1d(threads, 256), b(256);
+    if (scheme == 1) hipLaunchKernelGGL((k_xline_flux<1>), g, b, 0, ctx->stream, a);
+    else if (scheme == 2) hipLaunchKernelGGL((k_xline_flux<2>), g, b, 0, ctx->stream, a);
+    else hipLaunchKernelGGL((k_xline_flux<3>), g, b, 0, ctx->stream, a);
+    VD_CHECK_LAUNCH();
+    return VD_OK;
 }
 
 int check_transport_args(const vd_params* prm, int specid, int nspec) {
@@ -205,6 +451,11 @@ int transport_flux(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4
         a.fnz = box->hi[2] - box->lo[2] + 1 + (d == 2);
         a.dxmin = dxmin;
         a.dt = dt;
+        if (prm->hyp_order == 5) {
+            if (d > 0) VD_TRY(launch_pencil(ctx, a, d, prm->weno_scheme, box));
+            else VD_TRY(launch_xline(ctx, a, prm->weno_scheme, box));
+            continue;
+        }
         switch (prm->hyp_order) {
             case 1: launch_flux<1, 0>(ctx, a); break;
             case 2: launch_flux<2, 0>(ctx, a); break;
