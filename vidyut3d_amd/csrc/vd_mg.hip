@@ -653,17 +653,10 @@ __global__ void __launch_bounds__(256) k_face_max_cd(const double* __restrict__ 
     }
 }
 
-// coarse r = average of the 2^DIM fine residuals rhs - L e (fused);
-// 1-D grid of gx blocks per coarse row, XCD-remapped
+// coarse r = average of the 2^DIM fine residuals rhs - L e (fused)
 template <class T, int BM, int DIM>
-__global__ void __launch_bounds__(128) k_resid_restrict(LevV<T> F, const T* __restrict__ e,
-                                                        const T* __restrict__ rhs, int cnx, int cny,
-                                                        int cnz, T* __restrict__ rc, int gx) {
-    const unsigned lb = lblock();
-    const unsigned row = lb / gx;
-    const int J = (int)(row % cny), K = (int)(row / cny);
-    const int I = (int)(lb % gx) * blockDim.x + threadIdx.x;
-    if (I >= cnx) return;
+__device__ __forceinline__ T resid_restrict_cell(const LevV<T>& F, const T* __restrict__ e,
+                                                 const T* __restrict__ rhs, int I, int J, int K) {
     T s = T(0);
     const int kz = DIM == 3 ? 2 : 1;
     for (int kk = 0; kk < kz; kk++)
@@ -675,19 +668,26 @@ __global__ void __launch_bounds__(128) k_resid_restrict(LevV<T> F, const T* __re
                 int c = i + F.nx * (j + F.ny * k);
                 s += rhs[c] - (dg * e[c] - of);
             }
-    rc[I + cnx * (J + cny * K)] = s * (DIM == 3 ? T(0.125) : T(0.25));
+    return s * (DIM == 3 ? T(0.125) : T(0.25));
+}
+
+// 1-D grid of gx blocks per coarse row, XCD-remapped
+template <class T, int BM, int DIM>
+__global__ void __launch_bounds__(128) k_resid_restrict(LevV<T> F, const T* __restrict__ e,
+                                                        const T* __restrict__ rhs, int cnx, int cny,
+                                                        int cnz, T* __restrict__ rc, int gx) {
+    const unsigned lb = lblock();
+    const unsigned row = lb / gx;
+    const int J = (int)(row % cny), K = (int)(row / cny);
+    const int I = (int)(lb % gx) * blockDim.x + threadIdx.x;
+    if (I >= cnx) return;
+    rc[I + cnx * (J + cny * K)] = resid_restrict_cell<T, BM, DIM>(F, e, rhs, I, J, K);
 }
 
 // e_f += P e_c (linear, tensor 3/4-1/4; ghosts of the coarse correction
-// are c0 * interior, periodic wraps); 1-D grid of gx blocks per fine row
+// are c0 * interior, periodic wraps): the increment of fine cell (i,j,k)
 template <class T, int DIM>
-__global__ void __launch_bounds__(128) k_prolong(LevV<T> C, const T* __restrict__ ec, int fnx, int fny,
-                                                 int fnz, T* __restrict__ ef, int gx) {
-    const unsigned lb = lblock();
-    const unsigned row = lb / gx;
-    const int j = (int)(row % fny), k = (int)(row / fny);
-    const int i = (int)(lb % gx) * blockDim.x + threadIdx.x;
-    if (i >= fnx) return;
+__device__ __forceinline__ T prolong_cell(const LevV<T>& C, const T* __restrict__ ec, int i, int j, int k) {
     int f[3] = {i, j, k};
     int cn[3] = {C.nx, C.ny, C.nz};
     int pr[3] = {C.per0, C.per1, C.per2};
@@ -731,14 +731,26 @@ __global__ void __launch_bounds__(128) k_prolong(LevV<T> C, const T* __restrict_
                       (DIM == 3 ? (kz == 0 ? T(0.75) : T(0.25)) : T(1));
                 val += w * v;
             }
-    ef[i + fnx * (j + fny * k)] += val;
+    return val;
 }
 
-// CG bottom solve, one workgroup; dots accumulated in double
+// 1-D grid of gx blocks per fine row
+template <class T, int DIM>
+__global__ void __launch_bounds__(128) k_prolong(LevV<T> C, const T* __restrict__ ec, int fnx, int fny,
+                                                 int fnz, T* __restrict__ ef, int gx) {
+    const unsigned lb = lblock();
+    const unsigned row = lb / gx;
+    const int j = (int)(row % fny), k = (int)(row / fny);
+    const int i = (int)(lb % gx) * blockDim.x + threadIdx.x;
+    if (i >= fnx) return;
+    ef[i + fnx * (j + fny * k)] += prolong_cell<T, DIM>(C, ec, i, j, k);
+}
+
+// CG bottom solve by one workgroup (all threads of the block call it);
+// dots accumulated in double
 template <class T, int BM, int DIM>
-__global__ void __launch_bounds__(1024) k_bottom_cg(LevV<T> L, T* __restrict__ e, const T* __restrict__ b,
-                                                    T* __restrict__ r, T* __restrict__ p,
-                                                    T* __restrict__ q) {
+__device__ void bottom_cg(const LevV<T>& L, T* __restrict__ e, const T* __restrict__ b, T* __restrict__ r,
+                          T* __restrict__ p, T* __restrict__ q) {
     __shared__ double red[1024 / 64];
     __shared__ double bc;
     const int n = L.nx * L.ny * L.nz;
@@ -789,6 +801,78 @@ __global__ void __launch_bounds__(1024) k_bottom_cg(LevV<T> L, T* __restrict__ e
         for (int c = tid; c < n; c += nt) p[c] = r[c] + T(beta) * p[c];
         __syncthreads();
         rr = rn;
+    }
+}
+
+
+// The coarse end of a V-cycle in one workgroup: from level l0 (whose rhs
+// the finer level's restriction already holds) down to the bottom solve and
+// back up to l0's corrected e, the operations and order of the per-level
+// launches (zero, nu1 red/black sweeps, residual restriction; CG bottom;
+// prolongation, nu2 sweeps), with workgroup barriers in place of kernel
+// boundaries.  For levels of a few thousand cells, where a launch costs
+// more than the work.
+constexpr int VC_MAXL = 12;
+constexpr int64_t VC_TAIL_CELLS = 4096;
+template <class T>
+struct VTail {
+    LevV<T> v[VC_MAXL];  // levels l0 .. l0+nl-1, the last one the bottom
+    T* e[VC_MAXL];
+    T* r[VC_MAXL];
+    T *cr, *cp, *cq;  // bottom CG scratch
+    int nl, nu1, nu2;
+};
+
+template <class T, int BM, int DIM>
+__device__ __forceinline__ void wg_sweep(const LevV<T>& L, T* __restrict__ x, const T* __restrict__ rhs,
+                                         int color) {
+    const int nh = (L.nx + 1) / 2, nrow = L.ny * L.nz;
+    for (int t = threadIdx.x; t < nh * nrow; t += blockDim.x) {
+        const int row = t / nh, ii = t - row * nh;
+        const int j = row % L.ny, k = row / L.ny;
+        const int i = 2 * ii + ((color + j + k) & 1);
+        if (i >= L.nx) continue;
+        T dg, of;
+        stencil<T, BM, DIM>(L, x, i, j, k, dg, of);
+        const int c = i + L.nx * row;
+        x[c] = (rhs[c] + of) / dg;
+    }
+    __syncthreads();
+}
+
+template <class T, int BM, int DIM>
+__global__ void __launch_bounds__(1024) k_vcycle_tail(VTail<T> a) {
+    for (int l = 0; l < a.nl - 1; l++) {
+        const LevV<T>& F = a.v[l];
+        const int n = F.nx * F.ny * F.nz;
+        for (int c = threadIdx.x; c < n; c += blockDim.x) a.e[l][c] = T(0);
+        __syncthreads();
+        for (int s = 0; s < a.nu1; s++) {
+            wg_sweep<T, BM, DIM>(F, a.e[l], a.r[l], 0);
+            wg_sweep<T, BM, DIM>(F, a.e[l], a.r[l], 1);
+        }
+        const LevV<T>& Cv = a.v[l + 1];
+        const int nc = Cv.nx * Cv.ny * Cv.nz;
+        for (int t = threadIdx.x; t < nc; t += blockDim.x) {
+            const int I = t % Cv.nx, row = t / Cv.nx;
+            a.r[l + 1][t] = resid_restrict_cell<T, BM, DIM>(F, a.e[l], a.r[l], I, row % Cv.ny, row / Cv.ny);
+        }
+        __syncthreads();
+    }
+    bottom_cg<T, BM, DIM>(a.v[a.nl - 1], a.e[a.nl - 1], a.r[a.nl - 1], a.cr, a.cp, a.cq);
+    __syncthreads();
+    for (int l = a.nl - 2; l >= 0; l--) {
+        const LevV<T>& F = a.v[l];
+        const int n = F.nx * F.ny * F.nz;
+        for (int t = threadIdx.x; t < n; t += blockDim.x) {
+            const int i = t % F.nx, row = t / F.nx;
+            a.e[l][t] += prolong_cell<T, DIM>(a.v[l + 1], a.e[l + 1], i, row % F.ny, row / F.ny);
+        }
+        __syncthreads();
+        for (int s = 0; s < a.nu2; s++) {
+            wg_sweep<T, BM, DIM>(F, a.e[l], a.r[l], 0);
+            wg_sweep<T, BM, DIM>(F, a.e[l], a.r[l], 1);
+        }
     }
 }
 
@@ -1215,22 +1299,17 @@ static void launch_resid_restrict(vd_ctx* ctx, int dim, const LevV<T>& v, const 
                            Cc.n[1], Cc.n[2], Cc.r, gx);
 }
 
-template <class T, int BM>
-static void launch_bottom(vd_ctx* ctx, int dim, const LevV<T>& v, Lev<T>& B) {
-    if (dim == 3)
-        hipLaunchKernelGGL((k_bottom_cg<T, BM, 3>), dim3(1), dim3(1024), 0, ctx->stream, v, B.e, B.r, B.w,
-                           B.p, B.q);
-    else
-        hipLaunchKernelGGL((k_bottom_cg<T, BM, 2>), dim3(1), dim3(1024), 0, ctx->stream, v, B.e, B.r, B.w,
-                           B.p, B.q);
-}
 
 template <class T>
 static void vcycle(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L, double alpha,
                    double beta, double bconst, int nu1, int nu2) {
     const int nl = W.nlev;
+    // levels l0 .. nl-1 (at most VC_TAIL_CELLS cells each) run in one
+    // workgroup launch; the finer ones level by level
+    int l0 = nl - 1;
+    while (l0 > 0 && ncell(L[l0 - 1].n) <= VC_TAIL_CELLS && nl - l0 < VC_MAXL) l0--;
     // down
-    for (int l = 0; l < nl - 1; l++) {
+    for (int l = 0; l < l0; l++) {
         Lev<T>& F = L[l];
         LevV<T> v = view<T>(W, F, alpha, beta, bconst);
         int64_t nc = ncell(F.n);
@@ -1257,15 +1336,31 @@ static void vcycle(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L,
         if (bm == 1) launch_resid_restrict<T, 1>(ctx, W.dim, v, F, L[l + 1]);
         else launch_resid_restrict<T, 0>(ctx, W.dim, v, F, L[l + 1]);
     }
-    // bottom
+    // coarse levels and bottom
     {
+        VTail<T> a;
+        for (int l = l0; l < nl; l++) {
+            a.v[l - l0] = view<T>(W, L[l], alpha, beta, bconst);
+            a.e[l - l0] = L[l].e;
+            a.r[l - l0] = L[l].r;
+        }
         Lev<T>& B = L[nl - 1];
-        LevV<T> v = view<T>(W, B, alpha, beta, bconst);
-        if (bm == 1) launch_bottom<T, 1>(ctx, W.dim, v, B);
-        else launch_bottom<T, 0>(ctx, W.dim, v, B);
+        a.cr = B.w;
+        a.cp = B.p;
+        a.cq = B.q;
+        a.nl = nl - l0;
+        a.nu1 = nu1;
+        a.nu2 = nu2;
+        if (bm == 1) {
+            if (W.dim == 3) hipLaunchKernelGGL((k_vcycle_tail<T, 1, 3>), dim3(1), dim3(1024), 0, ctx->stream, a);
+            else hipLaunchKernelGGL((k_vcycle_tail<T, 1, 2>), dim3(1), dim3(1024), 0, ctx->stream, a);
+        } else {
+            if (W.dim == 3) hipLaunchKernelGGL((k_vcycle_tail<T, 0, 3>), dim3(1), dim3(1024), 0, ctx->stream, a);
+            else hipLaunchKernelGGL((k_vcycle_tail<T, 0, 2>), dim3(1), dim3(1024), 0, ctx->stream, a);
+        }
     }
     // up
-    for (int l = nl - 2; l >= 0; l--) {
+    for (int l = l0 - 1; l >= 0; l--) {
         Lev<T>& F = L[l];
         Lev<T>& Cc = L[l + 1];
         LevV<T> vc = view<T>(W, Cc, alpha, beta, bconst);
