@@ -23,18 +23,28 @@ pytestmark = pytest.mark.gpu
 
 def _with_fused(flag, fn):
     """Runs fn(dev) in a fresh context whose multigrid workspaces are
-    created with the fused smoother on (flag True) or off."""
-    old = os.environ.get("VD_FUSED_SMOOTHER")
+    created with the fused smoother on (flag True; 16-plane chunks, which
+    also enables it on grids smaller than it is chosen for by default) or
+    off, and checks from the kernel-timing counts that it ran or not."""
+    keys = ("VD_FUSED_SMOOTHER", "VD_FUSED_ZC")
+    old = {k: os.environ.get(k) for k in keys}
     os.environ["VD_FUSED_SMOOTHER"] = "1" if flag else "0"
+    os.environ["VD_FUSED_ZC"] = "16"
     dev = native.Device(0)
     try:
-        return fn(dev)
+        dev.set_kernel_timing(abi.KT_FUSED)
+        out = fn(dev)
+        launches = dev.kernel_timing()[1]
+        assert (launches > 0) == flag, launches
+        return out
     finally:
+        dev.set_kernel_timing(abi.KT_NONE)
         dev.close()
-        if old is None:
-            del os.environ["VD_FUSED_SMOOTHER"]
-        else:
-            os.environ["VD_FUSED_SMOOTHER"] = old
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
 
 
 def _problem(views, bcl, bch, prec):

@@ -1051,24 +1051,28 @@ static LevV<T> view(const MGWork& W, const Lev<T>& L, double alpha, double beta,
     return v;
 }
 
+// planes per block of the fused smoother: 32-plane chunks when that gives
+// at least half the CUs a block, else 16-plane chunks (more plane halo)
+// when that fills every CU; 0 when the level is too small for the fused
+// kernel to pay (per-colour launches are used).  Measured at 64^3 .. 256^3.
+// VD_FUSED_ZC overrides.
+static int fused_zc(const int n[3]) {
+    const char* e = getenv("VD_FUSED_ZC");
+    if (e && atoi(e) > 0) return std::min(atoi(e), n[2]);
+    const int64_t tiles = (int64_t)((n[0] + FZ_IX - 1) / FZ_IX) * ((n[1] + FZ_IY - 1) / FZ_IY);
+    if (tiles * ((n[2] + 31) / 32) >= 128) return 32;
+    if (tiles * ((n[2] + 15) / 16) >= 256) return 16;
+    return 0;
+}
+
 // finest levels the fused smoother handles: 3-D, z not periodic, even
-// nx and ny, 32-bit cell offsets; VD_FUSED_SMOOTHER=0 turns it off
+// nx and ny, 32-bit byte offsets, enough blocks; VD_FUSED_SMOOTHER=0
+// turns it off
 static bool fused_capable(const MGWork& W) {
     const char* e = getenv("VD_FUSED_SMOOTHER");  // read when a workspace is created
     const bool env = !(e && strcmp(e, "0") == 0);
     return env && W.dim == 3 && !W.per[2] && W.n[0] % 2 == 0 && W.n[1] % 2 == 0 && W.n[0] >= 8 &&
-           W.n[1] >= 8 && W.n[2] >= 2 && ncell(W.n) * 8 < ((int64_t)1 << 31);
-}
-
-// planes per block of the fused smoother (VD_FUSED_ZC overrides)
-static int fused_zc(int nz) {
-    static int env = -1;
-    if (env < 0) {
-        const char* e = getenv("VD_FUSED_ZC");
-        env = e ? atoi(e) : 0;
-    }
-    int zc = env > 0 ? env : 32;
-    return zc > nz ? nz : zc;
+           W.n[1] >= 8 && W.n[2] >= 2 && ncell(W.n) * 8 < ((int64_t)1 << 31) && fused_zc(W.n) > 0;
 }
 
 template <class T, int BM, int S, int RES>
@@ -1076,7 +1080,8 @@ static void launch_fused_k(vd_ctx* ctx, const LevV<T>& v, const T* xin, T* xout,
                            double* part, int* nblk) {
     // the kernel reads xin unless it starts from zero (ZI)
     const int ntx = (v.nx + FZ_IX - 1) / FZ_IX, nty = (v.ny + FZ_IY - 1) / FZ_IY;
-    const int zc = fused_zc(v.nz), nzc = (v.nz + zc - 1) / zc;
+    const int n3[3] = {v.nx, v.ny, v.nz};
+    const int zc = std::max(1, fused_zc(n3)), nzc = (v.nz + zc - 1) / zc;
     const int nb = ntx * nty * nzc;
     if (nblk) *nblk = nb;
     if (xin)
