@@ -840,6 +840,73 @@ __device__ __forceinline__ void wg_sweep(const LevV<T>& L, T* __restrict__ x, co
     __syncthreads();
 }
 
+// CG bottom solve by the first wave alone for a bottom level of at most
+// VC_WAVE_BOTTOM cells: e, r, p and q live in LDS, dots are wave
+// reductions, and no workgroup barrier is needed between the phases of an
+// iteration (a wave's LDS accesses complete in program order; the
+// wavefront fences keep the compiler from reordering them).  The other
+// waves wait at the caller's barrier.
+constexpr int VC_WAVE_BOTTOM = 1024;
+template <class T, int BM, int DIM>
+__device__ void bottom_cg_wave(const LevV<T>& L, T* __restrict__ e_out, const T* __restrict__ b, T* lds) {
+    const int n = L.nx * L.ny * L.nz;
+    const int lane = threadIdx.x;
+    T* e = lds;
+    T* r = lds + VC_WAVE_BOTTOM;
+    T* p = lds + 2 * VC_WAVE_BOTTOM;
+    T* q = lds + 3 * VC_WAVE_BOTTOM;
+    auto sync = [] {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    };
+    auto wsum = [](double v) {
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+        return v;
+    };
+    double rr = 0.0;
+    for (int c = lane; c < n; c += 64) {
+        const T bv = b[c];
+        e[c] = T(0);
+        r[c] = bv;
+        p[c] = bv;
+        rr += (double)bv * (double)bv;
+    }
+    rr = wsum(rr);
+    sync();
+    const double rr0 = rr;
+    const int maxit = 4 * n + 20;
+    for (int it = 0; it < maxit; it++) {
+        if (rr <= 1e-28 * rr0 || rr == 0.0) break;
+        double pq = 0.0;
+        for (int c = lane; c < n; c += 64) {
+            const int i = c % L.nx, row = c / L.nx;
+            T dg, of;
+            stencil<T, BM, DIM>(L, p, i, row % L.ny, row / L.ny, dg, of);
+            const T qv = dg * p[c] - of;
+            q[c] = qv;
+            pq += (double)p[c] * (double)qv;
+        }
+        pq = wsum(pq);
+        if (pq == 0.0) break;
+        const double alpha = rr / pq;
+        double rn = 0.0;
+        for (int c = lane; c < n; c += 64) {
+            e[c] += T(alpha) * p[c];
+            const T rv = r[c] - T(alpha) * q[c];
+            r[c] = rv;
+            rn += (double)rv * (double)rv;
+        }
+        rn = wsum(rn);
+        const double beta = rn / rr;
+        sync();
+        for (int c = lane; c < n; c += 64) p[c] = r[c] + T(beta) * p[c];
+        sync();
+        rr = rn;
+    }
+    for (int c = lane; c < n; c += 64) e_out[c] = e[c];
+}
+
 template <class T, int BM, int DIM>
 __global__ void __launch_bounds__(1024) k_vcycle_tail(VTail<T> a) {
     for (int l = 0; l < a.nl - 1; l++) {
@@ -859,7 +926,15 @@ __global__ void __launch_bounds__(1024) k_vcycle_tail(VTail<T> a) {
         }
         __syncthreads();
     }
-    bottom_cg<T, BM, DIM>(a.v[a.nl - 1], a.e[a.nl - 1], a.r[a.nl - 1], a.cr, a.cp, a.cq);
+    {
+        const LevV<T>& B = a.v[a.nl - 1];
+        if (B.nx * B.ny * B.nz <= VC_WAVE_BOTTOM) {
+            __shared__ T cg[4 * VC_WAVE_BOTTOM];
+            if (threadIdx.x < 64) bottom_cg_wave<T, BM, DIM>(B, a.e[a.nl - 1], a.r[a.nl - 1], cg);
+        } else {
+            bottom_cg<T, BM, DIM>(B, a.e[a.nl - 1], a.r[a.nl - 1], a.cr, a.cp, a.cq);
+        }
+    }
     __syncthreads();
     for (int l = a.nl - 2; l >= 0; l--) {
         const LevV<T>& F = a.v[l];
