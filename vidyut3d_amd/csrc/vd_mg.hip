@@ -77,7 +77,7 @@ struct MGWork {
     double* c00[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     double* bcv[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     double* rob[3][2][3];  // robin a, b, f per side
-    unsigned long long* dnorm = nullptr;  // 4 groups of NSLOT norm slots
+    unsigned long long* dnorm = nullptr;  // 8 groups of NSLOT norm slots
     double* dcell = nullptr;              // cell coefficient on the grown box
     double* part = nullptr;               // per-block partial norms
     int c0_types[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};
@@ -197,9 +197,9 @@ __device__ __forceinline__ void stencil_nb(const LevV<T>& L, int i, int j, int k
 // one) and the boundary terms are selected afterwards: loads under
 // divergent branches would be waited for branch by branch.  The arithmetic
 // is that of stencil_nb term for term.
-template <class T, int BM, int DIM>
+template <class T, int BM, int DIM, bool FM = false>
 __device__ __forceinline__ void stencil(const LevV<T>& L, const T* __restrict__ x, int i, int j,
-                                        int k, T& dgo, T& ofo) {
+                                        int k, T& dgo, T& ofo, T* fmax_d = nullptr) {
     const int c = i + L.nx * (j + L.ny * k);
     T xl[DIM], xh[DIM];
 #pragma unroll
@@ -222,6 +222,7 @@ __device__ __forceinline__ void stencil(const LevV<T>& L, const T* __restrict__ 
         const T s = d == 0 ? L.s0 : (d == 1 ? L.s1 : L.s2);
         T blo, bhi;
         face_coef<T, BM, DIM>(L, d, i, j, k, blo, bhi);
+        if (FM) fmax_d[d] = fmax(fabs(blo), fabs(bhi));
         T c0l = T(1), c0h = T(1);
         if (!per) {
             if (L.c0uni) {
@@ -539,6 +540,7 @@ constexpr int NORM_BLOCKS = 2048;
 // block max (wave64 shuffles, then LDS), one atomic per block
 __device__ __forceinline__ void block_max_atomic(double a, unsigned long long* slots) {
     __shared__ double red[16];
+    __syncthreads();  // red may still be read by a previous call
     for (int o = 32; o > 0; o >>= 1) a = fmax(a, __shfl_xor(a, o, 64));
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
     __syncthreads();
@@ -550,33 +552,60 @@ __device__ __forceinline__ void block_max_atomic(double a, unsigned long long* s
 }
 
 // r = f - L x (finest level, FP64), one block per gx-chunk of a row
-// (XCD-remapped); each block writes the max |r| of its cells to part[lb]
-template <int BM, int DIM>
+// (XCD-remapped); each block writes the max |r| of its cells to part[lb].
+// BN: also max |f| into the slot group bslots; FM (cell coefficient): also
+// the max face coefficient of each direction into slot groups fslots[0..2]
+template <int BM, int DIM, bool BN = false, bool FM = false>
 __global__ void __launch_bounds__(256) k_resid0(LevV<double> L, const double* __restrict__ x,
                                                 const double* __restrict__ f, double* __restrict__ r,
-                                                double* __restrict__ part, int gx) {
-    __shared__ double red[4];
+                                                double* __restrict__ part, int gx,
+                                                unsigned long long* __restrict__ bslots,
+                                                unsigned long long* __restrict__ fslots) {
+    constexpr int NV = 1 + (BN ? 1 : 0) + (FM ? DIM : 0);  // maxima reduced per block
+    __shared__ double red[4][NV];
     const unsigned lb = lblock();
     const unsigned row = lb / gx;
     const int j = (int)(row % L.ny), k = (int)(row / L.ny);
     const int i = (int)(lb % gx) * blockDim.x + threadIdx.x;
-    double a = 0.0;
+    double m[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) m[q] = 0.0;
     if (i < L.nx) {
-        double dg, of;
-        stencil<double, BM, DIM>(L, x, i, j, k, dg, of);
+        double dg, of, fm[3] = {0.0, 0.0, 0.0};
+        stencil<double, BM, DIM, FM>(L, x, i, j, k, dg, of, fm);
         const int c = i + L.nx * (int)row;
-        double v = f[c] - (dg * x[c] - of);
+        const double fc = f[c];
+        double v = fc - (dg * x[c] - of);
         if (r) r[c] = v;
-        a = fabs(v);
-        if (a != a) a = __builtin_huge_val();
+        m[0] = fabs(v);
+        if (BN) m[1] = fabs(fc);
+        if (FM)
+#pragma unroll
+            for (int d = 0; d < DIM; d++) m[(BN ? 2 : 1) + d] = fm[d];
+#pragma unroll
+        for (int q = 0; q < NV; q++)
+            if (m[q] != m[q]) m[q] = __builtin_huge_val();
     }
-    for (int o = 32; o > 0; o >>= 1) a = fmax(a, __shfl_xor(a, o, 64));
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = a;
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        for (int o = 32; o > 0; o >>= 1) m[q] = fmax(m[q], __shfl_xor(m[q], o, 64));
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][q] = m[q];
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        double m = red[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, red[w]);
-        part[lb] = m;
+        double mm[NV];
+#pragma unroll
+        for (int q = 0; q < NV; q++) {
+            mm[q] = red[0][q];
+            for (int w = 1; w < (int)(blockDim.x >> 6); w++) mm[q] = fmax(mm[q], red[w][q]);
+        }
+        part[lb] = mm[0];
+        if (BN) atomic_max_nonneg(bslots + (lb % NSLOT) * SLOT_STRIDE, mm[1]);
+        if (FM)
+#pragma unroll
+            for (int d = 0; d < DIM; d++)
+                atomic_max_nonneg(fslots + (size_t)d * NSLOT * SLOT_STRIDE + (lb % NSLOT) * SLOT_STRIDE,
+                                  mm[(BN ? 2 : 1) + d]);
     }
 }
 
@@ -607,51 +636,6 @@ __global__ void __launch_bounds__(256) k_maxabs(const double* __restrict__ f, un
     block_max_atomic(a, slots);
 }
 
-// per direction, the max over all faces of the harmonic face value of a
-// cell coefficient on the grown box, in one pass over the face-index box
-// (nx+1, ny+1, nz+1): point (i,j,k) owns face (i,j,k) of every direction
-// whose face box contains it; results go to slot groups 0..DIM-1
-template <int DIM>
-__global__ void __launch_bounds__(256) k_face_max_cd(const double* __restrict__ dcell, int nx, int ny,
-                                                     int nz, unsigned long long* __restrict__ slots) {
-    __shared__ double red[3][4];
-    const int p0 = nx + 1, p1 = ny + 1, p2 = DIM == 3 ? nz + 1 : 1;
-    const unsigned n = (unsigned)p0 * p1 * p2;
-    const int gx = nx + 2, gxy = gx * (ny + 2);
-    double a[3] = {0.0, 0.0, 0.0};
-    for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
-        int i = (int)(t % p0);
-        unsigned q = t / p0;
-        int j = (int)(q % p1), k = (int)(q / p1);
-        const int gc = (i + 1) + gx * (j + 1) + (DIM == 3 ? gxy * (k + 1) : 0);  // right cell
-        const int ext[3] = {nx, ny, nz};
-        const int ijk[3] = {i, j, k};
-#pragma unroll
-        for (int d = 0; d < DIM; d++) {
-            bool ok = true;
-#pragma unroll
-            for (int e = 0; e < DIM; e++)
-                if (e != d && ijk[e] >= ext[e]) ok = false;
-            if (!ok) continue;
-            const int gs = d == 0 ? 1 : (d == 1 ? gx : gxy);
-            double b = fabs(harm(dcell[gc - gs], dcell[gc]));
-            if (b != b) b = __builtin_huge_val();
-            a[d] = fmax(a[d], b);
-        }
-    }
-#pragma unroll
-    for (int d = 0; d < DIM; d++) {
-        double v = a[d];
-        for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-        if ((threadIdx.x & 63) == 0) red[d][threadIdx.x >> 6] = v;
-    }
-    __syncthreads();
-    if (threadIdx.x < DIM) {
-        double m = red[threadIdx.x][0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); w++) m = fmax(m, red[threadIdx.x][w]);
-        atomic_max_nonneg(slots + (size_t)threadIdx.x * NSLOT * SLOT_STRIDE + (blockIdx.x % NSLOT) * SLOT_STRIDE, m);
-    }
-}
 
 // coarse r = average of the 2^DIM fine residuals rhs - L e (fused)
 template <class T, int BM, int DIM>
@@ -1293,7 +1277,7 @@ static int get_work(vd_ctx* ctx, const vd_geom* g, const vd_box* box, int prec, 
     W->f = (double*)p;
     if ((st = dalloc(W, nc * 8, &p))) goto fail;
     W->res = (double*)p;
-    if ((st = dalloc(W, 4 * NSLOT * SLOT_STRIDE * sizeof(unsigned long long), &p))) goto fail;
+    if ((st = dalloc(W, 8 * NSLOT * SLOT_STRIDE * sizeof(unsigned long long), &p))) goto fail;
     W->dnorm = (unsigned long long*)p;
     {
         // per-block partial norms of k_resid0 and of the fused smoother
@@ -1520,41 +1504,79 @@ static int read_groups(vd_ctx* ctx, MGWork& W, int ngroups, double* out) {
     return VD_OK;
 }
 
-// residual of the finest level into W.res and its max-norm; with bnorm
-// also the max-norm of f (the rhs with boundary data)
+// residual of the finest level into W.res (unless write_res is false) and
+// its max-norm; with bnorm also the max-norm of f (the rhs with boundary
+// data), with fmax (cell coefficient, bm 2) the max face coefficient per
+// direction -- all from the one pass over the level
 static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double* nrm_out,
-                  double* bnorm = nullptr, bool write_res = true) {
-    zero_groups(ctx, W, 2);
+                  double* bnorm = nullptr, bool write_res = true, double* fmax = nullptr) {
+    zero_groups(ctx, W, 5);
     int64_t nc = ncell(W.n);
-    if (bnorm)
-        hipLaunchKernelGGL(k_maxabs, norm_grid(nc), dim3(256), 0, ctx->stream, W.f, (unsigned)nc,
-                           slot_group(W, 1));
     const int gx = (W.n[0] + 255) / 256;
     const int nblk = gx * W.n[1] * W.n[2];
     dim3 g((unsigned)nblk), b(256);
     unsigned long long* sl = slot_group(W, 0);
+    unsigned long long* bs = slot_group(W, 1);
+    unsigned long long* fs = slot_group(W, 2);
     double* part = W.part;
     double* res = write_res ? W.res : nullptr;  // the norm alone when no V-cycle follows
     kt_begin(ctx, VD_KT_RESID);
-    if (bm == 1) {
-        if (W.dim == 3) hipLaunchKernelGGL((k_resid0<1, 3>), g, b, 0, ctx->stream, v0, W.x, W.f, res, part, gx);
-        else hipLaunchKernelGGL((k_resid0<1, 2>), g, b, 0, ctx->stream, v0, W.x, W.f, res, part, gx);
-    } else if (bm == 0) {
-        if (W.dim == 3) hipLaunchKernelGGL((k_resid0<0, 3>), g, b, 0, ctx->stream, v0, W.x, W.f, res, part, gx);
-        else hipLaunchKernelGGL((k_resid0<0, 2>), g, b, 0, ctx->stream, v0, W.x, W.f, res, part, gx);
-    } else {
-        if (W.dim == 3) hipLaunchKernelGGL((k_resid0<2, 3>), g, b, 0, ctx->stream, v0, W.x, W.f, res, part, gx);
-        else hipLaunchKernelGGL((k_resid0<2, 2>), g, b, 0, ctx->stream, v0, W.x, W.f, res, part, gx);
+#define VD_RESID(BMV, DIMV)                                                                                   \
+    {                                                                                                         \
+        if (fmax && bnorm)                                                                                    \
+            hipLaunchKernelGGL((k_resid0<BMV, DIMV, true, true>), g, b, 0, ctx->stream, v0, W.x, W.f, res,     \
+                               part, gx, bs, fs);                                                             \
+        else if (bnorm)                                                                                       \
+            hipLaunchKernelGGL((k_resid0<BMV, DIMV, true, false>), g, b, 0, ctx->stream, v0, W.x, W.f, res,    \
+                               part, gx, bs, fs);                                                             \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_resid0<BMV, DIMV, false, false>), g, b, 0, ctx->stream, v0, W.x, W.f, res,   \
+                               part, gx, bs, fs);                                                             \
     }
+    if (bm == 1) {
+        if (W.dim == 3) VD_RESID(1, 3) else VD_RESID(1, 2)
+    } else if (bm == 0) {
+        if (W.dim == 3) VD_RESID(0, 3) else VD_RESID(0, 2)
+    } else {
+        if (W.dim == 3) VD_RESID(2, 3) else VD_RESID(2, 2)
+    }
+#undef VD_RESID
     hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, ctx->stream, part, nblk, sl);
     const int nb = bm == 1 ? 0 : (bm == 0 ? W.dim : 1);
     kt_end(ctx, VD_KT_RESID, 8.0 * (double)nc * ((write_res ? 4 : 3) + nb));
     VD_CHECK_LAUNCH();
-    double m[2];
-    VD_TRY(read_groups(ctx, W, bnorm ? 2 : 1, m));
+    double m[5];
+    const int ng = fmax ? 2 + W.dim : (bnorm ? 2 : 1);
+    VD_TRY(read_groups(ctx, W, ng, m));
     *nrm_out = m[0];
     if (bnorm) *bnorm = m[1];
+    if (fmax)
+        for (int d = 0; d < W.dim; d++) fmax[d] = m[2 + d];
     return VD_OK;
+}
+
+// harmonic face coefficient of direction d from the cell coefficient on the
+// grown box (ScalarSolve.cpp:511-512), plus the block max of the faces into
+// a norm slot group (Jacobi bound of the implicit operator)
+__global__ void __launch_bounds__(256) k_harmonic_faces(const double* __restrict__ dcell, int nx, int ny,
+                                                        int nz, int d, int dim, double* __restrict__ bf,
+                                                        unsigned long long* __restrict__ slots) {
+    int n0 = nx + (d == 0), n1 = ny + (d == 1), n2 = nz + (d == 2);
+    unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+    double m = 0.0;
+    if (t < (unsigned)n0 * n1 * n2) {
+        int i = (int)(t % n0);
+        unsigned r = t / n0;
+        int j = (int)(r % n1), k = (int)(r / n1);
+        const int gx = nx + 2, gxy = gx * (ny + 2);
+        const int gs = d == 0 ? 1 : (d == 1 ? gx : gxy);
+        const int gc = (i + 1) + gx * (j + 1) + (dim == 3 ? gxy * (k + 1) : 0);
+        const double b = harm(dcell[gc - gs], dcell[gc]);
+        bf[t] = b;
+        m = fabs(b);
+        if (m != m) m = __builtin_huge_val();
+    }
+    if (slots) block_max_atomic(m, slots);
 }
 
 // Jacobi bound of the finest operator, rho = sum_d 2|beta/dx_d^2| bmax_d / |alpha|.
@@ -1563,8 +1585,7 @@ static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double
 // runs finest-level sweeps only (the implicit species solves at small dt).
 constexpr double SMOOTH_ONLY_RHO = 1e-2;
 
-static int jacobi_rho(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta, double bconst,
-                      const double* dcell, double* rho) {
+static int jacobi_rho(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta, double bconst, double* rho) {
     *rho = INFINITY;
     if (alpha == 0.0) return VD_OK;
     double bmax[3] = {0.0, 0.0, 0.0};
@@ -1572,20 +1593,10 @@ static int jacobi_rho(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
         for (int d = 0; d < W.dim; d++) bmax[d] = fabs(bconst);
     } else {
         zero_groups(ctx, W, W.dim);
-        if (bm == 0) {
-            for (int d = 0; d < W.dim; d++) {
-                int64_t nf = nfacef(W.n, d);
-                hipLaunchKernelGGL(k_maxabs, norm_grid(nf), dim3(256), 0, ctx->stream, W.bf0[d], (unsigned)nf,
-                                   slot_group(W, d));
-            }
-        } else {
-            int64_t np = (int64_t)(W.n[0] + 1) * (W.n[1] + 1) * (W.dim == 3 ? W.n[2] + 1 : 1);
-            if (W.dim == 3)
-                hipLaunchKernelGGL((k_face_max_cd<3>), norm_grid(np), dim3(256), 0, ctx->stream, dcell, W.n[0],
-                                   W.n[1], W.n[2], slot_group(W, 0));
-            else
-                hipLaunchKernelGGL((k_face_max_cd<2>), norm_grid(np), dim3(256), 0, ctx->stream, dcell, W.n[0],
-                                   W.n[1], W.n[2], slot_group(W, 0));
+        for (int d = 0; d < W.dim; d++) {
+            int64_t nf = nfacef(W.n, d);
+            hipLaunchKernelGGL(k_maxabs, norm_grid(nf), dim3(256), 0, ctx->stream, W.bf0[d], (unsigned)nf,
+                               slot_group(W, d));
         }
         VD_CHECK_LAUNCH();
         VD_TRY(read_groups(ctx, W, W.dim, bmax));
@@ -1599,8 +1610,12 @@ static int jacobi_rho(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
 // Solve with the finest-level data already in W (x, f with BC shift,
 // bf0 / bconst / dcell, c00).  Builds the coarse hierarchy unless the
 // operator is diagonally dominant enough for sweeps alone, then iterates.
+// smooth_mode: 0 V-cycles, 1 finest-level sweeps only, 2 (bm 2, the cell
+// coefficient) decided from the Jacobi bound of the face maxima that the
+// first residual pass also returns; when V-cycles follow, the face arrays
+// are built and the solve continues on them (bm 0).
 static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta, double bconst,
-                      const double* dcell, const int tlo[3], const int thi[3], bool smooth_only,
+                      const double* dcell, const int tlo[3], const int thi[3], int smooth_mode,
                       double reltol, double abstol, int maxiter, int nu1, int nu2, vd_solve_info* info) {
     int64_t nc = ncell(W.n);
     W.c0uni = 1;
@@ -1610,6 +1625,32 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
             W.c0u[d][sd] = ty == VD_DIRCBC ? -1.0 : 1.0;
             if (d < W.dim && !W.per[d] && ty == VD_ROBINBC) W.c0uni = 0;
         }
+    Lev<double> fine;
+    for (int d = 0; d < 3; d++) {
+        fine.n[d] = W.n[d];
+        fine.dx[d] = W.dx[d];
+        fine.bf[d] = W.bf0[d];
+        for (int s = 0; s < 2; s++) fine.c0[d][s] = W.c00[d][s];
+    }
+    LevV<double> v0 = view<double>(W, fine, alpha, beta, bconst);
+    v0.dcell = dcell;
+    double r0, rn, bnorm;
+    bool smooth_only = smooth_mode == 1;
+    if (smooth_mode == 2) {
+        double bmax[3] = {0.0, 0.0, 0.0};
+        VD_TRY(resid0(ctx, W, bm, v0, &r0, &bnorm, true, bmax));
+        if (alpha != 0.0) {
+            double rho = 0.0;
+            for (int d = 0; d < W.dim; d++) rho += 2.0 * fabs(beta / (W.dx[d] * W.dx[d])) * bmax[d];
+            smooth_only = rho / fabs(alpha) <= SMOOTH_ONLY_RHO;
+        }
+        if (!smooth_only) {
+            for (int d = 0; d < W.dim; d++)
+                hipLaunchKernelGGL(k_harmonic_faces, grid1d(nfacef(W.n, d), 256), dim3(256), 0, ctx->stream, dcell,
+                                   W.n[0], W.n[1], W.n[2], d, W.dim, W.bf0[d], (unsigned long long*)nullptr);
+            bm = 0;
+        }
+    }
     if (!smooth_only) {
         // coarse c0 depends only on the BC types unless a side is Robin
         bool robin = false, same = W.c0_valid;
@@ -1645,17 +1686,7 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
         W.c0_valid = !robin;
         VD_CHECK_LAUNCH();
     }
-    Lev<double> fine;
-    for (int d = 0; d < 3; d++) {
-        fine.n[d] = W.n[d];
-        fine.dx[d] = W.dx[d];
-        fine.bf[d] = W.bf0[d];
-        for (int s = 0; s < 2; s++) fine.c0[d][s] = W.c00[d][s];
-    }
-    LevV<double> v0 = view<double>(W, fine, alpha, beta, bconst);
-    v0.dcell = dcell;
-    double r0, rn, bnorm;
-    VD_TRY(resid0(ctx, W, bm, v0, &r0, &bnorm, !smooth_only));
+    if (smooth_mode != 2) VD_TRY(resid0(ctx, W, bm, v0, &r0, &bnorm, !smooth_only));
     double target = std::max(abstol, reltol * std::max(bnorm, r0));
     rn = r0;
     int it = 0, st = VD_OK;
@@ -2061,30 +2092,6 @@ __global__ void k_cell_diff(CellDArgs a, double* __restrict__ dcell) {
 }
 
 // harmonic face average of the cell coefficient (ScalarSolve.cpp:511-512)
-// harmonic face coefficient of direction d from the cell coefficient on the
-// grown box (ScalarSolve.cpp:511-512), plus the block max of the faces into
-// a norm slot group (Jacobi bound of the implicit operator)
-__global__ void __launch_bounds__(256) k_harmonic_faces(const double* __restrict__ dcell, int nx, int ny,
-                                                        int nz, int d, int dim, double* __restrict__ bf,
-                                                        unsigned long long* __restrict__ slots) {
-    int n0 = nx + (d == 0), n1 = ny + (d == 1), n2 = nz + (d == 2);
-    unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
-    double m = 0.0;
-    if (t < (unsigned)n0 * n1 * n2) {
-        int i = (int)(t % n0);
-        unsigned r = t / n0;
-        int j = (int)(r % n1), k = (int)(r / n1);
-        const int gx = nx + 2, gxy = gx * (ny + 2);
-        const int gs = d == 0 ? 1 : (d == 1 ? gx : gxy);
-        const int gc = (i + 1) + gx * (j + 1) + (dim == 3 ? gxy * (k + 1) : 0);
-        const double b = harm(dcell[gc - gs], dcell[gc]);
-        bf[t] = b;
-        m = fabs(b);
-        if (m != m) m = __builtin_huge_val();
-    }
-    if (slots) block_max_atomic(m, slots);
-}
-
 struct SpecBcArgs {
     const DevMech* __restrict__ M;
     Arr<double> sb;
@@ -2282,20 +2289,12 @@ int implicit_solve(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sb,
     hipLaunchKernelGGL(k_spec_rhs, grid1d(nc, 256), dim3(256), 0, ctx->stream, r, W->f, W->x);
     VD_CHECK_LAUNCH();
     const double alpha = 1.0 / dt, beta = 1.0;
-    // the Jacobi bound rho of the operator (harmonic face maxima) decides
-    // between smoother-only iterations on the cell coefficient and full
-    // V-cycles on face arrays
-    double rho;
-    VD_TRY(jacobi_rho(ctx, *W, 2, alpha, beta, 0.0, W->dcell, &rho));
-    const bool smooth = rho <= SMOOTH_ONLY_RHO;
-    if (!smooth)
-        for (int d = 0; d < g->dim; d++)
-            hipLaunchKernelGGL(k_harmonic_faces, grid1d(nfacef(n, d), 256), dim3(256), 0, ctx->stream, W->dcell,
-                               n[0], n[1], n[2], d, g->dim, W->bf0[d], (unsigned long long*)nullptr);
     VD_TRY(side_setup(ctx, *W, bclo, bchi, beta, 0.0, W->dcell));
-    int st = mg_iterate(ctx, *W, smooth ? 2 : 0, alpha, beta, 0.0, W->dcell, bclo, bchi, smooth,
-                        prm->linsolve_reltol, prm->linsolve_abstol, prm->linsolve_maxiter,
-                        prm->mg_pre_smooth, prm->mg_post_smooth, info);
+    // smoother-only iterations on the cell coefficient when the operator is
+    // diagonally dominant enough, else V-cycles on harmonic face arrays
+    int st = mg_iterate(ctx, *W, 2, alpha, beta, 0.0, W->dcell, bclo, bchi, 2, prm->linsolve_reltol,
+                        prm->linsolve_abstol, prm->linsolve_maxiter, prm->mg_pre_smooth, prm->mg_post_smooth,
+                        info);
     if (st != VD_OK && st != VD_ENOCONV) return st;
     SpecPostArgs p;
     p.x = W->x;
@@ -2369,7 +2368,7 @@ int mlmg_solve(vd_ctx* ctx, const vd_mlmg_problem* P, vd_array4* phi, const vd_a
     VD_TRY(side_setup(ctx, *W, P->bc_lo, P->bc_hi, P->beta, 1.0));
     const int bm = constb ? 1 : 0;
     double rho;
-    VD_TRY(jacobi_rho(ctx, *W, bm, P->alpha_a, P->beta, 1.0, nullptr, &rho));
+    VD_TRY(jacobi_rho(ctx, *W, bm, P->alpha_a, P->beta, 1.0, &rho));
     int st = mg_iterate(ctx, *W, bm, P->alpha_a, P->beta, 1.0, nullptr, P->bc_lo, P->bc_hi,
                         rho <= SMOOTH_ONLY_RHO, P->reltol, P->abstol, P->maxiter, P->pre_smooth,
                         P->post_smooth, info);
