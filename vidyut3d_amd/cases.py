@@ -252,13 +252,14 @@ def synthetic_mechanism(species, charges, molwt, nreac, frac_electron, seed, ion
     return m
 
 
-def air_3d(n=256, seed=1, nsteps=200, dt=1e-12, L=0.1, voltage=1.0e4):
+def air_3d(n=256, seed=1, nsteps=200, dt=1e-12, L=0.1, voltage=1.0e4, state_seed=None):
     """3-D air-like discharge: 10 species (E, 3 positive ions, O2-, 5
     background neutrals at 2.4e25 m^-3), 30 seeded reactions, charged
     densities log-normal (median 1e15 m^-3, sigma_ln = 1), Te = 1 eV,
     Dirichlet 0 / 10 kV on the z faces, homogeneous Neumann on x and y,
     WENO-Z transport, L = 10 cm cube, dt = 1e-12 s (Source/Vidyut.H
-    default vidyut.dt)."""
+    default vidyut.dt).  `seed` draws the mechanism and, unless
+    `state_seed` is given, the initial state."""
     m = synthetic_mechanism(AIR_SPECIES, AIR_CHARGE, AIR_MW, 30, 0.5, seed,
                             ion_cap=(dt, TE_1EV))
     p = default_params()
@@ -280,7 +281,7 @@ def air_3d(n=256, seed=1, nsteps=200, dt=1e-12, L=0.1, voltage=1.0e4):
     g = abi.make_geom(nn, [0.0] * 3, [L] * 3, [0, 0, 0])
     ns = len(AIR_SPECIES)
     c = abi.comp(ns)
-    rng = np.random.default_rng(seed + 1000)
+    rng = np.random.default_rng((seed if state_seed is None else state_seed) + 1000)
     st = np.zeros((c["NVAR"], nn[2], nn[1], nn[0]))
     for s in range(ns):
         if AIR_CHARGE[s] != 0:
