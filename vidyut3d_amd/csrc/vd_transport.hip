@@ -196,6 +196,12 @@ struct PencilArgs {
     int3 flo;              // first face index (box lo)
     int nx, no, nd;        // cells along x, the other direction, and D
     int nchunk, chunk;     // face chunks along D
+    // divergence fused into the last direction (fuse = 1): dsdt of cell m
+    // from the x (and y) face arrays, rxn and this pencil's own two faces,
+    // in the order of k_dsdt; the D-direction faces are then not stored
+    int fuse;
+    Arr<double> fx, fy, rxn, ds;
+    double dx0, dx1, dx2;
 };
 
 template <int S, int D>
@@ -226,8 +232,11 @@ __global__ void __launch_bounds__(256) k_pencil_flux(PencilArgs a) {
     };
 #pragma unroll
     for (int q = 0; q < 5; q++) cell(f0 - 3 + q, w_u[q], w_f[q], w_v[q]);
-    double Lu_prev = 0.0, Lf_prev = 0.0, v_prev = 0.0;
-    for (int c = f0 - 1; c < f1; c++) {
+    // with the fused divergence a chunk also forms its first face above the
+    // chunk, so every cell of the chunk has both of its faces here
+    const int fend = a.fuse && ch != a.nchunk - 1 ? f1 + 1 : f1;
+    double Lu_prev = 0.0, Lf_prev = 0.0, v_prev = 0.0, F_prev = 0.0;
+    for (int c = f0 - 1; c < fend; c++) {
         double Lu, Ru, Lf, Rf;
         weno5_lr<S>(w_u[0], w_u[1], w_u[2], w_u[3], w_u[4], Lu, Ru);
         weno5_lr<S>(w_f[0], w_f[1], w_f[2], w_f[3], w_f[4], Lf, Rf);
@@ -238,13 +247,24 @@ __global__ void __launch_bounds__(256) k_pencil_flux(PencilArgs a) {
             double fl = 0.5 * (Lf_prev + lam * Lu_prev) + 0.5 * (Rf - lam * Ru);
             const int fd = mlo + c;
             if (fd == a.lo_zero || fd == a.hi_zero) fl = 0.0;
-            if (D == 1) a.flux.p[a.flux.off(ig, fd, kg)] = fl;
-            else a.flux.p[a.flux.off(ig, jg, fd)] = fl;
+            if (!a.fuse) {
+                if (D == 1) a.flux.p[a.flux.off(ig, fd, kg)] = fl;
+                else a.flux.p[a.flux.off(ig, jg, fd)] = fl;
+            } else if (c > f0) {
+                // cell m = c-1 has faces c-1 (F_prev) and c (fl)
+                const int mg = fd - 1;
+                const int jm = D == 1 ? mg : jg, km = D == 1 ? kg : mg;
+                double v = (a.fx(ig, jm, km) - a.fx(ig + 1, jm, km)) / a.dx0 + a.rxn(ig, jm, km, a.specid);
+                if (D == 2) v += (a.fy(ig, jm, km) - a.fy(ig, jm + 1, km)) / a.dx1;
+                v += (F_prev - fl) / (D == 1 ? a.dx1 : a.dx2);
+                a.ds(ig, jm, km) = v;
+            }
+            F_prev = fl;
         }
         Lu_prev = Lu;
         Lf_prev = Lf;
         v_prev = vc;
-        if (c + 1 < f1) {
+        if (c + 1 < fend) {
 #pragma unroll
             for (int q = 0; q < 4; q++) {
                 w_u[q] = w_u[q + 1];
@@ -343,8 +363,20 @@ static void launch_flux(vd_ctx* ctx, const FluxArgs& a) {
 
 // chunk of faces per pencil thread along D: long enough to amortise the
 // 4-cell ramp, short enough to give the device >= ~8 waves per CU
-static int launch_pencil(vd_ctx* ctx, const FluxArgs& f, int d, int scheme, const vd_box* box) {
+static int launch_pencil(vd_ctx* ctx, const FluxArgs& f, int d, int scheme, const vd_box* box,
+                         const vd_array4* fx = nullptr, const vd_array4* fy = nullptr,
+                         const vd_array4* rxn = nullptr, vd_array4* ds = nullptr, const vd_geom* geom = nullptr) {
     PencilArgs a;
+    a.fuse = ds != nullptr;
+    if (a.fuse) {
+        a.fx = arr(fx);
+        a.fy = fy ? arr(fy) : a.fx;
+        a.rxn = arr(rxn);
+        a.ds = arr(ds);
+        a.dx0 = geom->dx[0];
+        a.dx1 = geom->dx[1];
+        a.dx2 = geom->dx[2];
+    }
     a.sb = f.sb;
     a.flux = f.flux;
     a.specid = f.specid;
@@ -423,9 +455,13 @@ int check_transport_args(const vd_params* prm, int specid, int nspec) {
     return VD_OK;
 }
 
+// with rxn and ds given (WENO-5 only), the divergence plus rxn is formed in
+// the last direction's flux kernel and written to ds (that direction's face
+// array is then left unwritten)
 int transport_flux(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4 flux[3],
                    const vd_box* box, const vd_geom* g, const int bc_lo[3], const int bc_hi[3],
-                   const vd_params* prm, double dt, int specid) {
+                   const vd_params* prm, double dt, int specid, const vd_array4* rxn = nullptr,
+                   vd_array4* ds = nullptr) {
     VD_TRY(check_transport_args(prm, specid, m->hdev.nspec));
     int ng = prm->hyp_order == 5 ? 3 : prm->hyp_order;
     for (int d = 0; d < g->dim; d++)
@@ -452,7 +488,10 @@ int transport_flux(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4
         a.dxmin = dxmin;
         a.dt = dt;
         if (prm->hyp_order == 5) {
-            if (d > 0) VD_TRY(launch_pencil(ctx, a, d, prm->weno_scheme, box));
+            if (d > 0 && ds && d == g->dim - 1)
+                VD_TRY(launch_pencil(ctx, a, d, prm->weno_scheme, box, &flux[0], g->dim == 3 ? &flux[1] : nullptr,
+                                     rxn, ds, g));
+            else if (d > 0) VD_TRY(launch_pencil(ctx, a, d, prm->weno_scheme, box));
             else VD_TRY(launch_xline(ctx, a, prm->weno_scheme, box));
             continue;
         }
@@ -527,8 +566,14 @@ int transport_rhs(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, const vd_a
     double cells = (double)(box->hi[0] - box->lo[0] + 1) * (box->hi[1] - box->lo[1] + 1) *
                    (box->hi[2] - box->lo[2] + 1);
     kt_begin(ctx, VD_KT_TRANSPORT);
-    if (prm->do_transport) VD_TRY(transport_flux(ctx, m, sb, fl, box, g, bc_lo, bc_hi, prm, dt, specid));
-    int st = dsdt(ctx, fl, rxn, out, box, g, specid);
+    int st = VD_OK;
+    if (prm->do_transport && prm->hyp_order == 5 && g->dim > 1) {
+        // WENO fluxes with the divergence fused into the last direction
+        st = transport_flux(ctx, m, sb, fl, box, g, bc_lo, bc_hi, prm, dt, specid, rxn, out);
+    } else {
+        if (prm->do_transport) VD_TRY(transport_flux(ctx, m, sb, fl, box, g, bc_lo, bc_hi, prm, dt, specid));
+        st = dsdt(ctx, fl, rxn, out, box, g, specid);
+    }
     kt_end(ctx, VD_KT_TRANSPORT, 8.0 * cells * (m->hdev.nspec + 6));
     return st;
 }
