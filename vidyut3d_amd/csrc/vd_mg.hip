@@ -328,11 +328,14 @@ __device__ __forceinline__ void fused_unroll(F& f, int t, int tend) {
     }
 }
 
-template <class T, int BM, int S, int RES, int ZI>
+// ACC: instead of storing the smoothed x, add it to the FP64 array xacc
+// (the correction of the last post-smooth of a V-cycle, x += e)
+template <class T, int BM, int S, int RES, int ZI, int ACC = 0>
 __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* __restrict__ xin,
                                                            T* __restrict__ xout, const T* __restrict__ rhs,
                                                            int zc, int ntx, int nty,
-                                                           double* __restrict__ part) {
+                                                           double* __restrict__ part,
+                                                           double* __restrict__ xacc) {
     // Register ring indexed by plane: plane p lives in slot (p - zlo) mod NV
     // (NV even), and the step loop is unrolled NV times, so every slot index
     // is a compile-time constant and no values move between registers.
@@ -394,6 +397,8 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
     auto offs = [&](int pc, bool ok, int p) -> unsigned {
         return ok ? (unsigned)(pc + pstr * p) * (unsigned)sizeof(T) : OOB;
     };
+    const __amdgpu_buffer_rsrc_t ra = buf_rsrc(xacc, (unsigned)((int64_t)pstr * L.nz * 8));
+    auto offd = [&](int pc, bool ok, int p) -> unsigned { return ok ? (unsigned)(pc + pstr * p) * 8u : OOB; };
 
     T vr[NV], vb[NV], fr[NV], fb[NV];
 #pragma unroll
@@ -424,6 +429,13 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
     auto step = [&](auto kc, int t) {
         constexpr int k = decltype(kc)::value;  // (t - zlo + 1) mod NV
         PF& q = (k & 1) ? qb : qa;
+        double xa0 = 0.0, xa1 = 0.0;  // ACC: x of the plane finished this step
+        if constexpr (ACC) {
+            const int po = t - S;
+            const bool outp = po >= z0 && po < zout;
+            xa0 = buf_ld<double>(ra, offd(pc0, outp && oc0, po));
+            xa1 = buf_ld<double>(ra, offd(pc1, outp && oc1, po));
+        }
         {
             const bool c0red = ((par + t + 1) & 1) == 0;
             vr[k] = c0red ? q.x0 : q.x1;
@@ -505,8 +517,14 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
         }
         {
             const bool c0red = ((par + po) & 1) == 0;
-            buf_st<T>(ro, offs(pc0, outp && oc0, po), c0red ? vr[qo] : vb[qo]);
-            buf_st<T>(ro, offs(pc1, outp && oc1, po), c0red ? vb[qo] : vr[qo]);
+            const T o0 = c0red ? vr[qo] : vb[qo], o1 = c0red ? vb[qo] : vr[qo];
+            if constexpr (ACC) {
+                buf_st<double>(ra, offd(pc0, outp && oc0, po), xa0 + (double)o0);
+                buf_st<double>(ra, offd(pc1, outp && oc1, po), xa1 + (double)o1);
+            } else {
+                buf_st<T>(ro, offs(pc0, outp && oc0, po), o0);
+                buf_st<T>(ro, offs(pc1, outp && oc1, po), o1);
+            }
         }
         load(q, t + 3);
         __syncthreads();
@@ -1136,19 +1154,26 @@ static bool fused_capable(const MGWork& W) {
 
 template <class T, int BM, int S, int RES>
 static void launch_fused_k(vd_ctx* ctx, const LevV<T>& v, const T* xin, T* xout, const T* rhs,
-                           double* part, int* nblk) {
+                           double* part, int* nblk, double* xacc = nullptr) {
     // the kernel reads xin unless it starts from zero (ZI)
     const int ntx = (v.nx + FZ_IX - 1) / FZ_IX, nty = (v.ny + FZ_IY - 1) / FZ_IY;
     const int n3[3] = {v.nx, v.ny, v.nz};
     const int zc = std::max(1, fused_zc(n3)), nzc = (v.nz + zc - 1) / zc;
     const int nb = ntx * nty * nzc;
     if (nblk) *nblk = nb;
+    if constexpr (RES == 0) {
+        if (xin && xacc) {
+            hipLaunchKernelGGL((k_gsrb_fused<T, BM, S, RES, 0, 1>), dim3((unsigned)nb), dim3(FZ_THREADS), 0,
+                               ctx->stream, v, xin, xout, rhs, zc, ntx, nty, part, xacc);
+            return;
+        }
+    }
     if (xin)
         hipLaunchKernelGGL((k_gsrb_fused<T, BM, S, RES, 0>), dim3((unsigned)nb), dim3(FZ_THREADS), 0, ctx->stream,
-                           v, xin, xout, rhs, zc, ntx, nty, part);
+                           v, xin, xout, rhs, zc, ntx, nty, part, (double*)nullptr);
     else if constexpr (RES == 0)
         hipLaunchKernelGGL((k_gsrb_fused<T, BM, S, RES, 1>), dim3((unsigned)nb), dim3(FZ_THREADS), 0, ctx->stream,
-                           v, xin, xout, rhs, zc, ntx, nty, part);
+                           v, xin, xout, rhs, zc, ntx, nty, part, (double*)nullptr);
 }
 
 // nsweep (1 or 2) red-black sweeps of x (zeros when xin is null) into
@@ -1157,23 +1182,26 @@ static void launch_fused_k(vd_ctx* ctx, const LevV<T>& v, const T* xin, T* xout,
 // max-norm of the residual of the result into part[0..*nblk)
 template <class T>
 static void launch_fused(vd_ctx* ctx, const MGWork& W, int bm, const LevV<T>& v, const T* xin, T* xout,
-                         const T* rhs, int nsweep, int res, double* part, int* nblk) {
+                         const T* rhs, int nsweep, int res, double* part, int* nblk, double* xacc = nullptr) {
     kt_begin(ctx, VD_KT_FUSED);
 #define VD_FUSED_BM(BMV)                                                                      \
-    if (nsweep == 2) launch_fused_k<T, BMV, 4, 0>(ctx, v, xin, xout, rhs, part, nblk);     \
+    if (nsweep == 2) launch_fused_k<T, BMV, 4, 0>(ctx, v, xin, xout, rhs, part, nblk, xacc);     \
     else if constexpr (std::is_same<T, double>::value) {                                      \
         if (res) launch_fused_k<T, BMV, 2, 1>(ctx, v, xin, xout, rhs, part, nblk);          \
-        else launch_fused_k<T, BMV, 2, 0>(ctx, v, xin, xout, rhs, part, nblk);              \
-    } else launch_fused_k<T, BMV, 2, 0>(ctx, v, xin, xout, rhs, part, nblk);
+        else launch_fused_k<T, BMV, 2, 0>(ctx, v, xin, xout, rhs, part, nblk, xacc);        \
+    } else launch_fused_k<T, BMV, 2, 0>(ctx, v, xin, xout, rhs, part, nblk, xacc);
     if (bm == 1) {
         VD_FUSED_BM(1)
     } else {
         VD_FUSED_BM(0)
     }
 #undef VD_FUSED_BM
-    // one pass: read x (unless zero), rhs and the coefficient data, write x
     const int nb = bm == 1 ? 0 : (bm == 0 ? W.dim : 1);
-    kt_end(ctx, VD_KT_FUSED, (double)sizeof(T) * v.nx * v.ny * v.nz * ((xin ? 3 : 2) + nb));
+    // one pass: read x (unless zero), rhs and the coefficient data; write x,
+    // or read-modify-write the FP64 accumulation target
+    const double N = (double)v.nx * v.ny * v.nz;
+    kt_end(ctx, VD_KT_FUSED,
+           sizeof(T) * N * ((xin ? 1 : 0) + 1 + nb) + (xacc ? 16.0 * N : (double)sizeof(T) * N));
 }
 
 template <class T>
@@ -1364,9 +1392,12 @@ static void launch_resid_restrict(vd_ctx* ctx, int dim, const LevV<T>& v, const 
 }
 
 
+// returns true when the finest post-smooth also added the correction to
+// xacc (x += e), so the caller skips that update
 template <class T>
-static void vcycle(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L, double alpha,
-                   double beta, double bconst, int nu1, int nu2) {
+static bool vcycle(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L, double alpha,
+                   double beta, double bconst, int nu1, int nu2, double* xacc = nullptr) {
+    bool accumulated = false;
     const int nl = W.nlev;
     // levels l0 .. nl-1 (at most VC_TAIL_CELLS cells each) run in one
     // workgroup launch; the finer ones level by level
@@ -1439,8 +1470,14 @@ static void vcycle(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L,
         if (F.e2 && W.c0uni && bm != 2) {
             for (int s = 0; s < nu2;) {
                 const int k = nu2 - s >= 2 ? 2 : 1;
-                launch_fused<T>(ctx, W, bm, v, F.e, F.e2, F.r, k, 0, nullptr, nullptr);
-                std::swap(F.e, F.e2);
+                if (l == 0 && xacc && s + k == nu2) {
+                    // last post-smooth of the finest level: x += e in the same pass
+                    launch_fused<T>(ctx, W, bm, v, F.e, F.e2, F.r, k, 0, nullptr, nullptr, xacc);
+                    accumulated = true;
+                } else {
+                    launch_fused<T>(ctx, W, bm, v, F.e, F.e2, F.r, k, 0, nullptr, nullptr);
+                    std::swap(F.e, F.e2);
+                }
                 s += k;
             }
         } else {
@@ -1450,6 +1487,7 @@ static void vcycle(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L,
             }
         }
     }
+    return accumulated;
 }
 
 template <class T>
@@ -1709,15 +1747,15 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
                 launch_gsrb<double>(ctx, W, bm, v0, W.x, W.f, 0, true);
                 launch_gsrb<double>(ctx, W, bm, v0, W.x, W.f, 1, true);
             } else if (W.prec == 0) {
-                vcycle<double>(ctx, W, bm, W.L64, alpha, beta, bconst, nu1, nu2);
-                hipLaunchKernelGGL(k_axpy_to_double<double>, grid1d(nc, 256), dim3(256), 0, ctx->stream,
-                                   W.x, W.L64[0].e, nc);
+                if (!vcycle<double>(ctx, W, bm, W.L64, alpha, beta, bconst, nu1, nu2, W.x))
+                    hipLaunchKernelGGL(k_axpy_to_double<double>, grid1d(nc, 256), dim3(256), 0, ctx->stream,
+                                       W.x, W.L64[0].e, nc);
             } else {
                 hipLaunchKernelGGL((k_convert<double, float>), grid1d(nc, 256), dim3(256), 0, ctx->stream,
                                    W.res, W.L32[0].r, nc);
-                vcycle<float>(ctx, W, bm, W.L32, alpha, beta, bconst, nu1, nu2);
-                hipLaunchKernelGGL(k_axpy_to_double<float>, grid1d(nc, 256), dim3(256), 0, ctx->stream,
-                                   W.x, W.L32[0].e, nc);
+                if (!vcycle<float>(ctx, W, bm, W.L32, alpha, beta, bconst, nu1, nu2, W.x))
+                    hipLaunchKernelGGL(k_axpy_to_double<float>, grid1d(nc, 256), dim3(256), 0, ctx->stream,
+                                       W.x, W.L32[0].e, nc);
             }
             VD_CHECK_LAUNCH();
             VD_TRY(resid0(ctx, W, bm, v0, &rn, nullptr, !smooth_only));
