@@ -159,9 +159,11 @@ __device__ __forceinline__ void face_coef(const LevV<T>& L, int d, int i, int j,
 
 // dg and of of cell (i,j,k); nb(d, side) returns the neighbour value on
 // side 0 (low) / 1 (high) of direction d when that neighbour is a cell of
-// the level (interior or periodic image)
-template <class T, int BM, int DIM, class NB, bool C0U = false>
-__device__ __forceinline__ void stencil_nb(const LevV<T>& L, int i, int j, int k, NB nb, T& dgo, T& ofo) {
+// the level (interior or periodic image); fc(d, blo, bhi) gives the face
+// coefficients of direction d
+template <class T, int DIM, class NB, class FC, bool C0U = false>
+__device__ __forceinline__ void stencil_nbf(const LevV<T>& L, int i, int j, int k, NB nb, FC fc, T& dgo,
+                                            T& ofo) {
     T dg = L.alpha, of = T(0);
 #pragma unroll
     for (int d = 0; d < DIM; d++) {
@@ -170,7 +172,7 @@ __device__ __forceinline__ void stencil_nb(const LevV<T>& L, int i, int j, int k
         const int per = d == 0 ? L.per0 : (d == 1 ? L.per1 : L.per2);
         const T s = d == 0 ? L.s0 : (d == 1 ? L.s1 : L.s2);
         T blo, bhi;
-        face_coef<T, BM, DIM>(L, d, i, j, k, blo, bhi);
+        fc(d, blo, bhi);
         T dd = T(0), oo = T(0);
         if (cd > 0 || per) {
             dd += blo;
@@ -189,6 +191,12 @@ __device__ __forceinline__ void stencil_nb(const LevV<T>& L, int i, int j, int k
     }
     dgo = dg;
     ofo = of;
+}
+
+template <class T, int BM, int DIM, class NB, bool C0U = false>
+__device__ __forceinline__ void stencil_nb(const LevV<T>& L, int i, int j, int k, NB nb, T& dgo, T& ofo) {
+    auto fc = [&](int d, T& blo, T& bhi) { face_coef<T, BM, DIM>(L, d, i, j, k, blo, bhi); };
+    stencil_nbf<T, DIM, NB, decltype(fc), C0U>(L, i, j, k, nb, fc, dgo, ofo);
 }
 
 // dg and of of cell (i,j,k) from x in memory.  Every load is issued
@@ -625,6 +633,232 @@ __global__ void __launch_bounds__(256) k_resid0(LevV<double> L, const double* __
                 atomic_max_nonneg(fslots + (size_t)d * NSLOT * SLOT_STRIDE + (lb % NSLOT) * SLOT_STRIDE,
                                   mm[(BN ? 2 : 1) + d]);
     }
+}
+
+// Residual of a 3-D level streamed along z.  A block owns RZ_X x RZ_Y
+// columns and up to zc planes; each thread walks one (i, j) column keeping
+// x -- and with BM 2 the cell coefficient -- of planes k-1, k, k+1 in
+// registers, so every plane is fetched once per column rather than once per
+// z-neighbour (the three planes of each array a row-per-block sweep keeps
+// in flight overflow an XCD's L2 at 256^2-cell planes).  In-plane
+// neighbours come from L1.  The arithmetic is stencil()'s term for term.
+// Block maxima go straight to the norm slot groups: 0 the residual, 1 |f|
+// (BN), 2 + d the face coefficients of direction d (FM).  C0U: every
+// boundary c0 is uniform per side (L.c0uni).
+constexpr int RZ_X = 64, RZ_Y = 4;
+template <int BM, bool BN, bool FM, bool C0U>
+__global__ void __launch_bounds__(RZ_X * RZ_Y) k_resid_z(LevV<double> L, const double* __restrict__ x,
+                                                         const double* __restrict__ f, double* __restrict__ r,
+                                                         int zc, int ntx, int nty,
+                                                         unsigned long long* __restrict__ slots) {
+    constexpr int NV = 1 + (BN ? 1 : 0) + (FM ? 3 : 0);
+    __shared__ double red[RZ_X * RZ_Y / 64][NV];
+    const unsigned lb = lblock();
+    const int ntile = ntx * nty;
+    const int tile = (int)(lb % (unsigned)ntile), zb = (int)(lb / (unsigned)ntile);
+    const int i = (tile % ntx) * RZ_X + (int)threadIdx.x % RZ_X;
+    const int j = (tile / ntx) * RZ_Y + (int)threadIdx.x / RZ_X;
+    const int z0 = zb * zc, z1 = min(z0 + zc, L.nz);
+    const bool own = i < L.nx && j < L.ny;
+    const int ic = min(i, L.nx - 1), jc = min(j, L.ny - 1);
+    const int pstr = L.nx * L.ny, c2 = ic + L.nx * jc;
+    // neighbour offsets as stencil() takes them: wrapped on a periodic side,
+    // the cell itself (value unused) on a wall
+    const int oxl = ic > 0 ? -1 : (L.per0 ? L.nx - 1 : 0);
+    const int oxh = ic < L.nx - 1 ? 1 : (L.per0 ? 1 - L.nx : 0);
+    const int oyl = jc > 0 ? -L.nx : (L.per1 ? (L.ny - 1) * L.nx : 0);
+    const int oyh = jc < L.ny - 1 ? L.nx : (L.per1 ? (1 - L.ny) * L.nx : 0);
+    auto zw = [&](int k) {
+        return k < 0 ? (L.per2 ? k + L.nz : 0) : (k >= L.nz ? (L.per2 ? k - L.nz : L.nz - 1) : k);
+    };
+    // the cell coefficient lives on the box grown by one cell
+    const int gx = L.nx + 2, gxy = gx * (L.ny + 2), g2 = (ic + 1) + gx * (jc + 1);
+    double xm = x[c2 + pstr * zw(z0 - 1)], xc = x[c2 + pstr * z0], xp = x[c2 + pstr * zw(z0 + 1)];
+    double dm = 0.0, dc = 0.0, dp = 0.0;
+    if constexpr (BM == 2) {
+        dm = L.dcell[g2 + gxy * z0];
+        dc = L.dcell[g2 + gxy * (z0 + 1)];
+        dp = L.dcell[g2 + gxy * (z0 + 2)];
+    }
+    double m[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) m[q] = 0.0;
+    for (int k = z0; k < z1; k++) {
+        const int ck = c2 + pstr * k;
+        const double xn = x[c2 + pstr * zw(k + 2)];  // next column value, two planes ahead
+        const double xl0 = x[ck + oxl], xh0 = x[ck + oxh], xl1 = x[ck + oyl], xh1 = x[ck + oyh];
+        const double fv = f[ck];
+        double dn = 0.0, dl0 = 0.0, dh0 = 0.0, dl1 = 0.0, dh1 = 0.0;
+        if constexpr (BM == 2) {
+            const int gk = g2 + gxy * (k + 1);
+            dn = L.dcell[g2 + gxy * min(k + 3, L.nz + 1)];
+            dl0 = L.dcell[gk - 1];
+            dh0 = L.dcell[gk + 1];
+            dl1 = L.dcell[gk - gx];
+            dh1 = L.dcell[gk + gx];
+        }
+        double fm[3] = {0.0, 0.0, 0.0};
+        auto nb = [&](int d, int side) -> double {
+            if (d == 0) return side ? xh0 : xl0;
+            if (d == 1) return side ? xh1 : xl1;
+            return side ? xp : xm;
+        };
+        auto fc = [&](int d, double& blo, double& bhi) {
+            if constexpr (BM == 2) {
+                blo = harm(d == 0 ? dl0 : (d == 1 ? dl1 : dm), dc);
+                bhi = harm(dc, d == 0 ? dh0 : (d == 1 ? dh1 : dp));
+            } else {
+                face_coef<double, BM, 3>(L, d, ic, jc, k, blo, bhi);
+            }
+            if (FM) fm[d] = fmax(fabs(blo), fabs(bhi));
+        };
+        double dg, of;
+        stencil_nbf<double, 3, decltype(nb), decltype(fc), C0U>(L, ic, jc, k, nb, fc, dg, of);
+        const double v = fv - (dg * xc - of);
+        if (r && own) r[ck] = v;
+        if (own) {
+            double a[NV];
+            a[0] = fabs(v);
+            if (BN) a[1] = fabs(fv);
+            if (FM)
+#pragma unroll
+                for (int d = 0; d < 3; d++) a[(BN ? 2 : 1) + d] = fm[d];
+#pragma unroll
+            for (int q = 0; q < NV; q++) m[q] = fmax(m[q], a[q] != a[q] ? __builtin_huge_val() : a[q]);
+        }
+        xm = xc;
+        xc = xp;
+        xp = xn;
+        dm = dc;
+        dc = dp;
+        dp = dn;
+    }
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        for (int o = 32; o > 0; o >>= 1) m[q] = fmax(m[q], __shfl_xor(m[q], o, 64));
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][q] = m[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        const int q = (int)threadIdx.x;
+        double mm = red[0][q];
+        for (int w = 1; w < RZ_X * RZ_Y / 64; w++) mm = fmax(mm, red[w][q]);
+        const int grp = q == 0 ? 0 : (BN && q == 1 ? 1 : 2 + (q - (BN ? 2 : 1)));
+        atomic_max_nonneg(slots + ((size_t)grp * NSLOT + lb % NSLOT) * SLOT_STRIDE, mm);
+    }
+}
+
+// One colour of a red-black sweep of a 3-D level (even nx) streamed along
+// z.  A thread owns the x-pair (i0, i0+1) of one row and walks up to zc
+// planes; on plane k it updates the cell of the pair whose colour is
+// `color`.  Every neighbour of that cell has the other colour, which this
+// pass does not write, so the z-neighbours are the pair values of planes
+// k-1 and k+1 held in registers from earlier loads, and the pair mate is
+// the x-neighbour on one side.  Arithmetic as k_gsrb.
+constexpr int GZ_X = 32, GZ_Y = 8;  // pairs x rows per block
+template <class T, int BM, bool C0U>
+__global__ void __launch_bounds__(GZ_X * GZ_Y) k_gsrb_z(LevV<T> L, T* __restrict__ x, const T* __restrict__ rhs,
+                                                        int color, int zc, int ntx, int nty) {
+    const unsigned lb = lblock();
+    const int ntile = ntx * nty;
+    const int tile = (int)(lb % (unsigned)ntile), zb = (int)(lb / (unsigned)ntile);
+    const int i0 = 2 * ((tile % ntx) * GZ_X + (int)threadIdx.x % GZ_X);
+    const int j = (tile / ntx) * GZ_Y + (int)threadIdx.x / GZ_X;
+    const int z0 = zb * zc, z1 = min(z0 + zc, L.nz);
+    const bool own = i0 < L.nx && j < L.ny;
+    const int ic = min(i0, L.nx - 2), jc = min(j, L.ny - 1);
+    const int pstr = L.nx * L.ny, c2 = ic + L.nx * jc;
+    // x-neighbours outside the pair (left of ic, right of ic+1) and the
+    // y-neighbours, as stencil() takes them: wrapped on a periodic side,
+    // the cell itself (value unused) on a wall
+    const int oxl = ic > 0 ? -1 : (L.per0 ? L.nx - 1 : 0);
+    const int oxh = ic + 1 < L.nx - 1 ? 1 : (L.per0 ? 1 - L.nx : 0);
+    const int oyl = jc > 0 ? -L.nx : (L.per1 ? (L.ny - 1) * L.nx : 0);
+    const int oyh = jc < L.ny - 1 ? L.nx : (L.per1 ? (1 - L.ny) * L.nx : 0);
+    auto zw = [&](int k) {
+        return k < 0 ? (L.per2 ? k + L.nz : 0) : (k >= L.nz ? (L.per2 ? k - L.nz : L.nz - 1) : k);
+    };
+    const int gx = L.nx + 2, gxy = gx * (L.ny + 2), g2 = (ic + 1) + gx * (jc + 1);
+    T am0, am1, ac0, ac1, ap0, ap1;  // x pair of planes k-1, k, k+1
+    {
+        const int cm = c2 + pstr * zw(z0 - 1), cc = c2 + pstr * z0, cp = c2 + pstr * zw(z0 + 1);
+        am0 = x[cm];
+        am1 = x[cm + 1];
+        ac0 = x[cc];
+        ac1 = x[cc + 1];
+        ap0 = x[cp];
+        ap1 = x[cp + 1];
+    }
+    T dm0 = T(0), dm1 = T(0), dc0 = T(0), dc1 = T(0), dp0 = T(0), dp1 = T(0);
+    if constexpr (BM == 2) {
+        dm0 = L.dcell[g2 + gxy * z0];
+        dm1 = L.dcell[g2 + 1 + gxy * z0];
+        dc0 = L.dcell[g2 + gxy * (z0 + 1)];
+        dc1 = L.dcell[g2 + 1 + gxy * (z0 + 1)];
+        dp0 = L.dcell[g2 + gxy * (z0 + 2)];
+        dp1 = L.dcell[g2 + 1 + gxy * (z0 + 2)];
+    }
+    for (int k = z0; k < z1; k++) {
+        const int u = (color + j + k) & 1;  // position of the updated cell in the pair
+        const int i = ic + u;
+        const int ck = c2 + pstr * k;
+        const int cn = c2 + pstr * zw(k + 2);
+        const T an0 = x[cn], an1 = x[cn + 1];
+        const T xo = x[ck + (u ? 1 + oxh : oxl)];
+        const T yl = x[ck + u + oyl], yh = x[ck + u + oyh];
+        const T fv = rhs[ck + u];
+        const T mate = u ? ac0 : ac1;
+        const T xl0 = u ? mate : xo, xh0 = u ? xo : mate;
+        const T zl = u ? am1 : am0, zh = u ? ap1 : ap0;
+        T dn0 = T(0), dn1 = T(0), dxo = T(0), dyl = T(0), dyh = T(0);
+        if constexpr (BM == 2) {
+            const int gn = g2 + gxy * min(k + 3, L.nz + 1), gk = g2 + gxy * (k + 1);
+            dn0 = L.dcell[gn];
+            dn1 = L.dcell[gn + 1];
+            dxo = L.dcell[gk + (u ? 2 : -1)];
+            dyl = L.dcell[gk + u - gx];
+            dyh = L.dcell[gk + u + gx];
+        }
+        auto nb = [&](int d, int side) -> T {
+            if (d == 0) return side ? xh0 : xl0;
+            if (d == 1) return side ? yh : yl;
+            return side ? zh : zl;
+        };
+        auto fc = [&](int d, T& blo, T& bhi) {
+            if constexpr (BM == 2) {
+                const T dcc = u ? dc1 : dc0, dmate = u ? dc0 : dc1;
+                const T lo = d == 0 ? (u ? dmate : dxo) : (d == 1 ? dyl : (u ? dm1 : dm0));
+                const T hi = d == 0 ? (u ? dxo : dmate) : (d == 1 ? dyh : (u ? dp1 : dp0));
+                blo = harm(lo, dcc);
+                bhi = harm(dcc, hi);
+            } else {
+                face_coef<T, BM, 3>(L, d, i, jc, k, blo, bhi);
+            }
+        };
+        T dg, of;
+        stencil_nbf<T, 3, decltype(nb), decltype(fc), C0U>(L, i, jc, k, nb, fc, dg, of);
+        if (own) x[ck + u] = (fv + of) / dg;
+        am0 = ac0;
+        am1 = ac1;
+        ac0 = ap0;
+        ac1 = ap1;
+        ap0 = an0;
+        ap1 = an1;
+        dm0 = dc0;
+        dm1 = dc1;
+        dc0 = dp0;
+        dc1 = dp1;
+        dp0 = dn0;
+        dp1 = dn1;
+    }
+}
+
+// z-chunk of the streamed kernels: the largest power of two <= 32 that
+// still gives about 2048 blocks (8 per CU)
+static int resid_zc(const int n[3], int ntile) {
+    int zc = 32;
+    while (zc > 2 && (int64_t)ntile * ((n[2] + zc - 1) / zc) < 2048) zc >>= 1;
+    return zc;
 }
 
 // max of n per-block partials into slot 0 of a slot group (one workgroup)
@@ -1354,8 +1588,31 @@ fail:
     return st;
 }
 
+// streamed colour sweeps (k_gsrb_z) on 3-D levels with even nx; VD_STREAM_SWEEPS=0 keeps the
+// row-per-block kernel (read once per process)
+static bool stream_sweeps() {
+    static const bool on = [] {
+        const char* e = getenv("VD_STREAM_SWEEPS");
+        return !(e && strcmp(e, "0") == 0);
+    }();
+    return on;
+}
+
 template <class T, int BM>
 static void launch_gsrb_bm(vd_ctx* ctx, int dim, const LevV<T>& v, T* x, const T* rhs, int color) {
+    if (dim == 3 && v.nx % 2 == 0 && stream_sweeps()) {
+        const int ntx = (v.nx / 2 + GZ_X - 1) / GZ_X, nty = (v.ny + GZ_Y - 1) / GZ_Y;
+        const int n3[3] = {v.nx, v.ny, v.nz};
+        const int zc = resid_zc(n3, ntx * nty);
+        const dim3 g((unsigned)(ntx * nty * ((v.nz + zc - 1) / zc)));
+        if (v.c0uni)
+            hipLaunchKernelGGL((k_gsrb_z<T, BM, true>), g, dim3(GZ_X * GZ_Y), 0, ctx->stream, v, x, rhs, color, zc,
+                               ntx, nty);
+        else
+            hipLaunchKernelGGL((k_gsrb_z<T, BM, false>), g, dim3(GZ_X * GZ_Y), 0, ctx->stream, v, x, rhs, color,
+                               zc, ntx, nty);
+        return;
+    }
     const int nh = (v.nx + 1) / 2;
     const int bx = nh > 64 ? 128 : 64;
     const int gx = (nh + bx - 1) / bx;
@@ -1559,6 +1816,37 @@ static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double
     double* part = W.part;
     double* res = write_res ? W.res : nullptr;  // the norm alone when no V-cycle follows
     kt_begin(ctx, VD_KT_RESID);
+    if (W.dim == 3 && stream_sweeps()) {
+        const int ntx = (W.n[0] + RZ_X - 1) / RZ_X, nty = (W.n[1] + RZ_Y - 1) / RZ_Y;
+        const int zc = resid_zc(W.n, ntx * nty);
+        const dim3 gz((unsigned)(ntx * nty * ((W.n[2] + zc - 1) / zc)));
+#define VD_RESIDZ3(BMV, BNV, FMV)                                                                             \
+    {                                                                                                         \
+        if (v0.c0uni)                                                                                         \
+            hipLaunchKernelGGL((k_resid_z<BMV, BNV, FMV, true>), gz, dim3(RZ_X * RZ_Y), 0, ctx->stream, v0,   \
+                               W.x, W.f, res, zc, ntx, nty, sl);                                              \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_resid_z<BMV, BNV, FMV, false>), gz, dim3(RZ_X * RZ_Y), 0, ctx->stream, v0,  \
+                               W.x, W.f, res, zc, ntx, nty, sl);                                              \
+    }
+#define VD_RESIDZ(BMV)                                                                                        \
+    {                                                                                                         \
+        if (fmax && bnorm)                                                                                    \
+            VD_RESIDZ3(BMV, true, true)                                                                       \
+        else if (bnorm)                                                                                       \
+            VD_RESIDZ3(BMV, true, false)                                                                      \
+        else                                                                                                  \
+            VD_RESIDZ3(BMV, false, false)                                                                     \
+    }
+        if (bm == 1)
+            VD_RESIDZ(1)
+        else if (bm == 0)
+            VD_RESIDZ(0)
+        else
+            VD_RESIDZ(2)
+#undef VD_RESIDZ
+#undef VD_RESIDZ3
+    } else {  // one row per block
 #define VD_RESID(BMV, DIMV)                                                                                   \
     {                                                                                                         \
         if (fmax && bnorm)                                                                                    \
@@ -1571,15 +1859,16 @@ static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double
             hipLaunchKernelGGL((k_resid0<BMV, DIMV, false, false>), g, b, 0, ctx->stream, v0, W.x, W.f, res,   \
                                part, gx, bs, fs);                                                             \
     }
-    if (bm == 1) {
-        if (W.dim == 3) VD_RESID(1, 3) else VD_RESID(1, 2)
-    } else if (bm == 0) {
-        if (W.dim == 3) VD_RESID(0, 3) else VD_RESID(0, 2)
-    } else {
-        if (W.dim == 3) VD_RESID(2, 3) else VD_RESID(2, 2)
-    }
+        if (bm == 1) {
+            if (W.dim == 3) VD_RESID(1, 3) else VD_RESID(1, 2)
+        } else if (bm == 0) {
+            if (W.dim == 3) VD_RESID(0, 3) else VD_RESID(0, 2)
+        } else {
+            if (W.dim == 3) VD_RESID(2, 3) else VD_RESID(2, 2)
+        }
 #undef VD_RESID
-    hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, ctx->stream, part, nblk, sl);
+        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, ctx->stream, part, nblk, sl);
+    }
     const int nb = bm == 1 ? 0 : (bm == 0 ? W.dim : 1);
     kt_end(ctx, VD_KT_RESID, 8.0 * (double)nc * ((write_res ? 4 : 3) + nb));
     VD_CHECK_LAUNCH();
@@ -1682,7 +1971,7 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
             for (int d = 0; d < W.dim; d++) rho += 2.0 * fabs(beta / (W.dx[d] * W.dx[d])) * bmax[d];
             smooth_only = rho / fabs(alpha) <= SMOOTH_ONLY_RHO;
         }
-        if (!smooth_only) {
+        if (!smooth_only && bm == 2) {
             for (int d = 0; d < W.dim; d++)
                 hipLaunchKernelGGL(k_harmonic_faces, grid1d(nfacef(W.n, d), 256), dim3(256), 0, ctx->stream, dcell,
                                    W.n[0], W.n[1], W.n[2], d, W.dim, W.bf0[d], (unsigned long long*)nullptr);
@@ -1932,7 +2221,7 @@ static int grad_faces(vd_ctx* ctx, MGWork& W, const int tlo[3], const int thi[3]
 // c0 and rhs shift for every non-periodic side (sequential sides: no races
 // on corner cells).  bcv / rob must be filled.
 static int side_setup(vd_ctx* ctx, MGWork& W, const int tlo[3], const int thi[3], double beta,
-                      double bconst, const double* dcell = nullptr) {
+                      double bconst, const double* dcell = nullptr, bool use_const = false) {
     for (int d = 0; d < W.dim; d++) {
         if (W.per[d]) continue;
         for (int s = 0; s < 2; s++) {
@@ -1951,7 +2240,7 @@ static int side_setup(vd_ctx* ctx, MGWork& W, const int tlo[3], const int thi[3]
             a.ra = rob ? W.rob[d][s][0] : nullptr;
             a.rb = rob ? W.rob[d][s][1] : nullptr;
             a.rf = rob ? W.rob[d][s][2] : nullptr;
-            a.bf = (W.constb || dcell) ? nullptr : W.bf0[d];
+            a.bf = (W.constb || dcell || use_const) ? nullptr : W.bf0[d];
             a.dcell = dcell;
             a.bconst = bconst;
             a.c0 = W.c00[d][s];
@@ -2120,9 +2409,12 @@ struct CellDArgs {
 };
 
 // cell specDiff on the box grown by one (ScalarSolve.cpp:483-497)
-__global__ void k_cell_diff(CellDArgs a, double* __restrict__ dcell) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y, k = blockIdx.z;
-    if (i >= a.gnx) return;
+__global__ void __launch_bounds__(256) k_cell_diff(CellDArgs a, double* __restrict__ dcell) {
+    const int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)a.gnx * a.gny * a.gnz) return;
+    const int i = (int)(t % a.gnx);
+    const int64_t q = t / a.gnx;
+    const int j = (int)(q % a.gny), k = (int)(q / a.gny);
     const int gz = a.dim == 3 ? 1 : 0;
     const double* te = a.sb.comp(VD_ETEMP(a.ns));
     dcell[i + a.gnx * (j + a.gny * k)] =
@@ -2267,7 +2559,16 @@ int implicit_solve(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sb,
     dims_of(box, g, n, lo);
     int64_t nc = ncell(n);
     const double Tg = prm->gas_temperature;
-    {
+    // a diffusion coefficient that does not depend on Te (constant or
+    // Einstein at the gas temperature) is uniform, and so is every harmonic
+    // face average of it: the solve then runs on the constant-coefficient
+    // stencil (no cell-coefficient array, no per-face averages)
+    const bool constd = spec < ns && m->hdev.diff_kind[spec] != VD_DIFF_EINSTEIN_TE;
+    double bc = 0.0;
+    if (constd) {
+        const double D = spec_diff(m->hdev, spec, 0.0, Tg);
+        bc = (D + D == 0.0) ? 0.0 : (2.0 * D * D) / (D + D);
+    } else {
         CellDArgs cd;
         cd.M = m->dev;
         cd.sb = arr(sb);
@@ -2279,9 +2580,8 @@ int implicit_solve(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sb,
         cd.gny = n[1] + 2;
         cd.gnz = g->dim == 3 ? n[2] + 2 : 1;
         cd.Tg = Tg;
-        const int bx = cd.gnx > 64 ? 128 : 64;
-        hipLaunchKernelGGL(k_cell_diff, dim3((cd.gnx + bx - 1) / bx, cd.gny, cd.gnz), dim3(bx), 0, ctx->stream,
-                           cd, W->dcell);
+        hipLaunchKernelGGL(k_cell_diff, grid1d((int64_t)cd.gnx * cd.gny * cd.gnz, 256), dim3(256), 0,
+                           ctx->stream, cd, W->dcell);
     }
     for (int d = 0; d < g->dim; d++) {
         if (W->per[d]) continue;
@@ -2327,10 +2627,11 @@ int implicit_solve(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sb,
     hipLaunchKernelGGL(k_spec_rhs, grid1d(nc, 256), dim3(256), 0, ctx->stream, r, W->f, W->x);
     VD_CHECK_LAUNCH();
     const double alpha = 1.0 / dt, beta = 1.0;
-    VD_TRY(side_setup(ctx, *W, bclo, bchi, beta, 0.0, W->dcell));
-    // smoother-only iterations on the cell coefficient when the operator is
-    // diagonally dominant enough, else V-cycles on harmonic face arrays
-    int st = mg_iterate(ctx, *W, 2, alpha, beta, 0.0, W->dcell, bclo, bchi, 2, prm->linsolve_reltol,
+    const double* dcell = constd ? nullptr : W->dcell;
+    VD_TRY(side_setup(ctx, *W, bclo, bchi, beta, bc, dcell, constd));
+    // smoother-only iterations when the operator is diagonally dominant
+    // enough, else V-cycles (on harmonic face arrays of the cell coefficient)
+    int st = mg_iterate(ctx, *W, constd ? 1 : 2, alpha, beta, bc, dcell, bclo, bchi, 2, prm->linsolve_reltol,
                         prm->linsolve_abstol, prm->linsolve_maxiter, prm->mg_pre_smooth, prm->mg_post_smooth,
                         info);
     if (st != VD_OK && st != VD_ENOCONV) return st;

@@ -85,7 +85,15 @@ class Vidyut {
                     const std::vector<double>& gthr);
 
     MultiFab& phi_new() { return phi_new_[0]; }
-    MultiFab& phi_old() { return phi_old_[0]; }
+    // after an Evolve with one corrector the start-of-step species and EEN
+    // are taken from Sborder (which holds them unchanged) on first access
+    MultiFab& phi_old() {
+        if (phi_old_stale_) {
+            phi_old_[0].copy_from(Sborder_[0], 0, 0, ns_ + 1);
+            phi_old_stale_ = false;
+        }
+        return phi_old_[0];
+    }
     Real time() const { return t_new_; }
     void set_time(Real t) { t_new_ = t; }
     int step() const { return istep_; }
@@ -105,6 +113,7 @@ class Vidyut {
     int status_ = VD_OK;
     Real t_new_ = 0.0;
     int istep_ = 0;
+    bool phi_old_stale_ = false;
     std::vector<MultiFab> phi_new_, phi_old_;
     // persistent step data (the reference allocates these every step)
     std::vector<MultiFab> Sborder_, Sborder_old_, expl_src_, rxn_src_;

@@ -259,9 +259,14 @@ int Vidyut::Evolve(int nsteps, Real dt, int ncorr) {
         TRY(find_time_scales(0, dt_edrift, dt_ediff, dt_diel));
         const Real cur_time = t_new_;
         // Sborder_old is read only in its valid region by the implicit
-        // solves (species and energy), so phi_old stands in for it; with one
-        // corrector nothing else of phi_old is read.
-        TRY(phi_old_[0].copy_from(phi_new_[0], 0, 0, ncorr > 1 ? nvar_ : ns_ + 1));
+        // solves (species and energy).  With one corrector its species and
+        // EEN equal those of Sborder, which the step never writes (the
+        // solves write phi_new, the second FillPatch only POT..EFZ), so
+        // Sborder stands in for it and phi_old is copied only on access;
+        // with more correctors phi_old is kept for the averaging.
+        std::vector<MultiFab>& Sb_old = ncorr > 1 ? phi_old_ : Sborder_;
+        if (ncorr > 1) TRY(phi_old_[0].copy_from(phi_new_[0], 0, 0, nvar_));
+        phi_old_stale_ = false;
         for (int niter = 0; niter < ncorr; niter++) {
             // Sborder components read by the step: species, EEN, ETEMP (here)
             // and POT, EFX..EFZ (after the potential solve); EJH, EIH, EEH and
@@ -275,7 +280,7 @@ int Vidyut::Evolve(int nsteps, Real dt, int ncorr) {
                 TRY(rxn_src_[0].setVal(0.0));
             TRY(update_explsrc_at_all_levels(E, Sborder_, flux_, rxn_src_, expl_src_, prm_.eden_bc_lo,
                                              prm_.eden_bc_hi, cur_time));
-            int st = implicit_solve_scalar(cur_time, dt, E, Sborder_, phi_old_, expl_src_, prm_.eden_bc_lo,
+            int st = implicit_solve_scalar(cur_time, dt, E, Sborder_, Sb_old, expl_src_, prm_.eden_bc_lo,
                                            prm_.eden_bc_hi, gradne_fc_);
             if (st != VD_OK && st != VD_ENOCONV) return st;
             if (prm_.elecenergy_solve) {
@@ -283,7 +288,7 @@ int Vidyut::Evolve(int nsteps, Real dt, int ncorr) {
                                                  prm_.eenrg_bc_hi, cur_time));
                 TRY(compute_elecenergy_source(0, Sborder_[0], rxn_src_[0], efield_ec_[0], gradne_fc_[0],
                                               expl_src_[0], cur_time, dt, prm_.floor_jh));
-                st = implicit_solve_scalar(cur_time, dt, ns_, Sborder_, phi_old_, expl_src_, prm_.eenrg_bc_lo,
+                st = implicit_solve_scalar(cur_time, dt, ns_, Sborder_, Sb_old, expl_src_, prm_.eenrg_bc_lo,
                                            prm_.eenrg_bc_hi, grad_fc_);
                 if (st != VD_OK && st != VD_ENOCONV) return st;
             }
@@ -294,7 +299,7 @@ int Vidyut::Evolve(int nsteps, Real dt, int ncorr) {
                     const int* hi = desc_.charge[ind] != 0 ? prm_.ion_bc_hi : prm_.neutral_bc_hi;
                     TRY(update_explsrc_at_all_levels(ind, Sborder_, flux_, rxn_src_, expl_src_, lo, hi,
                                                      cur_time));
-                    st = implicit_solve_scalar(cur_time, dt, ind, Sborder_, phi_old_, expl_src_, lo, hi,
+                    st = implicit_solve_scalar(cur_time, dt, ind, Sborder_, Sb_old, expl_src_, lo, hi,
                                                grad_fc_);
                     if (st != VD_OK && st != VD_ENOCONV) return st;
                 } else if (prm_.do_bg_reactions) {
@@ -311,6 +316,7 @@ int Vidyut::Evolve(int nsteps, Real dt, int ncorr) {
         }
         istep_++;
         t_new_ = cur_time + dt;
+        phi_old_stale_ = ncorr == 1;
     }
     return vd_ctx_sync(ctx_);
 }
