@@ -431,6 +431,51 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
     };
     double amax = 0.0;
     const int txm = tx > 0 ? tx - 1 : tx, txp = tx < FZ_PX - 1 ? tx + 1 : tx;
+    // BM 1 (constant coefficient b, uniform c0): stencil_nb's arithmetic with
+    // the boundary selects hoisted.  A neighbour across a wall of the level
+    // holds 0 here (lanes outside the level load 0 and are never written,
+    // planes outside it are never loaded), so its term b * 0 adds nothing
+    // and every neighbour term is taken unconditionally; the diagonal
+    // depends only on which walls the cell touches, so its x and y parts are
+    // fixed per lane and its z part per plane.
+    const T bq = L.bconst;
+    // bq * (1 - c0) of each wall (uniform values)
+    T wx0 = T(0), wx1 = T(0), wy0 = T(0), wy1 = T(0), wz0 = T(0), wz1 = T(0);
+    if constexpr (BM == 1) {
+        wx0 = bq * (T(1) - L.c0u[0][0]);
+        wx1 = bq * (T(1) - L.c0u[0][1]);
+        wy0 = bq * (T(1) - L.c0u[1][0]);
+        wy1 = bq * (T(1) - L.c0u[1][1]);
+        wz0 = bq * (T(1) - L.c0u[2][0]);
+        wz1 = bq * (T(1) - L.c0u[2][1]);
+    }
+    auto ddw = [&](bool lin, bool hin, T wl, T wh) -> T {
+        T dd = T(0);
+        dd = lin ? dd + bq : dd + wl;
+        dd = hin ? dd + bq : dd + wh;
+        return dd;
+    };
+    T dgxy0 = T(0), dgxy1 = T(0);
+    if constexpr (BM == 1) {
+        const T ddy = ddw(jcc > 0 || L.per1, jcc < L.ny - 1 || L.per1, wy0, wy1);
+        dgxy0 = L.alpha + L.s0 * ddw(ic0c > 0 || L.per0, ic0c < L.nx - 1 || L.per0, wx0, wx1);
+        dgxy0 = dgxy0 + L.s1 * ddy;
+        dgxy1 = L.alpha + L.s0 * ddw(ic1c > 0 || L.per0, ic1c < L.nx - 1 || L.per0, wx0, wx1);
+        dgxy1 = dgxy1 + L.s1 * ddy;
+    }
+    auto st1 = [&](int u, int pc, T xl, T xh, T yl, T yh, T zl, T zh, T& dg, T& of) {
+        const T ddz = ddw(pc > 0 || L.per2, pc < L.nz - 1 || L.per2, wz0, wz1);
+        dg = (u ? dgxy1 : dgxy0) + L.s2 * ddz;
+        T o0 = bq * xl;
+        o0 = o0 + bq * xh;
+        T o1 = bq * yl;
+        o1 = o1 + bq * yh;
+        T o2 = bq * zl;
+        o2 = o2 + bq * zh;
+        of = L.s0 * o0;
+        of = of + L.s1 * o1;
+        of = of + L.s2 * o2;
+    };
     const int tym = ty > 0 ? ty - 1 : ty, typ = ty < FZ_RY - 1 ? ty + 1 : ty;
     const int tend = zout + S;
     PF qa, qb;
@@ -475,8 +520,12 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
             const T other = cs ? vr[q_s] : vb[q_s];
             const T zlo_v = cs ? vr[q_lo] : vb[q_lo];
             const T zhi_v = cs ? vr[q_hi] : vb[q_hi];
-            const T xm = u ? other : nbuf[s - 1][ty][txm];
-            const T xp = u ? nbuf[s - 1][ty][txp] : other;
+            // one LDS read: the x-neighbour outside the pair is on the left of
+            // cell 0 (u = 0) or on the right of cell 1 (u = 1); an address
+            // select, so the read is not predicated per lane
+            const T xo = nbuf[s - 1][ty][u ? txp : txm];
+            const T xm = u ? other : xo;
+            const T xp = u ? xo : other;
             const T ym = nbuf[s - 1][tym][tx], yp = nbuf[s - 1][typ][tx];
             auto nb = [&](int d, int side) -> T {
                 if (d == 0) return side == 0 ? xm : xp;
@@ -484,8 +533,11 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
                 return side == 0 ? zlo_v : zhi_v;
             };
             T dg, of;
-            stencil_nb<T, BM, 3, decltype(nb), true>(L, u ? ic1c : ic0c, jcc, min(max(p, 0), L.nz - 1), nb,
-                                                      dg, of);
+            if constexpr (BM == 1)
+                st1(u, min(max(p, 0), L.nz - 1), xm, xp, ym, yp, zlo_v, zhi_v, dg, of);
+            else
+                stencil_nb<T, BM, 3, decltype(nb), true>(L, u ? ic1c : ic0c, jcc, min(max(p, 0), L.nz - 1), nb,
+                                                          dg, of);
             if (cs) {
                 const T nv = (fb[q_s] + of) / dg;
                 vb[q_s] = valid ? nv : vb[q_s];
@@ -507,8 +559,9 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
                 const T other = c ? vr[qo] : vb[qo];
                 const T zlo_v = c ? vr[qlo] : vb[qlo];
                 const T zhi_v = c ? vr[qhi] : vb[qhi];
-                const T xm = u ? other : fbuf[1 - c][ty][txm];
-                const T xp = u ? fbuf[1 - c][ty][txp] : other;
+                const T xo = fbuf[1 - c][ty][u ? txp : txm];
+                const T xm = u ? other : xo;
+                const T xp = u ? xo : other;
                 const T ym = fbuf[1 - c][tym][tx], yp = fbuf[1 - c][typ][tx];
                 auto nb = [&](int d, int side) -> T {
                     if (d == 0) return side == 0 ? xm : xp;
@@ -516,7 +569,10 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
                     return side == 0 ? zlo_v : zhi_v;
                 };
                 T dg, of;
-                stencil_nb<T, BM, 3, decltype(nb), true>(L, u ? ic1c : ic0c, jcc, poc, nb, dg, of);
+                if constexpr (BM == 1)
+                    st1(u, poc, xm, xp, ym, yp, zlo_v, zhi_v, dg, of);
+                else
+                    stencil_nb<T, BM, 3, decltype(nb), true>(L, u ? ic1c : ic0c, jcc, poc, nb, dg, of);
                 const double r = (c ? fb[qo] : fr[qo]) - (dg * own - of);
                 double a = fabs(r);
                 if (a != a) a = __builtin_huge_val();
