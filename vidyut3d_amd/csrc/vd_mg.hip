@@ -1038,6 +1038,183 @@ __global__ void __launch_bounds__(128) k_prolong(LevV<T> C, const T* __restrict_
     ef[i + fnx * (j + fny * k)] += prolong_cell<T, DIM>(C, ec, i, j, k);
 }
 
+// Residual-restriction of a 3-D level, streamed along z: a thread owns one
+// coarse column (I, J) and walks coarse planes K; the 2 x 2 fine columns
+// under it keep e of fine planes 2K-1 .. 2K+2 in registers, so only the
+// in-plane neighbours outside the 2 x 2 block and rhs are loaded per
+// plane.  Arithmetic (terms and their order) as resid_restrict_cell.
+constexpr int RR_X = 64, RR_Y = 4;
+template <class T, int BM, bool C0U>
+__global__ void __launch_bounds__(RR_X * RR_Y) k_resid_restrict_z(LevV<T> F, const T* __restrict__ e,
+                                                                  const T* __restrict__ rhs, int cnx, int cny,
+                                                                  int cnz, T* __restrict__ rc, int zc, int ntx,
+                                                                  int nty) {
+    const unsigned lb = lblock();
+    const int ntile = ntx * nty;
+    const int tile = (int)(lb % (unsigned)ntile), zb = (int)(lb / (unsigned)ntile);
+    const int I = (tile % ntx) * RR_X + (int)threadIdx.x % RR_X;
+    const int J = (tile / ntx) * RR_Y + (int)threadIdx.x / RR_X;
+    const bool own = I < cnx && J < cny;
+    const int Ic = min(I, cnx - 1), Jc = min(J, cny - 1);
+    const int K0 = zb * zc, K1 = min(K0 + zc, cnz);
+    const int fi = 2 * Ic, fj = 2 * Jc, fps = F.nx * F.ny;
+    // in-plane offsets of the neighbours outside the 2 x 2 block (as
+    // stencil() takes them: wrapped when periodic, the cell itself on a
+    // wall, where the value is not used)
+    const int oxl = fi > 0 ? -1 : (F.per0 ? F.nx - 1 : 0);
+    const int oxh = fi + 1 < F.nx - 1 ? 1 : (F.per0 ? 1 - F.nx : 0);
+    const int oyl = fj > 0 ? -F.nx : (F.per1 ? (F.ny - 1) * F.nx : 0);
+    const int oyh = fj + 1 < F.ny - 1 ? F.nx : (F.per1 ? (1 - F.ny) * F.nx : 0);
+    auto kw = [&](int k) {
+        return k < 0 ? (F.per2 ? k + F.nz : 0) : (k >= F.nz ? (F.per2 ? k - F.nz : F.nz - 1) : k);
+    };
+    const int b00 = fi + F.nx * fj;  // fine (fi, fj) in a plane
+    struct P4 {
+        T v[2][2];  // [jj][ii]
+    };
+    auto ld4 = [&](int k) -> P4 {
+        const int b = b00 + fps * kw(k);
+        P4 q;
+        q.v[0][0] = e[b];
+        q.v[0][1] = e[b + 1];
+        q.v[1][0] = e[b + F.nx];
+        q.v[1][1] = e[b + F.nx + 1];
+        return q;
+    };
+    P4 pm = ld4(2 * K0 - 1), p0 = ld4(2 * K0), p1 = ld4(2 * K0 + 1), p2 = ld4(2 * K0 + 2);
+    for (int K = K0; K < K1; K++) {
+        const P4 p3 = ld4(2 * K + 3), p4 = ld4(2 * K + 4);
+        T s = T(0);
+#pragma unroll
+        for (int kk = 0; kk < 2; kk++) {
+            const int k = 2 * K + kk;
+            const P4& zl = kk ? p0 : pm;
+            const P4& zc_ = kk ? p1 : p0;
+            const P4& zh = kk ? p2 : p1;
+            const int bk = b00 + fps * k;
+            T xo[2][2], yo[2][2], rv[2][2];  // [row or column][side]
+#pragma unroll
+            for (int q = 0; q < 2; q++) {
+                xo[q][0] = e[bk + F.nx * q + oxl];
+                xo[q][1] = e[bk + F.nx * q + 1 + oxh];
+                yo[q][0] = e[bk + q + oyl];
+                yo[q][1] = e[bk + F.nx + q + oyh];
+                rv[q][0] = rhs[bk + F.nx * q];
+                rv[q][1] = rhs[bk + F.nx * q + 1];
+            }
+#pragma unroll
+            for (int jj = 0; jj < 2; jj++)
+#pragma unroll
+                for (int ii = 0; ii < 2; ii++) {
+                    const int i = fi + ii, j = fj + jj;
+                    auto nb = [&](int d, int side) -> T {
+                        if (d == 0) return side ? (ii ? xo[jj][1] : zc_.v[jj][1]) : (ii ? zc_.v[jj][0] : xo[jj][0]);
+                        if (d == 1) return side ? (jj ? yo[ii][1] : zc_.v[1][ii]) : (jj ? zc_.v[0][ii] : yo[ii][0]);
+                        return side ? zh.v[jj][ii] : zl.v[jj][ii];
+                    };
+                    auto fc = [&](int d, T& blo, T& bhi) { face_coef<T, BM, 3>(F, d, i, j, k, blo, bhi); };
+                    T dg, of;
+                    stencil_nbf<T, 3, decltype(nb), decltype(fc), C0U>(F, i, j, k, nb, fc, dg, of);
+                    s += rv[jj][ii] - (dg * zc_.v[jj][ii] - of);
+                }
+        }
+        if (own) rc[Ic + cnx * (Jc + cny * K)] = s * T(0.125);
+        pm = p1;
+        p0 = p2;
+        p1 = p3;
+        p2 = p4;
+    }
+}
+
+// e_f += P e_c on a 3-D level, streamed along z (uniform c0).  A thread
+// owns one fine column (i, j) and walks coarse planes K, producing fine
+// planes 2K and 2K+1.  The four coarse values a fine cell of the column
+// draws on in a coarse plane (its coarse column and the neighbours on its
+// side in x and y, x/y ghosts already scaled by c0) are kept for planes
+// K-1, K, K+1, so each is loaded once per column.  Arithmetic (terms,
+// order, ghost factors) as prolong_cell.
+constexpr int PZ_X = 64, PZ_Y = 4;
+template <class T>
+__global__ void __launch_bounds__(PZ_X * PZ_Y) k_prolong_z(LevV<T> C, const T* __restrict__ ec, int fnx, int fny,
+                                                           T* __restrict__ ef, int zc, int ntx, int nty) {
+    const unsigned lb = lblock();
+    const int ntile = ntx * nty;
+    const int tile = (int)(lb % (unsigned)ntile), zb = (int)(lb / (unsigned)ntile);
+    const int i = (tile % ntx) * PZ_X + (int)threadIdx.x % PZ_X;
+    const int j = (tile / ntx) * PZ_Y + (int)threadIdx.x / PZ_X;
+    const bool own = i < fnx && j < fny;
+    const int ic = min(i, fnx - 1), jc = min(j, fny - 1);
+    const int K0 = zb * zc, K1 = min(K0 + zc, C.nz);
+    const int I = ic >> 1, sx = ic & 1, J = jc >> 1, sy = jc & 1;
+    int nbx = sx ? I + 1 : I - 1, nby = sy ? J + 1 : J - 1;
+    T cx = T(1), cy = T(1);
+    if (nbx < 0 || nbx >= C.nx) {
+        if (C.per0) {
+            nbx = nbx < 0 ? C.nx - 1 : 0;
+        } else {
+            nbx = I;
+            cx = C.c0u[0][sx];
+        }
+    }
+    if (nby < 0 || nby >= C.ny) {
+        if (C.per1) {
+            nby = nby < 0 ? C.ny - 1 : 0;
+        } else {
+            nby = J;
+            cy = C.c0u[1][sy];
+        }
+    }
+    const int cps = C.nx * C.ny;
+    const int o00 = I + C.nx * J, o10 = nbx + C.nx * J, o01 = I + C.nx * nby, o11 = nbx + C.nx * nby;
+    // coarse plane of a plane index outside the level: wrapped (periodic)
+    // or the edge plane (its values are then scaled by the wall's c0)
+    auto kw = [&](int K) {
+        return K < 0 ? (C.per2 ? K + C.nz : 0) : (K >= C.nz ? (C.per2 ? K - C.nz : C.nz - 1) : K);
+    };
+    struct Q {
+        T a00, a10, a01, a11;
+    };
+    auto ld = [&](int K) -> Q {
+        const int b = cps * kw(K);
+        Q q;
+        q.a00 = ec[b + o00];
+        q.a10 = ec[b + o10] * cx;
+        q.a01 = ec[b + o01] * cy;
+        q.a11 = ec[b + o11] * cx * cy;
+        return q;
+    };
+    // the eight terms of prolong_cell: plane Kc first (kz = 0), then the
+    // neighbour plane (kz = 1, scaled by cz when it is a ghost)
+    auto interp = [&](const Q& qc, const Q& qn, T cz) -> T {
+        T v = T(0);
+        v += T(0.75) * T(0.75) * T(0.75) * qc.a00;
+        v += T(0.25) * T(0.75) * T(0.75) * qc.a10;
+        v += T(0.75) * T(0.25) * T(0.75) * qc.a01;
+        v += T(0.25) * T(0.25) * T(0.75) * qc.a11;
+        v += T(0.75) * T(0.75) * T(0.25) * (qn.a00 * cz);
+        v += T(0.25) * T(0.75) * T(0.25) * (qn.a10 * cz);
+        v += T(0.75) * T(0.25) * T(0.25) * (qn.a01 * cz);
+        v += T(0.25) * T(0.25) * T(0.25) * (qn.a11 * cz);
+        return v;
+    };
+    const int fps = fnx * fny, fc = ic + fnx * jc;
+    Q qm = ld(K0 - 1), qk = ld(K0), qp = ld(K0 + 1);
+    for (int K = K0; K < K1; K++) {
+        const Q qn = ld(K + 2);
+        const T czl = (K == 0 && !C.per2) ? C.c0u[2][0] : T(1);
+        const T czh = (K == C.nz - 1 && !C.per2) ? C.c0u[2][1] : T(1);
+        const int c0 = fc + fps * (2 * K), c1 = c0 + fps;
+        const T e0 = ef[c0], e1 = ef[c1];
+        if (own) {
+            ef[c0] = e0 + interp(qk, qm, czl);
+            ef[c1] = e1 + interp(qk, qp, czh);
+        }
+        qm = qk;
+        qk = qp;
+        qp = qn;
+    }
+}
+
 // CG bottom solve by one workgroup (all threads of the block call it);
 // dots accumulated in double
 template <class T, int BM, int DIM>
@@ -1693,6 +1870,18 @@ static void launch_gsrb(vd_ctx* ctx, const MGWork& W, int bm, const LevV<T>& v, 
 
 template <class T, int BM>
 static void launch_resid_restrict(vd_ctx* ctx, int dim, const LevV<T>& v, const Lev<T>& F, Lev<T>& Cc) {
+    if (dim == 3 && stream_sweeps()) {
+        const int ntx = (Cc.n[0] + RR_X - 1) / RR_X, nty = (Cc.n[1] + RR_Y - 1) / RR_Y;
+        const int zc = resid_zc(Cc.n, ntx * nty);
+        const dim3 g((unsigned)(ntx * nty * ((Cc.n[2] + zc - 1) / zc)));
+        if (v.c0uni)
+            hipLaunchKernelGGL((k_resid_restrict_z<T, BM, true>), g, dim3(RR_X * RR_Y), 0, ctx->stream, v, F.e, F.r,
+                               Cc.n[0], Cc.n[1], Cc.n[2], Cc.r, zc, ntx, nty);
+        else
+            hipLaunchKernelGGL((k_resid_restrict_z<T, BM, false>), g, dim3(RR_X * RR_Y), 0, ctx->stream, v, F.e,
+                               F.r, Cc.n[0], Cc.n[1], Cc.n[2], Cc.r, zc, ntx, nty);
+        return;
+    }
     const int bx = Cc.n[0] > 64 ? 128 : 64;
     const int gx = (Cc.n[0] + bx - 1) / bx;
     dim3 g((unsigned)gx * Cc.n[1] * Cc.n[2]), b(bx);
@@ -1773,13 +1962,23 @@ static bool vcycle(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L,
         Lev<T>& Cc = L[l + 1];
         LevV<T> vc = view<T>(W, Cc, alpha, beta, bconst);
         LevV<T> v = view<T>(W, F, alpha, beta, bconst);
-        const int bx = F.n[0] > 64 ? 128 : 64;
-        const int gx = (F.n[0] + bx - 1) / bx;
-        dim3 g((unsigned)gx * F.n[1] * F.n[2]), b(bx);
-        if (W.dim == 3)
-            hipLaunchKernelGGL((k_prolong<T, 3>), g, b, 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.n[2], F.e, gx);
-        else
-            hipLaunchKernelGGL((k_prolong<T, 2>), g, b, 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.n[2], F.e, gx);
+        if (W.dim == 3 && vc.c0uni && stream_sweeps()) {
+            const int ntx = (F.n[0] + PZ_X - 1) / PZ_X, nty = (F.n[1] + PZ_Y - 1) / PZ_Y;
+            const int zc = resid_zc(Cc.n, ntx * nty);
+            const dim3 g((unsigned)(ntx * nty * ((Cc.n[2] + zc - 1) / zc)));
+            hipLaunchKernelGGL((k_prolong_z<T>), g, dim3(PZ_X * PZ_Y), 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.e,
+                               zc, ntx, nty);
+        } else {
+            const int bx = F.n[0] > 64 ? 128 : 64;
+            const int gx = (F.n[0] + bx - 1) / bx;
+            dim3 g((unsigned)gx * F.n[1] * F.n[2]), b(bx);
+            if (W.dim == 3)
+                hipLaunchKernelGGL((k_prolong<T, 3>), g, b, 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.n[2], F.e,
+                                   gx);
+            else
+                hipLaunchKernelGGL((k_prolong<T, 2>), g, b, 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.n[2], F.e,
+                                   gx);
+        }
         if (F.e2 && W.c0uni && bm != 2) {
             for (int s = 0; s < nu2;) {
                 const int k = nu2 - s >= 2 ? 2 : 1;
