@@ -811,8 +811,10 @@ __global__ void __launch_bounds__(RZ_X * RZ_Y) k_resid_z(LevV<double> L, const d
 // pass does not write, so the z-neighbours are the pair values of planes
 // k-1 and k+1 held in registers from earlier loads, and the pair mate is
 // the x-neighbour on one side.  Arithmetic as k_gsrb.
+// ZI: the first colour of a sweep from x = 0 (no zeroing pass): every
+// neighbour is 0, and the thread also writes the 0 of the other cell.
 constexpr int GZ_X = 32, GZ_Y = 8;  // pairs x rows per block
-template <class T, int BM, bool C0U>
+template <class T, int BM, bool C0U, bool ZI = false>
 __global__ void __launch_bounds__(GZ_X * GZ_Y) k_gsrb_z(LevV<T> L, T* __restrict__ x, const T* __restrict__ rhs,
                                                         int color, int zc, int ntx, int nty) {
     const unsigned lb = lblock();
@@ -835,8 +837,8 @@ __global__ void __launch_bounds__(GZ_X * GZ_Y) k_gsrb_z(LevV<T> L, T* __restrict
         return k < 0 ? (L.per2 ? k + L.nz : 0) : (k >= L.nz ? (L.per2 ? k - L.nz : L.nz - 1) : k);
     };
     const int gx = L.nx + 2, gxy = gx * (L.ny + 2), g2 = (ic + 1) + gx * (jc + 1);
-    T am0, am1, ac0, ac1, ap0, ap1;  // x pair of planes k-1, k, k+1
-    {
+    T am0 = T(0), am1 = T(0), ac0 = T(0), ac1 = T(0), ap0 = T(0), ap1 = T(0);  // x pair of planes k-1, k, k+1
+    if constexpr (!ZI) {
         const int cm = c2 + pstr * zw(z0 - 1), cc = c2 + pstr * z0, cp = c2 + pstr * zw(z0 + 1);
         am0 = x[cm];
         am1 = x[cm + 1];
@@ -859,9 +861,14 @@ __global__ void __launch_bounds__(GZ_X * GZ_Y) k_gsrb_z(LevV<T> L, T* __restrict
         const int i = ic + u;
         const int ck = c2 + pstr * k;
         const int cn = c2 + pstr * zw(k + 2);
-        const T an0 = x[cn], an1 = x[cn + 1];
-        const T xo = x[ck + (u ? 1 + oxh : oxl)];
-        const T yl = x[ck + u + oyl], yh = x[ck + u + oyh];
+        T an0 = T(0), an1 = T(0), xo = T(0), yl = T(0), yh = T(0);
+        if constexpr (!ZI) {
+            an0 = x[cn];
+            an1 = x[cn + 1];
+            xo = x[ck + (u ? 1 + oxh : oxl)];
+            yl = x[ck + u + oyl];
+            yh = x[ck + u + oyh];
+        }
         const T fv = rhs[ck + u];
         const T mate = u ? ac0 : ac1;
         const T xl0 = u ? mate : xo, xh0 = u ? xo : mate;
@@ -893,7 +900,10 @@ __global__ void __launch_bounds__(GZ_X * GZ_Y) k_gsrb_z(LevV<T> L, T* __restrict
         };
         T dg, of;
         stencil_nbf<T, 3, decltype(nb), decltype(fc), C0U>(L, i, jc, k, nb, fc, dg, of);
-        if (own) x[ck + u] = (fv + of) / dg;
+        if (own) {
+            x[ck + u] = (fv + of) / dg;
+            if constexpr (ZI) x[ck + 1 - u] = T(0);
+        }
         am0 = ac0;
         am1 = ac1;
         ac0 = ap0;
@@ -1289,23 +1299,49 @@ struct VTail {
     T* e[VC_MAXL];
     T* r[VC_MAXL];
     T *cr, *cp, *cq;  // bottom CG scratch
+    int off[VC_MAXL];   // first cell of each level in the LDS arrays
     int nl, nu1, nu2;
 };
+
+// f(i, j, k) for i < ni and every (j, k) row of a level, by one workgroup:
+// each thread keeps one i and steps through the rows, so the row index is
+// divided out once per call rather than once per cell
+template <class F>
+__device__ __forceinline__ void wg_rows(int ni, int ny, int nz, F f) {
+    const int nt = (int)blockDim.x, tid = (int)threadIdx.x;
+    if (ni <= nt) {
+        const int rstep = nt / ni, i = tid % ni, r0 = tid / ni;
+        if (r0 >= rstep) return;
+        const int dk = rstep / ny, dj = rstep - dk * ny;
+        int j = r0 % ny, k = r0 / ny;
+        for (int row = r0; row < ny * nz; row += rstep) {
+            f(i, j, k);
+            j += dj;
+            k += dk;
+            if (j >= ny) {
+                j -= ny;
+                k++;
+            }
+        }
+    } else {
+        for (int t = tid; t < ni * ny * nz; t += nt) {
+            const int row = t / ni;
+            f(t - row * ni, row % ny, row / ny);
+        }
+    }
+}
 
 template <class T, int BM, int DIM>
 __device__ __forceinline__ void wg_sweep(const LevV<T>& L, T* __restrict__ x, const T* __restrict__ rhs,
                                          int color) {
-    const int nh = (L.nx + 1) / 2, nrow = L.ny * L.nz;
-    for (int t = threadIdx.x; t < nh * nrow; t += blockDim.x) {
-        const int row = t / nh, ii = t - row * nh;
-        const int j = row % L.ny, k = row / L.ny;
+    wg_rows((L.nx + 1) / 2, L.ny, L.nz, [&](int ii, int j, int k) {
         const int i = 2 * ii + ((color + j + k) & 1);
-        if (i >= L.nx) continue;
+        if (i >= L.nx) return;
         T dg, of;
         stencil<T, BM, DIM>(L, x, i, j, k, dg, of);
-        const int c = i + L.nx * row;
+        const int c = i + L.nx * (j + L.ny * k);
         x[c] = (rhs[c] + of) / dg;
-    }
+    });
     __syncthreads();
 }
 
@@ -1376,22 +1412,38 @@ __device__ void bottom_cg_wave(const LevV<T>& L, T* __restrict__ e_out, const T*
     for (int c = lane; c < n; c += 64) e_out[c] = e[c];
 }
 
+// the tail levels' e and r live in LDS: levels of at most VC_TAIL_CELLS
+// cells, each coarser one at most half as large per dimension, hold fewer
+// than 4/3 VC_TAIL_CELLS cells in all (2-D; 8/7 in 3-D)
+constexpr int VC_LDS_CELLS = 5464;
+
 template <class T, int BM, int DIM>
 __global__ void __launch_bounds__(1024) k_vcycle_tail(VTail<T> a) {
+    // every sweep, transfer and the bottom solve of the tail read and write
+    // LDS; only r of the first level comes from memory and only its e goes
+    // back (the next finer level prolongs from it)
+    __shared__ T le[VC_LDS_CELLS], lr[VC_LDS_CELLS];
+    auto e = [&](int l) { return le + a.off[l]; };
+    auto r = [&](int l) { return lr + a.off[l]; };
+    {
+        const int n0 = a.v[0].nx * a.v[0].ny * a.v[0].nz;
+        for (int c = threadIdx.x; c < n0; c += blockDim.x) lr[c] = a.r[0][c];
+    }
+    __syncthreads();
     for (int l = 0; l < a.nl - 1; l++) {
         const LevV<T>& F = a.v[l];
         const int n = F.nx * F.ny * F.nz;
-        for (int c = threadIdx.x; c < n; c += blockDim.x) a.e[l][c] = T(0);
+        for (int c = threadIdx.x; c < n; c += blockDim.x) e(l)[c] = T(0);
         __syncthreads();
         for (int s = 0; s < a.nu1; s++) {
-            wg_sweep<T, BM, DIM>(F, a.e[l], a.r[l], 0);
-            wg_sweep<T, BM, DIM>(F, a.e[l], a.r[l], 1);
+            wg_sweep<T, BM, DIM>(F, e(l), r(l), 0);
+            wg_sweep<T, BM, DIM>(F, e(l), r(l), 1);
         }
         const LevV<T>& Cv = a.v[l + 1];
         const int nc = Cv.nx * Cv.ny * Cv.nz;
         for (int t = threadIdx.x; t < nc; t += blockDim.x) {
             const int I = t % Cv.nx, row = t / Cv.nx;
-            a.r[l + 1][t] = resid_restrict_cell<T, BM, DIM>(F, a.e[l], a.r[l], I, row % Cv.ny, row / Cv.ny);
+            r(l + 1)[t] = resid_restrict_cell<T, BM, DIM>(F, e(l), r(l), I, row % Cv.ny, row / Cv.ny);
         }
         __syncthreads();
     }
@@ -1399,9 +1451,9 @@ __global__ void __launch_bounds__(1024) k_vcycle_tail(VTail<T> a) {
         const LevV<T>& B = a.v[a.nl - 1];
         if (B.nx * B.ny * B.nz <= VC_WAVE_BOTTOM) {
             __shared__ T cg[4 * VC_WAVE_BOTTOM];
-            if (threadIdx.x < 64) bottom_cg_wave<T, BM, DIM>(B, a.e[a.nl - 1], a.r[a.nl - 1], cg);
+            if (threadIdx.x < 64) bottom_cg_wave<T, BM, DIM>(B, e(a.nl - 1), r(a.nl - 1), cg);
         } else {
-            bottom_cg<T, BM, DIM>(B, a.e[a.nl - 1], a.r[a.nl - 1], a.cr, a.cp, a.cq);
+            bottom_cg<T, BM, DIM>(B, e(a.nl - 1), r(a.nl - 1), a.cr, a.cp, a.cq);
         }
     }
     __syncthreads();
@@ -1410,13 +1462,17 @@ __global__ void __launch_bounds__(1024) k_vcycle_tail(VTail<T> a) {
         const int n = F.nx * F.ny * F.nz;
         for (int t = threadIdx.x; t < n; t += blockDim.x) {
             const int i = t % F.nx, row = t / F.nx;
-            a.e[l][t] += prolong_cell<T, DIM>(a.v[l + 1], a.e[l + 1], i, row % F.ny, row / F.ny);
+            e(l)[t] += prolong_cell<T, DIM>(a.v[l + 1], e(l + 1), i, row % F.ny, row / F.ny);
         }
         __syncthreads();
         for (int s = 0; s < a.nu2; s++) {
-            wg_sweep<T, BM, DIM>(F, a.e[l], a.r[l], 0);
-            wg_sweep<T, BM, DIM>(F, a.e[l], a.r[l], 1);
+            wg_sweep<T, BM, DIM>(F, e(l), r(l), 0);
+            wg_sweep<T, BM, DIM>(F, e(l), r(l), 1);
         }
+    }
+    {
+        const int n0 = a.v[0].nx * a.v[0].ny * a.v[0].nz;
+        for (int c = threadIdx.x; c < n0; c += blockDim.x) a.e[0][c] = le[c];
     }
 }
 
@@ -1831,19 +1887,26 @@ static bool stream_sweeps() {
     return on;
 }
 
+// levels swept by k_gsrb_z (which can also start a sweep from zero)
+static bool streamed_sweep(int dim, int nx) { return dim == 3 && nx % 2 == 0 && stream_sweeps(); }
+
 template <class T, int BM>
-static void launch_gsrb_bm(vd_ctx* ctx, int dim, const LevV<T>& v, T* x, const T* rhs, int color) {
-    if (dim == 3 && v.nx % 2 == 0 && stream_sweeps()) {
+static void launch_gsrb_bm(vd_ctx* ctx, int dim, const LevV<T>& v, T* x, const T* rhs, int color,
+                           bool zi = false) {
+    if (streamed_sweep(dim, v.nx)) {
         const int ntx = (v.nx / 2 + GZ_X - 1) / GZ_X, nty = (v.ny + GZ_Y - 1) / GZ_Y;
         const int n3[3] = {v.nx, v.ny, v.nz};
         const int zc = resid_zc(n3, ntx * nty);
         const dim3 g((unsigned)(ntx * nty * ((v.nz + zc - 1) / zc)));
-        if (v.c0uni)
-            hipLaunchKernelGGL((k_gsrb_z<T, BM, true>), g, dim3(GZ_X * GZ_Y), 0, ctx->stream, v, x, rhs, color, zc,
-                               ntx, nty);
-        else
-            hipLaunchKernelGGL((k_gsrb_z<T, BM, false>), g, dim3(GZ_X * GZ_Y), 0, ctx->stream, v, x, rhs, color,
-                               zc, ntx, nty);
+#define VD_GZ(C0UV, ZIV)                                                                                      \
+    hipLaunchKernelGGL((k_gsrb_z<T, BM, C0UV, ZIV>), g, dim3(GZ_X * GZ_Y), 0, ctx->stream, v, x, rhs, color, zc, \
+                       ntx, nty)
+        if (v.c0uni) {
+            if (zi) VD_GZ(true, true); else VD_GZ(true, false);
+        } else {
+            if (zi) VD_GZ(false, true); else VD_GZ(false, false);
+        }
+#undef VD_GZ
         return;
     }
     const int nh = (v.nx + 1) / 2;
@@ -1855,13 +1918,14 @@ static void launch_gsrb_bm(vd_ctx* ctx, int dim, const LevV<T>& v, T* x, const T
 }
 
 // bm: 0 face arrays, 1 constant, 2 cell coefficient (harmonic on the fly)
+// zi: x is taken as 0 (streamed levels only: streamed_sweep)
 template <class T>
 static void launch_gsrb(vd_ctx* ctx, const MGWork& W, int bm, const LevV<T>& v, T* x, const T* rhs,
-                        int color, bool fine) {
+                        int color, bool fine, bool zi = false) {
     if (fine) kt_begin(ctx, VD_KT_GSRB);
-    if (bm == 1) launch_gsrb_bm<T, 1>(ctx, W.dim, v, x, rhs, color);
-    else if (bm == 0) launch_gsrb_bm<T, 0>(ctx, W.dim, v, x, rhs, color);
-    else launch_gsrb_bm<T, 2>(ctx, W.dim, v, x, rhs, color);
+    if (bm == 1) launch_gsrb_bm<T, 1>(ctx, W.dim, v, x, rhs, color, zi);
+    else if (bm == 0) launch_gsrb_bm<T, 0>(ctx, W.dim, v, x, rhs, color, zi);
+    else launch_gsrb_bm<T, 2>(ctx, W.dim, v, x, rhs, color, zi);
     // one colour = half of a sweep that reads phi, rhs and the coefficient
     // data (none, DIM face arrays, or one cell array) and writes phi
     const int nb = bm == 1 ? 0 : (bm == 0 ? W.dim : 1);
@@ -1924,9 +1988,11 @@ static bool vcycle(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L,
             }
             if (nu1 == 0) hipLaunchKernelGGL(k_zero<T>, grid1d(nc, 256), dim3(256), 0, ctx->stream, F.e, nc);
         } else {
-            hipLaunchKernelGGL(k_zero<T>, grid1d(nc, 256), dim3(256), 0, ctx->stream, F.e, nc);
+            // from a zero correction: the first colour writes e without reading it
+            const bool zi = nu1 > 0 && streamed_sweep(W.dim, F.n[0]);
+            if (!zi) hipLaunchKernelGGL(k_zero<T>, grid1d(nc, 256), dim3(256), 0, ctx->stream, F.e, nc);
             for (int s = 0; s < nu1; s++) {
-                launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 0, l == 0);
+                launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 0, l == 0, zi && s == 0);
                 launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 1, l == 0);
             }
         }
@@ -1946,6 +2012,10 @@ static bool vcycle(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L,
         a.cp = B.p;
         a.cq = B.q;
         a.nl = nl - l0;
+        for (int l = 0, o = 0; l < a.nl; l++) {
+            a.off[l] = o;
+            o += (int)ncell(L[l0 + l].n);
+        }
         a.nu1 = nu1;
         a.nu2 = nu2;
         if (bm == 1) {
@@ -2776,7 +2846,10 @@ __global__ void k_spec_floor(SpecPostArgs a) {
     if (a.floor_on && a.x[t] < a.floor_val) a.x[t] = a.floor_val;
 }
 
-// copy to phi_new and the Te update (ScalarSolve.cpp:632-667)
+// copy to phi_new and the Te update (ScalarSolve.cpp:632-667); FLOOR: the
+// density floor of k_spec_floor applied on the way (when nothing reads the
+// floored solution in between)
+template <bool FLOOR>
 __global__ void k_spec_post(SpecPostArgs a) {
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.nx * a.ny * a.nz) return;
@@ -2786,7 +2859,9 @@ __global__ void k_spec_post(SpecPostArgs a) {
     i += a.lo.x;
     j += a.lo.y;
     k += a.lo.z;
-    a.phi(i, j, k, a.spec) = a.x[t];
+    double xv = a.x[t];
+    if (FLOOR && a.floor_on && xv < a.floor_val) xv = a.floor_val;
+    a.phi(i, j, k, a.spec) = xv;
     if (a.energy) {
         const int ns = a.ns;
         double te = VD_TWOTHIRD / VD_K_B * a.phi(i, j, k, VD_EEN(ns)) / a.sb(i, j, k, a.E);
@@ -2905,9 +2980,14 @@ int implicit_solve(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sb,
     p.nx = n[0];
     p.ny = n[1];
     p.nz = n[2];
-    hipLaunchKernelGGL(k_spec_floor, grid1d(nc, 256), dim3(256), 0, ctx->stream, p);
-    if (spec == E && grad) VD_TRY(grad_faces(ctx, *W, bclo, bchi, grad, box, 1.0));
-    hipLaunchKernelGGL(k_spec_post, grid1d(nc, 256), dim3(256), 0, ctx->stream, p);
+    if (spec == E && grad) {
+        // the electron gradient faces see the floored density
+        hipLaunchKernelGGL(k_spec_floor, grid1d(nc, 256), dim3(256), 0, ctx->stream, p);
+        VD_TRY(grad_faces(ctx, *W, bclo, bchi, grad, box, 1.0));
+        hipLaunchKernelGGL(k_spec_post<false>, grid1d(nc, 256), dim3(256), 0, ctx->stream, p);
+    } else {
+        hipLaunchKernelGGL(k_spec_post<true>, grid1d(nc, 256), dim3(256), 0, ctx->stream, p);
+    }
     VD_CHECK_LAUNCH();
     return st;
 }
