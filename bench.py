@@ -228,6 +228,9 @@ def main():
     ap.add_argument("--poisson-n", type=int, default=512)
     ap.add_argument("--chem-n", type=int, default=256)
     ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="after the headline, copy 4 components of an n^3 array once (a plain copy of "
+                         "8*4*n^3 B each way that scales rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus %d is not supported" % args.gpus,
@@ -263,6 +266,14 @@ def main():
                 "traffic": pmc_traffic(args.kernel, args.n, args.precision), "kernel": args.kernel,
                 "launches": launches, "avg_launch_us": round(avg_s * 1e6, 2),
                 "algorithmic_bytes_per_launch": round(byts)}
+
+    if args.copy_probe:
+        g = native.Grid(dev, case.geom)
+        src, dst = g.array(4), g.array(4)
+        native.fill_patch(g, src, dst, 0, 4)  # no ghost cells: a plain copy
+        dev.sync()
+        del src, dst, g
+        log("copy probe: 4 x %d^3 x 8 B = %d B each way" % (args.n, 32 * case.ncells))
 
     configs = {}
     only = set(x for x in args.only.split(",") if x)
