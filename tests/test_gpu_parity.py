@@ -332,6 +332,34 @@ def test_evolve_discharge_2d_matches_oracle(dev, nsteps):
     assert not bad, bad
 
 
+@pytest.mark.parametrize("bg", [0, 1])
+def test_evolve_split_calls_match_one_call_and_oracle(dev, bg):
+    """Evolving in calls of 1, 3 and 2 steps runs a single-step call and the
+    first, middle and last steps of the one-corrector driver, which moves
+    the state between phi_new and two ghosted buffers; the state equals that
+    of one 6-step call bit for bit, and the oracle's (bg = 1: background
+    species advanced by their reaction source, the electron source added to
+    the solved density)."""
+    c = cases.discharge_2d(n=32, nsteps=6)
+    c.params.do_bg_reactions = bg
+    c.params.bound_specden = bg
+    tight(c.params, 1e-12)
+    ref, _ = O.evolve(c, 6)
+    one = native.Sim(dev, c)
+    one.evolve(6)
+    s_one = one.get_state()
+    one.close()
+    split = native.Sim(dev, c)
+    for n in (1, 3, 2):
+        split.evolve(n)
+    s_split = split.get_state()
+    split.close()
+    assert np.array_equal(s_one, s_split)
+    errs = compare.state_errors(ref, s_split, c.mech.nspec, c.geom.dim)
+    bad = {q: e for q, e in errs.items() if not e <= 1e-10}
+    assert not bad, bad
+
+
 def test_evolve_air_3d_matches_oracle(dev):
     c = cases.air_3d(n=16, nsteps=4)
     tight(c.params, 1e-12)

@@ -80,6 +80,61 @@ __global__ void k_fill_patch(Arr<double> src, Arr<double> dst, int dim, int3 lo,
     dst.comp(q)[dst.off(i, j, k)] = src.comp(q)[src.off(c[0], c[1], c[2])];
 }
 
+// FillPatch of an array into itself: its valid region already holds the
+// data, so only the ghost cells are written, each from the valid cell
+// k_fill_patch maps it to.  Threads enumerate the ghost cells slab by slab
+// (z ghost planes, then y ghost rows of the valid planes, then x ghost
+// cells of the valid rows); blockIdx.y = component.
+__global__ void k_fill_ghosts(Arr<double> a, int dim, int3 lo, int3 n, int3 per, int gx, int gy, int gz,
+                              int scomp) {
+    const int q = blockIdx.y + scomp;
+    const int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
+    const int gxl = lo.x - a.bx, gyl = lo.y - a.by, gzl = lo.z - a.bz;
+    const int64_t cz = (int64_t)(gz - n.z) * gy * gx, cy = (int64_t)n.z * (gy - n.y) * gx,
+                  cx = (int64_t)n.z * n.y * (gx - n.x);
+    int i, j, k;
+    if (t < cz) {
+        const int kk = (int)(t / ((int64_t)gy * gx));
+        const int rem = (int)(t - (int64_t)kk * gy * gx);
+        k = kk < gzl ? a.bz + kk : lo.z + n.z + (kk - gzl);
+        j = a.by + rem / gx;
+        i = a.bx + rem % gx;
+    } else if (t < cz + cy) {
+        const int64_t u = t - cz;
+        const int per_plane = (gy - n.y) * gx;
+        k = lo.z + (int)(u / per_plane);
+        const int rem = (int)(u % per_plane), jj = rem / gx;
+        j = jj < gyl ? a.by + jj : lo.y + n.y + (jj - gyl);
+        i = a.bx + rem % gx;
+    } else if (t < cz + cy + cx) {
+        const int64_t u = t - cz - cy;
+        const int per_plane = n.y * (gx - n.x);
+        k = lo.z + (int)(u / per_plane);
+        const int rem = (int)(u % per_plane), ii = rem % (gx - n.x);
+        j = lo.y + rem / (gx - n.x);
+        i = ii < gxl ? a.bx + ii : lo.x + n.x + (ii - gxl);
+    } else {
+        return;
+    }
+    int c[3] = {i, j, k};
+    const int l[3] = {lo.x, lo.y, lo.z}, nn[3] = {n.x, n.y, n.z}, pp[3] = {per.x, per.y, per.z};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (d >= dim) break;
+        int v = c[d] - l[d];
+        if (v < 0 || v >= nn[d]) {
+            if (pp[d]) {
+                v %= nn[d];
+                if (v < 0) v += nn[d];
+            } else {
+                v = v < 0 ? 0 : nn[d] - 1;
+            }
+        }
+        c[d] = v + l[d];
+    }
+    a.comp(q)[a.off(i, j, k)] = a.comp(q)[a.off(c[0], c[1], c[2])];
+}
+
 int fill_patch(vd_ctx* ctx, const vd_array4* src, vd_array4* dst, const vd_box* box,
                const vd_geom* g, int scomp, int ncomp) {
     for (int d = 0; d < g->dim; d++)
@@ -95,6 +150,21 @@ int fill_patch(vd_ctx* ctx, const vd_array4* src, vd_array4* dst, const vd_box* 
     int3 n = make_int3(box->hi[0] - box->lo[0] + 1, box->hi[1] - box->lo[1] + 1,
                        box->hi[2] - box->lo[2] + 1);
     int3 per = make_int3(g->is_periodic[0], g->is_periodic[1], g->dim > 2 ? g->is_periodic[2] : 0);
+    const bool same = src->p == dst->p && src->jstride == dst->jstride && src->kstride == dst->kstride &&
+                      src->nstride == dst->nstride && src->begin[0] == dst->begin[0] &&
+                      src->begin[1] == dst->begin[1] && src->begin[2] == dst->begin[2];
+    if (same) {
+        // into itself: the ghost cells only
+        const int64_t nv = (int64_t)n.x * n.y * n.z;
+        const int64_t ng = tot - nv;
+        if (ng <= 0) return VD_OK;
+        kt_begin(ctx, VD_KT_FILL);
+        hipLaunchKernelGGL(k_fill_ghosts, dim3((unsigned)((ng + 255) / 256), ncomp), dim3(256), 0, ctx->stream,
+                           arr(dst), g->dim, lo, n, per, nx, ny, nz, scomp);
+        kt_end(ctx, VD_KT_FILL, 16.0 * (double)ng * ncomp);
+        VD_CHECK_LAUNCH();
+        return VD_OK;
+    }
     kt_begin(ctx, VD_KT_FILL);
     hipLaunchKernelGGL(k_fill_patch, dim3((unsigned)((tot + 255) / 256), ncomp), dim3(256), 0,
                        ctx->stream, arr(src), arr(dst), g->dim, lo, n, per, nx, ny, nz, scomp);

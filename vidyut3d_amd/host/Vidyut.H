@@ -88,8 +88,8 @@ class Vidyut {
     // after an Evolve with one corrector the start-of-step species and EEN
     // are taken from Sborder (which holds them unchanged) on first access
     MultiFab& phi_old() {
-        if (phi_old_stale_) {
-            phi_old_[0].copy_from(Sborder_[0], 0, 0, ns_ + 1);
+        if (phi_old_stale_ && sb_last_) {
+            phi_old_[0].copy_from(*sb_last_, 0, 0, ns_ + 1);
             phi_old_stale_ = false;
         }
         return phi_old_[0];
@@ -103,6 +103,14 @@ class Vidyut {
 
    private:
     int alloc_step_data();
+    int step_body(std::vector<MultiFab>& sb, std::vector<MultiFab>& sb_old, Real cur_time, Real dt,
+                  bool pot_in_sb);
+    int evolve_one_corrector(int nsteps, Real dt);
+    int evolve_corrected(int nsteps, Real dt, int ncorr);
+    // where a step writes its results and reads the state (phi_new unless
+    // the one-corrector driver rotates buffers)
+    MultiFab& out() { return out_ ? *out_ : phi_new_[0]; }
+    MultiFab& state() { return state_ ? *state_ : phi_new_[0]; }
     vd_ctx* ctx_;
     vd_mech* mech_ = nullptr;
     vd_mech_desc desc_;
@@ -115,8 +123,12 @@ class Vidyut {
     int istep_ = 0;
     bool phi_old_stale_ = false;
     std::vector<MultiFab> phi_new_, phi_old_;
-    // persistent step data (the reference allocates these every step)
-    std::vector<MultiFab> Sborder_, Sborder_old_, expl_src_, rxn_src_;
+    // persistent step data (the reference allocates these every step);
+    // Swork_ is the second ghosted state buffer of the one-corrector driver
+    std::vector<MultiFab> Sborder_, Sborder_old_, Swork_, expl_src_, rxn_src_;
+    MultiFab* out_ = nullptr;
+    MultiFab* state_ = nullptr;
+    MultiFab* sb_last_ = nullptr;  // Sborder of the last one-corrector step
     std::vector<FaceMF> flux_, efield_ec_, gradne_fc_, grad_fc_;
     vd_solve_info pot_info_{};
     long long mg_iters_ = 0;

@@ -123,13 +123,14 @@ Vidyut::~Vidyut() {
 
 int Vidyut::alloc_step_data() {
     Sborder_.emplace_back(ctx_, dom_, dim_, nvar_, ngrow_fp_);
+    Swork_.emplace_back(ctx_, dom_, dim_, nvar_, ngrow_fp_);
     expl_src_.emplace_back(ctx_, dom_, dim_, 1, 0);
     rxn_src_.emplace_back(ctx_, dom_, dim_, ns_ + 1, 0);
     for (auto* v : {&flux_, &efield_ec_, &gradne_fc_, &grad_fc_}) {
         v->emplace_back();
         for (int d = 0; d < 3; d++) (*v)[0][d].define(ctx_, dom_, dim_, 1, 0, d < dim_ ? d : 0);
     }
-    if (!phi_new_[0].ok() || !Sborder_[0].ok()) return VD_ENOMEM;
+    if (!phi_new_[0].ok() || !Sborder_[0].ok() || !Swork_[0].ok()) return VD_ENOMEM;
     return VD_OK;
 }
 
@@ -192,7 +193,7 @@ int Vidyut::compute_elecenergy_source(int lev, MultiFab& Sborder, MultiFab& rxns
     vd_array4 ef[3] = {efield[0].array(), efield[1].array(), efield[2].array()};
     vd_array4 gn[3] = {gradne[0].array(), gradne[1].array(), gradne[2].array()};
     return vd_compute_elecenergy_source(ctx_, mech_, &Sborder.array(), &rxnsrc.array(), ef, gn,
-                                        &dsdt.array(), &phi_new_[0].array(), &dom_, &geom_, &p);
+                                        &dsdt.array(), &out().array(), &dom_, &geom_, &p);
 }
 
 int Vidyut::solve_potential(Real current_time, std::vector<MultiFab>& Sborder, const int* bc_lo,
@@ -203,8 +204,8 @@ int Vidyut::solve_potential(Real current_time, std::vector<MultiFab>& Sborder, c
         p.pot_bc_hi[d] = bc_hi[d];
     }
     vd_array4 ef[3] = {efield_ec[0][0].array(), efield_ec[0][1].array(), efield_ec[0][2].array()};
-    int st = vd_solve_potential(ctx_, mech_, &Sborder[0].array(), &phi_new_[0].array(), ef, &dom_, &geom_,
-                                &p, current_time, &pot_info_);
+    int st = vd_solve_potential(ctx_, mech_, &Sborder[0].array(), &out().array(), ef, &dom_, &geom_, &p,
+                                current_time, &pot_info_);
     mg_iters_ += pot_info_.iterations;
     return st;
 }
@@ -215,7 +216,7 @@ int Vidyut::implicit_solve_scalar(Real current_time, Real dt, int spec_id, std::
     vd_array4 g[3] = {grad_fc[0][0].array(), grad_fc[0][1].array(), grad_fc[0][2].array()};
     vd_solve_info info;
     int st = vd_implicit_solve_scalar(ctx_, mech_, spec_id, &Sborder[0].array(), &Sborder_old[0].array(),
-                                      &dsdt_expl[0].array(), &phi_new_[0].array(), g, &dom_, &geom_, bc_lo,
+                                      &dsdt_expl[0].array(), &out().array(), g, &dom_, &geom_, bc_lo,
                                       bc_hi, &prm_, current_time, dt, &info);
     mg_iters_ += info.iterations;
     return st;
@@ -224,7 +225,7 @@ int Vidyut::implicit_solve_scalar(Real current_time, Real dt, int spec_id, std::
 int Vidyut::find_time_scales(int lev, Real& dt_edrift, Real& dt_ediff, Real& dt_diel_relax) {
     (void)lev;
     double out[3];
-    TRY(vd_find_time_scales(ctx_, mech_, &phi_new_[0].array(), &dom_, &geom_, &prm_, out));
+    TRY(vd_find_time_scales(ctx_, mech_, &state().array(), &dom_, &geom_, &prm_, out));
     dt_edrift = out[0];
     dt_ediff = out[1];
     dt_diel_relax = out[2];
@@ -253,72 +254,155 @@ int Vidyut::ErrorEst(int lev, vd_tagarray4& tags, Real time, int ngrow) {
 
 // Source/Evolve.cpp:33-369 on one level with fixed dt
 int Vidyut::Evolve(int nsteps, Real dt, int ncorr) {
+    out_ = state_ = nullptr;
+    const int st = ncorr == 1 ? evolve_one_corrector(nsteps, dt) : evolve_corrected(nsteps, dt, ncorr);
+    out_ = state_ = nullptr;
+    if (st != VD_OK) return st;
+    return vd_ctx_sync(ctx_);
+}
+
+// The step body shared by both drivers, from the potential solve on:
+// Sborder sb holds species, EEN and ETEMP of the state (ghost cells
+// filled), results go to out(); Sborder_old is sb_old.
+int Vidyut::step_body(std::vector<MultiFab>& sb, std::vector<MultiFab>& sb_old, Real cur_time, Real dt,
+                      bool pot_in_sb) {
     const int E = desc_.e_idx;
+    if (pot_in_sb) {
+        // the potential, E and E/N are written into Sborder itself (its
+        // species are the state's, which k_pot_post reads), whose ghost
+        // cells are then filled in place
+        MultiFab* o = out_;
+        out_ = &sb[0];
+        const int st = solve_potential(cur_time, sb, prm_.pot_bc_lo, prm_.pot_bc_hi, efield_ec_);
+        out_ = o;
+        TRY(st);
+        TRY(vd_fill_patch(ctx_, &sb[0].array(), &sb[0].array(), &dom_, &geom_, VD_POT(ns_), 4));
+    } else {
+        TRY(solve_potential(cur_time, sb, prm_.pot_bc_lo, prm_.pot_bc_hi, efield_ec_));
+        TRY(FillPatch(0, cur_time + dt, sb[0], VD_POT(ns_), 4));
+    }
+    if (prm_.do_reactions)
+        TRY(update_rxnsrc_at_all_levels(sb, rxn_src_, cur_time));
+    else
+        TRY(rxn_src_[0].setVal(0.0));
+    TRY(update_explsrc_at_all_levels(E, sb, flux_, rxn_src_, expl_src_, prm_.eden_bc_lo, prm_.eden_bc_hi,
+                                     cur_time));
+    int st = implicit_solve_scalar(cur_time, dt, E, sb, sb_old, expl_src_, prm_.eden_bc_lo, prm_.eden_bc_hi,
+                                   gradne_fc_);
+    if (st != VD_OK && st != VD_ENOCONV) return st;
+    if (prm_.elecenergy_solve) {
+        TRY(update_explsrc_at_all_levels(ns_, sb, flux_, rxn_src_, expl_src_, prm_.eenrg_bc_lo,
+                                         prm_.eenrg_bc_hi, cur_time));
+        TRY(compute_elecenergy_source(0, sb[0], rxn_src_[0], efield_ec_[0], gradne_fc_[0], expl_src_[0],
+                                      cur_time, dt, prm_.floor_jh));
+        st = implicit_solve_scalar(cur_time, dt, ns_, sb, sb_old, expl_src_, prm_.eenrg_bc_lo, prm_.eenrg_bc_hi,
+                                   grad_fc_);
+        if (st != VD_OK && st != VD_ENOCONV) return st;
+    }
+    for (int ind = 0; ind < ns_; ind++) {
+        bool solve = !(desc_.is_bg[ind] || ind == E);
+        if (solve) {
+            const int* lo = desc_.charge[ind] != 0 ? prm_.ion_bc_lo : prm_.neutral_bc_lo;
+            const int* hi = desc_.charge[ind] != 0 ? prm_.ion_bc_hi : prm_.neutral_bc_hi;
+            TRY(update_explsrc_at_all_levels(ind, sb, flux_, rxn_src_, expl_src_, lo, hi, cur_time));
+            st = implicit_solve_scalar(cur_time, dt, ind, sb, sb_old, expl_src_, lo, hi, grad_fc_);
+            if (st != VD_OK && st != VD_ENOCONV) return st;
+        } else if (prm_.do_bg_reactions) {
+            // Source/Evolve.cpp:267-287 (also reached for the electron
+            // index, whose solved value it then updates): n_new = n + dt *
+            // rxn, a background species' n taken from Sborder, whose species
+            // equal those of the state
+            const vd_array4& n_old = ind == E ? out().array() : sb[0].array();  // E: the solved value
+            TRY(vd_lincomb(ctx_, &out().array(), 1.0, &n_old, ind, dt, &rxn_src_[0].array(), ind, ind, 1, &dom_));
+            if (prm_.bound_specden)
+                TRY(vd_clamp_min(ctx_, &out().array(), &dom_, ind, prm_.min_species_density));
+        }
+    }
+    return VD_OK;
+}
+
+// One corrector.  The state moves between phi_new and two ghosted buffers
+// (Sborder_ and Swork_) so that no step copies the whole state: a step
+// takes as its Sborder the buffer that holds the state -- its ghost cells
+// filled in place; the first step of a call copies phi_new in, as
+// FillPatch does -- and writes its results into the other buffer, or into
+// phi_new in the last step of the call.  The components it does not write
+// are copied over from the state (see below).  Sborder_old is Sborder: the
+// implicit solves read only its species and EEN, which equal the state's.
+int Vidyut::evolve_one_corrector(int nsteps, Real dt) {
+    MultiFab* P = &phi_new_[0];
+    std::vector<MultiFab>* G[2] = {&Sborder_, &Swork_};
+    MultiFab* cur = P;
+    for (int step = 0; step < nsteps; step++) {
+        const bool first = step == 0, last = step == nsteps - 1;
+        std::vector<MultiFab>& sb = first ? *G[0] : (cur == &(*G[0])[0] ? *G[0] : *G[1]);
+        std::vector<MultiFab>& other = &sb == G[0] ? *G[1] : *G[0];
+        MultiFab* out = last ? P : &other[0];
+        state_ = cur;
+        out_ = out;
+        phi_old_stale_ = false;
+        Real dt_edrift, dt_ediff, dt_diel;
+        TRY(find_time_scales(0, dt_edrift, dt_ediff, dt_diel));
+        const Real cur_time = t_new_;
+        // species, EEN, ETEMP into Sborder (a copy in the first step, the
+        // ghost cells only after)
+        TRY(vd_fill_patch(ctx_, &cur->array(), &sb[0].array(), &dom_, &geom_, 0, ns_ + 2));
+        TRY(step_body(sb, sb, cur_time, dt, true));
+        // what the step wrote into Sborder: E, which the next step's time
+        // scales read from the state, and -- for phi_new only, since no step
+        // reads them -- the potential and E/N
+        const int nef = dim_;
+        if (last) {
+            TRY(out->copy_from(sb[0], VD_POT(ns_), VD_POT(ns_), 1 + nef));
+            TRY(out->copy_from(sb[0], VD_REF(ns_), VD_REF(ns_), 1));
+        } else {
+            TRY(out->copy_from(sb[0], VD_EFX(ns_), VD_EFX(ns_), nef));
+        }
+        // the components no step writes keep their values, so each buffer
+        // needs them once per call: the two ghosted buffers receive them in
+        // the first two steps, and phi_new holds them already
+        if (out != cur && step < 2) {
+            if (nef < 3) TRY(out->copy_from(*cur, VD_EFX(ns_) + nef, VD_EFX(ns_) + nef, 3 - nef));
+            TRY(out->copy_from(*cur, VD_EEH(ns_), VD_EEH(ns_), 1));
+            if (!prm_.elecenergy_solve) {
+                TRY(out->copy_from(*cur, VD_EEN(ns_), VD_EEN(ns_), 2));
+                TRY(out->copy_from(*cur, VD_EJH(ns_), VD_EJH(ns_), 2));
+            }
+            if (!prm_.do_bg_reactions)
+                for (int ind = 0; ind < ns_; ind++)
+                    if (desc_.is_bg[ind] && ind != desc_.e_idx) TRY(out->copy_from(*cur, ind, ind, 1));
+        }
+        istep_++;
+        t_new_ = cur_time + dt;
+        sb_last_ = &sb[0];
+        phi_old_stale_ = true;
+        cur = out;
+    }
+    return VD_OK;
+}
+
+// More than one corrector (predictor-corrector averaging with phi_old).
+int Vidyut::evolve_corrected(int nsteps, Real dt, int ncorr) {
     for (int step = 0; step < nsteps; step++) {
         Real dt_edrift, dt_ediff, dt_diel;
         TRY(find_time_scales(0, dt_edrift, dt_ediff, dt_diel));
         const Real cur_time = t_new_;
-        // Sborder_old is read only in its valid region by the implicit
-        // solves (species and energy).  With one corrector its species and
-        // EEN equal those of Sborder, which the step never writes (the
-        // solves write phi_new, the second FillPatch only POT..EFZ), so
-        // Sborder stands in for it and phi_old is copied only on access;
-        // with more correctors phi_old is kept for the averaging.
-        std::vector<MultiFab>& Sb_old = ncorr > 1 ? phi_old_ : Sborder_;
-        if (ncorr > 1) TRY(phi_old_[0].copy_from(phi_new_[0], 0, 0, nvar_));
+        TRY(phi_old_[0].copy_from(phi_new_[0], 0, 0, nvar_));
         phi_old_stale_ = false;
         for (int niter = 0; niter < ncorr; niter++) {
             // Sborder components read by the step: species, EEN, ETEMP (here)
             // and POT, EFX..EFZ (after the potential solve); EJH, EIH, EEH and
             // REF of Sborder are never read.
             TRY(FillPatch(0, cur_time + dt, Sborder_[0], 0, ns_ + 2));
-            TRY(solve_potential(cur_time, Sborder_, prm_.pot_bc_lo, prm_.pot_bc_hi, efield_ec_));
-            TRY(FillPatch(0, cur_time + dt, Sborder_[0], VD_POT(ns_), 4));
-            if (prm_.do_reactions)
-                TRY(update_rxnsrc_at_all_levels(Sborder_, rxn_src_, cur_time));
-            else
-                TRY(rxn_src_[0].setVal(0.0));
-            TRY(update_explsrc_at_all_levels(E, Sborder_, flux_, rxn_src_, expl_src_, prm_.eden_bc_lo,
-                                             prm_.eden_bc_hi, cur_time));
-            int st = implicit_solve_scalar(cur_time, dt, E, Sborder_, Sb_old, expl_src_, prm_.eden_bc_lo,
-                                           prm_.eden_bc_hi, gradne_fc_);
-            if (st != VD_OK && st != VD_ENOCONV) return st;
-            if (prm_.elecenergy_solve) {
-                TRY(update_explsrc_at_all_levels(ns_, Sborder_, flux_, rxn_src_, expl_src_, prm_.eenrg_bc_lo,
-                                                 prm_.eenrg_bc_hi, cur_time));
-                TRY(compute_elecenergy_source(0, Sborder_[0], rxn_src_[0], efield_ec_[0], gradne_fc_[0],
-                                              expl_src_[0], cur_time, dt, prm_.floor_jh));
-                st = implicit_solve_scalar(cur_time, dt, ns_, Sborder_, Sb_old, expl_src_, prm_.eenrg_bc_lo,
-                                           prm_.eenrg_bc_hi, grad_fc_);
-                if (st != VD_OK && st != VD_ENOCONV) return st;
-            }
-            for (int ind = 0; ind < ns_; ind++) {
-                bool solve = !(desc_.is_bg[ind] || ind == E);
-                if (solve) {
-                    const int* lo = desc_.charge[ind] != 0 ? prm_.ion_bc_lo : prm_.neutral_bc_lo;
-                    const int* hi = desc_.charge[ind] != 0 ? prm_.ion_bc_hi : prm_.neutral_bc_hi;
-                    TRY(update_explsrc_at_all_levels(ind, Sborder_, flux_, rxn_src_, expl_src_, lo, hi,
-                                                     cur_time));
-                    st = implicit_solve_scalar(cur_time, dt, ind, Sborder_, Sb_old, expl_src_, lo, hi,
-                                               grad_fc_);
-                    if (st != VD_OK && st != VD_ENOCONV) return st;
-                } else if (prm_.do_bg_reactions) {
-                    // Source/Evolve.cpp:267-287 (also reached for the electron index)
-                    TRY(vd_lincomb(ctx_, &phi_new_[0].array(), 1.0, &phi_new_[0].array(), ind, dt,
-                                   &rxn_src_[0].array(), ind, ind, 1, &dom_));
-                    if (prm_.bound_specden)
-                        TRY(vd_clamp_min(ctx_, &phi_new_[0].array(), &dom_, ind, prm_.min_species_density));
-                }
-            }
+            TRY(step_body(Sborder_, phi_old_, cur_time, dt, false));
             if (niter < ncorr - 1)
                 TRY(vd_lincomb(ctx_, &phi_new_[0].array(), 0.5, &phi_old_[0].array(), 0, 0.5,
                                &phi_new_[0].array(), 0, 0, nvar_, &dom_));
         }
         istep_++;
         t_new_ = cur_time + dt;
-        phi_old_stale_ = ncorr == 1;
     }
-    return vd_ctx_sync(ctx_);
+    return VD_OK;
 }
 
 // --------------------------------------------------------------- plotfile

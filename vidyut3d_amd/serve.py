@@ -45,13 +45,14 @@ class Request:
 
 
 def footprint_bytes(dim, n, nspec):
-    """Device bytes of one case: state, old state and ghosted Sborder
-    (3*NVAR), reaction sources (NS+1), face arrays (12 + 3 flux scratch)
-    and the multigrid workspaces (~16 arrays), with ghost growth."""
+    """Device bytes of one case: state and old state (2*NVAR), the two
+    ghosted state buffers of the step (Sborder and its partner, 2*NVAR with
+    ghost growth), reaction sources (NS+1), face arrays (12 + 3 flux
+    scratch) and the multigrid workspaces (~16 arrays)."""
     cells = n ** dim
     nvar = nspec + abi.NUM_PLASMAVARS
     grow = ((n + 6) / n) ** dim
-    arrays = 2 * nvar + nvar * grow + (nspec + 1) + 1 + 15 + 16
+    arrays = 2 * nvar + 2 * nvar * grow + (nspec + 1) + 1 + 15 + 16
     return int(8 * cells * arrays * 1.1)
 
 
