@@ -1581,7 +1581,10 @@ struct SideArgs {
 };
 
 // finest-level c0 and rhs shift s*b*cinh for one boundary side
-__global__ void k_side_setup(SideArgs a) {
+// both sides of one direction in one launch (blockIdx.y = side): they
+// touch different cells, so their f updates do not meet
+__global__ void k_side_setup(SideArgs a0, SideArgs a1) {
+    const SideArgs& a = blockIdx.y ? a1 : a0;
     int t1 = a.d == 0 ? 1 : 0, t2 = a.d == 2 ? 1 : 2;
     int n[3] = {a.nx, a.ny, a.nz};
     int nt1 = n[t1], nt2 = a.dim == 3 ? n[t2] : 1;
@@ -2549,8 +2552,9 @@ static int side_setup(vd_ctx* ctx, MGWork& W, const int tlo[3], const int thi[3]
                       double bconst, const double* dcell = nullptr, bool use_const = false) {
     for (int d = 0; d < W.dim; d++) {
         if (W.per[d]) continue;
+        SideArgs sa[2];
         for (int s = 0; s < 2; s++) {
-            SideArgs a;
+            SideArgs& a = sa[s];
             a.d = d;
             a.side = s;
             a.type = s ? thi[d] : tlo[d];
@@ -2570,8 +2574,16 @@ static int side_setup(vd_ctx* ctx, MGWork& W, const int tlo[3], const int thi[3]
             a.bconst = bconst;
             a.c0 = W.c00[d][s];
             a.f = W.f;
-            int64_t nt = ntanf(W.n, d);
-            hipLaunchKernelGGL(k_side_setup, grid1d(nt, 256), dim3(256), 0, ctx->stream, a);
+        }
+        // a direction one cell thick has both sides on the same cells:
+        // keep them in order there
+        dim3 gs = grid1d(ntanf(W.n, d), 256);
+        if (W.n[d] > 1) {
+            gs.y = 2;
+            hipLaunchKernelGGL(k_side_setup, gs, dim3(256), 0, ctx->stream, sa[0], sa[1]);
+        } else {
+            hipLaunchKernelGGL(k_side_setup, gs, dim3(256), 0, ctx->stream, sa[0], sa[0]);
+            hipLaunchKernelGGL(k_side_setup, gs, dim3(256), 0, ctx->stream, sa[1], sa[1]);
         }
     }
     VD_CHECK_LAUNCH();
@@ -2757,8 +2769,18 @@ struct SpecBcArgs {
     double* out;
 };
 
+// the non-periodic sides of one solve, one launch (blockIdx.y = side)
+struct SpecBcSides {
+    int nside;
+    int d[6], side[6];
+    double* out[6];
+};
+
 // plasmachem_transport::species_bc (Source/PlasmaChem.H:74-184)
-__global__ void k_species_bc(SpecBcArgs a) {
+__global__ void k_species_bc(SpecBcArgs a, SpecBcSides sides) {
+    a.d = sides.d[blockIdx.y];
+    a.side = sides.side[blockIdx.y];
+    a.out = sides.out[blockIdx.y];
     int n[3] = {a.nx, a.ny, a.nz};
     int t1 = a.d == 0 ? 1 : 0, t2 = a.d == 2 ? 1 : 2;
     int nt1 = n[t1], nt2 = a.dim == 3 ? n[t2] : 1;
@@ -2913,24 +2935,37 @@ int implicit_solve(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sb,
         hipLaunchKernelGGL(k_cell_diff, grid1d((int64_t)cd.gnx * cd.gny * cd.gnz, 256), dim3(256), 0,
                            ctx->stream, cd, W->dcell);
     }
-    for (int d = 0; d < g->dim; d++) {
-        if (W->per[d]) continue;
-        for (int s = 0; s < 2; s++) {
-            SpecBcArgs b;
-            b.M = m->dev;
-            b.sb = arr(sb);
-            b.spec = spec;
-            b.d = d;
-            b.side = s;
-            b.ns = ns;
-            b.dim = g->dim;
-            b.lo = make_int3(lo[0], lo[1], lo[2]);
-            b.nx = n[0];
-            b.ny = n[1];
-            b.nz = n[2];
-            b.Tg = Tg;
-            b.out = W->bcv[d][s];
-            hipLaunchKernelGGL(k_species_bc, grid1d(ntanf(n, d), 256), dim3(256), 0, ctx->stream, b);
+    {
+        SpecBcArgs b;
+        b.M = m->dev;
+        b.sb = arr(sb);
+        b.spec = spec;
+        b.d = b.side = 0;
+        b.ns = ns;
+        b.dim = g->dim;
+        b.lo = make_int3(lo[0], lo[1], lo[2]);
+        b.nx = n[0];
+        b.ny = n[1];
+        b.nz = n[2];
+        b.Tg = Tg;
+        b.out = nullptr;
+        SpecBcSides sides;
+        sides.nside = 0;
+        int64_t ntmax = 0;
+        for (int d = 0; d < g->dim; d++) {
+            if (W->per[d]) continue;
+            for (int s = 0; s < 2; s++) {
+                sides.d[sides.nside] = d;
+                sides.side[sides.nside] = s;
+                sides.out[sides.nside] = W->bcv[d][s];
+                sides.nside++;
+            }
+            ntmax = std::max(ntmax, ntanf(n, d));
+        }
+        if (sides.nside > 0) {
+            dim3 gb = grid1d(ntmax, 256);
+            gb.y = (unsigned)sides.nside;
+            hipLaunchKernelGGL(k_species_bc, gb, dim3(256), 0, ctx->stream, b, sides);
         }
     }
     // Robin species sides keep the reference default (a, b, f) = (0, 1, 0)
