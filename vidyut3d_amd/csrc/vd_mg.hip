@@ -85,6 +85,9 @@ struct MGWork {
     // no Robin side: c0 is -1 (Dirichlet) or 1 (Neumann) on every boundary
     // face of every level, and the fused smoother takes it as a constant
     int c0uni = 0;
+    // z-streamed sweeps, residual, restriction and prolongation on 3-D
+    // levels (VD_STREAM_SWEEPS=0 at workspace creation: row-per-block)
+    int stream = 1;
     double c0u[3][2] = {{0, 0}, {0, 0}, {0, 0}};
     std::vector<Lev<double>> L64;
     std::vector<Lev<float>> L32;
@@ -1800,6 +1803,10 @@ static int get_work(vd_ctx* ctx, const vd_geom* g, const vd_box* box, int prec, 
     W->dim = g->dim;
     W->prec = prec;
     W->constb = constb;
+    {
+        const char* e = getenv("VD_STREAM_SWEEPS");
+        W->stream = !(e && strcmp(e, "0") == 0);
+    }
     for (int d = 0; d < 3; d++) {
         W->n[d] = n[d];
         W->per[d] = per[d];
@@ -1880,23 +1887,15 @@ fail:
     return st;
 }
 
-// streamed colour sweeps (k_gsrb_z) on 3-D levels with even nx; VD_STREAM_SWEEPS=0 keeps the
-// row-per-block kernel (read once per process)
-static bool stream_sweeps() {
-    static const bool on = [] {
-        const char* e = getenv("VD_STREAM_SWEEPS");
-        return !(e && strcmp(e, "0") == 0);
-    }();
-    return on;
-}
-
-// levels swept by k_gsrb_z (which can also start a sweep from zero)
-static bool streamed_sweep(int dim, int nx) { return dim == 3 && nx % 2 == 0 && stream_sweeps(); }
+// levels swept by k_gsrb_z (which can also start a sweep from zero): 3-D,
+// even nx, streamed kernels on for the workspace
+static bool streamed_sweep(const MGWork& W, int nx) { return W.dim == 3 && nx % 2 == 0 && W.stream; }
 
 template <class T, int BM>
-static void launch_gsrb_bm(vd_ctx* ctx, int dim, const LevV<T>& v, T* x, const T* rhs, int color,
+static void launch_gsrb_bm(vd_ctx* ctx, const MGWork& W, const LevV<T>& v, T* x, const T* rhs, int color,
                            bool zi = false) {
-    if (streamed_sweep(dim, v.nx)) {
+    const int dim = W.dim;
+    if (streamed_sweep(W, v.nx)) {
         const int ntx = (v.nx / 2 + GZ_X - 1) / GZ_X, nty = (v.ny + GZ_Y - 1) / GZ_Y;
         const int n3[3] = {v.nx, v.ny, v.nz};
         const int zc = resid_zc(n3, ntx * nty);
@@ -1926,9 +1925,9 @@ template <class T>
 static void launch_gsrb(vd_ctx* ctx, const MGWork& W, int bm, const LevV<T>& v, T* x, const T* rhs,
                         int color, bool fine, bool zi = false) {
     if (fine) kt_begin(ctx, VD_KT_GSRB);
-    if (bm == 1) launch_gsrb_bm<T, 1>(ctx, W.dim, v, x, rhs, color, zi);
-    else if (bm == 0) launch_gsrb_bm<T, 0>(ctx, W.dim, v, x, rhs, color, zi);
-    else launch_gsrb_bm<T, 2>(ctx, W.dim, v, x, rhs, color, zi);
+    if (bm == 1) launch_gsrb_bm<T, 1>(ctx, W, v, x, rhs, color, zi);
+    else if (bm == 0) launch_gsrb_bm<T, 0>(ctx, W, v, x, rhs, color, zi);
+    else launch_gsrb_bm<T, 2>(ctx, W, v, x, rhs, color, zi);
     // one colour = half of a sweep that reads phi, rhs and the coefficient
     // data (none, DIM face arrays, or one cell array) and writes phi
     const int nb = bm == 1 ? 0 : (bm == 0 ? W.dim : 1);
@@ -1936,8 +1935,9 @@ static void launch_gsrb(vd_ctx* ctx, const MGWork& W, int bm, const LevV<T>& v, 
 }
 
 template <class T, int BM>
-static void launch_resid_restrict(vd_ctx* ctx, int dim, const LevV<T>& v, const Lev<T>& F, Lev<T>& Cc) {
-    if (dim == 3 && stream_sweeps()) {
+static void launch_resid_restrict(vd_ctx* ctx, const MGWork& W, const LevV<T>& v, const Lev<T>& F, Lev<T>& Cc) {
+    const int dim = W.dim;
+    if (dim == 3 && W.stream) {
         const int ntx = (Cc.n[0] + RR_X - 1) / RR_X, nty = (Cc.n[1] + RR_Y - 1) / RR_Y;
         const int zc = resid_zc(Cc.n, ntx * nty);
         const dim3 g((unsigned)(ntx * nty * ((Cc.n[2] + zc - 1) / zc)));
@@ -1992,15 +1992,15 @@ static bool vcycle(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L,
             if (nu1 == 0) hipLaunchKernelGGL(k_zero<T>, grid1d(nc, 256), dim3(256), 0, ctx->stream, F.e, nc);
         } else {
             // from a zero correction: the first colour writes e without reading it
-            const bool zi = nu1 > 0 && streamed_sweep(W.dim, F.n[0]);
+            const bool zi = nu1 > 0 && streamed_sweep(W, F.n[0]);
             if (!zi) hipLaunchKernelGGL(k_zero<T>, grid1d(nc, 256), dim3(256), 0, ctx->stream, F.e, nc);
             for (int s = 0; s < nu1; s++) {
                 launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 0, l == 0, zi && s == 0);
                 launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 1, l == 0);
             }
         }
-        if (bm == 1) launch_resid_restrict<T, 1>(ctx, W.dim, v, F, L[l + 1]);
-        else launch_resid_restrict<T, 0>(ctx, W.dim, v, F, L[l + 1]);
+        if (bm == 1) launch_resid_restrict<T, 1>(ctx, W, v, F, L[l + 1]);
+        else launch_resid_restrict<T, 0>(ctx, W, v, F, L[l + 1]);
     }
     // coarse levels and bottom
     {
@@ -2035,7 +2035,7 @@ static bool vcycle(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L,
         Lev<T>& Cc = L[l + 1];
         LevV<T> vc = view<T>(W, Cc, alpha, beta, bconst);
         LevV<T> v = view<T>(W, F, alpha, beta, bconst);
-        if (W.dim == 3 && vc.c0uni && stream_sweeps()) {
+        if (W.dim == 3 && vc.c0uni && W.stream) {
             const int ntx = (F.n[0] + PZ_X - 1) / PZ_X, nty = (F.n[1] + PZ_Y - 1) / PZ_Y;
             const int zc = resid_zc(Cc.n, ntx * nty);
             const dim3 g((unsigned)(ntx * nty * ((Cc.n[2] + zc - 1) / zc)));
@@ -2144,7 +2144,7 @@ static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double
     double* part = W.part;
     double* res = write_res ? W.res : nullptr;  // the norm alone when no V-cycle follows
     kt_begin(ctx, VD_KT_RESID);
-    if (W.dim == 3 && stream_sweeps()) {
+    if (W.dim == 3 && W.stream) {
         const int ntx = (W.n[0] + RZ_X - 1) / RZ_X, nty = (W.n[1] + RZ_Y - 1) / RZ_Y;
         const int zc = resid_zc(W.n, ntx * nty);
         const dim3 gz((unsigned)(ntx * nty * ((W.n[2] + zc - 1) / zc)));
