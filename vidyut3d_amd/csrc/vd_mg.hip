@@ -355,6 +355,10 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
     // stage updates is a compile-time constant too.
     constexpr int NV = ((S + 2 + RES) + 1) & ~1;
     __shared__ T nbuf[S][FZ_RY][FZ_PX];
+    // BM 1: reciprocal diagonal of each cell of the thread's pair by the
+    // z class of its plane (bottom wall, interior, top wall), kept in LDS
+    // rather than registers
+    __shared__ T rdbuf[BM == 1 ? 3 : 1][2][FZ_THREADS];
     __shared__ T fbuf[RES ? 2 : 1][FZ_RY][FZ_PX];
     __shared__ double red[(FZ_THREADS + 63) / 64];
 
@@ -465,6 +469,15 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
         dgxy0 = dgxy0 + L.s1 * ddy;
         dgxy1 = L.alpha + L.s0 * ddw(ic1c > 0 || L.per0, ic1c < L.nx - 1 || L.per0, wx0, wx1);
         dgxy1 = dgxy1 + L.s1 * ddy;
+        // the z classes as st1 forms them for planes 0, 1 and nz-1 (nz >= 2)
+        const T ddl = ddw(L.per2, true, wz0, wz1), ddi = ddw(true, true, wz0, wz1),
+                ddh = ddw(true, L.per2, wz0, wz1);
+        rdbuf[0][0][threadIdx.x] = T(1) / (dgxy0 + L.s2 * ddl);
+        rdbuf[0][1][threadIdx.x] = T(1) / (dgxy1 + L.s2 * ddl);
+        rdbuf[1][0][threadIdx.x] = T(1) / (dgxy0 + L.s2 * ddi);
+        rdbuf[1][1][threadIdx.x] = T(1) / (dgxy1 + L.s2 * ddi);
+        rdbuf[2][0][threadIdx.x] = T(1) / (dgxy0 + L.s2 * ddh);
+        rdbuf[2][1][threadIdx.x] = T(1) / (dgxy1 + L.s2 * ddh);
     }
     auto st1 = [&](int u, int pc, T xl, T xh, T yl, T yh, T zl, T zh, T& dg, T& of) {
         const T ddz = ddw(pc > 0 || L.per2, pc < L.nz - 1 || L.per2, wz0, wz1);
@@ -536,17 +549,29 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
                 return side == 0 ? zlo_v : zhi_v;
             };
             T dg, of;
-            if constexpr (BM == 1)
-                st1(u, min(max(p, 0), L.nz - 1), xm, xp, ym, yp, zlo_v, zhi_v, dg, of);
-            else
+            if constexpr (BM == 1) {
+                // times the reciprocal diagonal (within an ulp of the quotient)
+                const int pc = min(max(p, 0), L.nz - 1);
+                st1(u, pc, xm, xp, ym, yp, zlo_v, zhi_v, dg, of);
+                const int zcl = pc == 0 ? 0 : (pc == L.nz - 1 ? 2 : 1);  // uniform
+                const T rd = rdbuf[zcl][u][threadIdx.x];
+                if (cs) {
+                    const T nv = (fb[q_s] + of) * rd;
+                    vb[q_s] = valid ? nv : vb[q_s];
+                } else {
+                    const T nv = (fr[q_s] + of) * rd;
+                    vr[q_s] = valid ? nv : vr[q_s];
+                }
+            } else {
                 stencil_nb<T, BM, 3, decltype(nb), true>(L, u ? ic1c : ic0c, jcc, min(max(p, 0), L.nz - 1), nb,
                                                           dg, of);
-            if (cs) {
-                const T nv = (fb[q_s] + of) / dg;
-                vb[q_s] = valid ? nv : vb[q_s];
-            } else {
-                const T nv = (fr[q_s] + of) / dg;
-                vr[q_s] = valid ? nv : vr[q_s];
+                if (cs) {
+                    const T nv = (fb[q_s] + of) / dg;
+                    vb[q_s] = valid ? nv : vb[q_s];
+                } else {
+                    const T nv = (fr[q_s] + of) / dg;
+                    vr[q_s] = valid ? nv : vr[q_s];
+                }
             }
         }
         const int po = t - S;  // final from here on
