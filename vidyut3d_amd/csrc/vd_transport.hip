@@ -287,7 +287,7 @@ struct XLineArgs {
     double mob;
     int lo_zero, hi_zero;
     int3 flo;
-    int nx, nrow, nseg;  // cells along x, rows (y, z), 63-face segments per row
+    int nx, nrow, nseg;  // cells along x, rows (y, z); nseg unused
     int ny;
 };
 
@@ -295,11 +295,18 @@ template <int S>
 __global__ void __launch_bounds__(256) k_xline_flux(XLineArgs a) {
     const unsigned wv = (lblock() * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
-    if (wv >= (unsigned)a.nrow * a.nseg) return;  // whole waves exit together
-    const int row = (int)(wv / a.nseg), seg = (int)(wv % a.nseg);
+    // the rows' reconstructed cells -1 .. nx laid end to end: wave wv
+    // covers positions 63 wv .. 63 wv + 63, so waves overlap by one cell
+    // and a wave may span the end of one row and the start of the next
+    const int rl = a.nx + 2;
+    const int64_t vtot = (int64_t)a.nrow * rl;
+    if ((int64_t)wv * 63 >= vtot) return;  // whole waves exit together
+    const int64_t v = (int64_t)wv * 63 + lane;
+    const bool live = v < vtot;
+    const int64_t vq = live ? v : vtot - 1;
+    const int row = (int)(vq / rl);
+    const int c = (int)(vq - (int64_t)row * rl) - 1;  // centre cell, relative to box lo
     const int jg = a.flo.y + row % a.ny, kg = a.flo.z + row / a.ny;
-    const int c = seg * 63 - 1 + lane;  // centre cell, relative to box lo
-    const bool live = c <= a.nx;        // cells -1 .. nx are reconstructed
     const double* __restrict__ u = a.sb.comp(a.specid);
     const double* __restrict__ E = a.sb.comp(a.efx);
     const int base = a.sb.off(a.flo.x, jg, kg);
@@ -319,8 +326,8 @@ __global__ void __launch_bounds__(256) k_xline_flux(XLineArgs a) {
     }
     // face c+1 between this lane's cell and the next lane's
     const double Ru1 = __shfl_down(Ru, 1, 64), Rf1 = __shfl_down(Rf, 1, 64), v1 = __shfl_down(vc, 1, 64);
-    const int f = c + 1;
-    if (lane < 63 && f >= 0 && f <= a.nx) {
+    const int f = c + 1;  // its right-hand cell is the next lane's when c < nx
+    if (lane < 63 && live && c < a.nx) {
         const double lam = fabs(0.5 * (vc + v1));
         double fl = 0.5 * (Lf + lam * Lu) + 0.5 * (Rf1 - lam * Ru1);
         const int fd = a.flo.x + f;
@@ -425,8 +432,8 @@ static int launch_xline(vd_ctx* ctx, const FluxArgs& f, int scheme, const vd_box
     a.nx = box->hi[0] - box->lo[0] + 1;
     a.ny = box->hi[1] - box->lo[1] + 1;
     a.nrow = a.ny * (box->hi[2] - box->lo[2] + 1);
-    a.nseg = (a.nx + 1 + 62) / 63;
-    const int64_t threads = (int64_t)a.nrow * a.nseg * 64;
+    a.nseg = 0;  // unused: waves run over the rows laid end to end
+    const int64_t threads = ((int64_t)a.nrow * (a.nx + 2) + 62) / 63 * 64;
     if (threads >= ((int64_t)1 << 32)) {
         set_error("transport: box too large for one flux launch");
         return VD_EINVAL;
