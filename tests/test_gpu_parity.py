@@ -290,6 +290,49 @@ def test_mlmg_dielectric_fp64_and_mixed(dev, prec):
     assert info.resnorm <= info.target
 
 
+@pytest.mark.parametrize("shape", [(5, 40, 36), (12, 20, 44), (8, 8, 1000), (6, 1000)])
+def test_mlmg_odd_and_thin_shapes(dev, shape):
+    """Levels that stop coarsening early: a 7200-cell bottom level that is
+    also the finest (the coarse V-cycle then keeps its levels in memory,
+    not LDS), odd dimensions after two coarsenings, a thin column whose
+    coarse tail starts two levels down, and a thin 2-D strip."""
+    rng = np.random.default_rng(sum(shape))
+    dim = len(shape)
+    g = abi.make_geom(list(shape), [0.0] * dim, [0.01 * n for n in shape], [0] * dim)  # cubic cells
+    og, dg = O.Grid(g), native.Grid(dev, g)
+    dims = [0, 1, 2] if dim == 3 else [0, 1, 0]
+    faces = []
+    for d in range(3):
+        fs = [1] + list(reversed(shape)) if dim == 3 else [1, 1] + list(reversed(shape))
+        if d < dim:
+            fs[len(fs) - 1 - d] += 1
+        faces.append(np.exp(0.3 * rng.normal(size=fs)))
+    rhs = rng.normal(size=[1] + ([1] if dim == 2 else []) + list(reversed(shape)))
+
+    def problem(views):
+        p = abi.vd_mlmg_problem()
+        p.alpha_a, p.beta = 0.0, 1.0
+        for d in range(3):
+            p.bface[d] = C.pointer(views[dims[d]])
+        p.bc_lo[:] = [abi.DIRCBC, abi.HNEUBC, abi.DIRCBC]
+        p.bc_hi[:] = [abi.HNEUBC, abi.DIRCBC, abi.DIRCBC]
+        p.reltol, p.abstol, p.maxiter = 1e-12, 0.0, 400
+        p.precision = 0
+        p.pre_smooth = p.post_smooth = 2
+        p.max_coarsening_level = 10
+        return p
+
+    hv = [og.view(faces[d], 0, d) for d in range(dim)]
+    ref, _ = O.mlmg_solve(og, problem(hv), og.zeros(1), rhs)
+    df = [dg.array(1, 0, d, host=faces[d]) for d in range(dim)]
+    dv = [f.view() for f in df]
+    phi = dg.array(1)
+    info = native.mlmg_solve(dg, problem(dv), phi, dg.array(1, host=rhs))
+    err = compare.field_errors(ref, phi.get())["rel_l2"]
+    assert err <= 1e-10, err
+    assert info.resnorm <= info.target
+
+
 # ---------------------------------------------------------- CFL and tags
 def test_time_scales(dev):
     c = rand_state_3d(12, 13)

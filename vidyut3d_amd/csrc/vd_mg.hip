@@ -1445,15 +1445,18 @@ __device__ void bottom_cg_wave(const LevV<T>& L, T* __restrict__ e_out, const T*
 // than 4/3 VC_TAIL_CELLS cells in all (2-D; 8/7 in 3-D)
 constexpr int VC_LDS_CELLS = 5464;
 
-template <class T, int BM, int DIM>
+// LDS: the tail's levels hold at most VC_LDS_CELLS cells per array (always
+// so when they coarsen; a bottom level that cannot coarsen may be larger,
+// and then every level stays in memory)
+template <class T, int BM, int DIM, bool LDS>
 __global__ void __launch_bounds__(1024) k_vcycle_tail(VTail<T> a) {
-    // every sweep, transfer and the bottom solve of the tail read and write
-    // LDS; only r of the first level comes from memory and only its e goes
-    // back (the next finer level prolongs from it)
-    __shared__ T le[VC_LDS_CELLS], lr[VC_LDS_CELLS];
-    auto e = [&](int l) { return le + a.off[l]; };
-    auto r = [&](int l) { return lr + a.off[l]; };
-    {
+    // with LDS, every sweep, transfer and the bottom solve of the tail read
+    // and write LDS; only r of the first level comes from memory and only
+    // its e goes back (the next finer level prolongs from it)
+    __shared__ T le[LDS ? VC_LDS_CELLS : 1], lr[LDS ? VC_LDS_CELLS : 1];
+    auto e = [&](int l) { return LDS ? le + a.off[l] : a.e[l]; };
+    auto r = [&](int l) { return LDS ? lr + a.off[l] : a.r[l]; };
+    if constexpr (LDS) {
         const int n0 = a.v[0].nx * a.v[0].ny * a.v[0].nz;
         for (int c = threadIdx.x; c < n0; c += blockDim.x) lr[c] = a.r[0][c];
     }
@@ -1498,7 +1501,7 @@ __global__ void __launch_bounds__(1024) k_vcycle_tail(VTail<T> a) {
             wg_sweep<T, BM, DIM>(F, e(l), r(l), 1);
         }
     }
-    {
+    if constexpr (LDS) {
         const int n0 = a.v[0].nx * a.v[0].ny * a.v[0].nz;
         for (int c = threadIdx.x; c < n0; c += blockDim.x) a.e[0][c] = le[c];
     }
@@ -2040,19 +2043,27 @@ static bool vcycle(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L,
         a.cp = B.p;
         a.cq = B.q;
         a.nl = nl - l0;
-        for (int l = 0, o = 0; l < a.nl; l++) {
-            a.off[l] = o;
-            o += (int)ncell(L[l0 + l].n);
+        int64_t tot = 0;
+        for (int l = 0; l < a.nl; l++) {
+            a.off[l] = (int)tot;
+            tot += ncell(L[l0 + l].n);
         }
+        const bool lds = tot <= VC_LDS_CELLS;
         a.nu1 = nu1;
         a.nu2 = nu2;
+#define VD_TAIL(BMV, DIMV)                                                                                   \
+    {                                                                                                        \
+        if (lds)                                                                                             \
+            hipLaunchKernelGGL((k_vcycle_tail<T, BMV, DIMV, true>), dim3(1), dim3(1024), 0, ctx->stream, a);  \
+        else                                                                                                 \
+            hipLaunchKernelGGL((k_vcycle_tail<T, BMV, DIMV, false>), dim3(1), dim3(1024), 0, ctx->stream, a); \
+    }
         if (bm == 1) {
-            if (W.dim == 3) hipLaunchKernelGGL((k_vcycle_tail<T, 1, 3>), dim3(1), dim3(1024), 0, ctx->stream, a);
-            else hipLaunchKernelGGL((k_vcycle_tail<T, 1, 2>), dim3(1), dim3(1024), 0, ctx->stream, a);
+            if (W.dim == 3) VD_TAIL(1, 3) else VD_TAIL(1, 2)
         } else {
-            if (W.dim == 3) hipLaunchKernelGGL((k_vcycle_tail<T, 0, 3>), dim3(1), dim3(1024), 0, ctx->stream, a);
-            else hipLaunchKernelGGL((k_vcycle_tail<T, 0, 2>), dim3(1), dim3(1024), 0, ctx->stream, a);
+            if (W.dim == 3) VD_TAIL(0, 3) else VD_TAIL(0, 2)
         }
+#undef VD_TAIL
     }
     // up
     for (int l = l0 - 1; l >= 0; l--) {
