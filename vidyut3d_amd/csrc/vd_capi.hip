@@ -115,12 +115,18 @@ int vd_ctx_create(int device, vd_ctx** out) {
         return VD_EHIP;
     }
     e = hipHostMalloc((void**)&c->pinned, 8192 * sizeof(double), hipHostMallocDefault);
+    if (e == hipSuccess)
+        e = hipHostMalloc((void**)&c->hnorm, 64 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent);
+    if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&c->hnorm_d, c->hnorm, 0);
     if (e != hipSuccess) {
+        if (c->hnorm) (void)hipHostFree(c->hnorm);
+        if (c->pinned) (void)hipHostFree(c->pinned);
         (void)hipStreamDestroy(c->stream);
         delete c;
         set_error("hipHostMalloc: %s", hipGetErrorString(e));
         return VD_EHIP;
     }
+    memset(c->hnorm, 0, 64 * sizeof(double));
     *out = c;
     return VD_OK;
 }
@@ -134,6 +140,7 @@ int vd_ctx_destroy(vd_ctx* ctx) {
     for (auto& kv : ctx->mg) mgwork_free(kv.second);
     for (auto e : ctx->kt_pool) (void)hipEventDestroy(e);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
+    if (ctx->hnorm) (void)hipHostFree(ctx->hnorm);
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return VD_OK;

@@ -160,6 +160,11 @@ struct vd_ctx {
     std::map<std::string, vd::MGWork*> mg;
     // small pinned host buffer for scalar read-backs
     double* pinned = nullptr;
+    // host-mapped coherent buffer the multigrid norm kernels write into
+    // (values, then a sequence number in the last slot the host waits on)
+    double* hnorm = nullptr;
+    double* hnorm_d = nullptr;  // its device address
+    unsigned long long norm_seq = 0;
     // kernel timing
     int kt_family = VD_KT_NONE;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> kt_events;

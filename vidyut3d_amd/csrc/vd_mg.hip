@@ -20,6 +20,7 @@
 //    hierarchy; the finest-level residual, the update x += e and the
 //    stopping test stay FP64 (iterative refinement).
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -2147,19 +2148,54 @@ static void zero_groups(vd_ctx* ctx, MGWork& W, int ngroups) {
 }
 
 // copies the first ngroups slot groups to the host and reduces each
-static int read_groups(vd_ctx* ctx, MGWork& W, int ngroups, double* out) {
-    size_t bytes = (size_t)ngroups * NSLOT * SLOT_STRIDE * sizeof(unsigned long long);
-    VD_HIP(hipMemcpyAsync(ctx->pinned, W.dnorm, bytes, hipMemcpyDeviceToHost, ctx->stream));
-    VD_HIP(hipStreamSynchronize(ctx->stream));
+// the max of each slot group into the host-mapped norm buffer, then the
+// call's sequence number into its last slot (one wave).  Slots hold the bit
+// patterns of non-negative doubles (NaN is stored as +inf), so their
+// maximum as doubles is exact.
+constexpr int HNORM_SEQ = 63;
+__global__ void __launch_bounds__(64) k_norm_out(const unsigned long long* __restrict__ slots, int ngroups,
+                                                 double* __restrict__ hout, unsigned long long seq) {
+    const int s = (int)threadIdx.x;
     for (int q = 0; q < ngroups; q++) {
-        double m = 0.0;
-        for (int s = 0; s < NSLOT; s++) {
-            double v = ctx->pinned[(size_t)q * NSLOT * SLOT_STRIDE + (size_t)s * SLOT_STRIDE];
-            if (v != v || v > m) m = (v != v) ? v : m > v ? m : v;
-            if (m != m) break;
-        }
-        out[q] = m;
+        double v = s < NSLOT ? __longlong_as_double((long long)slots[((size_t)q * NSLOT + s) * SLOT_STRIDE]) : 0.0;
+        for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+        if (s == 0) hout[q] = v;
     }
+    if (s == 0) {
+        __threadfence_system();
+        ((volatile unsigned long long*)hout)[HNORM_SEQ] = seq;
+    }
+}
+
+// the norms of up to 5 slot groups on the host: the host waits for the
+// sequence number k_norm_out writes last, rather than for a copy and a
+// stream synchronisation, and checks the stream for errors while it waits
+static int read_groups(vd_ctx* ctx, MGWork& W, int ngroups, double* out) {
+    const unsigned long long seq = ++ctx->norm_seq;
+    hipLaunchKernelGGL(k_norm_out, dim3(1), dim3(64), 0, ctx->stream, W.dnorm, ngroups, ctx->hnorm_d, seq);
+    VD_CHECK_LAUNCH();
+    volatile unsigned long long* flag = (volatile unsigned long long*)(ctx->hnorm + HNORM_SEQ);
+    for (unsigned long it = 1; *flag != seq; it++) {
+        if ((it & 1023) == 0) {
+            const hipError_t e = hipStreamQuery(ctx->stream);
+            if (e == hipSuccess) {
+                // the stream is done: the value is there unless the write
+                // did not reach host memory, which is an error
+                if (*flag != seq) {
+                    set_error("multigrid norm read-back: the device write did not arrive");
+                    return VD_EHIP;
+                }
+                break;
+            }
+            if (e != hipErrorNotReady) {
+                set_error("multigrid norm read-back: %s", hipGetErrorString(e));
+                return VD_EHIP;
+            }
+        }
+        __builtin_ia32_pause();
+    }
+    std::atomic_thread_fence(std::memory_order_acquire);
+    for (int q = 0; q < ngroups; q++) out[q] = ((volatile double*)ctx->hnorm)[q];
     return VD_OK;
 }
 
