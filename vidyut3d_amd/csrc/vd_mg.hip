@@ -930,8 +930,15 @@ __global__ void __launch_bounds__(GZ_X * GZ_Y) k_gsrb_z(LevV<T> L, T* __restrict
         T dg, of;
         stencil_nbf<T, 3, decltype(nb), decltype(fc), C0U>(L, i, jc, k, nb, fc, dg, of);
         if (own) {
-            x[ck + u] = (fv + of) / dg;
-            if constexpr (ZI) x[ck + 1 - u] = T(0);
+            // the whole pair in one store: the other cell keeps its value (0
+            // from zero), so every written line is written in full
+            using V2 = typename std::conditional<std::is_same<T, double>::value, double2, float2>::type;
+            const T nv = (fv + of) / dg;
+            const T keep = ZI ? T(0) : (u ? ac0 : ac1);
+            V2 w;
+            w.x = u ? keep : nv;
+            w.y = u ? nv : keep;
+            *reinterpret_cast<V2*>(x + ck) = w;
         }
         am0 = ac0;
         am1 = ac1;
