@@ -76,7 +76,23 @@ __device__ __forceinline__ void weno5_lr(double a, double b, double c, double d,
     double x1, x2, x3, y1, y2, y3;
     if (S == 3) {
         const double tau = fabs(b1 - b3);
-        const double r1 = sq(fdiv(tau, b1 + eps)), r2 = sq(fdiv(tau, b2 + eps)), r3 = sq(fdiv(tau, b3 + eps));
+        // tau / (b_k + eps) for k = 1..3 from one reciprocal of the product
+        // (within a few ulps of the three quotients); the separate quotients
+        // where the product is not a finite normal number
+        const double e1 = b1 + eps, e2 = b2 + eps, e3 = b3 + eps;
+        const double e12 = e1 * e2, prod = e12 * e3;
+        double t1, t2, t3;
+        if (__builtin_expect(prod < 1e300 && prod > 1e-300, 1)) {
+            const double ti = fdiv(tau, prod);
+            t1 = ti * (e2 * e3);
+            t2 = ti * (e1 * e3);
+            t3 = ti * e12;
+        } else {
+            t1 = fdiv(tau, e1);
+            t2 = fdiv(tau, e2);
+            t3 = fdiv(tau, e3);
+        }
+        const double r1 = sq(t1), r2 = sq(t2), r3 = sq(t3);
         x1 = g1 * (1.0 + r1);
         x2 = g2 * (1.0 + r2);
         x3 = g3 * (1.0 + r3);
