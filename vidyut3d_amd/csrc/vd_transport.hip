@@ -117,8 +117,19 @@ __device__ __forceinline__ void weno5_lr(double a, double b, double c, double d,
             y3 = gk(y3 * sy, g3);
         }
     }
-    L = fdiv(x1 * p1 + x2 * p2 + x3 * p3, x1 + x2 + x3);
-    R = fdiv(y1 * q1 + y2 * q2 + y3 * q3, y1 + y2 + y3);
+    // both normalisations from one reciprocal of the product of the weight
+    // sums, or separately where that product leaves the normal range
+    const double nl = x1 * p1 + x2 * p2 + x3 * p3, dl = x1 + x2 + x3;
+    const double nr = y1 * q1 + y2 * q2 + y3 * q3, dr = y1 + y2 + y3;
+    const double dlr = dl * dr;
+    if (__builtin_expect(dlr < 1e300 && dlr > 1e-300, 1)) {
+        const double ti = fdiv(1.0, dlr);
+        L = nl * (dr * ti);
+        R = nr * (dl * ti);
+    } else {
+        L = fdiv(nl, dl);
+        R = fdiv(nr, dr);
+    }
 }
 
 __device__ __forceinline__ double slope_lim(double c, double dl, double dm, double dr) {
