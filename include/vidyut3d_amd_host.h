@@ -30,6 +30,14 @@ double vd_sim_time(const vd_sim* s);
 long long vd_sim_mg_iterations(const vd_sim* s);
 /* Vidyut::WritePlotFile (Source/Plot.cpp:29-38): AMReX HyperCLaw-V1.1 */
 int vd_sim_write_plotfile(vd_sim* s, const char* dir);
+/* Vidyut::WriteCheckpointFile (Source/Plot.cpp:40-119): Header (finest
+ * level, istep, dt, t_new, BoxArray) + VisMF Level_0/phi of phi_new */
+int vd_sim_write_checkpoint(vd_sim* s, const char* dir);
+/* Vidyut::ReadCheckpointFile (Source/Plot.cpp:121-209): restores istep, dt,
+ * t_new and phi_new from a single-level checkpoint of the same domain */
+int vd_sim_read_checkpoint(vd_sim* s, const char* dir);
+/* istep of level 0 */
+int vd_sim_step(const vd_sim* s);
 /* ErrorEst on phi_new (1-ghost FillPatch then the Tagging.H kernels);
  * tags is a host array over the valid box */
 int vd_sim_error_est(vd_sim* s, int ntag, const int* comps, const double* thr, const double* gthr,

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
-from vidyut3d_amd import abi, cases, compare, native
+from vidyut3d_amd import abi, cases, compare, native, plotfile
 
 pytestmark = pytest.mark.gpu
 
@@ -486,3 +486,30 @@ def test_plotfile_from_sim_reads_back_and_matches_oracle(dev, tmp_path):
             assert e["err"] <= 1e-10 * vec, (name, e)  # component of a vector: scale by |E|
         else:
             assert e["rel"] <= 1e-10, (name, e)
+
+
+def test_checkpoint_restart_is_bit_exact(dev, tmp_path):
+    """WriteCheckpointFile / ReadCheckpointFile (Source/Plot.cpp:40-209):
+    the checkpoint holds the state, step, dt and time (read back by the
+    Python reader of the same layout), and a run restarted from it takes
+    the same steps as the run that wrote it, bit for bit."""
+    c = cases.air_3d(n=24, nsteps=5)
+    a = native.Sim(dev, c)
+    a.evolve(3)
+    chk = str(tmp_path / "chk00003")
+    a.write_checkpoint(chk)
+    s3, t3 = a.get_state(), a.time
+    a.evolve(2)
+    s_a = a.get_state()
+    a.close()
+    cp = plotfile.read_checkpoint(chk)
+    assert (cp.step, cp.dt, cp.time) == (3, c.dt, t3)
+    assert np.array_equal(cp.data, s3)
+    b = native.Sim(dev, cases.air_3d(n=24, nsteps=5))
+    b.read_checkpoint(chk)
+    assert (b.step, b.time) == (3, t3)
+    assert np.array_equal(b.get_state(), s3)
+    b.evolve(2)
+    s_b = b.get_state()
+    b.close()
+    assert np.array_equal(s_a, s_b)

@@ -60,3 +60,23 @@ def test_cli(tmp_path, capsys):
     assert plotfile.main([a, a, "y"]) == 0
     out = capsys.readouterr().out
     assert out.startswith("y") and "rel 0.000000e+00" in out
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_checkpoint_round_trip(tmp_path, dim):
+    """Vidyut's checkpoint layout (Source/Plot.cpp:40-119): Header lines
+    (title, finest level, istep, dt, t_new, BoxArray) and Level_0/phi."""
+    rng = np.random.default_rng(10 + dim)
+    shape = (4, 1, 5, 6) if dim == 2 else (4, 3, 5, 6)
+    data = rng.normal(size=shape) * 1e15
+    p = str(tmp_path / "chk00012")
+    plotfile.write_checkpoint(p, data, dim, 12, 1e-12, 1.2e-11)
+    lines = (tmp_path / "chk00012" / "Header").read_text().splitlines()
+    assert lines[0] == "Checkpoint file for Vidyut" and lines[1] == "0"
+    assert lines[2].split() == ["12"] and float(lines[3]) == 1e-12 and float(lines[4]) == 1.2e-11
+    assert lines[5] == "(1 0" and lines[-1] == ")"
+    c = plotfile.read_checkpoint(p)
+    assert (c.step, c.dt, c.time) == (12, 1e-12, 1.2e-11)
+    assert c.hi == [shape[3] - 1, shape[2] - 1, shape[1] - 1][:dim]
+    assert np.array_equal(c.data, data)
+    assert (tmp_path / "chk00012" / "Level_0" / "phi_H").read_text().count("FabOnDisk: phi_D_00000 0") == 1

@@ -81,6 +81,8 @@ class Vidyut {
     // Evolve (Source/Evolve.cpp:33-369) for nsteps at fixed dt
     int Evolve(int nsteps, Real dt, int ncorr);
     int WritePlotFile(const std::string& dir) const;
+    int WriteCheckpointFile(const std::string& dir) const;
+    int ReadCheckpointFile(const std::string& dir);
     int set_tagging(const std::vector<int>& comps, const std::vector<double>& thr,
                     const std::vector<double>& gthr);
 
@@ -120,6 +122,7 @@ class Vidyut {
     int ns_, nvar_, dim_, ngrow_fp_;
     int status_ = VD_OK;
     Real t_new_ = 0.0;
+    Real dt_ = 0.0;  // the last step's dt (checkpoint Header)
     int istep_ = 0;
     bool phi_old_stale_ = false;
     std::vector<MultiFab> phi_new_, phi_old_;

@@ -10,9 +10,11 @@
 #include <cerrno>
 #include <cfloat>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <iomanip>
+#include <iterator>
 #include <sstream>
 
 #include "../../include/vidyut3d_amd_host.h"
@@ -254,6 +256,7 @@ int Vidyut::ErrorEst(int lev, vd_tagarray4& tags, Real time, int ngrow) {
 
 // Source/Evolve.cpp:33-369 on one level with fixed dt
 int Vidyut::Evolve(int nsteps, Real dt, int ncorr) {
+    dt_ = dt;
     out_ = state_ = nullptr;
     const int st = ncorr == 1 ? evolve_one_corrector(nsteps, dt) : evolve_corrected(nsteps, dt, ncorr);
     out_ = state_ = nullptr;
@@ -423,6 +426,42 @@ static std::string box_str(const vd_box& b, int dim) {
     return o.str();
 }
 
+// One single-box VisMF MultiFab (version 1 header): <dir>/<prefix>_H and
+// <dir>/<prefix>_D_00000 holding the FAB (little-endian IEEE doubles,
+// component after component, x fastest), with per-component minima and
+// maxima in the header.  h holds ncomp x ncell values.
+static int write_vismf(const std::string& dir, const std::string& prefix, const std::vector<double>& h,
+                       int ncomp, int64_t ncell, const vd_box& box, int dim) {
+    const std::string base = prefix.substr(prefix.find_last_of('/') + 1);
+    {
+        std::ostringstream f;
+        f << "FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))" << box_str(box, dim) << " " << ncomp
+          << "\n";
+        const std::string hdr = f.str();
+        std::ofstream D(dir + "/" + prefix + "_D_00000", std::ios::binary);
+        D.write(hdr.data(), hdr.size());
+        D.write((const char*)h.data(), (size_t)ncomp * ncell * sizeof(double));
+        if (!D.good()) return VD_EINVAL;
+    }
+    std::vector<double> mn(ncomp, DBL_MAX), mx(ncomp, -DBL_MAX);
+    for (int q = 0; q < ncomp; q++)
+        for (int64_t c = 0; c < ncell; c++) {
+            double v = h[(size_t)q * ncell + c];
+            mn[q] = std::min(mn[q], v);
+            mx[q] = std::max(mx[q], v);
+        }
+    std::ofstream C(dir + "/" + prefix + "_H");
+    C.precision(16);
+    C << std::scientific;
+    C << "1\n1\n" << ncomp << "\n0\n(1 0\n" << box_str(box, dim) << "\n)\n1\nFabOnDisk: " << base << "_D_00000 0\n\n";
+    C << "1," << ncomp << "\n";
+    for (int q = 0; q < ncomp; q++) C << mn[q] << ",";
+    C << "\n\n1," << ncomp << "\n";
+    for (int q = 0; q < ncomp; q++) C << mx[q] << ",";
+    C << "\n";
+    return C.good() ? VD_OK : VD_EINVAL;
+}
+
 // WritePlotFile -> amrex::WriteMultiLevelPlotfile (Source/Plot.cpp:29-38):
 // HyperCLaw-V1.1 Header, Level_0/Cell_H (VisMF v1) and one Cell_D file
 // holding the FAB (little-endian IEEE doubles, Fortran order per comp).
@@ -454,32 +493,82 @@ int Vidyut::WritePlotFile(const std::string& dir) const {
         }
         H << "Level_0/Cell\n";
     }
-    std::string hdr;
+    return write_vismf(dir, "Level_0/Cell", h, nvar_, a.nstride, dom_, dim_);
+}
+
+// WriteCheckpointFile (Source/Plot.cpp:40-119): a "Checkpoint file for
+// Vidyut" Header (finest level, istep, dt and t_new per level, 17
+// significant digits, then the BoxArray) and phi_new as the VisMF MultiFab
+// Level_0/phi.
+int Vidyut::WriteCheckpointFile(const std::string& dir) const {
+    const vd_array4& a = phi_new_[0].array();
+    size_t n = (size_t)a.nstride * nvar_;
+    std::vector<double> h(n);
+    TRY(vd_memcpy_d2h(ctx_, h.data(), a.p, n * sizeof(double)));
+    TRY(mkdir_p(dir));
+    TRY(mkdir_p(dir + "/Level_0"));
     {
-        std::ostringstream f;
-        f << "FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))" << box_str(dom_, dim_) << " "
-          << nvar_ << "\n";
-        hdr = f.str();
-        std::ofstream D(dir + "/Level_0/Cell_D_00000", std::ios::binary);
-        D.write(hdr.data(), hdr.size());
-        D.write((const char*)h.data(), n * sizeof(double));
+        std::ofstream H(dir + "/Header");
+        H.precision(17);
+        H << "Checkpoint file for Vidyut\n" << 0 << "\n";
+        H << istep_ << " \n" << dt_ << " \n" << t_new_ << " \n";
+        H << "(1 0\n" << box_str(dom_, dim_) << "\n)\n";
+        if (!H.good()) return VD_EINVAL;
     }
-    std::vector<double> mn(nvar_, DBL_MAX), mx(nvar_, -DBL_MAX);
-    for (int q = 0; q < nvar_; q++)
-        for (int64_t c = 0; c < a.nstride; c++) {
-            double v = h[(size_t)q * a.nstride + c];
-            mn[q] = std::min(mn[q], v);
-            mx[q] = std::max(mx[q], v);
+    return write_vismf(dir, "Level_0/phi", h, nvar_, a.nstride, dom_, dim_);
+}
+
+// ReadCheckpointFile (Source/Plot.cpp:121-209) for a single-level
+// checkpoint of this domain: istep, dt, t_new and phi_new.  The FAB may be
+// in either byte order.
+int Vidyut::ReadCheckpointFile(const std::string& dir) {
+    std::ifstream H(dir + "/Header");
+    if (!H.good()) return VD_ENOTFOUND;
+    std::string line;
+    std::getline(H, line);  // title
+    int finest = -1;
+    H >> finest;
+    if (finest != 0) return VD_EINVAL;  // one level
+    long long st = 0;
+    double dtv = 0.0, tv = 0.0;
+    H >> st >> dtv >> tv;
+    std::getline(H, line);
+    std::string rest((std::istreambuf_iterator<char>(H)), std::istreambuf_iterator<char>());
+    if (rest.find(box_str(dom_, dim_)) == std::string::npos) return VD_EINVAL;  // another domain
+    std::ifstream C(dir + "/Level_0/phi_H");
+    if (!C.good()) return VD_ENOTFOUND;
+    std::string ch((std::istreambuf_iterator<char>(C)), std::istreambuf_iterator<char>());
+    const size_t fp = ch.find("FabOnDisk: ");
+    if (fp == std::string::npos) return VD_EINVAL;
+    std::istringstream fl(ch.substr(fp + 11));
+    std::string fname;
+    long long off = 0;
+    fl >> fname >> off;
+    std::ifstream D(dir + "/Level_0/" + fname, std::ios::binary);
+    if (!D.good()) return VD_ENOTFOUND;
+    D.seekg(off);
+    std::string fab;
+    std::getline(D, fab);
+    const bool le = fab.find("(8 7 6 5 4 3 2 1)") != std::string::npos;
+    const bool be = fab.find("(1 2 3 4 5 6 7 8)") != std::string::npos;
+    if (fab.compare(0, 4, "FAB ") != 0 || !(le || be)) return VD_EINVAL;
+    const size_t nb = fab.rfind(' ');
+    if (nb == std::string::npos || std::atoi(fab.c_str() + nb + 1) != nvar_) return VD_EINVAL;
+    const vd_array4& a = phi_new_[0].array();
+    const size_t n = (size_t)a.nstride * nvar_;
+    std::vector<double> h(n);
+    D.read((char*)h.data(), n * sizeof(double));
+    if ((size_t)D.gcount() != n * sizeof(double)) return VD_EINVAL;
+    if (be)
+        for (auto& v : h) {
+            unsigned char* b = (unsigned char*)&v;
+            std::reverse(b, b + 8);
         }
-    std::ofstream C(dir + "/Level_0/Cell_H");
-    C.precision(16);
-    C << std::scientific;
-    C << "1\n1\n" << nvar_ << "\n0\n(1 0\n" << box_str(dom_, dim_) << "\n)\n1\nFabOnDisk: Cell_D_00000 0\n\n";
-    C << "1," << nvar_ << "\n";
-    for (int q = 0; q < nvar_; q++) C << mn[q] << ",";
-    C << "\n\n1," << nvar_ << "\n";
-    for (int q = 0; q < nvar_; q++) C << mx[q] << ",";
-    C << "\n";
+    TRY(vd_memcpy_h2d(ctx_, a.p, h.data(), n * sizeof(double)));
+    istep_ = (int)st;
+    dt_ = dtv;
+    t_new_ = tv;
+    phi_old_stale_ = false;
     return VD_OK;
 }
 
@@ -541,6 +630,18 @@ double vd_sim_time(const vd_sim* s) { return s->v->time(); }
 long long vd_sim_mg_iterations(const vd_sim* s) { return s->v->mg_iterations(); }
 
 int vd_sim_write_plotfile(vd_sim* s, const char* dir) { return s->v->WritePlotFile(dir); }
+
+int vd_sim_write_checkpoint(vd_sim* s, const char* dir) {
+    if (!s || !dir) return VD_EINVAL;
+    return s->v->WriteCheckpointFile(dir);
+}
+
+int vd_sim_read_checkpoint(vd_sim* s, const char* dir) {
+    if (!s || !dir) return VD_EINVAL;
+    return s->v->ReadCheckpointFile(dir);
+}
+
+int vd_sim_step(const vd_sim* s) { return s->v->step(); }
 
 int vd_sim_error_est(vd_sim* s, int ntag, const int* comps, const double* thr, const double* gthr,
                      char* host_tags) {

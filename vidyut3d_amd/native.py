@@ -319,6 +319,18 @@ class Sim:
     def write_plotfile(self, path):
         check(lib().vd_sim_write_plotfile(self.ptr, path.encode()), "vd_sim_write_plotfile")
 
+    def write_checkpoint(self, path):
+        """WriteCheckpointFile: Header + Level_0/phi (VisMF) of the state."""
+        check(lib().vd_sim_write_checkpoint(self.ptr, path.encode()), "vd_sim_write_checkpoint")
+
+    def read_checkpoint(self, path):
+        """ReadCheckpointFile: restores the state, step, dt and time."""
+        check(lib().vd_sim_read_checkpoint(self.ptr, path.encode()), "vd_sim_read_checkpoint")
+
+    @property
+    def step(self):
+        return lib().vd_sim_step(self.ptr)
+
     def error_est(self, comps, thr, gthr):
         n = len(comps)
         out = np.zeros(self.shape[1:], dtype=np.int8)

@@ -15,6 +15,12 @@ amrex::WriteMultiLevelPlotfile (Source/Plot.cpp:29-38):
   followed by the box data, IEEE doubles in the byte order the descriptor
   names, component after component, x fastest.
 
+Checkpoints (``read_checkpoint`` / ``write_checkpoint``) follow
+Vidyut::WriteCheckpointFile (Source/Plot.cpp:40-119): a ``Header`` with the
+title ``Checkpoint file for Vidyut``, the finest level, istep, dt and t_new
+per level and the BoxArray, and the state as the VisMF MultiFab
+``Level_0/phi`` (``phi_H`` + ``phi_D_00000``, the same FAB layout).
+
 ``compare`` follows py_utils/compare_fields_from_twoplotfiles.py:39-46
 (err = sqrt(mean((f1-f2)^2)), rel = err / mean(|f1|)) and also reports
 the relative L2 norm ||f1-f2||_2 / ||f1||_2.
@@ -74,13 +80,20 @@ def read(path):
     dx = [float(x) for x in lines[p + 8].split()]
     rest = lines[p + 9:]
     cell = [ln.strip() for ln in rest if ln.strip().endswith("/Cell")][-1]
-    with open(os.path.join(path, cell + "_H")) as f:
+    data = _read_vismf(os.path.join(path, cell), dim)
+    return Plotfile(names, data, dim, time, step, prob_lo, prob_hi, dx, lo, hi)
+
+
+def _read_vismf(prefix, dim):
+    """The single FAB of the VisMF MultiFab <prefix>_H / its data file, as
+    an array (ncomp, nz, ny, nx)."""
+    with open(prefix + "_H") as f:
         vh = f.read()
     fabs = re.findall(r"FabOnDisk:\s+(\S+)\s+(\d+)", vh)
     if len(fabs) != 1:
-        raise ValueError("%s: %d boxes on level 0; this reader expects one" % (path, len(fabs)))
+        raise ValueError("%s: %d boxes on level 0; this reader expects one" % (prefix, len(fabs)))
     fname, off = fabs[0][0], int(fabs[0][1])
-    ddir = os.path.dirname(os.path.join(path, cell))
+    ddir = os.path.dirname(prefix)
     with open(os.path.join(ddir, fname), "rb") as f:
         f.seek(off)
         hdr = b""
@@ -105,8 +118,66 @@ def read(path):
         ext = [bhi[d] - blo[d] + 1 for d in range(dim)] + [1] * (3 - dim)
         n = ext[0] * ext[1] * ext[2] * ncomp
         raw = np.frombuffer(f.read(n * 8), dtype=dt, count=n)
-    data = raw.astype(np.float64).reshape(ncomp, ext[2], ext[1], ext[0])
-    return Plotfile(names, data, dim, time, step, prob_lo, prob_hi, dx, lo, hi)
+    return raw.astype(np.float64).reshape(ncomp, ext[2], ext[1], ext[0])
+
+
+def _write_vismf(prefix, data, box_text):
+    """data (ncomp, nz, ny, nx) as the single-FAB VisMF MultiFab <prefix>."""
+    ncomp = data.shape[0]
+    base = os.path.basename(prefix)
+    hdr = "FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))%s %d\n" % (box_text, ncomp)
+    with open(prefix + "_D_00000", "wb") as f:
+        f.write(hdr.encode())
+        f.write(np.ascontiguousarray(data, dtype="<f8").tobytes())
+    with open(prefix + "_H", "w") as f:
+        f.write("1\n1\n%d\n0\n(1 0\n%s\n)\n1\nFabOnDisk: %s_D_00000 0\n\n" % (ncomp, box_text, base))
+        flat = data.reshape(ncomp, -1)
+        f.write("1,%d\n%s\n\n1,%d\n%s\n" % (ncomp, "".join("%.16e," % v for v in flat.min(1)), ncomp,
+                                          "".join("%.16e," % v for v in flat.max(1))))
+
+
+class Checkpoint:
+    """Level-0 contents of a Vidyut checkpoint: step, dt, time and the state
+    (ncomp, nz, ny, nx)."""
+
+    def __init__(self, step, dt, time, data, lo, hi):
+        self.step, self.dt, self.time, self.data = step, dt, time, data
+        self.lo, self.hi = list(lo), list(hi)
+
+
+def read_checkpoint(path):
+    """Reads the checkpoint Vidyut::WriteCheckpointFile writes
+    (Source/Plot.cpp:40-119): the Header "Checkpoint file for Vidyut",
+    finest level, istep / dt / t_new per level, the BoxArray, and the VisMF
+    MultiFab Level_0/phi."""
+    with open(os.path.join(path, "Header")) as f:
+        lines = [ln.rstrip("\n") for ln in f]
+    if not lines or lines[0].strip() != "Checkpoint file for Vidyut":
+        raise ValueError("%s: not a Vidyut checkpoint" % path)
+    if int(lines[1]) != 0:
+        raise ValueError("%s: %d levels; this reader handles a single level" % (path, int(lines[1]) + 1))
+    step = int(lines[2].split()[0])
+    dt = float(lines[3].split()[0])
+    time = float(lines[4].split()[0])
+    lo, hi = _box("\n".join(lines[5:]))
+    data = _read_vismf(os.path.join(path, "Level_0", "phi"), len(lo))
+    return Checkpoint(step, dt, time, data, lo, hi)
+
+
+def write_checkpoint(path, data, dim, step, dt, time, lo=None):
+    """Writes data (ncomp, nz, ny, nx) as a single-level checkpoint in the
+    layout above."""
+    data = np.ascontiguousarray(data, dtype="<f8")
+    ncomp, nz, ny, nx = data.shape
+    ext = [nx, ny, nz][:dim]
+    lo = [0] * dim if lo is None else list(lo)
+    hi = [lo[d] + ext[d] - 1 for d in range(dim)]
+    j = lambda v: ",".join(str(x) for x in v)
+    box = "((%s) (%s) (%s))" % (j(lo), j(hi), j([0] * dim))
+    os.makedirs(os.path.join(path, "Level_0"), exist_ok=True)
+    with open(os.path.join(path, "Header"), "w") as f:
+        f.write("Checkpoint file for Vidyut\n0\n%d \n%r \n%r \n(1 0\n%s\n)\n" % (step, float(dt), float(time), box))
+    _write_vismf(os.path.join(path, "Level_0", "phi"), data, box)
 
 
 def write(path, names, data, dim, time, step, prob_lo, prob_hi, lo=None):
@@ -134,15 +205,7 @@ def write(path, names, data, dim, time, step, prob_lo, prob_hi, lo=None):
         for d in range(dim):
             f.write("%r %r\n" % (prob_lo[d] + dx[d] * lo[d], prob_lo[d] + dx[d] * (hi[d] + 1)))
         f.write("Level_0/Cell\n")
-    hdr = "FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))%s %d\n" % (box(lo, hi), ncomp)
-    with open(os.path.join(path, "Level_0", "Cell_D_00000"), "wb") as f:
-        f.write(hdr.encode())
-        f.write(data.tobytes())
-    with open(os.path.join(path, "Level_0", "Cell_H"), "w") as f:
-        f.write("1\n1\n%d\n0\n(1 0\n%s\n)\n1\nFabOnDisk: Cell_D_00000 0\n\n" % (ncomp, box(lo, hi)))
-        flat = data.reshape(ncomp, -1)
-        f.write("1,%d\n%s\n\n1,%d\n%s\n" % (ncomp, "".join("%.16e," % v for v in flat.min(1)), ncomp,
-                                          "".join("%.16e," % v for v in flat.max(1))))
+    _write_vismf(os.path.join(path, "Level_0", "Cell"), data, box(lo, hi))
 
 
 def compare(a, b, fields=None):
