@@ -2370,7 +2370,14 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
     v0.dcell = dcell;
     double r0, rn, bnorm;
     bool smooth_only = smooth_mode == 1;
-    if (smooth_mode == 2) {
+    if (smooth_mode == 2 && bm == 1) {
+        // constant coefficient: every face is b, so the bound needs no pass,
+        // and the residual is stored only when V-cycles follow
+        double rho = 0.0;
+        for (int d = 0; d < W.dim; d++) rho += 2.0 * fabs(beta / (W.dx[d] * W.dx[d])) * fabs(bconst);
+        smooth_only = alpha != 0.0 && rho / fabs(alpha) <= SMOOTH_ONLY_RHO;
+        VD_TRY(resid0(ctx, W, bm, v0, &r0, &bnorm, !smooth_only));
+    } else if (smooth_mode == 2) {
         double bmax[3] = {0.0, 0.0, 0.0};
         VD_TRY(resid0(ctx, W, bm, v0, &r0, &bnorm, true, bmax));
         if (alpha != 0.0) {
