@@ -69,3 +69,19 @@ def test_params_default_matches_reference_defaults():
 
 def test_version_string():
     assert b"gfx950" in native.lib().vd_version()
+
+
+def test_null_context_is_rejected_without_touching_the_device():
+    """Memory entry points given no context return VD_EINVAL (a handle that
+    outlives its context must not dereference it)."""
+    L = native.lib()
+    p = ctypes.c_void_p()
+    buf = (ctypes.c_char * 8)()
+    assert L.vd_free(None, None) == abi.VD_EINVAL
+    assert L.vd_malloc(None, ctypes.c_size_t(8), ctypes.byref(p)) == abi.VD_EINVAL
+    assert L.vd_memcpy_h2d(None, None, buf, ctypes.c_size_t(8)) == abi.VD_EINVAL
+    assert L.vd_memcpy_d2h(None, buf, None, ctypes.c_size_t(8)) == abi.VD_EINVAL
+    assert L.vd_memset_zero(None, None, ctypes.c_size_t(8)) == abi.VD_EINVAL
+    assert L.vd_ctx_sync(None) == abi.VD_EINVAL
+    assert L.vd_mech_destroy(None) == abi.VD_EINVAL
+    assert L.vd_sim_destroy(None) == abi.VD_EINVAL

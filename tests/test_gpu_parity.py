@@ -118,6 +118,33 @@ def test_transport_rhs_3d_weno_z(dev):
         assert relmax(ds, dds.get()) < 1e-12, spec
 
 
+@pytest.mark.parametrize("scale", [1e40, 1e-30])
+def test_transport_rhs_3d_weno_z_extreme_magnitudes(dev, scale):
+    """WENO-Z forms its three tau/(beta+eps) ratios and its two weight
+    normalisations from one reciprocal each; where those products leave the
+    normal range (densities ~1e55: smoothness indicators ~1e110) it falls
+    back to separate quotients.  Both regimes against the oracle's
+    per-face IEEE evaluation."""
+    c = rand_state_3d(10, 6)
+    ns = c.mech.nspec
+    c.state[:ns] *= scale
+    og, dg = O.Grid(c.geom), native.Grid(dev, c.geom)
+    sb = O.fill_patch(og, c.state, 3)
+    rng = np.random.default_rng(8)
+    rxn = rng.normal(size=(ns + 1,) + c.state.shape[1:]) * 1e20 * scale
+    mech = native.Mech(dev, c.mech)
+    dsb = dg.array(c.state.shape[0], 3, host=sb)
+    drx = dg.array(ns + 1, host=rxn)
+    for spec in (0, 2):
+        fl = O.transport_flux(c.mech, c.params, og, sb, 3, spec, list(c.params.ion_bc_lo),
+                              list(c.params.ion_bc_hi), dt=c.dt)
+        ds = O.dsdt(og, fl, rxn, spec)
+        dds = dg.array(1)
+        native.transport_rhs(mech, c.params, dg, dsb, drx, dds, spec, list(c.params.ion_bc_lo),
+                             list(c.params.ion_bc_hi), dt=c.dt)
+        assert relmax(ds, dds.get()) < 1e-12, spec
+
+
 # ------------------------------------------------------------ chemistry
 def test_rxn_ckwc_2d(dev):
     c = cases.discharge_2d(n=16)
@@ -513,3 +540,17 @@ def test_checkpoint_restart_is_bit_exact(dev, tmp_path):
     s_b = b.get_state()
     b.close()
     assert np.array_equal(s_a, s_b)
+
+
+@pytest.mark.gpu
+def test_handles_outliving_their_device_are_released_with_it():
+    """Closing a device releases the arrays, mechanisms and sims made on it;
+    dropping them afterwards frees nothing through the destroyed context."""
+    d = native.Device(0)
+    c = cases.discharge_2d(n=16, seed=1, nsteps=1)
+    a = native.Grid(d, c.geom).array(2, 1)
+    m = native.Mech(d, c.mech)
+    s = native.Sim(d, c)
+    d.close()
+    assert not a.ptr and not m.ptr and not s.ptr
+    del a, m, s

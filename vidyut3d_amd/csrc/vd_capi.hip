@@ -147,11 +147,13 @@ int vd_ctx_destroy(vd_ctx* ctx) {
 }
 
 int vd_ctx_sync(vd_ctx* ctx) {
+    if (!ctx) return VD_EINVAL;
     VD_HIP(hipStreamSynchronize(ctx->stream));
     return VD_OK;
 }
 
 int vd_ctx_release_workspaces(vd_ctx* ctx) {
+    if (!ctx) return VD_EINVAL;
     VD_HIP(hipStreamSynchronize(ctx->stream));
     for (auto& kv : ctx->scratch)
         if (kv.second.p) VD_HIP(hipFree(kv.second.p));
@@ -164,6 +166,7 @@ int vd_ctx_release_workspaces(vd_ctx* ctx) {
 void* vd_ctx_stream(vd_ctx* ctx) { return (void*)ctx->stream; }
 
 int vd_malloc(vd_ctx* ctx, size_t bytes, void** out) {
+    if (!ctx) return VD_EINVAL;
     VD_HIP(hipSetDevice(ctx->device));
     hipError_t e = hipMalloc(out, bytes ? bytes : 8);
     if (e != hipSuccess) {
@@ -174,24 +177,28 @@ int vd_malloc(vd_ctx* ctx, size_t bytes, void** out) {
 }
 
 int vd_free(vd_ctx* ctx, void* p) {
+    if (!ctx) return VD_EINVAL;
     VD_HIP(hipStreamSynchronize(ctx->stream));
     VD_HIP(hipFree(p));
     return VD_OK;
 }
 
 int vd_memcpy_h2d(vd_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return VD_EINVAL;
     VD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
     VD_HIP(hipStreamSynchronize(ctx->stream));
     return VD_OK;
 }
 
 int vd_memcpy_d2h(vd_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return VD_EINVAL;
     VD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     VD_HIP(hipStreamSynchronize(ctx->stream));
     return VD_OK;
 }
 
 int vd_memcpy_d2d(vd_ctx* ctx, void* dst, const void* src, size_t bytes) {
+    if (!ctx) return VD_EINVAL;
     VD_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
     return VD_OK;
 }
@@ -199,7 +206,10 @@ int vd_memcpy_d2d(vd_ctx* ctx, void* dst, const void* src, size_t bytes) {
 int vd_memset_zero_kernel(vd_ctx* ctx, void* dst, size_t bytes);
 
 // a fill kernel, not hipMemsetAsync, so captured steps replay correctly
-int vd_memset_zero(vd_ctx* ctx, void* dst, size_t bytes) { return vd_memset_zero_kernel(ctx, dst, bytes); }
+int vd_memset_zero(vd_ctx* ctx, void* dst, size_t bytes) {
+    if (!ctx) return VD_EINVAL;
+    return vd_memset_zero_kernel(ctx, dst, bytes);
+}
 
 int vd_mech_create(vd_ctx* ctx, const vd_mech_desc* d, vd_mech** out) {
     if (!ctx || !d || !out) return VD_EINVAL;

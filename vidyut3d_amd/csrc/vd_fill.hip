@@ -37,6 +37,40 @@ int fill_value(vd_ctx* ctx, double* p, int64_t n, double v) {
     return VD_OK;
 }
 
+// several constant fills in one launch (blockIdx.y = fill)
+constexpr int FILL_BATCH = 32;
+struct FillBatch {
+    double* p[FILL_BATCH];
+    int64_t n[FILL_BATCH];
+    double v[FILL_BATCH];
+};
+
+__global__ void k_fill_batch(FillBatch b) {
+    const int q = blockIdx.y;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < b.n[q]; i += stride) b.p[q][i] = b.v[q];
+}
+
+int fill_values(vd_ctx* ctx, double* const* p, const int64_t* n, const double* v, int count) {
+    for (int base = 0; base < count; base += FILL_BATCH) {
+        FillBatch b;
+        const int k = count - base < FILL_BATCH ? count - base : FILL_BATCH;
+        int64_t nmax = 0;
+        for (int q = 0; q < k; q++) {
+            b.p[q] = p[base + q];
+            b.n[q] = n[base + q];
+            b.v[q] = v[base + q];
+            if (n[base + q] > nmax) nmax = n[base + q];
+        }
+        if (nmax <= 0) continue;
+        dim3 g = grid_stride(nmax, 256);
+        g.y = (unsigned)k;
+        hipLaunchKernelGGL(k_fill_batch, g, dim3(256), 0, ctx->stream, b);
+    }
+    VD_CHECK_LAUNCH();
+    return VD_OK;
+}
+
 int fill_value_f(vd_ctx* ctx, float* p, int64_t n, float v) {
     if (n <= 0) return VD_OK;
     hipLaunchKernelGGL(k_fill_value_f, grid_stride(n, 256), dim3(256), 0, ctx->stream, p, n, v);

@@ -31,6 +31,7 @@
 namespace vd {
 
 int fill_value(vd_ctx* ctx, double* p, int64_t n, double v);
+int fill_values(vd_ctx* ctx, double* const* p, const int64_t* n, const double* v, int count);
 int fill_value_f(vd_ctx* ctx, float* p, int64_t n, float v);
 
 // ------------------------------------------------------------ data
@@ -1876,6 +1877,9 @@ static int get_work(vd_ctx* ctx, const vd_geom* g, const vd_box* box, int prec, 
     W->res = (double*)p;
     if ((st = dalloc(W, 8 * NSLOT * SLOT_STRIDE * sizeof(unsigned long long), &p))) goto fail;
     W->dnorm = (unsigned long long*)p;
+    // the norm slots start at zero; k_norm_out clears what it reads
+    hipLaunchKernelGGL(k_zero<unsigned long long>, grid1d(8 * NSLOT * SLOT_STRIDE, 256), dim3(256), 0,
+                       ctx->stream, W->dnorm, (int64_t)8 * NSLOT * SLOT_STRIDE);
     {
         // per-block partial norms of k_resid0 and of the fused smoother
         int64_t np = ((n[0] + 255) / 256) * (int64_t)n[1] * n[2];
@@ -2149,22 +2153,23 @@ static inline dim3 norm_grid(int64_t n) {
 // slot group q of the norm buffer
 static inline unsigned long long* slot_group(MGWork& W, int q) { return W.dnorm + (size_t)q * NSLOT * SLOT_STRIDE; }
 
-static void zero_groups(vd_ctx* ctx, MGWork& W, int ngroups) {
-    int64_t n = (int64_t)ngroups * NSLOT * SLOT_STRIDE;
-    hipLaunchKernelGGL(k_zero<unsigned long long>, grid1d(n, 256), dim3(256), 0, ctx->stream, W.dnorm, n);
-}
-
-// copies the first ngroups slot groups to the host and reduces each
 // the max of each slot group into the host-mapped norm buffer, then the
 // call's sequence number into its last slot (one wave).  Slots hold the bit
 // patterns of non-negative doubles (NaN is stored as +inf), so their
-// maximum as doubles is exact.
+// maximum as doubles is exact.  Each slot is cleared by the lane that read
+// it, so the groups start at zero for the next norm (every accumulation
+// into slot groups is read back by exactly one read_groups call).
 constexpr int HNORM_SEQ = 63;
-__global__ void __launch_bounds__(64) k_norm_out(const unsigned long long* __restrict__ slots, int ngroups,
+__global__ void __launch_bounds__(64) k_norm_out(unsigned long long* __restrict__ slots, int ngroups,
                                                  double* __restrict__ hout, unsigned long long seq) {
     const int s = (int)threadIdx.x;
     for (int q = 0; q < ngroups; q++) {
-        double v = s < NSLOT ? __longlong_as_double((long long)slots[((size_t)q * NSLOT + s) * SLOT_STRIDE]) : 0.0;
+        double v = 0.0;
+        if (s < NSLOT) {
+            const size_t o = ((size_t)q * NSLOT + s) * SLOT_STRIDE;
+            v = __longlong_as_double((long long)slots[o]);
+            slots[o] = 0ull;
+        }
         for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
         if (s == 0) hout[q] = v;
     }
@@ -2212,7 +2217,6 @@ static int read_groups(vd_ctx* ctx, MGWork& W, int ngroups, double* out) {
 // direction -- all from the one pass over the level
 static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double* nrm_out,
                   double* bnorm = nullptr, bool write_res = true, double* fmax = nullptr) {
-    zero_groups(ctx, W, 5);
     int64_t nc = ncell(W.n);
     const int gx = (W.n[0] + 255) / 256;
     const int nblk = gx * W.n[1] * W.n[2];
@@ -2326,7 +2330,6 @@ static int jacobi_rho(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
     if (bm == 1) {
         for (int d = 0; d < W.dim; d++) bmax[d] = fabs(bconst);
     } else {
-        zero_groups(ctx, W, W.dim);
         for (int d = 0; d < W.dim; d++) {
             int64_t nf = nfacef(W.n, d);
             hipLaunchKernelGGL(k_maxabs, norm_grid(nf), dim3(256), 0, ctx->stream, W.bf0[d], (unsigned)nf,
@@ -2435,7 +2438,6 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
         for (it = 1; it <= maxiter; it++) {
             if (smooth_only && W.x2 && W.c0uni && bm != 2) {
                 // one red-black sweep and the residual norm of its result in one pass
-                zero_groups(ctx, W, 1);
                 int nblk = 0;
                 launch_fused<double>(ctx, W, bm, v0, W.x, W.x2, W.f, 1, 1, W.part, &nblk);
                 std::swap(W.x, W.x2);
@@ -2766,6 +2768,8 @@ static void dims_of(const vd_box* box, const vd_geom* g, int n[3], int lo[3]) {
     }
 }
 
+constexpr int FILL_SIDES = 25;  // 6 sides x (value, Robin a, b, f) + initial guess
+
 int solve_potential(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4* phi,
                     vd_array4 ef[3], const vd_box* box, const vd_geom* g, const vd_params* prm,
                     double time, vd_solve_info* info) {
@@ -2784,21 +2788,37 @@ int solve_potential(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array
     ra.nz = n[2];
     ra.spacechrg = prm->do_spacechrg;
     hipLaunchKernelGGL(k_pot_rhs, grid1d(nc, 256), dim3(256), 0, ctx->stream, ra, W->f);
-    // potential_bc (PlasmaChem.H:18-72): Dirichlet / Robin(1,0,V) value V
-    for (int d = 0; d < g->dim; d++) {
-        if (W->per[d]) continue;
-        for (int s = 0; s < 2; s++) {
-            double V = applied_potential(time, s ? 1 : -1, prm);
-            int64_t nt = ntanf(n, d);
-            VD_TRY(fill_value(ctx, W->bcv[d][s], nt, V));
-            VD_TRY(fill_value(ctx, W->rob[d][s][0], nt, 1.0));
-            VD_TRY(fill_value(ctx, W->rob[d][s][1], nt, 0.0));
-            VD_TRY(fill_value(ctx, W->rob[d][s][2], nt, V));
+    // potential_bc (PlasmaChem.H:18-72): Dirichlet / Robin(1,0,V) value V,
+    // every side's arrays in one launch
+    {
+        double* fp[FILL_SIDES];
+        int64_t fn[FILL_SIDES];
+        double fv[FILL_SIDES];
+        int nf = 0;
+        for (int d = 0; d < g->dim; d++) {
+            if (W->per[d]) continue;
+            for (int s = 0; s < 2; s++) {
+                const double V = applied_potential(time, s ? 1 : -1, prm);
+                const int64_t nt = ntanf(n, d);
+                double* arrs[4] = {W->bcv[d][s], W->rob[d][s][0], W->rob[d][s][1], W->rob[d][s][2]};
+                const double vals[4] = {V, 1.0, 0.0, V};
+                for (int q = 0; q < 4; q++) {
+                    fp[nf] = arrs[q];
+                    fn[nf] = nt;
+                    fv[nf] = vals[q];
+                    nf++;
+                }
+            }
         }
+        // zero initial guess (PotentialSolve.cpp:162-168)
+        fp[nf] = W->x;
+        fn[nf] = nc;
+        fv[nf] = 0.0;
+        nf++;
+        VD_TRY(fill_values(ctx, fp, fn, fv, nf));
     }
     const double bconst = -1.0, alpha = 0.0, beta = 1.0;
     VD_TRY(side_setup(ctx, *W, prm->pot_bc_lo, prm->pot_bc_hi, beta, bconst));
-    VD_TRY(fill_value(ctx, W->x, nc, 0.0));
     int st = mg_iterate(ctx, *W, 1, alpha, beta, bconst, nullptr, prm->pot_bc_lo, prm->pot_bc_hi, false,
                         prm->linsolve_reltol, prm->linsolve_abstol, prm->linsolve_maxiter,
                         prm->mg_pre_smooth, prm->mg_post_smooth, info);

@@ -3,6 +3,8 @@
 //
 // Restates compute_flux (Source/compute_explicit_flux.H:200-326) with the
 // constant-mobility compute_vel of Source/PlasmaChem.H:186-206.
+#include <cfloat>
+
 #include "vd_internal.hpp"
 
 namespace vd {
@@ -20,6 +22,9 @@ __device__ __forceinline__ double gk(double wk, double g) {
 
 // weno (compute_explicit_flux.H:25-112); pow(x,2.0) evaluated as x*x, and
 // the normalised weights applied as (a1 p1 + a2 p2 + a3 p3) / (a1 + a2 + a3)
+// where that numerator is finite, else in the reference's order
+// (w_k = a_k / sum first, then sum w_k p_k) so that large alpha_k times
+// large p_k does not overflow
 template <int S>
 __device__ __forceinline__ double weno5(double im2, double im1, double i0, double ip1,
                                         double ip2) {
@@ -50,7 +55,9 @@ __device__ __forceinline__ double weno5(double im2, double im1, double i0, doubl
         a2 = g2 * (1.0 + sq(fdiv(tau, b2 + eps)));
         a3 = g3 * (1.0 + sq(fdiv(tau, b3 + eps)));
     }
-    return fdiv(a1 * p1 + a2 * p2 + a3 * p3, a1 + a2 + a3);
+    const double num = a1 * p1 + a2 * p2 + a3 * p3, den = a1 + a2 + a3;
+    if (__builtin_expect(fabs(num) <= DBL_MAX, 1)) return fdiv(num, den);
+    return fdiv(a1, den) * p1 + fdiv(a2, den) * p2 + fdiv(a3, den) * p3;
 }
 
 // Both WENO reconstructions centred on cell c from (a, b, c, d, e) =
@@ -118,17 +125,19 @@ __device__ __forceinline__ void weno5_lr(double a, double b, double c, double d,
         }
     }
     // both normalisations from one reciprocal of the product of the weight
-    // sums, or separately where that product leaves the normal range
+    // sums; where that product leaves the normal range or a numerator
+    // sum x_k p_k overflows, the weights are normalised first
+    // (w_k = x_k / sum, the reference's order)
     const double nl = x1 * p1 + x2 * p2 + x3 * p3, dl = x1 + x2 + x3;
     const double nr = y1 * q1 + y2 * q2 + y3 * q3, dr = y1 + y2 + y3;
     const double dlr = dl * dr;
-    if (__builtin_expect(dlr < 1e300 && dlr > 1e-300, 1)) {
+    if (__builtin_expect(dlr < 1e300 && dlr > 1e-300 && fabs(nl) <= DBL_MAX && fabs(nr) <= DBL_MAX, 1)) {
         const double ti = fdiv(1.0, dlr);
         L = nl * (dr * ti);
         R = nr * (dl * ti);
     } else {
-        L = fdiv(nl, dl);
-        R = fdiv(nr, dr);
+        L = fdiv(x1, dl) * p1 + fdiv(x2, dl) * p2 + fdiv(x3, dl) * p3;
+        R = fdiv(y1, dr) * q1 + fdiv(y2, dr) * q2 + fdiv(y3, dr) * q3;
     }
 }
 

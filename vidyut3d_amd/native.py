@@ -8,6 +8,7 @@ raise.
 """
 import ctypes as C
 import os
+import weakref
 
 import numpy as np
 
@@ -54,14 +55,26 @@ def check(st, what, allow=(abi.VD_OK,)):
 
 
 class Device:
-    """A vd_ctx (one HIP stream + workspaces) on one GPU."""
+    """A vd_ctx (one HIP stream + workspaces) on one GPU.
+
+    Arrays, mechanisms and sims made on a device register with it; close()
+    releases the ones still alive before it destroys the context, so that a
+    handle outliving its device (held by a traceback, say) frees nothing
+    through a destroyed context."""
 
     def __init__(self, device=0):
         self.ptr = C.c_void_p()
+        self._deps = weakref.WeakSet()
         check(lib().vd_ctx_create(int(device), C.byref(self.ptr)), "vd_ctx_create")
+
+    def adopt(self, obj):
+        self._deps.add(obj)
+        return obj
 
     def close(self):
         if self.ptr:
+            for o in list(self._deps):
+                o.release()
             lib().vd_ctx_destroy(self.ptr)
             self.ptr = C.c_void_p()
 
@@ -114,6 +127,7 @@ class DeviceArray:
         self.dtype = np.dtype(dtype)
         self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
         self.ptr = dev.alloc(self.nbytes)
+        dev.adopt(self)
         if host is not None:
             self.upload(host)
         else:
@@ -135,11 +149,14 @@ class DeviceArray:
     def tag_view(self):
         return abi.tag_view(self.ptr, self.begin, self.end)
 
+    def release(self):
+        if self.ptr and self.dev.ptr:
+            self.dev.free(self.ptr)
+        self.ptr = None
+
     def __del__(self):
         try:
-            if self.ptr:
-                self.dev.free(self.ptr)
-                self.ptr = None
+            self.release()
         except Exception:
             pass
 
@@ -149,6 +166,7 @@ class Mech:
         self.dev, self.desc = dev, desc
         self.ptr = C.c_void_p()
         check(lib().vd_mech_create(dev.ptr, C.byref(desc), C.byref(self.ptr)), "vd_mech_create")
+        dev.adopt(self)
 
     def find_id(self, name):
         return lib().vd_mech_find_id(self.ptr, name.encode())
@@ -162,11 +180,14 @@ class Mech:
     def specialised(self):
         return bool(lib().vd_mech_is_specialised(self.ptr))
 
+    def release(self):
+        if self.ptr and self.dev.ptr:
+            lib().vd_mech_destroy(self.ptr)
+        self.ptr = None
+
     def __del__(self):
         try:
-            if self.ptr:
-                lib().vd_mech_destroy(self.ptr)
-                self.ptr = None
+            self.release()
         except Exception:
             pass
 
@@ -289,6 +310,7 @@ class Sim:
         self.ptr = C.c_void_p()
         check(lib().vd_sim_create(dev.ptr, C.byref(case.mech), C.byref(case.geom),
                                   C.byref(case.params), C.byref(self.ptr)), "vd_sim_create")
+        dev.adopt(self)
         self.shape = case.state.shape
         self.set_state(case.state, 0.0)
 
@@ -340,9 +362,11 @@ class Sim:
         return out
 
     def close(self):
-        if self.ptr:
+        if self.ptr and self.dev.ptr:
             lib().vd_sim_destroy(self.ptr)
-            self.ptr = C.c_void_p()
+        self.ptr = C.c_void_p()
+
+    release = close
 
     def __del__(self):
         try:
