@@ -37,35 +37,52 @@ int fill_value(vd_ctx* ctx, double* p, int64_t n, double v) {
     return VD_OK;
 }
 
-// several constant fills in one launch (blockIdx.y = fill)
+// several constant fills in one launch: the blocks of fill q are
+// [first[q], first[q+1]) of a 1-D grid, FILL_CHUNK elements per block, so
+// the grid is as large as the data and no block is idle
 constexpr int FILL_BATCH = 32;
+constexpr int FILL_CHUNK = 256 * 8;
 struct FillBatch {
     double* p[FILL_BATCH];
     int64_t n[FILL_BATCH];
     double v[FILL_BATCH];
+    unsigned first[FILL_BATCH + 1];
+    int count;
 };
 
-__global__ void k_fill_batch(FillBatch b) {
-    const int q = blockIdx.y;
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < b.n[q]; i += stride) b.p[q][i] = b.v[q];
+__global__ void __launch_bounds__(256) k_fill_batch(FillBatch b) {
+    const unsigned blk = blockIdx.x;
+    int q = 0;
+    while (q + 1 < b.count && blk >= b.first[q + 1]) q++;
+    const int64_t base = (int64_t)(blk - b.first[q]) * FILL_CHUNK;
+    const int64_t end = base + FILL_CHUNK < b.n[q] ? base + FILL_CHUNK : b.n[q];
+    double* __restrict__ p = b.p[q];
+    const double v = b.v[q];
+    for (int64_t i = base + threadIdx.x; i < end; i += 256) p[i] = v;
 }
 
 int fill_values(vd_ctx* ctx, double* const* p, const int64_t* n, const double* v, int count) {
     for (int base = 0; base < count; base += FILL_BATCH) {
         FillBatch b;
         const int k = count - base < FILL_BATCH ? count - base : FILL_BATCH;
-        int64_t nmax = 0;
+        int64_t nb = 0;
+        b.count = 0;
         for (int q = 0; q < k; q++) {
-            b.p[q] = p[base + q];
-            b.n[q] = n[base + q];
-            b.v[q] = v[base + q];
-            if (n[base + q] > nmax) nmax = n[base + q];
+            if (n[base + q] <= 0) continue;
+            const int c = b.count++;
+            b.p[c] = p[base + q];
+            b.n[c] = n[base + q];
+            b.v[c] = v[base + q];
+            b.first[c] = (unsigned)nb;
+            nb += (n[base + q] + FILL_CHUNK - 1) / FILL_CHUNK;
         }
-        if (nmax <= 0) continue;
-        dim3 g = grid_stride(nmax, 256);
-        g.y = (unsigned)k;
-        hipLaunchKernelGGL(k_fill_batch, g, dim3(256), 0, ctx->stream, b);
+        if (nb == 0) continue;
+        if (nb > 0xffffffffLL / 256) {  // grid x extent times block size
+            set_error("fill_values: %lld blocks", (long long)nb);
+            return VD_EINVAL;
+        }
+        b.first[b.count] = (unsigned)nb;
+        hipLaunchKernelGGL(k_fill_batch, dim3((unsigned)nb), dim3(256), 0, ctx->stream, b);
     }
     VD_CHECK_LAUNCH();
     return VD_OK;

@@ -124,20 +124,22 @@ __device__ __forceinline__ void weno5_lr(double a, double b, double c, double d,
             y3 = gk(y3 * sy, g3);
         }
     }
-    // both normalisations from one reciprocal of the product of the weight
-    // sums; where that product leaves the normal range or a numerator
-    // sum x_k p_k overflows, the weights are normalised first
-    // (w_k = x_k / sum, the reference's order)
+    // Both normalisations from one reciprocal of the product of the weight
+    // sums, or separately where that product leaves the normal range.  The
+    // numerators sum x_k p_k are formed before the division (the reference
+    // divides the weights first): the same to a few ulps while they stay
+    // finite, which holds for |u| below ~1e50 (WENO-Z weights reach
+    // (tau / eps)^2 ~ |u|^4 / 1e-40 beside a flat stencil).
     const double nl = x1 * p1 + x2 * p2 + x3 * p3, dl = x1 + x2 + x3;
     const double nr = y1 * q1 + y2 * q2 + y3 * q3, dr = y1 + y2 + y3;
     const double dlr = dl * dr;
-    if (__builtin_expect(dlr < 1e300 && dlr > 1e-300 && fabs(nl) <= DBL_MAX && fabs(nr) <= DBL_MAX, 1)) {
+    if (__builtin_expect(dlr < 1e300 && dlr > 1e-300, 1)) {
         const double ti = fdiv(1.0, dlr);
         L = nl * (dr * ti);
         R = nr * (dl * ti);
     } else {
-        L = fdiv(x1, dl) * p1 + fdiv(x2, dl) * p2 + fdiv(x3, dl) * p3;
-        R = fdiv(y1, dr) * q1 + fdiv(y2, dr) * q2 + fdiv(y3, dr) * q3;
+        L = fdiv(nl, dl);
+        R = fdiv(nr, dr);
     }
 }
 
