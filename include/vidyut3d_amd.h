@@ -351,6 +351,47 @@ typedef struct vd_mlmg_problem {
 int vd_mlmg_solve(vd_ctx* ctx, const vd_mlmg_problem* prob, vd_array4* phi, const vd_array4* rhs,
                   const vd_box* box, const vd_geom* geom, vd_solve_info* info);
 
+/* ---------------------------------------------- level transfers (AMR)
+ * Level l+1 refines level l by ratio[d] (2 or 4; 1 in the unused third
+ * direction of a 2-D run).  Fine index i lies in coarse cell
+ * floor(i / ratio).  `dim` is 2 or 3. */
+
+/* Vidyut::AverageDown / AverageDownTo (Source/GridWork.cpp:94-108) ->
+ * amrex::average_down for one fine box: each coarse cell of crse inside
+ * coarsen(fbox) becomes volfrac * (sum of its ratio^dim children, k
+ * outermost, i innermost), volfrac = 1 / ratio^dim, components
+ * [scomp, scomp+ncomp) of fine into the same components of crse. */
+int vd_average_down(vd_ctx* ctx, const vd_array4* fine, const vd_box* fbox, vd_array4* crse, int scomp,
+                    int ncomp, const int ratio[3], int dim);
+
+/* average_down_faces of the species fluxes (Source/ScalarSolve.cpp:88-94)
+ * for one fine box: each coarse face of direction d inside the coarsened
+ * face box of fbox becomes the mean of the ratio^(dim-1) fine faces that
+ * tile it (outer loop over the higher tangential direction).  fine[d] and
+ * crse[d] are face arrays (1 component). */
+int vd_average_down_faces(vd_ctx* ctx, const vd_array4 fine[3], const vd_box* fbox, vd_array4 crse[3],
+                          const int ratio[3], int dim);
+
+/* Vidyut::FillPatch at lev > 0 (Source/GridWork.cpp:123-139,
+ * FillPatchTwoLevels with cell_cons_interp) and, with nfine = 0,
+ * Vidyut::FillCoarsePatch (GridWork.cpp:143-164, InterpFromCoarseLevel).
+ * Fills components [scomp, scomp+ncomp) of every cell of dst (a fine
+ * box with its ghost cells, fine index space):
+ *  - a cell inside the fine domain (periodic directions wrapped) that a
+ *    fine valid box fboxes[b] covers takes fine[b]'s value there;
+ *  - any other cell inside the domain takes the conservative linear
+ *    interpolation of the coarse level crse (the whole coarse domain,
+ *    valid data): MC-limited central slopes per direction, scaled by one
+ *    factor per coarse cell and component so that no child leaves the
+ *    min/max of the 3^dim coarse neighbourhood; coarse neighbours wrap
+ *    in periodic directions and are clamped (foextrap) at domain faces;
+ *  - a cell outside a non-periodic domain face takes the value of the
+ *    nearest cell inside (foextrap, Source/Vidyut.cpp:93-104).
+ * The fine boxes must not overlap. */
+int vd_fill_patch_two_levels(vd_ctx* ctx, const vd_array4* crse, const vd_geom* cgeom, const vd_array4* fine,
+                             const vd_box* fboxes, int nfine, vd_array4* dst, const vd_geom* fgeom,
+                             const int ratio[3], int scomp, int ncomp);
+
 /* ------------------------------------------------- kernel timing (bench)
  * When enabled for a kernel family, every launch of that family is
  * bracketed by hipEvents on the context's stream; the totals (time,

@@ -194,3 +194,37 @@ def evolve(case, nsteps=None, ncorr=1, time=0.0):
                          C.c_double(case.dt), int(ncorr), C.byref(t))
     _ck(st, "or_evolve", (abi.VD_OK, abi.VD_ENOCONV))
     return pn, t.value
+
+
+# ------------------------------------------------ level transfers (AMR)
+def _ratio(ratio):
+    return (C.c_int * 3)(*[int(x) for x in ratio])
+
+
+def average_down(fine, fbegin, fend, fbox, crse, cbegin, cend, scomp, ncomp, ratio, dim):
+    f = abi.host_array4(fine, fbegin, fend)
+    c = abi.host_array4(crse, cbegin, cend)
+    _ck(lib().or_average_down(C.byref(f), C.byref(fbox), C.byref(c), int(scomp), int(ncomp), _ratio(ratio),
+                              int(dim)), "or_average_down")
+    return crse
+
+
+def average_down_faces(fine, fext, fbox, crse, cext, ratio, dim):
+    """fine/crse: three face arrays; fext/cext: their (begin, end) pairs."""
+    fv = (abi.vd_array4 * 3)(*[abi.host_array4(a, *e) for a, e in zip(fine, fext)])
+    cv = (abi.vd_array4 * 3)(*[abi.host_array4(a, *e) for a, e in zip(crse, cext)])
+    _ck(lib().or_average_down_faces(fv, C.byref(fbox), cv, _ratio(ratio), int(dim)), "or_average_down_faces")
+    return crse
+
+
+def fill_patch_two_levels(crse, cext, cgeom, fines, fexts, fboxes, dst, dext, fgeom, ratio, scomp=0,
+                          ncomp=None):
+    ncomp = dst.shape[0] - scomp if ncomp is None else ncomp
+    nf = len(fboxes)
+    fv = (abi.vd_array4 * max(nf, 1))(*[abi.host_array4(a, *e) for a, e in zip(fines, fexts)])
+    bv = (abi.vd_box * max(nf, 1))(*fboxes)
+    cv = abi.host_array4(crse, *cext)
+    dv = abi.host_array4(dst, *dext)
+    _ck(lib().or_fill_patch_two_levels(C.byref(cv), C.byref(cgeom), fv, bv, nf, C.byref(dv), C.byref(fgeom),
+                                       _ratio(ratio), int(scomp), int(ncomp)), "or_fill_patch_two_levels")
+    return dst

@@ -1620,3 +1620,176 @@ int or_evolve(const vd_mech_desc* m, vd_array4* phi_new, vd_array4* phi_old, con
     }
     return st;
 }
+
+/* ------------------------------------------------------ level transfers
+ * Level l+1 refines level l by ratio r (2 or 4 per direction).  Fine cell
+ * i lies in coarse cell floor(i / r). */
+static int or_coarsen(int i, int r) { return i >= 0 ? i / r : -((-i - 1) / r) - 1; }
+
+/* amrex::average_down as called by Vidyut::AverageDown
+ * (Source/GridWork.cpp:94-108): coarse = (1/r^dim) * sum of the children,
+ * summed k-outer, i-inner. */
+int or_average_down(const vd_array4* fine, const vd_box* fbox, vd_array4* crse, int scomp, int ncomp,
+                    const int ratio[3], int dim) {
+    int r[3] = {ratio[0], ratio[1], dim == 3 ? ratio[2] : 1};
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; d++) {
+        lo[d] = d < dim ? or_coarsen(fbox->lo[d], r[d]) : fbox->lo[d];
+        hi[d] = d < dim ? or_coarsen(fbox->hi[d], r[d]) : fbox->hi[d];
+    }
+    double volfrac = 1.0 / (double)(r[0] * r[1] * r[2]);
+    for (int n = scomp; n < scomp + ncomp; n++)
+        for (int k = lo[2]; k <= hi[2]; k++)
+            for (int j = lo[1]; j <= hi[1]; j++)
+                for (int i = lo[0]; i <= hi[0]; i++) {
+                    double sum = 0.0;
+                    for (int kk = 0; kk < r[2]; kk++)
+                        for (int jj = 0; jj < r[1]; jj++)
+                            for (int ii = 0; ii < r[0]; ii++)
+                                sum += AT(fine, i * r[0] + ii, j * r[1] + jj, k * r[2] + kk, n);
+                    AT(crse, i, j, k, n) = volfrac * sum;
+                }
+    return VD_OK;
+}
+
+/* amrex::average_down_faces (Source/ScalarSolve.cpp:88-94): a coarse face
+ * of direction d is the mean of the r^(dim-1) fine faces tiling it, the
+ * higher tangential direction outermost. */
+int or_average_down_faces(const vd_array4 fine[3], const vd_box* fbox, vd_array4 crse[3], const int ratio[3],
+                          int dim) {
+    int r[3] = {ratio[0], ratio[1], dim == 3 ? ratio[2] : 1};
+    for (int d = 0; d < dim; d++) {
+        int lo[3], hi[3];
+        for (int e = 0; e < 3; e++) {
+            int fl = fbox->lo[e], fh = fbox->hi[e] + (e == d);
+            lo[e] = e < dim ? or_coarsen(fl, r[e]) : fl;
+            hi[e] = e < dim ? or_coarsen(fh, r[e]) : fh;
+        }
+        /* tangential directions t1 < t2 (t2 absent in 2-D) */
+        int t1 = d == 0 ? 1 : 0, t2 = d == 2 ? 1 : 2;
+        double inv = 1.0 / (double)(r[t1] * r[t2]);
+        for (int k = lo[2]; k <= hi[2]; k++)
+            for (int j = lo[1]; j <= hi[1]; j++)
+                for (int i = lo[0]; i <= hi[0]; i++) {
+                    double sum = 0.0;
+                    for (int b = 0; b < r[t2]; b++)
+                        for (int a = 0; a < r[t1]; a++) {
+                            int c[3] = {i * r[0], j * r[1], k * r[2]};
+                            c[t1] += a;
+                            c[t2] += b;
+                            sum += ATV(&fine[d], c, 0);
+                        }
+                    AT(&crse[d], i, j, k, 0) = inv * sum;
+                }
+    }
+    return VD_OK;
+}
+
+/* periodic wrap or clamp of index c into [lo, hi] */
+static int or_wrap_clamp(int c, int lo, int hi, int per) {
+    if (c >= lo && c <= hi) return c;
+    if (per) {
+        int n = hi - lo + 1, m = (c - lo) % n;
+        if (m < 0) m += n;
+        return lo + m;
+    }
+    return c < lo ? lo : hi;
+}
+
+/* cell_cons_interp (CellConservativeLinear without the cross-component
+ * linear limiting) of coarse cell ic at fine offsets xo: MC-limited
+ * central slopes, then a factor alpha <= 1 keeping every child within the
+ * min/max of the 3^dim coarse neighbourhood.  Coarse neighbours wrap in
+ * periodic directions and are clamped (foextrap) at domain faces. */
+static double or_cons_lin(const vd_array4* C, const vd_geom* cg, int n, const int ic[3], const double xo[3],
+                          const int r[3]) {
+    int dim = cg->dim;
+    double u0 = ATV(C, ic, n), s[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < dim; d++) {
+        int cp[3] = {ic[0], ic[1], ic[2]}, cm[3] = {ic[0], ic[1], ic[2]};
+        cp[d] = or_wrap_clamp(ic[d] + 1, cg->dom_lo[d], cg->dom_hi[d], cg->is_periodic[d]);
+        cm[d] = or_wrap_clamp(ic[d] - 1, cg->dom_lo[d], cg->dom_hi[d], cg->is_periodic[d]);
+        double up = ATV(C, cp, n), um = ATV(C, cm, n);
+        double cen = 0.5 * (up - um);
+        double forw = 2.0 * (up - u0);
+        double back = 2.0 * (u0 - um);
+        double lim = 0.0;
+        if (forw * back >= 0.0) lim = fabs(forw) < fabs(back) ? fabs(forw) : fabs(back);
+        double mag = lim < fabs(cen) ? lim : fabs(cen);
+        s[d] = (cen < 0.0 || (cen == 0.0 && signbit(cen)) ? -1.0 : 1.0) * mag;
+    }
+    double cmin = u0, cmax = u0;
+    int kr = dim == 3 ? 1 : 0;
+    for (int dk = -kr; dk <= kr; dk++)
+        for (int dj = -1; dj <= 1; dj++)
+            for (int di = -1; di <= 1; di++) {
+                int c[3] = {or_wrap_clamp(ic[0] + di, cg->dom_lo[0], cg->dom_hi[0], cg->is_periodic[0]),
+                            or_wrap_clamp(ic[1] + dj, cg->dom_lo[1], cg->dom_hi[1], cg->is_periodic[1]),
+                            dim == 3 ? or_wrap_clamp(ic[2] + dk, cg->dom_lo[2], cg->dom_hi[2], cg->is_periodic[2])
+                                     : ic[2]};
+                double v = ATV(C, c, n);
+                if (v < cmin) cmin = v;
+                if (v > cmax) cmax = v;
+            }
+    double dlo = 0.0, dhi = 0.0;
+    for (int c2 = 0; c2 < (dim == 3 ? r[2] : 1); c2++)
+        for (int c1 = 0; c1 < r[1]; c1++)
+            for (int c0 = 0; c0 < r[0]; c0++) {
+                double y = ((c0 + 0.5) / r[0] - 0.5) * s[0] + ((c1 + 0.5) / r[1] - 0.5) * s[1];
+                if (dim == 3) y = y + ((c2 + 0.5) / r[2] - 0.5) * s[2];
+                if (y < dlo) dlo = y;
+                if (y > dhi) dhi = y;
+            }
+    double alpha = 1.0;
+    if (dlo != 0.0 && u0 + dlo < cmin) {
+        double f = (cmin - u0) / dlo;
+        if (f < alpha) alpha = f;
+    }
+    if (dhi != 0.0 && u0 + dhi > cmax) {
+        double f = (cmax - u0) / dhi;
+        if (f < alpha) alpha = f;
+    }
+    double v = u0 + xo[0] * (alpha * s[0]);
+    v = v + xo[1] * (alpha * s[1]);
+    if (dim == 3) v = v + xo[2] * (alpha * s[2]);
+    return v;
+}
+
+/* FillPatchTwoLevels / InterpFromCoarseLevel (Source/GridWork.cpp:123-164):
+ * fine valid data where a fine box covers the (wrapped) cell, coarse
+ * interpolation elsewhere inside the domain, foextrap outside. */
+int or_fill_patch_two_levels(const vd_array4* crse, const vd_geom* cg, const vd_array4* fine,
+                             const vd_box* fboxes, int nfine, vd_array4* dst, const vd_geom* fg,
+                             const int ratio[3], int scomp, int ncomp) {
+    int dim = fg->dim;
+    int r[3] = {ratio[0], ratio[1], dim == 3 ? ratio[2] : 1};
+#pragma omp parallel for collapse(2)
+    for (int k = dst->begin[2]; k < dst->end[2]; k++)
+        for (int j = dst->begin[1]; j < dst->end[1]; j++)
+            for (int i = dst->begin[0]; i < dst->end[0]; i++) {
+                int c[3] = {i, j, k};
+                for (int d = 0; d < dim; d++)
+                    c[d] = or_wrap_clamp(c[d], fg->dom_lo[d], fg->dom_hi[d], fg->is_periodic[d]);
+                int b = 0;
+                for (; b < nfine; b++) {
+                    int in = 1;
+                    for (int d = 0; d < dim; d++)
+                        if (c[d] < fboxes[b].lo[d] || c[d] > fboxes[b].hi[d]) in = 0;
+                    if (in) break;
+                }
+                for (int n = scomp; n < scomp + ncomp; n++) {
+                    if (b < nfine) {
+                        AT(dst, i, j, k, n) = ATV(&fine[b], c, n);
+                    } else {
+                        int ic[3] = {c[0], c[1], c[2]};
+                        double xo[3] = {0.0, 0.0, 0.0};
+                        for (int d = 0; d < dim; d++) {
+                            ic[d] = or_coarsen(c[d], r[d]);
+                            xo[d] = (c[d] - ic[d] * r[d] + 0.5) / r[d] - 0.5;
+                        }
+                        AT(dst, i, j, k, n) = or_cons_lin(crse, cg, n, ic, xo, r);
+                    }
+                }
+            }
+    return VD_OK;
+}

@@ -53,6 +53,17 @@ int or_error_est(const vd_array4* state, vd_tagarray4* tags, const vd_box* box, 
 int or_mlmg_solve(const vd_mlmg_problem* prob, vd_array4* phi, const vd_array4* rhs,
                   const vd_box* box, const vd_geom* g, vd_solve_info* info);
 
+/* level transfers (AMR): Vidyut::AverageDown (Source/GridWork.cpp:94-108),
+ * the flux average_down_faces (Source/ScalarSolve.cpp:88-94) and
+ * Vidyut::FillPatch lev > 0 / FillCoarsePatch (GridWork.cpp:123-164) */
+int or_average_down(const vd_array4* fine, const vd_box* fbox, vd_array4* crse, int scomp, int ncomp,
+                    const int ratio[3], int dim);
+int or_average_down_faces(const vd_array4 fine[3], const vd_box* fbox, vd_array4 crse[3], const int ratio[3],
+                          int dim);
+int or_fill_patch_two_levels(const vd_array4* crse, const vd_geom* cg, const vd_array4* fine,
+                             const vd_box* fboxes, int nfine, vd_array4* dst, const vd_geom* fg,
+                             const int ratio[3], int scomp, int ncomp);
+
 /* Vidyut::Evolve (Source/Evolve.cpp:17-376) for `nsteps` steps at fixed dt
  * on a single-level, single-box domain; phi_new/phi_old have no ghosts.
  * ncorr = num_timestep_correctors.  time is advanced in place. */

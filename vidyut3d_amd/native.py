@@ -373,3 +373,37 @@ class Sim:
             self.close()
         except Exception:
             pass
+
+
+# ------------------------------------------------ level transfers (AMR)
+def _ratio(ratio):
+    return (C.c_int * 3)(*[int(x) for x in ratio])
+
+
+def average_down(dev, fine, fbox, crse, scomp, ncomp, ratio, dim):
+    """Vidyut::AverageDown (Source/GridWork.cpp:94-108) for one fine box:
+    the coarse cells under fbox become the mean of their children."""
+    f, c = fine.view(), crse.view()
+    check(lib().vd_average_down(dev.ptr, C.byref(f), C.byref(fbox), C.byref(c), int(scomp), int(ncomp),
+                                _ratio(ratio), int(dim)), "vd_average_down")
+
+
+def average_down_faces(dev, fine_faces, fbox, crse_faces, ratio, dim):
+    """average_down_faces of the species fluxes (Source/ScalarSolve.cpp:88-94)."""
+    fv = (abi.vd_array4 * 3)(*[a.view() for a in fine_faces])
+    cv = (abi.vd_array4 * 3)(*[a.view() for a in crse_faces])
+    check(lib().vd_average_down_faces(dev.ptr, fv, C.byref(fbox), cv, _ratio(ratio), int(dim)),
+          "vd_average_down_faces")
+
+
+def fill_patch_two_levels(dev, crse, cgeom, fines, fboxes, dst, fgeom, ratio, scomp=0, ncomp=None):
+    """Vidyut::FillPatch at lev > 0 (FillPatchTwoLevels, cell_cons_interp);
+    with no fine boxes, Vidyut::FillCoarsePatch (Source/GridWork.cpp:123-164)."""
+    ncomp = dst.ncomp - scomp if ncomp is None else ncomp
+    nf = len(fboxes)
+    fv = (abi.vd_array4 * max(nf, 1))(*[a.view() for a in fines])
+    bv = (abi.vd_box * max(nf, 1))(*fboxes)
+    cv, dv = crse.view(), dst.view()
+    check(lib().vd_fill_patch_two_levels(dev.ptr, C.byref(cv), C.byref(cgeom), fv, bv, nf, C.byref(dv),
+                                         C.byref(fgeom), _ratio(ratio), int(scomp), int(ncomp)),
+          "vd_fill_patch_two_levels")
