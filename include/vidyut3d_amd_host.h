@@ -43,6 +43,33 @@ int vd_sim_step(const vd_sim* s);
 int vd_sim_error_est(vd_sim* s, int ntag, const int* comps, const double* thr, const double* gthr,
                      char* host_tags);
 
+/* ------------------------------------------------------------- regrid
+ * AmrMesh::MakeNewGrids for the level above a single-box level, as
+ * Vidyut::Evolve's regrid(0, time) runs it every regrid_int steps
+ * (Source/Evolve.cpp:66-72) on the tags of Vidyut::ErrorEst
+ * (Source/Vidyut.cpp:171-255), with the amr.* keys below.  Host code: no
+ * device work. */
+typedef struct vd_regrid_params {
+    int ref_ratio[3];       /* amr.ref_ratio: 2 or 4 */
+    int blocking_factor[3]; /* amr.blocking_factor of the new level (multiple of ref_ratio) */
+    int max_grid_size[3];   /* amr.max_grid_size of the new level (multiple of blocking_factor) */
+    int n_error_buf;        /* amr.n_error_buf: cells of buffer around each tag (AMReX default 1) */
+    double grid_eff;        /* amr.grid_eff: minimum tagged fraction of a box (AMReX default 0.7) */
+} vd_regrid_params;
+
+/* tags: host char array over `domain` (level lev, i fastest), nonzero =
+ * tagged.  The tags are buffered by n_error_buf cells (periodic
+ * directions wrap, cells beyond other faces are dropped), coarsened by
+ * blocking_factor / ref_ratio, clustered (Berger-Rigoutsos: a box is kept
+ * once its tagged fraction reaches grid_eff, else cut at a hole of the
+ * tag signature, else at the strongest inflection of its Laplacian, else
+ * in the middle of its longest side), refined to the new level and
+ * chopped to max_grid_size.  Writes at most max_out boxes (fine index
+ * space) to out and their number to *nout (VD_EINVAL, with the count
+ * needed in *nout, when out is too small). */
+int vd_make_new_grids(const char* tags, const vd_box* domain, const int is_periodic[3], int dim,
+                      const vd_regrid_params* prm, vd_box* out, int max_out, int* nout);
+
 #ifdef __cplusplus
 }
 #endif

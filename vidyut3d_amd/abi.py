@@ -129,6 +129,11 @@ class vd_mlmg_problem(C.Structure):
                 ("max_coarsening_level", C.c_int)]
 
 
+class vd_regrid_params(C.Structure):
+    _fields_ = [("ref_ratio", C.c_int * 3), ("blocking_factor", C.c_int * 3),
+                ("max_grid_size", C.c_int * 3), ("n_error_buf", C.c_int), ("grid_eff", C.c_double)]
+
+
 def ivec(v):
     return (C.c_int * 3)(*[int(x) for x in v])
 

@@ -407,3 +407,37 @@ def fill_patch_two_levels(dev, crse, cgeom, fines, fboxes, dst, fgeom, ratio, sc
     check(lib().vd_fill_patch_two_levels(dev.ptr, C.byref(cv), C.byref(cgeom), fv, bv, nf, C.byref(dv),
                                          C.byref(fgeom), _ratio(ratio), int(scomp), int(ncomp)),
           "vd_fill_patch_two_levels")
+
+
+# ------------------------------------------------------------- regrid
+def regrid_params(ref_ratio=2, blocking_factor=2, max_grid_size=32, n_error_buf=1, grid_eff=0.7):
+    """amr.* keys of the new level (scalars apply to every direction)."""
+    p = abi.vd_regrid_params()
+    for name, v in (("ref_ratio", ref_ratio), ("blocking_factor", blocking_factor),
+                    ("max_grid_size", max_grid_size)):
+        v = [v] * 3 if np.isscalar(v) else list(v) + [1] * (3 - len(v))
+        getattr(p, name)[:] = [int(x) for x in v]
+    p.n_error_buf = int(n_error_buf)
+    p.grid_eff = float(grid_eff)
+    return p
+
+
+def make_new_grids(tags, geom, prm):
+    """AmrMesh::MakeNewGrids on the tags of a single-box level (host code):
+    tags is an int8 array (nz, ny, nx) over the domain; returns the new
+    level's boxes as a list of vd_box (fine index space)."""
+    tags = np.ascontiguousarray(tags, dtype=np.int8)
+    dom = abi.domain_box(geom)
+    per = (C.c_int * 3)(*[int(geom.is_periodic[d]) for d in range(3)])
+    n = C.c_int()
+    cap = 64
+    while True:
+        out = (abi.vd_box * cap)()
+        st = lib().vd_make_new_grids(tags.ctypes.data_as(C.c_char_p), C.byref(dom), per, int(geom.dim),
+                                     C.byref(prm), out, cap, C.byref(n))
+        if st == abi.VD_OK:
+            return [out[i] for i in range(n.value)]
+        if n.value > cap:
+            cap = n.value
+            continue
+        check(st, "vd_make_new_grids")
