@@ -2,6 +2,7 @@
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 
 #include "vd_internal.hpp"
 
@@ -14,6 +15,50 @@ void set_error(const char* fmt, ...) {
     va_start(ap, fmt);
     vsnprintf(g_err, sizeof(g_err), fmt, ap);
     va_end(ap);
+}
+
+// Workgroups of `kernel` (block threads) one CU keeps resident, times the
+// CU count: the device's capacity for that launch shape.  Cached per kernel
+// and device (contexts on several host threads share the cache).
+int resident_blocks(vd_ctx* ctx, const void* kernel, int block) {
+    static std::mutex mu;
+    static std::map<std::pair<const void*, int>, int> cache;
+    const std::pair<const void*, int> key(kernel, ctx->device * 4096 + block);
+    {
+        std::lock_guard<std::mutex> g(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    int per_cu = 0, ncu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, block, 0) != hipSuccess || per_cu <= 0)
+        per_cu = 1;
+    if (hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, ctx->device) != hipSuccess || ncu <= 0)
+        ncu = 256;
+    const int cap = per_cu * ncu;
+    std::lock_guard<std::mutex> g(mu);
+    cache[key] = cap;
+    return cap;
+}
+
+// The chunk count n in [nmin, nmax] whose launch of n * per_chunk blocks
+// fills the device's resident capacity `cap` best over whole rounds (a
+// last round that is mostly empty leaves most CUs idle while the long
+// streaming workgroups of the round before finish); the fewest chunks
+// among equally good counts.
+int round_filling_chunks(int64_t per_chunk, int cap, int nmin, int nmax) {
+    if (nmax < nmin) nmax = nmin;
+    int best = nmin;
+    double beff = -1.0;
+    for (int n = nmin; n <= nmax; n++) {
+        const int64_t b = per_chunk * n;
+        const int64_t rounds = (b + cap - 1) / cap;
+        const double eff = (double)b / (double)(rounds * cap);
+        if (eff > beff + 1e-9) {
+            beff = eff;
+            best = n;
+        }
+    }
+    return best;
 }
 
 int scratch(vd_ctx* ctx, const std::string& name, size_t bytes, void** out) {

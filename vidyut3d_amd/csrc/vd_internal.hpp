@@ -178,6 +178,12 @@ namespace vd {
 // returns a device buffer of at least `bytes` registered under `name`
 int scratch(vd_ctx* ctx, const std::string& name, size_t bytes, void** out);
 
+// resident workgroups of `kernel` (block threads) on the whole device
+int resident_blocks(vd_ctx* ctx, const void* kernel, int block);
+// chunk count in [nmin, nmax] that best fills whole rounds of `cap`
+// resident workgroups with per_chunk workgroups per chunk
+int round_filling_chunks(int64_t per_chunk, int cap, int nmin, int nmax);
+
 // kernel-timing brackets (no-ops unless the family is selected)
 void kt_begin(vd_ctx* ctx, int family);
 void kt_end(vd_ctx* ctx, int family, double bytes);

@@ -3,6 +3,7 @@
 //
 // Restates compute_flux (Source/compute_explicit_flux.H:200-326) with the
 // constant-mobility compute_vel of Source/PlasmaChem.H:186-206.
+#include <algorithm>
 #include <cfloat>
 
 #include "vd_internal.hpp"
@@ -233,7 +234,7 @@ struct PencilArgs {
     int lo_zero, hi_zero;  // face index zeroed (IHNEU) or INT_MIN
     int3 flo;              // first face index (box lo)
     int nx, no, nd;        // cells along x, the other direction, and D
-    int nchunk, chunk;     // face chunks along D
+    int nchunk;            // face chunks along D
     // divergence fused into the last direction (fuse = 1): dsdt of cell m
     // from the x (and y) face arrays, rxn and this pencil's own two faces,
     // in the order of k_dsdt; the D-direction faces are then not stored
@@ -249,8 +250,9 @@ __global__ void __launch_bounds__(256) k_pencil_flux(PencilArgs a) {
     const int i = (int)(t % a.nx);
     const unsigned r = t / a.nx;
     const int o = (int)(r % a.no), ch = (int)(r / a.no);
-    const int f0 = ch * a.chunk;
-    const int f1 = ch == a.nchunk - 1 ? a.nd + 1 : f0 + a.chunk;  // faces [f0, f1)
+    // faces [f0, f1): the nd + 1 faces split as evenly as the chunk count allows
+    const int f0 = (int)((int64_t)ch * (a.nd + 1) / a.nchunk);
+    const int f1 = (int)((int64_t)(ch + 1) * (a.nd + 1) / a.nchunk);
     // cell (i, o, m) along D -> Array4 offset
     const int ig = a.flo.x + i;
     const int jg = D == 1 ? 0 : a.flo.y + o;
@@ -434,10 +436,18 @@ static int launch_pencil(vd_ctx* ctx, const FluxArgs& f, int d, int scheme, cons
     a.nd = box->hi[d] - box->lo[d] + 1;
     a.no = d == 1 ? box->hi[2] - box->lo[2] + 1 : box->hi[1] - box->lo[1] + 1;
     const int64_t cols = (int64_t)a.nx * a.no;
-    int chunk = 32;
-    while (chunk > 8 && cols * ((a.nd + 1) / chunk) < 256 * 64 * 8) chunk /= 2;
-    a.nchunk = (a.nd + 1) / chunk > 0 ? (a.nd + 1) / chunk : 1;
-    a.chunk = chunk;
+    // chunks of 12-48 faces, as many as fill whole rounds of resident
+    // workgroups (each thread streams its chunk, so a nearly empty last
+    // round would run alone)
+    const void* kfn = d == 1 ? (scheme == 1 ? (const void*)k_pencil_flux<1, 1>
+                                            : (scheme == 2 ? (const void*)k_pencil_flux<2, 1>
+                                                           : (const void*)k_pencil_flux<3, 1>))
+                             : (scheme == 1 ? (const void*)k_pencil_flux<1, 2>
+                                            : (scheme == 2 ? (const void*)k_pencil_flux<2, 2>
+                                                           : (const void*)k_pencil_flux<3, 2>));
+    const int nf = a.nd + 1;
+    const int nmin = std::max(1, (nf + 47) / 48), nmax = std::max(nmin, nf / 12);
+    a.nchunk = round_filling_chunks((cols + 255) / 256, resident_blocks(ctx, kfn, 256), nmin, nmax);
     const int64_t tot = cols * a.nchunk;
     if (tot >= ((int64_t)1 << 32)) {
         set_error("transport: box too large for one pencil launch");
