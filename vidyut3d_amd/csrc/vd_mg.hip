@@ -964,6 +964,23 @@ static int resid_zc(const int n[3], int ntile) {
     return zc;
 }
 
+// z-chunk of a streamed launch of `kern`: the power-of-two rule, unless
+// on a level large enough to fill the device its last round of resident
+// workgroups would be mostly empty (each workgroup streams its chunk, so
+// that round would run nearly alone); then the chunk count (4..48 planes
+// per chunk) that best fills whole rounds
+static int stream_zc(vd_ctx* ctx, const void* kern, int block, const int n[3], int ntile) {
+    const int zc0 = resid_zc(n, ntile);
+    const int64_t b0 = (int64_t)ntile * ((n[2] + zc0 - 1) / zc0);
+    if (b0 < 1024) return zc0;
+    const int cap = resident_blocks(ctx, kern, block);
+    if ((double)b0 / (double)(((b0 + cap - 1) / cap) * cap) >= 0.75) return zc0;
+    const int nz = n[2];
+    const int nmin = std::max(1, (nz + 47) / 48), nmax = std::max(nmin, nz / 4);
+    const int nc = round_filling_chunks(ntile, cap, nmin, nmax);
+    return (nz + nc - 1) / nc;
+}
+
 // max of n per-block partials into slot 0 of a slot group (one workgroup)
 __global__ void __launch_bounds__(1024) k_reduce_partials(const double* __restrict__ part, int n,
                                                           unsigned long long* __restrict__ slots) {
@@ -1938,11 +1955,13 @@ static void launch_gsrb_bm(vd_ctx* ctx, const MGWork& W, const LevV<T>& v, T* x,
     if (streamed_sweep(W, v.nx)) {
         const int ntx = (v.nx / 2 + GZ_X - 1) / GZ_X, nty = (v.ny + GZ_Y - 1) / GZ_Y;
         const int n3[3] = {v.nx, v.ny, v.nz};
-        const int zc = resid_zc(n3, ntx * nty);
-        const dim3 g((unsigned)(ntx * nty * ((v.nz + zc - 1) / zc)));
 #define VD_GZ(C0UV, ZIV)                                                                                      \
-    hipLaunchKernelGGL((k_gsrb_z<T, BM, C0UV, ZIV>), g, dim3(GZ_X * GZ_Y), 0, ctx->stream, v, x, rhs, color, zc, \
-                       ntx, nty)
+    do {                                                                                                      \
+        const int zc = stream_zc(ctx, (const void*)&k_gsrb_z<T, BM, C0UV, ZIV>, GZ_X * GZ_Y, n3, ntx * nty);   \
+        const dim3 g((unsigned)(ntx * nty * ((v.nz + zc - 1) / zc)));                                        \
+        hipLaunchKernelGGL((k_gsrb_z<T, BM, C0UV, ZIV>), g, dim3(GZ_X * GZ_Y), 0, ctx->stream, v, x, rhs, color, \
+                           zc, ntx, nty);                                                                     \
+    } while (0)
         if (v.c0uni) {
             if (zi) VD_GZ(true, true); else VD_GZ(true, false);
         } else {
@@ -1979,7 +1998,9 @@ static void launch_resid_restrict(vd_ctx* ctx, const MGWork& W, const LevV<T>& v
     const int dim = W.dim;
     if (dim == 3 && W.stream) {
         const int ntx = (Cc.n[0] + RR_X - 1) / RR_X, nty = (Cc.n[1] + RR_Y - 1) / RR_Y;
-        const int zc = resid_zc(Cc.n, ntx * nty);
+        const void* kfn = v.c0uni ? (const void*)&k_resid_restrict_z<T, BM, true>
+                                  : (const void*)&k_resid_restrict_z<T, BM, false>;
+        const int zc = stream_zc(ctx, kfn, RR_X * RR_Y, Cc.n, ntx * nty);
         const dim3 g((unsigned)(ntx * nty * ((Cc.n[2] + zc - 1) / zc)));
         if (v.c0uni)
             hipLaunchKernelGGL((k_resid_restrict_z<T, BM, true>), g, dim3(RR_X * RR_Y), 0, ctx->stream, v, F.e, F.r,
@@ -2085,7 +2106,7 @@ static bool vcycle(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L,
         LevV<T> v = view<T>(W, F, alpha, beta, bconst);
         if (W.dim == 3 && vc.c0uni && W.stream) {
             const int ntx = (F.n[0] + PZ_X - 1) / PZ_X, nty = (F.n[1] + PZ_Y - 1) / PZ_Y;
-            const int zc = resid_zc(Cc.n, ntx * nty);
+            const int zc = stream_zc(ctx, (const void*)&k_prolong_z<T>, PZ_X * PZ_Y, Cc.n, ntx * nty);
             const dim3 g((unsigned)(ntx * nty * ((Cc.n[2] + zc - 1) / zc)));
             hipLaunchKernelGGL((k_prolong_z<T>), g, dim3(PZ_X * PZ_Y), 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.e,
                                zc, ntx, nty);
@@ -2229,10 +2250,13 @@ static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double
     kt_begin(ctx, VD_KT_RESID);
     if (W.dim == 3 && W.stream) {
         const int ntx = (W.n[0] + RZ_X - 1) / RZ_X, nty = (W.n[1] + RZ_Y - 1) / RZ_Y;
-        const int zc = resid_zc(W.n, ntx * nty);
-        const dim3 gz((unsigned)(ntx * nty * ((W.n[2] + zc - 1) / zc)));
 #define VD_RESIDZ3(BMV, BNV, FMV)                                                                             \
     {                                                                                                         \
+        const int zc = stream_zc(ctx,                                                                         \
+                                 v0.c0uni ? (const void*)&k_resid_z<BMV, BNV, FMV, true>                      \
+                                          : (const void*)&k_resid_z<BMV, BNV, FMV, false>,                    \
+                                 RZ_X * RZ_Y, W.n, ntx * nty);                                                \
+        const dim3 gz((unsigned)(ntx * nty * ((W.n[2] + zc - 1) / zc)));                                      \
         if (v0.c0uni)                                                                                         \
             hipLaunchKernelGGL((k_resid_z<BMV, BNV, FMV, true>), gz, dim3(RZ_X * RZ_Y), 0, ctx->stream, v0,   \
                                W.x, W.f, res, zc, ntx, nty, sl);                                              \
