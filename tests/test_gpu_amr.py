@@ -104,3 +104,32 @@ def test_fill_patch_two_levels_rejects_mismatched_domains(dev):
         native.fill_patch_two_levels(dev, dc, cg, [], [], dd, bad, [2, 2, 1])
     with pytest.raises(native.VidyutError):
         native.fill_patch_two_levels(dev, dc, cg, [], [], dd, fg, [3, 2, 1])
+
+
+def test_level_transfers_reject_bad_boxes_and_components(dev):
+    """Fine boxes not aligned to the refinement ratio, components beyond an
+    array and arrays that do not cover the coarsened box are errors, not
+    launches."""
+    cbox = abi.make_box([0, 0, 0], [7, 7, 7])
+    cb, ce = abi.grown_extent(cbox, 0, 3)
+    dc = native.DeviceArray(dev, cb, ce, 2)
+    fbox = abi.make_box([2, 2, 2], [9, 9, 9])
+    fb, fe = abi.grown_extent(fbox, 0, 3)
+    df = native.DeviceArray(dev, fb, fe, 2)
+    native.average_down(dev, df, fbox, dc, 0, 2, [2, 2, 2], 3)  # aligned: fine
+    with pytest.raises(native.VidyutError):  # lo not a multiple of 4
+        native.average_down(dev, df, fbox, dc, 0, 2, [4, 4, 4], 3)
+    with pytest.raises(native.VidyutError):  # components beyond the arrays
+        native.average_down(dev, df, fbox, dc, 1, 2, [2, 2, 2], 3)
+    odd = abi.make_box([3, 2, 2], [9, 9, 9])
+    with pytest.raises(native.VidyutError):
+        native.average_down(dev, df, odd, dc, 0, 1, [2, 2, 2], 3)
+    small = native.DeviceArray(dev, [0, 0, 0], [3, 3, 3], 2)  # does not hold coarse cells 1..4
+    with pytest.raises(native.VidyutError):
+        native.average_down(dev, df, fbox, small, 0, 1, [2, 2, 2], 3)
+    fext = [abi.grown_extent(fbox, 0, 3, d) for d in range(3)]
+    ffs = [native.DeviceArray(dev, *e, 1) for e in fext]
+    cfs = [native.DeviceArray(dev, *abi.grown_extent(cbox, 0, 3, d), 1) for d in range(3)]
+    native.average_down_faces(dev, ffs, fbox, cfs, [2, 2, 2], 3)
+    with pytest.raises(native.VidyutError):
+        native.average_down_faces(dev, ffs, fbox, cfs, [3, 2, 2], 3)
