@@ -2200,6 +2200,19 @@ __global__ void __launch_bounds__(64) k_norm_out(unsigned long long* __restrict_
     }
 }
 
+// max |a| over n values into a slot group (grid-stride, 2 values per
+// thread and step)
+__global__ void __launch_bounds__(256) k_absmax(const double* __restrict__ a, int64_t n,
+                                                unsigned long long* __restrict__ slots) {
+    double m = 0.0;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x * 2;
+    for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2; i < n; i += stride) {
+        m = fmax(m, fabs(a[i]));
+        if (i + 1 < n) m = fmax(m, fabs(a[i + 1]));
+    }
+    block_max_atomic(m, slots);
+}
+
 // the norms of up to 5 slot groups on the host: the host waits for the
 // sequence number k_norm_out writes last, rather than for a copy and a
 // stream synchronisation, and checks the stream for errors while it waits
@@ -2375,9 +2388,13 @@ static int jacobi_rho(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
 // coefficient) decided from the Jacobi bound of the face maxima that the
 // first residual pass also returns; when V-cycles follow, the face arrays
 // are built and the solve continues on them (bm 0).
+// x_zero: the solution array holds zeros on entry (a solve from a zero
+// initial guess), so the initial residual is the rhs itself: its norm needs
+// one read of f, and the first V-cycle takes f as its finest rhs
 static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta, double bconst,
                       const double* dcell, const int tlo[3], const int thi[3], int smooth_mode,
-                      double reltol, double abstol, int maxiter, int nu1, int nu2, vd_solve_info* info) {
+                      double reltol, double abstol, int maxiter, int nu1, int nu2, vd_solve_info* info,
+                      bool x_zero = false) {
     int64_t nc = ncell(W.n);
     W.c0uni = 1;
     for (int d = 0; d < 3; d++)
@@ -2454,7 +2471,20 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
         W.c0_valid = !robin;
         VD_CHECK_LAUNCH();
     }
-    if (smooth_mode != 2) VD_TRY(resid0(ctx, W, bm, v0, &r0, &bnorm, !smooth_only));
+    // the first V-cycle's finest rhs: the stored residual, or f when x is 0
+    bool rhs_is_f = false;
+    if (smooth_mode != 2) {
+        if (x_zero && !smooth_only) {
+            // r = f - L 0 = f bit for bit (every stencil term is 0)
+            hipLaunchKernelGGL(k_absmax, dim3(NORM_BLOCKS), dim3(256), 0, ctx->stream, W.f, nc, slot_group(W, 0));
+            VD_CHECK_LAUNCH();
+            VD_TRY(read_groups(ctx, W, 1, &bnorm));
+            r0 = bnorm;
+            rhs_is_f = true;
+        } else {
+            VD_TRY(resid0(ctx, W, bm, v0, &r0, &bnorm, !smooth_only));
+        }
+    }
     double target = std::max(abstol, reltol * std::max(bnorm, r0));
     rn = r0;
     int it = 0, st = VD_OK;
@@ -2476,17 +2506,21 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
                 launch_gsrb<double>(ctx, W, bm, v0, W.x, W.f, 0, true);
                 launch_gsrb<double>(ctx, W, bm, v0, W.x, W.f, 1, true);
             } else if (W.prec == 0) {
-                if (!vcycle<double>(ctx, W, bm, W.L64, alpha, beta, bconst, nu1, nu2, W.x))
+                if (rhs_is_f) W.L64[0].r = W.f;  // read only by the V-cycle
+                const bool acc = vcycle<double>(ctx, W, bm, W.L64, alpha, beta, bconst, nu1, nu2, W.x);
+                W.L64[0].r = W.res;
+                if (!acc)
                     hipLaunchKernelGGL(k_axpy_to_double<double>, grid1d(nc, 256), dim3(256), 0, ctx->stream,
                                        W.x, W.L64[0].e, nc);
             } else {
                 hipLaunchKernelGGL((k_convert<double, float>), grid1d(nc, 256), dim3(256), 0, ctx->stream,
-                                   W.res, W.L32[0].r, nc);
+                                   rhs_is_f ? W.f : W.res, W.L32[0].r, nc);
                 if (!vcycle<float>(ctx, W, bm, W.L32, alpha, beta, bconst, nu1, nu2, W.x))
                     hipLaunchKernelGGL(k_axpy_to_double<float>, grid1d(nc, 256), dim3(256), 0, ctx->stream,
                                        W.x, W.L32[0].e, nc);
             }
             VD_CHECK_LAUNCH();
+            rhs_is_f = false;
             VD_TRY(resid0(ctx, W, bm, v0, &rn, nullptr, !smooth_only));
             if (rn <= target) break;
         }
@@ -2843,9 +2877,10 @@ int solve_potential(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array
     }
     const double bconst = -1.0, alpha = 0.0, beta = 1.0;
     VD_TRY(side_setup(ctx, *W, prm->pot_bc_lo, prm->pot_bc_hi, beta, bconst));
+    // W->x was set to zero above (the batched fill)
     int st = mg_iterate(ctx, *W, 1, alpha, beta, bconst, nullptr, prm->pot_bc_lo, prm->pot_bc_hi, false,
                         prm->linsolve_reltol, prm->linsolve_abstol, prm->linsolve_maxiter,
-                        prm->mg_pre_smooth, prm->mg_post_smooth, info);
+                        prm->mg_pre_smooth, prm->mg_post_smooth, info, true);
     if (st != VD_OK && st != VD_ENOCONV) return st;
     VD_TRY(grad_faces(ctx, *W, prm->pot_bc_lo, prm->pot_bc_hi, ef, box, -1.0));
     PotPostArgs pa;
