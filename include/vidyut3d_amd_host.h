@@ -70,6 +70,22 @@ typedef struct vd_regrid_params {
 int vd_make_new_grids(const char* tags, const vd_box* domain, const int is_periodic[3], int dim,
                       const vd_regrid_params* prm, vd_box* out, int max_out, int* nout);
 
+/* AMR level 1 of a simulation (the step driver advances level 0 only).
+ * Vidyut::regrid(0, time) (Source/Evolve.cpp:66-72): ErrorEst on level 0
+ * with these tagging rules, vd_make_new_grids, then
+ * MakeNewLevelFromCoarse (Source/GridWork.cpp:17-29) for a new level 1,
+ * RemakeLevel (:33-49) for an existing one, or ClearLevel (:53-57) when no
+ * cell is tagged.  *nboxes = the number of level-1 boxes afterwards. */
+int vd_sim_regrid(vd_sim* s, int ntag, const int* comps, const double* thr, const double* gthr,
+                  const vd_regrid_params* rp, int* nboxes);
+/* the boxes of level lev (0: the domain; 1: the level-1 BoxArray) */
+int vd_sim_level_boxes(vd_sim* s, int lev, vd_box* out, int max_out, int* n);
+/* phi_new of level-1 box b (NVAR comps over the box, component-major) */
+int vd_sim_get_fine(vd_sim* s, int b, double* host);
+int vd_sim_set_fine(vd_sim* s, int b, const double* host);
+/* Vidyut::AverageDown (Source/GridWork.cpp:94-101) */
+int vd_sim_average_down(vd_sim* s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -133,3 +133,60 @@ def test_level_transfers_reject_bad_boxes_and_components(dev):
     native.average_down_faces(dev, ffs, fbox, cfs, [2, 2, 2], 3)
     with pytest.raises(native.VidyutError):
         native.average_down_faces(dev, ffs, fbox, cfs, [3, 2, 2], 3)
+
+
+def test_sim_regrid_cycle_make_remake_average_clear(dev):
+    """The level-1 cycle of a simulation: regrid builds level 1 on the
+    grids make_new_grids gives for the level-0 tags, filled by
+    FillCoarsePatch (bit for bit the oracle's interpolation); AverageDown
+    then returns level 0 to itself (conservative interpolation, to
+    rounding); a second regrid on other tags remakes level 1 from its old
+    data and the coarse level (oracle FillPatchTwoLevels with the old
+    boxes); with no tags the level is cleared."""
+    from vidyut3d_amd import cases
+    c = cases.discharge_2d(n=32, seed=1, nsteps=1)
+    sim = native.Sim(dev, c)
+    E = c.mech.e_idx
+    st = sim.get_state()
+    ratio = [2, 2, 1]
+    prm = native.regrid_params(ref_ratio=2, blocking_factor=4, max_grid_size=16, n_error_buf=1)
+    thr = 0.3 * st[E].max()
+    boxes = sim.regrid([E], [thr], [1e300], prm)
+    assert boxes
+    tags = sim.error_est([E], [thr], [1e300])
+    expect = native.make_new_grids(tags, c.geom, prm)
+    assert [(tuple(b.lo), tuple(b.hi)) for b in boxes] == [(tuple(b.lo), tuple(b.hi)) for b in expect]
+    cg = c.geom
+    fg = abi.make_geom([2 * (cg.dom_hi[d] + 1) for d in range(2)], [cg.prob_lo[0], cg.prob_lo[1]],
+                       [cg.prob_hi[0], cg.prob_hi[1]], [cg.is_periodic[0], cg.is_periodic[1]])
+    cext = abi.grown_extent(abi.domain_box(cg), 0, 2)
+    for b, box in enumerate(boxes):
+        ext = abi.grown_extent(box, 0, 2)
+        ref = O.fill_patch_two_levels(st, cext, cg, [], [], [], np.zeros(abi.shape_of(*ext, st.shape[0])), ext, fg,
+                                      ratio)
+        assert np.array_equal(sim.get_fine(b, box), ref), b
+    # AverageDown of interpolated data gives level 0 back
+    sim.average_down()
+    st2 = sim.get_state()
+    scale = np.max(np.abs(st), axis=(1, 2, 3), keepdims=True) + 1e-300
+    assert np.max(np.abs(st2 - st) / scale) < 1e-14
+    # new fine data, other tags: RemakeLevel
+    rng = np.random.default_rng(4)
+    fines = []
+    for b, box in enumerate(boxes):
+        f = sim.get_fine(b, box) * (1.0 + 0.1 * rng.uniform(size=sim.fine_shape(box)))
+        sim.set_fine(b, f)
+        fines.append(f)
+    thr2 = 0.1 * st2[E].max()
+    boxes2 = sim.regrid([E], [thr2], [1e300], prm)
+    assert len(boxes2) >= 1 and boxes2 != boxes
+    st3 = sim.get_state()
+    fexts = [abi.grown_extent(b, 0, 2) for b in boxes]
+    for b, box in enumerate(boxes2):
+        ext = abi.grown_extent(box, 0, 2)
+        ref = O.fill_patch_two_levels(st3, cext, cg, fines, fexts, boxes, np.zeros(abi.shape_of(*ext, st.shape[0])),
+                                      ext, fg, ratio)
+        assert np.array_equal(sim.get_fine(b, box), ref), b
+    # nothing tagged: level 1 cleared
+    assert sim.regrid([E], [1e300], [1e300], prm) == []
+    sim.close()

@@ -8,7 +8,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/vidyut3d_amd.h"
+#include "../../include/vidyut3d_amd_host.h"
 
 namespace vidyut_amd {
 
@@ -86,6 +86,23 @@ class Vidyut {
     int set_tagging(const std::vector<int>& comps, const std::vector<double>& thr,
                     const std::vector<double>& gthr);
 
+    // ---- AMR level 1 (GridWork.cpp:17-164): the regrid cycle and the
+    // level transfers; the step driver advances level 0 only ---------------
+    // regrid(0, time) as Evolve.cpp:66-72 runs it: ErrorEst on level 0,
+    // MakeNewGrids (vd_make_new_grids), then MakeNewLevelFromCoarse for a
+    // new level, RemakeLevel for an existing one, ClearLevel when no cell
+    // is tagged
+    int regrid(int lbase, Real time, const vd_regrid_params& rp);
+    int MakeNewLevelFromCoarse(int lev, Real time, const std::vector<vd_box>& ba);
+    int RemakeLevel(int lev, Real time, const std::vector<vd_box>& ba);
+    void ClearLevel(int lev);
+    int AverageDown();
+    int AverageDownTo(int crse_lev);
+    int FillCoarsePatch(int lev, Real time, MultiFab& mf, int icomp, int ncomp);
+    int finest_level() const { return fine_new_.empty() ? 0 : 1; }
+    const std::vector<vd_box>& fine_grids() const { return fine_ba_; }
+    std::vector<MultiFab>& fine_phi_new() { return fine_new_; }
+
     MultiFab& phi_new() { return phi_new_[0]; }
     // after an Evolve with one corrector the start-of-step species and EEN
     // are taken from Sborder (which holds them unchanged) on first access
@@ -137,6 +154,15 @@ class Vidyut {
     long long mg_iters_ = 0;
     std::vector<int> tag_comps_;
     std::vector<double> tag_thr_, tag_gthr_;
+    // level 1: geometry, BoxArray and per-box state (no ghost cells)
+    vd_geom fgeom_{};
+    int ratio_[3] = {2, 2, 1};
+    std::vector<vd_box> fine_ba_;
+    std::vector<MultiFab> fine_new_, fine_old_;
+    Real t_new_fine_ = 0.0;
+    int set_fine_geom(const int ratio[3]);
+    int fill_fine(MultiFab& mf, const std::vector<MultiFab>* src, const std::vector<vd_box>* src_ba, int icomp,
+                  int ncomp);
 };
 
 }  // namespace vidyut_amd

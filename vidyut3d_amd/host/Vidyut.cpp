@@ -148,9 +148,143 @@ std::vector<std::string> Vidyut::varnames() const {
 }
 
 int Vidyut::FillPatch(int lev, Real time, MultiFab& mf, int icomp, int ncomp) {
-    (void)lev;
     (void)time;
-    return vd_fill_patch(ctx_, &phi_new_[0].array(), &mf.array(), &dom_, &geom_, icomp, ncomp);
+    if (lev == 0) return vd_fill_patch(ctx_, &phi_new_[0].array(), &mf.array(), &dom_, &geom_, icomp, ncomp);
+    if (lev != 1 || fine_new_.empty()) return VD_EINVAL;
+    return fill_fine(mf, &fine_new_, &fine_ba_, icomp, ncomp);
+}
+
+// ------------------------------------------------------------ level 1
+// the refined geometry of level 1 (refRatio(0) = ratio)
+int Vidyut::set_fine_geom(const int ratio[3]) {
+    fgeom_ = geom_;
+    for (int d = 0; d < 3; d++) {
+        ratio_[d] = d < dim_ ? ratio[d] : 1;
+        if (d >= dim_) continue;
+        if (ratio_[d] != 2 && ratio_[d] != 4) return VD_EINVAL;
+        fgeom_.dom_lo[d] = geom_.dom_lo[d] * ratio_[d];
+        fgeom_.dom_hi[d] = (geom_.dom_hi[d] + 1) * ratio_[d] - 1;
+        fgeom_.dx[d] = geom_.dx[d] / ratio_[d];
+    }
+    return VD_OK;
+}
+
+// FillPatchTwoLevels into mf from the level-1 boxes src (none: from the
+// coarse level alone, InterpFromCoarseLevel)
+int Vidyut::fill_fine(MultiFab& mf, const std::vector<MultiFab>* src, const std::vector<vd_box>* src_ba, int icomp,
+                      int ncomp) {
+    std::vector<vd_array4> fa;
+    if (src)
+        for (const MultiFab& m : *src) fa.push_back(m.array());
+    const int nf = src ? (int)src->size() : 0;
+    return vd_fill_patch_two_levels(ctx_, &phi_new_[0].array(), &geom_, nf ? fa.data() : nullptr,
+                                    nf ? src_ba->data() : nullptr, nf, &mf.array(), &fgeom_, ratio_, icomp, ncomp);
+}
+
+// Source/GridWork.cpp:143-164
+int Vidyut::FillCoarsePatch(int lev, Real time, MultiFab& mf, int icomp, int ncomp) {
+    (void)time;
+    if (lev != 1) return VD_EINVAL;
+    return fill_fine(mf, nullptr, nullptr, icomp, ncomp);
+}
+
+// Source/GridWork.cpp:17-29: phi_new of the new level from the coarse one
+int Vidyut::MakeNewLevelFromCoarse(int lev, Real time, const std::vector<vd_box>& ba) {
+    if (lev != 1) return VD_EINVAL;
+    std::vector<MultiFab> nn(ba.size()), no(ba.size());
+    for (size_t b = 0; b < ba.size(); b++) {
+        nn[b].define(ctx_, ba[b], dim_, nvar_, 0);
+        no[b].define(ctx_, ba[b], dim_, nvar_, 0);
+        if (!nn[b].ok() || !no[b].ok()) return VD_ENOMEM;
+        TRY(FillCoarsePatch(lev, time, nn[b], 0, nvar_));
+    }
+    fine_new_ = std::move(nn);
+    fine_old_ = std::move(no);
+    fine_ba_ = ba;
+    t_new_fine_ = time;
+    return VD_OK;
+}
+
+// Source/GridWork.cpp:33-49: the level on new grids, filled from its old
+// data where the old grids cover a cell and from the coarse level elsewhere
+int Vidyut::RemakeLevel(int lev, Real time, const std::vector<vd_box>& ba) {
+    if (lev != 1 || fine_new_.empty()) return VD_EINVAL;
+    std::vector<MultiFab> nn(ba.size()), no(ba.size());
+    for (size_t b = 0; b < ba.size(); b++) {
+        nn[b].define(ctx_, ba[b], dim_, nvar_, 0);
+        no[b].define(ctx_, ba[b], dim_, nvar_, 0);
+        if (!nn[b].ok() || !no[b].ok()) return VD_ENOMEM;
+        TRY(fill_fine(nn[b], &fine_new_, &fine_ba_, 0, nvar_));
+    }
+    fine_new_ = std::move(nn);
+    fine_old_ = std::move(no);
+    fine_ba_ = ba;
+    t_new_fine_ = time;
+    return VD_OK;
+}
+
+// Source/GridWork.cpp:53-57
+void Vidyut::ClearLevel(int lev) {
+    if (lev != 1) return;
+    fine_new_.clear();
+    fine_old_.clear();
+    fine_ba_.clear();
+}
+
+// Source/GridWork.cpp:94-108: covered coarse cells become the average of
+// the fine cells over them
+int Vidyut::AverageDownTo(int crse_lev) {
+    if (crse_lev != 0) return VD_EINVAL;
+    for (size_t b = 0; b < fine_new_.size(); b++)
+        TRY(vd_average_down(ctx_, &fine_new_[b].array(), &fine_ba_[b], &phi_new_[0].array(), 0, nvar_, ratio_,
+                            dim_));
+    return VD_OK;
+}
+
+int Vidyut::AverageDown() { return fine_new_.empty() ? VD_OK : AverageDownTo(0); }
+
+// regrid(lbase = 0): new level-1 grids from the tags of level 0
+int Vidyut::regrid(int lbase, Real time, const vd_regrid_params& rp) {
+    if (lbase != 0) return VD_EINVAL;
+    if (!fine_new_.empty()) {
+        for (int d = 0; d < dim_; d++)
+            if (rp.ref_ratio[d] != ratio_[d]) return VD_EINVAL;  // refRatio is fixed for the run
+    } else {
+        TRY(set_fine_geom(rp.ref_ratio));
+    }
+    const size_t n = (size_t)(dom_.hi[0] - dom_.lo[0] + 1) * (dom_.hi[1] - dom_.lo[1] + 1) *
+                     (dom_.hi[2] - dom_.lo[2] + 1);
+    void* d = nullptr;
+    TRY(vd_malloc(ctx_, n, &d));
+    vd_tagarray4 tags;
+    tags.p = (char*)d;
+    for (int q = 0; q < 3; q++) {
+        tags.begin[q] = dom_.lo[q];
+        tags.end[q] = dom_.hi[q] + 1;
+    }
+    tags.jstride = tags.end[0] - tags.begin[0];
+    tags.kstride = tags.jstride * (tags.end[1] - tags.begin[1]);
+    std::vector<char> host(n);
+    int st = vd_memset_zero(ctx_, d, n);
+    if (st == VD_OK) st = ErrorEst(0, tags, time, 0);
+    if (st == VD_OK) st = vd_memcpy_d2h(ctx_, host.data(), d, n);
+    vd_free(ctx_, d);
+    if (st != VD_OK) return st;
+    const int per[3] = {geom_.is_periodic[0], geom_.is_periodic[1], geom_.is_periodic[2]};
+    std::vector<vd_box> ba(64);
+    int nb = 0;
+    st = vd_make_new_grids(host.data(), &dom_, per, dim_, &rp, ba.data(), (int)ba.size(), &nb);
+    if (st == VD_EINVAL && nb > (int)ba.size()) {
+        ba.resize((size_t)nb);
+        st = vd_make_new_grids(host.data(), &dom_, per, dim_, &rp, ba.data(), (int)ba.size(), &nb);
+    }
+    if (st != VD_OK) return st;
+    ba.resize((size_t)nb);
+    if (ba.empty()) {
+        ClearLevel(1);
+        return VD_OK;
+    }
+    return fine_new_.empty() ? MakeNewLevelFromCoarse(1, time, ba) : RemakeLevel(1, time, ba);
 }
 
 int Vidyut::compute_scalar_transport_flux(int lev, MultiFab& Sborder, FaceMF& flux, const int* bc_lo,
@@ -642,6 +776,51 @@ int vd_sim_read_checkpoint(vd_sim* s, const char* dir) {
 }
 
 int vd_sim_step(const vd_sim* s) { return s->v->step(); }
+
+int vd_sim_regrid(vd_sim* s, int ntag, const int* comps, const double* thr, const double* gthr,
+                  const vd_regrid_params* rp, int* nboxes) {
+    if (!s || !rp || !nboxes || ntag < 0 || (ntag > 0 && (!comps || !thr || !gthr))) return VD_EINVAL;
+    std::vector<int> c(comps, comps + ntag);
+    std::vector<double> t(thr, thr + ntag), g(gthr, gthr + ntag);
+    int st = s->v->set_tagging(c, t, g);
+    if (st) return st;
+    st = s->v->regrid(0, s->v->time(), *rp);
+    *nboxes = (int)s->v->fine_grids().size();
+    return st;
+}
+
+int vd_sim_level_boxes(vd_sim* s, int lev, vd_box* out, int max_out, int* n) {
+    if (!s || !n || (lev != 0 && lev != 1) || (max_out > 0 && !out)) return VD_EINVAL;
+    if (lev == 0) {
+        *n = 1;
+        if (max_out < 1) return VD_EINVAL;
+        out[0] = s->v->phi_new().box();
+        return VD_OK;
+    }
+    const std::vector<vd_box>& ba = s->v->fine_grids();
+    *n = (int)ba.size();
+    if ((int)ba.size() > max_out) return VD_EINVAL;
+    for (size_t b = 0; b < ba.size(); b++) out[b] = ba[b];
+    return VD_OK;
+}
+
+int vd_sim_get_fine(vd_sim* s, int b, double* host) {
+    if (!s || !host || b < 0 || b >= (int)s->v->fine_phi_new().size()) return VD_EINVAL;
+    auto& mf = s->v->fine_phi_new()[(size_t)b];
+    return vd_memcpy_d2h(s->ctx, host, mf.data(), mf.bytes());
+}
+
+int vd_sim_set_fine(vd_sim* s, int b, const double* host) {
+    if (!s || !host || b < 0 || b >= (int)s->v->fine_phi_new().size()) return VD_EINVAL;
+    auto& mf = s->v->fine_phi_new()[(size_t)b];
+    return vd_memcpy_h2d(s->ctx, mf.data(), host, mf.bytes());
+}
+
+int vd_sim_average_down(vd_sim* s) {
+    if (!s) return VD_EINVAL;
+    const int st = s->v->AverageDown();
+    return st != VD_OK ? st : vd_ctx_sync(s->ctx);
+}
 
 int vd_sim_error_est(vd_sim* s, int ntag, const int* comps, const double* thr, const double* gthr,
                      char* host_tags) {

@@ -353,6 +353,41 @@ class Sim:
     def step(self):
         return lib().vd_sim_step(self.ptr)
 
+    # ---- AMR level 1 (the step driver advances level 0 only)
+    def regrid(self, comps, thr, gthr, prm):
+        """Vidyut::regrid(0, time): tags of level 0 -> level-1 grids
+        (MakeNewLevelFromCoarse / RemakeLevel / ClearLevel); returns the
+        level-1 boxes."""
+        n = len(comps)
+        nb = C.c_int()
+        check(lib().vd_sim_regrid(self.ptr, n, (C.c_int * n)(*comps), (C.c_double * n)(*thr),
+                                  (C.c_double * n)(*gthr), C.byref(prm), C.byref(nb)), "vd_sim_regrid")
+        return self.level_boxes(1)
+
+    def level_boxes(self, lev):
+        n = C.c_int()
+        lib().vd_sim_level_boxes(self.ptr, int(lev), None, 0, C.byref(n))
+        out = (abi.vd_box * max(n.value, 1))()
+        check(lib().vd_sim_level_boxes(self.ptr, int(lev), out, max(n.value, 1), C.byref(n)),
+              "vd_sim_level_boxes")
+        return [out[i] for i in range(n.value)]
+
+    def fine_shape(self, box):
+        b, e = abi.grown_extent(box, 0, self.case.geom.dim)
+        return abi.shape_of(b, e, self.shape[0])
+
+    def get_fine(self, b, box):
+        out = np.empty(self.fine_shape(box))
+        check(lib().vd_sim_get_fine(self.ptr, int(b), out.ctypes.data_as(C.c_void_p)), "vd_sim_get_fine")
+        return out
+
+    def set_fine(self, b, host):
+        host = np.ascontiguousarray(host, dtype=np.float64)
+        check(lib().vd_sim_set_fine(self.ptr, int(b), host.ctypes.data_as(C.c_void_p)), "vd_sim_set_fine")
+
+    def average_down(self):
+        check(lib().vd_sim_average_down(self.ptr), "vd_sim_average_down")
+
     def error_est(self, comps, thr, gthr):
         n = len(comps)
         out = np.zeros(self.shape[1:], dtype=np.int8)
