@@ -190,3 +190,40 @@ def test_sim_regrid_cycle_make_remake_average_clear(dev):
     # nothing tagged: level 1 cleared
     assert sim.regrid([E], [1e300], [1e300], prm) == []
     sim.close()
+
+
+def test_two_level_plotfile_and_checkpoint_round_trip(dev, tmp_path):
+    """With a level 1, WritePlotFile and WriteCheckpointFile write both
+    levels (Level_1/Cell, Level_1/phi: every box of the level in one data
+    file); the readers give back each box's data bit for bit, and a new
+    simulation restarted from the checkpoint holds the same two levels."""
+    from vidyut3d_amd import cases, plotfile
+    c = cases.discharge_2d(n=32, seed=2, nsteps=1)
+    sim = native.Sim(dev, c)
+    E = c.mech.e_idx
+    st = sim.get_state()
+    prm = native.regrid_params(ref_ratio=2, blocking_factor=4, max_grid_size=8, n_error_buf=1)
+    boxes = sim.regrid([E], [0.2 * st[E].max()], [1e300], prm)
+    assert len(boxes) >= 2  # several boxes in one level file
+    fines = [sim.get_fine(b, box) for b, box in enumerate(boxes)]
+    plt = str(tmp_path / "plt")
+    sim.write_plotfile(plt)
+    pf = plotfile.read(plt)
+    assert pf.finest_level == 1 and pf.ref_ratio == [2]
+    assert np.array_equal(pf.data, st)
+    got = {(tuple(lo), tuple(hi)): d for lo, hi, d in pf.levels[1]}
+    for box, f in zip(boxes, fines):
+        assert np.array_equal(got[(tuple(box.lo[:2]), tuple(box.hi[:2]))], f)
+    chk = str(tmp_path / "chk")
+    sim.write_checkpoint(chk)
+    ck = plotfile.read_checkpoint(chk)
+    assert len(ck.levels) == 2 and len(ck.levels[1]) == len(boxes)
+    sim2 = native.Sim(dev, c)
+    sim2.read_checkpoint(chk)
+    assert np.array_equal(sim2.get_state(), st)
+    b2 = sim2.level_boxes(1)
+    assert [(tuple(b.lo), tuple(b.hi)) for b in b2] == [(tuple(b.lo), tuple(b.hi)) for b in boxes]
+    for b, box in enumerate(b2):
+        assert np.array_equal(sim2.get_fine(b, box), fines[b])
+    sim2.close()
+    sim.close()

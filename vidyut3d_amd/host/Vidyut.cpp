@@ -564,70 +564,136 @@ static std::string box_str(const vd_box& b, int dim) {
 // <dir>/<prefix>_D_00000 holding the FAB (little-endian IEEE doubles,
 // component after component, x fastest), with per-component minima and
 // maxima in the header.  h holds ncomp x ncell values.
-static int write_vismf(const std::string& dir, const std::string& prefix, const std::vector<double>& h,
-                       int ncomp, int64_t ncell, const vd_box& box, int dim) {
+// one FAB of a VisMF MultiFab: host data (ncomp x ncell, component-major)
+// over a box
+struct HostFab {
+    std::vector<double> h;
+    int64_t ncell;
+    vd_box box;
+};
+
+// The VisMF MultiFab <prefix>: every FAB in <prefix>_D_00000 (a text
+// header line, then the doubles), and <prefix>_H with the BoxArray, each
+// FAB's file offset and the per-FAB, per-component minima and maxima.
+static int write_vismf(const std::string& dir, const std::string& prefix, const std::vector<HostFab>& fabs,
+                       int ncomp, int dim) {
     const std::string base = prefix.substr(prefix.find_last_of('/') + 1);
+    std::vector<long long> off;
     {
-        std::ostringstream f;
-        f << "FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))" << box_str(box, dim) << " " << ncomp
-          << "\n";
-        const std::string hdr = f.str();
         std::ofstream D(dir + "/" + prefix + "_D_00000", std::ios::binary);
-        D.write(hdr.data(), hdr.size());
-        D.write((const char*)h.data(), (size_t)ncomp * ncell * sizeof(double));
+        long long pos = 0;
+        for (const HostFab& f : fabs) {
+            std::ostringstream o;
+            o << "FAB ((8, (64 11 52 0 1 12 0 1023)),(8, (8 7 6 5 4 3 2 1)))" << box_str(f.box, dim) << " " << ncomp
+              << "\n";
+            const std::string hdr = o.str();
+            off.push_back(pos);
+            D.write(hdr.data(), hdr.size());
+            D.write((const char*)f.h.data(), (size_t)ncomp * f.ncell * sizeof(double));
+            pos += (long long)hdr.size() + (long long)ncomp * f.ncell * (long long)sizeof(double);
+        }
         if (!D.good()) return VD_EINVAL;
     }
-    std::vector<double> mn(ncomp, DBL_MAX), mx(ncomp, -DBL_MAX);
-    for (int q = 0; q < ncomp; q++)
-        for (int64_t c = 0; c < ncell; c++) {
-            double v = h[(size_t)q * ncell + c];
-            mn[q] = std::min(mn[q], v);
-            mx[q] = std::max(mx[q], v);
-        }
+    const size_t nf = fabs.size();
+    std::vector<double> mn(nf * ncomp, DBL_MAX), mx(nf * ncomp, -DBL_MAX);
+    for (size_t b = 0; b < nf; b++)
+        for (int q = 0; q < ncomp; q++)
+            for (int64_t c = 0; c < fabs[b].ncell; c++) {
+                const double v = fabs[b].h[(size_t)q * fabs[b].ncell + c];
+                mn[b * ncomp + q] = std::min(mn[b * ncomp + q], v);
+                mx[b * ncomp + q] = std::max(mx[b * ncomp + q], v);
+            }
     std::ofstream C(dir + "/" + prefix + "_H");
     C.precision(16);
     C << std::scientific;
-    C << "1\n1\n" << ncomp << "\n0\n(1 0\n" << box_str(box, dim) << "\n)\n1\nFabOnDisk: " << base << "_D_00000 0\n\n";
-    C << "1," << ncomp << "\n";
-    for (int q = 0; q < ncomp; q++) C << mn[q] << ",";
-    C << "\n\n1," << ncomp << "\n";
-    for (int q = 0; q < ncomp; q++) C << mx[q] << ",";
-    C << "\n";
+    C << "1\n1\n" << ncomp << "\n0\n(" << nf << " 0\n";
+    for (const HostFab& f : fabs) C << box_str(f.box, dim) << "\n";
+    C << ")\n" << nf << "\n";
+    for (size_t b = 0; b < nf; b++) C << "FabOnDisk: " << base << "_D_00000 " << off[b] << "\n";
+    C << "\n" << nf << "," << ncomp << "\n";
+    for (size_t b = 0; b < nf; b++) {
+        for (int q = 0; q < ncomp; q++) C << mn[b * ncomp + q] << ",";
+        C << "\n";
+    }
+    C << "\n" << nf << "," << ncomp << "\n";
+    for (size_t b = 0; b < nf; b++) {
+        for (int q = 0; q < ncomp; q++) C << mx[b * ncomp + q] << ",";
+        C << "\n";
+    }
     return C.good() ? VD_OK : VD_EINVAL;
+}
+
+// host copies of a level's phi_new FABs
+static int level_fabs(vd_ctx* ctx, std::vector<MultiFab>& mfs, int nvar, std::vector<HostFab>& out) {
+    for (MultiFab& m : mfs) {
+        HostFab f;
+        f.box = m.box();
+        f.ncell = m.array().nstride;
+        f.h.resize((size_t)f.ncell * nvar);
+        TRY(vd_memcpy_d2h(ctx, f.h.data(), m.data(), f.h.size() * sizeof(double)));
+        out.push_back(std::move(f));
+    }
+    return VD_OK;
 }
 
 // WritePlotFile -> amrex::WriteMultiLevelPlotfile (Source/Plot.cpp:29-38):
 // HyperCLaw-V1.1 Header, Level_0/Cell_H (VisMF v1) and one Cell_D file
 // holding the FAB (little-endian IEEE doubles, Fortran order per comp).
 int Vidyut::WritePlotFile(const std::string& dir) const {
-    const vd_array4& a = phi_new_[0].array();
-    size_t n = (size_t)a.nstride * nvar_;
-    std::vector<double> h(n);
-    TRY(vd_memcpy_d2h(ctx_, h.data(), a.p, n * sizeof(double)));
+    Vidyut* self = const_cast<Vidyut*>(this);  // device reads only
+    const int finest = finest_level();
+    std::vector<std::vector<HostFab>> lv(finest + 1);
+    TRY(level_fabs(ctx_, self->phi_new_, nvar_, lv[0]));
+    if (finest == 1) TRY(level_fabs(ctx_, self->fine_new_, nvar_, lv[1]));
     TRY(mkdir_p(dir));
-    TRY(mkdir_p(dir + "/Level_0"));
+    const vd_geom* gl[2] = {&geom_, &fgeom_};
     std::vector<std::string> names = varnames();
     {
         std::ofstream H(dir + "/Header");
         H.precision(17);
         H << "HyperCLaw-V1.1\n" << nvar_ << "\n";
         for (auto& s : names) H << s << "\n";
-        H << dim_ << "\n" << t_new_ << "\n" << 0 << "\n";
+        H << dim_ << "\n" << t_new_ << "\n" << finest << "\n";
         for (int d = 0; d < dim_; d++) H << geom_.prob_lo[d] << " ";
         H << "\n";
         for (int d = 0; d < dim_; d++) H << geom_.prob_hi[d] << " ";
-        H << "\n\n" << box_str(dom_, dim_) << " \n" << istep_ << " \n";
-        for (int d = 0; d < dim_; d++) H << geom_.dx[d] << " ";
-        H << "\n0\n0\n";
-        H << 0 << " " << 1 << " " << t_new_ << "\n" << istep_ << "\n";
-        for (int d = 0; d < dim_; d++) {
-            double lo = geom_.prob_lo[d] + geom_.dx[d] * (dom_.lo[d] - geom_.dom_lo[d]);
-            double hi = geom_.prob_lo[d] + geom_.dx[d] * (dom_.hi[d] + 1 - geom_.dom_lo[d]);
-            H << lo << " " << hi << "\n";
+        H << "\n";
+        for (int l = 0; l < finest; l++) H << ratio_[0] << " ";
+        H << "\n";
+        for (int l = 0; l <= finest; l++) {
+            vd_box db;
+            for (int d = 0; d < 3; d++) {
+                db.lo[d] = gl[l]->dom_lo[d];
+                db.hi[d] = gl[l]->dom_hi[d];
+            }
+            H << box_str(db, dim_) << " ";
         }
-        H << "Level_0/Cell\n";
+        H << "\n";
+        for (int l = 0; l <= finest; l++) H << istep_ << " ";
+        H << "\n";
+        for (int l = 0; l <= finest; l++) {
+            for (int d = 0; d < dim_; d++) H << gl[l]->dx[d] << " ";
+            H << "\n";
+        }
+        H << "0\n0\n";
+        for (int l = 0; l <= finest; l++) {
+            H << l << " " << lv[l].size() << " " << t_new_ << "\n" << istep_ << "\n";
+            for (const HostFab& f : lv[l])
+                for (int d = 0; d < dim_; d++) {
+                    const vd_geom& g = *gl[l];
+                    const double lo = g.prob_lo[d] + g.dx[d] * (f.box.lo[d] - g.dom_lo[d]);
+                    const double hi = g.prob_lo[d] + g.dx[d] * (f.box.hi[d] + 1 - g.dom_lo[d]);
+                    H << lo << " " << hi << "\n";
+                }
+            H << "Level_" << l << "/Cell\n";
+        }
+        if (!H.good()) return VD_EINVAL;
     }
-    return write_vismf(dir, "Level_0/Cell", h, nvar_, a.nstride, dom_, dim_);
+    for (int l = 0; l <= finest; l++) {
+        TRY(mkdir_p(dir + "/Level_" + std::to_string(l)));
+        TRY(write_vismf(dir, "Level_" + std::to_string(l) + "/Cell", lv[l], nvar_, dim_));
+    }
+    return VD_OK;
 }
 
 // WriteCheckpointFile (Source/Plot.cpp:40-119): a "Checkpoint file for
@@ -635,26 +701,122 @@ int Vidyut::WritePlotFile(const std::string& dir) const {
 // significant digits, then the BoxArray) and phi_new as the VisMF MultiFab
 // Level_0/phi.
 int Vidyut::WriteCheckpointFile(const std::string& dir) const {
-    const vd_array4& a = phi_new_[0].array();
-    size_t n = (size_t)a.nstride * nvar_;
-    std::vector<double> h(n);
-    TRY(vd_memcpy_d2h(ctx_, h.data(), a.p, n * sizeof(double)));
+    Vidyut* self = const_cast<Vidyut*>(this);  // device reads only
+    const int finest = finest_level();
+    std::vector<std::vector<HostFab>> lv(finest + 1);
+    TRY(level_fabs(ctx_, self->phi_new_, nvar_, lv[0]));
+    if (finest == 1) TRY(level_fabs(ctx_, self->fine_new_, nvar_, lv[1]));
     TRY(mkdir_p(dir));
-    TRY(mkdir_p(dir + "/Level_0"));
     {
         std::ofstream H(dir + "/Header");
         H.precision(17);
-        H << "Checkpoint file for Vidyut\n" << 0 << "\n";
-        H << istep_ << " \n" << dt_ << " \n" << t_new_ << " \n";
-        H << "(1 0\n" << box_str(dom_, dim_) << "\n)\n";
+        H << "Checkpoint file for Vidyut\n" << finest << "\n";
+        for (int l = 0; l <= finest; l++) H << istep_ << " ";
+        H << "\n";
+        for (int l = 0; l <= finest; l++) H << dt_ << " ";
+        H << "\n";
+        for (int l = 0; l <= finest; l++) H << t_new_ << " ";
+        H << "\n";
+        for (int l = 0; l <= finest; l++) {
+            H << "(" << lv[l].size() << " 0\n";
+            for (const HostFab& f : lv[l]) H << box_str(f.box, dim_) << "\n";
+            H << ")\n";
+        }
         if (!H.good()) return VD_EINVAL;
     }
-    return write_vismf(dir, "Level_0/phi", h, nvar_, a.nstride, dom_, dim_);
+    for (int l = 0; l <= finest; l++) {
+        TRY(mkdir_p(dir + "/Level_" + std::to_string(l)));
+        TRY(write_vismf(dir, "Level_" + std::to_string(l) + "/phi", lv[l], nvar_, dim_));
+    }
+    return VD_OK;
 }
 
 // ReadCheckpointFile (Source/Plot.cpp:121-209) for a single-level
 // checkpoint of this domain: istep, dt, t_new and phi_new.  The FAB may be
 // in either byte order.
+// the FABs of the VisMF MultiFab <dir>/<prefix> (phi_H and its data file),
+// in BoxArray order, as host doubles (ncomp x cells of each box)
+static int read_vismf(const std::string& dir, const std::string& prefix, int ncomp,
+                      std::vector<std::vector<double>>& out) {
+    std::ifstream C(dir + "/" + prefix + "_H");
+    if (!C.good()) return VD_ENOTFOUND;
+    std::string ch((std::istreambuf_iterator<char>(C)), std::istreambuf_iterator<char>());
+    const std::string sub = prefix.substr(0, prefix.find_last_of('/') + 1);
+    size_t fp = 0;
+    while ((fp = ch.find("FabOnDisk: ", fp)) != std::string::npos) {
+        std::istringstream fl(ch.substr(fp + 11));
+        fp += 11;
+        std::string fname;
+        long long off = 0;
+        fl >> fname >> off;
+        std::ifstream D(dir + "/" + sub + fname, std::ios::binary);
+        if (!D.good()) return VD_ENOTFOUND;
+        D.seekg(off);
+        std::string fab;
+        std::getline(D, fab);
+        const bool le = fab.find("(8 7 6 5 4 3 2 1)") != std::string::npos;
+        const bool be = fab.find("(1 2 3 4 5 6 7 8)") != std::string::npos;
+        if (fab.compare(0, 4, "FAB ") != 0 || !(le || be)) return VD_EINVAL;
+        const size_t nb = fab.rfind(' ');
+        if (nb == std::string::npos || std::atoi(fab.c_str() + nb + 1) != ncomp) return VD_EINVAL;
+        // the box ((lo) (hi) (type)) and ncomp after the real descriptor
+        const size_t bp = fab.find(")))");
+        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        if (bp == std::string::npos) return VD_EINVAL;
+        std::string bx = fab.substr(bp + 3);
+        for (char& c : bx)
+            if (c == '(' || c == ')' || c == ',') c = ' ';
+        std::istringstream bs(bx);
+        std::vector<long long> v;
+        long long x;
+        while (bs >> x) v.push_back(x);
+        // lo[dim], hi[dim], type[dim], ncomp
+        const int dim = (int)(v.size() - 1) / 3;
+        if (dim < 1 || dim > 3) return VD_EINVAL;
+        for (int d = 0; d < dim; d++) {
+            lo[d] = (int)v[(size_t)d];
+            hi[d] = (int)v[(size_t)(dim + d)];
+        }
+        size_t cells = 1;
+        for (int d = 0; d < dim; d++) cells *= (size_t)(hi[d] - lo[d] + 1);
+        std::vector<double> h(cells * (size_t)ncomp);
+        D.read((char*)h.data(), h.size() * sizeof(double));
+        if ((size_t)D.gcount() != h.size() * sizeof(double)) return VD_EINVAL;
+        if (be)
+            for (auto& w : h) {
+                unsigned char* b = (unsigned char*)&w;
+                std::reverse(b, b + 8);
+            }
+        out.push_back(std::move(h));
+    }
+    return out.empty() ? VD_EINVAL : VD_OK;
+}
+
+// the BoxArray "(n 0\n((lo) (hi) (t))\n...)" at the stream's position
+static bool read_boxarray(std::istream& H, int dim, std::vector<vd_box>& ba) {
+    std::string line;
+    while (std::getline(H, line) && line.find('(') == std::string::npos) {
+    }
+    int n = 0;
+    if (std::sscanf(line.c_str(), " (%d", &n) != 1 || n < 0) return false;
+    for (int b = 0; b < n; b++) {
+        if (!std::getline(H, line)) return false;
+        for (char& c : line)
+            if (c == '(' || c == ')' || c == ',') c = ' ';
+        std::istringstream bs(line);
+        vd_box x{};
+        for (int d = 0; d < dim; d++) bs >> x.lo[d];
+        for (int d = 0; d < dim; d++) bs >> x.hi[d];
+        if (!bs) return false;
+        ba.push_back(x);
+    }
+    return (bool)std::getline(H, line);  // ")"
+}
+
+// ReadCheckpointFile (Source/Plot.cpp:121-209): istep, dt and t_new, the
+// level-0 state of this domain and, for a two-level checkpoint, level 1 on
+// its BoxArray (refinement ratio: the last regrid's, else 2).  The FABs
+// may be in either byte order.
 int Vidyut::ReadCheckpointFile(const std::string& dir) {
     std::ifstream H(dir + "/Header");
     if (!H.good()) return VD_ENOTFOUND;
@@ -662,43 +824,55 @@ int Vidyut::ReadCheckpointFile(const std::string& dir) {
     std::getline(H, line);  // title
     int finest = -1;
     H >> finest;
-    if (finest != 0) return VD_EINVAL;  // one level
+    if (finest != 0 && finest != 1) return VD_EINVAL;  // at most two levels
     long long st = 0;
-    double dtv = 0.0, tv = 0.0;
-    H >> st >> dtv >> tv;
+    double dtv = 0.0, tv = 0.0, skip = 0.0;
     std::getline(H, line);
-    std::string rest((std::istreambuf_iterator<char>(H)), std::istreambuf_iterator<char>());
-    if (rest.find(box_str(dom_, dim_)) == std::string::npos) return VD_EINVAL;  // another domain
-    std::ifstream C(dir + "/Level_0/phi_H");
-    if (!C.good()) return VD_ENOTFOUND;
-    std::string ch((std::istreambuf_iterator<char>(C)), std::istreambuf_iterator<char>());
-    const size_t fp = ch.find("FabOnDisk: ");
-    if (fp == std::string::npos) return VD_EINVAL;
-    std::istringstream fl(ch.substr(fp + 11));
-    std::string fname;
-    long long off = 0;
-    fl >> fname >> off;
-    std::ifstream D(dir + "/Level_0/" + fname, std::ios::binary);
-    if (!D.good()) return VD_ENOTFOUND;
-    D.seekg(off);
-    std::string fab;
-    std::getline(D, fab);
-    const bool le = fab.find("(8 7 6 5 4 3 2 1)") != std::string::npos;
-    const bool be = fab.find("(1 2 3 4 5 6 7 8)") != std::string::npos;
-    if (fab.compare(0, 4, "FAB ") != 0 || !(le || be)) return VD_EINVAL;
-    const size_t nb = fab.rfind(' ');
-    if (nb == std::string::npos || std::atoi(fab.c_str() + nb + 1) != nvar_) return VD_EINVAL;
+    std::getline(H, line);
+    {
+        std::istringstream l(line);
+        l >> st;
+    }
+    std::getline(H, line);
+    {
+        std::istringstream l(line);
+        l >> dtv;
+    }
+    std::getline(H, line);
+    {
+        std::istringstream l(line);
+        l >> tv;
+    }
+    (void)skip;
+    std::vector<vd_box> ba0, ba1;
+    if (!read_boxarray(H, dim_, ba0) || ba0.size() != 1) return VD_EINVAL;
+    for (int d = 0; d < dim_; d++)
+        if (ba0[0].lo[d] != dom_.lo[d] || ba0[0].hi[d] != dom_.hi[d]) return VD_EINVAL;  // another domain
+    if (finest == 1 && (!read_boxarray(H, dim_, ba1) || ba1.empty())) return VD_EINVAL;
+    std::vector<std::vector<double>> f0, f1;
+    TRY(read_vismf(dir, "Level_0/phi", nvar_, f0));
     const vd_array4& a = phi_new_[0].array();
-    const size_t n = (size_t)a.nstride * nvar_;
-    std::vector<double> h(n);
-    D.read((char*)h.data(), n * sizeof(double));
-    if ((size_t)D.gcount() != n * sizeof(double)) return VD_EINVAL;
-    if (be)
-        for (auto& v : h) {
-            unsigned char* b = (unsigned char*)&v;
-            std::reverse(b, b + 8);
+    if (f0.size() != 1 || f0[0].size() != (size_t)a.nstride * nvar_) return VD_EINVAL;
+    if (finest == 1) {
+        TRY(read_vismf(dir, "Level_1/phi", nvar_, f1));
+        if (f1.size() != ba1.size()) return VD_EINVAL;
+        if (fine_new_.empty()) TRY(set_fine_geom(ratio_));
+        std::vector<MultiFab> nn(ba1.size()), no(ba1.size());
+        for (size_t b = 0; b < ba1.size(); b++) {
+            nn[b].define(ctx_, ba1[b], dim_, nvar_, 0);
+            no[b].define(ctx_, ba1[b], dim_, nvar_, 0);
+            if (!nn[b].ok() || !no[b].ok()) return VD_ENOMEM;
+            if (f1[b].size() * sizeof(double) != nn[b].bytes()) return VD_EINVAL;
+            TRY(vd_memcpy_h2d(ctx_, nn[b].data(), f1[b].data(), nn[b].bytes()));
         }
-    TRY(vd_memcpy_h2d(ctx_, a.p, h.data(), n * sizeof(double)));
+        fine_new_ = std::move(nn);
+        fine_old_ = std::move(no);
+        fine_ba_ = ba1;
+        t_new_fine_ = tv;
+    } else {
+        ClearLevel(1);
+    }
+    TRY(vd_memcpy_h2d(ctx_, a.p, f0[0].data(), f0[0].size() * sizeof(double)));
     istep_ = (int)st;
     dt_ = dtv;
     t_new_ = tv;

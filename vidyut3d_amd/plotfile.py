@@ -1,4 +1,4 @@
-"""AMReX plotfile reader, writer and field comparator (single level).
+"""AMReX plotfile reader, writer and field comparator.
 
 The on-disk format is the one Vidyut writes through
 amrex::WriteMultiLevelPlotfile (Source/Plot.cpp:29-38):
@@ -20,6 +20,11 @@ Vidyut::WriteCheckpointFile (Source/Plot.cpp:40-119): a ``Header`` with the
 title ``Checkpoint file for Vidyut``, the finest level, istep, dt and t_new
 per level and the BoxArray, and the state as the VisMF MultiFab
 ``Level_0/phi`` (``phi_H`` + ``phi_D_00000``, the same FAB layout).
+
+Both readers also take a second level (finest level 1): ``levels`` holds,
+per level, the list of (lo, hi, data) of its boxes (the ``Level_1/Cell``
+or ``Level_1/phi`` MultiFab, several FABs in one data file at the offsets
+the VisMF header gives); ``data`` stays the level-0 state.
 
 ``compare`` follows py_utils/compare_fields_from_twoplotfiles.py:39-46
 (err = sqrt(mean((f1-f2)^2)), rel = err / mean(|f1|)) and also reports
@@ -46,14 +51,21 @@ def _box(text):
 
 
 class Plotfile:
-    """Level-0 contents of a plotfile: names, geometry and data of shape
-    (ncomp, nz, ny, nx) (nz = 1 in 2-D)."""
+    """Contents of a plotfile: names, geometry, the level-0 data of shape
+    (ncomp, nz, ny, nx) (nz = 1 in 2-D) and, per level, its boxes' data."""
 
-    def __init__(self, names, data, dim, time, step, prob_lo, prob_hi, dx, lo, hi):
+    def __init__(self, names, data, dim, time, step, prob_lo, prob_hi, dx, lo, hi, levels=None,
+                 ref_ratio=None):
         self.names, self.data, self.dim = list(names), data, dim
         self.time, self.step = time, step
         self.prob_lo, self.prob_hi, self.dx = list(prob_lo), list(prob_hi), list(dx)
         self.lo, self.hi = list(lo), list(hi)
+        self.levels = levels if levels is not None else [[(list(lo), list(hi), data)]]
+        self.ref_ratio = list(ref_ratio or [])
+
+    @property
+    def finest_level(self):
+        return len(self.levels) - 1
 
     def field(self, name):
         return self.data[self.names.index(name)]
@@ -70,31 +82,44 @@ def read(path):
     dim = int(lines[p])
     time = float(lines[p + 1])
     finest = int(lines[p + 2])
-    if finest != 0:
-        raise ValueError("%s: %d levels; this reader handles a single level" % (path, finest + 1))
     prob_lo = [float(x) for x in lines[p + 3].split()]
     prob_hi = [float(x) for x in lines[p + 4].split()]
-    # p+5: refinement ratios (empty for one level)
-    lo, hi = _box(lines[p + 6])
+    ratios = _ints(lines[p + 5])  # refinement ratios (empty for one level)
+    lo, hi = _box(lines[p + 6])  # the level-0 domain comes first
     step = int(lines[p + 7].split()[0])
     dx = [float(x) for x in lines[p + 8].split()]
-    rest = lines[p + 9:]
-    cell = [ln.strip() for ln in rest if ln.strip().endswith("/Cell")][-1]
-    data = _read_vismf(os.path.join(path, cell), dim)
-    return Plotfile(names, data, dim, time, step, prob_lo, prob_hi, dx, lo, hi)
+    cells = [ln.strip() for ln in lines[p + 9 + finest:] if ln.strip().endswith("/Cell")]
+    if len(cells) != finest + 1:
+        raise ValueError("%s: %d level paths for %d levels" % (path, len(cells), finest + 1))
+    levels = [_read_fabs(os.path.join(path, c), dim) for c in cells]
+    if len(levels[0]) != 1:
+        raise ValueError("%s: %d boxes on level 0; this reader expects one" % (path, len(levels[0])))
+    return Plotfile(names, levels[0][0][2], dim, time, step, prob_lo, prob_hi, dx, lo, hi, levels, ratios)
 
 
 def _read_vismf(prefix, dim):
     """The single FAB of the VisMF MultiFab <prefix>_H / its data file, as
     an array (ncomp, nz, ny, nx)."""
-    with open(prefix + "_H") as f:
-        vh = f.read()
-    fabs = re.findall(r"FabOnDisk:\s+(\S+)\s+(\d+)", vh)
+    fabs = _read_fabs(prefix, dim)
     if len(fabs) != 1:
         raise ValueError("%s: %d boxes on level 0; this reader expects one" % (prefix, len(fabs)))
-    fname, off = fabs[0][0], int(fabs[0][1])
-    ddir = os.path.dirname(prefix)
-    with open(os.path.join(ddir, fname), "rb") as f:
+    return fabs[0][2]
+
+
+def _read_fabs(prefix, dim):
+    """Every FAB of the VisMF MultiFab <prefix>, in header order, as
+    (lo, hi, array (ncomp, nz, ny, nx))."""
+    with open(prefix + "_H") as f:
+        vh = f.read()
+    out = []
+    for fname, off in re.findall(r"FabOnDisk:\s+(\S+)\s+(\d+)", vh):
+        out.append(_read_fab(os.path.join(os.path.dirname(prefix), fname), int(off), dim))
+    return out
+
+
+def _read_fab(fpath, off, dim):
+    fname = os.path.basename(fpath)
+    with open(fpath, "rb") as f:
         f.seek(off)
         hdr = b""
         while not hdr.endswith(b"\n"):
@@ -118,7 +143,7 @@ def _read_vismf(prefix, dim):
         ext = [bhi[d] - blo[d] + 1 for d in range(dim)] + [1] * (3 - dim)
         n = ext[0] * ext[1] * ext[2] * ncomp
         raw = np.frombuffer(f.read(n * 8), dtype=dt, count=n)
-    return raw.astype(np.float64).reshape(ncomp, ext[2], ext[1], ext[0])
+    return blo, bhi, raw.astype(np.float64).reshape(ncomp, ext[2], ext[1], ext[0])
 
 
 def _write_vismf(prefix, data, box_text):
@@ -137,12 +162,13 @@ def _write_vismf(prefix, data, box_text):
 
 
 class Checkpoint:
-    """Level-0 contents of a Vidyut checkpoint: step, dt, time and the state
-    (ncomp, nz, ny, nx)."""
+    """Contents of a Vidyut checkpoint: step, dt, time, the level-0 state
+    (ncomp, nz, ny, nx) and, per level, its boxes' data."""
 
-    def __init__(self, step, dt, time, data, lo, hi):
+    def __init__(self, step, dt, time, data, lo, hi, levels=None):
         self.step, self.dt, self.time, self.data = step, dt, time, data
         self.lo, self.hi = list(lo), list(hi)
+        self.levels = levels if levels is not None else [[(list(lo), list(hi), data)]]
 
 
 def read_checkpoint(path):
@@ -154,14 +180,15 @@ def read_checkpoint(path):
         lines = [ln.rstrip("\n") for ln in f]
     if not lines or lines[0].strip() != "Checkpoint file for Vidyut":
         raise ValueError("%s: not a Vidyut checkpoint" % path)
-    if int(lines[1]) != 0:
-        raise ValueError("%s: %d levels; this reader handles a single level" % (path, int(lines[1]) + 1))
+    finest = int(lines[1])
     step = int(lines[2].split()[0])
     dt = float(lines[3].split()[0])
     time = float(lines[4].split()[0])
     lo, hi = _box("\n".join(lines[5:]))
-    data = _read_vismf(os.path.join(path, "Level_0", "phi"), len(lo))
-    return Checkpoint(step, dt, time, data, lo, hi)
+    levels = [_read_fabs(os.path.join(path, "Level_%d" % l, "phi"), len(lo)) for l in range(finest + 1)]
+    if len(levels[0]) != 1:
+        raise ValueError("%s: %d boxes on level 0; this reader expects one" % (path, len(levels[0])))
+    return Checkpoint(step, dt, time, levels[0][0][2], lo, hi, levels)
 
 
 def write_checkpoint(path, data, dim, step, dt, time, lo=None):
