@@ -2022,68 +2022,84 @@ static void launch_resid_restrict(vd_ctx* ctx, const MGWork& W, const LevV<T>& v
 }
 
 
-// returns true when the finest post-smooth also added the correction to
-// xacc (x += e), so the caller skips that update
+// first level of the coarse tail (levels l0 .. nlev-1 run in one
+// workgroup launch)
 template <class T>
-static bool vcycle(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L, double alpha,
-                   double beta, double bconst, int nu1, int nu2, double* xacc = nullptr) {
-    bool accumulated = false;
+static int vc_tail_level(const MGWork& W, const std::vector<Lev<T>>& L) {
     const int nl = W.nlev;
-    // levels l0 .. nl-1 (at most VC_TAIL_CELLS cells each) run in one
-    // workgroup launch; the finer ones level by level
     int l0 = nl - 1;
     while (l0 > 0 && ncell(L[l0 - 1].n) <= VC_TAIL_CELLS && nl - l0 < VC_MAXL) l0--;
-    // down
-    for (int l = 0; l < l0; l++) {
-        Lev<T>& F = L[l];
-        LevV<T> v = view<T>(W, F, alpha, beta, bconst);
-        int64_t nc = ncell(F.n);
-        if (F.e2 && W.c0uni && bm != 2) {
-            // from a zero correction: the first launch writes e without reading it
-            for (int s = 0; s < nu1;) {
-                const int k = nu1 - s >= 2 ? 2 : 1;
-                if (s == 0) {
-                    launch_fused<T>(ctx, W, bm, v, nullptr, F.e, F.r, k, 0, nullptr, nullptr);
-                } else {
-                    launch_fused<T>(ctx, W, bm, v, F.e, F.e2, F.r, k, 0, nullptr, nullptr);
-                    std::swap(F.e, F.e2);
-                }
-                s += k;
+    return l0;
+}
+
+// the down leg on level l: pre-smooth from a zero correction, then the
+// residual restricted to level l+1
+template <class T>
+static void vc_down(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L, double alpha, double beta,
+                    double bconst, int nu1, int l) {
+    Lev<T>& F = L[l];
+    LevV<T> v = view<T>(W, F, alpha, beta, bconst);
+    int64_t nc = ncell(F.n);
+    if (F.e2 && W.c0uni && bm != 2) {
+        // from a zero correction: the first launch writes e without reading it
+        for (int s = 0; s < nu1;) {
+            const int k = nu1 - s >= 2 ? 2 : 1;
+            if (s == 0) {
+                launch_fused<T>(ctx, W, bm, v, nullptr, F.e, F.r, k, 0, nullptr, nullptr);
+            } else {
+                launch_fused<T>(ctx, W, bm, v, F.e, F.e2, F.r, k, 0, nullptr, nullptr);
+                std::swap(F.e, F.e2);
             }
-            if (nu1 == 0) hipLaunchKernelGGL(k_zero<T>, grid1d(nc, 256), dim3(256), 0, ctx->stream, F.e, nc);
-        } else {
-            // from a zero correction: the first colour writes e without reading it
-            const bool zi = nu1 > 0 && streamed_sweep(W, F.n[0]);
-            if (!zi) hipLaunchKernelGGL(k_zero<T>, grid1d(nc, 256), dim3(256), 0, ctx->stream, F.e, nc);
-            for (int s = 0; s < nu1; s++) {
-                launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 0, l == 0, zi && s == 0);
-                launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 1, l == 0);
-            }
+            s += k;
         }
-        if (bm == 1) launch_resid_restrict<T, 1>(ctx, W, v, F, L[l + 1]);
-        else launch_resid_restrict<T, 0>(ctx, W, v, F, L[l + 1]);
+        if (nu1 == 0) hipLaunchKernelGGL(k_zero<T>, grid1d(nc, 256), dim3(256), 0, ctx->stream, F.e, nc);
+    } else {
+        // from a zero correction: the first colour writes e without reading it
+        const bool zi = nu1 > 0 && streamed_sweep(W, F.n[0]);
+        if (!zi) hipLaunchKernelGGL(k_zero<T>, grid1d(nc, 256), dim3(256), 0, ctx->stream, F.e, nc);
+        for (int s = 0; s < nu1; s++) {
+            launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 0, l == 0, zi && s == 0);
+            launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 1, l == 0);
+        }
     }
-    // coarse levels and bottom
-    {
-        VTail<T> a;
-        for (int l = l0; l < nl; l++) {
-            a.v[l - l0] = view<T>(W, L[l], alpha, beta, bconst);
-            a.e[l - l0] = L[l].e;
-            a.r[l - l0] = L[l].r;
-        }
-        Lev<T>& B = L[nl - 1];
-        a.cr = B.w;
-        a.cp = B.p;
-        a.cq = B.q;
-        a.nl = nl - l0;
-        int64_t tot = 0;
-        for (int l = 0; l < a.nl; l++) {
-            a.off[l] = (int)tot;
-            tot += ncell(L[l0 + l].n);
-        }
-        const bool lds = tot <= VC_LDS_CELLS;
-        a.nu1 = nu1;
-        a.nu2 = nu2;
+    if (bm == 1) launch_resid_restrict<T, 1>(ctx, W, v, F, L[l + 1]);
+    else launch_resid_restrict<T, 0>(ctx, W, v, F, L[l + 1]);
+}
+
+// the arguments of the coarse tail (levels l0 .. nlev-1); *lds: whether
+// its levels fit in LDS
+template <class T>
+static VTail<T> tail_args(const MGWork& W, std::vector<Lev<T>>& L, double alpha, double beta, double bconst,
+                          int nu1, int nu2, int l0, bool* lds) {
+    const int nl = W.nlev;
+    VTail<T> a;
+    for (int l = l0; l < nl; l++) {
+        a.v[l - l0] = view<T>(W, L[l], alpha, beta, bconst);
+        a.e[l - l0] = L[l].e;
+        a.r[l - l0] = L[l].r;
+    }
+    Lev<T>& B = L[nl - 1];
+    a.cr = B.w;
+    a.cp = B.p;
+    a.cq = B.q;
+    a.nl = nl - l0;
+    int64_t tot = 0;
+    for (int l = 0; l < a.nl; l++) {
+        a.off[l] = (int)tot;
+        tot += ncell(L[l0 + l].n);
+    }
+    *lds = tot <= VC_LDS_CELLS;
+    a.nu1 = nu1;
+    a.nu2 = nu2;
+    return a;
+}
+
+// levels l0 .. nlev-1 (and the bottom solve) in one workgroup
+template <class T>
+static void vc_tail(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L, double alpha, double beta,
+                    double bconst, int nu1, int nu2, int l0) {
+    bool lds;
+    const VTail<T> a = tail_args<T>(W, L, alpha, beta, bconst, nu1, nu2, l0, &lds);
 #define VD_TAIL(BMV, DIMV)                                                                                   \
     {                                                                                                        \
         if (lds)                                                                                             \
@@ -2091,56 +2107,74 @@ static bool vcycle(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L,
         else                                                                                                 \
             hipLaunchKernelGGL((k_vcycle_tail<T, BMV, DIMV, false>), dim3(1), dim3(1024), 0, ctx->stream, a); \
     }
-        if (bm == 1) {
-            if (W.dim == 3) VD_TAIL(1, 3) else VD_TAIL(1, 2)
-        } else {
-            if (W.dim == 3) VD_TAIL(0, 3) else VD_TAIL(0, 2)
-        }
+    if (bm == 1) {
+        if (W.dim == 3) VD_TAIL(1, 3) else VD_TAIL(1, 2)
+    } else {
+        if (W.dim == 3) VD_TAIL(0, 3) else VD_TAIL(0, 2)
+    }
 #undef VD_TAIL
+}
+
+// the up leg on level l: the correction of level l+1 prolonged into e, then
+// the post-smooth (on the finest level with xacc, x += e in its last pass);
+// returns whether x was updated
+template <class T>
+static bool vc_up(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L, double alpha, double beta,
+                  double bconst, int nu2, int l, double* xacc) {
+    bool accumulated = false;
+    Lev<T>& F = L[l];
+    Lev<T>& Cc = L[l + 1];
+    LevV<T> vc = view<T>(W, Cc, alpha, beta, bconst);
+    LevV<T> v = view<T>(W, F, alpha, beta, bconst);
+    if (W.dim == 3 && vc.c0uni && W.stream) {
+        const int ntx = (F.n[0] + PZ_X - 1) / PZ_X, nty = (F.n[1] + PZ_Y - 1) / PZ_Y;
+        const int zc = stream_zc(ctx, (const void*)&k_prolong_z<T>, PZ_X * PZ_Y, Cc.n, ntx * nty);
+        const dim3 g((unsigned)(ntx * nty * ((Cc.n[2] + zc - 1) / zc)));
+        hipLaunchKernelGGL((k_prolong_z<T>), g, dim3(PZ_X * PZ_Y), 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.e,
+                           zc, ntx, nty);
+    } else {
+        const int bx = F.n[0] > 64 ? 128 : 64;
+        const int gx = (F.n[0] + bx - 1) / bx;
+        dim3 g((unsigned)gx * F.n[1] * F.n[2]), b(bx);
+        if (W.dim == 3)
+            hipLaunchKernelGGL((k_prolong<T, 3>), g, b, 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.n[2], F.e,
+                               gx);
+        else
+            hipLaunchKernelGGL((k_prolong<T, 2>), g, b, 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.n[2], F.e,
+                               gx);
     }
-    // up
-    for (int l = l0 - 1; l >= 0; l--) {
-        Lev<T>& F = L[l];
-        Lev<T>& Cc = L[l + 1];
-        LevV<T> vc = view<T>(W, Cc, alpha, beta, bconst);
-        LevV<T> v = view<T>(W, F, alpha, beta, bconst);
-        if (W.dim == 3 && vc.c0uni && W.stream) {
-            const int ntx = (F.n[0] + PZ_X - 1) / PZ_X, nty = (F.n[1] + PZ_Y - 1) / PZ_Y;
-            const int zc = stream_zc(ctx, (const void*)&k_prolong_z<T>, PZ_X * PZ_Y, Cc.n, ntx * nty);
-            const dim3 g((unsigned)(ntx * nty * ((Cc.n[2] + zc - 1) / zc)));
-            hipLaunchKernelGGL((k_prolong_z<T>), g, dim3(PZ_X * PZ_Y), 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.e,
-                               zc, ntx, nty);
-        } else {
-            const int bx = F.n[0] > 64 ? 128 : 64;
-            const int gx = (F.n[0] + bx - 1) / bx;
-            dim3 g((unsigned)gx * F.n[1] * F.n[2]), b(bx);
-            if (W.dim == 3)
-                hipLaunchKernelGGL((k_prolong<T, 3>), g, b, 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.n[2], F.e,
-                                   gx);
-            else
-                hipLaunchKernelGGL((k_prolong<T, 2>), g, b, 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.n[2], F.e,
-                                   gx);
+    if (F.e2 && W.c0uni && bm != 2) {
+        for (int s = 0; s < nu2;) {
+            const int k = nu2 - s >= 2 ? 2 : 1;
+            if (l == 0 && xacc && s + k == nu2) {
+                // last post-smooth of the finest level: x += e in the same pass
+                launch_fused<T>(ctx, W, bm, v, F.e, F.e2, F.r, k, 0, nullptr, nullptr, xacc);
+                accumulated = true;
+            } else {
+                launch_fused<T>(ctx, W, bm, v, F.e, F.e2, F.r, k, 0, nullptr, nullptr);
+                std::swap(F.e, F.e2);
+            }
+            s += k;
         }
-        if (F.e2 && W.c0uni && bm != 2) {
-            for (int s = 0; s < nu2;) {
-                const int k = nu2 - s >= 2 ? 2 : 1;
-                if (l == 0 && xacc && s + k == nu2) {
-                    // last post-smooth of the finest level: x += e in the same pass
-                    launch_fused<T>(ctx, W, bm, v, F.e, F.e2, F.r, k, 0, nullptr, nullptr, xacc);
-                    accumulated = true;
-                } else {
-                    launch_fused<T>(ctx, W, bm, v, F.e, F.e2, F.r, k, 0, nullptr, nullptr);
-                    std::swap(F.e, F.e2);
-                }
-                s += k;
-            }
-        } else {
-            for (int s = 0; s < nu2; s++) {
-                launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 0, l == 0);
-                launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 1, l == 0);
-            }
+    } else {
+        for (int s = 0; s < nu2; s++) {
+            launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 0, l == 0);
+            launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 1, l == 0);
         }
     }
+    return accumulated;
+}
+
+// returns true when the finest post-smooth also added the correction to
+// xacc (x += e), so the caller skips that update
+template <class T>
+static bool vcycle(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L, double alpha,
+                   double beta, double bconst, int nu1, int nu2, double* xacc = nullptr) {
+    const int l0 = vc_tail_level<T>(W, L);
+    for (int l = 0; l < l0; l++) vc_down<T>(ctx, W, bm, L, alpha, beta, bconst, nu1, l);
+    vc_tail<T>(ctx, W, bm, L, alpha, beta, bconst, nu1, nu2, l0);
+    bool accumulated = false;
+    for (int l = l0 - 1; l >= 0; l--) accumulated |= vc_up<T>(ctx, W, bm, L, alpha, beta, bconst, nu2, l, xacc);
     return accumulated;
 }
 
