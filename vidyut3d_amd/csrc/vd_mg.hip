@@ -494,6 +494,14 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
         of = of + L.s1 * o1;
         of = of + L.s2 * o2;
     };
+    // ZI, BM 1: the first stage updates red cells from a zero correction,
+    // so every neighbour term is b * 0 and its stencil sum is a constant
+    // (formed once, in st1's order, for the same bits)
+    T of0 = T(0);
+    if constexpr (ZI && BM == 1) {
+        T dg0;
+        st1(0, 1, T(0), T(0), T(0), T(0), T(0), T(0), dg0, of0);
+    }
     const int tym = ty > 0 ? ty - 1 : ty, typ = ty < FZ_RY - 1 ? ty + 1 : ty;
     const int tend = zout + S;
     PF qa, qb;
@@ -516,7 +524,7 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
         }
         // publish the opposite-colour value of each stage's plane
 #pragma unroll
-        for (int s = 1; s <= S; s++) {
+        for (int s = (ZI && BM == 1) ? 2 : 1; s <= S; s++) {
             const int q_s = (k - s + 2 * NV) % NV;  // plane t+1-s
             nbuf[s - 1][ty][tx] = ((s - 1) & 1) ? vr[q_s] : vb[q_s];
         }
@@ -554,7 +562,10 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
             if constexpr (BM == 1) {
                 // times the reciprocal diagonal (within an ulp of the quotient)
                 const int pc = min(max(p, 0), L.nz - 1);
-                st1(u, pc, xm, xp, ym, yp, zlo_v, zhi_v, dg, of);
+                if (ZI && s == 1)
+                    of = of0;  // all neighbours 0
+                else
+                    st1(u, pc, xm, xp, ym, yp, zlo_v, zhi_v, dg, of);
                 const int zcl = pc == 0 ? 0 : (pc == L.nz - 1 ? 2 : 1);  // uniform
                 const T rd = rdbuf[zcl][u][threadIdx.x];
                 if (cs) {
