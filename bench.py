@@ -220,7 +220,7 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--precision", type=int, default=0, help="headline: 0 FP64 MG, 1 FP32 V-cycles in FP64 IR")
     ap.add_argument("--kernel", default="fused", choices=["fused", "gsrb", "transport", "chem", "resid", "none"])
-    ap.add_argument("--cpu-n", type=int, default=128)
+    ap.add_argument("--cpu-n", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--headline-only", action="store_true", help="skip the other configurations")
