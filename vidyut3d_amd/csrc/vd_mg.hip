@@ -845,6 +845,91 @@ __global__ void __launch_bounds__(RZ_X * RZ_Y) k_resid_z(LevV<double> L, const d
     }
 }
 
+// The residual of a 3-D level with a constant coefficient (BM 1) streamed
+// along z, one x-pair of cells per thread: the pair, the rows beside it and
+// the rhs as 16-byte loads, the two outer x-neighbours as 8-byte loads --
+// half the load instructions of one cell per thread.  Per cell the
+// arithmetic of k_resid_z (stencil_nbf, term for term).  Even nx.
+constexpr int RZ2_X = 32, RZ2_Y = 8;  // pairs x rows per block
+template <bool BN, bool C0U>
+__global__ void __launch_bounds__(RZ2_X * RZ2_Y) k_resid_z2(LevV<double> L, const double* __restrict__ x,
+                                                            const double* __restrict__ f, double* __restrict__ r,
+                                                            int zc, int ntx, int nty,
+                                                            unsigned long long* __restrict__ slots) {
+    constexpr int NV = 1 + (BN ? 1 : 0);
+    __shared__ double red[RZ2_X * RZ2_Y / 64][NV];
+    const unsigned lb = lblock();
+    const int ntile = ntx * nty;
+    const int tile = (int)(lb % (unsigned)ntile), zb = (int)(lb / (unsigned)ntile);
+    const int i0 = 2 * ((tile % ntx) * RZ2_X + (int)threadIdx.x % RZ2_X);
+    const int j = (tile / ntx) * RZ2_Y + (int)threadIdx.x / RZ2_X;
+    const int z0 = zb * zc, z1 = min(z0 + zc, L.nz);
+    const bool own = i0 < L.nx && j < L.ny;  // both cells of the pair (nx even)
+    const int ic = min(i0, L.nx - 2), jc = min(j, L.ny - 1);
+    const int pstr = L.nx * L.ny, c2 = ic + L.nx * jc;
+    // the left neighbour of cell ic and the right one of ic+1 (offsets from
+    // ic), and the rows: wrapped on a periodic side, a cell of the pair
+    // (value unused) on a wall
+    const int oxl = ic > 0 ? -1 : (L.per0 ? L.nx - 1 : 0);
+    const int oxh = ic + 1 < L.nx - 1 ? 2 : (L.per0 ? 2 - L.nx : 1);
+    const int oyl = jc > 0 ? -L.nx : (L.per1 ? (L.ny - 1) * L.nx : 0);
+    const int oyh = jc < L.ny - 1 ? L.nx : (L.per1 ? (1 - L.ny) * L.nx : 0);
+    auto zw = [&](int k) {
+        return k < 0 ? (L.per2 ? k + L.nz : 0) : (k >= L.nz ? (L.per2 ? k - L.nz : L.nz - 1) : k);
+    };
+    auto ld2 = [](const double* p) { return *(const double2*)p; };  // even offsets: 16-byte aligned
+    double2 xm = ld2(x + c2 + pstr * zw(z0 - 1)), xc = ld2(x + c2 + pstr * z0), xp = ld2(x + c2 + pstr * zw(z0 + 1));
+    double m[NV];
+#pragma unroll
+    for (int q = 0; q < NV; q++) m[q] = 0.0;
+    for (int k = z0; k < z1; k++) {
+        const int ck = c2 + pstr * k;
+        const double2 xn = ld2(x + c2 + pstr * zw(k + 2));  // two planes ahead
+        const double xl = x[ck + oxl], xh = x[ck + oxh];
+        const double2 yl = ld2(x + ck + oyl), yh = ld2(x + ck + oyh);
+        const double2 fv = ld2(f + ck);
+        double v[2];
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            auto nb = [&](int d, int side) -> double {
+                if (d == 0) return c == 0 ? (side ? xc.y : xl) : (side ? xh : xc.x);
+                if (d == 1) return side ? (c ? yh.y : yh.x) : (c ? yl.y : yl.x);
+                return side ? (c ? xp.y : xp.x) : (c ? xm.y : xm.x);
+            };
+            auto fc = [&](int d, double& blo, double& bhi) { face_coef<double, 1, 3>(L, d, ic + c, jc, k, blo, bhi); };
+            double dg, of;
+            stencil_nbf<double, 3, decltype(nb), decltype(fc), C0U>(L, ic + c, jc, k, nb, fc, dg, of);
+            v[c] = (c ? fv.y : fv.x) - (dg * (c ? xc.y : xc.x) - of);
+        }
+        if (r && own) *(double2*)(r + ck) = make_double2(v[0], v[1]);
+        if (own) {
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                double a[NV];
+                a[0] = fabs(v[c]);
+                if (BN) a[NV - 1] = fabs(c ? fv.y : fv.x);
+#pragma unroll
+                for (int q = 0; q < NV; q++) m[q] = fmax(m[q], a[q] != a[q] ? __builtin_huge_val() : a[q]);
+            }
+        }
+        xm = xc;
+        xc = xp;
+        xp = xn;
+    }
+#pragma unroll
+    for (int q = 0; q < NV; q++) {
+        for (int o = 32; o > 0; o >>= 1) m[q] = fmax(m[q], __shfl_xor(m[q], o, 64));
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][q] = m[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < NV) {
+        const int q = (int)threadIdx.x;
+        double mm = red[0][q];
+        for (int w = 1; w < RZ2_X * RZ2_Y / 64; w++) mm = fmax(mm, red[w][q]);
+        atomic_max_nonneg(slots + ((size_t)q * NSLOT + lb % NSLOT) * SLOT_STRIDE, mm);
+    }
+}
+
 // One colour of a red-black sweep of a 3-D level (even nx) streamed along
 // z.  A thread owns the x-pair (i0, i0+1) of one row and walks up to zc
 // planes; on plane k it updates the cell of the pair whose colour is
@@ -2306,7 +2391,22 @@ static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double
     double* part = W.part;
     double* res = write_res ? W.res : nullptr;  // the norm alone when no V-cycle follows
     kt_begin(ctx, VD_KT_RESID);
-    if (W.dim == 3 && W.stream) {
+    if (W.dim == 3 && W.stream && bm == 1 && !fmax && W.n[0] % 2 == 0) {
+        const int ntx = (W.n[0] / 2 + RZ2_X - 1) / RZ2_X, nty = (W.n[1] + RZ2_Y - 1) / RZ2_Y;
+#define VD_RESIDZ2(BNV, C0UV)                                                                                 \
+    {                                                                                                         \
+        const int zc = stream_zc(ctx, (const void*)&k_resid_z2<BNV, C0UV>, RZ2_X * RZ2_Y, W.n, ntx * nty);    \
+        const dim3 gz((unsigned)(ntx * nty * ((W.n[2] + zc - 1) / zc)));                                      \
+        hipLaunchKernelGGL((k_resid_z2<BNV, C0UV>), gz, dim3(RZ2_X * RZ2_Y), 0, ctx->stream, v0, W.x, W.f, res, \
+                           zc, ntx, nty, sl);                                                                 \
+    }
+        if (bnorm) {
+            if (v0.c0uni) VD_RESIDZ2(true, true) else VD_RESIDZ2(true, false)
+        } else {
+            if (v0.c0uni) VD_RESIDZ2(false, true) else VD_RESIDZ2(false, false)
+        }
+#undef VD_RESIDZ2
+    } else if (W.dim == 3 && W.stream) {
         const int ntx = (W.n[0] + RZ_X - 1) / RZ_X, nty = (W.n[1] + RZ_Y - 1) / RZ_Y;
 #define VD_RESIDZ3(BMV, BNV, FMV)                                                                             \
     {                                                                                                         \
