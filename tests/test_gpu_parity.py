@@ -556,3 +556,24 @@ def test_handles_outliving_their_device_are_released_with_it():
     d.close()
     assert not a.ptr and not m.ptr and not s.ptr
     del a, m, s
+
+
+def test_memset_zero_clears_exactly_the_range(dev):
+    """vd_memset_zero on ranges of every alignment and length: the bytes in
+    the range become 0, every byte around it keeps its value."""
+    nb = (1 << 20) + 64
+    base = dev.alloc(nb)
+    try:
+        rng = np.random.default_rng(7)
+        for off, n in [(0, 0), (3, 1), (1, 15), (0, 16), (5, 17), (16, 31), (7, 1000), (13, (1 << 20) + 3),
+                       (0, 1 << 20)]:
+            init = rng.integers(1, 256, nb, dtype=np.uint8)
+            dev.upload(base, init)
+            native.check(native.lib().vd_memset_zero(dev.ptr, C.c_void_p(base + off), C.c_size_t(n)),
+                         "vd_memset_zero")
+            got = dev.download(base, np.empty(nb, np.uint8))
+            want = init.copy()
+            want[off:off + n] = 0
+            assert np.array_equal(got, want), (off, n)
+    finally:
+        dev.free(base)

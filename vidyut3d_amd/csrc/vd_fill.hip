@@ -17,10 +17,20 @@ __global__ void k_fill_value_f(float* __restrict__ p, int64_t n, float v) {
     for (; i < n; i += stride) p[i] = v;
 }
 
+// zeroes n bytes at p: the bytes before the first 16-byte boundary and
+// after the last one one per lane, the aligned middle in 16-byte stores
+// (the grid has at least 16 lanes, so every head and tail byte has one)
 __global__ void k_fill_bytes(unsigned char* __restrict__ p, int64_t n) {
-    int64_t i = (int64_t)lblock() * blockDim.x + threadIdx.x;
-    int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (; i < n; i += stride) p[i] = 0;
+    const int64_t i = (int64_t)lblock() * blockDim.x + threadIdx.x;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    int64_t head = (int64_t)((16u - ((uintptr_t)p & 15u)) & 15u);
+    if (head > n) head = n;
+    const int64_t nv = (n - head) >> 4;
+    uint4* __restrict__ q = reinterpret_cast<uint4*>(p + head);
+    for (int64_t v = i; v < nv; v += stride) q[v] = make_uint4(0u, 0u, 0u, 0u);
+    const int64_t tail0 = head + (nv << 4);
+    if (i < head) p[i] = 0;
+    if (i < n - tail0) p[tail0 + i] = 0;
 }
 
 static inline dim3 grid_stride(int64_t n, int block) {
@@ -234,7 +244,7 @@ extern "C" int vd_fill_patch(vd_ctx* ctx, const vd_array4* src, vd_array4* dst, 
 
 extern "C" int vd_memset_zero_kernel(vd_ctx* ctx, void* dst, size_t bytes) {
     if (!bytes) return VD_OK;
-    hipLaunchKernelGGL(vd::k_fill_bytes, vd::grid_stride((int64_t)bytes, 256), dim3(256), 0,
+    hipLaunchKernelGGL(vd::k_fill_bytes, vd::grid_stride((int64_t)(bytes >> 4) + 1, 256), dim3(256), 0,
                        ctx->stream, (unsigned char*)dst, (int64_t)bytes);
     VD_CHECK_LAUNCH();
     return VD_OK;
