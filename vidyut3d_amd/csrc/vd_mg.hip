@@ -2754,13 +2754,9 @@ struct GradArgs {
     const double* rf[2];
 };
 
-__global__ void k_grad_faces(GradArgs a) {
-    int n0 = a.nx + (a.d == 0), n1 = a.ny + (a.d == 1), n2 = a.nz + (a.d == 2);
-    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)n0 * n1 * n2) return;
-    int i = (int)(t % n0);
-    int64_t r = t / n0;
-    int j = (int)(r % n1), k = (int)(r / n1);
+// the gradient on face (i, j, k) of direction a.d (face f[d] lies between
+// cells f[d]-1 and f[d]), times a.sign
+__device__ __forceinline__ double grad_face(const GradArgs& a, int i, int j, int k) {
     int f[3] = {i, j, k};
     int n[3] = {a.nx, a.ny, a.nz};
     const int st = a.d == 0 ? 1 : (a.d == 1 ? a.nx : a.nx * a.ny);
@@ -2803,33 +2799,49 @@ __global__ void k_grad_faces(GradArgs a) {
         }
     }
     if (a.sign != 1.0) gv *= a.sign;
-    a.out(i + a.lo.x, j + a.lo.y, k + a.lo.z) = gv;
+    return gv;
+}
+
+__global__ void k_grad_faces(GradArgs a) {
+    int n0 = a.nx + (a.d == 0), n1 = a.ny + (a.d == 1), n2 = a.nz + (a.d == 2);
+    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)n0 * n1 * n2) return;
+    int i = (int)(t % n0);
+    int64_t r = t / n0;
+    int j = (int)(r % n1), k = (int)(r / n1);
+    a.out(i + a.lo.x, j + a.lo.y, k + a.lo.z) = grad_face(a, i, j, k);
+}
+
+static GradArgs grad_args(const MGWork& W, const int tlo[3], const int thi[3], vd_array4* out, int d,
+                          const vd_box* box, double sign) {
+    GradArgs a;
+    a.x = W.x;
+    a.out = arr(out);
+    a.d = d;
+    a.dim = W.dim;
+    a.nx = W.n[0];
+    a.ny = W.n[1];
+    a.nz = W.n[2];
+    a.lo = make_int3(box->lo[0], box->lo[1], box->lo[2]);
+    a.per = W.per[d];
+    a.tlo = tlo[d];
+    a.thi = thi[d];
+    a.dx = W.dx[d];
+    a.sign = sign;
+    for (int s = 0; s < 2; s++) {
+        a.bcv[s] = W.bcv[d][s];
+        bool rob = (s ? thi[d] : tlo[d]) == VD_ROBINBC;
+        a.ra[s] = rob ? W.rob[d][s][0] : nullptr;
+        a.rb[s] = rob ? W.rob[d][s][1] : nullptr;
+        a.rf[s] = rob ? W.rob[d][s][2] : nullptr;
+    }
+    return a;
 }
 
 static int grad_faces(vd_ctx* ctx, MGWork& W, const int tlo[3], const int thi[3], vd_array4 out[3],
                       const vd_box* box, double sign) {
     for (int d = 0; d < W.dim; d++) {
-        GradArgs a;
-        a.x = W.x;
-        a.out = arr(&out[d]);
-        a.d = d;
-        a.dim = W.dim;
-        a.nx = W.n[0];
-        a.ny = W.n[1];
-        a.nz = W.n[2];
-        a.lo = make_int3(box->lo[0], box->lo[1], box->lo[2]);
-        a.per = W.per[d];
-        a.tlo = tlo[d];
-        a.thi = thi[d];
-        a.dx = W.dx[d];
-        a.sign = sign;
-        for (int s = 0; s < 2; s++) {
-            a.bcv[s] = W.bcv[d][s];
-            bool rob = (s ? thi[d] : tlo[d]) == VD_ROBINBC;
-            a.ra[s] = rob ? W.rob[d][s][0] : nullptr;
-            a.rb[s] = rob ? W.rob[d][s][1] : nullptr;
-            a.rf[s] = rob ? W.rob[d][s][2] : nullptr;
-        }
+        GradArgs a = grad_args(W, tlo, thi, &out[d], d, box, sign);
         int64_t nf = nfacef(W.n, d);
         hipLaunchKernelGGL(k_grad_faces, grid1d(nf, 256), dim3(256), 0, ctx->stream, a);
     }
@@ -2921,29 +2933,43 @@ __global__ void k_pot_rhs(PotRhsArgs a, double* __restrict__ f) {
 
 struct PotPostArgs {
     const double* x;
-    Arr<double> ef0, ef1, ef2, phi;
+    GradArgs g[3];  // the face gradients times -1 (E) and their arrays
+    Arr<double> phi;
     int ns, dim;
     int3 lo;
     int nx, ny, nz;
 };
 
-// POT copy, E cell averages (average_face_to_cellcenter) and E/N in Td
-// (PotentialSolve.cpp:340-379)
+// E on the faces (MLMG::getGradSolution times -1), the POT copy, E cell
+// averages (average_face_to_cellcenter) and E/N in Td
+// (PotentialSolve.cpp:340-379) in one pass over the cells: a cell writes
+// its low face in each direction, the last cell of a line also the high
+// face, and the cell average takes both faces as computed, so the face
+// arrays are written once and not read back
+template <int D>
+__device__ __forceinline__ double pot_face_pair(const GradArgs& g, const int c[3], const int n[3], int gi, int gj,
+                                                int gk) {
+    const double flo = grad_face(g, c[0], c[1], c[2]);
+    const double fhi = grad_face(g, c[0] + (D == 0), c[1] + (D == 1), c[2] + (D == 2));
+    g.out(gi, gj, gk) = flo;
+    if (c[D] == n[D] - 1) g.out(gi + (D == 0), gj + (D == 1), gk + (D == 2)) = fhi;
+    return 0.5 * (flo + fhi);
+}
+
 __global__ void k_pot_post(PotPostArgs a) {
-    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
-    if (t >= (int64_t)a.nx * a.ny * a.nz) return;
-    int i = (int)(t % a.nx);
-    int64_t q = t / a.nx;
-    int j = (int)(q % a.ny), k = (int)(q / a.ny);
-    i += a.lo.x;
-    j += a.lo.y;
-    k += a.lo.z;
+    // the box holds fewer than 2^31 cells (32-bit cell offsets)
+    const int t = (int)((int64_t)lblock() * blockDim.x + threadIdx.x);
+    if (t >= a.nx * a.ny * a.nz) return;
+    const int q = t / a.nx;
+    const int c[3] = {t - q * a.nx, q % a.ny, q / a.ny};
+    const int n[3] = {a.nx, a.ny, a.nz};
+    const int i = c[0] + a.lo.x, j = c[1] + a.lo.y, k = c[2] + a.lo.z;
     const int ns = a.ns;
     a.phi(i, j, k, VD_POT(ns)) = a.x[t];
     double e[3] = {0.0, 0.0, 0.0};
-    e[0] = 0.5 * (a.ef0(i, j, k) + a.ef0(i + 1, j, k));
-    if (a.dim > 1) e[1] = 0.5 * (a.ef1(i, j, k) + a.ef1(i, j + 1, k));
-    if (a.dim > 2) e[2] = 0.5 * (a.ef2(i, j, k) + a.ef2(i, j, k + 1));
+    e[0] = pot_face_pair<0>(a.g[0], c, n, i, j, k);
+    if (a.dim > 1) e[1] = pot_face_pair<1>(a.g[1], c, n, i, j, k);
+    if (a.dim > 2) e[2] = pot_face_pair<2>(a.g[2], c, n, i, j, k);
     for (int d = 0; d < a.dim; d++) a.phi(i, j, k, VD_EFX(ns) + d) = e[d];
     double nd = 0.0;
     for (int sp = 0; sp < ns; sp++) nd += a.phi(i, j, k, sp);
@@ -3027,12 +3053,11 @@ int solve_potential(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array
                         prm->linsolve_reltol, prm->linsolve_abstol, prm->linsolve_maxiter,
                         prm->mg_pre_smooth, prm->mg_post_smooth, info, true);
     if (st != VD_OK && st != VD_ENOCONV) return st;
-    VD_TRY(grad_faces(ctx, *W, prm->pot_bc_lo, prm->pot_bc_hi, ef, box, -1.0));
     PotPostArgs pa;
     pa.x = W->x;
-    pa.ef0 = arr(&ef[0]);
-    pa.ef1 = g->dim > 1 ? arr(&ef[1]) : pa.ef0;
-    pa.ef2 = g->dim > 2 ? arr(&ef[2]) : pa.ef0;
+    for (int d = 0; d < 3; d++)
+        pa.g[d] = grad_args(*W, prm->pot_bc_lo, prm->pot_bc_hi, &ef[d < g->dim ? d : 0], d < g->dim ? d : 0, box,
+                            -1.0);
     pa.phi = arr(phi);
     pa.ns = m->hdev.nspec;
     pa.dim = g->dim;
