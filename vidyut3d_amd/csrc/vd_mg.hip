@@ -849,11 +849,13 @@ __global__ void __launch_bounds__(RZ_X * RZ_Y) k_resid_z(LevV<double> L, const d
 // along z, one x-pair of cells per thread: the pair, the rows beside it and
 // the rhs as 16-byte loads, the two outer x-neighbours as 8-byte loads --
 // half the load instructions of one cell per thread.  Per cell the
-// arithmetic of k_resid_z (stencil_nbf, term for term).  Even nx.
+// arithmetic of k_resid_z (stencil_nbf, term for term).  Even nx.  RT
+// float stores the residual rounded to FP32, the finest rhs of an FP32
+// V-cycle.
 constexpr int RZ2_X = 32, RZ2_Y = 8;  // pairs x rows per block
-template <bool BN, bool C0U>
+template <bool BN, bool C0U, class RT = double>
 __global__ void __launch_bounds__(RZ2_X * RZ2_Y) k_resid_z2(LevV<double> L, const double* __restrict__ x,
-                                                            const double* __restrict__ f, double* __restrict__ r,
+                                                            const double* __restrict__ f, RT* __restrict__ r,
                                                             int zc, int ntx, int nty,
                                                             unsigned long long* __restrict__ slots) {
     constexpr int NV = 1 + (BN ? 1 : 0);
@@ -901,7 +903,12 @@ __global__ void __launch_bounds__(RZ2_X * RZ2_Y) k_resid_z2(LevV<double> L, cons
             stencil_nbf<double, 3, decltype(nb), decltype(fc), C0U>(L, ic + c, jc, k, nb, fc, dg, of);
             v[c] = (c ? fv.y : fv.x) - (dg * (c ? xc.y : xc.x) - of);
         }
-        if (r && own) *(double2*)(r + ck) = make_double2(v[0], v[1]);
+        if (r && own) {
+            if constexpr (std::is_same<RT, float>::value)
+                *(float2*)(r + ck) = make_float2((float)v[0], (float)v[1]);
+            else
+                *(double2*)(r + ck) = make_double2(v[0], v[1]);
+        }
         if (own) {
 #pragma unroll
             for (int c = 0; c < 2; c++) {
@@ -2379,8 +2386,12 @@ static int read_groups(vd_ctx* ctx, MGWork& W, int ngroups, double* out) {
 // its max-norm; with bnorm also the max-norm of f (the rhs with boundary
 // data), with fmax (cell coefficient, bm 2) the max face coefficient per
 // direction -- all from the one pass over the level
+// With res32 the residual goes straight into res32 in FP32 instead, where
+// the kernel allows (*in32 says whether it did).
 static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double* nrm_out,
-                  double* bnorm = nullptr, bool write_res = true, double* fmax = nullptr) {
+                  double* bnorm = nullptr, bool write_res = true, double* fmax = nullptr,
+                  float* res32 = nullptr, bool* in32 = nullptr) {
+    if (in32) *in32 = false;
     int64_t nc = ncell(W.n);
     const int gx = (W.n[0] + 255) / 256;
     const int nblk = gx * W.n[1] * W.n[2];
@@ -2397,8 +2408,14 @@ static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double
     {                                                                                                         \
         const int zc = stream_zc(ctx, (const void*)&k_resid_z2<BNV, C0UV>, RZ2_X * RZ2_Y, W.n, ntx * nty);    \
         const dim3 gz((unsigned)(ntx * nty * ((W.n[2] + zc - 1) / zc)));                                      \
-        hipLaunchKernelGGL((k_resid_z2<BNV, C0UV>), gz, dim3(RZ2_X * RZ2_Y), 0, ctx->stream, v0, W.x, W.f, res, \
-                           zc, ntx, nty, sl);                                                                 \
+        if (res && res32) {                                                                                   \
+            hipLaunchKernelGGL((k_resid_z2<BNV, C0UV, float>), gz, dim3(RZ2_X * RZ2_Y), 0, ctx->stream, v0,   \
+                               W.x, W.f, res32, zc, ntx, nty, sl);                                            \
+            if (in32) *in32 = true;                                                                           \
+        } else {                                                                                              \
+            hipLaunchKernelGGL((k_resid_z2<BNV, C0UV>), gz, dim3(RZ2_X * RZ2_Y), 0, ctx->stream, v0, W.x,     \
+                               W.f, res, zc, ntx, nty, sl);                                                   \
+        }                                                                                                     \
     }
         if (bnorm) {
             if (v0.c0uni) VD_RESIDZ2(true, true) else VD_RESIDZ2(true, false)
@@ -2616,8 +2633,10 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
         W.c0_valid = !robin;
         VD_CHECK_LAUNCH();
     }
-    // the first V-cycle's finest rhs: the stored residual, or f when x is 0
-    bool rhs_is_f = false;
+    // the first V-cycle's finest rhs: the stored residual, or f when x is 0;
+    // for FP32 V-cycles the residual pass may store it in FP32 directly
+    bool rhs_is_f = false, r_in32 = false;
+    float* r32p = W.prec == 1 && !W.L32.empty() ? W.L32[0].r : nullptr;
     if (smooth_mode != 2) {
         if (x_zero && !smooth_only) {
             // r = f - L 0 = f bit for bit (every stencil term is 0)
@@ -2627,7 +2646,7 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
             r0 = bnorm;
             rhs_is_f = true;
         } else {
-            VD_TRY(resid0(ctx, W, bm, v0, &r0, &bnorm, !smooth_only));
+            VD_TRY(resid0(ctx, W, bm, v0, &r0, &bnorm, !smooth_only, nullptr, r32p, &r_in32));
         }
     }
     double target = std::max(abstol, reltol * std::max(bnorm, r0));
@@ -2658,15 +2677,16 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
                     hipLaunchKernelGGL(k_axpy_to_double<double>, grid1d(nc, 256), dim3(256), 0, ctx->stream,
                                        W.x, W.L64[0].e, nc);
             } else {
-                hipLaunchKernelGGL((k_convert<double, float>), grid1d(nc, 256), dim3(256), 0, ctx->stream,
-                                   rhs_is_f ? W.f : W.res, W.L32[0].r, nc);
+                if (!r_in32)
+                    hipLaunchKernelGGL((k_convert<double, float>), grid1d(nc, 256), dim3(256), 0, ctx->stream,
+                                       rhs_is_f ? W.f : W.res, W.L32[0].r, nc);
                 if (!vcycle<float>(ctx, W, bm, W.L32, alpha, beta, bconst, nu1, nu2, W.x))
                     hipLaunchKernelGGL(k_axpy_to_double<float>, grid1d(nc, 256), dim3(256), 0, ctx->stream,
                                        W.x, W.L32[0].e, nc);
             }
             VD_CHECK_LAUNCH();
             rhs_is_f = false;
-            VD_TRY(resid0(ctx, W, bm, v0, &rn, nullptr, !smooth_only));
+            VD_TRY(resid0(ctx, W, bm, v0, &rn, nullptr, !smooth_only, nullptr, r32p, &r_in32));
             if (rn <= target) break;
         }
         if (it > maxiter) {
