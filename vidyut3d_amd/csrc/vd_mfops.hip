@@ -9,6 +9,21 @@ struct BoxIter {
     int nx, ny, nz;
 };
 
+// cell (i, j, k) of the box for the linear index t (x fastest); boxes hold
+// fewer than 2^31 cells (32-bit cell offsets), so 32-bit unsigned division
+// suffices
+__device__ __forceinline__ void box_cell(const BoxIter& it, unsigned t, int& i, int& j, int& k) {
+    const unsigned q = t / (unsigned)it.nx;
+    i = (int)(t - q * (unsigned)it.nx) + it.lo.x;
+    j = (int)(q % (unsigned)it.ny) + it.lo.y;
+    k = (int)(q / (unsigned)it.ny) + it.lo.z;
+}
+
+static int box_too_large(int64_t cells) {
+    set_error("box of %lld cells: at most 2^31 - 1 per box", (long long)cells);
+    return VD_EINVAL;
+}
+
 static BoxIter box_iter(const vd_box* b, const vd_array4* a) {
     BoxIter it;
     it.lo = make_int3(b->lo[0], b->lo[1], b->lo[2]);
@@ -20,25 +35,20 @@ static BoxIter box_iter(const vd_box* b, const vd_array4* a) {
 }
 
 __global__ void k_setval(Arr<double> a, BoxIter it, int scomp, double v) {
-    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)it.nx * it.ny * it.nz) return;
-    int i = (int)(t % it.nx);
-    int64_t q = t / it.nx;
-    int j = (int)(q % it.ny), k = (int)(q / it.ny);
-    a(i + it.lo.x, j + it.lo.y, k + it.lo.z, scomp + (int)blockIdx.y) = v;
+    int i, j, k;
+    box_cell(it, (unsigned)t, i, j, k);
+    a(i, j, k, scomp + (int)blockIdx.y) = v;
 }
 
 // dst = a*x + b*y  (b == 0 and y == x gives a scaled copy)
 __global__ void k_lincomb(Arr<double> dst, Arr<double> x, Arr<double> y, BoxIter it, int xc, int yc,
                           int dc, double a, double b, int copy) {
-    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)it.nx * it.ny * it.nz) return;
-    int i = (int)(t % it.nx);
-    int64_t q = t / it.nx;
-    int j = (int)(q % it.ny), k = (int)(q / it.ny);
-    i += it.lo.x;
-    j += it.lo.y;
-    k += it.lo.z;
+    int i, j, k;
+    box_cell(it, (unsigned)t, i, j, k);
     int n = (int)blockIdx.y;
     if (copy)
         dst(i, j, k, dc + n) = x(i, j, k, xc + n);
@@ -55,6 +65,7 @@ extern "C" int vd_setval(vd_ctx* ctx, vd_array4* a, const vd_box* box, int scomp
     if (!ncomp) return VD_OK;
     BoxIter it = box_iter(box, a);
     int64_t tot = (int64_t)it.nx * it.ny * it.nz;
+    if (tot >= ((int64_t)1 << 31)) return box_too_large(tot);
     hipLaunchKernelGGL(k_setval, dim3((unsigned)((tot + 255) / 256), ncomp), dim3(256), 0, ctx->stream,
                        arr(a), it, scomp, v);
     VD_CHECK_LAUNCH();
@@ -68,6 +79,7 @@ extern "C" int vd_copy(vd_ctx* ctx, const vd_array4* src, vd_array4* dst, const 
     if (!ncomp) return VD_OK;
     BoxIter it = box_iter(box, dst);
     int64_t tot = (int64_t)it.nx * it.ny * it.nz;
+    if (tot >= ((int64_t)1 << 31)) return box_too_large(tot);
     hipLaunchKernelGGL(k_lincomb, dim3((unsigned)((tot + 255) / 256), ncomp), dim3(256), 0, ctx->stream,
                        arr(dst), arr(src), arr(src), it, scomp, scomp, dcomp, 1.0, 0.0, 1);
     VD_CHECK_LAUNCH();
@@ -80,6 +92,7 @@ extern "C" int vd_lincomb(vd_ctx* ctx, vd_array4* dst, double a, const vd_array4
     if (!ncomp) return VD_OK;
     BoxIter it = box_iter(box, dst);
     int64_t tot = (int64_t)it.nx * it.ny * it.nz;
+    if (tot >= ((int64_t)1 << 31)) return box_too_large(tot);
     hipLaunchKernelGGL(k_lincomb, dim3((unsigned)((tot + 255) / 256), ncomp), dim3(256), 0, ctx->stream,
                        arr(dst), arr(x), arr(y), it, xcomp, ycomp, dcomp, a, b, 0);
     VD_CHECK_LAUNCH();
@@ -88,12 +101,11 @@ extern "C" int vd_lincomb(vd_ctx* ctx, vd_array4* dst, double a, const vd_array4
 
 namespace vd {
 __global__ void k_clamp_min(Arr<double> a, BoxIter it, int comp, double lo) {
-    int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)it.nx * it.ny * it.nz) return;
-    int i = (int)(t % it.nx);
-    int64_t q = t / it.nx;
-    int j = (int)(q % it.ny), k = (int)(q / it.ny);
-    double& v = a(i + it.lo.x, j + it.lo.y, k + it.lo.z, comp);
+    int i, j, k;
+    box_cell(it, (unsigned)t, i, j, k);
+    double& v = a(i, j, k, comp);
     if (v < lo) v = lo;
 }
 }  // namespace vd
@@ -103,6 +115,7 @@ extern "C" int vd_clamp_min(vd_ctx* ctx, vd_array4* a, const vd_box* box, int co
     if (!ctx || !a || !box || comp < 0 || comp >= a->ncomp) return VD_EINVAL;
     BoxIter it = box_iter(box, a);
     int64_t tot = (int64_t)it.nx * it.ny * it.nz;
+    if (tot >= ((int64_t)1 << 31)) return box_too_large(tot);
     hipLaunchKernelGGL(k_clamp_min, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, arr(a),
                        it, comp, lo);
     VD_CHECK_LAUNCH();
