@@ -150,3 +150,17 @@ def test_no_tags_no_boxes_and_bad_parameters_are_rejected():
         native.make_new_grids(tags, g, native.regrid_params(blocking_factor=6, ref_ratio=4))
     with pytest.raises(native.VidyutError):
         native.make_new_grids(tags, g, native.regrid_params(blocking_factor=4, max_grid_size=10))
+
+
+def test_ratio_four_in_3d_keeps_the_invariants():
+    """Ratio 4 with blocking factor 8: tags clustered on a 2-cell coarsening,
+    boxes on the new level aligned to 8 and chopped to 16."""
+    g = abi.make_geom([16, 16, 16], [0, 0, 0], [1, 1, 1], [0, 1, 0])
+    tags = np.zeros((16, 16, 16), np.int8)
+    tags[3:9, 0:2, 5:12] = 1   # touches y = 0 (periodic in y)
+    tags[12:14, 10:15, 1:3] = 1
+    prm = native.regrid_params(ref_ratio=4, blocking_factor=8, max_grid_size=16, n_error_buf=1, grid_eff=0.7)
+    bl = native.make_new_grids(tags, g, prm)
+    check_invariants(tags, g, prm, bl)
+    # the buffered y-range of the first blob wraps to the top of the domain
+    assert any(b.hi[1] == 63 for b in bl)
