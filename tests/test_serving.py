@@ -2,7 +2,8 @@
 import numpy as np
 import pytest
 
-from vidyut3d_amd import serve
+from oracle import oracle as O
+from vidyut3d_amd import compare, serve
 
 
 def test_draw_requests_sizes_and_rates():
@@ -51,5 +52,13 @@ def test_results_are_the_same_alone_and_in_the_mix():
             s1.close()
         assert np.isfinite(r.state).all()
         assert np.array_equal(alone.state, r.state), r.rid
+        # the served (FP32 V-cycle) result against the all-FP64 oracle
+        c = r.make_case(0)
+        ref, _ = O.evolve(c, r.nsteps)
+        errs = compare.state_errors(ref, r.state, c.mech.nspec, c.geom.dim)
+        worst = max(errs.values())
+        print("request %d (%dD %d^%d): worst field rel L2 vs FP64 oracle %.2e"
+              % (r.rid, r.dim, r.n, r.dim, worst), flush=True)
+        assert worst <= 1e-8, (r.rid, errs)
     out = serve.summarize(reqs)
     assert out["p99_time_to_solution_s"] >= out["p50_time_to_solution_s"] > 0

@@ -16,9 +16,11 @@
 //    CG bottom solve in one workgroup
 //  * stop when ||r||_inf <= max(abstol, reltol*max(||b||_inf, ||r0||_inf)),
 //    b = rhs with the boundary data folded in
-//  * precision 1: the V-cycle runs in FP32 on FP32 copies of the
-//    hierarchy; the finest-level residual, the update x += e and the
-//    stopping test stay FP64 (iterative refinement).
+//  * precision 1: the V-cycle (smoothing, restriction, prolongation)
+//    runs in FP32 on FP32 copies of the hierarchy; the finest-level
+//    residual, the update x += e, the stopping test and the bottom solve
+//    (CG on the bottom operator coarsened in FP64) stay FP64 (iterative
+//    refinement).
 #include <algorithm>
 #include <atomic>
 #include <cmath>
@@ -93,6 +95,10 @@ struct MGWork {
     double c0u[3][2] = {{0, 0}, {0, 0}, {0, 0}};
     std::vector<Lev<double>> L64;
     std::vector<Lev<float>> L32;
+    // FP32 V-cycles: the face and boundary coefficients of levels 1 ..
+    // nlev-1 coarsened in FP64 (level 0 is bf0 / c00), and FP64 CG arrays
+    // of the bottom level, for the FP64 bottom solve
+    std::vector<Lev<double>> C64;
 };
 
 void mgwork_free(MGWork* w) {
@@ -1383,10 +1389,13 @@ __global__ void __launch_bounds__(PZ_X * PZ_Y) k_prolong_z(LevV<T> C, const T* _
 }
 
 // CG bottom solve by one workgroup (all threads of the block call it);
-// dots accumulated in double
-template <class T, int BM, int DIM>
-__device__ void bottom_cg(const LevV<T>& L, T* __restrict__ e, const T* __restrict__ b, T* __restrict__ r,
-                          T* __restrict__ p, T* __restrict__ q) {
+// dots accumulated in double.  The iteration runs in T on the operator L;
+// the rhs b comes in and the solution goes out in TB (e_out), so an FP32
+// V-cycle can hand its bottom level to an FP64 solve; e is the T-typed
+// solution array (e_out itself when T is TB).
+template <class T, class TB, int BM, int DIM>
+__device__ void bottom_cg(const LevV<T>& L, TB* e_out, const TB* __restrict__ b, T* __restrict__ e,
+                          T* __restrict__ r, T* __restrict__ p, T* __restrict__ q) {
     __shared__ double red[1024 / 64];
     __shared__ double bc;
     const int n = L.nx * L.ny * L.nz;
@@ -1407,9 +1416,10 @@ __device__ void bottom_cg(const LevV<T>& L, T* __restrict__ e, const T* __restri
         return bc;
     };
     for (int c = tid; c < n; c += nt) {
+        const T bv = T(b[c]);
         e[c] = T(0);
-        r[c] = b[c];
-        p[c] = b[c];
+        r[c] = bv;
+        p[c] = bv;
     }
     __syncthreads();
     double rr = bdot(r, r), rr0 = rr;
@@ -1438,6 +1448,9 @@ __device__ void bottom_cg(const LevV<T>& L, T* __restrict__ e, const T* __restri
         __syncthreads();
         rr = rn;
     }
+    if constexpr (!std::is_same<T, TB>::value) {
+        for (int c = tid; c < n; c += nt) e_out[c] = TB(e[c]);
+    }
 }
 
 
@@ -1458,6 +1471,11 @@ struct VTail {
     T *cr, *cp, *cq;  // bottom CG scratch
     int off[VC_MAXL];   // first cell of each level in the LDS arrays
     int nl, nu1, nu2;
+    // FP64 bottom solve of an FP32 V-cycle: the bottom level's operator
+    // coarsened in FP64, and FP64 CG arrays for bottoms above
+    // VC_WAVE_BOTTOM cells
+    LevV<double> b64;
+    double *be, *br, *bp, *bq;
 };
 
 // f(i, j, k) for i < ni and every (j, k) row of a level, by one workgroup:
@@ -1509,8 +1527,8 @@ __device__ __forceinline__ void wg_sweep(const LevV<T>& L, T* __restrict__ x, co
 // wavefront fences keep the compiler from reordering them).  The other
 // waves wait at the caller's barrier.
 constexpr int VC_WAVE_BOTTOM = 1024;
-template <class T, int BM, int DIM>
-__device__ void bottom_cg_wave(const LevV<T>& L, T* __restrict__ e_out, const T* __restrict__ b, T* lds) {
+template <class T, class TB, int BM, int DIM>
+__device__ void bottom_cg_wave(const LevV<T>& L, TB* __restrict__ e_out, const TB* __restrict__ b, T* lds) {
     const int n = L.nx * L.ny * L.nz;
     const int lane = threadIdx.x;
     T* e = lds;
@@ -1528,7 +1546,7 @@ __device__ void bottom_cg_wave(const LevV<T>& L, T* __restrict__ e_out, const T*
     };
     double rr = 0.0;
     for (int c = lane; c < n; c += 64) {
-        const T bv = b[c];
+        const T bv = T(b[c]);
         e[c] = T(0);
         r[c] = bv;
         p[c] = bv;
@@ -1566,7 +1584,7 @@ __device__ void bottom_cg_wave(const LevV<T>& L, T* __restrict__ e_out, const T*
         sync();
         rr = rn;
     }
-    for (int c = lane; c < n; c += 64) e_out[c] = e[c];
+    for (int c = lane; c < n; c += 64) e_out[c] = TB(e[c]);
 }
 
 // the tail levels' e and r live in LDS: levels of at most VC_TAIL_CELLS
@@ -1577,7 +1595,7 @@ constexpr int VC_LDS_CELLS = 5464;
 // LDS: the tail's levels hold at most VC_LDS_CELLS cells per array (always
 // so when they coarsen; a bottom level that cannot coarsen may be larger,
 // and then every level stays in memory)
-template <class T, int BM, int DIM, bool LDS>
+template <class T, int BM, int DIM, bool LDS, bool B64>
 __global__ void __launch_bounds__(1024) k_vcycle_tail(VTail<T> a) {
     // with LDS, every sweep, transfer and the bottom solve of the tail read
     // and write LDS; only r of the first level comes from memory and only
@@ -1607,13 +1625,23 @@ __global__ void __launch_bounds__(1024) k_vcycle_tail(VTail<T> a) {
         }
         __syncthreads();
     }
-    {
+    if constexpr (B64) {
+        // FP64 bottom solve: the FP32 restriction is the rhs, the FP64
+        // solution is rounded once into the FP32 correction
+        const LevV<double>& B = a.b64;
+        if (B.nx * B.ny * B.nz <= VC_WAVE_BOTTOM) {
+            __shared__ double cg[4 * VC_WAVE_BOTTOM];
+            if (threadIdx.x < 64) bottom_cg_wave<double, T, BM, DIM>(B, e(a.nl - 1), r(a.nl - 1), cg);
+        } else {
+            bottom_cg<double, T, BM, DIM>(B, e(a.nl - 1), r(a.nl - 1), a.be, a.br, a.bp, a.bq);
+        }
+    } else {
         const LevV<T>& B = a.v[a.nl - 1];
         if (B.nx * B.ny * B.nz <= VC_WAVE_BOTTOM) {
             __shared__ T cg[4 * VC_WAVE_BOTTOM];
-            if (threadIdx.x < 64) bottom_cg_wave<T, BM, DIM>(B, e(a.nl - 1), r(a.nl - 1), cg);
+            if (threadIdx.x < 64) bottom_cg_wave<T, T, BM, DIM>(B, e(a.nl - 1), r(a.nl - 1), cg);
         } else {
-            bottom_cg<T, BM, DIM>(B, e(a.nl - 1), r(a.nl - 1), a.cr, a.cp, a.cq);
+            bottom_cg<T, T, BM, DIM>(B, e(a.nl - 1), r(a.nl - 1), e(a.nl - 1), a.cr, a.cp, a.cq);
         }
     }
     __syncthreads();
@@ -1940,6 +1968,54 @@ static int alloc_levels(MGWork* W, std::vector<Lev<T>>& L, bool share0) {
     return VD_OK;
 }
 
+// the FP64 coefficient chain of an FP32 hierarchy (MGWork::C64): level 0
+// views the FP64 finest data, levels 1 .. nlev-1 hold face b (variable
+// coefficients) and boundary c0 (non-periodic sides), the bottom level
+// also the FP64 CG arrays
+static int alloc_coef64(MGWork* W) {
+    std::vector<Lev<double>>& L = W->C64;
+    L.resize(W->nlev);
+    int n[3] = {W->n[0], W->n[1], W->n[2]};
+    double dx[3] = {W->dx[0], W->dx[1], W->dx[2]};
+    void* p;
+    for (int l = 0; l < W->nlev; l++) {
+        Lev<double>& v = L[l];
+        for (int d = 0; d < 3; d++) {
+            v.n[d] = n[d];
+            v.dx[d] = dx[d];
+        }
+        for (int d = 0; d < W->dim; d++) {
+            if (l == 0) {
+                v.bf[d] = W->bf0[d];
+                for (int sd = 0; sd < 2; sd++) v.c0[d][sd] = W->c00[d][sd];
+                continue;
+            }
+            if (!W->constb) {
+                VD_TRY(dalloc(W, nfacef(n, d) * sizeof(double), &p));
+                v.bf[d] = (double*)p;
+            }
+            if (!W->per[d])
+                for (int sd = 0; sd < 2; sd++) {
+                    VD_TRY(dalloc(W, ntanf(n, d) * sizeof(double), &p));
+                    v.c0[d][sd] = (double*)p;
+                }
+        }
+        if (l == W->nlev - 1) {
+            const int64_t nc = ncell(n);
+            double** arr[4] = {&v.e, &v.r, &v.p, &v.q};
+            for (double** a : arr) {
+                VD_TRY(dalloc(W, nc * sizeof(double), &p));
+                *a = (double*)p;
+            }
+        }
+        for (int d = 0; d < W->dim; d++) {
+            n[d] /= 2;
+            dx[d] *= 2.0;
+        }
+    }
+    return VD_OK;
+}
+
 // workspace for a shape / precision / coefficient kind, cached in the ctx
 static int get_work(vd_ctx* ctx, const vd_geom* g, const vd_box* box, int prec, int constb,
                     int maxc, MGWork** out) {
@@ -2038,6 +2114,7 @@ static int get_work(vd_ctx* ctx, const vd_geom* g, const vd_box* box, int prec, 
         }
     } else {
         if ((st = alloc_levels<float>(W, W->L32, false))) goto fail;
+        if ((st = alloc_coef64(W))) goto fail;
     }
     ctx->mg[key] = W;
     *out = W;
@@ -2194,6 +2271,15 @@ static VTail<T> tail_args(const MGWork& W, std::vector<Lev<T>>& L, double alpha,
     *lds = tot <= VC_LDS_CELLS;
     a.nu1 = nu1;
     a.nu2 = nu2;
+    a.be = a.br = a.bp = a.bq = nullptr;
+    if (std::is_same<T, float>::value && !W.C64.empty()) {
+        const Lev<double>& B64 = W.C64[nl - 1];
+        a.b64 = view<double>(W, B64, alpha, beta, bconst);
+        a.be = B64.e;
+        a.br = B64.r;
+        a.bp = B64.p;
+        a.bq = B64.q;
+    }
     return a;
 }
 
@@ -2203,12 +2289,14 @@ static void vc_tail(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L
                     double bconst, int nu1, int nu2, int l0) {
     bool lds;
     const VTail<T> a = tail_args<T>(W, L, alpha, beta, bconst, nu1, nu2, l0, &lds);
-#define VD_TAIL(BMV, DIMV)                                                                                   \
-    {                                                                                                        \
-        if (lds)                                                                                             \
-            hipLaunchKernelGGL((k_vcycle_tail<T, BMV, DIMV, true>), dim3(1), dim3(1024), 0, ctx->stream, a);  \
-        else                                                                                                 \
-            hipLaunchKernelGGL((k_vcycle_tail<T, BMV, DIMV, false>), dim3(1), dim3(1024), 0, ctx->stream, a); \
+    // FP32 V-cycles solve their bottom level in FP64
+    constexpr bool b64 = std::is_same<T, float>::value;
+#define VD_TAIL(BMV, DIMV)                                                                                        \
+    {                                                                                                             \
+        if (lds)                                                                                                  \
+            hipLaunchKernelGGL((k_vcycle_tail<T, BMV, DIMV, true, b64>), dim3(1), dim3(1024), 0, ctx->stream, a);  \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_vcycle_tail<T, BMV, DIMV, false, b64>), dim3(1), dim3(1024), 0, ctx->stream, a); \
     }
     if (bm == 1) {
         if (W.dim == 3) VD_TAIL(1, 3) else VD_TAIL(1, 2)
@@ -2625,6 +2713,8 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
                     }
             }
             coarsen_hierarchy<float>(ctx, W, W.L32, true, c0);
+            // the FP64 bottom solve's operator, coarsened in FP64
+            coarsen_hierarchy<double>(ctx, W, W.C64, true, c0);
         }
         for (int d = 0; d < 3; d++) {
             W.c0_types[d][0] = tlo[d];
