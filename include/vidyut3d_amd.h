@@ -271,6 +271,23 @@ int vd_transport_rhs(vd_ctx* ctx, const vd_mech* m, const vd_array4* sborder,
                      const vd_geom* geom, const int bc_lo[3], const int bc_hi[3],
                      const vd_params* prm, double time, double dt, int specid);
 
+/* vd_transport_rhs without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops).  Same arguments and results as
+ * vd_transport_rhs; the face fluxes it keeps between its kernels live in a
+ * fixed per-GPU array of VD_NOALLOC_SCRATCH_BYTES (held from the library's
+ * load until the process ends) instead of the context's buffers.  A box
+ * whose face fluxes do not fit (the x and, in 3-D, y faces of a WENO box,
+ * every direction otherwise, each rounded up to 256 bytes; about 160^3
+ * cells for 3-D WENO) is rejected with VD_EINVAL.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (different streams or
+ * host threads) race, so the caller orders them. */
+#define VD_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int vd_transport_rhs_noalloc(vd_ctx* ctx, const vd_mech* m, const vd_array4* sborder,
+                             const vd_array4* rxn_src, vd_array4* dsdt, const vd_box* box,
+                             const vd_geom* geom, const int bc_lo[3], const int bc_hi[3],
+                             const vd_params* prm, double time, double dt, int specid);
+
 /* Vidyut::update_rxnsrc_at_all_levels (Source/ScalarSolve.cpp:113-168)
  * with the mechanism's CKWC / reaction_source.H source: rxn_src gets
  * NS species rates (1/m^3/s) and, at comp NS, the electron energy

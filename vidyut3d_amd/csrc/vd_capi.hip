@@ -172,6 +172,15 @@ int vd_ctx_create(int device, vd_ctx** out) {
         return VD_EHIP;
     }
     memset(c->hnorm, 0, 64 * sizeof(double));
+    // the device address of the no-allocation scratch on this device (once)
+    const int st = noalloc_setup(device);
+    if (st != VD_OK) {
+        (void)hipHostFree(c->hnorm);
+        (void)hipHostFree(c->pinned);
+        (void)hipStreamDestroy(c->stream);
+        delete c;
+        return st;
+    }
     *out = c;
     return VD_OK;
 }

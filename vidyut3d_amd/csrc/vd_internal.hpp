@@ -178,6 +178,10 @@ namespace vd {
 // returns a device buffer of at least `bytes` registered under `name`
 int scratch(vd_ctx* ctx, const std::string& name, size_t bytes, void** out);
 
+// takes the device address of the no-allocation entry points' fixed
+// scratch array on `device` (current), once per device
+int noalloc_setup(int device);
+
 // resident workgroups of `kernel` (block threads) on the whole device
 int resident_blocks(vd_ctx* ctx, const void* kernel, int block);
 // chunk count in [nmin, nmax] that best fills whole rounds of `cap`

@@ -600,27 +600,48 @@ static vd_array4 face_view(double* p, const vd_box* box, int d) {
     return a;
 }
 
+// face arrays transport_rhs writes: with the WENO divergence fused into
+// the last direction's pencils that direction's faces stay on chip,
+// otherwise every direction's array is written (and zeroed without
+// transport)
+int transport_rhs_nflux(const vd_params* prm, const vd_geom* g) {
+    return prm->do_transport && prm->hyp_order == 5 && g->dim > 1 ? g->dim - 1 : g->dim;
+}
+
+int64_t transport_face_count(const vd_box* box, int d) {
+    return (int64_t)(box->hi[0] - box->lo[0] + 1 + (d == 0)) * (box->hi[1] - box->lo[1] + 1 + (d == 1)) *
+           (box->hi[2] - box->lo[2] + 1 + (d == 2));
+}
+
 // update_explsrc_at_all_levels for one level (Source/ScalarSolve.cpp:57-111):
-// fluxes zeroed, computed if do_transport, then the divergence.
+// fluxes zeroed, computed if do_transport, then the divergence.  The face
+// arrays are the context's scratch buffers, or with fbuf the caller's
+// buffers for the first transport_rhs_nflux directions (no allocation and
+// no kernel timing then).
 int transport_rhs(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, const vd_array4* rxn,
                   vd_array4* out, const vd_box* box, const vd_geom* g, const int bc_lo[3],
-                  const int bc_hi[3], const vd_params* prm, double dt, int specid) {
+                  const int bc_hi[3], const vd_params* prm, double dt, int specid,
+                  double* const* fbuf) {
     vd_array4 fl[3];
+    const int nused = transport_rhs_nflux(prm, g);
     for (int d = 0; d < 3; d++) {
         int dd = d < g->dim ? d : 0;
-        int64_t n = (int64_t)(box->hi[0] - box->lo[0] + 1 + (dd == 0)) *
-                    (box->hi[1] - box->lo[1] + 1 + (dd == 1)) *
-                    (box->hi[2] - box->lo[2] + 1 + (dd == 2));
+        int64_t n = transport_face_count(box, dd);
         void* p;
-        char name[32];
-        snprintf(name, sizeof(name), "transport_flux%d", d);
-        VD_TRY(scratch(ctx, name, n * sizeof(double), &p));
+        if (fbuf) {
+            // unwritten directions view the first array (never stored to)
+            p = fbuf[d < nused ? d : 0];
+        } else {
+            char name[32];
+            snprintf(name, sizeof(name), "transport_flux%d", d);
+            VD_TRY(scratch(ctx, name, n * sizeof(double), &p));
+        }
         fl[d] = face_view((double*)p, box, dd);
         if (!prm->do_transport && d < g->dim) VD_TRY(fill_value(ctx, (double*)p, n, 0.0));
     }
     double cells = (double)(box->hi[0] - box->lo[0] + 1) * (box->hi[1] - box->lo[1] + 1) *
                    (box->hi[2] - box->lo[2] + 1);
-    kt_begin(ctx, VD_KT_TRANSPORT);
+    if (!fbuf) kt_begin(ctx, VD_KT_TRANSPORT);
     int st = VD_OK;
     if (prm->do_transport && prm->hyp_order == 5 && g->dim > 1) {
         // WENO fluxes with the divergence fused into the last direction
@@ -629,7 +650,7 @@ int transport_rhs(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, const vd_a
         if (prm->do_transport) VD_TRY(transport_flux(ctx, m, sb, fl, box, g, bc_lo, bc_hi, prm, dt, specid));
         st = dsdt(ctx, fl, rxn, out, box, g, specid);
     }
-    kt_end(ctx, VD_KT_TRANSPORT, 8.0 * cells * (m->hdev.nspec + 6));
+    if (!fbuf) kt_end(ctx, VD_KT_TRANSPORT, 8.0 * cells * (m->hdev.nspec + 6));
     return st;
 }
 
@@ -661,5 +682,5 @@ extern "C" int vd_transport_rhs(vd_ctx* ctx, const vd_mech* m, const vd_array4* 
     (void)time;
     if (!ctx || !m || !sborder || !rxn_src || !dsdt_out || !box || !geom || !prm) return VD_EINVAL;
     VD_TRY(check_transport_args(prm, specid, m->hdev.nspec));
-    return transport_rhs(ctx, m, sborder, rxn_src, dsdt_out, box, geom, bc_lo, bc_hi, prm, dt, specid);
+    return transport_rhs(ctx, m, sborder, rxn_src, dsdt_out, box, geom, bc_lo, bc_hi, prm, dt, specid, nullptr);
 }
