@@ -32,6 +32,15 @@ def lib():
         L.or_weno.restype = C.c_double
         L.or_weno.argtypes = [C.c_double] * 5 + [C.c_int]
         L.or_num_threads.restype = C.c_int
+        L.or_species_bc_value.restype = C.c_double
+        L.or_species_bc_value.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
+                                          C.c_double]
+        L.or_gk.restype = C.c_double
+        L.or_gk.argtypes = [C.c_double, C.c_double]
+        L.or_slope_lim.restype = C.c_double
+        L.or_slope_lim.argtypes = [C.c_double] * 4
+        L.or_applied_potential.restype = C.c_double
+        L.or_applied_potential.argtypes = [C.c_double, C.c_int, C.c_void_p]
         _lib = L
     return _lib
 
@@ -72,6 +81,27 @@ class Grid:
 
 def weno(u, scheme):
     return lib().or_weno(*[float(x) for x in u], int(scheme))
+
+
+def species_bc_value(mech, spec, grid, state1, face, d, sgn, Tg):
+    """species_bc's boundary value at face (i, j, k) of direction d;
+    state1 has one ghost cell."""
+    v = grid.view(state1, 1)
+    f = (C.c_int * 3)(*face)
+    return lib().or_species_bc_value(C.addressof(mech), int(spec), C.addressof(v), C.addressof(f), int(d),
+                                     int(sgn), float(Tg))
+
+
+def gk(w, g):
+    return lib().or_gk(float(w), float(g))
+
+
+def slope_lim(c, dl, dm, dr):
+    return lib().or_slope_lim(float(c), float(dl), float(dm), float(dr))
+
+
+def applied_potential(t, domain_end, prm):
+    return lib().or_applied_potential(float(t), int(domain_end), C.addressof(prm))
 
 
 def fill_patch(grid, src, ng):

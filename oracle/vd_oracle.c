@@ -1793,3 +1793,14 @@ int or_fill_patch_two_levels(const vd_array4* crse, const vd_geom* cg, const vd_
             }
     return VD_OK;
 }
+
+/* ------------------------------------------------ single-point hooks */
+double or_species_bc_value(const vd_mech_desc* m, int spec, const vd_array4* phi, const int face[3], int dir,
+                           int sgn, double Tg) {
+    return species_bc_value(m, spec, phi, face, dir, sgn, Tg);
+}
+double or_gk(double w, double g) { return gk(w, g); }
+double or_slope_lim(double c, double dl, double dm, double dr) { return slope_lim(c, dl, dm, dr); }
+double or_applied_potential(double t, int domain_end, const vd_params* p) {
+    return applied_potential(t, domain_end, p);
+}

@@ -79,6 +79,22 @@ int or_mlmg_assemble(const vd_mlmg_problem* prob, const vd_box* box, const vd_ge
 
 int or_num_threads(void);
 
+/* Single-point hooks of internal oracle functions, for their pins in the
+ * tests (no counterpart in the HIP library's ABI):
+ *  - or_species_bc_value: plasmachem_transport::species_bc's boundary
+ *    value (Source/PlasmaChem.H:74-184) for species `spec` (nspec: the
+ *    electron energy) at boundary face `face` of direction dir, sgn -1 on
+ *    the low side and +1 on the high side; phi needs one ghost cell;
+ *  - or_gk: the WENO-M mapping (Source/compute_explicit_flux.H:16-23);
+ *  - or_slope_lim: the MINBEE limiter (compute_explicit_flux.H:130-160);
+ *  - or_applied_potential: get_applied_potential (Source/HelperFuncs.H:
+ *    75-92), domain_end -1 (low) or +1 (high). */
+double or_species_bc_value(const vd_mech_desc* m, int spec, const vd_array4* phi, const int face[3], int dir,
+                           int sgn, double Tg);
+double or_gk(double w, double g);
+double or_slope_lim(double c, double dl, double dm, double dr);
+double or_applied_potential(double t, int domain_end, const vd_params* p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -577,3 +577,32 @@ def test_memset_zero_clears_exactly_the_range(dev):
             assert np.array_equal(got, want), (off, n)
     finally:
         dev.free(base)
+
+
+# ---------------------------------------------------- voltage profiles
+@pytest.mark.parametrize("profile,frac", [(1, 0.25), (1, 0.6), (2, 0.0), (2, 0.3), (2, 0.7)])
+def test_potential_solve_voltage_profiles_match_closed_form_and_oracle(dev, profile, frac):
+    """Sine and triangle applied voltages (Source/HelperFuncs.H:75-92) on a
+    vacuum gap: the potential is linear between the two plate voltages at
+    time t, E = -(V2 - V1)/L, and equals the oracle's solve."""
+    import test_oracle_closed_forms as cf
+    c, p, g, st, cc = cf.laplace_case(profile)
+    t = frac / p.voltage_freq if profile == 1 else p.voltage_center + frac * p.voltage_dur
+    v1, v2 = O.applied_potential(t, -1, p), O.applied_potential(t, 1, p)
+    og, dg = O.Grid(g), native.Grid(dev, g)
+    ref_pn, _, _ = O.solve_potential(c.mech, p, og, O.fill_patch(og, st, 1), 1, st, time=t)
+    m = native.Mech(dev, c.mech)
+    src = dg.array(st.shape[0], 0, host=st)
+    sb = dg.array(st.shape[0], 1)
+    native.fill_patch(dg, src, sb)
+    pn = dg.array(st.shape[0], 0, host=st)
+    ef = dg.faces()
+    native.solve_potential(m, p, dg, sb, pn, ef, time=t)
+    got = pn.get()
+    x = (np.arange(16) + 0.5) * g.dx[0]
+    scale = max(abs(v1), abs(v2), 1.0)
+    np.testing.assert_allclose(got[cc["POT"], 0], np.broadcast_to(v1 + (v2 - v1) * x / 0.01, (4, 16)),
+                               rtol=0, atol=1e-10 * scale)
+    np.testing.assert_allclose(got[cc["EFX"], 0], np.full((4, 16), -(v2 - v1) / 0.01), rtol=1e-10,
+                               atol=1e-8 * scale)
+    assert relmax(ref_pn[cc["POT"]], got[cc["POT"]]) < 1e-12
