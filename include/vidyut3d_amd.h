@@ -152,6 +152,29 @@ enum {
     VD_DIFF_CONST = 2        /* D = diff[s]                             */
 };
 
+/* State-dependent transport coefficients, the form of a case Chemistry.H's
+ * specMob / specDiff(specid, Te, ndens, |E|, Tg) (Source/PlasmaChem.H:
+ * 186-206).  For kind VD_TR_FIT the coefficient is
+ *   sign * exp( c0 + c1 ln|E| + c2 ln Te + c3 ln Tg + c4 ln N
+ *             + c5 ln e + c6/e + c7/e^2 + c8/e^3
+ *             + c9 ln(1 + c10 x^2 + c11 x^4) )
+ * with N = sum of all species densities, e = 1.5 Te / (1 eV) the mean
+ * electron energy in eV and x = |E| / N; terms whose coefficient is 0 are
+ * skipped.  This covers the reference cases' constant, power-law
+ * (test/verification/Streamer/Chemistry.H:165-200), hard-sphere
+ * (test/models/Ar_DC_1d/Chemistry.H specMob) and fitted
+ * (test/verification/He_RF_1d/Chemistry.H specMob) coefficients. */
+#define VD_TR_NCOEF 12
+enum {
+    VD_TR_DEFAULT = 0, /* mob[] / diff_kind[] of the species (constant mobility) */
+    VD_TR_FIT = 1      /* the fit above */
+};
+typedef struct vd_transport_fit {
+    int kind;          /* VD_TR_* */
+    double sign;       /* sign of the coefficient (a mobility carries the charge sign) */
+    double c[VD_TR_NCOEF];
+} vd_transport_fit;
+
 typedef struct vd_reaction {
     double A;          /* k = A * T^beta * exp(-Ea/T), SI units */
     double beta;
@@ -184,6 +207,26 @@ typedef struct vd_mech_desc {
     int elastic_loss;                /* VD_CHEM_CKWC: add the e-neutral elastic loss */
     double elastic_partner_amu;      /* ... with this partner mass */
     vd_reaction reac[VD_MAX_REACTIONS];
+    /* specMob / specDiff, index nspec = the electron energy.  With
+     * VD_TR_DEFAULT the energy takes 5/3 of the electron mobility and
+     * (5/3)(Te/1 eV)|mu_e|; Einstein diffusion uses the species' own
+     * (possibly fitted) mobility. */
+    vd_transport_fit mob_fit[VD_MAX_SPECIES + 1];
+    vd_transport_fit diff_fit[VD_MAX_SPECIES + 1];
+    /* user_transport::compute_vel (selected by vidyut.user_defined_vel,
+     * Source/ScalarSolve.cpp:237-243): a prescribed drift velocity per
+     * direction replaces mobility * E in the face fluxes of species s */
+    int vel_set[VD_MAX_SPECIES + 1];
+    double vel[VD_MAX_SPECIES + 1][3];
+    /* user_transport::species_bc Robin data (vidyut.user_defined_species,
+     * Source/ScalarSolve.cpp:542-553): where species s has the VD_ROBINBC
+     * code, a*n + b*dn/dn_out = f on the face, per direction and side (0
+     * low, 1 high); without robin_set the reference default (0, 1, 0) */
+    int robin_set[VD_MAX_SPECIES + 1];
+    double robin[VD_MAX_SPECIES + 1][3][2][3];
+    /* user_sources::add_user_react_sources: a constant added to each
+     * species' (and, at nspec, the electron energy's) reaction source */
+    double user_src[VD_MAX_SPECIES + 1];
 } vd_mech_desc;
 
 /* ---------------------------------------------------------- handles */

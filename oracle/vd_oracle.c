@@ -150,18 +150,80 @@ static double spec_diff(const vd_mech_desc* m, int s, double Te, double Tg) {
     }
 }
 
+/* a vd_transport_fit coefficient (the state-dependent specMob / specDiff
+ * of a case Chemistry.H, e.g. test/verification/Streamer/Chemistry.H:
+ * 165-200, test/verification/He_RF_1d/Chemistry.H specMob) */
+static double tr_eval(const vd_transport_fit* f, double Te, double Tg, double nd, double emag) {
+    const double* c = f->c;
+    double l = c[0];
+    if (c[1] != 0.0) l += c[1] * log(emag);
+    if (c[2] != 0.0) l += c[2] * log(Te);
+    if (c[3] != 0.0) l += c[3] * log(Tg);
+    if (c[4] != 0.0) l += c[4] * log(nd);
+    if (c[5] != 0.0 || c[6] != 0.0 || c[7] != 0.0 || c[8] != 0.0) {
+        double e = 1.5 * Te / EV;
+        if (c[5] != 0.0) l += c[5] * log(e);
+        if (c[6] != 0.0) l += c[6] / e;
+        if (c[7] != 0.0) l += c[7] / (e * e);
+        if (c[8] != 0.0) l += c[8] / (e * e * e);
+    }
+    if (c[9] != 0.0) {
+        double x = emag / nd;
+        l += c[9] * log(1.0 + c[10] * (x * x) + c[11] * (x * x * x * x));
+    }
+    return f->sign * exp(l);
+}
+
+/* specMob(specid, Te, ndens, |E|, Tg) / specDiff of the case plug-in
+ * (index nspec = the electron energy) */
+static double spec_mob_st(const vd_mech_desc* m, int s, double Te, double Tg, double nd, double emag) {
+    const vd_transport_fit* f = &m->mob_fit[s];
+    if (f->kind == VD_TR_FIT) return tr_eval(f, Te, Tg, nd, emag);
+    if (s == m->nspec) {
+        const vd_transport_fit* fe = &m->mob_fit[m->e_idx];
+        if (fe->kind == VD_TR_FIT) return FIVEBYTHREE * tr_eval(fe, Te, Tg, nd, emag);
+    }
+    return spec_mob(m, s);
+}
+static double spec_diff_st(const vd_mech_desc* m, int s, double Te, double Tg, double nd, double emag) {
+    const vd_transport_fit* f = &m->diff_fit[s];
+    if (f->kind == VD_TR_FIT) return tr_eval(f, Te, Tg, nd, emag);
+    const int e = m->e_idx;
+    if (s == m->nspec) {
+        if (m->diff_fit[e].kind == VD_TR_FIT) return FIVEBYTHREE * tr_eval(&m->diff_fit[e], Te, Tg, nd, emag);
+        if (m->mob_fit[e].kind == VD_TR_FIT)
+            return FIVEBYTHREE * (Te / EV) * fabs(tr_eval(&m->mob_fit[e], Te, Tg, nd, emag));
+        return spec_diff(m, s, Te, Tg);
+    }
+    if (m->mob_fit[s].kind == VD_TR_FIT && m->diff_kind[s] != VD_DIFF_CONST) {
+        double mu = fabs(tr_eval(&m->mob_fit[s], Te, Tg, nd, emag));
+        return (m->diff_kind[s] == VD_DIFF_EINSTEIN_TE ? Te / EV : Tg / EV) * mu;
+    }
+    return spec_diff(m, s, Te, Tg);
+}
+
+/* ndens = sum of the species densities and |E| = sqrt(sum_d E_d^2) at one
+ * cell (the arguments every specMob / specDiff call site forms) */
+static void cell_nd_emag(const vd_array4* phi, const int c[3], int ns, int dim, double* nd, double* emag) {
+    double n = 0.0, es = 0.0;
+    for (int sp = 0; sp < ns; sp++) n += ATV(phi, c, sp);
+    for (int d = 0; d < dim; d++) es += pow(ATV(phi, c, VD_EFX(ns) + d), 2.0);
+    *nd = n;
+    *emag = sqrt(es);
+}
+
 /* plasmachem_transport::compute_vel (Source/PlasmaChem.H:186-206) */
 static double compute_vel(const vd_mech_desc* m, const vd_array4* phi, const int c[3], int dir,
                           int specid, double Tg) {
     int ns = m->nspec;
+    /* user_transport::compute_vel: a prescribed velocity */
+    if (m->vel_set[specid]) return m->vel[specid][dir];
     double efieldmag = sqrt(pow(ATV(phi, c, VD_EFX(ns)), 2.0) + pow(ATV(phi, c, VD_EFY(ns)), 2.0) +
                             pow(ATV(phi, c, VD_EFZ(ns)), 2.0));
     double ndens = 0.0;
     for (int sp = 0; sp < ns; sp++) ndens += ATV(phi, c, sp);
-    (void)efieldmag;
-    (void)ndens;
-    (void)Tg;
-    double mu = spec_mob(m, specid);
+    double Te = ATV(phi, c, VD_ETEMP(ns));
+    double mu = spec_mob_st(m, specid, Te, Tg, ndens, efieldmag);
     return mu * ATV(phi, c, VD_EFX(ns) + dir);
 }
 
@@ -432,8 +494,7 @@ static void ckwc_source(const vd_mech_desc* m, double T, double* C, double* wdot
         /* ionelec_DC_1d/Chemistry.H:334-341 */
         double ndens = 0.0;
         for (int sp = 0; sp < ns; sp++) ndens += C[sp] * N_A;
-        (void)ndens;
-        double mu = spec_mob(m, E);
+        double mu = spec_mob_st(m, E, Te, T, ndens, 0.0);
         double nu = -ECHARGE / ME / mu;
         double term = 3.0 / 2.0 * K_B * C[E] * N_A * (Te - T) * nu *
                       (2.0 * ME / (m->elastic_partner_amu * M_AMU));
@@ -468,6 +529,9 @@ int or_update_rxnsrc(const vd_mech_desc* m, const vd_array4* sb, vd_array4* rxn,
                     for (int sp = 0; sp < ns; sp++) AT(rxn, i, j, k, sp) = wdot[sp] * N_A * 1.0e6;
                     AT(rxn, i, j, k, ns) = ener;
                 }
+                /* user_sources::add_user_react_sources (ScalarSolve.cpp:158-163) */
+                for (int sp = 0; sp <= ns; sp++)
+                    if (m->user_src[sp] != 0.0) AT(rxn, i, j, k, sp) += m->user_src[sp];
             }
     return VD_OK;
 }
@@ -506,10 +570,11 @@ int or_compute_elecenergy_source(const vd_mech_desc* m, const vd_array4* sb, con
                         double ndens = 0.0;
                         for (int sp = 0; sp < ns; sp++)
                             ndens += 0.5 * (ATV(sb, lc, sp) + ATV(sb, rc, sp));
-                        (void)efield_mag;
-                        double mu = prm->const_ele_trans ? prm->ele_mob / ndens : spec_mob(m, eidx);
-                        double dcoeff = prm->const_ele_trans ? prm->ele_diff / ndens
-                                                             : spec_diff(m, eidx, etemp, Tg);
+                        double mu = prm->const_ele_trans ? prm->ele_mob / ndens
+                                                         : spec_mob_st(m, eidx, etemp, Tg, ndens, efield_mag);
+                        double dcoeff = prm->const_ele_trans
+                                            ? prm->ele_diff / ndens
+                                            : spec_diff_st(m, eidx, etemp, Tg, ndens, efield_mag);
                         double cur = charge * (mu * ne * efield_face - dcoeff * gradne_face);
                         jheat += cur * efield_face;
                     }
@@ -541,8 +606,9 @@ int or_find_time_scales(const vd_mech_desc* m, const vd_array4* phi, const vd_bo
                 double ndens = 0.0;
                 for (int sp = 0; sp < ns; sp++) ndens += AT(phi, i, j, k, sp);
                 double dcoeff = prm->const_ele_trans ? prm->ele_diff / ndens
-                                                     : spec_diff(m, eidx, etemp, Tg);
-                double mu = prm->const_ele_trans ? prm->ele_mob / ndens : spec_mob(m, eidx);
+                                                     : spec_diff_st(m, eidx, etemp, Tg, ndens, emag);
+                double mu = prm->const_ele_trans ? prm->ele_mob / ndens
+                                                 : spec_mob_st(m, eidx, etemp, Tg, ndens, emag);
                 double a = fabs(fabs(mu) * emag), b = fabs(dcoeff),
                        c = fabs(fabs(mu) * AT(phi, i, j, k, eidx));
                 if (a > mdrift) mdrift = a;
@@ -1354,31 +1420,30 @@ static double species_bc_value(const vd_mech_desc* m, int spec, const vd_array4*
     int cell[3] = {face[0], face[1], face[2]};
     if (sgn == 1) cell[dir] -= 1;
     double on = (double)sgn;
-    double ndens = 0.0;
-    for (int sp = 0; sp < ns; sp++) ndens += ATV(phi, cell, sp);
-    (void)ndens;
+    double ndens, emag;
+    cell_nd_emag(phi, cell, ns, 3, &ndens, &emag);
     double Te = ATV(phi, cell, VD_ETEMP(ns));
     if (spec == E) {
         double ne = ATV(phi, cell, E);
-        double D = spec_diff(m, spec, Te, Tg);
+        double D = spec_diff_st(m, spec, Te, Tg, ndens, emag);
         double cbar = sqrt(8.0 * K_B * Te / PI / ME);
         double flx = ne * cbar / 4.0 * on;
         return -flx / D;
     } else if (spec == ns) {
         double ne = ATV(phi, cell, E);
-        double D = spec_diff(m, spec, Te, Tg);
+        double D = spec_diff_st(m, spec, Te, Tg, ndens, emag);
         double cbar = sqrt(8.0 * K_B * Te / PI / ME);
         double flx = (2.0 * K_B * Te) * ne * cbar / 4.0 * (double)sgn;
         return -flx / D;
     } else if (m->charge[spec] != 0) {
-        double mu = spec_mob(m, spec);
-        double D = spec_diff(m, spec, Te, Tg);
+        double mu = spec_mob_st(m, spec, Te, Tg, ndens, emag);
+        double D = spec_diff_st(m, spec, Te, Tg, ndens, emag);
         double efield = ATV(phi, cell, VD_EFX(ns) + dir);
         double idf = mu * ATV(phi, cell, spec) * efield;
         double dotn = idf * on;
         return (dotn >= 0.0) ? (-idf / D) : 0.0;
     } else {
-        double D = spec_diff(m, spec, Te, Tg);
+        double D = spec_diff_st(m, spec, Te, Tg, ndens, emag);
         double cbar = sqrt(8.0 * K_B * Tg / PI / (m->molwt[spec] * M_AMU));
         double flx = ATV(phi, face, spec) * cbar / 4.0 * (double)sgn;
         return -flx / D;
@@ -1402,8 +1467,12 @@ int or_implicit_solve_scalar(const vd_mech_desc* m, int spec, const vd_array4* s
     /* D on the grown box (ScalarSolve.cpp:483-497) */
     for (int k = bcell.begin[2]; k < bcell.end[2]; k++)
         for (int j = bcell.begin[1]; j < bcell.end[1]; j++)
-            for (int i = bcell.begin[0]; i < bcell.end[0]; i++)
-                AT(&bcell, i, j, k, 0) = spec_diff(m, spec, AT(sb, i, j, k, VD_ETEMP(ns)), Tg);
+            for (int i = bcell.begin[0]; i < bcell.end[0]; i++) {
+                int c[3] = {i, j, k};
+                double nd, emag;
+                cell_nd_emag(sb, c, ns, dim, &nd, &emag);
+                AT(&bcell, i, j, k, 0) = spec_diff_st(m, spec, AT(sb, i, j, k, VD_ETEMP(ns)), Tg, nd, emag);
+            }
     /* harmonic face average (ScalarSolve.cpp:511-512) */
     vd_array4 bf[3];
     const vd_array4* bptr[3] = {NULL, NULL, NULL};
@@ -1458,6 +1527,25 @@ int or_implicit_solve_scalar(const vd_mech_desc* m, int spec, const vd_array4* s
         P.bc_hi[d] = bchi[d];
     }
     P.bcval = &specdata;
+    /* user_transport::species_bc Robin data (ScalarSolve.cpp:542-553) */
+    vd_array4 rob[3];
+    if (m->robin_set[spec]) {
+        for (int q = 0; q < 3; q++) rob[q] = make_scratch(box, 1, 1, 0.0);
+        for (int d = 0; d < dim; d++) {
+            if (g->is_periodic[d]) continue;
+            for (int side = 0; side < 2; side++) {
+                int lo[3] = {box->lo[0], box->lo[1], box->lo[2]}, hi[3] = {box->hi[0], box->hi[1], box->hi[2]};
+                lo[d] = hi[d] = side ? box->hi[d] + 1 : box->lo[d] - 1;
+                for (int k = lo[2]; k <= hi[2]; k++)
+                    for (int j = lo[1]; j <= hi[1]; j++)
+                        for (int i = lo[0]; i <= hi[0]; i++)
+                            for (int q = 0; q < 3; q++) AT(&rob[q], i, j, k, 0) = m->robin[spec][d][side][q];
+            }
+        }
+        P.robin_a = &rob[0];
+        P.robin_b = &rob[1];
+        P.robin_f = &rob[2];
+    }
     P.reltol = prm->linsolve_reltol;
     P.abstol = prm->linsolve_abstol;
     P.maxiter = prm->linsolve_maxiter;
@@ -1496,6 +1584,8 @@ int or_implicit_solve_scalar(const vd_mech_desc* m, int spec, const vd_array4* s
     free(specdata.p);
     free(rhs.p);
     free(sol.p);
+    if (m->robin_set[spec])
+        for (int q = 0; q < 3; q++) free(rob[q].p);
     for (int d = 0; d < dim; d++) free(bf[d].p);
     return st;
 }

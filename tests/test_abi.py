@@ -39,6 +39,7 @@ def test_struct_layouts_match_the_c_header(tmp_path):
     structs = {"vd_array4": abi.vd_array4, "vd_tagarray4": abi.vd_tagarray4, "vd_box": abi.vd_box,
                "vd_geom": abi.vd_geom, "vd_params": abi.vd_params, "vd_reaction": abi.vd_reaction,
                "vd_mech_desc": abi.vd_mech_desc, "vd_solve_info": abi.vd_solve_info,
+               "vd_transport_fit": abi.vd_transport_fit,
                "vd_mlmg_problem": abi.vd_mlmg_problem, "vd_regrid_params": abi.vd_regrid_params}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "vidyut3d_amd_host.h"', "int main(void){"]
     for n, st in structs.items():

@@ -99,6 +99,14 @@ class vd_reaction(C.Structure):
                 ("elidx", C.c_int)]
 
 
+VD_TR_NCOEF = 12
+VD_TR_DEFAULT, VD_TR_FIT = 0, 1
+
+
+class vd_transport_fit(C.Structure):
+    _fields_ = [("kind", C.c_int), ("sign", C.c_double), ("c", C.c_double * VD_TR_NCOEF)]
+
+
 class vd_mech_desc(C.Structure):
     _fields_ = [("nspec", C.c_int), ("nreac", C.c_int), ("e_idx", C.c_int),
                 ("names", (C.c_char * 16) * VD_MAX_SPECIES),
@@ -110,7 +118,24 @@ class vd_mech_desc(C.Structure):
                 ("diff", C.c_double * VD_MAX_SPECIES),
                 ("chem_form", C.c_int), ("elastic_loss", C.c_int),
                 ("elastic_partner_amu", C.c_double),
-                ("reac", vd_reaction * VD_MAX_REACTIONS)]
+                ("reac", vd_reaction * VD_MAX_REACTIONS),
+                ("mob_fit", vd_transport_fit * (VD_MAX_SPECIES + 1)),
+                ("diff_fit", vd_transport_fit * (VD_MAX_SPECIES + 1)),
+                ("vel_set", C.c_int * (VD_MAX_SPECIES + 1)),
+                ("vel", (C.c_double * 3) * (VD_MAX_SPECIES + 1)),
+                ("robin_set", C.c_int * (VD_MAX_SPECIES + 1)),
+                ("robin", (((C.c_double * 3) * 2) * 3) * (VD_MAX_SPECIES + 1)),
+                ("user_src", C.c_double * (VD_MAX_SPECIES + 1))]
+
+
+def set_fit(f, sign, **coef):
+    """Fills a vd_transport_fit: sign and the named coefficients c0..c11
+    (the rest 0)."""
+    f.kind = VD_TR_FIT
+    f.sign = float(sign)
+    for q in range(VD_TR_NCOEF):
+        f.c[q] = float(coef.get("c%d" % q, 0.0))
+    return f
 
 
 class vd_solve_info(C.Structure):

@@ -359,3 +359,79 @@ def dielectric_poisson(n=512, seed=5, eps_slab=4.0):
             f = -np.broadcast_to(epsc[:, None, None], shp).copy()
         faces.append(np.ascontiguousarray(f))
     return g, faces, rhs
+
+
+# ------------------------------------------- verification cases (plug-ins)
+def drift_diffuse(n=32, dim=2, dt=0.1, nsteps=40):
+    """test/verification/driftDiffuse: species E, S1, S2 on x in [0, 1]
+    (periodic across, 0.125 wide).  The potential is 1 V at x = 0 and 0 V at
+    x = 1 (Robin a = 1, b = 0), so E_x = 1 V/m; S1 drifts with velocity E
+    (the case's user_transport::compute_vel, here a fitted mobility of 1)
+    and diffuses with D = 0.1, Dirichlet 0 / 1 at the ends; S2 diffuses with
+    D = 1 under a unit source (user_sources::add_user_react_sources),
+    Dirichlet 0 at both ends.  Steady states (verify_spec.py:28,42):
+    S1 = (exp(10 x) - 1) / (exp(10) - 1), S2 = (x - x^2) / 2.  Upwind
+    fluxes and dt = 0.1 as in inputs_x."""
+    m = _mech(["E", "S1", "S2"], [-1, 0, 0], [0.000549, 1.0, 1.0], [0.0, 0.0, 0.0],
+              [abi.VD_DIFF_CONST] * 3, [0, 0, 0])
+    m.diff[0], m.diff[1], m.diff[2] = 1.0, 0.1, 1.0
+    m.chem_form = abi.VD_CHEM_TABLE
+    m.nreac = 0
+    abi.set_fit(m.mob_fit[1], 1.0, c0=0.0)  # mu = 1: velocity = E
+    for s, fhi in ((1, 1.0), (2, 0.0)):
+        m.robin_set[s] = 1
+        for q, v in enumerate((1.0, 0.0, 0.0)):
+            m.robin[s][0][0][q] = v
+        for q, v in enumerate((1.0, 0.0, fhi)):
+            m.robin[s][0][1][q] = v
+    m.user_src[2] = 1.0
+    p = default_params()
+    p.hyp_order = 1
+    p.elecenergy_solve = 0
+    p.voltage_amp_1, p.voltage_amp_2 = 1.0, 0.0
+    per = [abi.PERBC] * 3
+    p.pot_bc_lo[:] = p.pot_bc_hi[:] = [abi.DIRCBC] + per[1:]
+    for nm in ("eden", "eenrg", "ion"):
+        getattr(p, nm + "_bc_lo")[:] = [abi.HNEUBC] + per[1:]
+        getattr(p, nm + "_bc_hi")[:] = [abi.HNEUBC] + per[1:]
+    p.neutral_bc_lo[:] = p.neutral_bc_hi[:] = [abi.ROBINBC] + per[1:]
+    nn = [n, 2, 2][:dim]
+    g = abi.make_geom(nn, [0.0] * dim, [1.0, 0.125, 0.125][:dim], [0] + [1] * (dim - 1))
+    c = abi.comp(3)
+    shp = (c["NVAR"],) + ((nn[2],) if dim == 3 else (1,)) + (nn[1], nn[0])
+    st = np.zeros(shp)
+    st[c["ETEMP"]] = 0.5 * TE_1EV
+    return Case("drift_diffuse_%d" % n, m, p, g, st, dt, nsteps)
+
+
+def drift_diffuse_exact(x):
+    """verify_spec.py:28,42: the S1 and S2 steady states."""
+    return (np.exp(10.0 * x) - 1.0) / (np.exp(10.0) - 1.0), 0.5 * (x - x * x)
+
+
+def advect_1d(n=20, dt=1e-4, nsteps=20000, weno_scheme=3):
+    """test/verification/Advect (inputs1d, Prob.H one_dim): species E, AR on
+    x in [-1, 1] x [0, 0.4], periodic; AR moves with the prescribed velocity
+    (1, 0) (user_transport::compute_vel), no diffusion; WENO fluxes with two
+    timestep correctors.  AR(x, 0) = sin(pi x - sin(pi x) / pi); after t = 2
+    it returns to its start (get_error_1d.py:52)."""
+    m = _mech(["E", "AR"], [-1, 0], [0.000549, 39.948], [-1.0, -1.0], [abi.VD_DIFF_CONST] * 2, [0, 0])
+    m.diff[0] = m.diff[1] = 0.0
+    m.chem_form = abi.VD_CHEM_TABLE
+    m.nreac = 0
+    m.vel_set[1] = 1
+    m.vel[1][0], m.vel[1][1], m.vel[1][2] = 1.0, 0.0, 0.0
+    p = default_params()
+    p.hyp_order, p.weno_scheme = 5, weno_scheme
+    p.elecenergy_solve = 0
+    for nm in ("pot", "eden", "eenrg", "ion", "neutral"):
+        getattr(p, nm + "_bc_lo")[:] = [abi.PERBC] * 3
+        getattr(p, nm + "_bc_hi")[:] = [abi.PERBC] * 3
+    ny = max(2, n // 5)
+    g = abi.make_geom([n, ny], [-1.0, 0.0], [1.0, 2.0 * ny / n], [1, 1])  # square cells
+    c = abi.comp(2)
+    st = np.zeros((c["NVAR"], 1, ny, n))
+    x = -1.0 + (np.arange(n) + 0.5) * g.dx[0]
+    st[1] = np.sin(np.pi * x - np.sin(np.pi * x) / np.pi)[None, :]
+    st[c["ETEMP"]] = 300.0
+    return Case("advect_%d" % n, m, p, g, st, dt, nsteps)

@@ -294,6 +294,28 @@ int vd_mech_create(vd_ctx* ctx, const vd_mech_desc* d, vd_mech** out) {
         h.diff_kind[s] = d->diff_kind[s];
         h.diff[s] = d->diff[s];
     }
+    for (int s = 0; s <= d->nspec; s++) {
+        const vd_transport_fit* f[2] = {&d->mob_fit[s], &d->diff_fit[s]};
+        for (int q = 0; q < 2; q++)
+            if (f[q]->kind != VD_TR_DEFAULT && f[q]->kind != VD_TR_FIT) {
+                delete m;
+                set_error("mechanism: species %d %s fit kind %d unknown", s, q ? "diffusion" : "mobility",
+                          f[q]->kind);
+                return VD_EINVAL;
+            }
+        h.mfit_kind[s] = d->mob_fit[s].kind;
+        h.mfit_sign[s] = d->mob_fit[s].sign;
+        h.dfit_kind[s] = d->diff_fit[s].kind;
+        h.dfit_sign[s] = d->diff_fit[s].sign;
+        for (int c = 0; c < VD_TR_NCOEF; c++) {
+            h.mfit_c[s][c] = d->mob_fit[s].c[c];
+            h.dfit_c[s][c] = d->diff_fit[s].c[c];
+        }
+        h.vel_set[s] = d->vel_set[s] != 0;
+        for (int q = 0; q < 3; q++) h.vel[s][q] = d->vel[s][q];
+        h.user_src[s] = d->user_src[s];
+        if (d->user_src[s] != 0.0) h.any_user_src = 1;
+    }
     for (int r = 0; r < d->nreac; r++) {
         const vd_reaction& R = d->reac[r];
         if (R.nreact < 0 || R.nreact > VD_MAX_REACTANTS || R.nprod < 0 || R.nprod > VD_MAX_PRODUCTS) {

@@ -135,7 +135,9 @@ __global__ void __launch_bounds__(CHEM_BLOCK) k_rxn_ckwc(ChemArgs a) {
     }
     if (M.elastic_loss) {
         // test/models/ionelec_DC_1d/Chemistry.H:334-341
-        double mu = spec_mob(M, E);
+        double ndens = 0.0;
+        for (int sp = 0; sp < ns; sp++) ndens += CONC(sp) * VD_N_A;
+        double mu = spec_mob_st(M, E, Te, T, ndens, 0.0);
         double nu = -VD_ECHARGE / VD_ME / mu;
         double term = 3.0 / 2.0 * VD_K_B * CE * VD_N_A * (Te - T) * nu *
                       (2.0 * VD_ME / (M.elastic_partner_amu * VD_M_AMU));
@@ -146,6 +148,27 @@ __global__ void __launch_bounds__(CHEM_BLOCK) k_rxn_ckwc(ChemArgs a) {
     for (int sp = 0; sp < ns; sp++)
         a.rxn.p[(int64_t)sp * a.rxn.ns + orx] = ((w[sp * CHEM_BLOCK + tid] * 1.0e-6) * VD_N_A) * 1.0e6;
     a.rxn.p[(int64_t)ns * a.rxn.ns + orx] = ener;
+}
+
+// user_sources::add_user_react_sources (Source/ScalarSolve.cpp:158-163):
+// the case's constant sources added to every component of rxn_src
+__global__ void __launch_bounds__(256) k_user_src(ChemArgs a) {
+    const int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)a.nx * a.ny * a.nz) return;
+    const int i = (int)(t % a.nx);
+    const int64_t q = t / a.nx;
+    const int j = (int)(q % a.ny), k = (int)(q / a.ny);
+    const DevMech& M = *a.M;
+    const int o = a.rxn.off(a.lo.x + i, a.lo.y + j, a.lo.z + k);
+    for (int sp = 0; sp <= M.nspec; sp++)
+        if (M.user_src[sp] != 0.0) a.rxn.p[(int64_t)sp * a.rxn.ns + o] += M.user_src[sp];
+}
+
+static int add_user_src(vd_ctx* ctx, const vd_mech* m, const ChemArgs& a) {
+    if (!m->hdev.any_user_src) return VD_OK;
+    hipLaunchKernelGGL(k_user_src, grid1d((int64_t)a.nx * a.ny * a.nz, 256), dim3(256), 0, ctx->stream, a);
+    VD_CHECK_LAUNCH();
+    return VD_OK;
 }
 
 int update_rxnsrc(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4* rxn,
@@ -169,7 +192,8 @@ int update_rxnsrc(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4*
     if (m->hdev.chem_form == VD_CHEM_TABLE && m->use_spec && m->fn) {
         int st = chem_jit_launch(ctx, m, sb, rxn, box, prm->gas_temperature);
         kt_end(ctx, VD_KT_CHEM, 8.0 * (double)tot * (2 * m->hdev.nspec + 3));
-        return st;
+        if (st != VD_OK) return st;
+        return add_user_src(ctx, m, a);
     }
     if (m->hdev.chem_form == VD_CHEM_TABLE)
         hipLaunchKernelGGL(k_rxn_table, grid1d(tot, CHEM_BLOCK), dim3(CHEM_BLOCK), lds, ctx->stream, a);
@@ -177,7 +201,7 @@ int update_rxnsrc(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4*
         hipLaunchKernelGGL(k_rxn_ckwc, grid1d(tot, CHEM_BLOCK), dim3(CHEM_BLOCK), lds, ctx->stream, a);
     kt_end(ctx, VD_KT_CHEM, 8.0 * (double)tot * (2 * m->hdev.nspec + 3));
     VD_CHECK_LAUNCH();
-    return VD_OK;
+    return add_user_src(ctx, m, a);
 }
 
 struct JouleArgs {
@@ -185,6 +209,7 @@ struct JouleArgs {
     Arr<double> sb, rxn, ef0, ef1, ef2, gn0, gn1, gn2, dsdt, phi;
     double Tg, ele_mob, ele_diff;
     int const_ele_trans, floor_jh, dim;
+    int state_trans;  // the electron mobility or diffusivity is a fit of the state
     int3 dlo, dhi, lo;
     int nx, ny, nz;
 };
@@ -224,11 +249,24 @@ __global__ void __launch_bounds__(256) k_joule(JouleArgs a) {
             double efield_face = EF(face[0], face[1], face[2]);
             double gradne_face = GN(face[0], face[1], face[2]);
             double mu, dcoeff;
-            if (a.const_ele_trans) {
+            if (a.const_ele_trans || a.state_trans) {
                 double ndens = 0.0;
                 for (int sp = 0; sp < ns; sp++) ndens += 0.5 * (S[(int64_t)sp * N + ol] + S[(int64_t)sp * N + orr]);
-                mu = a.ele_mob / ndens;
-                dcoeff = a.ele_diff / ndens;
+                if (a.const_ele_trans) {
+                    mu = a.ele_mob / ndens;
+                    dcoeff = a.ele_diff / ndens;
+                } else {
+                    // specMob / specDiff of the face-averaged state
+                    // (ElecEnergySource.cpp:102-131)
+                    double es = 0.0;
+                    for (int q = 0; q < a.dim; q++) {
+                        const double e = 0.5 * (S[(int64_t)(VD_EFX(ns) + q) * N + ol] +
+                                                S[(int64_t)(VD_EFX(ns) + q) * N + orr]);
+                        es += e * e;
+                    }
+                    mu = spec_mob_st(M, eidx, etemp, a.Tg, ndens, sqrt(es));
+                    dcoeff = spec_diff_st(M, eidx, etemp, a.Tg, ndens, sqrt(es));
+                }
             } else {
                 mu = spec_mob(M, eidx);
                 dcoeff = spec_diff(M, eidx, etemp, a.Tg);
@@ -263,6 +301,7 @@ int elecenergy_source(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, const 
     a.ele_mob = prm->ele_mob;
     a.ele_diff = prm->ele_diff;
     a.const_ele_trans = prm->const_ele_trans;
+    a.state_trans = m->hdev.mfit_kind[m->hdev.e_idx] == VD_TR_FIT || m->hdev.dfit_kind[m->hdev.e_idx] == VD_TR_FIT;
     a.floor_jh = prm->floor_jh;
     a.dim = g->dim;
     a.dlo = make_int3(g->dom_lo[0], g->dom_lo[1], g->dom_lo[2]);

@@ -103,6 +103,17 @@ struct DevMech {
     int netsp[VD_MAX_REACTANTS + VD_MAX_PRODUCTS][VD_MAX_REACTIONS];
     double netnu[VD_MAX_REACTANTS + VD_MAX_PRODUCTS][VD_MAX_REACTIONS];
     int order[VD_MAX_REACTIONS];
+    // state-dependent specMob / specDiff (vd_transport_fit), index nspec =
+    // the electron energy
+    int mfit_kind[VD_MAX_SPECIES + 1], dfit_kind[VD_MAX_SPECIES + 1];
+    double mfit_sign[VD_MAX_SPECIES + 1], dfit_sign[VD_MAX_SPECIES + 1];
+    double mfit_c[VD_MAX_SPECIES + 1][VD_TR_NCOEF], dfit_c[VD_MAX_SPECIES + 1][VD_TR_NCOEF];
+    // user_transport::compute_vel: prescribed drift velocities
+    int vel_set[VD_MAX_SPECIES + 1];
+    double vel[VD_MAX_SPECIES + 1][3];
+    // user_sources::add_user_react_sources constants
+    double user_src[VD_MAX_SPECIES + 1];
+    int any_user_src;
 };
 
 // specMob / specDiff of the case plug-in (constant mobilities, Einstein
@@ -119,6 +130,70 @@ __host__ __device__ inline double spec_diff(const DevMech& m, int s, double Te, 
     if (k == VD_DIFF_EINSTEIN_TE) return (Te / VD_EV) * fabs(m.mob[s]);
     if (k == VD_DIFF_EINSTEIN_TG) return (Tg / VD_EV) * fabs(m.mob[s]);
     return m.diff[s];
+}
+
+// a vd_transport_fit coefficient: sign * exp(c0 + c1 ln|E| + c2 ln Te +
+// c3 ln Tg + c4 ln N + c5 ln e + c6/e + c7/e^2 + c8/e^3 + c9 ln(1 + c10 x^2
+// + c11 x^4)), e = 1.5 Te / 1 eV, x = |E| / N; zero terms skipped.  The
+// same operations in the same order as the oracle, without contraction.
+__host__ __device__ inline double tr_eval(const double* c, double sign, double Te, double Tg, double nd,
+                                          double emag) {
+#pragma clang fp contract(off)
+    double l = c[0];
+    if (c[1] != 0.0) l += c[1] * log(emag);
+    if (c[2] != 0.0) l += c[2] * log(Te);
+    if (c[3] != 0.0) l += c[3] * log(Tg);
+    if (c[4] != 0.0) l += c[4] * log(nd);
+    if (c[5] != 0.0 || c[6] != 0.0 || c[7] != 0.0 || c[8] != 0.0) {
+        const double e = 1.5 * Te / VD_EV;
+        if (c[5] != 0.0) l += c[5] * log(e);
+        if (c[6] != 0.0) l += c[6] / e;
+        if (c[7] != 0.0) l += c[7] / (e * e);
+        if (c[8] != 0.0) l += c[8] / (e * e * e);
+    }
+    if (c[9] != 0.0) {
+        const double x = emag / nd;
+        l += c[9] * log(1.0 + c[10] * (x * x) + c[11] * (x * x * x * x));
+    }
+    return sign * exp(l);
+}
+
+// specMob / specDiff(specid, Te, ndens, |E|, Tg) of the case plug-in
+__host__ __device__ inline double spec_mob_st(const DevMech& m, int s, double Te, double Tg, double nd,
+                                              double emag) {
+#pragma clang fp contract(off)
+    if (m.mfit_kind[s] == VD_TR_FIT) return tr_eval(m.mfit_c[s], m.mfit_sign[s], Te, Tg, nd, emag);
+    if (s == m.nspec && m.mfit_kind[m.e_idx] == VD_TR_FIT)
+        return VD_FIVEBYTHREE * tr_eval(m.mfit_c[m.e_idx], m.mfit_sign[m.e_idx], Te, Tg, nd, emag);
+    return spec_mob(m, s);
+}
+__host__ __device__ inline double spec_diff_st(const DevMech& m, int s, double Te, double Tg, double nd,
+                                               double emag) {
+#pragma clang fp contract(off)
+    if (m.dfit_kind[s] == VD_TR_FIT) return tr_eval(m.dfit_c[s], m.dfit_sign[s], Te, Tg, nd, emag);
+    const int e = m.e_idx;
+    if (s == m.nspec) {
+        if (m.dfit_kind[e] == VD_TR_FIT) return VD_FIVEBYTHREE * tr_eval(m.dfit_c[e], m.dfit_sign[e], Te, Tg, nd, emag);
+        if (m.mfit_kind[e] == VD_TR_FIT)
+            return VD_FIVEBYTHREE * (Te / VD_EV) * fabs(tr_eval(m.mfit_c[e], m.mfit_sign[e], Te, Tg, nd, emag));
+        return spec_diff(m, s, Te, Tg);
+    }
+    if (m.mfit_kind[s] == VD_TR_FIT && m.diff_kind[s] != VD_DIFF_CONST) {
+        const double mu = fabs(tr_eval(m.mfit_c[s], m.mfit_sign[s], Te, Tg, nd, emag));
+        return (m.diff_kind[s] == VD_DIFF_EINSTEIN_TE ? Te / VD_EV : Tg / VD_EV) * mu;
+    }
+    return spec_diff(m, s, Te, Tg);
+}
+
+// whether species s's drift velocity needs the cell state (a fitted
+// mobility) or is prescribed; else it is a constant mobility times E
+inline bool drift_is_state(const DevMech& m, int s) {
+    return m.vel_set[s] || m.mfit_kind[s] == VD_TR_FIT || (s == m.nspec && m.mfit_kind[m.e_idx] == VD_TR_FIT);
+}
+// whether species s's diffusion coefficient is the same in every cell
+inline bool diff_is_uniform(const DevMech& m, int s) {
+    return s < m.nspec && m.dfit_kind[s] != VD_TR_FIT && m.diff_kind[s] != VD_DIFF_EINSTEIN_TE &&
+           !(m.diff_kind[s] == VD_DIFF_EINSTEIN_TG && m.mfit_kind[s] == VD_TR_FIT);
 }
 
 struct Buf {

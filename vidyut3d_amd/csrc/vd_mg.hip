@@ -3185,6 +3185,7 @@ struct CellDArgs {
     const DevMech* __restrict__ M;
     Arr<double> sb;
     int spec, ns, dim;
+    int state;          // the coefficient is a fit of the state (else it depends on Te only)
     int3 lo;
     int gnx, gny, gnz;  // grown box extent
     double Tg;
@@ -3198,9 +3199,24 @@ __global__ void __launch_bounds__(256) k_cell_diff(CellDArgs a, double* __restri
     const int64_t q = t / a.gnx;
     const int j = (int)(q % a.gny), k = (int)(q / a.gny);
     const int gz = a.dim == 3 ? 1 : 0;
-    const double* te = a.sb.comp(VD_ETEMP(a.ns));
-    dcell[i + a.gnx * (j + a.gny * k)] =
-        spec_diff(*a.M, a.spec, te[a.sb.off(a.lo.x + i - 1, a.lo.y + j - 1, a.lo.z + k - gz)], a.Tg);
+    const int o = a.sb.off(a.lo.x + i - 1, a.lo.y + j - 1, a.lo.z + k - gz);
+    const double* S = a.sb.p;
+    const int64_t N = a.sb.ns;
+    const double te = S[(int64_t)VD_ETEMP(a.ns) * N + o];
+    double D;
+    if (a.state) {
+        // ndens and |E| of the cell (ScalarSolve.cpp:484-490)
+        double nd = 0.0, es = 0.0;
+        for (int sp = 0; sp < a.ns; sp++) nd += S[(int64_t)sp * N + o];
+        for (int d = 0; d < a.dim; d++) {
+            const double e = S[(int64_t)(VD_EFX(a.ns) + d) * N + o];
+            es += e * e;
+        }
+        D = spec_diff_st(*a.M, a.spec, te, a.Tg, nd, sqrt(es));
+    } else {
+        D = spec_diff(*a.M, a.spec, te, a.Tg);
+    }
+    dcell[i + a.gnx * (j + a.gny * k)] = D;
 }
 
 // harmonic face average of the cell coefficient (ScalarSolve.cpp:511-512)
@@ -3247,28 +3263,36 @@ __global__ void k_species_bc(SpecBcArgs a, SpecBcSides sides) {
     const int64_t N = a.sb.ns;
     double sgn = a.side ? 1.0 : -1.0;
     double Te = S[VD_ETEMP(ns) * N + oc];
+    // ndens and |E| of the boundary cell (PlasmaChem.H:107-108,114-116)
+    double nd = 0.0, es = 0.0;
+    for (int sp = 0; sp < ns; sp++) nd += S[(int64_t)sp * N + oc];
+    for (int d = 0; d < 3; d++) {
+        const double e = S[(int64_t)(VD_EFX(ns) + d) * N + oc];
+        es += e * e;
+    }
+    const double emag = sqrt(es);
     double v;
     if (a.spec == E) {
         double ne = S[(int64_t)E * N + oc];
-        double D = spec_diff(M, a.spec, Te, a.Tg);
+        double D = spec_diff_st(M, a.spec, Te, a.Tg, nd, emag);
         double cbar = sqrt(8.0 * VD_K_B * Te / VD_PI / VD_ME);
         double flx = ne * cbar / 4.0 * sgn;
         v = -flx / D;
     } else if (a.spec == ns) {
         double ne = S[(int64_t)E * N + oc];
-        double D = spec_diff(M, a.spec, Te, a.Tg);
+        double D = spec_diff_st(M, a.spec, Te, a.Tg, nd, emag);
         double cbar = sqrt(8.0 * VD_K_B * Te / VD_PI / VD_ME);
         double flx = (2.0 * VD_K_B * Te) * ne * cbar / 4.0 * sgn;
         v = -flx / D;
     } else if (M.charge[a.spec] != 0) {
-        double mu = spec_mob(M, a.spec);
-        double D = spec_diff(M, a.spec, Te, a.Tg);
+        double mu = spec_mob_st(M, a.spec, Te, a.Tg, nd, emag);
+        double D = spec_diff_st(M, a.spec, Te, a.Tg, nd, emag);
         double efield = S[(int64_t)(VD_EFX(ns) + a.d) * N + oc];
         double idf = mu * S[(int64_t)a.spec * N + oc] * efield;
         double dotn = idf * sgn;
         v = (dotn >= 0.0) ? (-idf / D) : 0.0;
     } else {
-        double D = spec_diff(M, a.spec, Te, a.Tg);
+        double D = spec_diff_st(M, a.spec, Te, a.Tg, nd, emag);
         double cbar = sqrt(8.0 * VD_K_B * a.Tg / VD_PI / (M.molwt[a.spec] * VD_M_AMU));
         double flx = S[(int64_t)a.spec * N + a.sb.off(face[0], face[1], face[2])] * cbar / 4.0 * sgn;
         v = -flx / D;
@@ -3360,7 +3384,7 @@ int implicit_solve(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sb,
     // Einstein at the gas temperature) is uniform, and so is every harmonic
     // face average of it: the solve then runs on the constant-coefficient
     // stencil (no cell-coefficient array, no per-face averages)
-    const bool constd = spec < ns && m->hdev.diff_kind[spec] != VD_DIFF_EINSTEIN_TE;
+    const bool constd = diff_is_uniform(m->hdev, spec);
     double bc = 0.0;
     if (constd) {
         const double D = spec_diff(m->hdev, spec, 0.0, Tg);
@@ -3377,6 +3401,12 @@ int implicit_solve(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sb,
         cd.gny = n[1] + 2;
         cd.gnz = g->dim == 3 ? n[2] + 2 : 1;
         cd.Tg = Tg;
+        {
+            const DevMech& h = m->hdev;
+            const int ef = spec == ns ? h.e_idx : spec;  // the energy follows the electron's fits
+            cd.state = h.dfit_kind[spec] == VD_TR_FIT || h.mfit_kind[spec] == VD_TR_FIT ||
+                       h.dfit_kind[ef] == VD_TR_FIT || h.mfit_kind[ef] == VD_TR_FIT;
+        }
         hipLaunchKernelGGL(k_cell_diff, grid1d((int64_t)cd.gnx * cd.gny * cd.gnz, 256), dim3(256), 0,
                            ctx->stream, cd, W->dcell);
     }
@@ -3413,15 +3443,17 @@ int implicit_solve(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sb,
             hipLaunchKernelGGL(k_species_bc, gb, dim3(256), 0, ctx->stream, b, sides);
         }
     }
-    // Robin species sides keep the reference default (a, b, f) = (0, 1, 0)
+    // Robin species sides: the case's user_transport::species_bc data, or
+    // the reference default (a, b, f) = (0, 1, 0)
     for (int d = 0; d < g->dim; d++) {
         if (W->per[d]) continue;
         for (int s = 0; s < 2; s++)
             if ((s ? bchi[d] : bclo[d]) == VD_ROBINBC) {
                 int64_t nt = ntanf(n, d);
-                VD_TRY(fill_value(ctx, W->rob[d][s][0], nt, 0.0));
-                VD_TRY(fill_value(ctx, W->rob[d][s][1], nt, 1.0));
-                VD_TRY(fill_value(ctx, W->rob[d][s][2], nt, 0.0));
+                const bool user = m->host.robin_set[spec] != 0;
+                for (int q = 0; q < 3; q++)
+                    VD_TRY(fill_value(ctx, W->rob[d][s][q], nt,
+                                      user ? m->host.robin[spec][d][s][q] : (q == 1 ? 1.0 : 0.0)));
             }
     }
     SpecRhsArgs r;

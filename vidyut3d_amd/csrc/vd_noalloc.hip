@@ -15,7 +15,7 @@
 #include "vd_internal.hpp"
 
 // 64 MiB: the x (and, in 3-D, y) face fluxes of a WENO box up to about
-// 160^3 cells
+// 160^3 cells (with per-cell drift velocities too, smaller boxes)
 static __device__ __align__(256) double scratch[VD_NOALLOC_SCRATCH_BYTES / sizeof(double)];
 
 namespace {
@@ -28,8 +28,9 @@ namespace vd {
 
 int transport_rhs(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, const vd_array4* rxn, vd_array4* out,
                   const vd_box* box, const vd_geom* g, const int bc_lo[3], const int bc_hi[3],
-                  const vd_params* prm, double dt, int specid, double* const* fbuf);
+                  const vd_params* prm, double dt, int specid, double* const* fbuf, double* vbuf);
 int transport_rhs_nflux(const vd_params* prm, const vd_geom* g);
+int64_t drift_vel_doubles(const vd_array4* sb, int dim);
 int64_t transport_face_count(const vd_box* box, int d);
 int check_transport_args(const vd_params* prm, int specid, int nspec);
 
@@ -77,13 +78,17 @@ extern "C" int vd_transport_rhs_noalloc(vd_ctx* ctx, const vd_mech* m, const vd_
             set_error("transport (no allocation): empty box");
             return VD_EINVAL;
         }
-    // the face arrays written, each starting on a 256-byte boundary
+    // the face arrays written and, for a species whose drift velocity is
+    // computed per cell, the velocities; each starts on a 256-byte boundary
     const int nflux = transport_rhs_nflux(prm, geom);
     int64_t need = 0, first[3] = {0, 0, 0};
     for (int d = 0; d < nflux; d++) {
         first[d] = need;
         need += (transport_face_count(box, d) + 31) / 32 * 32;
     }
+    const bool drift = prm->do_transport && drift_is_state(m->hdev, specid);
+    const int64_t vfirst = need;
+    if (drift) need += (drift_vel_doubles(sborder, geom->dim) + 31) / 32 * 32;
     const int64_t cap = (int64_t)(VD_NOALLOC_SCRATCH_BYTES / sizeof(double));
     if (need > cap) {
         set_error("transport (no allocation): the box's face fluxes need %lld bytes, beyond the %lld-byte "
@@ -97,5 +102,6 @@ extern "C" int vd_transport_rhs_noalloc(vd_ctx* ctx, const vd_mech* m, const vd_
         return VD_EHIP;
     }
     double* fbuf[3] = {base + first[0], base + first[1], base + first[2]};
-    return transport_rhs(ctx, m, sborder, rxn_src, dsdt_out, box, geom, bc_lo, bc_hi, prm, dt, specid, fbuf);
+    return transport_rhs(ctx, m, sborder, rxn_src, dsdt_out, box, geom, bc_lo, bc_hi, prm, dt, specid, fbuf,
+                         drift ? base + vfirst : nullptr);
 }

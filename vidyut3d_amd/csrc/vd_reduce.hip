@@ -11,6 +11,7 @@ struct CflArgs {
     const DevMech* __restrict__ M;
     Arr<double> phi;
     int dim, const_ele_trans;
+    int state_trans;  // the electron mobility or diffusivity is a fit of the state
     double Tg, ele_mob, ele_diff;
     int3 lo;
     int nx, ny, nz;
@@ -54,6 +55,11 @@ __global__ void __launch_bounds__(256) k_cfl(CflArgs a) {
             for (int sp = 0; sp < ns; sp++) nd += P[(int64_t)sp * N + o];
             dc = a.ele_diff / nd;
             mu = a.ele_mob / nd;
+        } else if (a.state_trans) {
+            double nd = 0.0;
+            for (int sp = 0; sp < ns; sp++) nd += P[(int64_t)sp * N + o];
+            dc = spec_diff_st(M, e, etemp, a.Tg, nd, emag);
+            mu = spec_mob_st(M, e, etemp, a.Tg, nd, emag);
         } else {
             dc = spec_diff(M, e, etemp, a.Tg);
             mu = spec_mob(M, e);
@@ -93,6 +99,7 @@ int find_time_scales(vd_ctx* ctx, const vd_mech* m, const vd_array4* phi, const 
     a.phi = arr(phi);
     a.dim = g->dim;
     a.const_ele_trans = prm->const_ele_trans;
+    a.state_trans = m->hdev.mfit_kind[m->hdev.e_idx] == VD_TR_FIT || m->hdev.dfit_kind[m->hdev.e_idx] == VD_TR_FIT;
     a.Tg = prm->gas_temperature;
     a.ele_mob = prm->ele_mob;
     a.ele_diff = prm->ele_diff;

@@ -160,6 +160,9 @@ struct FluxArgs {
     Arr<double> sb;
     Arr<double> flux;
     int dir, specid, efcomp;
+    // the x component of E (or of a precomputed drift velocity, then mob =
+    // 1), laid out like sb, one sb component stride per direction
+    const double* ev;
     double mob;          // spec_mob(specid)
     int lo_zero, hi_zero; // face index that is zeroed (IHNEU) or INT_MIN
     int3 flo;            // first face index
@@ -183,7 +186,7 @@ __global__ void __launch_bounds__(256) k_face_flux(FluxArgs a) {
     const int d = a.dir;
     const int st = d == 0 ? 1 : (d == 1 ? a.sb.js : a.sb.ks);
     const double* __restrict__ u = a.sb.comp(a.specid);
-    const double* __restrict__ E = a.sb.comp(a.efcomp + d);
+    const double* __restrict__ E = a.ev + (int64_t)d * a.sb.ns;
     int o = a.sb.off(i, j, k);  // right cell
     const int face_d = d == 0 ? i : (d == 1 ? j : k);
     double f;
@@ -230,6 +233,7 @@ __global__ void __launch_bounds__(256) k_face_flux(FluxArgs a) {
 struct PencilArgs {
     Arr<double> sb, flux;
     int specid, efcomp;
+    const double* ev;      // as FluxArgs::ev
     double mob;
     int lo_zero, hi_zero;  // face index zeroed (IHNEU) or INT_MIN
     int3 flo;              // first face index (box lo)
@@ -261,7 +265,7 @@ __global__ void __launch_bounds__(256) k_pencil_flux(PencilArgs a) {
     const int mlo = D == 1 ? a.flo.y : a.flo.z;
     const int base = D == 1 ? a.sb.off(ig, mlo, kg) : a.sb.off(ig, jg, mlo);
     const double* __restrict__ u = a.sb.comp(a.specid);
-    const double* __restrict__ E = a.sb.comp(a.efcomp + D);
+    const double* __restrict__ E = a.ev + (int64_t)D * a.sb.ns;
     // window of cells c-2 .. c+2 (u, v and f = u v) for centre c
     double w_u[5], w_f[5], w_v[5];
     auto cell = [&](int m, double& uu, double& ff, double& vv) {
@@ -324,6 +328,7 @@ __global__ void __launch_bounds__(256) k_pencil_flux(PencilArgs a) {
 struct XLineArgs {
     Arr<double> sb, flux;
     int specid, efx;
+    const double* ev;  // as FluxArgs::ev
     double mob;
     int lo_zero, hi_zero;
     int3 flo;
@@ -348,7 +353,7 @@ __global__ void __launch_bounds__(256) k_xline_flux(XLineArgs a) {
     const int c = (int)(vq - (int64_t)row * rl) - 1;  // centre cell, relative to box lo
     const int jg = a.flo.y + row % a.ny, kg = a.flo.z + row / a.ny;
     const double* __restrict__ u = a.sb.comp(a.specid);
-    const double* __restrict__ E = a.sb.comp(a.efx);
+    const double* __restrict__ E = a.ev;
     const int base = a.sb.off(a.flo.x, jg, kg);
     double wu[5], wf[5], vc = 0.0;
 #pragma unroll
@@ -428,6 +433,7 @@ static int launch_pencil(vd_ctx* ctx, const FluxArgs& f, int d, int scheme, cons
     a.flux = f.flux;
     a.specid = f.specid;
     a.efcomp = f.efcomp;
+    a.ev = f.ev;
     a.mob = f.mob;
     a.lo_zero = f.lo_zero;
     a.hi_zero = f.hi_zero;
@@ -473,6 +479,7 @@ static int launch_xline(vd_ctx* ctx, const FluxArgs& f, int scheme, const vd_box
     a.flux = f.flux;
     a.specid = f.specid;
     a.efx = f.efcomp;
+    a.ev = f.ev;
     a.mob = f.mob;
     a.lo_zero = f.lo_zero;
     a.hi_zero = f.hi_zero;
@@ -493,6 +500,48 @@ static int launch_xline(vd_ctx* ctx, const FluxArgs& f, int scheme, const vd_box
     VD_CHECK_LAUNCH();
     return VD_OK;
 }
+
+// The drift velocity of species s in every cell of sb's allocated box
+// (ghost cells included), for species whose velocity is not a constant
+// mobility times E: user_transport::compute_vel's prescribed velocity, or
+// specMob(s, Te, ndens, |E|, Tg) * E_d (Source/PlasmaChem.H:186-206).
+// vel has sb's layout, one component per direction.
+struct DriftArgs {
+    const DevMech* __restrict__ M;
+    Arr<double> sb;
+    int spec, dim;
+    int nx, ny, nz;  // sb's allocated extent
+    double Tg;
+    double* vel;
+};
+
+__global__ void __launch_bounds__(256) k_drift_vel(DriftArgs a) {
+    const int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
+    if (t >= (int64_t)a.nx * a.ny * a.nz) return;
+    const int i = (int)(t % a.nx);
+    const int64_t q = t / a.nx;
+    const int j = (int)(q % a.ny), k = (int)(q / a.ny);
+    const DevMech& M = *a.M;
+    const int ns = M.nspec;
+    const int o = a.sb.off(a.sb.bx + i, a.sb.by + j, a.sb.bz + k);
+    const double* S = a.sb.p;
+    const int64_t N = a.sb.ns;
+    if (M.vel_set[a.spec]) {
+        for (int d = 0; d < a.dim; d++) a.vel[(int64_t)d * N + o] = M.vel[a.spec][d];
+        return;
+    }
+    double nd = 0.0, es = 0.0;
+    for (int sp = 0; sp < ns; sp++) nd += S[(int64_t)sp * N + o];
+    for (int d = 0; d < 3; d++) {
+        const double e = S[(int64_t)(VD_EFX(ns) + d) * N + o];
+        es += e * e;
+    }
+    const double mu = spec_mob_st(M, a.spec, S[(int64_t)VD_ETEMP(ns) * N + o], a.Tg, nd, sqrt(es));
+    for (int d = 0; d < a.dim; d++) a.vel[(int64_t)d * N + o] = mu * S[(int64_t)(VD_EFX(ns) + d) * N + o];
+}
+
+// doubles of the drift-velocity array k_drift_vel writes for sb
+int64_t drift_vel_doubles(const vd_array4* sb, int dim) { return sb->nstride * (int64_t)dim; }
 
 int check_transport_args(const vd_params* prm, int specid, int nspec) {
     if (prm->hyp_order != 1 && prm->hyp_order != 2 && prm->hyp_order != 5) {
@@ -516,7 +565,7 @@ int check_transport_args(const vd_params* prm, int specid, int nspec) {
 int transport_flux(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4 flux[3],
                    const vd_box* box, const vd_geom* g, const int bc_lo[3], const int bc_hi[3],
                    const vd_params* prm, double dt, int specid, const vd_array4* rxn = nullptr,
-                   vd_array4* ds = nullptr) {
+                   vd_array4* ds = nullptr, double* vbuf = nullptr) {
     VD_TRY(check_transport_args(prm, specid, m->hdev.nspec));
     int ng = prm->hyp_order == 5 ? 3 : prm->hyp_order;
     for (int d = 0; d < g->dim; d++)
@@ -524,6 +573,32 @@ int transport_flux(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4
             set_error("transport: Sborder needs %d ghost cells", ng);
             return VD_EINVAL;
         }
+    // drift velocities: a constant mobility times E read in the flux
+    // kernels, or (fitted mobility, prescribed velocity) precomputed per cell
+    const double* ev = sb->p + (int64_t)VD_EFX(m->hdev.nspec) * sb->nstride;
+    double mob = spec_mob(m->hdev, specid);
+    if (drift_is_state(m->hdev, specid)) {
+        const int64_t nv = drift_vel_doubles(sb, g->dim);
+        if (!vbuf) {
+            void* p;
+            VD_TRY(scratch(ctx, "transport_vel", nv * sizeof(double), &p));
+            vbuf = (double*)p;
+        }
+        DriftArgs v;
+        v.M = m->dev;
+        v.sb = arr(sb);
+        v.spec = specid;
+        v.dim = g->dim;
+        v.nx = sb->end[0] - sb->begin[0];
+        v.ny = sb->end[1] - sb->begin[1];
+        v.nz = sb->end[2] - sb->begin[2];
+        v.Tg = prm->gas_temperature;
+        v.vel = vbuf;
+        hipLaunchKernelGGL(k_drift_vel, grid1d((int64_t)v.nx * v.ny * v.nz, 256), dim3(256), 0, ctx->stream, v);
+        VD_CHECK_LAUNCH();
+        ev = vbuf;
+        mob = 1.0;
+    }
     double dxmin = g->dx[0];
     for (int d = 1; d < g->dim; d++) dxmin = fmin(dxmin, g->dx[d]);
     for (int d = 0; d < g->dim; d++) {
@@ -533,7 +608,8 @@ int transport_flux(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4
         a.dir = d;
         a.specid = specid;
         a.efcomp = VD_EFX(m->hdev.nspec);
-        a.mob = spec_mob(m->hdev, specid);
+        a.ev = ev;
+        a.mob = mob;
         a.lo_zero = bc_lo[d] == VD_IHNEUBC ? g->dom_lo[d] : INT32_MIN;
         a.hi_zero = bc_hi[d] == VD_IHNEUBC ? g->dom_hi[d] + 1 : INT32_MIN;
         a.flo = make_int3(box->lo[0], box->lo[1], box->lo[2]);
@@ -617,11 +693,13 @@ int64_t transport_face_count(const vd_box* box, int d) {
 // fluxes zeroed, computed if do_transport, then the divergence.  The face
 // arrays are the context's scratch buffers, or with fbuf the caller's
 // buffers for the first transport_rhs_nflux directions (no allocation and
-// no kernel timing then).
+// no kernel timing then); likewise the per-cell drift velocities of a
+// species with a fitted mobility or a prescribed velocity, in vbuf
+// (drift_vel_doubles) when given.
 int transport_rhs(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, const vd_array4* rxn,
                   vd_array4* out, const vd_box* box, const vd_geom* g, const int bc_lo[3],
                   const int bc_hi[3], const vd_params* prm, double dt, int specid,
-                  double* const* fbuf) {
+                  double* const* fbuf, double* vbuf) {
     vd_array4 fl[3];
     const int nused = transport_rhs_nflux(prm, g);
     for (int d = 0; d < 3; d++) {
@@ -645,9 +723,10 @@ int transport_rhs(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, const vd_a
     int st = VD_OK;
     if (prm->do_transport && prm->hyp_order == 5 && g->dim > 1) {
         // WENO fluxes with the divergence fused into the last direction
-        st = transport_flux(ctx, m, sb, fl, box, g, bc_lo, bc_hi, prm, dt, specid, rxn, out);
+        st = transport_flux(ctx, m, sb, fl, box, g, bc_lo, bc_hi, prm, dt, specid, rxn, out, vbuf);
     } else {
-        if (prm->do_transport) VD_TRY(transport_flux(ctx, m, sb, fl, box, g, bc_lo, bc_hi, prm, dt, specid));
+        if (prm->do_transport)
+            VD_TRY(transport_flux(ctx, m, sb, fl, box, g, bc_lo, bc_hi, prm, dt, specid, nullptr, nullptr, vbuf));
         st = dsdt(ctx, fl, rxn, out, box, g, specid);
     }
     if (!fbuf) kt_end(ctx, VD_KT_TRANSPORT, 8.0 * cells * (m->hdev.nspec + 6));
@@ -682,5 +761,6 @@ extern "C" int vd_transport_rhs(vd_ctx* ctx, const vd_mech* m, const vd_array4* 
     (void)time;
     if (!ctx || !m || !sborder || !rxn_src || !dsdt_out || !box || !geom || !prm) return VD_EINVAL;
     VD_TRY(check_transport_args(prm, specid, m->hdev.nspec));
-    return transport_rhs(ctx, m, sborder, rxn_src, dsdt_out, box, geom, bc_lo, bc_hi, prm, dt, specid, nullptr);
+    return transport_rhs(ctx, m, sborder, rxn_src, dsdt_out, box, geom, bc_lo, bc_hi, prm, dt, specid, nullptr,
+                         nullptr);
 }
