@@ -85,3 +85,25 @@ def test_noalloc_transport_rejects_boxes_beyond_its_scratch():
         assert st == abi.VD_EINVAL and "no-allocation scratch" in msg, (st, msg)
     finally:
         dev.close()
+
+
+def test_noalloc_transport_with_per_cell_drift_velocities():
+    """Species whose drift velocity is computed per cell (a prescribed
+    velocity; a fitted mobility) also take the velocity array from the fixed
+    scratch: bit-equal to vd_transport_rhs, two calls in a row."""
+    import test_gpu_plugins as tp
+    dev = native.Device(0)
+    try:
+        for q, c in enumerate([cases.advect_1d(64), cases.advect_1d(32)]):
+            print("no-alloc transport, prescribed velocity: n %d" % c.geom.dom_hi[0], flush=True)
+            ref, got = _rhs_pair(dev, c, 1, 40 + q)
+            assert np.array_equal(ref, got)
+        for q, n in enumerate([24, 16]):
+            c = tp.fitted_air(n=n)
+            print("no-alloc transport, fitted mobilities: n %d" % n, flush=True)
+            for spec in (0, 1, 10):
+                ref, got = _rhs_pair(dev, c, spec, 50 + q)
+                assert np.isfinite(ref).all()
+                assert np.array_equal(ref, got), (n, spec)
+    finally:
+        dev.close()
