@@ -123,6 +123,11 @@ typedef struct vd_params {
     int eenrg_bc_lo[3], eenrg_bc_hi[3];
     int ion_bc_lo[3], ion_bc_hi[3];
     int neutral_bc_lo[3], neutral_bc_hi[3];
+    /* Vidyut::ComputeDt (Source/TimeStep.cpp:14-31): vidyut.adaptive_dt and
+     * its keys (Source/Vidyut.cpp:281-291, defaults Source/Vidyut.H:173-182) */
+    int adaptive_dt;
+    double advective_cfl, diffusive_cfl, dielectric_cfl;
+    double dt_min, dt_max, adaptive_dt_delay, dt_stretch;
 } vd_params;
 
 /* fills the reference defaults of Source/Vidyut.H:149-272 */
@@ -377,6 +382,14 @@ int vd_implicit_solve_scalar(vd_ctx* ctx, const vd_mech* m, int spec, const vd_a
  * until the three scalars are on the host. */
 int vd_find_time_scales(vd_ctx* ctx, const vd_mech* m, const vd_array4* phi_new, const vd_box* box,
                         const vd_geom* geom, const vd_params* prm, double out[3]);
+
+/* Vidyut::ComputeDt (Source/TimeStep.cpp:14-31) on host scalars: with
+ * adaptive_dt and cur_time > adaptive_dt_delay, the next dt is
+ * min(old_dt * dt_stretch, clamp(min(dt_edrift * advective_cfl (with
+ * transport), dt_ediff * diffusive_cfl (with transport), dt_diel *
+ * dielectric_cfl), dt_min, dt_max)); otherwise old_dt.  ts = the three
+ * time scales of vd_find_time_scales. */
+int vd_compute_dt(const vd_params* prm, double cur_time, double old_dt, const double ts[3], double* dt);
 
 /* Vidyut::ErrorEst kernels (Source/Tagging.H:8-69, Source/Vidyut.cpp:
  * 221-253): state needs 1 ghost; tags[cell] = tagval where

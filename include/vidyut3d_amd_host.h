@@ -24,9 +24,14 @@ int vd_sim_set_state(vd_sim* s, const double* host, double time);
 int vd_sim_get_state(vd_sim* s, double* host);
 /* device view of phi_new (no ghost cells) */
 int vd_sim_state_view(vd_sim* s, vd_array4* out);
-/* Vidyut::Evolve for nsteps at fixed dt with ncorr correctors */
+/* Vidyut::Evolve for nsteps with ncorr correctors.  dt is vidyut.dt: the
+ * step of every step, or with prm->adaptive_dt the step before the
+ * simulation's first (each step's dt then comes from ComputeDt of the
+ * previous dt, continuing across calls and restarts) */
 int vd_sim_evolve(vd_sim* s, int nsteps, double dt, int ncorr);
 double vd_sim_time(const vd_sim* s);
+/* the last step's dt (0 before the first step) */
+double vd_sim_dt(const vd_sim* s);
 long long vd_sim_mg_iterations(const vd_sim* s);
 /* Vidyut::WritePlotFile (Source/Plot.cpp:29-38): AMReX HyperCLaw-V1.1 */
 int vd_sim_write_plotfile(vd_sim* s, const char* dir);

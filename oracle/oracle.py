@@ -212,17 +212,22 @@ def mlmg_assemble(grid, prob):
     return A, bs
 
 
-def evolve(case, nsteps=None, ncorr=1, time=0.0):
-    """Runs Vidyut::Evolve on a copy of case.state; returns (state, time)."""
+def evolve(case, nsteps=None, ncorr=1, time=0.0, dt=None, return_dt=False):
+    """Runs Vidyut::Evolve on a copy of case.state; returns (state, time)
+    (and the last step's dt with return_dt).  dt: the step before the first
+    (case.dt by default)."""
     grid = Grid(case.geom)
     pn = case.state.copy()
     po = np.zeros_like(pn)
     t = C.c_double(time)
+    d = C.c_double(case.dt if dt is None else dt)
     n = case.nsteps if nsteps is None else nsteps
     st = lib().or_evolve(C.byref(case.mech), C.byref(grid.view(pn)), C.byref(grid.view(po)),
                          C.byref(grid.box), C.byref(case.geom), C.byref(case.params), int(n),
-                         C.c_double(case.dt), int(ncorr), C.byref(t))
+                         C.byref(d), int(ncorr), C.byref(t))
     _ck(st, "or_evolve", (abi.VD_OK, abi.VD_ENOCONV))
+    if return_dt:
+        return pn, t.value, d.value
     return pn, t.value
 
 

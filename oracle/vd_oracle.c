@@ -89,6 +89,14 @@ void or_params_default(vd_params* p) {
     p->mg_pre_smooth = 2;
     p->mg_post_smooth = 2;
     p->mg_precision = 0;
+    p->adaptive_dt = 0;
+    p->advective_cfl = 0.7;
+    p->diffusive_cfl = 1.0;
+    p->dielectric_cfl = 1.0;
+    p->dt_min = 1e-14;
+    p->dt_max = 1e-9;
+    p->adaptive_dt_delay = 0.0;
+    p->dt_stretch = DBL_MAX;
     for (int d = 0; d < 3; d++) {
         p->pot_bc_lo[d] = p->pot_bc_hi[d] = VD_ROBINBC;
         p->eden_bc_lo[d] = p->eden_bc_hi[d] = VD_ROBINBC;
@@ -1594,9 +1602,21 @@ int or_implicit_solve_scalar(const vd_mech_desc* m, int spec, const vd_array4* s
  * Vidyut::Evolve (Source/Evolve.cpp:33-369), single level, fixed dt,
  * no output.  Sborder_old is only read in its valid region, so phi_old
  * is used directly. */
+/* Vidyut::ComputeDt (Source/TimeStep.cpp:14-31) */
+static double compute_dt(const vd_params* prm, double cur_time, double old_dt, const double ts[3]) {
+    if (!(prm->adaptive_dt && cur_time > prm->adaptive_dt_delay)) return old_dt;
+    double trans_min_dt = DBL_MAX;
+    if (prm->do_transport) trans_min_dt = fmin(ts[0] * prm->advective_cfl, ts[1] * prm->diffusive_cfl);
+    double adp = fmin(trans_min_dt, ts[2] * prm->dielectric_cfl);
+    if (adp > prm->dt_max) adp = prm->dt_max;
+    if (adp < prm->dt_min) adp = prm->dt_min;
+    return fmin(old_dt * prm->dt_stretch, adp);
+}
+
 int or_evolve(const vd_mech_desc* m, vd_array4* phi_new, vd_array4* phi_old, const vd_box* box,
-              const vd_geom* g, const vd_params* prm, int nsteps, double dt, int ncorr,
+              const vd_geom* g, const vd_params* prm, int nsteps, double* dt_io, int ncorr,
               double* time) {
+    double dt = *dt_io;
     int ns = m->nspec, nv = VD_NVAR(ns), dim = g->dim, E = m->e_idx;
     int ng = prm->hyp_order == 5 ? 3 : (prm->hyp_order == 2 ? 2 : 1);
     if (check_flux_args(prm)) return VD_EINVAL;
@@ -1621,6 +1641,7 @@ int or_evolve(const vd_mech_desc* m, vd_array4* phi_new, vd_array4* phi_old, con
     for (int step = 0; step < nsteps; step++) {
         double ts[3];
         or_find_time_scales(m, phi_new, box, g, prm, ts);
+        dt = compute_dt(prm, cur, dt, ts);
         for (int q = 0; q < nv; q++)
             for (int k = box->lo[2]; k <= box->hi[2]; k++)
                 for (int j = box->lo[1]; j <= box->hi[1]; j++)
@@ -1698,6 +1719,7 @@ int or_evolve(const vd_mech_desc* m, vd_array4* phi_new, vd_array4* phi_old, con
         cur += dt;
     }
     *time = cur;
+    *dt_io = dt;
     free(sb.p);
     free(rxn.p);
     free(expl.p);

@@ -64,11 +64,13 @@ int or_fill_patch_two_levels(const vd_array4* crse, const vd_geom* cg, const vd_
                              const vd_box* fboxes, int nfine, vd_array4* dst, const vd_geom* fg,
                              const int ratio[3], int scomp, int ncomp);
 
-/* Vidyut::Evolve (Source/Evolve.cpp:17-376) for `nsteps` steps at fixed dt
- * on a single-level, single-box domain; phi_new/phi_old have no ghosts.
- * ncorr = num_timestep_correctors.  time is advanced in place. */
+/* Vidyut::Evolve (Source/Evolve.cpp:17-376) for `nsteps` steps on a
+ * single-level, single-box domain; phi_new/phi_old have no ghosts.
+ * ncorr = num_timestep_correctors.  *dt: in, the step before the first
+ * (vidyut.dt); out, the last step's dt (each step's dt from ComputeDt when
+ * prm->adaptive_dt).  time is advanced in place. */
 int or_evolve(const vd_mech_desc* m, vd_array4* phi_new, vd_array4* phi_old, const vd_box* box,
-              const vd_geom* g, const vd_params* prm, int nsteps, double dt, int ncorr,
+              const vd_geom* g, const vd_params* prm, int nsteps, double* dt, int ncorr,
               double* time);
 
 /* dense assembled operator of the multigrid level-0 problem (tiny grids):

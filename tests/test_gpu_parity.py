@@ -606,3 +606,27 @@ def test_potential_solve_voltage_profiles_match_closed_form_and_oracle(dev, prof
     np.testing.assert_allclose(got[cc["EFX"], 0], np.full((4, 16), -(v2 - v1) / 0.01), rtol=1e-10,
                                atol=1e-8 * scale)
     assert relmax(ref_pn[cc["POT"]], got[cc["POT"]]) < 1e-12
+
+
+# ------------------------------------------------------ adaptive time step
+def test_adaptive_dt_steps_match_oracle_across_calls(dev):
+    """ComputeDt (Source/TimeStep.cpp:14-31) in the GPU driver: the same dt
+    sequence as the oracle's (time after each call), the same states, and
+    calls of 2 + 3 steps equal one call of 5 (the dt carries over)."""
+    c = cases.discharge_2d(n=32, nsteps=5)
+    tight(c.params, 1e-12)
+    c.params.adaptive_dt = 1
+    c.params.dt_max = 1e-6
+    c.params.dt_stretch = 4.0
+    ref, tref, dref = O.evolve(c, 5, return_dt=True)
+    sim = native.Sim(dev, c)
+    sim.evolve(2)
+    sim.evolve(3)
+    got = sim.get_state()
+    assert sim.time == pytest.approx(tref, rel=1e-12, abs=0)
+    assert sim.dt == pytest.approx(dref, rel=1e-12, abs=0)
+    assert dref > 10 * c.dt
+    sim.close()
+    errs = compare.state_errors(ref, got, c.mech.nspec, c.geom.dim)
+    bad = {q: e for q, e in errs.items() if not e <= 1e-10}
+    assert not bad, bad

@@ -85,6 +85,9 @@ class vd_params(C.Structure):
         ("eenrg_bc_lo", C.c_int * 3), ("eenrg_bc_hi", C.c_int * 3),
         ("ion_bc_lo", C.c_int * 3), ("ion_bc_hi", C.c_int * 3),
         ("neutral_bc_lo", C.c_int * 3), ("neutral_bc_hi", C.c_int * 3),
+        ("adaptive_dt", C.c_int), ("advective_cfl", C.c_double), ("diffusive_cfl", C.c_double),
+        ("dielectric_cfl", C.c_double), ("dt_min", C.c_double), ("dt_max", C.c_double),
+        ("adaptive_dt_delay", C.c_double), ("dt_stretch", C.c_double),
     ]
 
 

@@ -15,6 +15,7 @@ Prob.H, inputs) of the reference, restated as data for the C ABI.
 * ``dielectric_poisson`` -- variable-permittivity Poisson problem.
 """
 import math
+import sys
 
 import numpy as np
 
@@ -72,6 +73,10 @@ def default_params():
     p.linsolve_maxiter, p.linsolve_max_coarsening_level = 100, 10
     p.mg_pre_smooth = p.mg_post_smooth = 2
     p.mg_precision = 0
+    p.adaptive_dt = 0
+    p.advective_cfl, p.diffusive_cfl, p.dielectric_cfl = 0.7, 1.0, 1.0
+    p.dt_min, p.dt_max, p.adaptive_dt_delay = 1e-14, 1e-9, 0.0
+    p.dt_stretch = sys.float_info.max
     for name in ("pot", "eden", "eenrg", "ion", "neutral"):
         getattr(p, name + "_bc_lo")[:] = [abi.ROBINBC] * 3
         getattr(p, name + "_bc_hi")[:] = [abi.ROBINBC] * 3

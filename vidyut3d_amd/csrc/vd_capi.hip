@@ -1,4 +1,6 @@
 // vd_capi.hip -- context, memory, case plug-in upload and kernel timing.
+#include <algorithm>
+#include <cfloat>
 #include <cstdarg>
 #include <cstdlib>
 #include <cstring>
@@ -133,6 +135,14 @@ void vd_params_default(vd_params* p) {
     p->linsolve_max_coarsening_level = 10;
     p->mg_pre_smooth = 2;
     p->mg_post_smooth = 2;
+    p->adaptive_dt = 0;
+    p->advective_cfl = 0.7;
+    p->diffusive_cfl = 1.0;
+    p->dielectric_cfl = 1.0;
+    p->dt_min = 1e-14;
+    p->dt_max = 1e-9;
+    p->adaptive_dt_delay = 0.0;
+    p->dt_stretch = DBL_MAX;
     for (int d = 0; d < 3; d++) {
         p->pot_bc_lo[d] = p->pot_bc_hi[d] = VD_ROBINBC;
         p->eden_bc_lo[d] = p->eden_bc_hi[d] = VD_ROBINBC;
@@ -140,6 +150,21 @@ void vd_params_default(vd_params* p) {
         p->ion_bc_lo[d] = p->ion_bc_hi[d] = VD_ROBINBC;
         p->neutral_bc_lo[d] = p->neutral_bc_hi[d] = VD_ROBINBC;
     }
+}
+
+int vd_compute_dt(const vd_params* prm, double cur_time, double old_dt, const double ts[3], double* dt) {
+    if (!prm || !ts || !dt) return VD_EINVAL;
+    double v = old_dt;
+    if (prm->adaptive_dt && cur_time > prm->adaptive_dt_delay) {
+        double trans_min_dt = DBL_MAX;
+        if (prm->do_transport) trans_min_dt = std::min(ts[0] * prm->advective_cfl, ts[1] * prm->diffusive_cfl);
+        double adp = std::min(trans_min_dt, ts[2] * prm->dielectric_cfl);
+        if (adp > prm->dt_max) adp = prm->dt_max;
+        if (adp < prm->dt_min) adp = prm->dt_min;
+        v = std::min(old_dt * prm->dt_stretch, adp);
+    }
+    *dt = v;
+    return VD_OK;
 }
 
 int vd_ctx_create(int device, vd_ctx** out) {

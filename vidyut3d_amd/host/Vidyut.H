@@ -74,12 +74,16 @@ class Vidyut {
                               std::vector<MultiFab>& Sborder_old, std::vector<MultiFab>& dsdt_expl,
                               const int* bc_lo, const int* bc_hi, std::vector<FaceMF>& grad_fc);
     int find_time_scales(int lev, Real& dt_edrift, Real& dt_ediff, Real& dt_diel_relax);
+    // ComputeDt (Source/TimeStep.cpp:14-31): dt in, the next step's dt out
+    int ComputeDt(Real cur_time, Real& dt);
     int ErrorEst(int lev, vd_tagarray4& tags, Real time, int ngrow);
     int FillPatch(int lev, Real time, MultiFab& mf, int icomp, int ncomp);
 
     // ---- driver ---------------------------------------------------------
-    // Evolve (Source/Evolve.cpp:33-369) for nsteps at fixed dt
+    // Evolve (Source/Evolve.cpp:33-369) for nsteps: at fixed dt, or with
+    // adaptive_dt from dt (the first call) or the last step's dt
     int Evolve(int nsteps, Real dt, int ncorr);
+    Real dt() const { return dt_; }
     int WritePlotFile(const std::string& dir) const;
     int WriteCheckpointFile(const std::string& dir) const;
     int ReadCheckpointFile(const std::string& dir);
@@ -124,8 +128,8 @@ class Vidyut {
     int alloc_step_data();
     int step_body(std::vector<MultiFab>& sb, std::vector<MultiFab>& sb_old, Real cur_time, Real dt,
                   bool pot_in_sb);
-    int evolve_one_corrector(int nsteps, Real dt);
-    int evolve_corrected(int nsteps, Real dt, int ncorr);
+    int evolve_one_corrector(int nsteps);
+    int evolve_corrected(int nsteps, int ncorr);
     // where a step writes its results and reads the state (phi_new unless
     // the one-corrector driver rotates buffers)
     MultiFab& out() { return out_ ? *out_ : phi_new_[0]; }

@@ -388,11 +388,21 @@ int Vidyut::ErrorEst(int lev, vd_tagarray4& tags, Real time, int ngrow) {
                         tag_thr_.data(), tag_gthr_.data(), 1);
 }
 
-// Source/Evolve.cpp:33-369 on one level with fixed dt
+// Vidyut::ComputeDt (Source/TimeStep.cpp:14-31): the next step's dt from
+// the time scales of this step's state (Source/Evolve.cpp:41-64)
+int Vidyut::ComputeDt(Real cur_time, Real& dt) {
+    Real ts[3];
+    TRY(find_time_scales(0, ts[0], ts[1], ts[2]));
+    return vd_compute_dt(&prm_, cur_time, dt, ts, &dt);
+}
+
+// Source/Evolve.cpp:33-369 on one level.  dt is vidyut.dt: every step's dt
+// with a fixed step; with adaptive_dt the step before a simulation's first
+// (later calls continue from the last step's dt, as the reference's dt[0])
 int Vidyut::Evolve(int nsteps, Real dt, int ncorr) {
-    dt_ = dt;
+    if (!(prm_.adaptive_dt && dt_ > 0.0)) dt_ = dt;
     out_ = state_ = nullptr;
-    const int st = ncorr == 1 ? evolve_one_corrector(nsteps, dt) : evolve_corrected(nsteps, dt, ncorr);
+    const int st = ncorr == 1 ? evolve_one_corrector(nsteps) : evolve_corrected(nsteps, ncorr);
     out_ = state_ = nullptr;
     if (st != VD_OK) return st;
     return vd_ctx_sync(ctx_);
@@ -466,7 +476,7 @@ int Vidyut::step_body(std::vector<MultiFab>& sb, std::vector<MultiFab>& sb_old, 
 // phi_new in the last step of the call.  The components it does not write
 // are copied over from the state (see below).  Sborder_old is Sborder: the
 // implicit solves read only its species and EEN, which equal the state's.
-int Vidyut::evolve_one_corrector(int nsteps, Real dt) {
+int Vidyut::evolve_one_corrector(int nsteps) {
     MultiFab* P = &phi_new_[0];
     std::vector<MultiFab>* G[2] = {&Sborder_, &Swork_};
     MultiFab* cur = P;
@@ -478,9 +488,9 @@ int Vidyut::evolve_one_corrector(int nsteps, Real dt) {
         state_ = cur;
         out_ = out;
         phi_old_stale_ = false;
-        Real dt_edrift, dt_ediff, dt_diel;
-        TRY(find_time_scales(0, dt_edrift, dt_ediff, dt_diel));
         const Real cur_time = t_new_;
+        TRY(ComputeDt(cur_time, dt_));
+        const Real dt = dt_;
         // species, EEN, ETEMP into Sborder (a copy in the first step, the
         // ghost cells only after)
         TRY(vd_fill_patch(ctx_, &cur->array(), &sb[0].array(), &dom_, &geom_, 0, ns_ + 2));
@@ -519,11 +529,11 @@ int Vidyut::evolve_one_corrector(int nsteps, Real dt) {
 }
 
 // More than one corrector (predictor-corrector averaging with phi_old).
-int Vidyut::evolve_corrected(int nsteps, Real dt, int ncorr) {
+int Vidyut::evolve_corrected(int nsteps, int ncorr) {
     for (int step = 0; step < nsteps; step++) {
-        Real dt_edrift, dt_ediff, dt_diel;
-        TRY(find_time_scales(0, dt_edrift, dt_ediff, dt_diel));
         const Real cur_time = t_new_;
+        TRY(ComputeDt(cur_time, dt_));
+        const Real dt = dt_;
         TRY(phi_old_[0].copy_from(phi_new_[0], 0, 0, nvar_));
         phi_old_stale_ = false;
         for (int niter = 0; niter < ncorr; niter++) {
@@ -935,6 +945,7 @@ int vd_sim_evolve(vd_sim* s, int nsteps, double dt, int ncorr) {
 }
 
 double vd_sim_time(const vd_sim* s) { return s->v->time(); }
+double vd_sim_dt(const vd_sim* s) { return s->v->dt(); }
 long long vd_sim_mg_iterations(const vd_sim* s) { return s->v->mg_iterations(); }
 
 int vd_sim_write_plotfile(vd_sim* s, const char* dir) { return s->v->WritePlotFile(dir); }

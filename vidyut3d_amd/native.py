@@ -41,6 +41,8 @@ def lib():
         L.vd_ctx_stream.restype = C.c_void_p
         L.vd_sim_time.restype = C.c_double
         L.vd_sim_time.argtypes = [C.c_void_p]
+        L.vd_sim_dt.restype = C.c_double
+        L.vd_sim_dt.argtypes = [C.c_void_p]
         L.vd_sim_mg_iterations.restype = C.c_longlong
         L.vd_sim_mg_iterations.argtypes = [C.c_void_p]
         L.vd_mech_find_id.argtypes = [C.c_void_p, C.c_char_p]
@@ -333,6 +335,11 @@ class Sim:
     @property
     def time(self):
         return lib().vd_sim_time(self.ptr)
+
+    @property
+    def dt(self):
+        """the last step's dt"""
+        return lib().vd_sim_dt(self.ptr)
 
     @property
     def mg_iterations(self):
