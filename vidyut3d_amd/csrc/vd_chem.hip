@@ -215,6 +215,9 @@ struct JouleArgs {
 };
 
 // Vidyut::compute_elecenergy_source (Source/ElecEnergySource.cpp:65-171)
+// STATE: the electron coefficients are fits of the face state (else
+// constants / Te only; a.state_trans mirrors it)
+template <bool STATE>
 __global__ void __launch_bounds__(256) k_joule(JouleArgs a) {
     const DevMech& M = *a.M;
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
@@ -249,10 +252,10 @@ __global__ void __launch_bounds__(256) k_joule(JouleArgs a) {
             double efield_face = EF(face[0], face[1], face[2]);
             double gradne_face = GN(face[0], face[1], face[2]);
             double mu, dcoeff;
-            if (a.const_ele_trans || a.state_trans) {
+            if (a.const_ele_trans || STATE) {
                 double ndens = 0.0;
                 for (int sp = 0; sp < ns; sp++) ndens += 0.5 * (S[(int64_t)sp * N + ol] + S[(int64_t)sp * N + orr]);
-                if (a.const_ele_trans) {
+                if (!STATE || a.const_ele_trans) {
                     mu = a.ele_mob / ndens;
                     dcoeff = a.ele_diff / ndens;
                 } else {
@@ -311,7 +314,8 @@ int elecenergy_source(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, const 
     a.ny = box->hi[1] - box->lo[1] + 1;
     a.nz = box->hi[2] - box->lo[2] + 1;
     int64_t tot = (int64_t)a.nx * a.ny * a.nz;
-    hipLaunchKernelGGL(k_joule, grid1d(tot, 256), dim3(256), 0, ctx->stream, a);
+    if (a.state_trans) hipLaunchKernelGGL(k_joule<true>, grid1d(tot, 256), dim3(256), 0, ctx->stream, a);
+    else hipLaunchKernelGGL(k_joule<false>, grid1d(tot, 256), dim3(256), 0, ctx->stream, a);
     VD_CHECK_LAUNCH();
     return VD_OK;
 }

@@ -3192,6 +3192,7 @@ struct CellDArgs {
 };
 
 // cell specDiff on the box grown by one (ScalarSolve.cpp:483-497)
+template <bool STATE>
 __global__ void __launch_bounds__(256) k_cell_diff(CellDArgs a, double* __restrict__ dcell) {
     const int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.gnx * a.gny * a.gnz) return;
@@ -3204,7 +3205,7 @@ __global__ void __launch_bounds__(256) k_cell_diff(CellDArgs a, double* __restri
     const int64_t N = a.sb.ns;
     const double te = S[(int64_t)VD_ETEMP(a.ns) * N + o];
     double D;
-    if (a.state) {
+    if (STATE) {
         // ndens and |E| of the cell (ScalarSolve.cpp:484-490)
         double nd = 0.0, es = 0.0;
         for (int sp = 0; sp < a.ns; sp++) nd += S[(int64_t)sp * N + o];
@@ -3238,6 +3239,9 @@ struct SpecBcSides {
 };
 
 // plasmachem_transport::species_bc (Source/PlasmaChem.H:74-184)
+// STATE: the species' coefficients are fits of the state (then ndens and
+// |E| of the boundary cell are formed)
+template <bool STATE>
 __global__ void k_species_bc(SpecBcArgs a, SpecBcSides sides) {
     a.d = sides.d[blockIdx.y];
     a.side = sides.side[blockIdx.y];
@@ -3264,35 +3268,38 @@ __global__ void k_species_bc(SpecBcArgs a, SpecBcSides sides) {
     double sgn = a.side ? 1.0 : -1.0;
     double Te = S[VD_ETEMP(ns) * N + oc];
     // ndens and |E| of the boundary cell (PlasmaChem.H:107-108,114-116)
-    double nd = 0.0, es = 0.0;
-    for (int sp = 0; sp < ns; sp++) nd += S[(int64_t)sp * N + oc];
-    for (int d = 0; d < 3; d++) {
-        const double e = S[(int64_t)(VD_EFX(ns) + d) * N + oc];
-        es += e * e;
+    double nd = 0.0, emag = 0.0;
+    if (STATE) {
+        double es = 0.0;
+        for (int sp = 0; sp < ns; sp++) nd += S[(int64_t)sp * N + oc];
+        for (int d = 0; d < 3; d++) {
+            const double e = S[(int64_t)(VD_EFX(ns) + d) * N + oc];
+            es += e * e;
+        }
+        emag = sqrt(es);
     }
-    const double emag = sqrt(es);
     double v;
     if (a.spec == E) {
         double ne = S[(int64_t)E * N + oc];
-        double D = spec_diff_st(M, a.spec, Te, a.Tg, nd, emag);
+        double D = STATE ? spec_diff_st(M, a.spec, Te, a.Tg, nd, emag) : spec_diff(M, a.spec, Te, a.Tg);
         double cbar = sqrt(8.0 * VD_K_B * Te / VD_PI / VD_ME);
         double flx = ne * cbar / 4.0 * sgn;
         v = -flx / D;
     } else if (a.spec == ns) {
         double ne = S[(int64_t)E * N + oc];
-        double D = spec_diff_st(M, a.spec, Te, a.Tg, nd, emag);
+        double D = STATE ? spec_diff_st(M, a.spec, Te, a.Tg, nd, emag) : spec_diff(M, a.spec, Te, a.Tg);
         double cbar = sqrt(8.0 * VD_K_B * Te / VD_PI / VD_ME);
         double flx = (2.0 * VD_K_B * Te) * ne * cbar / 4.0 * sgn;
         v = -flx / D;
     } else if (M.charge[a.spec] != 0) {
-        double mu = spec_mob_st(M, a.spec, Te, a.Tg, nd, emag);
-        double D = spec_diff_st(M, a.spec, Te, a.Tg, nd, emag);
+        double mu = STATE ? spec_mob_st(M, a.spec, Te, a.Tg, nd, emag) : spec_mob(M, a.spec);
+        double D = STATE ? spec_diff_st(M, a.spec, Te, a.Tg, nd, emag) : spec_diff(M, a.spec, Te, a.Tg);
         double efield = S[(int64_t)(VD_EFX(ns) + a.d) * N + oc];
         double idf = mu * S[(int64_t)a.spec * N + oc] * efield;
         double dotn = idf * sgn;
         v = (dotn >= 0.0) ? (-idf / D) : 0.0;
     } else {
-        double D = spec_diff_st(M, a.spec, Te, a.Tg, nd, emag);
+        double D = STATE ? spec_diff_st(M, a.spec, Te, a.Tg, nd, emag) : spec_diff(M, a.spec, Te, a.Tg);
         double cbar = sqrt(8.0 * VD_K_B * a.Tg / VD_PI / (M.molwt[a.spec] * VD_M_AMU));
         double flx = S[(int64_t)a.spec * N + a.sb.off(face[0], face[1], face[2])] * cbar / 4.0 * sgn;
         v = -flx / D;
@@ -3385,6 +3392,15 @@ int implicit_solve(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sb,
     // face average of it: the solve then runs on the constant-coefficient
     // stencil (no cell-coefficient array, no per-face averages)
     const bool constd = diff_is_uniform(m->hdev, spec);
+    // the coefficients of this solve depend on more of the state than Te
+    // (fits; the energy follows the electron's)
+    bool cd_state;
+    {
+        const DevMech& h = m->hdev;
+        const int ef = spec == ns ? h.e_idx : spec;
+        cd_state = h.dfit_kind[spec] == VD_TR_FIT || h.mfit_kind[spec] == VD_TR_FIT ||
+                   h.dfit_kind[ef] == VD_TR_FIT || h.mfit_kind[ef] == VD_TR_FIT;
+    }
     double bc = 0.0;
     if (constd) {
         const double D = spec_diff(m->hdev, spec, 0.0, Tg);
@@ -3401,14 +3417,10 @@ int implicit_solve(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sb,
         cd.gny = n[1] + 2;
         cd.gnz = g->dim == 3 ? n[2] + 2 : 1;
         cd.Tg = Tg;
-        {
-            const DevMech& h = m->hdev;
-            const int ef = spec == ns ? h.e_idx : spec;  // the energy follows the electron's fits
-            cd.state = h.dfit_kind[spec] == VD_TR_FIT || h.mfit_kind[spec] == VD_TR_FIT ||
-                       h.dfit_kind[ef] == VD_TR_FIT || h.mfit_kind[ef] == VD_TR_FIT;
-        }
-        hipLaunchKernelGGL(k_cell_diff, grid1d((int64_t)cd.gnx * cd.gny * cd.gnz, 256), dim3(256), 0,
-                           ctx->stream, cd, W->dcell);
+        cd.state = cd_state;
+        const dim3 gcd = grid1d((int64_t)cd.gnx * cd.gny * cd.gnz, 256);
+        if (cd.state) hipLaunchKernelGGL(k_cell_diff<true>, gcd, dim3(256), 0, ctx->stream, cd, W->dcell);
+        else hipLaunchKernelGGL(k_cell_diff<false>, gcd, dim3(256), 0, ctx->stream, cd, W->dcell);
     }
     {
         SpecBcArgs b;
@@ -3440,7 +3452,8 @@ int implicit_solve(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sb,
         if (sides.nside > 0) {
             dim3 gb = grid1d(ntmax, 256);
             gb.y = (unsigned)sides.nside;
-            hipLaunchKernelGGL(k_species_bc, gb, dim3(256), 0, ctx->stream, b, sides);
+            if (cd_state) hipLaunchKernelGGL(k_species_bc<true>, gb, dim3(256), 0, ctx->stream, b, sides);
+            else hipLaunchKernelGGL(k_species_bc<false>, gb, dim3(256), 0, ctx->stream, b, sides);
         }
     }
     // Robin species sides: the case's user_transport::species_bc data, or

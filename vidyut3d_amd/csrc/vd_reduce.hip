@@ -28,6 +28,8 @@ __device__ __forceinline__ unsigned long long bits(double v) {
     return (unsigned long long)__double_as_longlong(v);
 }
 
+// STATE: the electron coefficients are fits of the state (a.state_trans)
+template <bool STATE>
 __global__ void __launch_bounds__(256) k_cfl(CflArgs a) {
     __shared__ double red[3][4];
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
@@ -55,7 +57,7 @@ __global__ void __launch_bounds__(256) k_cfl(CflArgs a) {
             for (int sp = 0; sp < ns; sp++) nd += P[(int64_t)sp * N + o];
             dc = a.ele_diff / nd;
             mu = a.ele_mob / nd;
-        } else if (a.state_trans) {
+        } else if (STATE) {
             double nd = 0.0;
             for (int sp = 0; sp < ns; sp++) nd += P[(int64_t)sp * N + o];
             dc = spec_diff_st(M, e, etemp, a.Tg, nd, emag);
@@ -112,7 +114,8 @@ int find_time_scales(vd_ctx* ctx, const vd_mech* m, const vd_array4* phi, const 
     int64_t blocks = (tot + 255) / 256;
     if (blocks > 2048) blocks = 2048;  // grid-stride: 8 blocks per CU
     hipLaunchKernelGGL(k_zero3, dim3(1), dim3(64), 0, ctx->stream, a.out);
-    hipLaunchKernelGGL(k_cfl, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a);
+    if (a.state_trans) hipLaunchKernelGGL(k_cfl<true>, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a);
+    else hipLaunchKernelGGL(k_cfl<false>, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a);
     VD_CHECK_LAUNCH();
     VD_HIP(hipMemcpyAsync(ctx->pinned, acc, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
     VD_HIP(hipStreamSynchronize(ctx->stream));
