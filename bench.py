@@ -39,9 +39,18 @@ sys.path.insert(0, ROOT)
 #            8*N*(3 + n_b); one colour is half (n_b = 0 for the constant-
 #            coefficient potential solve, 1 for the cell coefficient of the
 #            implicit species solves, 3 for face arrays)
-#  transport flux + divergence of one species: 8*N*(NS + 6)
+#  transport flux + divergence of one species, what the launches must
+#            move: 8*N*4*dim with the divergence fused into the last
+#            direction (u and E_d read and a face flux written per other
+#            direction; u, E_d, those faces and rxn read and dsdt written by
+#            the last), 8*N*(4*dim + 2) otherwise
 #  chem      one source evaluation: 8*N*(2*NS + 3)
 #  resid     finest residual + norm: 8*N*(4 + n_b)
+#  mg_coarse the V-cycle legs below the finest level, per level of N_l
+#            cells: (nu1 + nu2)*(3 + n_b) + (2 + n_b) + 2 + 1/4 arrays of N_l
+#  species_solve  one implicit species solve: 8*N*(4 + n_b + iterations *
+#            (3 + n_b)) (+ 3 arrays for the Te update of the energy)
+#  pot_post  the potential's E faces, cell E and E/N: 8*N*(NS + 3 + 2*dim)
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFLOPS = 78.6
 
@@ -64,37 +73,85 @@ def pmc_traffic(kernel, n, precision):
     return None
 
 
-def run_steps(native, abi, dev, case, steps, warmup, family=None):
+def run_steps(native, abi, dev, case, steps, warmup, families=None, state=True):
+    """Times `steps` steps after `warmup` untimed ones; with `families`,
+    HIP-event timing of those kernel families during the timed steps
+    (returns {family: (ms, launches, canonical bytes)})."""
     sim = native.Sim(dev, case)
     sim.evolve(warmup)
     dev.sync()
-    if family is not None:
-        dev.set_kernel_timing(family)
+    if families:
+        dev.set_kernel_timing_mask(families)
     it0 = sim.mg_iterations
     t0 = time.perf_counter()
     sim.evolve(steps)
     dev.sync()
     wall = time.perf_counter() - t0
-    roof = None
-    if family is not None:
-        roof = dev.kernel_timing()
+    fam = None
+    if families:
+        fam = {f: dev.kernel_timing_family(f) for f in families}
         dev.set_kernel_timing(abi.KT_NONE)
-    st = sim.get_state()
+    st = sim.get_state() if state else None
     iters = sim.mg_iterations - it0
     sim.close()
-    return wall, st, iters, roof
+    return wall, st, iters, fam
+
+
+def family_rooflines(abi, fam, steps):
+    """Per-family HIP-event time and canonical bytes -> GB/s and fraction of
+    HBM peak."""
+    out = {}
+    for f, (ms, n, by) in fam.items():
+        if n == 0:
+            continue
+        avg = ms / 1e3 / n
+        gbs = by / n / avg / 1e9
+        out[abi.KT_NAMES[f]] = {"ms_per_step": round(ms / steps, 3), "launches_per_step": round(n / steps, 2),
+                                "avg_launch_us": round(avg * 1e6, 2), "bytes_per_launch": round(by / n),
+                                "achieved_GBs": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
 
 def cpu_baseline_steps(case_fn, sample_n, sample_steps):
+    """The oracle (C, OpenMP over the host's cores) on `sample_steps` steps of
+    the headline case at sample_n^3; returns the baseline and its state."""
     from oracle import oracle as O
     c = case_fn(sample_n)
     t0 = time.perf_counter()
-    O.evolve(c, sample_steps)
+    ref, _ = O.evolve(c, sample_steps)
     dt = time.perf_counter() - t0
     return {"value": c.ncells * sample_steps / dt, "unit": "cell-steps/s",
-            "cores": int(O.lib().or_num_threads()), "kind": "port",
+            "cores": int(O.lib().or_num_threads()), "kind": "port", "cpu_model": cpu_model(),
             "sample": "oracle (C, OpenMP) %d step(s) of the same case at %d^3 cells, %.1f s"
-                      % (sample_steps, sample_n, dt)}
+                      % (sample_steps, sample_n, dt)}, ref
+
+
+def cpu_single_thread(case_fn, sample_n, sample_steps):
+    """The same oracle on one thread, in a child process (OMP_NUM_THREADS=1)."""
+    import subprocess
+    code = ("import sys, time; sys.path.insert(0, %r)\n"
+            "from oracle import oracle as O\nfrom vidyut3d_amd import cases\n"
+            "c = cases.air_3d(n=%d)\nt = time.perf_counter(); O.evolve(c, %d)\n"
+            "print(c.ncells * %d / (time.perf_counter() - t))\n" % (ROOT, sample_n, sample_steps, sample_steps))
+    env = dict(os.environ, OMP_NUM_THREADS="1")
+    try:
+        out = subprocess.run([sys.executable, "-c", code], env=env, capture_output=True, text=True, timeout=600)
+        v = float(out.stdout.strip().splitlines()[-1])
+    except (subprocess.SubprocessError, ValueError, IndexError):
+        return None
+    return {"value": v, "unit": "cell-steps/s", "cores": 1,
+            "sample": "oracle, 1 thread, %d step(s) of the same case at %d^3 cells" % (sample_steps, sample_n)}
 
 
 # ------------------------------------------------------------ sub-benchmarks
@@ -122,12 +179,32 @@ def bench_discharge_2d(native, abi, cases, compare, dev, with_cpu):
     return out
 
 
-def bench_dielectric(native, abi, cases, compare, dev, n):
+def bench_dielectric(native, abi, cases, compare, dev, n, with_oracle):
     """Config 2: n^3 variable-permittivity Poisson, FP64 multigrid vs FP32
     V-cycles in FP64 iterative refinement, both to 1e-10 relative residual
-    from a zero initial guess."""
+    from a zero initial guess; with_oracle, both against the oracle's
+    all-FP64 multigrid solve of the same problem."""
     import ctypes as C
     g, faces, rhs = cases.dielectric_poisson(n=n)
+    oracle_sol = None
+    if with_oracle:
+        from oracle import oracle as O
+        og = O.Grid(g)
+        hf = [np.ascontiguousarray(faces[d][None]) for d in range(3)]
+        hv = [og.view(hf[d], 0, d) for d in range(3)]
+        p = abi.vd_mlmg_problem()
+        p.alpha_a, p.beta = 0.0, 1.0
+        for d in range(3):
+            p.bface[d] = C.pointer(hv[d])
+        p.bc_lo[:] = [0, 0, abi.DIRCBC]
+        p.bc_hi[:] = [0, 0, abi.DIRCBC]
+        p.reltol, p.abstol, p.maxiter = 1e-10, 0.0, 100
+        p.pre_smooth = p.post_smooth = 2
+        p.max_coarsening_level = 10
+        t0 = time.perf_counter()
+        oracle_sol, _ = O.mlmg_solve(og, p, og.zeros(1), np.ascontiguousarray(rhs[None]))
+        log("oracle FP64 solve %.1f s" % (time.perf_counter() - t0))
+        del hf, hv
     grid = native.Grid(dev, g)
     df = [grid.array(1, 0, d, host=np.ascontiguousarray(faces[d][None])) for d in range(3)]
     drhs = grid.array(1, host=rhs[None])
@@ -165,6 +242,9 @@ def bench_dielectric(native, abi, cases, compare, dev, n):
     res.update({"cells": n ** 3, "speedup_mixed_over_fp64": res["fp64"]["solve_s"] / res["mixed"]["solve_s"],
                 "mixed_rel_l2_vs_fp64": err["rel_l2"],
                 "mixed_rms_over_meanabs_vs_fp64": err["rel_rms_over_meanabs"]})
+    if oracle_sol is not None:
+        for prec, key in ((0, "fp64"), (1, "mixed")):
+            res[key]["rel_l2_vs_fp64_oracle"] = compare.field_errors(oracle_sol, sols[prec])["rel_l2"]
     return res
 
 
@@ -249,15 +329,23 @@ def main():
            "resid": abi.KT_RESID, "none": None}[args.kernel]
     case = case_fn(args.n)
     log("headline: %d^3 air-like discharge, %d warmup + %d timed steps" % (args.n, args.warmup, args.steps))
-    wall, st, iters, kt = run_steps(native, abi, dev, case, args.steps, args.warmup, fam)
+    wall, st, iters, _ = run_steps(native, abi, dev, case, args.steps, args.warmup)
     finite = bool(np.isfinite(st).all())
     del st
     value = case.ncells * args.steps / wall
     log("%d steps in %.3f s (%.2f ms/step), %d multigrid iterations, finite %s"
         % (args.steps, wall, wall / args.steps * 1e3, iters, finite))
+    # the same steps again with HIP-event timing of every kernel family (the
+    # value above is measured without the events)
+    families = [abi.KT_FUSED, abi.KT_TRANSPORT, abi.KT_MGCOARSE, abi.KT_SPECIES, abi.KT_CHEM, abi.KT_POTPOST]
+    if fam is not None and fam not in families:
+        families.append(fam)
+    wall_t, _, _, kt = run_steps(native, abi, dev, case_fn(args.n), args.steps, args.warmup, families, state=False)
+    fam_roofs = family_rooflines(abi, kt, args.steps)
+    log("families: " + ", ".join("%s %.3f ms/step %.2f" % (k, v["ms_per_step"], v["frac"]) for k, v in fam_roofs.items()))
     roof = None
-    if kt is not None and kt[1] > 0:
-        ms_total, launches, kbytes = kt
+    if fam is not None and kt[fam][1] > 0:
+        ms_total, launches, kbytes = kt[fam]
         avg_s = ms_total / 1e3 / launches
         byts = kbytes / launches
         ach = byts / avg_s / 1e9
@@ -295,7 +383,8 @@ def main():
         configs["discharge_2d_128"] = bench_discharge_2d(native, abi, cases, compare, dev, not args.no_cpu)
     if want("poisson"):
         log("%d^3 dielectric Poisson FP64 vs mixed" % args.poisson_n)
-        configs["dielectric_poisson"] = bench_dielectric(native, abi, cases, compare, dev, args.poisson_n)
+        configs["dielectric_poisson"] = bench_dielectric(native, abi, cases, compare, dev, args.poisson_n,
+                                                         not args.no_cpu)
     if want("chemistry"):
         log("%d^3 40-species chemistry" % args.chem_n)
         configs["chemistry_40sp_200rx"] = bench_chemistry(native, abi, cases, dev, args.chem_n)
@@ -307,8 +396,27 @@ def main():
     cpu = None
     if not args.no_cpu:
         log("cpu baseline (oracle) %d step(s) at %d^3" % (args.cpu_steps, args.cpu_n))
-        cpu = cpu_baseline_steps(lambda n: case_fn(n, 0), args.cpu_n, args.cpu_steps)
+        cpu, ref = cpu_baseline_steps(lambda n: case_fn(n, 0), args.cpu_n, args.cpu_steps)
+        # the headline legs against the all-FP64 oracle on the same sample
+        errs = {}
+        for prec in (0, 1):
+            c = case_fn(args.cpu_n, precision=prec)
+            sim = native.Sim(dev, c)
+            sim.evolve(args.cpu_steps)
+            got = sim.get_state()
+            sim.close()
+            e = compare.state_errors(ref, got, c.mech.nspec, c.geom.dim)
+            names = [c.mech.names[s].value.decode() for s in range(c.mech.nspec)] + abi.PLASMA_VAR_NAMES
+            q = max(e, key=e.get)
+            errs["fp64" if prec == 0 else "mixed"] = {"worst_rel_l2": e[q], "field": names[q]}
+            del got
+        del ref
+        cpu["rel_l2_vs_fp64_oracle"] = errs
+        log("headline vs oracle after %d step(s): %s" % (args.cpu_steps, errs))
+        cpu["single_thread"] = cpu_single_thread(lambda n: case_fn(n, 0), 64, 1)
 
+    if fam_roofs:
+        configs["kernel_families"] = fam_roofs
     out = {
         "metric": "cell-steps/s FP64 vs mixed-precision+batched; p50/p99 time-to-solution",
         "value": value, "unit": "cell-steps/s", "n_gpus": 1, "steps": args.steps,

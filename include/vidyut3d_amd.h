@@ -477,10 +477,20 @@ enum {
     VD_KT_CHEM = 3,       /* per-cell chemistry source */
     VD_KT_RESID = 4,      /* multigrid residual, finest level */
     VD_KT_FILL = 5,       /* ghost fill */
-    VD_KT_FUSED = 6       /* fused multi-sweep red-black smoother, finest level */
+    VD_KT_FUSED = 6,      /* fused multi-sweep red-black smoother, finest level */
+    VD_KT_MGCOARSE = 7,   /* a V-cycle's legs below the finest level (sweeps,
+                             restriction, prolongation, the coarse tail) */
+    VD_KT_SPECIES = 8,    /* one whole implicit species solve */
+    VD_KT_POTPOST = 9     /* the potential solve's E faces, cell E and E/N */
 };
+/* times one family (VD_KT_NONE: none); resets the totals */
 int vd_ctx_set_kernel_timing(vd_ctx* ctx, int family);
+/* times every family whose bit (1 << family) is set; resets the totals */
+int vd_ctx_set_kernel_timing_mask(vd_ctx* ctx, unsigned mask);
+/* totals of every timed family */
 int vd_ctx_kernel_timing(vd_ctx* ctx, double* total_ms, long long* launches, double* bytes);
+/* totals of one family */
+int vd_ctx_kernel_timing_family(vd_ctx* ctx, int family, double* total_ms, long long* launches, double* bytes);
 
 #ifdef __cplusplus
 }

@@ -18,7 +18,9 @@ VD_MAX_REACTIONS = 256
 VD_MAX_REACTANTS = 3
 VD_MAX_PRODUCTS = 4
 NUM_PLASMAVARS = 10
-KT_NONE, KT_GSRB, KT_TRANSPORT, KT_CHEM, KT_RESID, KT_FILL, KT_FUSED = range(7)
+KT_NONE, KT_GSRB, KT_TRANSPORT, KT_CHEM, KT_RESID, KT_FILL, KT_FUSED, KT_MGCOARSE, KT_SPECIES, KT_POTPOST = range(10)
+KT_NAMES = {KT_GSRB: "gsrb", KT_TRANSPORT: "transport", KT_CHEM: "chem", KT_RESID: "resid", KT_FILL: "fill",
+            KT_FUSED: "fused", KT_MGCOARSE: "mg_coarse", KT_SPECIES: "species_solve", KT_POTPOST: "pot_post"}
 
 # Source/UnivConstants.H:4-19
 N_A = 6.02214076e23

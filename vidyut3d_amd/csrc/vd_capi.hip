@@ -91,18 +91,24 @@ static hipEvent_t kt_event(vd_ctx* ctx) {
     return e;
 }
 
+static bool kt_on(const vd_ctx* ctx, int family) {
+    return family > 0 && family < 32 && (ctx->kt_mask >> family) & 1u;
+}
+
 void kt_begin(vd_ctx* ctx, int family) {
-    if (ctx->kt_family == VD_KT_NONE || ctx->kt_family != family) return;
+    if (!kt_on(ctx, family)) return;
     hipEvent_t a = kt_event(ctx), b = kt_event(ctx);
     if (!a || !b) return;
     (void)hipEventRecord(a, ctx->stream);
-    ctx->kt_events.push_back({a, b});
+    ctx->kt_events.push_back({a, b, family});
+    ctx->kt_open[family] = (long)ctx->kt_events.size();
 }
 
 void kt_end(vd_ctx* ctx, int family, double bytes) {
-    if (ctx->kt_family == VD_KT_NONE || ctx->kt_family != family || ctx->kt_events.empty()) return;
-    (void)hipEventRecord(ctx->kt_events.back().second, ctx->stream);
-    ctx->kt_bytes += bytes;
+    if (!kt_on(ctx, family) || ctx->kt_open[family] == 0) return;
+    (void)hipEventRecord(ctx->kt_events[ctx->kt_open[family] - 1].b, ctx->stream);
+    ctx->kt_open[family] = 0;
+    ctx->kt_bytes[family] += bytes;
 }
 
 }  // namespace vd
@@ -447,27 +453,48 @@ int vd_mech_find_id(const vd_mech* m, const char* name) {
     return -1;
 }
 
-int vd_ctx_set_kernel_timing(vd_ctx* ctx, int family) {
+int vd_ctx_set_kernel_timing_mask(vd_ctx* ctx, unsigned mask) {
+    if (!ctx) return VD_EINVAL;
     VD_HIP(hipStreamSynchronize(ctx->stream));
-    ctx->kt_family = family;
+    ctx->kt_mask = mask & ~1u;
     ctx->kt_events.clear();
     ctx->kt_next = 0;
-    ctx->kt_bytes = 0.0;
+    for (int f = 0; f < 32; f++) {
+        ctx->kt_bytes[f] = 0.0;
+        ctx->kt_open[f] = 0;
+    }
+    return VD_OK;
+}
+
+int vd_ctx_set_kernel_timing(vd_ctx* ctx, int family) {
+    if (!ctx || family < 0 || family >= 32) return VD_EINVAL;
+    return vd_ctx_set_kernel_timing_mask(ctx, family == VD_KT_NONE ? 0u : 1u << family);
+}
+
+int vd_ctx_kernel_timing_family(vd_ctx* ctx, int family, double* total_ms, long long* launches,
+                                double* bytes) {
+    if (!ctx) return VD_EINVAL;
+    VD_HIP(hipStreamSynchronize(ctx->stream));
+    double tot = 0.0;
+    long long n = 0;
+    double by = 0.0;
+    for (auto& e : ctx->kt_events) {
+        if (family != VD_KT_NONE && e.family != family) continue;
+        float ms = 0.f;
+        VD_HIP(hipEventElapsedTime(&ms, e.a, e.b));
+        tot += ms;
+        n++;
+    }
+    for (int f = 1; f < 32; f++)
+        if (family == VD_KT_NONE || f == family) by += ctx->kt_bytes[f];
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = n;
+    if (bytes) *bytes = by;
     return VD_OK;
 }
 
 int vd_ctx_kernel_timing(vd_ctx* ctx, double* total_ms, long long* launches, double* bytes) {
-    VD_HIP(hipStreamSynchronize(ctx->stream));
-    double tot = 0.0;
-    for (auto& pr : ctx->kt_events) {
-        float ms = 0.f;
-        VD_HIP(hipEventElapsedTime(&ms, pr.first, pr.second));
-        tot += ms;
-    }
-    if (total_ms) *total_ms = tot;
-    if (launches) *launches = (long long)ctx->kt_events.size();
-    if (bytes) *bytes = ctx->kt_bytes;
-    return VD_OK;
+    return vd_ctx_kernel_timing_family(ctx, VD_KT_NONE, total_ms, launches, bytes);
 }
 
 }  // extern "C"

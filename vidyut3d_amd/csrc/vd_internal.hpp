@@ -240,11 +240,17 @@ struct vd_ctx {
     double* hnorm = nullptr;
     double* hnorm_d = nullptr;  // its device address
     unsigned long long norm_seq = 0;
-    // kernel timing
-    int kt_family = VD_KT_NONE;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> kt_events;
+    // kernel timing: families selected by bit, one event pair per launch
+    // (families may nest: a species solve contains sweep launches)
+    struct KtEvent {
+        hipEvent_t a, b;
+        int family;
+    };
+    unsigned kt_mask = 0;
+    std::vector<KtEvent> kt_events;
     std::vector<hipEvent_t> kt_pool;
-    double kt_bytes = 0.0;
+    double kt_bytes[32] = {0.0};
+    long kt_open[32] = {0};  // index of the family's open pair + 1 (0: none)
     size_t kt_next = 0;
 };
 

@@ -59,7 +59,8 @@ struct Lev {
     T* bf[3] = {nullptr, nullptr, nullptr};
     T* c0[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     T* e = nullptr;
-    T* e2 = nullptr;  // second correction buffer of the fused smoother (finest level)
+    T* e2 = nullptr;  // second correction buffer of the fused smoother
+    int fzc = 0;      // its plane chunk on this level (0: per-colour sweeps)
     T* r = nullptr;
     T* p = nullptr;  // CG scratch (bottom level)
     T* q = nullptr;
@@ -1856,6 +1857,28 @@ static int fused_zc(const int n[3]) {
     return 0;
 }
 
+// plane chunk of the fused smoother on a coarse level of a V-cycle
+// (VD_FUSED_COARSE=1; off by default): the largest of 32 / 16 planes that
+// still gives 256 blocks, else 8 planes.  One launch of two red-black
+// sweeps replaces four colour launches, but with few blocks per level its
+// plane loop takes longer than the launches it saves (FP64 256^3
+// headline: coarse legs 1.81 -> 1.90 ms per step).  0 when the level
+// cannot take it (the conditions of fused_capable; levels of the
+// one-workgroup coarse tail never sweep here).
+static int fused_zc_coarse(const MGWork& W, const int n[3]) {
+    const char* e = getenv("VD_FUSED_COARSE");
+    if (!(e && strcmp(e, "1") == 0)) return 0;
+    const char* f = getenv("VD_FUSED_SMOOTHER");
+    if (f && strcmp(f, "0") == 0) return 0;
+    if (W.dim != 3 || W.per[2] || n[0] % 2 || n[1] % 2 || n[0] < 8 || n[1] < 8 || n[2] < 2 ||
+        ncell(n) <= VC_TAIL_CELLS)
+        return 0;
+    const int64_t tiles = (int64_t)((n[0] + FZ_IX - 1) / FZ_IX) * ((n[1] + FZ_IY - 1) / FZ_IY);
+    if (tiles * ((n[2] + 31) / 32) >= 256) return 32;
+    if (tiles * ((n[2] + 15) / 16) >= 256) return 16;
+    return std::min(8, n[2]);
+}
+
 // finest levels the fused smoother handles: 3-D, z not periodic, even
 // nx and ny, 32-bit byte offsets, enough blocks; VD_FUSED_SMOOTHER=0
 // turns it off
@@ -1867,12 +1890,11 @@ static bool fused_capable(const MGWork& W) {
 }
 
 template <class T, int BM, int S, int RES>
-static void launch_fused_k(vd_ctx* ctx, const LevV<T>& v, const T* xin, T* xout, const T* rhs,
+static void launch_fused_k(vd_ctx* ctx, const LevV<T>& v, int zc, const T* xin, T* xout, const T* rhs,
                            double* part, int* nblk, double* xacc = nullptr) {
     // the kernel reads xin unless it starts from zero (ZI)
     const int ntx = (v.nx + FZ_IX - 1) / FZ_IX, nty = (v.ny + FZ_IY - 1) / FZ_IY;
-    const int n3[3] = {v.nx, v.ny, v.nz};
-    const int zc = std::max(1, fused_zc(n3)), nzc = (v.nz + zc - 1) / zc;
+    const int nzc = (v.nz + zc - 1) / zc;
     const int nb = ntx * nty * nzc;
     if (nblk) *nblk = nb;
     if constexpr (RES == 0) {
@@ -1894,16 +1916,19 @@ static void launch_fused_k(vd_ctx* ctx, const LevV<T>& v, const T* xin, T* xout,
 // xout in one launch, for face-array (bm 0) or constant (bm 1) coefficients;
 // with res (FP64, one sweep) also the per-block
 // max-norm of the residual of the result into part[0..*nblk)
+// zc: the level's plane chunk (Lev::fzc); only finest-level launches are
+// timed as the VD_KT_FUSED family
 template <class T>
-static void launch_fused(vd_ctx* ctx, const MGWork& W, int bm, const LevV<T>& v, const T* xin, T* xout,
+static void launch_fused(vd_ctx* ctx, const MGWork& W, int bm, const LevV<T>& v, int zc, const T* xin, T* xout,
                          const T* rhs, int nsweep, int res, double* part, int* nblk, double* xacc = nullptr) {
-    kt_begin(ctx, VD_KT_FUSED);
-#define VD_FUSED_BM(BMV)                                                                      \
-    if (nsweep == 2) launch_fused_k<T, BMV, 4, 0>(ctx, v, xin, xout, rhs, part, nblk, xacc);     \
-    else if constexpr (std::is_same<T, double>::value) {                                      \
-        if (res) launch_fused_k<T, BMV, 2, 1>(ctx, v, xin, xout, rhs, part, nblk);          \
-        else launch_fused_k<T, BMV, 2, 0>(ctx, v, xin, xout, rhs, part, nblk, xacc);        \
-    } else launch_fused_k<T, BMV, 2, 0>(ctx, v, xin, xout, rhs, part, nblk, xacc);
+    const bool fine = v.nx == W.n[0] && v.ny == W.n[1] && v.nz == W.n[2];
+    if (fine) kt_begin(ctx, VD_KT_FUSED);
+#define VD_FUSED_BM(BMV)                                                                          \
+    if (nsweep == 2) launch_fused_k<T, BMV, 4, 0>(ctx, v, zc, xin, xout, rhs, part, nblk, xacc);     \
+    else if constexpr (std::is_same<T, double>::value) {                                          \
+        if (res) launch_fused_k<T, BMV, 2, 1>(ctx, v, zc, xin, xout, rhs, part, nblk);          \
+        else launch_fused_k<T, BMV, 2, 0>(ctx, v, zc, xin, xout, rhs, part, nblk, xacc);        \
+    } else launch_fused_k<T, BMV, 2, 0>(ctx, v, zc, xin, xout, rhs, part, nblk, xacc);
     if (bm == 1) {
         VD_FUSED_BM(1)
     } else {
@@ -1914,8 +1939,9 @@ static void launch_fused(vd_ctx* ctx, const MGWork& W, int bm, const LevV<T>& v,
     // one pass: read x (unless zero), rhs and the coefficient data; write x,
     // or read-modify-write the FP64 accumulation target
     const double N = (double)v.nx * v.ny * v.nz;
-    kt_end(ctx, VD_KT_FUSED,
-           sizeof(T) * N * ((xin ? 1 : 0) + 1 + nb) + (xacc ? 16.0 * N : (double)sizeof(T) * N));
+    if (fine)
+        kt_end(ctx, VD_KT_FUSED,
+               sizeof(T) * N * ((xin ? 1 : 0) + 1 + nb) + (xacc ? 16.0 * N : (double)sizeof(T) * N));
 }
 
 template <class T>
@@ -1933,7 +1959,8 @@ static int alloc_levels(MGWork* W, std::vector<Lev<T>>& L, bool share0) {
         void* p;
         VD_TRY(dalloc(W, nc * sizeof(T), &p));
         v.e = (T*)p;
-        if (l == 0 && fused_capable(*W)) {
+        v.fzc = l == 0 ? (fused_capable(*W) ? fused_zc(n) : 0) : fused_zc_coarse(*W, n);
+        if (v.fzc > 0) {
             VD_TRY(dalloc(W, nc * sizeof(T), &p));
             v.e2 = (T*)p;
         }
@@ -2225,9 +2252,9 @@ static void vc_down(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L
         for (int s = 0; s < nu1;) {
             const int k = nu1 - s >= 2 ? 2 : 1;
             if (s == 0) {
-                launch_fused<T>(ctx, W, bm, v, nullptr, F.e, F.r, k, 0, nullptr, nullptr);
+                launch_fused<T>(ctx, W, bm, v, F.fzc, nullptr, F.e, F.r, k, 0, nullptr, nullptr);
             } else {
-                launch_fused<T>(ctx, W, bm, v, F.e, F.e2, F.r, k, 0, nullptr, nullptr);
+                launch_fused<T>(ctx, W, bm, v, F.fzc, F.e, F.e2, F.r, k, 0, nullptr, nullptr);
                 std::swap(F.e, F.e2);
             }
             s += k;
@@ -2339,10 +2366,10 @@ static bool vc_up(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L, 
             const int k = nu2 - s >= 2 ? 2 : 1;
             if (l == 0 && xacc && s + k == nu2) {
                 // last post-smooth of the finest level: x += e in the same pass
-                launch_fused<T>(ctx, W, bm, v, F.e, F.e2, F.r, k, 0, nullptr, nullptr, xacc);
+                launch_fused<T>(ctx, W, bm, v, F.fzc, F.e, F.e2, F.r, k, 0, nullptr, nullptr, xacc);
                 accumulated = true;
             } else {
-                launch_fused<T>(ctx, W, bm, v, F.e, F.e2, F.r, k, 0, nullptr, nullptr);
+                launch_fused<T>(ctx, W, bm, v, F.fzc, F.e, F.e2, F.r, k, 0, nullptr, nullptr);
                 std::swap(F.e, F.e2);
             }
             s += k;
@@ -2362,10 +2389,26 @@ template <class T>
 static bool vcycle(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L, double alpha,
                    double beta, double bconst, int nu1, int nu2, double* xacc = nullptr) {
     const int l0 = vc_tail_level<T>(W, L);
-    for (int l = 0; l < l0; l++) vc_down<T>(ctx, W, bm, L, alpha, beta, bconst, nu1, l);
+    // the legs below the finest level form one timing span (VD_KT_MGCOARSE)
+    // whose canonical bytes are, per level of N cells: nu1 + nu2 sweeps of
+    // (3 + n_b) arrays, the residual-restriction (2 + n_b, + N/8 written)
+    // and the prolongation (2, + N/8 read)
+    double cb = 0.0;
+    const int nb = bm == 1 ? 0 : W.dim;
+    for (int l = 1; l < W.nlev; l++)
+        cb += (double)sizeof(T) * (double)ncell(L[l].n) * ((nu1 + nu2) * (3 + nb) + (2 + nb) + 2 + 0.25);
+    for (int l = 0; l < l0; l++) {
+        vc_down<T>(ctx, W, bm, L, alpha, beta, bconst, nu1, l);
+        if (l == 0) kt_begin(ctx, VD_KT_MGCOARSE);
+    }
+    if (l0 == 0) kt_begin(ctx, VD_KT_MGCOARSE);
     vc_tail<T>(ctx, W, bm, L, alpha, beta, bconst, nu1, nu2, l0);
     bool accumulated = false;
-    for (int l = l0 - 1; l >= 0; l--) accumulated |= vc_up<T>(ctx, W, bm, L, alpha, beta, bconst, nu2, l, xacc);
+    for (int l = l0 - 1; l >= 0; l--) {
+        if (l == 0) kt_end(ctx, VD_KT_MGCOARSE, cb);
+        accumulated |= vc_up<T>(ctx, W, bm, L, alpha, beta, bconst, nu2, l, xacc);
+    }
+    if (l0 == 0) kt_end(ctx, VD_KT_MGCOARSE, cb);
     return accumulated;
 }
 
@@ -2747,7 +2790,8 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
             if (smooth_only && W.x2 && W.c0uni && bm != 2) {
                 // one red-black sweep and the residual norm of its result in one pass
                 int nblk = 0;
-                launch_fused<double>(ctx, W, bm, v0, W.x, W.x2, W.f, 1, 1, W.part, &nblk);
+                launch_fused<double>(ctx, W, bm, v0, std::max(1, fused_zc(W.n)), W.x, W.x2, W.f, 1, 1, W.part,
+                                     &nblk);
                 std::swap(W.x, W.x2);
                 hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, ctx->stream, W.part, nblk,
                                    slot_group(W, 0));
@@ -3175,7 +3219,11 @@ int solve_potential(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array
     pa.nx = n[0];
     pa.ny = n[1];
     pa.nz = n[2];
+    // reads the solution and the species (E/N), writes the potential, E
+    // cell and face components and E/N
+    kt_begin(ctx, VD_KT_POTPOST);
     hipLaunchKernelGGL(k_pot_post, grid1d(nc, 256), dim3(256), 0, ctx->stream, pa);
+    kt_end(ctx, VD_KT_POTPOST, 8.0 * (double)nc * (1 + m->hdev.nspec + 2 + 2 * g->dim));
     VD_CHECK_LAUNCH();
     return st;
 }
@@ -3371,10 +3419,37 @@ __global__ void k_spec_post(SpecPostArgs a) {
     }
 }
 
+static int implicit_solve_body(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sb,
+                               const vd_array4* old, const vd_array4* ds, vd_array4* phi, vd_array4 grad[3],
+                               const vd_box* box, const vd_geom* g, const int bclo[3], const int bchi[3],
+                               const vd_params* prm, double dt, vd_solve_info* info);
+
+// one implicit species solve, timed as a whole (VD_KT_SPECIES) with the
+// canonical bytes: n_old, dsdt and the initial guess read, the solution
+// written, the cell coefficient once (n_b = 1, or 0 for a uniform D), each
+// smoother iteration (3 + n_b) arrays, and for the energy the Te update
+// (EEN, n_e read, Te written)
 int implicit_solve(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sb, const vd_array4* old,
                    const vd_array4* ds, vd_array4* phi, vd_array4 grad[3], const vd_box* box,
                    const vd_geom* g, const int bclo[3], const int bchi[3], const vd_params* prm,
                    double dt, vd_solve_info* info) {
+    vd_solve_info local;
+    if (!info) info = &local;
+    info->iterations = 0;
+    kt_begin(ctx, VD_KT_SPECIES);
+    const int st = implicit_solve_body(ctx, m, spec, sb, old, ds, phi, grad, box, g, bclo, bchi, prm, dt, info);
+    const double nc = (double)(box->hi[0] - box->lo[0] + 1) * (box->hi[1] - box->lo[1] + 1) *
+                      (box->hi[2] - box->lo[2] + 1);
+    const int nb = spec < m->hdev.nspec && diff_is_uniform(m->hdev, spec) ? 0 : 1;
+    kt_end(ctx, VD_KT_SPECIES,
+           8.0 * nc * (4 + nb + info->iterations * (3 + nb) + (spec == m->hdev.nspec ? 3 : 0)));
+    return st;
+}
+
+static int implicit_solve_body(vd_ctx* ctx, const vd_mech* m, int spec, const vd_array4* sb,
+                               const vd_array4* old, const vd_array4* ds, vd_array4* phi, vd_array4 grad[3],
+                               const vd_box* box, const vd_geom* g, const int bclo[3], const int bchi[3],
+                               const vd_params* prm, double dt, vd_solve_info* info) {
     const int ns = m->hdev.nspec, E = m->hdev.e_idx;
     if (spec < 0 || spec > ns) {
         set_error("implicit solve: species %d out of range", spec);

@@ -729,7 +729,18 @@ int transport_rhs(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, const vd_a
             VD_TRY(transport_flux(ctx, m, sb, fl, box, g, bc_lo, bc_hi, prm, dt, specid, nullptr, nullptr, vbuf));
         st = dsdt(ctx, fl, rxn, out, box, g, specid);
     }
-    if (!fbuf) kt_end(ctx, VD_KT_TRANSPORT, 8.0 * cells * (m->hdev.nspec + 6));
+    // what the launches must move, per cell: with the divergence fused into
+    // the last direction, u and E_d read and the face flux written by each
+    // of the other directions, and the last reading u, E_d, those faces and
+    // rxn and writing dsdt (4 dim words); otherwise 3 per direction plus
+    // dsdt's dim + 2; a per-cell drift velocity adds its pass over Sborder
+    if (!fbuf) {
+        const bool fused = prm->do_transport && prm->hyp_order == 5 && g->dim > 1;
+        double words = fused ? 4.0 * g->dim : 4.0 * g->dim + 2.0;
+        if (prm->do_transport && drift_is_state(m->hdev, specid))
+            words += (double)sb->nstride / cells * (m->hdev.nspec + 4 + g->dim);
+        kt_end(ctx, VD_KT_TRANSPORT, 8.0 * cells * words);
+    }
     return st;
 }
 

@@ -113,6 +113,19 @@ class Device:
     def set_kernel_timing(self, family):
         check(lib().vd_ctx_set_kernel_timing(self.ptr, int(family)), "vd_ctx_set_kernel_timing")
 
+    def set_kernel_timing_mask(self, families):
+        mask = 0
+        for f in families:
+            mask |= 1 << int(f)
+        check(lib().vd_ctx_set_kernel_timing_mask(self.ptr, C.c_uint(mask)), "vd_ctx_set_kernel_timing_mask")
+
+    def kernel_timing_family(self, family):
+        """(total ms, launches, canonical bytes) of one timed family"""
+        ms, n, by = C.c_double(), C.c_longlong(), C.c_double()
+        check(lib().vd_ctx_kernel_timing_family(self.ptr, int(family), C.byref(ms), C.byref(n), C.byref(by)),
+              "vd_ctx_kernel_timing_family")
+        return ms.value, n.value, by.value
+
     def kernel_timing(self):
         ms, n, cells = C.c_double(), C.c_longlong(), C.c_double()
         check(lib().vd_ctx_kernel_timing(self.ptr, C.byref(ms), C.byref(n), C.byref(cells)),
