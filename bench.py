@@ -307,7 +307,10 @@ def main():
     ap.add_argument("--only", default="", help="comma-separated subset of: mixed,discharge,poisson,chemistry,serving")
     ap.add_argument("--poisson-n", type=int, default=512)
     ap.add_argument("--chem-n", type=int, default=256)
-    ap.add_argument("--serve-seconds", type=float, default=20.0)
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate of the serving leg (1/s)")
+    ap.add_argument("--serve-sequential", default="sampled", choices=["sampled", "run", "none"],
+                    help="FP64 one-at-a-time baseline of the serving leg: per-shape samples, or a full run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="after the headline, copy 4 components of an n^3 array once (a plain copy of "
                          "8*4*n^3 B each way that scales rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)")
@@ -388,10 +391,15 @@ def main():
     if want("chemistry"):
         log("%d^3 40-species chemistry" % args.chem_n)
         configs["chemistry_40sp_200rx"] = bench_chemistry(native, abi, cases, dev, args.chem_n)
-    if want("serving") and args.serve_seconds > 0:
+    served = []
+    if want("serving") and args.serve_requests > 0:
         from vidyut3d_amd import serve
-        log("serving loop for %.0f s of arrivals" % args.serve_seconds)
-        configs["serving"] = serve.bench(dev_index=0, duration_s=args.serve_seconds)
+        log("serving loop: %d requests, Poisson arrivals at %.2f/s" % (args.serve_requests, args.serve_rate))
+        configs["serving"], served = serve.bench(dev_index=0, requests=args.serve_requests, rate_hz=args.serve_rate,
+                                                 sequential=args.serve_sequential)
+        log("serving: p50 %.2f s, p99 %.2f s, aggregate %.3g cell-steps/s"
+            % (configs["serving"]["p50_time_to_solution_s"], configs["serving"]["p99_time_to_solution_s"],
+               configs["serving"]["aggregate_cell_steps_per_s"]))
 
     cpu = None
     if not args.no_cpu:
@@ -414,6 +422,30 @@ def main():
         cpu["rel_l2_vs_fp64_oracle"] = errs
         log("headline vs oracle after %d step(s): %s" % (args.cpu_steps, errs))
         cpu["single_thread"] = cpu_single_thread(lambda n: case_fn(n, 0), 64, 1)
+        kept = [r for r in served if r.state is not None]
+        if kept:
+            # served (FP32 V-cycle) results, and the same requests run alone
+            # in FP64 on the GPU, against the all-FP64 oracle
+            from oracle import oracle as O
+            rows = []
+            for r in kept:
+                c = r.make_case(0)
+                ref, _ = O.evolve(c, r.nsteps)
+                names = [c.mech.names[s].value.decode() for s in range(c.mech.nspec)] + abi.PLASMA_VAR_NAMES
+                row = {"request": r.rid, "shape": "%dD %d" % (r.dim, r.n), "voltage": round(r.voltage, 1)}
+                for key, st in (("served_mixed", r.state), ("alone_fp64", r.state_fp64)):
+                    e = compare.state_errors(ref, st, c.mech.nspec, c.geom.dim)
+                    q = max(e, key=e.get)
+                    row[key] = {"worst": e[q], "field": names[q]}
+                rows.append(row)
+                r.state = r.state_fp64 = None
+                log("serving request %d (%s) vs oracle: served %.2e (%s), FP64 alone %.2e (%s)"
+                    % (r.rid, row["shape"], row["served_mixed"]["worst"], row["served_mixed"]["field"],
+                       row["alone_fp64"]["worst"], row["alone_fp64"]["field"]))
+            configs["serving"]["rel_l2_vs_fp64_oracle"] = {
+                "worst_served_mixed": max(x["served_mixed"]["worst"] for x in rows),
+                "worst_alone_fp64": max(x["alone_fp64"]["worst"] for x in rows),
+                "requests_checked": len(rows), "of": len(served), "per_request": rows}
 
     if fam_roofs:
         configs["kernel_families"] = fam_roofs

@@ -62,3 +62,12 @@ def test_results_are_the_same_alone_and_in_the_mix():
         assert worst <= 1e-8, (r.rid, errs)
     out = serve.summarize(reqs)
     assert out["p99_time_to_solution_s"] >= out["p50_time_to_solution_s"] > 0
+
+
+def test_draw_a_fixed_number_of_requests():
+    reqs = serve.draw_requests(rate_hz=2.0, seed=3, nsteps=500, count=200)
+    assert len(reqs) == 200
+    arr = np.array([r.arrival for r in reqs])
+    assert np.all(np.diff(arr) > 0)
+    # mean inter-arrival close to 1 / rate
+    assert 0.4 < arr[-1] / 200 < 0.6

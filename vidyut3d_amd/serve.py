@@ -26,6 +26,8 @@ class Request:
         self.voltage, self.nsteps, self.seed = voltage, nsteps, seed
         self.start = self.done = None
         self.state = None
+        self.keep = False  # keep the final state (host copy) of this request
+        self.state_fp64 = None
         self.error = None
 
     @property
@@ -56,14 +58,16 @@ def footprint_bytes(dim, n, nspec):
     return int(8 * cells * arrays * 1.1)
 
 
-def draw_requests(rate_hz, duration_s, seed=0, nsteps=500, min_cells=64 ** 2, max_cells=256 ** 3):
+def draw_requests(rate_hz, duration_s=None, seed=0, nsteps=500, min_cells=64 ** 2, max_cells=256 ** 3,
+                  count=None):
     """Poisson arrivals, log-uniform cell counts (2-D up to 256^2, 3-D
-    above), sizes rounded to multiples of 16, voltages U[0.5, 10] kV."""
+    above), sizes rounded to multiples of 16, voltages U[0.5, 10] kV; the
+    arrivals within duration_s, or the first `count` of them."""
     rng = np.random.default_rng(seed)
     t, out = 0.0, []
     while True:
         t += rng.exponential(1.0 / rate_hz)
-        if t > duration_s:
+        if (duration_s is not None and t > duration_s) or (count is not None and len(out) >= count):
             break
         cells = math.exp(rng.uniform(math.log(min_cells), math.log(max_cells)))
         if cells <= 256 ** 2:
@@ -102,7 +106,7 @@ class Server:
             sim = native.Sim(self.devs[slot], case)
             req.t_ready = time.perf_counter()
             sim.evolve(req.nsteps)
-            if self.keep_state:
+            if self.keep_state or req.keep:
                 req.state = sim.get_state()
             sim.close()
             self.devs[slot].release_workspaces()
@@ -171,14 +175,82 @@ def summarize(reqs):
             "sizes": sorted("%dD %d" % (r.dim, r.n) for r in reqs)}
 
 
-def bench(dev_index=0, duration_s=20.0, rate_hz=0.5, nsteps=500, slots=4, seed=7):
-    reqs = draw_requests(rate_hz, duration_s, seed=seed, nsteps=nsteps)
+def step_times(dev_index, shapes, steps=10, warmup=2, mg_precision=0):
+    """ms per step of one case of each (dim, n) shape alone on the device,
+    from `steps` steps after `warmup` (the same physics as the requests)."""
+    dev = native.Device(dev_index)
+    out = {}
+    try:
+        for dim, n in sorted(set(shapes)):
+            c = Request(0, 0.0, dim, n, 5000.0, steps, 1).make_case(mg_precision)
+            sim = native.Sim(dev, c)
+            sim.evolve(warmup)
+            dev.sync()
+            t0 = time.perf_counter()
+            sim.evolve(steps)
+            dev.sync()
+            out[(dim, n)] = (time.perf_counter() - t0) / steps * 1e3
+            sim.close()
+            dev.release_workspaces()
+    finally:
+        dev.close()
+    return out
+
+
+def run_sequential(dev_index, reqs, mg_precision=0):
+    """The requests one after the other on one context (no overlap, no
+    arrival gaps); returns the total wall time."""
+    srv = Server(dev_index, slots=1, mg_precision=mg_precision)
+    seq = [Request(r.rid, 0.0, r.dim, r.n, r.voltage, r.nsteps, r.seed) for r in reqs]
+    t0 = time.perf_counter()
+    try:
+        srv.run(seq)
+    finally:
+        srv.close()
+    return time.perf_counter() - t0, seq
+
+
+def bench(dev_index=0, requests=48, rate_hz=1.0, nsteps=500, slots=4, seed=7, sequential="sampled",
+          keep_cells_steps=2.0e7):
+    """Serves `requests` Poisson arrivals with FP32 V-cycles (mixed
+    precision); the baseline is the same request list in FP64, one at a time
+    (sequential="run": actually run; "sampled": the sum over requests of
+    nsteps times each shape's measured FP64 ms/step).  Requests whose
+    cells * steps are at most keep_cells_steps keep their final state, and
+    are run again alone in FP64 (state_fp64), for a check of both against
+    the FP64 oracle."""
+    reqs = draw_requests(rate_hz, seed=seed, nsteps=nsteps, count=requests)
+    for r in reqs:
+        r.keep = r.cells * r.nsteps <= keep_cells_steps
     srv = Server(dev_index, slots=slots)
     try:
         srv.run(reqs)
     finally:
         srv.close()
     out = summarize(reqs)
-    out.update({"arrival_rate_hz": rate_hz, "arrival_window_s": duration_s, "steps_per_case": nsteps,
-                "slots": slots, "mg_precision": 1})
-    return out
+    out.update({"arrival_rate_hz": rate_hz, "steps_per_case": nsteps, "slots": slots, "mg_precision": 1})
+    # the kept requests once more, alone and all-FP64: the state the FP64
+    # path gives for the same request, beside the served one
+    kept = [Request(r.rid, 0.0, r.dim, r.n, r.voltage, r.nsteps, r.seed) for r in reqs if r.keep]
+    if kept:
+        solo = Server(dev_index, slots=1, mg_precision=0, keep_state=True)
+        try:
+            solo.run(kept)
+        finally:
+            solo.close()
+        by_id = {r.rid: r for r in kept}
+        for r in reqs:
+            if r.keep:
+                r.state_fp64 = by_id[r.rid].state
+    cell_steps = sum(r.cells * r.nsteps for r in reqs)
+    if sequential == "run":
+        wall, _ = run_sequential(dev_index, reqs, 0)
+        out["fp64_sequential"] = {"kind": "run", "wall_s": wall, "cell_steps_per_s": cell_steps / wall}
+    elif sequential == "sampled":
+        ms = step_times(dev_index, [(r.dim, r.n) for r in reqs])
+        wall = sum(r.nsteps * ms[(r.dim, r.n)] for r in reqs) / 1e3
+        out["fp64_sequential"] = {"kind": "sampled: sum of nsteps x the shape's FP64 ms/step (10 steps alone)",
+                                  "wall_s": wall, "cell_steps_per_s": cell_steps / wall}
+    if "fp64_sequential" in out:
+        out["speedup_over_fp64_sequential"] = out["aggregate_cell_steps_per_s"] / out["fp64_sequential"]["cell_steps_per_s"]
+    return out, reqs
