@@ -292,6 +292,86 @@ def bench_chemistry(native, abi, cases, dev, n):
                          "hbm_GBs": round(kbytes / launches / t / 1e9, 1)}}
 
 
+def gen_flops_per_cell(m, abi):
+    """Canonical FP64 flop count of one cell of a generated mechanism
+    (VD_CHEM_GEN): per cell 13 pow + log + tanh + the mixture sum (20 per
+    transcendental); per reaction the rate (gas-T Arrhenius: 0, taken from
+    the per-launch table; Te power law 22; Te Arrhenius 44; Janev 40;
+    1/Te fit 30), the forward product (order + 1, + 1 with a third body),
+    the reverse rate (terms + 22 + its product), 1 for qf - qr, 2 per wdot
+    update and 6 for an energy exchange."""
+    f = 15 * 20 + m.nspec
+    rate = {abi.VD_RATE_ARRH: 0, abi.VD_RATE_TEPOW: 22, abi.VD_RATE_TEARRH: 44,
+            abi.VD_RATE_JANEV: 40, abi.VD_RATE_FFIT: 30, abi.VD_RATE_JANEV_FFIT: 40}
+    for r in range(m.nreac):
+        R = m.reac[r]
+        order = int(sum(R.react_nu[q] for q in range(R.nreact)))
+        f += rate[R.rate_kind] + order + 1 + (1 if R.third_body else 0) + 1 + 2 * R.nwd
+        if R.reversible:
+            f += R.ng + 22 + int(sum(R.rev_nu[q] for q in range(R.nrev)))
+        if R.rxntype:
+            f += 6
+    return f
+
+
+def bench_generated_chemistry(native, abi, cases, dev, n, with_oracle):
+    """The 44-species / 391-reaction mechanism converted from
+    test/Reactor/CO2_H2_Ar_mechanism_reduced/Chemistry.H (VD_CHEM_GEN,
+    mechanism-specialised kernel), one source evaluation over n^3 cells of
+    a 64^3 seeded state tiled; the first tile against the oracle."""
+    m = cases.generated_mech("co2_h2_ar_reduced")
+    blk = min(n, 64)
+    rng = np.random.default_rng(11)
+    cc = abi.comp(m.nspec)
+    st = np.zeros((cc["NVAR"], blk, blk, blk))
+    for sp in range(m.nspec):
+        st[sp] = (1e16 if m.charge[sp] else 1e21) * np.exp(rng.standard_normal((blk, blk, blk)))
+    names = [m.names[q].value.decode() for q in range(m.nspec)]
+    for nm, f in (("AR", 0.9), ("H2", 0.075), ("CO2", 0.025)):
+        st[names.index(nm)] = f * 2.446e25
+    st[cc["ETEMP"]] = rng.uniform(0.3, 8.0, (blk, blk, blk)) * abi.EV
+    st[cc["EEN"]] = 1.5 * st[m.e_idx] * abi.K_B * st[cc["ETEMP"]]
+    tile = st
+    reps = n // blk
+    st = np.ascontiguousarray(np.tile(st, (1, reps, reps, reps)))
+    g = abi.make_geom([n] * 3, [0] * 3, [1] * 3, [0] * 3)
+    grid = native.Grid(dev, g)
+    mech = native.Mech(dev, m)
+    dst = grid.array(st.shape[0], host=st)
+    del st
+    out = grid.array(m.nspec + 1)
+    prm = cases.default_params()
+    native.rxn_source(mech, prm, grid, dst, out)
+    dev.sync()
+    dev.set_kernel_timing(abi.KT_CHEM)
+    for _ in range(5):
+        native.rxn_source(mech, prm, grid, dst, out)
+    ms, launches, kbytes = dev.kernel_timing()
+    dev.set_kernel_timing(abi.KT_NONE)
+    t = ms / 1e3 / launches
+    fpc = gen_flops_per_cell(m, abi)
+    tfl = fpc * n ** 3 / t / 1e12
+    res = {"mechanism": "test/Reactor/CO2_H2_Ar_mechanism_reduced (converted)", "cells": n ** 3,
+           "nspec": m.nspec, "nreac": m.nreac, "specialised": mech.specialised, "launch_ms": round(t * 1e3, 3),
+           "cells_per_s": n ** 3 / t, "flop_per_cell": fpc,
+           "roofline": {"bound": "fp64", "achieved": round(tfl, 2), "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(tfl / FP64_PEAK_TFLOPS, 4),
+                        "hbm_GBs": round(kbytes / launches / t / 1e9, 1)}}
+    if with_oracle:
+        from oracle import oracle as O
+        got = out.get()[:, :blk, :blk, :blk]
+        og = O.Grid(abi.make_geom([blk] * 3, [0] * 3, [1] * 3, [0] * 3))
+        t0 = time.perf_counter()
+        ref = O.rxn_source(m, prm, og, O.fill_patch(og, tile, 1), 1)
+        cpu = time.perf_counter() - t0
+        worst = max(float(np.max(np.abs(ref[q] - got[q])) / np.max(np.abs(ref[q])))
+                    for q in range(ref.shape[0]) if np.max(np.abs(ref[q])) > 0)
+        res["vs_oracle"] = {"tile_cells": blk ** 3, "worst_per_component_max_rel": worst,
+                            "oracle_s": round(cpu, 3), "oracle_cells_per_s": blk ** 3 / cpu,
+                            "oracle_threads": O.lib().or_num_threads()}
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -304,7 +384,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--headline-only", action="store_true", help="skip the other configurations")
-    ap.add_argument("--only", default="", help="comma-separated subset of: mixed,discharge,poisson,chemistry,serving")
+    ap.add_argument("--only", default="", help="comma-separated subset of: mixed,discharge,poisson,chemistry,generated,serving")
     ap.add_argument("--poisson-n", type=int, default=512)
     ap.add_argument("--chem-n", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=32)
@@ -391,6 +471,12 @@ def main():
     if want("chemistry"):
         log("%d^3 40-species chemistry" % args.chem_n)
         configs["chemistry_40sp_200rx"] = bench_chemistry(native, abi, cases, dev, args.chem_n)
+    if want("generated"):
+        log("%d^3 44-species / 391-reaction generated mechanism" % args.chem_n)
+        configs["chemistry_co2_h2_ar_44sp"] = bench_generated_chemistry(native, abi, cases, dev, args.chem_n,
+                                                                         not args.no_cpu)
+        log("generated mechanism: %.3f ms per launch, %s" % (configs["chemistry_co2_h2_ar_44sp"]["launch_ms"],
+                                                            configs["chemistry_co2_h2_ar_44sp"].get("vs_oracle")))
     served = []
     if want("serving") and args.serve_requests > 0:
         from vidyut3d_amd import serve

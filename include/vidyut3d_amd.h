@@ -138,16 +138,37 @@ void vd_params_default(vd_params* p);
  * species table (CKSYMS_STR, CKCHRG, mw), transport (specMob, specDiff)
  * and a table-driven reaction mechanism. */
 #define VD_MAX_SPECIES 48
-#define VD_MAX_REACTIONS 256
+#define VD_MAX_REACTIONS 512
 #define VD_MAX_REACTANTS 3
 #define VD_MAX_PRODUCTS 4
+#define VD_GEN_NG 8  /* g/RT terms of one reverse rate (VD_CHEM_GEN) */
+#define VD_GEN_NW 8  /* wdot updates of one reaction (VD_CHEM_GEN) */
 
 /* chemistry forms */
 enum {
     VD_CHEM_TABLE = 0, /* Source/reaction_source.H:10-64: number densities,
                           rate = k*prod n^nu, energy += e_r*rate          */
-    VD_CHEM_CKWC = 1   /* generated Chemistry.H CKWC/productionRate form:
+    VD_CHEM_CKWC = 1,  /* generated Chemistry.H CKWC/productionRate form:
                           mol units round trip + comp_ener_exch           */
+    VD_CHEM_GEN = 2    /* a generated productionRate read as data
+                          (vidyut3d_amd/mechgen.py): the rate kinds,
+                          third bodies, reverse rates from g/RT and wdot
+                          updates below, in mol/m^3 and the file's units,
+                          behind the same CKWC round trip                  */
+};
+
+/* VD_CHEM_GEN forward rate kinds, as productionRate writes them
+ * (py_utils/ceptr_noThermo; test/Reactor/CO2_H2_Ar_mechanism_reduced/
+ * Chemistry.H:913-6380): T = gas temperature, TeV = Te / 11595,
+ * NA = 6.02214085774e23 */
+enum {
+    VD_RATE_ARRH = 0,      /* A exp(beta ln(T/300) - Ea / T)                */
+    VD_RATE_TEPOW = 1,     /* A pow(Te / tref, beta), / T when tgas_div     */
+    VD_RATE_TEARRH = 2,    /* A pow(Te, beta) exp(-Ea / Te)                 */
+    VD_RATE_JANEV = 3,     /* A exp(sum_{j<9} fit[j] pow(ln TeV, j)) NA     */
+    VD_RATE_FFIT = 4,      /* A exp(sum_{j<4} fit[j] pow(1/TeV, j+1)) NA    */
+    VD_RATE_JANEV_FFIT = 5 /* JANEV (A, fit) for TeV < fit_switch_ev, else
+                              FFIT (A2, fit2)                               */
 };
 
 /* diffusion models per species (specDiff) */
@@ -195,6 +216,30 @@ typedef struct vd_reaction {
                           3 recombination, 5 elastic exchange               */
     double eexci;      /* VD_CHEM_CKWC: eV */
     int elidx;         /* VD_CHEM_CKWC: collision partner for rxntype 5 */
+    /* VD_CHEM_GEN (rxntype also 4 attachment and 7 detachment; reactants
+     * in the order of the forward product, which multiplies them in turn) */
+    int rate_kind;     /* VD_RATE_* */
+    double tref;       /* VD_RATE_TEPOW */
+    int tgas_div;      /* VD_RATE_TEPOW */
+    double fit[9];
+    double A2;
+    double fit2[4];
+    double fit_switch_ev;
+    double kmax;       /* > 0: k_f = min(k_f, kmax) */
+    int third_body;    /* qf and qr times the mixture concentration */
+    int reversible;    /* qr = k_f exp(-(sum_q g_nu[q] g/RT(g_sp[q])))
+                          refC^refc_pow prod_q C(rev_sp[q])^rev_nu[q],
+                          refC = 101325 / 8.31446 / T */
+    int ng;
+    int g_sp[VD_GEN_NG];
+    double g_nu[VD_GEN_NG];
+    int refc_pow;
+    int nrev;
+    int rev_sp[VD_MAX_PRODUCTS];
+    double rev_nu[VD_MAX_PRODUCTS];
+    int nwd;           /* wdot[wd_sp[q]] += wd_nu[q] * (qf - qr), in order */
+    int wd_sp[VD_GEN_NW];
+    double wd_nu[VD_GEN_NW];
 } vd_reaction;
 
 typedef struct vd_mech_desc {
@@ -232,6 +277,12 @@ typedef struct vd_mech_desc {
     /* user_sources::add_user_react_sources: a constant added to each
      * species' (and, at nspec, the electron energy's) reaction source */
     double user_src[VD_MAX_SPECIES + 1];
+    /* VD_CHEM_GEN: g/RT of each species (gibbs() of the generated file),
+     * c0/T + c1 + c2 ln T + c3 T + c4 T^2 + c5 T^3 + c6 T^4 with gibbs_lo
+     * below gibbs_tmid and gibbs_hi at and above it */
+    double gibbs_tmid[VD_MAX_SPECIES];
+    double gibbs_lo[VD_MAX_SPECIES][7];
+    double gibbs_hi[VD_MAX_SPECIES][7];
 } vd_mech_desc;
 
 /* ---------------------------------------------------------- handles */

@@ -35,6 +35,8 @@ def lib():
         L.or_species_bc_value.restype = C.c_double
         L.or_species_bc_value.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_int, C.c_int,
                                           C.c_double]
+        L.or_gen_gibbs.restype = C.c_double
+        L.or_gen_gibbs.argtypes = [C.c_void_p, C.c_int, C.c_double]
         L.or_gk.restype = C.c_double
         L.or_gk.argtypes = [C.c_double, C.c_double]
         L.or_slope_lim.restype = C.c_double
@@ -129,6 +131,11 @@ def dsdt(grid, fl, rxn, specid):
                               C.byref(grid.view(out)), C.byref(grid.box), C.byref(grid.geom),
                               int(specid)), "or_compute_dsdt")
     return out
+
+
+def gen_gibbs(mech, k, T):
+    """g/RT of species k of a VD_CHEM_GEN mechanism at gas temperature T."""
+    return lib().or_gen_gibbs(C.byref(mech), int(k), float(T))
 
 
 def rxn_source(mech, prm, grid, sb, ng):

@@ -514,6 +514,145 @@ static void ckwc_source(const vd_mech_desc* m, double T, double* C, double* wdot
     }
 }
 
+/* g/RT of species k at gas temperature T: gibbs() of a generated
+ * Chemistry.H (test/Reactor/CO2_H2_Ar_mechanism_reduced/Chemistry.H:
+ * 415-834), c0/T + c1 + c2 ln T + c3 T + c4 T^2 + c5 T^3 + c6 T^4 summed
+ * left to right, the low range below gibbs_tmid */
+static double gen_gibbs(const vd_mech_desc* m, int k, double T) {
+    const double T2 = T * T, T3 = T * T * T, T4 = T * T * T * T;
+    const double invT = 1.0 / T, logT = log(T);
+    const double* c = (m->gibbs_tmid[k] > 0.0 && !(T < m->gibbs_tmid[k])) ? m->gibbs_hi[k] : m->gibbs_lo[k];
+    return c[0] * invT + c[1] + c[2] * logT + c[3] * T + c[4] * T2 + c[5] * T3 + c[6] * T4;
+}
+
+/* g/RT of species k (VD_CHEM_GEN thermo), for the tests */
+double or_gen_gibbs(const vd_mech_desc* m, int k, double T) { return gen_gibbs(m, k, T); }
+
+/* CKWC -> productionRate -> comp_ener_exch of a mechanism read from its
+ * generated Chemistry.H (VD_CHEM_GEN; test/Reactor/
+ * CO2_H2_Ar_mechanism_reduced/Chemistry.H:841-6399): the prelude of
+ * :913-950, then per reaction block k_f by its rate kind, qf = [mixture *]
+ * k_f * prod C, qr from the g/RT sum, the wdot updates as written and the
+ * energy exchange of :841-885.  C is mol/cm^3 on entry, wdot mol/cm^3/s on
+ * exit. */
+static void gen_source(const vd_mech_desc* m, double T, double* C, double* wdot, double Te, double* ener) {
+    const int ns = m->nspec, E = m->e_idx;
+    for (int id = 0; id < ns; ++id) C[id] *= 1.0e6;
+    const double invT = 1.0 / T;
+    const double logT = log(T / 300.0);
+    const double refC = 101325 / 8.31446 * invT;
+    const double refCinv = 1 / refC;
+    for (int i = 0; i < ns; ++i) wdot[i] = 0.0;
+    double mixture = 0.0;
+    for (int i = 0; i < ns; ++i) mixture += C[i];
+    double g_RT[VD_MAX_SPECIES];
+    for (int k = 0; k < ns; k++) g_RT[k] = gen_gibbs(m, k, T);
+    const double ne_pr = C[E] * 6.02214085774e23;
+    const double Ue = 1.5 * Te * ne_pr * 1.380649e-23;
+    const double TeeV = Te / 11595.0;
+    const double logTe = log(TeeV);
+    const double invTeeV = (Te == 0) ? 1.0 : 1.0 / (TeeV);
+    double Te_pow[9], invTe_pow[4];
+    for (int j = 0; j < 9; j++) Te_pow[j] = pow(logTe, j);
+    for (int j = 0; j < 4; j++) invTe_pow[j] = pow(invTeeV, j + 1);
+    /* comp_ener_exch constants */
+    const double eV2J = 1.602176620800000e-19, NA = 6.02214085774e23, kB = 1.380649e-23;
+    const double mee = Ue / (C[E] * NA);
+    const double sefact = (tanh((Te - T) / 0.5) + 1.0) / 2.0;
+    for (int r = 0; r < m->nreac; r++) {
+        const vd_reaction* R = &m->reac[r];
+        double k_f, sum;
+        switch (R->rate_kind) {
+        case VD_RATE_ARRH:
+            k_f = R->A * exp(R->beta * logT - R->Ea * invT);
+            break;
+        case VD_RATE_TEPOW:
+            k_f = R->A * pow(Te / R->tref, R->beta);
+            if (R->tgas_div) k_f = k_f / T;
+            break;
+        case VD_RATE_TEARRH:
+            k_f = R->A * pow(Te, R->beta) * exp(-R->Ea / Te);
+            break;
+        case VD_RATE_JANEV:
+            sum = 0.0;
+            for (int j = 0; j < 9; j++) sum += R->fit[j] * Te_pow[j];
+            k_f = R->A * exp(sum) * 6.02214085774e23;
+            break;
+        case VD_RATE_FFIT:
+            sum = 0.0;
+            for (int j = 0; j < 4; j++) sum += R->fit[j] * invTe_pow[j];
+            k_f = R->A * exp(sum) * 6.02214085774e23;
+            break;
+        default: /* VD_RATE_JANEV_FFIT */
+            sum = 0.0;
+            if (TeeV < R->fit_switch_ev) {
+                for (int j = 0; j < 9; j++) sum += R->fit[j] * Te_pow[j];
+                k_f = R->A * exp(sum) * 6.02214085774e23;
+            } else {
+                for (int j = 0; j < 4; j++) sum += R->fit2[j] * invTe_pow[j];
+                k_f = R->A2 * exp(sum) * 6.02214085774e23;
+            }
+            break;
+        }
+        if (R->kmax > 0.0 && R->kmax < k_f) k_f = R->kmax; /* std::min(k_f, kmax) */
+        const double pre = R->third_body ? mixture * k_f : k_f;
+        double cprod = 1.0;
+        int first = 1;
+        for (int q = 0; q < R->nreact; q++)
+            for (int c = 0; c < (int)R->react_nu[q]; c++) {
+                cprod = first ? C[R->react_sp[q]] : cprod * C[R->react_sp[q]];
+                first = 0;
+            }
+        const double qf = pre * cprod;
+        double qr = 0.0;
+        if (R->reversible) {
+            double s = 0.0;
+            for (int q = 0; q < R->ng; q++) {
+                const double t = R->g_nu[q] == 1.0 ? g_RT[R->g_sp[q]]
+                                 : R->g_nu[q] == -1.0 ? -g_RT[R->g_sp[q]]
+                                                      : R->g_nu[q] * g_RT[R->g_sp[q]];
+                s = q == 0 ? t : s + t;
+            }
+            double v = pre * exp(-(s));
+            for (int p = 0; p < R->refc_pow; p++) v = v * refC;
+            for (int p = 0; p < -R->refc_pow; p++) v = v * refCinv;
+            double rp = 1.0;
+            first = 1;
+            for (int q = 0; q < R->nrev; q++)
+                for (int c = 0; c < (int)R->rev_nu[q]; c++) {
+                    rp = first ? C[R->rev_sp[q]] : rp * C[R->rev_sp[q]];
+                    first = 0;
+                }
+            qr = v * rp;
+        }
+        const double qdot = qf - qr;
+        for (int q = 0; q < R->nwd; q++) {
+            const double nu = R->wd_nu[q];
+            double* w = &wdot[R->wd_sp[q]];
+            if (nu == 1.0)
+                *w += qdot;
+            else if (nu == -1.0)
+                *w -= qdot;
+            else if (nu > 0.0)
+                *w += nu * qdot;
+            else
+                *w -= (-nu) * qdot;
+        }
+        const int rt = R->rxntype;
+        if (rt == 1 || rt == 2) *ener -= (R->eexci * eV2J) * NA * (qf - qr) * sefact;
+        if (rt == 3 || rt == 4) *ener -= mee * NA * (qf - qr);
+        if (rt == 5) {
+            const double qq = C[E] * C[R->elidx] * k_f;
+            *ener -= 3.0 * kB * (m->molwt[E] / m->molwt[R->elidx]) * (Te - T) * qq * NA;
+        }
+        if (rt == 7) *ener += mee * NA * (qf - qr);
+    }
+    for (int id = 0; id < ns; ++id) {
+        C[id] *= 1.0e-6;
+        wdot[id] *= 1.0e-6;
+    }
+}
+
 /* Vidyut::update_rxnsrc_at_all_levels (Source/ScalarSolve.cpp:113-168) */
 int or_update_rxnsrc(const vd_mech_desc* m, const vd_array4* sb, vd_array4* rxn,
                      const vd_box* box, const vd_params* prm, double time) {
@@ -533,7 +672,10 @@ int or_update_rxnsrc(const vd_mech_desc* m, const vd_array4* sb, vd_array4* rxn,
                 } else {
                     double C[VD_MAX_SPECIES], wdot[VD_MAX_SPECIES], ener = 0.0;
                     for (int sp = 0; sp < ns; sp++) C[sp] = AT(sb, i, j, k, sp) * 1.0e-6 / N_A;
-                    ckwc_source(m, Tg, C, wdot, Te, &ener);
+                    if (m->chem_form == VD_CHEM_GEN)
+                        gen_source(m, Tg, C, wdot, Te, &ener);
+                    else
+                        ckwc_source(m, Tg, C, wdot, Te, &ener);
                     for (int sp = 0; sp < ns; sp++) AT(rxn, i, j, k, sp) = wdot[sp] * N_A * 1.0e6;
                     AT(rxn, i, j, k, ns) = ener;
                 }

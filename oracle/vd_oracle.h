@@ -97,6 +97,9 @@ double or_gk(double w, double g);
 double or_slope_lim(double c, double dl, double dm, double dr);
 double or_applied_potential(double t, int domain_end, const vd_params* p);
 
+/* g/RT of species k of a VD_CHEM_GEN mechanism at gas temperature T */
+double or_gen_gibbs(const vd_mech_desc* m, int k, double T);
+
 #ifdef __cplusplus
 }
 #endif

@@ -11,12 +11,15 @@ import numpy as np
 
 VD_OK, VD_EINVAL, VD_ENOCONV, VD_ENOMEM, VD_EHIP, VD_ENOTFOUND = range(6)
 PERBC, DIRCBC, HNEUBC, IHNEUBC, ROBINBC = range(5)
-VD_CHEM_TABLE, VD_CHEM_CKWC = 0, 1
+VD_CHEM_TABLE, VD_CHEM_CKWC, VD_CHEM_GEN = 0, 1, 2
+VD_RATE_ARRH, VD_RATE_TEPOW, VD_RATE_TEARRH, VD_RATE_JANEV, VD_RATE_FFIT, VD_RATE_JANEV_FFIT = range(6)
 VD_DIFF_EINSTEIN_TE, VD_DIFF_EINSTEIN_TG, VD_DIFF_CONST = 0, 1, 2
 VD_MAX_SPECIES = 48
-VD_MAX_REACTIONS = 256
+VD_MAX_REACTIONS = 512
 VD_MAX_REACTANTS = 3
 VD_MAX_PRODUCTS = 4
+VD_GEN_NG = 8
+VD_GEN_NW = 8
 NUM_PLASMAVARS = 10
 KT_NONE, KT_GSRB, KT_TRANSPORT, KT_CHEM, KT_RESID, KT_FILL, KT_FUSED, KT_MGCOARSE, KT_SPECIES, KT_POTPOST = range(10)
 KT_NAMES = {KT_GSRB: "gsrb", KT_TRANSPORT: "transport", KT_CHEM: "chem", KT_RESID: "resid", KT_FILL: "fill",
@@ -101,7 +104,14 @@ class vd_reaction(C.Structure):
                 ("prod_sp", C.c_int * VD_MAX_PRODUCTS),
                 ("prod_nu", C.c_double * VD_MAX_PRODUCTS),
                 ("energy", C.c_double), ("rxntype", C.c_int), ("eexci", C.c_double),
-                ("elidx", C.c_int)]
+                ("elidx", C.c_int),
+                ("rate_kind", C.c_int), ("tref", C.c_double), ("tgas_div", C.c_int),
+                ("fit", C.c_double * 9), ("A2", C.c_double), ("fit2", C.c_double * 4),
+                ("fit_switch_ev", C.c_double), ("kmax", C.c_double), ("third_body", C.c_int),
+                ("reversible", C.c_int), ("ng", C.c_int), ("g_sp", C.c_int * VD_GEN_NG),
+                ("g_nu", C.c_double * VD_GEN_NG), ("refc_pow", C.c_int), ("nrev", C.c_int),
+                ("rev_sp", C.c_int * VD_MAX_PRODUCTS), ("rev_nu", C.c_double * VD_MAX_PRODUCTS),
+                ("nwd", C.c_int), ("wd_sp", C.c_int * VD_GEN_NW), ("wd_nu", C.c_double * VD_GEN_NW)]
 
 
 VD_TR_NCOEF = 12
@@ -130,7 +140,10 @@ class vd_mech_desc(C.Structure):
                 ("vel", (C.c_double * 3) * (VD_MAX_SPECIES + 1)),
                 ("robin_set", C.c_int * (VD_MAX_SPECIES + 1)),
                 ("robin", (((C.c_double * 3) * 2) * 3) * (VD_MAX_SPECIES + 1)),
-                ("user_src", C.c_double * (VD_MAX_SPECIES + 1))]
+                ("user_src", C.c_double * (VD_MAX_SPECIES + 1)),
+                ("gibbs_tmid", C.c_double * VD_MAX_SPECIES),
+                ("gibbs_lo", (C.c_double * 7) * VD_MAX_SPECIES),
+                ("gibbs_hi", (C.c_double * 7) * VD_MAX_SPECIES)]
 
 
 def set_fit(f, sign, **coef):

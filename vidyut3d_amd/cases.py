@@ -440,3 +440,71 @@ def advect_1d(n=20, dt=1e-4, nsteps=20000, weno_scheme=3):
     st[1] = np.sin(np.pi * x - np.sin(np.pi * x) / np.pi)[None, :]
     st[c["ETEMP"]] = 300.0
     return Case("advect_%d" % n, m, p, g, st, dt, nsteps)
+
+
+# ------------------------------------- generated mechanisms (VD_CHEM_GEN)
+def generated_mech(name, mob_e=-0.04, mob_ion=2e-4, bg_neutrals=True, keep=None):
+    """A mechanism converted from a reference case's generated Chemistry.H
+    (vidyut3d_amd/mechanisms/<name>.json) with constant transport: the
+    electron mobility mob_e (Einstein diffusion at Te), ions +-mob_ion
+    (Einstein at Tg), neutrals as background species (updated by their
+    reactions only, vidyut.bg_species_ids).  `keep(reaction)` selects a
+    subset of the reactions."""
+    from . import mechgen
+    mj = mechgen.load(mechgen.builtin(name))
+    if keep is not None:
+        mj["reactions"] = [r for r in mj["reactions"] if keep(r)]
+    m = mechgen.to_desc(mj)
+    for s in range(m.nspec):
+        q = m.charge[s]
+        if s == m.e_idx:
+            m.mob[s], m.diff_kind[s] = mob_e, abi.VD_DIFF_EINSTEIN_TE
+        else:
+            m.mob[s] = mob_ion * np.sign(q) if q else 0.0
+            m.diff_kind[s] = abi.VD_DIFF_EINSTEIN_TG
+        m.is_bg[s] = int(bg_neutrals and q == 0)
+    return m
+
+
+def co2_h2_ar_3d(n=32, nsteps=20, dt=1e-12, L=1e-2, voltage=2000.0, seed=5):
+    """3-D discharge in the Ar / H2 / CO2 mixture of
+    test/Reactor/CO2_H2_Ar_mechanism_reduced (0.9 / 0.075 / 0.025 of
+    2.446e25 m^-3, inputs1d and Prob.H) with that case's converted
+    mechanism (44 species, 391 reactions): electrons log-normal (median
+    1e16 m^-3, sigma_ln = 1) neutralised by Ar+, Te = 2 eV, Tg = 300 K,
+    Dirichlet 0 / `voltage` on z, WENO-Z transport, neutrals background
+    species with background reactions on (vidyut.do_bg_reactions = 1)."""
+    m = generated_mech("co2_h2_ar_reduced")
+    names = [m.names[s].value.decode() for s in range(m.nspec)]
+    p = default_params()
+    p.hyp_order, p.weno_scheme = 5, 3
+    p.elecenergy_solve = 1
+    p.do_bg_reactions = 1
+    p.bound_specden = 1
+    p.min_species_density, p.min_electron_density = 0.0, 1e9
+    p.gas_temperature = 300.0
+    p.gas_pressure = 101325.0
+    p.voltage_amp_1, p.voltage_amp_2 = 0.0, voltage
+    p.pot_bc_lo[:] = [abi.HNEUBC, abi.HNEUBC, abi.DIRCBC]
+    p.pot_bc_hi[:] = [abi.HNEUBC, abi.HNEUBC, abi.DIRCBC]
+    for nm in ("eden", "eenrg", "ion"):
+        getattr(p, nm + "_bc_lo")[:] = [abi.HNEUBC, abi.HNEUBC, abi.IHNEUBC]
+        getattr(p, nm + "_bc_hi")[:] = [abi.HNEUBC, abi.HNEUBC, abi.IHNEUBC]
+    p.neutral_bc_lo[:] = [abi.HNEUBC] * 3
+    p.neutral_bc_hi[:] = [abi.HNEUBC] * 3
+    nn = [n] * 3 if np.isscalar(n) else list(n)
+    g = abi.make_geom(nn, [0.0] * 3, [L] * 3, [0, 0, 0])
+    c = abi.comp(m.nspec)
+    rng = np.random.default_rng(seed)
+    st = np.zeros((c["NVAR"], nn[2], nn[1], nn[0]))
+    N = 2.446e25
+    st[names.index("AR")] = 0.9 * N
+    st[names.index("H2")] = 0.075 * N
+    st[names.index("CO2")] = 0.025 * N
+    ne = 1e16 * np.exp(rng.standard_normal((nn[2], nn[1], nn[0])))
+    st[m.e_idx] = ne
+    st[names.index("AR+")] = ne
+    te = 2.0 * TE_1EV
+    st[c["ETEMP"]] = te
+    st[c["EEN"]] = 1.5 * ne * abi.K_B * te
+    return Case("co2_h2_ar_3d_%s" % "x".join(map(str, nn)), m, p, g, st, dt, nsteps)

@@ -296,6 +296,80 @@ int vd_memset_zero(vd_ctx* ctx, void* dst, size_t bytes) {
     return vd_memset_zero_kernel(ctx, dst, bytes);
 }
 
+// the VD_CHEM_GEN fields of reaction r, checked, into the device copy
+static int gen_reaction(const vd_mech_desc* d, int r, DevMech& h) {
+    const vd_reaction& R = d->reac[r];
+    const int ns = d->nspec;
+    auto bad_sp = [&](int s) { return s < 0 || s >= ns; };
+    if (R.rate_kind < VD_RATE_ARRH || R.rate_kind > VD_RATE_JANEV_FFIT) {
+        set_error("reaction %d: rate kind %d unknown", r, R.rate_kind);
+        return VD_EINVAL;
+    }
+    if (R.rate_kind == VD_RATE_TEPOW && !(R.tref > 0.0)) {
+        set_error("reaction %d: Te power law needs tref > 0", r);
+        return VD_EINVAL;
+    }
+    if (R.ng < 0 || R.ng > VD_GEN_NG || R.nrev < 0 || R.nrev > VD_MAX_PRODUCTS || R.nwd < 0 ||
+        R.nwd > VD_GEN_NW || R.refc_pow < -3 || R.refc_pow > 3 || (R.reversible && R.ng == 0)) {
+        set_error("reaction %d: bad reverse-rate or wdot counts", r);
+        return VD_EINVAL;
+    }
+    for (int q = 0; q < R.ng; q++)
+        if (bad_sp(R.g_sp[q])) {
+            set_error("reaction %d: g/RT species index %d", r, R.g_sp[q]);
+            return VD_EINVAL;
+        }
+    for (int q = 0; q < R.nrev; q++)
+        if (bad_sp(R.rev_sp[q]) || !(R.rev_nu[q] >= 1.0) || R.rev_nu[q] != (double)(int)R.rev_nu[q]) {
+            set_error("reaction %d: reverse factor %d", r, q);
+            return VD_EINVAL;
+        }
+    for (int q = 0; q < R.nreact; q++)
+        if (!(R.react_nu[q] >= 1.0) || R.react_nu[q] != (double)(int)R.react_nu[q]) {
+            set_error("reaction %d: reactant order must be a positive integer", r);
+            return VD_EINVAL;
+        }
+    for (int q = 0; q < R.nwd; q++)
+        if (bad_sp(R.wd_sp[q])) {
+            set_error("reaction %d: wdot species index %d", r, R.wd_sp[q]);
+            return VD_EINVAL;
+        }
+    if (R.rxntype == 5 && bad_sp(R.elidx)) {
+        set_error("reaction %d: elastic partner %d", r, R.elidx);
+        return VD_EINVAL;
+    }
+    h.rkind[r] = R.rate_kind;
+    h.tref[r] = R.tref;
+    h.tgdiv[r] = R.tgas_div != 0;
+    for (int q = 0; q < 9; q++) h.fit[q][r] = R.fit[q];
+    h.A2[r] = R.A2;
+    for (int q = 0; q < 4; q++) h.fit2[q][r] = R.fit2[q];
+    h.fsw[r] = R.fit_switch_ev;
+    h.kmax[r] = R.kmax;
+    h.tb[r] = R.third_body != 0;
+    h.rev[r] = R.reversible != 0;
+    h.ng[r] = R.ng;
+    for (int q = 0; q < R.ng; q++) {
+        h.gsp[q][r] = R.g_sp[q];
+        h.gnu[q][r] = R.g_nu[q];
+    }
+    h.refcp[r] = R.refc_pow;
+    h.nrev[r] = R.nrev;
+    for (int q = 0; q < R.nrev; q++) {
+        h.revsp[q][r] = R.rev_sp[q];
+        h.revnu[q][r] = R.rev_nu[q];
+    }
+    h.nwd[r] = R.nwd;
+    for (int q = 0; q < R.nwd; q++) {
+        h.wdsp[q][r] = R.wd_sp[q];
+        h.wdnu[q][r] = R.wd_nu[q];
+    }
+    h.any_rev |= h.rev[r];
+    h.any_tb |= h.tb[r];
+    h.any_fit |= R.rate_kind >= VD_RATE_JANEV;
+    return VD_OK;
+}
+
 int vd_mech_create(vd_ctx* ctx, const vd_mech_desc* d, vd_mech** out) {
     if (!ctx || !d || !out) return VD_EINVAL;
     if (d->nspec < 1 || d->nspec > VD_MAX_SPECIES || d->nreac < 0 || d->nreac > VD_MAX_REACTIONS ||
@@ -303,7 +377,7 @@ int vd_mech_create(vd_ctx* ctx, const vd_mech_desc* d, vd_mech** out) {
         set_error("mechanism: bad sizes nspec=%d nreac=%d e_idx=%d", d->nspec, d->nreac, d->e_idx);
         return VD_EINVAL;
     }
-    if (d->chem_form != VD_CHEM_TABLE && d->chem_form != VD_CHEM_CKWC) {
+    if (d->chem_form != VD_CHEM_TABLE && d->chem_form != VD_CHEM_CKWC && d->chem_form != VD_CHEM_GEN) {
         set_error("mechanism: unknown chem_form %d", d->chem_form);
         return VD_EINVAL;
     }
@@ -406,7 +480,22 @@ int vd_mech_create(vd_ctx* ctx, const vd_mech_desc* d, vd_mech** out) {
             nn++;
         }
         h.nnet[r] = nn;
+        if (d->chem_form == VD_CHEM_GEN) {
+            const int bad = gen_reaction(d, r, h);
+            if (bad) {
+                delete m;
+                return bad;
+            }
+        }
     }
+    if (d->chem_form == VD_CHEM_GEN)
+        for (int k = 0; k < d->nspec; k++) {
+            h.gtmid[k] = d->gibbs_tmid[k];
+            for (int q = 0; q < 7; q++) {
+                h.glo[k][q] = d->gibbs_lo[k][q];
+                h.ghi[k][q] = d->gibbs_hi[k][q];
+            }
+        }
     hipError_t e = hipMalloc((void**)&m->dev, sizeof(DevMech));
     if (e == hipSuccess) e = hipMemcpy(m->dev, &h, sizeof(DevMech), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
@@ -414,7 +503,7 @@ int vd_mech_create(vd_ctx* ctx, const vd_mech_desc* d, vd_mech** out) {
         set_error("mechanism upload: %s", hipGetErrorString(e));
         return VD_EHIP;
     }
-    if (h.chem_form == VD_CHEM_TABLE && h.nreac > 0) {
+    if ((h.chem_form == VD_CHEM_TABLE || h.chem_form == VD_CHEM_GEN) && h.nreac > 0) {
         const char* env = getenv("VD_CHEM_SPECIALISE");
         if (!(env && strcmp(env, "0") == 0)) {
             if (chem_jit_load(m) == VD_OK) m->use_spec = 1;

@@ -150,6 +150,169 @@ __global__ void __launch_bounds__(CHEM_BLOCK) k_rxn_ckwc(ChemArgs a) {
     a.rxn.p[(int64_t)ns * a.rxn.ns + orx] = ener;
 }
 
+// VD_CHEM_GEN prologue: the k_f of every VD_RATE_ARRH reaction and g/RT of
+// every species at the gas temperature T (one block; the generated
+// productionRate computes them per cell from the same T,
+// test/Reactor/CO2_H2_Ar_mechanism_reduced/Chemistry.H:913-950)
+__global__ void __launch_bounds__(256) k_gen_tconst(const DevMech* __restrict__ Mp, double T,
+                                                    double* __restrict__ tc) {
+    const DevMech& M = *Mp;
+    const double invT = 1.0 / T;
+    const double logT = log(T / 300.0);
+    for (int r = threadIdx.x; r < M.nreac; r += blockDim.x)
+        tc[r] = M.rkind[r] == VD_RATE_ARRH ? M.A[r] * exp(M.beta[r] * logT - M.Ea[r] * invT) : 0.0;
+    for (int k = threadIdx.x; k < M.nspec; k += blockDim.x) tc[VD_MAX_REACTIONS + k] = gen_gibbs(M, k, T);
+}
+
+// k_f of a VD_CHEM_GEN reaction at one cell (the T-only kinds from tc)
+__device__ __forceinline__ double gen_kf(const DevMech& M, int r, const double* __restrict__ tc, double Te,
+                                         double T, double TeeV, const double* Te_pow, const double* invTe_pow) {
+    double k_f, sum = 0.0;
+    switch (M.rkind[r]) {
+    case VD_RATE_ARRH:
+        k_f = tc[r];
+        break;
+    case VD_RATE_TEPOW:
+        k_f = M.A[r] * pow(Te / M.tref[r], M.beta[r]);
+        if (M.tgdiv[r]) k_f = k_f / T;
+        break;
+    case VD_RATE_TEARRH:
+        k_f = M.A[r] * pow(Te, M.beta[r]) * exp(-M.Ea[r] / Te);
+        break;
+    case VD_RATE_JANEV:
+        for (int j = 0; j < 9; j++) sum += M.fit[j][r] * Te_pow[j];
+        k_f = M.A[r] * exp(sum) * 6.02214085774e23;
+        break;
+    case VD_RATE_FFIT:
+        for (int j = 0; j < 4; j++) sum += M.fit[j][r] * invTe_pow[j];
+        k_f = M.A[r] * exp(sum) * 6.02214085774e23;
+        break;
+    default:
+        if (TeeV < M.fsw[r]) {
+            for (int j = 0; j < 9; j++) sum += M.fit[j][r] * Te_pow[j];
+            k_f = M.A[r] * exp(sum) * 6.02214085774e23;
+        } else {
+            for (int j = 0; j < 4; j++) sum += M.fit2[j][r] * invTe_pow[j];
+            k_f = M.A2[r] * exp(sum) * 6.02214085774e23;
+        }
+        break;
+    }
+    if (M.kmax[r] > 0.0 && M.kmax[r] < k_f) k_f = M.kmax[r];
+    return k_f;
+}
+
+// CKWC / productionRate / comp_ener_exch of a mechanism read from its
+// generated Chemistry.H (test/Reactor/CO2_H2_Ar_mechanism_reduced/
+// Chemistry.H:841-6399) behind update_rxnsrc_at_all_levels
+// (Source/ScalarSolve.cpp:143-165): the generic form, reading the reaction
+// records from the device descriptor; accumulators in LDS columns
+__global__ void __launch_bounds__(CHEM_BLOCK) k_rxn_gen(ChemArgs a, const double* __restrict__ tc) {
+    extern __shared__ double w[];
+    const DevMech& M = *a.M;
+    const int tid = threadIdx.x;
+    const int ns = M.nspec, E = M.e_idx;
+    int64_t t = (int64_t)blockIdx.x * CHEM_BLOCK + tid;
+    int64_t tot = (int64_t)a.nx * a.ny * a.nz;
+    if (t >= tot) return;
+    int i = (int)(t % a.nx);
+    int64_t r0 = t / a.nx;
+    int j = (int)(r0 % a.ny);
+    int k = (int)(r0 / a.ny);
+    i += a.lo.x;
+    j += a.lo.y;
+    k += a.lo.z;
+    for (int sp = 0; sp < ns; sp++) w[sp * CHEM_BLOCK + tid] = 0.0;
+    const int o = a.sb.off(i, j, k);
+    const double* __restrict__ sbp = a.sb.p;
+    const int64_t nst = a.sb.ns;
+#define CONC(sp) ((sbp[(int64_t)(sp) * nst + o] * 1.0e-6 / VD_N_A) * 1.0e6)
+    const double Te = sbp[(int64_t)VD_ETEMP(ns) * nst + o];
+    const double T = a.Tg;
+    const double invT = 1.0 / T;
+    const double refC = 101325 / 8.31446 * invT;
+    const double refCinv = 1 / refC;
+    double mixture = 0.0;
+    if (M.any_tb)
+        for (int sp = 0; sp < ns; sp++) mixture += CONC(sp);
+    const double CE = CONC(E);
+    const double ne_pr = CE * 6.02214085774e23;
+    const double Ue = 1.5 * Te * ne_pr * 1.380649e-23;
+    const double TeeV = Te / 11595.0;
+    double Te_pow[9], invTe_pow[4];
+    if (M.any_fit) {
+        const double logTe = log(TeeV);
+        const double invTeeV = (Te == 0) ? 1.0 : 1.0 / (TeeV);
+        for (int q = 0; q < 9; q++) Te_pow[q] = pow(logTe, q);
+        for (int q = 0; q < 4; q++) invTe_pow[q] = pow(invTeeV, q + 1);
+    }
+    const double eV2J = 1.602176620800000e-19, NA = 6.02214085774e23, kB = 1.380649e-23;
+    const double mee = Ue / (CE * NA);
+    const double sefact = (tanh((Te - T) / 0.5) + 1.0) / 2.0;
+    double ener = 0.0;
+    const int nr = M.nreac;
+    for (int r = 0; r < nr; r++) {
+        const double k_f = gen_kf(M, r, tc, Te, T, TeeV, Te_pow, invTe_pow);
+        const double pre = M.tb[r] ? mixture * k_f : k_f;
+        double cprod = 1.0;
+        bool first = true;
+        for (int q = 0; q < M.nreact[r]; q++) {
+            const double c = CONC(M.rsp[q][r]);
+            const int nu = (int)M.rnu[q][r];
+            for (int e = 0; e < nu; e++) {
+                cprod = first ? c : cprod * c;
+                first = false;
+            }
+        }
+        const double qf = pre * cprod;
+        double qr = 0.0;
+        if (M.rev[r]) {
+            double s = 0.0;
+            for (int q = 0; q < M.ng[r]; q++) {
+                const double g = tc[VD_MAX_REACTIONS + M.gsp[q][r]], nu = M.gnu[q][r];
+                const double tm = nu == 1.0 ? g : (nu == -1.0 ? -g : nu * g);
+                s = q == 0 ? tm : s + tm;
+            }
+            double v = pre * exp(-(s));
+            for (int p = 0; p < M.refcp[r]; p++) v = v * refC;
+            for (int p = 0; p < -M.refcp[r]; p++) v = v * refCinv;
+            double rp = 1.0;
+            first = true;
+            for (int q = 0; q < M.nrev[r]; q++) {
+                const double c = CONC(M.revsp[q][r]);
+                const int nu = (int)M.revnu[q][r];
+                for (int e = 0; e < nu; e++) {
+                    rp = first ? c : rp * c;
+                    first = false;
+                }
+            }
+            qr = v * rp;
+        }
+        const double qdot = qf - qr;
+        for (int q = 0; q < M.nwd[r]; q++) {
+            const double nu = M.wdnu[q][r];
+            double* wp = &w[M.wdsp[q][r] * CHEM_BLOCK + tid];
+            if (nu == 1.0) *wp += qdot;
+            else if (nu == -1.0) *wp -= qdot;
+            else if (nu > 0.0) *wp += nu * qdot;
+            else *wp -= (-nu) * qdot;
+        }
+        const int rt = M.rxntype[r];
+        if (rt == 1 || rt == 2) ener -= (M.eexci[r] * eV2J) * NA * (qf - qr) * sefact;
+        if (rt == 3 || rt == 4) ener -= mee * NA * (qf - qr);
+        if (rt == 5) {
+            const int el = M.elidx[r];
+            const double qq = CE * CONC(el) * k_f;
+            ener -= 3.0 * kB * (M.molwt[E] / M.molwt[el]) * (Te - T) * qq * NA;
+        }
+        if (rt == 7) ener += mee * NA * (qf - qr);
+    }
+#undef CONC
+    const int orx = a.rxn.off(i, j, k);
+    for (int sp = 0; sp < ns; sp++)
+        a.rxn.p[(int64_t)sp * a.rxn.ns + orx] = ((w[sp * CHEM_BLOCK + tid] * 1.0e-6) * VD_N_A) * 1.0e6;
+    a.rxn.p[(int64_t)ns * a.rxn.ns + orx] = ener;
+}
+
 // user_sources::add_user_react_sources (Source/ScalarSolve.cpp:158-163):
 // the case's constant sources added to every component of rxn_src
 __global__ void __launch_bounds__(256) k_user_src(ChemArgs a) {
@@ -188,9 +351,25 @@ int update_rxnsrc(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4*
     a.nz = box->hi[2] - box->lo[2] + 1;
     int64_t tot = (int64_t)a.nx * a.ny * a.nz;
     size_t lds = sizeof(double) * (m->hdev.nspec + 1) * CHEM_BLOCK;
+    if (m->hdev.chem_form == VD_CHEM_GEN) {
+        double* tc = nullptr;
+        VD_TRY(scratch(ctx, "gen_tconst", sizeof(double) * GEN_TCONST_DOUBLES, (void**)&tc));
+        kt_begin(ctx, VD_KT_CHEM);
+        hipLaunchKernelGGL(k_gen_tconst, dim3(1), dim3(256), 0, ctx->stream, m->dev, a.Tg, tc);
+        if (m->use_spec && m->fn) {
+            int st = chem_jit_launch(ctx, m, sb, rxn, box, prm->gas_temperature, tc);
+            kt_end(ctx, VD_KT_CHEM, 8.0 * (double)tot * (2 * m->hdev.nspec + 3));
+            if (st != VD_OK) return st;
+        } else {
+            hipLaunchKernelGGL(k_rxn_gen, grid1d(tot, CHEM_BLOCK), dim3(CHEM_BLOCK), lds, ctx->stream, a, tc);
+            kt_end(ctx, VD_KT_CHEM, 8.0 * (double)tot * (2 * m->hdev.nspec + 3));
+        }
+        VD_CHECK_LAUNCH();
+        return add_user_src(ctx, m, a);
+    }
     kt_begin(ctx, VD_KT_CHEM);
     if (m->hdev.chem_form == VD_CHEM_TABLE && m->use_spec && m->fn) {
-        int st = chem_jit_launch(ctx, m, sb, rxn, box, prm->gas_temperature);
+        int st = chem_jit_launch(ctx, m, sb, rxn, box, prm->gas_temperature, nullptr);
         kt_end(ctx, VD_KT_CHEM, 8.0 * (double)tot * (2 * m->hdev.nspec + 3));
         if (st != VD_OK) return st;
         return add_user_src(ctx, m, a);

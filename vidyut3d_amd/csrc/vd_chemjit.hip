@@ -30,6 +30,7 @@ namespace vd {
 #define VD_GEN_ARGS_BODY                                                        \
     const double* sb;                                                           \
     double* rx;                                                                 \
+    const double* tc;                                                           \
     long long sbn, rxn;                                                         \
     int sbo, rxo, sbj, sbk, rxj, rxk, nx, ny, nz, pad;                          \
     double Tg;
@@ -134,6 +135,153 @@ static bool chem_source(const DevMech& h, std::string& s) {
     return true;
 }
 
+static std::string num(double v) {
+    std::string o;
+    lit(o, v);
+    return o;
+}
+
+// VD_CHEM_GEN: the generated productionRate of a mechanism read as data
+// (test/Reactor/CO2_H2_Ar_mechanism_reduced/Chemistry.H:913-6380) printed
+// back as straight-line code, operation for operation as k_rxn_gen and the
+// oracle evaluate it; the gas-temperature-only k_f and g/RT come from the
+// per-launch table a.tc (k_gen_tconst)
+static bool chem_source_gen(const DevMech& h, std::string& s) {
+    const int ns = h.nspec, E = h.e_idx;
+    for (int r = 0; r < h.nreac; r++)
+        if (!std::isfinite(h.A[r]) || !std::isfinite(h.A2[r]) || !std::isfinite(h.beta[r]) || !std::isfinite(h.Ea[r]))
+            return false;
+    s.reserve(256 * 1024);
+    s += "struct GenArgs { " VD_STR(VD_GEN_ARGS_BODY) " };\n";
+    s += "extern \"C\" __global__ void __launch_bounds__(256) vd_chem_spec(GenArgs a) {\n";
+    s += "  const unsigned nb = gridDim.x, b = blockIdx.x;\n"
+         "  const unsigned q8 = nb >> 3, r8 = nb & 7u, x8 = b & 7u;\n"
+         "  const unsigned lb = x8 * q8 + (x8 < r8 ? x8 : r8) + (b >> 3);\n"
+         "  const unsigned t = lb * 256u + threadIdx.x;\n"
+         "  const unsigned nx = (unsigned)a.nx, ny = (unsigned)a.ny;\n"
+         "  if (t >= nx * ny * (unsigned)a.nz) return;\n"
+         "  const unsigned i = t % nx, r0 = t / nx, j = r0 % ny, k = r0 / ny;\n"
+         "  const double* __restrict__ s = a.sb + (a.sbo + (int)i + (int)j * a.sbj + (int)k * a.sbk);\n"
+         "  double* __restrict__ o = a.rx + (a.rxo + (int)i + (int)j * a.rxj + (int)k * a.rxk);\n"
+         "  const double* __restrict__ tc = a.tc;\n"
+         "  const long long n = a.sbn;\n";
+    s += "  const double Te = s[" + std::to_string(VD_ETEMP(ns)) + " * n];\n";
+    s += "  const double T = a.Tg;\n  const double invT = 1.0 / T;\n"
+         "  const double refC = 101325 / 8.31446 * invT;\n  const double refCinv = 1 / refC;\n";
+    const std::string NA_UNIV = num(VD_N_A);
+    for (int sp = 0; sp < ns; sp++)
+        s += "  const double c" + std::to_string(sp) + " = ((s[" + std::to_string(sp) + " * n] * 1.0e-6 / " + NA_UNIV +
+             ") * 1.0e6);\n";
+    if (h.any_tb) {
+        s += "  double mixture = 0.0;\n";
+        for (int sp = 0; sp < ns; sp++) s += "  mixture += c" + std::to_string(sp) + ";\n";
+    }
+    const std::string cE = "c" + std::to_string(E);
+    s += "  const double ne_pr = " + cE + " * 6.02214085774e23;\n"
+         "  const double Ue = 1.5 * Te * ne_pr * 1.380649e-23;\n"
+         "  const double TeeV = Te / 11595.0;\n";
+    if (h.any_fit) {
+        s += "  const double logTe = log(TeeV);\n  const double invTeeV = (Te == 0) ? 1.0 : 1.0 / (TeeV);\n";
+        for (int q = 0; q < 9; q++) s += "  const double tp" + std::to_string(q) + " = pow(logTe, " + std::to_string(q) + ");\n";
+        for (int q = 0; q < 4; q++)
+            s += "  const double itp" + std::to_string(q) + " = pow(invTeeV, " + std::to_string(q + 1) + ");\n";
+    }
+    s += "  const double eV2J = 1.602176620800000e-19, NA = 6.02214085774e23, kB = 1.380649e-23;\n"
+         "  const double mee = Ue / (" + cE + " * NA);\n"
+         "  const double sefact = (tanh((Te - T) / 0.5) + 1.0) / 2.0;\n"
+         "  double ener = 0.0;\n";
+    for (int sp = 0; sp < ns; sp++) s += "  double w" + std::to_string(sp) + " = 0.0;\n";
+    auto fit_sum = [&](const double* c, int nterm, const char* pw) {
+        std::string o = "double s_ = 0.0;";
+        for (int q = 0; q < nterm; q++) o += " s_ += " + num(c[q]) + " * " + pw + std::to_string(q) + ";";
+        return o;
+    };
+    for (int r = 0; r < h.nreac; r++) {
+        s += "  { double k_;\n";
+        switch (h.rkind[r]) {
+        case VD_RATE_ARRH:
+            s += "    k_ = tc[" + std::to_string(r) + "];\n";
+            break;
+        case VD_RATE_TEPOW:
+            s += "    k_ = " + num(h.A[r]) + " * pow(Te / " + num(h.tref[r]) + ", " + num(h.beta[r]) + ");\n";
+            if (h.tgdiv[r]) s += "    k_ = k_ / T;\n";
+            break;
+        case VD_RATE_TEARRH:
+            s += "    k_ = " + num(h.A[r]) + " * pow(Te, " + num(h.beta[r]) + ") * exp(" + num(-h.Ea[r]) + " / Te);\n";
+            break;
+        case VD_RATE_JANEV:
+        case VD_RATE_FFIT: {
+            double c[9];
+            for (int q = 0; q < 9; q++) c[q] = h.fit[q][r];
+            const bool j = h.rkind[r] == VD_RATE_JANEV;
+            s += "    { " + fit_sum(c, j ? 9 : 4, j ? "tp" : "itp") + " k_ = " + num(h.A[r]) +
+                 " * exp(s_) * 6.02214085774e23; }\n";
+            break;
+        }
+        default: {
+            double c[9], c2[4];
+            for (int q = 0; q < 9; q++) c[q] = h.fit[q][r];
+            for (int q = 0; q < 4; q++) c2[q] = h.fit2[q][r];
+            s += "    if (TeeV < " + num(h.fsw[r]) + ") { " + fit_sum(c, 9, "tp") + " k_ = " + num(h.A[r]) +
+                 " * exp(s_) * 6.02214085774e23; }\n    else { " + fit_sum(c2, 4, "itp") + " k_ = " + num(h.A2[r]) +
+                 " * exp(s_) * 6.02214085774e23; }\n";
+            break;
+        }
+        }
+        if (h.kmax[r] > 0.0) s += "    k_ = (" + num(h.kmax[r]) + " < k_) ? " + num(h.kmax[r]) + " : k_;\n";
+        s += std::string("    const double pre_ = ") + (h.tb[r] ? "mixture * k_;\n" : "k_;\n");
+        auto prod = [&](const int* sp, const double* nu, int cnt, int stride_r) -> std::string {
+            std::string p;
+            for (int q = 0; q < cnt; q++)
+                for (int e = 0; e < (int)nu[q * stride_r]; e++) {
+                    const std::string c = "c" + std::to_string(sp[q * stride_r]);
+                    p = p.empty() ? c : "(" + p + " * " + c + ")";
+                }
+            return p.empty() ? "1.0" : p;
+        };
+        s += "    const double qf_ = pre_ * " + prod(&h.rsp[0][r], &h.rnu[0][r], h.nreact[r], VD_MAX_REACTIONS) + ";\n";
+        if (h.rev[r]) {
+            std::string gs;
+            for (int q = 0; q < h.ng[r]; q++) {
+                const std::string g = "tc[" + std::to_string(VD_MAX_REACTIONS + h.gsp[q][r]) + "]";
+                const double nu = h.gnu[q][r];
+                const std::string t = nu == 1.0 ? g : (nu == -1.0 ? "(-" + g + ")" : "(" + num(nu) + " * " + g + ")");
+                gs = q == 0 ? t : "(" + gs + " + " + t + ")";
+            }
+            std::string v = "(pre_ * exp(-(" + gs + ")))";
+            for (int p = 0; p < h.refcp[r]; p++) v = "(" + v + " * refC)";
+            for (int p = 0; p < -h.refcp[r]; p++) v = "(" + v + " * refCinv)";
+            s += "    const double qr_ = " + v + " * " + prod(&h.revsp[0][r], &h.revnu[0][r], h.nrev[r], VD_MAX_REACTIONS) +
+                 ";\n";
+        } else {
+            s += "    const double qr_ = 0.0;\n";
+        }
+        s += "    const double qd_ = qf_ - qr_;\n";
+        for (int q = 0; q < h.nwd[r]; q++) {
+            const double nu = h.wdnu[q][r];
+            const std::string w = "    w" + std::to_string(h.wdsp[q][r]);
+            if (nu == 1.0) s += w + " += qd_;\n";
+            else if (nu == -1.0) s += w + " -= qd_;\n";
+            else if (nu > 0.0) s += w + " += " + num(nu) + " * qd_;\n";
+            else s += w + " -= " + num(-nu) + " * qd_;\n";
+        }
+        const int rt = h.rxntype[r];
+        if (rt == 1 || rt == 2) s += "    ener -= (" + num(h.eexci[r]) + " * eV2J) * NA * (qf_ - qr_) * sefact;\n";
+        if (rt == 3 || rt == 4) s += "    ener -= mee * NA * (qf_ - qr_);\n";
+        if (rt == 5) {
+            const int el = h.elidx[r];
+            s += "    { const double qq = " + cE + " * c" + std::to_string(el) + " * k_;\n"
+                 "      ener -= 3.0 * kB * (" + num(h.molwt[E]) + " / " + num(h.molwt[el]) + ") * (Te - T) * qq * NA; }\n";
+        }
+        if (rt == 7) s += "    ener += mee * NA * (qf_ - qr_);\n";
+        s += "  }\n";
+    }
+    for (int sp = 0; sp < ns; sp++)
+        s += "  o[" + std::to_string(sp) + " * a.rxn] = ((w" + std::to_string(sp) + " * 1.0e-6) * " + NA_UNIV + ") * 1.0e6;\n";
+    s += "  o[" + std::to_string(ns) + " * a.rxn] = ener;\n}\n";
+    return true;
+}
+
 // code objects by generated source, shared by every context in the process
 static std::mutex g_jit_mu;
 static std::map<std::string, std::vector<char>> g_jit_cache;
@@ -177,12 +325,12 @@ static int jit_compile(const std::string& src, std::vector<char>& code) {
 
 int chem_jit_load(vd_mech* m) {
     if (m->fn) return VD_OK;
-    if (m->hdev.chem_form != VD_CHEM_TABLE) {
-        set_error("the specialised source kernel exists for table mechanisms only");
+    if (m->hdev.chem_form != VD_CHEM_TABLE && m->hdev.chem_form != VD_CHEM_GEN) {
+        set_error("the specialised source kernel exists for table and generated mechanisms only");
         return VD_EINVAL;
     }
     std::string src;
-    if (!chem_source(m->hdev, src)) {
+    if (!(m->hdev.chem_form == VD_CHEM_GEN ? chem_source_gen(m->hdev, src) : chem_source(m->hdev, src))) {
         set_error("mechanism has a non-finite constant; no specialised kernel");
         return VD_EINVAL;
     }
@@ -210,10 +358,11 @@ void chem_jit_unload(vd_mech* m) {
 }
 
 int chem_jit_launch(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4* rxn, const vd_box* box,
-                    double Tg) {
+                    double Tg, const double* tc) {
     GenArgs a;
     a.sb = sb->p;
     a.rx = rxn->p;
+    a.tc = tc;
     a.sbn = sb->nstride;
     a.rxn = rxn->nstride;
     a.sbj = (int)sb->jstride;

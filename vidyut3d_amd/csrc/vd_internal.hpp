@@ -114,7 +114,36 @@ struct DevMech {
     // user_sources::add_user_react_sources constants
     double user_src[VD_MAX_SPECIES + 1];
     int any_user_src;
+    // VD_CHEM_GEN (a generated productionRate read as data)
+    int rkind[VD_MAX_REACTIONS], tgdiv[VD_MAX_REACTIONS], tb[VD_MAX_REACTIONS], rev[VD_MAX_REACTIONS];
+    double tref[VD_MAX_REACTIONS], A2[VD_MAX_REACTIONS], fsw[VD_MAX_REACTIONS], kmax[VD_MAX_REACTIONS];
+    double fit[9][VD_MAX_REACTIONS], fit2[4][VD_MAX_REACTIONS];
+    int ng[VD_MAX_REACTIONS], gsp[VD_GEN_NG][VD_MAX_REACTIONS];
+    double gnu[VD_GEN_NG][VD_MAX_REACTIONS];
+    int refcp[VD_MAX_REACTIONS], nrev[VD_MAX_REACTIONS], revsp[VD_MAX_PRODUCTS][VD_MAX_REACTIONS];
+    double revnu[VD_MAX_PRODUCTS][VD_MAX_REACTIONS];
+    int nwd[VD_MAX_REACTIONS], wdsp[VD_GEN_NW][VD_MAX_REACTIONS];
+    double wdnu[VD_GEN_NW][VD_MAX_REACTIONS];
+    double gtmid[VD_MAX_SPECIES], glo[VD_MAX_SPECIES][7], ghi[VD_MAX_SPECIES][7];
+    int any_rev, any_tb, any_fit;
 };
+
+// VD_CHEM_GEN: the per-launch table of what depends on the gas temperature
+// only (uniform over a launch): k_f of every VD_RATE_ARRH reaction, then
+// g/RT of every species.  Computed once per launch by k_gen_tconst with the
+// generated file's own arithmetic, so every cell sees the values its own
+// evaluation would give.
+constexpr int GEN_TCONST_DOUBLES = VD_MAX_REACTIONS + VD_MAX_SPECIES;
+
+// g/RT of species k (gibbs() of a generated Chemistry.H): the terms summed
+// left to right, the low range below gtmid
+__host__ __device__ inline double gen_gibbs(const DevMech& m, int k, double T) {
+#pragma clang fp contract(off)
+    const double T2 = T * T, T3 = T * T * T, T4 = T * T * T * T;
+    const double invT = 1.0 / T, logT = log(T);
+    const double* c = (m.gtmid[k] > 0.0 && !(T < m.gtmid[k])) ? m.ghi[k] : m.glo[k];
+    return c[0] * invT + c[1] + c[2] * logT + c[3] * T + c[4] * T2 + c[5] * T3 + c[6] * T4;
+}
 
 // specMob / specDiff of the case plug-in (constant mobilities, Einstein
 // diffusion; index nspec = electron energy with 5/3 of the electron values)
@@ -218,7 +247,7 @@ namespace vd {
 int chem_jit_load(vd_mech* m);
 void chem_jit_unload(vd_mech* m);
 int chem_jit_launch(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4* rxn, const vd_box* box,
-                    double Tg);
+                    double Tg, const double* tc);
 }
 
 namespace vd {
