@@ -4,14 +4,16 @@ Headline workload (the `value` of the JSON line): 3-D 256^3 single-level
 discharge, FP64, 10 species / 30 seeded reactions (air-like), WENO-Z
 transport of electrons, electron energy and 4 ions with implicit
 multigrid diffusion, chemistry, Joule heating, the potential multigrid
-solve and the CFL reduction every step.  One "step" = the body of
-Vidyut::Evolve's step loop (Source/Evolve.cpp:33-369).
+solve and the CFL reduction every step, for the configuration's 200 steps
+(after 2 warm-up steps).  One "step" = the body of Vidyut::Evolve's step
+loop (Source/Evolve.cpp:33-369).
 
 The same run also measures (under "configs"): the 2-D 128^2 discharge for
 100 steps against the CPU oracle, the headline case with FP32 V-cycles in
 FP64 iterative refinement, the 512^3 dielectric Poisson solve FP64 vs
 mixed precision, one 40-species / 200-reaction source evaluation at 256^3
-(FP64 FLOP/s) and the serving loop (p50 / p99 time-to-solution).
+(FP64 FLOP/s), the 44-species mechanism converted from a reference case's
+generated Chemistry.H, and the serving loop (p50 / p99 time-to-solution).
 
 Prints ONE JSON line.  --gpus above 1 is refused: this build runs on one
 GPU.
@@ -375,7 +377,7 @@ def bench_generated_chemistry(native, abi, cases, dev, n, with_oracle):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=200, help="timed steps of the headline (the configuration's 200)")
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--precision", type=int, default=0, help="headline: 0 FP64 MG, 1 FP32 V-cycles in FP64 IR")
