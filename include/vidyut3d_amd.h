@@ -349,6 +349,26 @@ int vd_clamp_min(vd_ctx* ctx, vd_array4* a, const vd_box* box, int comp, double 
 int vd_fill_patch(vd_ctx* ctx, const vd_array4* src, vd_array4* dst, const vd_box* box,
                   const vd_geom* geom, int scomp, int ncomp);
 
+/* Vidyut::FillPatch single level over a BoxArray of nbox boxes (level 0
+ * chopped by amr.max_grid_size; Source/GridWork.cpp:112-124, FillBoundary
+ * between the boxes, periodic images and foextrap faces): src[b] holds the
+ * valid data of boxes[b], dst[b] (grown) receives comps [scomp,
+ * scomp+ncomp) on its whole extent, each cell from the domain cell found
+ * by wrapping periodic indices and clamping the others, read from the box
+ * that owns it.  The boxes must tile the domain without overlap; dst may
+ * be src (ghost cells filled in place). */
+int vd_fill_patch_boxes(vd_ctx* ctx, int nbox, const vd_array4* src, vd_array4* dst,
+                        const vd_box* boxes, const vd_geom* geom, int scomp, int ncomp);
+/* The valid cells of a level held in nbox boxes (tiling the domain) into
+ * one array over the domain, and back: the potential and implicit solves
+ * (vd_solve_potential, vd_implicit_solve_scalar, vd_mlmg_solve) take a
+ * level as one box, as the MLMG of Source/PotentialSolve.cpp:148-336
+ * takes the whole MultiFab. */
+int vd_level_gather(vd_ctx* ctx, int nbox, const vd_array4* parts, const vd_box* boxes,
+                    vd_array4* whole, const vd_geom* geom, int scomp, int ncomp);
+int vd_level_scatter(vd_ctx* ctx, int nbox, const vd_array4* whole, vd_array4* parts,
+                     const vd_box* boxes, const vd_geom* geom, int scomp, int ncomp);
+
 /* Vidyut::compute_scalar_transport_flux (Source/ScalarSolve.cpp:170-255)
  * -> compute_flux (Source/compute_explicit_flux.H:200-326) on every face
  * of `box` in each direction.  sborder needs hyp_order-dependent ghosts

@@ -234,7 +234,287 @@ int fill_patch(vd_ctx* ctx, const vd_array4* src, vd_array4* dst, const vd_box* 
     return VD_OK;
 }
 
+// ---------------------------------------------- levels held in several boxes
+// Vidyut::FillPatch at level 0 over a BoxArray chopped by amr.max_grid_size
+// (Source/GridWork.cpp:112-124): AMReX fills a box's ghost cells from the
+// valid cells of the other boxes (FillBoundary, periodic images included)
+// and then the physical faces by foextrap, which together give every
+// destination cell the value of the domain cell found by wrapping periodic
+// indices and clamping the others -- the single-box rule, with the source
+// cell read from the box that owns it.
+struct BoxPair {
+    Arr<double> src, dst;
+    int3 glo;           // first cell written (the grown box for a fill, the box for gather/scatter)
+    int gx, gy, gz;     // extents of the cells written
+    unsigned blk0;      // first block of this box in the batched grid
+};
+
+// the box of the batched grid that block `blk` belongs to
+__device__ __forceinline__ int box_of_block(const BoxPair* __restrict__ t, int nbox, unsigned blk) {
+    int lo = 0, hi = nbox - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (t[mid].blk0 <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+struct OwnerMap {
+    const int* owner;   // box id per (granule) of the domain
+    int3 lo, n, g, m;   // domain lo and extent, granule size, map extent
+};
+
+__global__ void __launch_bounds__(256) k_fill_patch_boxes(const BoxPair* __restrict__ tab, int nbox, OwnerMap om,
+                                                         int dim, int3 per, int scomp) {
+    const int q = blockIdx.y + scomp;
+    const int b = box_of_block(tab, nbox, blockIdx.x);
+    const BoxPair& B = tab[b];
+    const int64_t t = (int64_t)(blockIdx.x - B.blk0) * 256 + threadIdx.x;
+    if (t >= (int64_t)B.gx * B.gy * B.gz) return;
+    const int i = B.glo.x + (int)(t % B.gx);
+    const int64_t r = t / B.gx;
+    const int j = B.glo.y + (int)(r % B.gy), k = B.glo.z + (int)(r / B.gy);
+    int c[3] = {i, j, k};
+    const int l[3] = {om.lo.x, om.lo.y, om.lo.z}, nn[3] = {om.n.x, om.n.y, om.n.z}, pp[3] = {per.x, per.y, per.z};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        if (d >= dim) break;
+        int v = c[d] - l[d];
+        if (v < 0 || v >= nn[d]) {
+            if (pp[d]) {
+                v %= nn[d];
+                if (v < 0) v += nn[d];
+            } else {
+                v = v < 0 ? 0 : nn[d] - 1;
+            }
+        }
+        c[d] = v + l[d];
+    }
+    const int gi = (c[0] - om.lo.x) / om.g.x, gj = dim > 1 ? (c[1] - om.lo.y) / om.g.y : 0,
+              gk = dim > 2 ? (c[2] - om.lo.z) / om.g.z : 0;
+    const Arr<double>& S = tab[om.owner[(gk * om.m.y + gj) * om.m.x + gi]].src;
+    B.dst.comp(q)[B.dst.off(i, j, k)] = S.comp(q)[S.off(c[0], c[1], c[2])];
+}
+
+// valid cells of each part into the whole level array (gather) or back
+template <bool GATHER>
+__global__ void __launch_bounds__(256) k_level_copy(const BoxPair* __restrict__ tab, int nbox, int scomp) {
+    const int q = blockIdx.y + scomp;
+    const int b = box_of_block(tab, nbox, blockIdx.x);
+    const BoxPair& B = tab[b];
+    const int64_t t = (int64_t)(blockIdx.x - B.blk0) * 256 + threadIdx.x;
+    if (t >= (int64_t)B.gx * B.gy * B.gz) return;
+    const int i = B.glo.x + (int)(t % B.gx);
+    const int64_t r = t / B.gx;
+    const int j = B.glo.y + (int)(r % B.gy), k = B.glo.z + (int)(r / B.gy);
+    // src = the part, dst = the whole level
+    if (GATHER) B.dst.comp(q)[B.dst.off(i, j, k)] = B.src.comp(q)[B.src.off(i, j, k)];
+    else B.src.comp(q)[B.src.off(i, j, k)] = B.dst.comp(q)[B.dst.off(i, j, k)];
+}
+
+static int gcd_i(int a, int b) {
+    while (b) {
+        const int t = a % b;
+        a = b;
+        b = t;
+    }
+    return a < 0 ? -a : a;
+}
+
+static bool covers(const vd_array4* a, const int lo[3], const int hi[3], int dim) {
+    for (int d = 0; d < dim; d++)
+        if (a->begin[d] > lo[d] || a->end[d] <= hi[d]) return false;
+    return true;
+}
+
+// the boxes must lie in the domain and tile it without overlap: checked on
+// an owner map at the granularity of the box corners (which the fill
+// kernel uses to find the box holding a cell); the map is cached per
+// context for the last box list
+static int level_owner(vd_ctx* ctx, int nbox, const vd_box* boxes, const vd_geom* g, OwnerMap* om) {
+    const int dim = g->dim;
+    int lo[3] = {g->dom_lo[0], g->dom_lo[1], g->dom_lo[2]};
+    int n[3] = {1, 1, 1}, gr[3] = {1, 1, 1}, m[3] = {1, 1, 1};
+    for (int d = 0; d < dim; d++) {
+        n[d] = g->dom_hi[d] - g->dom_lo[d] + 1;
+        gr[d] = n[d];
+    }
+    for (int b = 0; b < nbox; b++)
+        for (int d = 0; d < dim; d++) {
+            if (boxes[b].lo[d] < g->dom_lo[d] || boxes[b].hi[d] > g->dom_hi[d] || boxes[b].hi[d] < boxes[b].lo[d]) {
+                set_error("level boxes: box %d is empty or leaves the domain", b);
+                return VD_EINVAL;
+            }
+            gr[d] = gcd_i(gr[d], boxes[b].lo[d] - lo[d]);
+            gr[d] = gcd_i(gr[d], boxes[b].hi[d] + 1 - lo[d]);
+        }
+    int64_t cells = 1;
+    for (int d = 0; d < dim; d++) {
+        m[d] = n[d] / gr[d];
+        cells *= m[d];
+    }
+    if (cells > (1 << 24)) {
+        set_error("level boxes: corners on a %d x %d x %d grid need an owner map of %lld entries", gr[0], gr[1], gr[2],
+                  (long long)cells);
+        return VD_EINVAL;
+    }
+    std::vector<int> key;
+    key.reserve(6 * nbox + 9);
+    for (int d = 0; d < 3; d++) key.push_back(d < dim ? g->dom_lo[d] : 0), key.push_back(d < dim ? g->dom_hi[d] : 0);
+    key.push_back(dim), key.push_back(nbox), key.push_back(0);
+    for (int b = 0; b < nbox; b++)
+        for (int d = 0; d < 3; d++) key.push_back(d < dim ? boxes[b].lo[d] : 0), key.push_back(d < dim ? boxes[b].hi[d] : 0);
+    void* dmap = nullptr;
+    VD_TRY(scratch(ctx, "level_owner", sizeof(int) * (size_t)cells, &dmap));
+    if (ctx->owner_key != key) {
+        std::vector<int> own((size_t)cells, -1);
+        for (int b = 0; b < nbox; b++) {
+            int a[3] = {0, 0, 0}, e[3] = {1, 1, 1};
+            for (int d = 0; d < dim; d++) {
+                a[d] = (boxes[b].lo[d] - lo[d]) / gr[d];
+                e[d] = (boxes[b].hi[d] + 1 - lo[d]) / gr[d];
+            }
+            for (int k = a[2]; k < e[2]; k++)
+                for (int j = a[1]; j < e[1]; j++)
+                    for (int i = a[0]; i < e[0]; i++) {
+                        int& o = own[((size_t)k * m[1] + j) * m[0] + i];
+                        if (o >= 0) {
+                            set_error("level boxes: boxes %d and %d overlap", o, b);
+                            return VD_EINVAL;
+                        }
+                        o = b;
+                    }
+        }
+        for (size_t c = 0; c < own.size(); c++)
+            if (own[c] < 0) {
+                set_error("level boxes: the boxes leave part of the domain uncovered");
+                return VD_EINVAL;
+            }
+        VD_HIP(hipMemcpy(dmap, own.data(), sizeof(int) * own.size(), hipMemcpyHostToDevice));
+        ctx->owner_key = key;
+    }
+    om->owner = (const int*)dmap;
+    om->lo = make_int3(lo[0], lo[1], lo[2]);
+    om->n = make_int3(n[0], n[1], n[2]);
+    om->g = make_int3(gr[0], gr[1], gr[2]);
+    om->m = make_int3(m[0], m[1], m[2]);
+    return VD_OK;
+}
+
+// the per-box table on the device, uploaded on the context's stream from
+// pageable memory (the call returns once the runtime has taken the data)
+static int upload_pairs(vd_ctx* ctx, const std::vector<BoxPair>& tab, BoxPair** out) {
+    void* p = nullptr;
+    VD_TRY(scratch(ctx, "level_pairs", sizeof(BoxPair) * tab.size(), &p));
+    VD_HIP(hipMemcpyAsync(p, tab.data(), sizeof(BoxPair) * tab.size(), hipMemcpyHostToDevice, ctx->stream));
+    *out = (BoxPair*)p;
+    return VD_OK;
+}
+
+int fill_patch_boxes(vd_ctx* ctx, int nbox, const vd_array4* src, vd_array4* dst, const vd_box* boxes,
+                     const vd_geom* g, int scomp, int ncomp) {
+    if (nbox < 1 || nbox > 65536) {
+        set_error("fill_patch_boxes: %d boxes", nbox);
+        return VD_EINVAL;
+    }
+    OwnerMap om;
+    VD_TRY(level_owner(ctx, nbox, boxes, g, &om));
+    if (ncomp <= 0) return VD_OK;
+    std::vector<BoxPair> tab((size_t)nbox);
+    unsigned blk = 0;
+    for (int b = 0; b < nbox; b++) {
+        if (!covers(&src[b], boxes[b].lo, boxes[b].hi, g->dim) || src[b].ncomp < scomp + ncomp ||
+            dst[b].ncomp < scomp + ncomp) {
+            set_error("fill_patch_boxes: array %d does not hold box %d or components [%d, %d)", b, b, scomp,
+                      scomp + ncomp);
+            return VD_EINVAL;
+        }
+        BoxPair& P = tab[b];
+        P.src = arr(&src[b]);
+        P.dst = arr(&dst[b]);
+        P.glo = make_int3(dst[b].begin[0], dst[b].begin[1], dst[b].begin[2]);
+        P.gx = dst[b].end[0] - dst[b].begin[0];
+        P.gy = dst[b].end[1] - dst[b].begin[1];
+        P.gz = dst[b].end[2] - dst[b].begin[2];
+        P.blk0 = blk;
+        const int64_t nb = ((int64_t)P.gx * P.gy * P.gz + 255) / 256;
+        if ((int64_t)blk + nb > 0x7fffffffLL) {
+            set_error("fill_patch_boxes: too many cells for one launch");
+            return VD_EINVAL;
+        }
+        blk += (unsigned)nb;
+    }
+    BoxPair* dtab = nullptr;
+    VD_TRY(upload_pairs(ctx, tab, &dtab));
+    const int3 per = make_int3(g->is_periodic[0], g->dim > 1 ? g->is_periodic[1] : 0, g->dim > 2 ? g->is_periodic[2] : 0);
+    hipLaunchKernelGGL(k_fill_patch_boxes, dim3(blk, ncomp), dim3(256), 0, ctx->stream, dtab, nbox, om, g->dim, per,
+                       scomp);
+    VD_CHECK_LAUNCH();
+    return VD_OK;
+}
+
+int level_copy(vd_ctx* ctx, bool gather, int nbox, const vd_array4* parts, const vd_box* boxes, const vd_array4* whole,
+               const vd_geom* g, int scomp, int ncomp) {
+    if (nbox < 1 || nbox > 65536) {
+        set_error("level gather/scatter: %d boxes", nbox);
+        return VD_EINVAL;
+    }
+    OwnerMap om;
+    VD_TRY(level_owner(ctx, nbox, boxes, g, &om));
+    if (ncomp <= 0) return VD_OK;
+    const int dom_lo[3] = {g->dom_lo[0], g->dom_lo[1], g->dom_lo[2]};
+    const int dom_hi[3] = {g->dom_hi[0], g->dom_hi[1], g->dom_hi[2]};
+    if (!covers(whole, dom_lo, dom_hi, g->dim) || whole->ncomp < scomp + ncomp) {
+        set_error("level gather/scatter: the whole-level array does not hold the domain");
+        return VD_EINVAL;
+    }
+    std::vector<BoxPair> tab((size_t)nbox);
+    unsigned blk = 0;
+    for (int b = 0; b < nbox; b++) {
+        if (!covers(&parts[b], boxes[b].lo, boxes[b].hi, g->dim) || parts[b].ncomp < scomp + ncomp) {
+            set_error("level gather/scatter: array %d does not hold box %d", b, b);
+            return VD_EINVAL;
+        }
+        BoxPair& P = tab[b];
+        P.src = arr(&parts[b]);
+        P.dst = arr(whole);
+        P.glo = make_int3(boxes[b].lo[0], boxes[b].lo[1], boxes[b].lo[2]);
+        P.gx = boxes[b].hi[0] - boxes[b].lo[0] + 1;
+        P.gy = boxes[b].hi[1] - boxes[b].lo[1] + 1;
+        P.gz = boxes[b].hi[2] - boxes[b].lo[2] + 1;
+        P.blk0 = blk;
+        blk += (unsigned)(((int64_t)P.gx * P.gy * P.gz + 255) / 256);
+    }
+    BoxPair* dtab = nullptr;
+    VD_TRY(upload_pairs(ctx, tab, &dtab));
+    if (gather)
+        hipLaunchKernelGGL(k_level_copy<true>, dim3(blk, ncomp), dim3(256), 0, ctx->stream, dtab, nbox, scomp);
+    else
+        hipLaunchKernelGGL(k_level_copy<false>, dim3(blk, ncomp), dim3(256), 0, ctx->stream, dtab, nbox, scomp);
+    VD_CHECK_LAUNCH();
+    return VD_OK;
+}
+
 }  // namespace vd
+
+extern "C" int vd_fill_patch_boxes(vd_ctx* ctx, int nbox, const vd_array4* src, vd_array4* dst,
+                                   const vd_box* boxes, const vd_geom* geom, int scomp, int ncomp) {
+    if (!ctx || !src || !dst || !boxes || !geom) return VD_EINVAL;
+    return vd::fill_patch_boxes(ctx, nbox, src, dst, boxes, geom, scomp, ncomp);
+}
+
+extern "C" int vd_level_gather(vd_ctx* ctx, int nbox, const vd_array4* parts, const vd_box* boxes,
+                               vd_array4* whole, const vd_geom* geom, int scomp, int ncomp) {
+    if (!ctx || !parts || !boxes || !whole || !geom) return VD_EINVAL;
+    return vd::level_copy(ctx, true, nbox, parts, boxes, whole, geom, scomp, ncomp);
+}
+
+extern "C" int vd_level_scatter(vd_ctx* ctx, int nbox, const vd_array4* whole, vd_array4* parts,
+                                const vd_box* boxes, const vd_geom* geom, int scomp, int ncomp) {
+    if (!ctx || !parts || !boxes || !whole || !geom) return VD_EINVAL;
+    return vd::level_copy(ctx, false, nbox, parts, boxes, whole, geom, scomp, ncomp);
+}
 
 extern "C" int vd_fill_patch(vd_ctx* ctx, const vd_array4* src, vd_array4* dst, const vd_box* box,
                              const vd_geom* geom, int scomp, int ncomp) {

@@ -260,6 +260,8 @@ struct vd_ctx {
     hipStream_t stream = nullptr;
     // named grow-only scratch buffers (device memory), reused across calls
     std::map<std::string, vd::Buf> scratch;
+    // the box list whose owner map is in scratch "level_owner"
+    std::vector<int> owner_key;
     // multigrid workspaces keyed by shape + precision
     std::map<std::string, vd::MGWork*> mg;
     // small pinned host buffer for scalar read-backs

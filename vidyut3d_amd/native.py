@@ -231,6 +231,76 @@ class Grid:
         return (abi.vd_array4 * 3)(*[f.view() for f in fs])
 
 
+def chop(geom, max_grid_size):
+    """The domain cut into boxes of at most max_grid_size cells per
+    direction (the last piece of a direction takes the remainder), as
+    (lo, hi) pairs in x-fastest order."""
+    cuts = []
+    for d in range(3):
+        if d < geom.dim:
+            lo, hi = geom.dom_lo[d], geom.dom_hi[d]
+            cuts.append([(a, min(a + max_grid_size - 1, hi)) for a in range(lo, hi + 1, max_grid_size)])
+        else:
+            cuts.append([(0, 0)])
+    return [((x[0], y[0], z[0]), (x[1], y[1], z[1])) for z in cuts[2] for y in cuts[1] for x in cuts[0]]
+
+
+class Level:
+    """A single-level domain held in several boxes (a BoxArray, e.g. level 0
+    chopped by amr.max_grid_size), with per-box device arrays."""
+
+    def __init__(self, dev, geom, boxes):
+        self.dev, self.geom, self.dim = dev, geom, geom.dim
+        self.boxes = [abi.make_box(lo, hi) for lo, hi in boxes]
+        self.cboxes = (abi.vd_box * len(self.boxes))(*self.boxes)
+        self.whole = Grid(dev, geom)
+
+    def arrays(self, ncomp, ng=0, host=None, host_ng=0):
+        """One array per box, grown by ng; with `host` (a domain array grown
+        by host_ng >= ng), each box's part of it."""
+        out = []
+        for b in self.boxes:
+            be, en = abi.grown_extent(b, ng, self.dim)
+            h = None
+            if host is not None:
+                sl = tuple(slice(be[d] - self.geom.dom_lo[d] + (host_ng if d < self.dim else 0),
+                                 en[d] - self.geom.dom_lo[d] + (host_ng if d < self.dim else 0)) for d in (2, 1, 0))
+                h = host[(slice(None),) + sl]
+            out.append(DeviceArray(self.dev, be, en, ncomp, h))
+        return out
+
+    def box_grid(self, b):
+        """A Grid whose box is box b (for the per-box entry points)."""
+        g = Grid(self.dev, self.geom)
+        g.box = self.boxes[b]
+        return g
+
+    @staticmethod
+    def views(arrs):
+        return (abi.vd_array4 * len(arrs))(*[a.view() for a in arrs])
+
+
+def fill_patch_boxes(level, src, dst, scomp=0, ncomp=None):
+    ncomp = src[0].ncomp if ncomp is None else ncomp
+    check(lib().vd_fill_patch_boxes(level.dev.ptr, len(level.boxes), Level.views(src), Level.views(dst),
+                                    level.cboxes, C.byref(level.geom), int(scomp), int(ncomp)),
+          "vd_fill_patch_boxes")
+
+
+def level_gather(level, parts, whole, scomp=0, ncomp=None):
+    ncomp = parts[0].ncomp if ncomp is None else ncomp
+    w = whole.view()
+    check(lib().vd_level_gather(level.dev.ptr, len(level.boxes), Level.views(parts), level.cboxes, C.byref(w),
+                                C.byref(level.geom), int(scomp), int(ncomp)), "vd_level_gather")
+
+
+def level_scatter(level, whole, parts, scomp=0, ncomp=None):
+    ncomp = parts[0].ncomp if ncomp is None else ncomp
+    w = whole.view()
+    check(lib().vd_level_scatter(level.dev.ptr, len(level.boxes), C.byref(w), Level.views(parts), level.cboxes,
+                                 C.byref(level.geom), int(scomp), int(ncomp)), "vd_level_scatter")
+
+
 # --------------------------------------------------------- entry points
 def fill_patch(grid, src, dst, scomp=0, ncomp=None):
     ncomp = src.ncomp if ncomp is None else ncomp
