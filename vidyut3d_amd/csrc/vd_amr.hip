@@ -220,6 +220,7 @@ using namespace vd;
 extern "C" int vd_average_down(vd_ctx* ctx, const vd_array4* fine, const vd_box* fbox, vd_array4* crse,
                                int scomp, int ncomp, const int ratio[3], int dim) {
     if (!ctx || !fine || !fbox || !crse || !ratio) return VD_EINVAL;
+    VD_TRY(check_extents("average_down", fbox, {fine, crse}));
     VD_TRY(check_ratio(ratio, dim));
     VD_TRY(check_aligned(fbox, ratio, dim));
     if (ncomp <= 0) return VD_OK;
@@ -249,6 +250,9 @@ extern "C" int vd_average_down(vd_ctx* ctx, const vd_array4* fine, const vd_box*
 extern "C" int vd_average_down_faces(vd_ctx* ctx, const vd_array4 fine[3], const vd_box* fbox, vd_array4 crse[3],
                                      const int ratio[3], int dim) {
     if (!ctx || !fine || !fbox || !crse || !ratio) return VD_EINVAL;
+    VD_TRY(check_box_extent("average_down_faces", fbox));
+    VD_TRY(check_face_extents("average_down_faces", fine, dim));
+    VD_TRY(check_face_extents("average_down_faces", crse, dim));
     VD_TRY(check_ratio(ratio, dim));
     VD_TRY(check_aligned(fbox, ratio, dim));
     const int r[3] = {ratio[0], ratio[1], dim == 3 ? ratio[2] : 1};
@@ -280,6 +284,8 @@ extern "C" int vd_fill_patch_two_levels(vd_ctx* ctx, const vd_array4* crse, cons
                                         const int ratio[3], int scomp, int ncomp) {
     if (!ctx || !crse || !cg || !dst || !fg || !ratio || nfine < 0 || (nfine > 0 && (!fine || !fboxes)))
         return VD_EINVAL;
+    VD_TRY(check_extents("fill_patch_two_levels", nullptr, {crse, dst}));
+    for (int b = 0; b < nfine; b++) VD_TRY(check_extents("fill_patch_two_levels", &fboxes[b], {&fine[b]}));
     const int dim = fg->dim;
     if (cg->dim != dim) {
         set_error("vd_fill_patch_two_levels: coarse and fine geometries differ in dim");

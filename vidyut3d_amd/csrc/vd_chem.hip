@@ -508,6 +508,7 @@ extern "C" int vd_update_rxnsrc(vd_ctx* ctx, const vd_mech* m, const vd_array4* 
                                 double time) {
     (void)time;
     if (!ctx || !m || !sborder || !rxn_src || !box || !prm) return VD_EINVAL;
+    VD_TRY(check_extents("update_rxnsrc", box, {sborder, rxn_src}));
     return update_rxnsrc(ctx, m, sborder, rxn_src, box, prm);
 }
 
@@ -519,5 +520,8 @@ extern "C" int vd_compute_elecenergy_source(vd_ctx* ctx, const vd_mech* m, const
     if (!ctx || !m || !sborder || !rxn_src || !efield || !gradne || !dsdt || !phi_new || !box ||
         !geom || !prm)
         return VD_EINVAL;
+    VD_TRY(check_extents("compute_elecenergy_source", box, {sborder, rxn_src, dsdt, phi_new}));
+    VD_TRY(check_face_extents("compute_elecenergy_source", efield, geom->dim));
+    VD_TRY(check_face_extents("compute_elecenergy_source", gradne, geom->dim));
     return elecenergy_source(ctx, m, sborder, rxn_src, efield, gradne, dsdt, phi_new, box, geom, prm);
 }

@@ -501,24 +501,30 @@ int level_copy(vd_ctx* ctx, bool gather, int nbox, const vd_array4* parts, const
 extern "C" int vd_fill_patch_boxes(vd_ctx* ctx, int nbox, const vd_array4* src, vd_array4* dst,
                                    const vd_box* boxes, const vd_geom* geom, int scomp, int ncomp) {
     if (!ctx || !src || !dst || !boxes || !geom) return VD_EINVAL;
+    for (int b = 0; b < nbox; b++) VD_TRY(vd::check_extents("fill_patch_boxes", &boxes[b], {&src[b], &dst[b]}));
     return vd::fill_patch_boxes(ctx, nbox, src, dst, boxes, geom, scomp, ncomp);
 }
 
 extern "C" int vd_level_gather(vd_ctx* ctx, int nbox, const vd_array4* parts, const vd_box* boxes,
                                vd_array4* whole, const vd_geom* geom, int scomp, int ncomp) {
     if (!ctx || !parts || !boxes || !whole || !geom) return VD_EINVAL;
+    VD_TRY(vd::check_extents("level gather/scatter", nullptr, {whole}));
+    for (int b = 0; b < nbox; b++) VD_TRY(vd::check_extents("level gather/scatter", &boxes[b], {&parts[b]}));
     return vd::level_copy(ctx, true, nbox, parts, boxes, whole, geom, scomp, ncomp);
 }
 
 extern "C" int vd_level_scatter(vd_ctx* ctx, int nbox, const vd_array4* whole, vd_array4* parts,
                                 const vd_box* boxes, const vd_geom* geom, int scomp, int ncomp) {
     if (!ctx || !parts || !boxes || !whole || !geom) return VD_EINVAL;
+    VD_TRY(vd::check_extents("level gather/scatter", nullptr, {whole}));
+    for (int b = 0; b < nbox; b++) VD_TRY(vd::check_extents("level gather/scatter", &boxes[b], {&parts[b]}));
     return vd::level_copy(ctx, false, nbox, parts, boxes, whole, geom, scomp, ncomp);
 }
 
 extern "C" int vd_fill_patch(vd_ctx* ctx, const vd_array4* src, vd_array4* dst, const vd_box* box,
                              const vd_geom* geom, int scomp, int ncomp) {
     if (!ctx || !src || !dst || !box || !geom) return VD_EINVAL;
+    VD_TRY(vd::check_extents("fill_patch", box, {src, dst}));
     return vd::fill_patch(ctx, src, dst, box, geom, scomp, ncomp);
 }
 

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <initializer_list>
 #include <cstdio>
 #include <map>
 #include <string>
@@ -77,6 +78,52 @@ inline Arr<double> arr(const vd_array4* a) {
     r.ks = (int)a->kstride;
     r.ns = a->nstride;
     return r;
+}
+
+// Kernels index a box, and every array with its ghost cells, with 32-bit
+// offsets: the entry points reject boxes and arrays of 2^31 cells or more
+// (checked first, before the context is used).
+constexpr int64_t MAX_CELLS_32 = ((int64_t)1 << 31) - 1;
+
+inline int64_t span_cells(const int* begin, const int* end) {
+    int64_t c = 1;
+    for (int d = 0; d < 3; d++) c *= (int64_t)end[d] - begin[d] > 0 ? (int64_t)end[d] - begin[d] : 0;
+    return c;
+}
+
+inline int extent_error(const char* what, const char* kind, int64_t cells) {
+    set_error("%s: %s of %lld cells: at most 2^31 - 1 (32-bit cell offsets)", what, kind, (long long)cells);
+    return VD_EINVAL;
+}
+
+inline int check_box_extent(const char* what, const vd_box* box) {
+    if (!box) return VD_OK;
+    const int end[3] = {box->hi[0] + 1, box->hi[1] + 1, box->hi[2] + 1};
+    const int64_t c = span_cells(box->lo, end);
+    return c > MAX_CELLS_32 ? extent_error(what, "box", c) : VD_OK;
+}
+
+inline int check_array_extent(const char* what, const int* begin, const int* end, int64_t jstride,
+                              int64_t kstride) {
+    const int64_t c = span_cells(begin, end);
+    if (c > MAX_CELLS_32 || jstride > MAX_CELLS_32 || kstride > MAX_CELLS_32)
+        return extent_error(what, "array", c);
+    return VD_OK;
+}
+
+inline int check_extents(const char* what, const vd_box* box, std::initializer_list<const vd_array4*> arrs) {
+    VD_TRY(check_box_extent(what, box));
+    for (const vd_array4* a : arrs)
+        if (a) VD_TRY(check_array_extent(what, a->begin, a->end, a->jstride, a->kstride));
+    return VD_OK;
+}
+
+// the first `dim` face arrays of a set of three (unused entries of a 2-D
+// set are not read)
+inline int check_face_extents(const char* what, const vd_array4* f, int dim) {
+    if (!f) return VD_OK;
+    for (int d = 0; d < dim && d < 3; d++) VD_TRY(check_array_extent(what, f[d].begin, f[d].end, f[d].jstride, f[d].kstride));
+    return VD_OK;
 }
 
 // Device copy of the case plug-in descriptor (vd_mech_desc), reaction

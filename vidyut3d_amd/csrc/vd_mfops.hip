@@ -62,6 +62,7 @@ using namespace vd;
 
 extern "C" int vd_setval(vd_ctx* ctx, vd_array4* a, const vd_box* box, int scomp, int ncomp, double v) {
     if (!ctx || !a || !box || ncomp < 0 || scomp < 0 || scomp + ncomp > a->ncomp) return VD_EINVAL;
+    VD_TRY(check_extents("setval", box, {a}));
     if (!ncomp) return VD_OK;
     BoxIter it = box_iter(box, a);
     int64_t tot = (int64_t)it.nx * it.ny * it.nz;
@@ -75,6 +76,7 @@ extern "C" int vd_setval(vd_ctx* ctx, vd_array4* a, const vd_box* box, int scomp
 extern "C" int vd_copy(vd_ctx* ctx, const vd_array4* src, vd_array4* dst, const vd_box* box, int scomp,
                        int dcomp, int ncomp) {
     if (!ctx || !src || !dst || !box || ncomp < 0) return VD_EINVAL;
+    VD_TRY(check_extents("copy", box, {src, dst}));
     if (scomp + ncomp > src->ncomp || dcomp + ncomp > dst->ncomp) return VD_EINVAL;
     if (!ncomp) return VD_OK;
     BoxIter it = box_iter(box, dst);
@@ -89,6 +91,7 @@ extern "C" int vd_copy(vd_ctx* ctx, const vd_array4* src, vd_array4* dst, const 
 extern "C" int vd_lincomb(vd_ctx* ctx, vd_array4* dst, double a, const vd_array4* x, int xcomp, double b,
                           const vd_array4* y, int ycomp, int dcomp, int ncomp, const vd_box* box) {
     if (!ctx || !dst || !x || !y || !box || ncomp < 0) return VD_EINVAL;
+    VD_TRY(check_extents("lincomb", box, {dst, x, y}));
     if (!ncomp) return VD_OK;
     BoxIter it = box_iter(box, dst);
     int64_t tot = (int64_t)it.nx * it.ny * it.nz;
@@ -113,6 +116,7 @@ __global__ void k_clamp_min(Arr<double> a, BoxIter it, int comp, double lo) {
 // bounded background-species update floor (Source/Evolve.cpp:278-283)
 extern "C" int vd_clamp_min(vd_ctx* ctx, vd_array4* a, const vd_box* box, int comp, double lo) {
     if (!ctx || !a || !box || comp < 0 || comp >= a->ncomp) return VD_EINVAL;
+    VD_TRY(check_extents("clamp_min", box, {a}));
     BoxIter it = box_iter(box, a);
     int64_t tot = (int64_t)it.nx * it.ny * it.nz;
     if (tot >= ((int64_t)1 << 31)) return box_too_large(tot);

@@ -2051,6 +2051,13 @@ static int get_work(vd_ctx* ctx, const vd_geom* g, const vd_box* box, int prec, 
         n[d] = d < g->dim ? box->hi[d] - box->lo[d] + 1 : 1;
         per[d] = d < g->dim ? g->is_periodic[d] : 0;
     }
+    {
+        // the workspace arrays hold the level with a ghost layer (the cell
+        // coefficient), indexed with 32-bit offsets
+        int64_t grown = 1;
+        for (int d = 0; d < g->dim; d++) grown *= (int64_t)n[d] + 2;
+        if (grown > MAX_CELLS_32) return extent_error("multigrid workspace", "level with its ghost cells", grown);
+    }
     char key[160];
     snprintf(key, sizeof(key), "%d:%dx%dx%d:p%d%d%d:prec%d:cb%d:mc%d:dx%.17g,%.17g,%.17g", g->dim, n[0],
              n[1], n[2], per[0], per[1], per[2], prec, constb, maxc, g->dx[0], g->dx[1], g->dx[2]);
@@ -3662,6 +3669,8 @@ extern "C" int vd_solve_potential(vd_ctx* ctx, const vd_mech* m, const vd_array4
                                   const vd_geom* geom, const vd_params* prm, double time,
                                   vd_solve_info* info) {
     if (!ctx || !m || !sborder || !phi_new || !efield || !box || !geom || !prm) return VD_EINVAL;
+    VD_TRY(check_extents("solve_potential", box, {sborder, phi_new}));
+    VD_TRY(check_face_extents("solve_potential", efield, geom->dim));
     if (prm->mg_precision != 0 && prm->mg_precision != 1) return VD_EINVAL;
     return solve_potential(ctx, m, sborder, phi_new, efield, box, geom, prm, time, info);
 }
@@ -3675,6 +3684,8 @@ extern "C" int vd_implicit_solve_scalar(vd_ctx* ctx, const vd_mech* m, int spec,
     (void)time;
     if (!ctx || !m || !sborder || !phi_old || !dsdt_expl || !phi_new || !box || !geom || !prm)
         return VD_EINVAL;
+    VD_TRY(check_extents("implicit_solve_scalar", box, {sborder, phi_old, dsdt_expl, phi_new}));
+    VD_TRY(check_face_extents("implicit_solve_scalar", grad_fc, geom->dim));
     if (prm->mg_precision != 0 && prm->mg_precision != 1) return VD_EINVAL;
     if (!(dt > 0.0)) {
         set_error("implicit solve: dt must be positive");
@@ -3688,6 +3699,8 @@ extern "C" int vd_mlmg_solve(vd_ctx* ctx, const vd_mlmg_problem* prob, vd_array4
                              const vd_array4* rhs, const vd_box* box, const vd_geom* geom,
                              vd_solve_info* info) {
     if (!ctx || !prob || !phi || !rhs || !box || !geom) return VD_EINVAL;
+    VD_TRY(check_extents("mlmg_solve", box, {phi, rhs, prob->bcval, prob->robin_a, prob->robin_b, prob->robin_f}));
+    for (int d = 0; d < geom->dim && d < 3; d++) VD_TRY(check_extents("mlmg_solve", nullptr, {prob->bface[d]}));
     if (prob->precision != 0 && prob->precision != 1) return VD_EINVAL;
     return mlmg_solve(ctx, prob, phi, rhs, box, geom, info);
 }

@@ -240,6 +240,7 @@ extern "C" int vd_find_time_scales(vd_ctx* ctx, const vd_mech* m, const vd_array
                                    const vd_box* box, const vd_geom* geom, const vd_params* prm,
                                    double out[3]) {
     if (!ctx || !m || !phi_new || !box || !geom || !prm || !out) return VD_EINVAL;
+    VD_TRY(check_extents("find_time_scales", box, {phi_new}));
     return find_time_scales(ctx, m, phi_new, box, geom, prm, out);
 }
 
@@ -248,5 +249,7 @@ extern "C" int vd_error_est(vd_ctx* ctx, const vd_array4* state, vd_tagarray4* t
                             const double* refine_phigrad, char tagval) {
     if (!ctx || !state || !tags || !box || (ntagvars > 0 && (!comps || !refine_phi || !refine_phigrad)))
         return VD_EINVAL;
+    VD_TRY(check_extents("error_est", box, {state}));
+    VD_TRY(check_array_extent("error_est", tags->begin, tags->end, tags->jstride, tags->kstride));
     return error_est(ctx, state, tags, box, ntagvars, comps, refine_phi, refine_phigrad, tagval);
 }

@@ -755,6 +755,8 @@ extern "C" int vd_compute_scalar_transport_flux(vd_ctx* ctx, const vd_mech* m, c
                                                 double time, double dt, int specid) {
     (void)time;
     if (!ctx || !m || !sborder || !flux || !box || !geom || !prm) return VD_EINVAL;
+    VD_TRY(check_extents("compute_scalar_transport_flux", box, {sborder}));
+    VD_TRY(check_face_extents("compute_scalar_transport_flux", flux, geom->dim));
     return transport_flux(ctx, m, sborder, flux, box, geom, bc_lo, bc_hi, prm, dt, specid);
 }
 
@@ -762,6 +764,8 @@ extern "C" int vd_compute_dsdt(vd_ctx* ctx, const vd_array4 flux[3], const vd_ar
                                vd_array4* dsdt_out, const vd_box* box, const vd_geom* geom,
                                int specid) {
     if (!ctx || !flux || !rxn_src || !dsdt_out || !box || !geom) return VD_EINVAL;
+    VD_TRY(check_extents("compute_dsdt", box, {rxn_src, dsdt_out}));
+    VD_TRY(check_face_extents("compute_dsdt", flux, geom->dim));
     return dsdt(ctx, flux, rxn_src, dsdt_out, box, geom, specid);
 }
 
@@ -771,6 +775,7 @@ extern "C" int vd_transport_rhs(vd_ctx* ctx, const vd_mech* m, const vd_array4* 
                                 const vd_params* prm, double time, double dt, int specid) {
     (void)time;
     if (!ctx || !m || !sborder || !rxn_src || !dsdt_out || !box || !geom || !prm) return VD_EINVAL;
+    VD_TRY(check_extents("transport_rhs", box, {sborder, rxn_src, dsdt_out}));
     VD_TRY(check_transport_args(prm, specid, m->hdev.nspec));
     return transport_rhs(ctx, m, sborder, rxn_src, dsdt_out, box, geom, bc_lo, bc_hi, prm, dt, specid, nullptr,
                          nullptr);
