@@ -308,6 +308,18 @@ constexpr int FZ_IX = 64, FZ_IY = 16, FZ_H = 4;
 constexpr int FZ_PX = (FZ_IX + 2 * FZ_H) / 2;  // 36 thread columns, 2 cells each
 constexpr int FZ_RY = FZ_IY + 2 * FZ_H;        // 24 thread rows
 constexpr int FZ_THREADS = FZ_PX * FZ_RY;      // 864
+// the tile by coefficient mode: the cell coefficient (BM 2) keeps three
+// more plane rings of face values in registers and takes a 32 x 16 tile
+// with a 3-row y halo, enough for its one sweep and residual (440
+// threads: two waves per SIMD, so up to 256 VGPRs each)
+template <int BM>
+struct FzTile {
+    static constexpr int IX = BM == 2 ? 32 : FZ_IX;
+    static constexpr int HY = BM == 2 ? 3 : FZ_H;
+    static constexpr int PX = (IX + 2 * FZ_H) / 2;
+    static constexpr int RY = FZ_IY + 2 * HY;
+    static constexpr int THREADS = PX * RY;
+};
 
 // raw buffer resource over `bytes` of memory (gfx9 family config word);
 // accesses at offsets >= bytes return 0 (loads) or are dropped (stores)
@@ -351,7 +363,7 @@ __device__ __forceinline__ void fused_unroll(F& f, int t, int tend) {
 // ACC: instead of storing the smoothed x, add it to the FP64 array xacc
 // (the correction of the last post-smooth of a V-cycle, x += e)
 template <class T, int BM, int S, int RES, int ZI, int ACC = 0>
-__global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* __restrict__ xin,
+__global__ void __launch_bounds__(FzTile<BM>::THREADS) k_gsrb_fused(LevV<T> L, const T* __restrict__ xin,
                                                            T* __restrict__ xout, const T* __restrict__ rhs,
                                                            int zc, int ntx, int nty,
                                                            double* __restrict__ part,
@@ -363,21 +375,34 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
     // values are kept by colour (red = (i+j+k) even) so that the colour a
     // stage updates is a compile-time constant too.
     constexpr int NV = ((S + 2 + RES) + 1) & ~1;
-    __shared__ T nbuf[S][FZ_RY][FZ_PX];
+    constexpr int TZ_IX = FzTile<BM>::IX, TZ_PX = FzTile<BM>::PX, TZ_THREADS = FzTile<BM>::THREADS;
+    constexpr int TZ_RY = FzTile<BM>::RY, TZ_HY = FzTile<BM>::HY;
+    __shared__ T nbuf[S][TZ_RY][TZ_PX];
     // BM 1: reciprocal diagonal of each cell of the thread's pair by the
     // z class of its plane (bottom wall, interior, top wall), kept in LDS
     // rather than registers
-    __shared__ T rdbuf[BM == 1 ? 3 : 1][2][FZ_THREADS];
-    __shared__ T fbuf[RES ? 2 : 1][FZ_RY][FZ_PX];
-    __shared__ double red[(FZ_THREADS + 63) / 64];
+    __shared__ T rdbuf[BM == 1 ? 3 : 1][2][BM == 1 ? TZ_THREADS : 1];
+    // BM 2 (cell coefficient): the harmonic faces of a plane are formed once,
+    // as the plane arrives, instead of six per stencil evaluation.  The low-x
+    // face of cell 0 and the low-y faces of both cells go to LDS, where the
+    // pair on the left and the row below read them as their high faces; the
+    // face between the pair's cells and the low-z faces stay in registers.
+    // Four planes are in use at a step (the arriving one and the three its
+    // stages and residual read), ring-indexed by p & 3.
+    static_assert(BM != 2 || (S == 2 && RES == 1 && !ZI && !ACC), "BM 2: one sweep and its residual");
+    constexpr int FR = BM == 2 ? 4 : 1, FRY = BM == 2 ? TZ_RY : 1, FPX = BM == 2 ? TZ_PX : 1;
+    __shared__ T fxs[FR][FRY][FPX];
+    __shared__ T fys[FR][2][FRY][FPX];
+    __shared__ T fbuf[RES ? 2 : 1][TZ_RY][TZ_PX];
+    __shared__ double red[(TZ_THREADS + 63) / 64];
 
     const unsigned lb = lblock();
     const int ntile = ntx * nty;
     const int tile = (int)(lb % (unsigned)ntile), zb = (int)(lb / (unsigned)ntile);
-    const int x0 = (tile % ntx) * FZ_IX, y0 = (tile / ntx) * FZ_IY, z0 = zb * zc;
-    const int tx = (int)threadIdx.x % FZ_PX, ty = (int)threadIdx.x / FZ_PX;
+    const int x0 = (tile % ntx) * TZ_IX, y0 = (tile / ntx) * FZ_IY, z0 = zb * zc;
+    const int tx = (int)threadIdx.x % TZ_PX, ty = (int)threadIdx.x / TZ_PX;
     // cells ia, ia+1 of row ja; global (periodically wrapped) indices
-    const int ia = x0 - FZ_H + 2 * tx, ja = y0 - FZ_H + ty;
+    const int ia = x0 - FZ_H + 2 * tx, ja = y0 - TZ_HY + ty;
     int ic0 = ia, ic1 = ia + 1, jc = ja;
     bool ok0, ok1, okj;
     if (L.per0) {
@@ -398,7 +423,7 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
         okj = jc >= 0 && jc < L.ny;
     }
     // interior cells of this tile that lie in the level (written / normed)
-    const bool orow = tx >= FZ_H / 2 && tx < FZ_PX - FZ_H / 2 && ty >= FZ_H && ty < FZ_RY - FZ_H &&
+    const bool orow = tx >= FZ_H / 2 && tx < TZ_PX - FZ_H / 2 && ty >= TZ_HY && ty < TZ_RY - TZ_HY &&
                       ja < L.ny;
     const bool oc0 = orow && ia < L.nx, oc1 = orow && ia + 1 < L.nx;
 
@@ -427,6 +452,63 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
     T vr[NV], vb[NV], fr[NV], fb[NV];
 #pragma unroll
     for (int q = 0; q < NV; q++) vr[q] = vb[q] = fr[q] = fb[q] = T(0);
+    // BM 2: the cell coefficient lives on the level grown by one cell.  A
+    // lane reads it at its two cells, the cell left of cell 0 and the two
+    // cells below; columns, rows and planes are clamped to the grown box
+    // (wrapped on a periodic side, where the grown box's ghost cells hold
+    // the periodic images), so the faces formed for the lanes just outside
+    // a wall are the wall faces the cells inside it take as high faces.
+    // By plane slot: fx1 the face between the pair's cells, fzr / fzb the
+    // low-z faces of the red / black cell (the high-z face of a cell is the
+    // low-z face of the opposite colour on the plane above), and ofb / dgb
+    // the off-diagonal sum and diagonal of the black update, which are those
+    // of the black cell's residual bit for bit (same neighbour values, same
+    // arithmetic) and are not formed twice.
+    T fx1[BM == 2 ? NV : 1], fzr[BM == 2 ? NV : 1], fzb[BM == 2 ? NV : 1];
+    T ofb[BM == 2 ? NV : 1], dgb[BM == 2 ? NV : 1];
+    int gA = 0, gB = 0, gL = 0, gAb = 0, gBb = 0;
+    const int ggx = L.nx + 2, ggxy = ggx * (L.ny + 2);
+    const __amdgpu_buffer_rsrc_t rdc =
+        buf_rsrc(L.dcell, BM == 2 ? (unsigned)((int64_t)ggxy * (L.nz + 2) * (int64_t)sizeof(T)) : 0u);
+    if constexpr (BM == 2) {
+#pragma unroll
+        for (int q = 0; q < NV; q++) fx1[q] = fzr[q] = fzb[q] = ofb[q] = dgb[q] = T(0);
+        int c0, c1, cl, r0, rb;
+        if (L.per0) {
+            c0 = ic0;
+            c1 = ic1;
+            cl = ic0 - 1;
+        } else {
+            c0 = min(max(ia, -1), L.nx);
+            c1 = min(max(ia + 1, -1), L.nx);
+            cl = min(max(ia - 1, -1), L.nx);
+        }
+        if (L.per1) {
+            r0 = jc;
+            rb = jc - 1;
+        } else {
+            r0 = min(max(ja, -1), L.ny);
+            rb = min(max(ja - 1, -1), L.ny);
+        }
+        gA = (c0 + 1) + ggx * (r0 + 1);
+        gB = (c1 + 1) + ggx * (r0 + 1);
+        gL = (cl + 1) + ggx * (r0 + 1);
+        gAb = (c0 + 1) + ggx * (rb + 1);
+        gBb = (c1 + 1) + ggx * (rb + 1);
+    }
+    struct PD {
+        T a, b, l, ab, bb;  // cell 0, cell 1, left of cell 0, below cell 0, below cell 1
+    };
+    auto load_d = [&](PD& d, int p) {
+        const unsigned o = (unsigned)(ggxy * (min(max(p, -1), L.nz) + 1));
+        d.a = buf_ld<T>(rdc, (o + (unsigned)gA) * (unsigned)sizeof(T));
+        d.b = buf_ld<T>(rdc, (o + (unsigned)gB) * (unsigned)sizeof(T));
+        d.l = buf_ld<T>(rdc, (o + (unsigned)gL) * (unsigned)sizeof(T));
+        d.ab = buf_ld<T>(rdc, (o + (unsigned)gAb) * (unsigned)sizeof(T));
+        d.bb = buf_ld<T>(rdc, (o + (unsigned)gBb) * (unsigned)sizeof(T));
+    };
+    PD pd{};
+    T dpa = T(0), dpb = T(0);  // the coefficient of the thread's cells on the previous plane
     // planes are prefetched two steps ahead into two alternating buffers
     // (no register moves of in-flight loads), issued at the end of a step
     // after that step's own loads
@@ -446,7 +528,7 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
         }
     };
     double amax = 0.0;
-    const int txm = tx > 0 ? tx - 1 : tx, txp = tx < FZ_PX - 1 ? tx + 1 : tx;
+    const int txm = tx > 0 ? tx - 1 : tx, txp = tx < TZ_PX - 1 ? tx + 1 : tx;
     // BM 1 (constant coefficient b, uniform c0): stencil_nb's arithmetic with
     // the boundary selects hoisted.  A neighbour across a wall of the level
     // holds 0 here (lanes outside the level load 0 and are never written,
@@ -501,6 +583,45 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
         of = of + L.s1 * o1;
         of = of + L.s2 * o2;
     };
+    // BM 2 (cell coefficient, uniform c0): stencil_nbf's arithmetic term for
+    // term without its branches.  A face enters the diagonal times 1 inside
+    // the level and times (1 - c0) on a wall (times 1 is exact, so the
+    // factor stands for stencil_nbf's two cases); the factors are per lane
+    // (x, y) or per plane (z, never periodic here).  A neighbour across a
+    // wall holds 0, so its product with the (finite) wall face adds nothing
+    // and every neighbour term is taken.  f[6]: low / high faces of x, y, z.
+    T mlx0 = T(1), mhx0 = T(1), mlx1 = T(1), mhx1 = T(1), mly = T(1), mhy = T(1);
+    if constexpr (BM == 2) {
+        const T w[2][2] = {{T(1) - L.c0u[0][0], T(1) - L.c0u[0][1]}, {T(1) - L.c0u[1][0], T(1) - L.c0u[1][1]}};
+        mlx0 = ic0c > 0 || L.per0 ? T(1) : w[0][0];
+        mhx0 = ic0c < L.nx - 1 || L.per0 ? T(1) : w[0][1];
+        mlx1 = ic1c > 0 || L.per0 ? T(1) : w[0][0];
+        mhx1 = ic1c < L.nx - 1 || L.per0 ? T(1) : w[0][1];
+        mly = jcc > 0 || L.per1 ? T(1) : w[1][0];
+        mhy = jcc < L.ny - 1 || L.per1 ? T(1) : w[1][1];
+    }
+    auto st2 = [&](int u, int pc, T xl, T xh, T yl, T yh, T zl, T zh, const T* f, T& dg, T& of) {
+        const T mlx = u ? mlx1 : mlx0, mhx = u ? mhx1 : mhx0;
+        const T mlz = pc > 0 ? T(1) : T(1) - L.c0u[2][0], mhz = pc < L.nz - 1 ? T(1) : T(1) - L.c0u[2][1];
+        T ddx = f[0] * mlx;
+        ddx = ddx + f[1] * mhx;
+        T ddy = f[2] * mly;
+        ddy = ddy + f[3] * mhy;
+        T ddz = f[4] * mlz;
+        ddz = ddz + f[5] * mhz;
+        dg = L.alpha + L.s0 * ddx;
+        dg = dg + L.s1 * ddy;
+        dg = dg + L.s2 * ddz;
+        T o0 = f[0] * xl;
+        o0 = o0 + f[1] * xh;
+        T o1 = f[2] * yl;
+        o1 = o1 + f[3] * yh;
+        T o2 = f[4] * zl;
+        o2 = o2 + f[5] * zh;
+        of = L.s0 * o0;
+        of = of + L.s1 * o1;
+        of = of + L.s2 * o2;
+    };
     // ZI, BM 1: the first stage updates red cells from a zero correction,
     // so every neighbour term is b * 0 and its stencil sum is a constant
     // (formed once, in st1's order, for the same bits)
@@ -509,7 +630,7 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
         T dg0;
         st1(0, 1, T(0), T(0), T(0), T(0), T(0), T(0), dg0, of0);
     }
-    const int tym = ty > 0 ? ty - 1 : ty, typ = ty < FZ_RY - 1 ? ty + 1 : ty;
+    const int tym = ty > 0 ? ty - 1 : ty, typ = ty < TZ_RY - 1 ? ty + 1 : ty;
     const int tend = zout + S;
     PF qa, qb;
     auto step = [&](auto kc, int t) {
@@ -528,6 +649,24 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
             vb[k] = c0red ? q.x1 : q.x0;
             fr[k] = c0red ? q.f0 : q.f1;
             fb[k] = c0red ? q.f1 : q.f0;
+        }
+        if constexpr (BM == 2) {
+            // faces of the arriving plane t+1; its LDS slot last held plane
+            // t-3, whose last reader (the residual) ran before the previous
+            // step's barrier
+            fx1[k] = harm(pd.a, pd.b);
+            {
+                const T za = harm(dpa, pd.a), zb = harm(dpb, pd.b);
+                const bool c0red = ((par + t + 1) & 1) == 0;
+                fzr[k] = c0red ? za : zb;
+                fzb[k] = c0red ? zb : za;
+            }
+            dpa = pd.a;
+            dpb = pd.b;
+            const int sl = (t + 1) & 3;
+            fxs[sl][ty][tx] = harm(pd.l, pd.a);
+            fys[sl][0][ty][tx] = harm(pd.ab, pd.a);
+            fys[sl][1][ty][tx] = harm(pd.bb, pd.b);
         }
         // publish the opposite-colour value of each stage's plane
 #pragma unroll
@@ -583,8 +722,24 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
                     vr[q_s] = valid ? nv : vr[q_s];
                 }
             } else {
-                stencil_nb<T, BM, 3, decltype(nb), true>(L, u ? ic1c : ic0c, jcc, min(max(p, 0), L.nz - 1), nb,
-                                                          dg, of);
+                if constexpr (BM == 2) {
+                    // the faces of plane p: x and y from LDS (own / right pair,
+                    // own / upper row), the pair's middle face and z from the ring
+                    const int sl = p & 3;
+                    const T fxo = fxs[sl][ty][u ? txp : tx];
+                    const T fyl = fys[sl][u][ty][tx], fyh = fys[sl][u][typ][tx];
+                    const T fxl = u ? fx1[q_s] : fxo, fxh = u ? fxo : fx1[q_s];
+                    const T fzl = cs ? fzb[q_s] : fzr[q_s], fzh = cs ? fzr[q_hi] : fzb[q_hi];
+                    const T fc[6] = {fxl, fxh, fyl, fyh, fzl, fzh};
+                    st2(u, min(max(p, 0), L.nz - 1), xm, xp, ym, yp, zlo_v, zhi_v, fc, dg, of);
+                    if (cs) {
+                        ofb[q_s] = of;
+                        dgb[q_s] = dg;
+                    }
+                } else {
+                    stencil_nb<T, BM, 3, decltype(nb), true>(L, u ? ic1c : ic0c, jcc, min(max(p, 0), L.nz - 1),
+                                                              nb, dg, of);
+                }
                 if (cs) {
                     const T nv = (fb[q_s] + of) / dg;
                     vb[q_s] = valid ? nv : vb[q_s];
@@ -617,10 +772,23 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
                     return side == 0 ? zlo_v : zhi_v;
                 };
                 T dg, of;
-                if constexpr (BM == 1)
+                if constexpr (BM == 1) {
                     st1(u, poc, xm, xp, ym, yp, zlo_v, zhi_v, dg, of);
-                else
+                } else if constexpr (BM == 2) {
+                    if (c) {
+                        of = ofb[qo];
+                        dg = dgb[qo];
+                    } else {
+                        const int sl = po & 3;
+                        const T fxo = fxs[sl][ty][u ? txp : tx];
+                        const T fyl = fys[sl][u][ty][tx], fyh = fys[sl][u][typ][tx];
+                        const T fxl = u ? fx1[qo] : fxo, fxh = u ? fxo : fx1[qo];
+                        const T fc[6] = {fxl, fxh, fyl, fyh, fzr[qo], fzb[qhi]};
+                        st2(u, poc, xm, xp, ym, yp, zlo_v, zhi_v, fc, dg, of);
+                    }
+                } else {
                     stencil_nb<T, BM, 3, decltype(nb), true>(L, u ? ic1c : ic0c, jcc, poc, nb, dg, of);
+                }
                 const double r = (c ? fb[qo] : fr[qo]) - (dg * own - of);
                 double a = fabs(r);
                 if (a != a) a = __builtin_huge_val();
@@ -639,10 +807,18 @@ __global__ void __launch_bounds__(FZ_THREADS) k_gsrb_fused(LevV<T> L, const T* _
             }
         }
         load(q, t + 3);
+        if constexpr (BM == 2) load_d(pd, t + 2);  // one step ahead
         __syncthreads();
     };
     load(qa, zlo);
     load(qb, zlo + 1);
+    if constexpr (BM == 2) {
+        PD d0;
+        load_d(d0, zlo - 1);
+        dpa = d0.a;
+        dpb = d0.b;
+        load_d(pd, zlo);
+    }
     for (int t = zlo - 1; t < tend; t += NV) fused_unroll<0, NV>(step, t, tend);
     if constexpr (RES) {
         for (int o = 32; o > 0; o >>= 1) amax = fmax(amax, __shfl_xor(amax, o, 64));
@@ -1889,26 +2065,38 @@ static bool fused_capable(const MGWork& W) {
            W.n[1] >= 8 && W.n[2] >= 2 && ncell(W.n) * 8 < ((int64_t)1 << 31) && fused_zc(W.n) > 0;
 }
 
+// the smoother-only solves on the cell coefficient (bm 2) run one fused
+// sweep-plus-residual launch per iteration when the coefficient's grown box
+// also has 32-bit byte offsets; VD_FUSED_CELLCOEF=0 keeps them on the
+// per-colour launches and the streamed residual
+static bool fused_cellcoef(const MGWork& W) {
+    const char* e = getenv("VD_FUSED_CELLCOEF");
+    if (e && strcmp(e, "0") == 0) return false;
+    const int64_t ng = (int64_t)(W.n[0] + 2) * (W.n[1] + 2) * (W.n[2] + 2);
+    return W.dim == 3 && ng * 8 < ((int64_t)1 << 31);
+}
+
 template <class T, int BM, int S, int RES>
 static void launch_fused_k(vd_ctx* ctx, const LevV<T>& v, int zc, const T* xin, T* xout, const T* rhs,
                            double* part, int* nblk, double* xacc = nullptr) {
     // the kernel reads xin unless it starts from zero (ZI)
-    const int ntx = (v.nx + FZ_IX - 1) / FZ_IX, nty = (v.ny + FZ_IY - 1) / FZ_IY;
+    constexpr int TIX = FzTile<BM>::IX, TT = FzTile<BM>::THREADS;
+    const int ntx = (v.nx + TIX - 1) / TIX, nty = (v.ny + FZ_IY - 1) / FZ_IY;
     const int nzc = (v.nz + zc - 1) / zc;
     const int nb = ntx * nty * nzc;
     if (nblk) *nblk = nb;
     if constexpr (RES == 0) {
         if (xin && xacc) {
-            hipLaunchKernelGGL((k_gsrb_fused<T, BM, S, RES, 0, 1>), dim3((unsigned)nb), dim3(FZ_THREADS), 0,
+            hipLaunchKernelGGL((k_gsrb_fused<T, BM, S, RES, 0, 1>), dim3((unsigned)nb), dim3(TT), 0,
                                ctx->stream, v, xin, xout, rhs, zc, ntx, nty, part, xacc);
             return;
         }
     }
     if (xin)
-        hipLaunchKernelGGL((k_gsrb_fused<T, BM, S, RES, 0>), dim3((unsigned)nb), dim3(FZ_THREADS), 0, ctx->stream,
+        hipLaunchKernelGGL((k_gsrb_fused<T, BM, S, RES, 0>), dim3((unsigned)nb), dim3(TT), 0, ctx->stream,
                            v, xin, xout, rhs, zc, ntx, nty, part, (double*)nullptr);
     else if constexpr (RES == 0)
-        hipLaunchKernelGGL((k_gsrb_fused<T, BM, S, RES, 1>), dim3((unsigned)nb), dim3(FZ_THREADS), 0, ctx->stream,
+        hipLaunchKernelGGL((k_gsrb_fused<T, BM, S, RES, 1>), dim3((unsigned)nb), dim3(TT), 0, ctx->stream,
                            v, xin, xout, rhs, zc, ntx, nty, part, (double*)nullptr);
 }
 
@@ -1931,6 +2119,10 @@ static void launch_fused(vd_ctx* ctx, const MGWork& W, int bm, const LevV<T>& v,
     } else launch_fused_k<T, BMV, 2, 0>(ctx, v, zc, xin, xout, rhs, part, nblk, xacc);
     if (bm == 1) {
         VD_FUSED_BM(1)
+    } else if (bm == 2) {
+        // the cell coefficient: one FP64 sweep and its residual norm only
+        if constexpr (std::is_same<T, double>::value)
+            if (nsweep == 1 && res && xin && !xacc) launch_fused_k<T, 2, 2, 1>(ctx, v, zc, xin, xout, rhs, part, nblk);
     } else {
         VD_FUSED_BM(0)
     }
@@ -2113,7 +2305,8 @@ static int get_work(vd_ctx* ctx, const vd_geom* g, const vd_box* box, int prec, 
     {
         // per-block partial norms of k_resid0 and of the fused smoother
         int64_t np = ((n[0] + 255) / 256) * (int64_t)n[1] * n[2];
-        int64_t nf = (int64_t)((n[0] + FZ_IX - 1) / FZ_IX) * ((n[1] + FZ_IY - 1) / FZ_IY) * n[2];
+        constexpr int TIX = FzTile<2>::IX;  // the narrowest tile
+        int64_t nf = (int64_t)((n[0] + TIX - 1) / TIX) * ((n[1] + FZ_IY - 1) / FZ_IY) * n[2];
         if ((st = dalloc(W, std::max(np, nf) * 8, &p))) goto fail;
     }
     W->part = (double*)p;
@@ -2794,7 +2987,7 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
     int it = 0, st = VD_OK;
     if (!(r0 <= target)) {
         for (it = 1; it <= maxiter; it++) {
-            if (smooth_only && W.x2 && W.c0uni && bm != 2) {
+            if (smooth_only && W.x2 && W.c0uni && (bm != 2 || fused_cellcoef(W))) {
                 // one red-black sweep and the residual norm of its result in one pass
                 int nblk = 0;
                 launch_fused<double>(ctx, W, bm, v0, std::max(1, fused_zc(W.n)), W.x, W.x2, W.f, 1, 1, W.part,
