@@ -303,6 +303,43 @@ int vd_ctx_sync(vd_ctx* ctx);
 int vd_ctx_release_workspaces(vd_ctx* ctx);
 void* vd_ctx_stream(vd_ctx* ctx); /* the hipStream_t, as void* */
 
+/* ---------------------------------------------------- batched contexts
+ * Same-shape cases stepped together, one launch per kernel for all of
+ * them (the serving loop's batching; no counterpart in the reference,
+ * which runs one case per executable).  A batched context holds ncase
+ * identical regions of case_bytes device memory; every allocation the
+ * context makes (vd_malloc, the solver workspaces, scratch, a mechanism's
+ * device copy) is carved out of case 0's region, and case c's copy of it
+ * sits case_bytes * c further on.  Every entry point of the step path
+ * (transport, sources, Joule heating, the potential and implicit solves,
+ * time scales, ghost fill, the bulk MultiFab operations) then runs on the
+ * active cases at once: launches take one z-block per case, and the
+ * solvers keep each case's convergence test, so every case takes exactly
+ * the operations, and gets exactly the bits, it would alone.  Arrays
+ * passed to those entry points must lie in case 0's region; the other
+ * entry points (AMR, several boxes, tags, the no-allocation and generic
+ * solver calls) reject a batched context with more than one active case.
+ * A region that runs out gives VD_ENOMEM (vd_ctx_arena_used reports the
+ * bytes one case has taken). */
+#define VD_MAX_BATCH 16
+struct vd_solve_info;
+int vd_ctx_create_batched(int device, int ncase, size_t case_bytes, vd_ctx** out);
+int vd_ctx_ncase(const vd_ctx* ctx);
+/* byte distance between two cases' copies of an allocation */
+long long vd_ctx_case_stride(const vd_ctx* ctx);
+int vd_ctx_arena_used(const vd_ctx* ctx, size_t* used);
+/* the cases (distinct, ascending or not, < ncase) the following calls run
+ * on; n = 0 is refused */
+int vd_ctx_set_active(vd_ctx* ctx, int n, const int* cases);
+/* per-case applied voltage amplitudes (vidyut.voltage_amp_1/2) used by
+ * the potential solve in place of vd_params' (until cleared with
+ * use = 0) */
+int vd_ctx_set_case_voltage(vd_ctx* ctx, int c, int use, double amp_1, double amp_2);
+/* the iterations and norms of case c's last potential or implicit solve */
+int vd_ctx_case_solve_info(const vd_ctx* ctx, int c, struct vd_solve_info* info);
+/* case c's time scales from the last vd_find_time_scales */
+int vd_ctx_case_time_scales(const vd_ctx* ctx, int c, double out[3]);
+
 int vd_malloc(vd_ctx* ctx, size_t bytes, void** out);
 int vd_free(vd_ctx* ctx, void* p);
 int vd_memcpy_h2d(vd_ctx* ctx, void* dst, const void* src, size_t bytes);

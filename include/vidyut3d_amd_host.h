@@ -22,7 +22,14 @@ int vd_sim_destroy(vd_sim* s);
 /* whole state (NVAR comps, valid box, component-major, i fastest) */
 int vd_sim_set_state(vd_sim* s, const double* host, double time);
 int vd_sim_get_state(vd_sim* s, double* host);
-/* device view of phi_new (no ghost cells) */
+/* A simulation on a batched context (vd_ctx_create_batched) holds one
+ * case per region, all of one shape and vd_params (per-case voltages via
+ * vd_ctx_set_case_voltage; adaptive_dt is refused), and vd_sim_evolve
+ * steps the active cases (vd_ctx_set_active) together; the cases of the
+ * serving loop's same-shape batches.  Case c's whole state: */
+int vd_sim_set_state_case(vd_sim* s, int c, const double* host);
+int vd_sim_get_state_case(vd_sim* s, int c, double* host);
+/* device view of phi_new (no ghost cells; case 0's in a batched context) */
 int vd_sim_state_view(vd_sim* s, vd_array4* out);
 /* Vidyut::Evolve for nsteps with ncorr correctors.  dt is vidyut.dt: the
  * step of every step, or with prm->adaptive_dt the step before the

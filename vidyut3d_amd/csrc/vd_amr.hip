@@ -223,6 +223,7 @@ extern "C" int vd_average_down(vd_ctx* ctx, const vd_array4* fine, const vd_box*
     VD_TRY(check_extents("average_down", fbox, {fine, crse}));
     VD_TRY(check_ratio(ratio, dim));
     VD_TRY(check_aligned(fbox, ratio, dim));
+    VD_TRY(require_plain(ctx, "average_down"));
     if (ncomp <= 0) return VD_OK;
     if (scomp < 0 || scomp + ncomp > fine->ncomp || scomp + ncomp > crse->ncomp) {
         set_error("vd_average_down: components [%d, %d) beyond the arrays", scomp, scomp + ncomp);
@@ -255,6 +256,7 @@ extern "C" int vd_average_down_faces(vd_ctx* ctx, const vd_array4 fine[3], const
     VD_TRY(check_face_extents("average_down_faces", crse, dim));
     VD_TRY(check_ratio(ratio, dim));
     VD_TRY(check_aligned(fbox, ratio, dim));
+    VD_TRY(require_plain(ctx, "average_down_faces"));
     const int r[3] = {ratio[0], ratio[1], dim == 3 ? ratio[2] : 1};
     for (int d = 0; d < dim; d++) {
         int flo[3], fhi[3], clo[3], chi[3];
@@ -286,6 +288,7 @@ extern "C" int vd_fill_patch_two_levels(vd_ctx* ctx, const vd_array4* crse, cons
         return VD_EINVAL;
     VD_TRY(check_extents("fill_patch_two_levels", nullptr, {crse, dst}));
     for (int b = 0; b < nfine; b++) VD_TRY(check_extents("fill_patch_two_levels", &fboxes[b], {&fine[b]}));
+    VD_TRY(require_plain(ctx, "fill_patch_two_levels"));
     const int dim = fg->dim;
     if (cg->dim != dim) {
         set_error("vd_fill_patch_two_levels: coarse and fine geometries differ in dim");

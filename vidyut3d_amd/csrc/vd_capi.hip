@@ -63,17 +63,62 @@ int round_filling_chunks(int64_t per_chunk, int cap, int nmin, int nmax) {
     return best;
 }
 
+int dev_alloc(vd_ctx* ctx, size_t bytes, void** out) {
+    if (ctx->arena) {
+        // 256-byte pieces of case 0's region; every case's copy of a piece
+        // has the same alignment (regions are whole multiples of 4 KiB)
+        const size_t sz = ((bytes ? bytes : 8) + 255) & ~(size_t)255;
+        if (ctx->arena_used + sz > ctx->arena_case) {
+            set_error("batched context: %zu more bytes per case needed, %zu of %zu used", sz, ctx->arena_used,
+                      ctx->arena_case);
+            return VD_ENOMEM;
+        }
+        *out = ctx->arena + ctx->arena_used;
+        ctx->arena_used += sz;
+        return VD_OK;
+    }
+    hipError_t e = hipMalloc(out, bytes ? bytes : 8);
+    if (e != hipSuccess) {
+        set_error("hipMalloc %zu: %s", bytes, hipGetErrorString(e));
+        *out = nullptr;
+        return VD_ENOMEM;
+    }
+    return VD_OK;
+}
+
+void dev_free(vd_ctx* ctx, void* p) {
+    if (p && !ctx->arena) (void)hipFree(p);
+}
+
+BatchScope::BatchScope(vd_ctx* ctx, const int* cases, int n) : c(ctx), bt(ctx->bt), bn(ctx->bn) {
+    for (int i = 0; i < n; i++) ctx->bt.idx[i] = (unsigned char)cases[i];
+    ctx->bn = n;
+}
+BatchScope::~BatchScope() {
+    c->bt = bt;
+    c->bn = bn;
+}
+
+int require_plain(vd_ctx* ctx, const char* what) {
+    if (ctx->arena) {
+        set_error("%s: not available in a batched context", what);
+        return VD_EINVAL;
+    }
+    return VD_OK;
+}
+
 int scratch(vd_ctx* ctx, const std::string& name, size_t bytes, void** out) {
     Buf& b = ctx->scratch[name];
     if (b.bytes < bytes) {
-        if (b.p) VD_HIP(hipFree(b.p));
+        if (b.p) {
+            VD_HIP(hipStreamSynchronize(ctx->stream));
+            dev_free(ctx, b.p);
+        }
         b.p = nullptr;
         b.bytes = 0;
         size_t sz = bytes < 256 ? 256 : bytes;
-        hipError_t e = hipMalloc(&b.p, sz);
-        if (e != hipSuccess) {
-            set_error("scratch '%s' %zu bytes: %s", name.c_str(), sz, hipGetErrorString(e));
-            b.p = nullptr;
+        if (dev_alloc(ctx, sz, &b.p) != VD_OK) {
+            b.p = nullptr;  // dev_alloc's message names the size
             return VD_ENOMEM;
         }
         b.bytes = sz;
@@ -175,6 +220,7 @@ int vd_compute_dt(const vd_params* prm, double cur_time, double old_dt, const do
 
 int vd_ctx_create(int device, vd_ctx** out) {
     if (!out) return VD_EINVAL;
+    *out = nullptr;
     int n = 0;
     VD_HIP(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) {
@@ -192,7 +238,8 @@ int vd_ctx_create(int device, vd_ctx** out) {
     }
     e = hipHostMalloc((void**)&c->pinned, 8192 * sizeof(double), hipHostMallocDefault);
     if (e == hipSuccess)
-        e = hipHostMalloc((void**)&c->hnorm, 64 * sizeof(double), hipHostMallocMapped | hipHostMallocCoherent);
+        e = hipHostMalloc((void**)&c->hnorm, HNORM_DOUBLES * sizeof(double),
+                          hipHostMallocMapped | hipHostMallocCoherent);
     if (e == hipSuccess) e = hipHostGetDevicePointer((void**)&c->hnorm_d, c->hnorm, 0);
     if (e != hipSuccess) {
         if (c->hnorm) (void)hipHostFree(c->hnorm);
@@ -202,7 +249,9 @@ int vd_ctx_create(int device, vd_ctx** out) {
         set_error("hipHostMalloc: %s", hipGetErrorString(e));
         return VD_EHIP;
     }
-    memset(c->hnorm, 0, 64 * sizeof(double));
+    memset(c->hnorm, 0, HNORM_DOUBLES * sizeof(double));
+    c->bt.stride = 0;
+    memset(c->bt.idx, 0, sizeof(c->bt.idx));
     // the device address of the no-allocation scratch on this device (once)
     const int st = noalloc_setup(device);
     if (st != VD_OK) {
@@ -220,10 +269,10 @@ int vd_ctx_destroy(vd_ctx* ctx) {
     if (!ctx) return VD_EINVAL;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (auto& kv : ctx->scratch)
-        if (kv.second.p) (void)hipFree(kv.second.p);
+    for (auto& kv : ctx->scratch) dev_free(ctx, kv.second.p);
     for (auto& kv : ctx->mg) mgwork_free(kv.second);
     for (auto e : ctx->kt_pool) (void)hipEventDestroy(e);
+    if (ctx->arena) (void)hipFree(ctx->arena);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     if (ctx->hnorm) (void)hipHostFree(ctx->hnorm);
     (void)hipStreamDestroy(ctx->stream);
@@ -240,6 +289,9 @@ int vd_ctx_sync(vd_ctx* ctx) {
 int vd_ctx_release_workspaces(vd_ctx* ctx) {
     if (!ctx) return VD_EINVAL;
     VD_HIP(hipStreamSynchronize(ctx->stream));
+    // a batched context's regions hold the cases' state as well: its
+    // workspaces live until the context is destroyed
+    if (ctx->arena) return VD_OK;
     for (auto& kv : ctx->scratch)
         if (kv.second.p) VD_HIP(hipFree(kv.second.p));
     ctx->scratch.clear();
@@ -251,20 +303,96 @@ int vd_ctx_release_workspaces(vd_ctx* ctx) {
 void* vd_ctx_stream(vd_ctx* ctx) { return (void*)ctx->stream; }
 
 int vd_malloc(vd_ctx* ctx, size_t bytes, void** out) {
-    if (!ctx) return VD_EINVAL;
+    if (!ctx || !out) return VD_EINVAL;
     VD_HIP(hipSetDevice(ctx->device));
-    hipError_t e = hipMalloc(out, bytes ? bytes : 8);
-    if (e != hipSuccess) {
-        set_error("hipMalloc %zu: %s", bytes, hipGetErrorString(e));
-        return VD_ENOMEM;
-    }
-    return VD_OK;
+    return dev_alloc(ctx, bytes, out);
 }
 
 int vd_free(vd_ctx* ctx, void* p) {
     if (!ctx) return VD_EINVAL;
     VD_HIP(hipStreamSynchronize(ctx->stream));
-    VD_HIP(hipFree(p));
+    dev_free(ctx, p);
+    return VD_OK;
+}
+
+int vd_ctx_create_batched(int device, int ncase, size_t case_bytes, vd_ctx** out) {
+    if (!out) return VD_EINVAL;
+    *out = nullptr;
+    if (ncase < 1 || ncase > VD_MAX_BATCH || case_bytes == 0) {
+        set_error("vd_ctx_create_batched: ncase %d (1 .. %d), case_bytes %zu", ncase, VD_MAX_BATCH, case_bytes);
+        return VD_EINVAL;
+    }
+    vd_ctx* c;
+    VD_TRY(vd_ctx_create(device, &c));
+    const size_t region = (case_bytes + 4095) & ~(size_t)4095;
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, region * (size_t)ncase);
+    if (e != hipSuccess) {
+        vd_ctx_destroy(c);
+        set_error("vd_ctx_create_batched: hipMalloc %d x %zu bytes: %s", ncase, region, hipGetErrorString(e));
+        return VD_ENOMEM;
+    }
+    c->arena = (char*)p;
+    c->arena_case = region;
+    c->ncase = ncase;
+    c->bt.stride = (int64_t)region;
+    for (int i = 0; i < ncase; i++) {
+        c->bt.idx[i] = (unsigned char)i;
+        c->act[i] = i;
+    }
+    c->bn = c->nact = ncase;
+    *out = c;
+    return VD_OK;
+}
+
+int vd_ctx_ncase(const vd_ctx* ctx) { return ctx ? ctx->ncase : 0; }
+
+long long vd_ctx_case_stride(const vd_ctx* ctx) { return ctx ? (long long)ctx->bt.stride : 0; }
+
+int vd_ctx_arena_used(const vd_ctx* ctx, size_t* used) {
+    if (!ctx || !used) return VD_EINVAL;
+    *used = ctx->arena_used;
+    return VD_OK;
+}
+
+int vd_ctx_set_active(vd_ctx* ctx, int n, const int* cases) {
+    if (!ctx || n < 1 || n > ctx->ncase || !cases) {
+        set_error("vd_ctx_set_active: %d cases of %d", n, ctx ? ctx->ncase : 0);
+        return VD_EINVAL;
+    }
+    int seen[VD_MAX_BATCH] = {0};
+    for (int i = 0; i < n; i++) {
+        if (cases[i] < 0 || cases[i] >= ctx->ncase || seen[cases[i]]) {
+            set_error("vd_ctx_set_active: case %d repeated or outside 0 .. %d", cases[i], ctx->ncase - 1);
+            return VD_EINVAL;
+        }
+        seen[cases[i]] = 1;
+    }
+    for (int i = 0; i < n; i++) {
+        ctx->act[i] = cases[i];
+        ctx->bt.idx[i] = (unsigned char)cases[i];
+    }
+    ctx->nact = ctx->bn = n;
+    return VD_OK;
+}
+
+int vd_ctx_set_case_voltage(vd_ctx* ctx, int c, int use, double amp_1, double amp_2) {
+    if (!ctx || c < 0 || c >= ctx->ncase) return VD_EINVAL;
+    ctx->case_volt_set[c] = use != 0;
+    ctx->case_volt[c][0] = amp_1;
+    ctx->case_volt[c][1] = amp_2;
+    return VD_OK;
+}
+
+int vd_ctx_case_solve_info(const vd_ctx* ctx, int c, vd_solve_info* info) {
+    if (!ctx || !info || c < 0 || c >= ctx->ncase) return VD_EINVAL;
+    *info = ctx->case_info[c];
+    return VD_OK;
+}
+
+int vd_ctx_case_time_scales(const vd_ctx* ctx, int c, double out[3]) {
+    if (!ctx || !out || c < 0 || c >= ctx->ncase) return VD_EINVAL;
+    for (int q = 0; q < 3; q++) out[q] = ctx->case_ts[c][q];
     return VD_OK;
 }
 
@@ -496,9 +624,17 @@ int vd_mech_create(vd_ctx* ctx, const vd_mech_desc* d, vd_mech** out) {
                 h.ghi[k][q] = d->gibbs_hi[k][q];
             }
         }
-    hipError_t e = hipMalloc((void**)&m->dev, sizeof(DevMech));
-    if (e == hipSuccess) e = hipMemcpy(m->dev, &h, sizeof(DevMech), hipMemcpyHostToDevice);
+    // in a batched context every case holds a copy (at its own offset), so
+    // kernels may move any pointer they take by the case's offset
+    if (dev_alloc(ctx, sizeof(DevMech), (void**)&m->dev) != VD_OK) {
+        delete m;
+        return VD_ENOMEM;
+    }
+    hipError_t e = hipSuccess;
+    for (int c = 0; c < ctx->ncase && e == hipSuccess; c++)
+        e = hipMemcpy((char*)m->dev + (size_t)c * ctx->bt.stride, &h, sizeof(DevMech), hipMemcpyHostToDevice);
     if (e != hipSuccess) {
+        dev_free(ctx, m->dev);
         delete m;
         set_error("mechanism upload: %s", hipGetErrorString(e));
         return VD_EHIP;
@@ -518,7 +654,7 @@ int vd_mech_create(vd_ctx* ctx, const vd_mech_desc* d, vd_mech** out) {
 int vd_mech_destroy(vd_mech* m) {
     if (!m) return VD_EINVAL;
     chem_jit_unload(m);
-    (void)hipFree(m->dev);
+    dev_free(m->ctx, m->dev);
     delete m;
     return VD_OK;
 }

@@ -13,6 +13,11 @@ struct ChemArgs {
     double Tg;
     int3 lo;
     int nx, ny, nz;
+    __device__ void rebase(int64_t o) {
+        M = rb(M, o);
+        vd::rebase(sb, o);
+        vd::rebase(rxn, o);
+    }
 };
 
 constexpr int CHEM_BLOCK = 128;
@@ -24,7 +29,8 @@ __device__ __forceinline__ double powi_nu(double n, double nu) {
 }
 
 // Source/reaction_source.H:10-64 (table form)
-__global__ void __launch_bounds__(CHEM_BLOCK) k_rxn_table(ChemArgs a) {
+__global__ void __launch_bounds__(CHEM_BLOCK) k_rxn_table(ChemArgs a, Batch bt) {
+    a.rebase(case_off(bt));
     extern __shared__ double w[];
     const DevMech& M = *a.M;
     const int tid = threadIdx.x;
@@ -66,7 +72,8 @@ __global__ void __launch_bounds__(CHEM_BLOCK) k_rxn_table(ChemArgs a) {
 // CKWC / productionRate / comp_ener_exch of a generated Chemistry.H
 // (test/models/ionelec_DC_1d/Chemistry.H:240-357) behind
 // update_rxnsrc_at_all_levels (Source/ScalarSolve.cpp:143-165).
-__global__ void __launch_bounds__(CHEM_BLOCK) k_rxn_ckwc(ChemArgs a) {
+__global__ void __launch_bounds__(CHEM_BLOCK) k_rxn_ckwc(ChemArgs a, Batch bt) {
+    a.rebase(case_off(bt));
     extern __shared__ double w[];
     const DevMech& M = *a.M;
     const int tid = threadIdx.x;
@@ -155,7 +162,9 @@ __global__ void __launch_bounds__(CHEM_BLOCK) k_rxn_ckwc(ChemArgs a) {
 // productionRate computes them per cell from the same T,
 // test/Reactor/CO2_H2_Ar_mechanism_reduced/Chemistry.H:913-950)
 __global__ void __launch_bounds__(256) k_gen_tconst(const DevMech* __restrict__ Mp, double T,
-                                                    double* __restrict__ tc) {
+                                                    double* __restrict__ tc, Batch bt) {
+    Mp = rb(Mp, case_off(bt));
+    tc = rb(tc, case_off(bt));
     const DevMech& M = *Mp;
     const double invT = 1.0 / T;
     const double logT = log(T / 300.0);
@@ -206,7 +215,9 @@ __device__ __forceinline__ double gen_kf(const DevMech& M, int r, const double* 
 // Chemistry.H:841-6399) behind update_rxnsrc_at_all_levels
 // (Source/ScalarSolve.cpp:143-165): the generic form, reading the reaction
 // records from the device descriptor; accumulators in LDS columns
-__global__ void __launch_bounds__(CHEM_BLOCK) k_rxn_gen(ChemArgs a, const double* __restrict__ tc) {
+__global__ void __launch_bounds__(CHEM_BLOCK) k_rxn_gen(ChemArgs a, const double* __restrict__ tc, Batch bt) {
+    a.rebase(case_off(bt));
+    tc = rb(tc, case_off(bt));
     extern __shared__ double w[];
     const DevMech& M = *a.M;
     const int tid = threadIdx.x;
@@ -315,7 +326,8 @@ __global__ void __launch_bounds__(CHEM_BLOCK) k_rxn_gen(ChemArgs a, const double
 
 // user_sources::add_user_react_sources (Source/ScalarSolve.cpp:158-163):
 // the case's constant sources added to every component of rxn_src
-__global__ void __launch_bounds__(256) k_user_src(ChemArgs a) {
+__global__ void __launch_bounds__(256) k_user_src(ChemArgs a, Batch bt) {
+    a.rebase(case_off(bt));
     const int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.nx * a.ny * a.nz) return;
     const int i = (int)(t % a.nx);
@@ -329,7 +341,8 @@ __global__ void __launch_bounds__(256) k_user_src(ChemArgs a) {
 
 static int add_user_src(vd_ctx* ctx, const vd_mech* m, const ChemArgs& a) {
     if (!m->hdev.any_user_src) return VD_OK;
-    hipLaunchKernelGGL(k_user_src, grid1d((int64_t)a.nx * a.ny * a.nz, 256), dim3(256), 0, ctx->stream, a);
+    hipLaunchKernelGGL(k_user_src, bz(ctx, grid1d((int64_t)a.nx * a.ny * a.nz, 256)), dim3(256), 0, ctx->stream, a,
+                       ctx->bt);
     VD_CHECK_LAUNCH();
     return VD_OK;
 }
@@ -355,13 +368,14 @@ int update_rxnsrc(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4*
         double* tc = nullptr;
         VD_TRY(scratch(ctx, "gen_tconst", sizeof(double) * GEN_TCONST_DOUBLES, (void**)&tc));
         kt_begin(ctx, VD_KT_CHEM);
-        hipLaunchKernelGGL(k_gen_tconst, dim3(1), dim3(256), 0, ctx->stream, m->dev, a.Tg, tc);
+        hipLaunchKernelGGL(k_gen_tconst, bz(ctx, dim3(1)), dim3(256), 0, ctx->stream, m->dev, a.Tg, tc, ctx->bt);
         if (m->use_spec && m->fn) {
             int st = chem_jit_launch(ctx, m, sb, rxn, box, prm->gas_temperature, tc);
             kt_end(ctx, VD_KT_CHEM, 8.0 * (double)tot * (2 * m->hdev.nspec + 3));
             if (st != VD_OK) return st;
         } else {
-            hipLaunchKernelGGL(k_rxn_gen, grid1d(tot, CHEM_BLOCK), dim3(CHEM_BLOCK), lds, ctx->stream, a, tc);
+            hipLaunchKernelGGL(k_rxn_gen, bz(ctx, grid1d(tot, CHEM_BLOCK)), dim3(CHEM_BLOCK), lds, ctx->stream, a, tc,
+                               ctx->bt);
             kt_end(ctx, VD_KT_CHEM, 8.0 * (double)tot * (2 * m->hdev.nspec + 3));
         }
         VD_CHECK_LAUNCH();
@@ -375,9 +389,11 @@ int update_rxnsrc(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4*
         return add_user_src(ctx, m, a);
     }
     if (m->hdev.chem_form == VD_CHEM_TABLE)
-        hipLaunchKernelGGL(k_rxn_table, grid1d(tot, CHEM_BLOCK), dim3(CHEM_BLOCK), lds, ctx->stream, a);
+        hipLaunchKernelGGL(k_rxn_table, bz(ctx, grid1d(tot, CHEM_BLOCK)), dim3(CHEM_BLOCK), lds, ctx->stream, a,
+                           ctx->bt);
     else
-        hipLaunchKernelGGL(k_rxn_ckwc, grid1d(tot, CHEM_BLOCK), dim3(CHEM_BLOCK), lds, ctx->stream, a);
+        hipLaunchKernelGGL(k_rxn_ckwc, bz(ctx, grid1d(tot, CHEM_BLOCK)), dim3(CHEM_BLOCK), lds, ctx->stream, a,
+                           ctx->bt);
     kt_end(ctx, VD_KT_CHEM, 8.0 * (double)tot * (2 * m->hdev.nspec + 3));
     VD_CHECK_LAUNCH();
     return add_user_src(ctx, m, a);
@@ -391,14 +407,30 @@ struct JouleArgs {
     int state_trans;  // the electron mobility or diffusivity is a fit of the state
     int3 dlo, dhi, lo;
     int nx, ny, nz;
+    __device__ void rebase(int64_t o) {
+        M = rb(M, o);
+        vd::rebase(sb, o);
+        vd::rebase(rxn, o);
+        vd::rebase(ef0, o);
+        vd::rebase(ef1, o);
+        vd::rebase(ef2, o);
+        vd::rebase(gn0, o);
+        vd::rebase(gn1, o);
+        vd::rebase(gn2, o);
+        vd::rebase(dsdt, o);
+        vd::rebase(phi, o);
+    }
 };
 
 // Vidyut::compute_elecenergy_source (Source/ElecEnergySource.cpp:65-171)
 // STATE: the electron coefficients are fits of the face state (else
 // constants / Te only; a.state_trans mirrors it)
 template <bool STATE>
-__global__ void __launch_bounds__(256) k_joule(JouleArgs a) {
-    const DevMech& M = *a.M;
+__global__ void __launch_bounds__(256) k_joule(JouleArgs a, Batch bt) {
+    // a stays unmodified (its face arrays are selected by direction at run
+    // time); the case's offset is applied where each array is read
+    const int64_t co = case_off(bt);
+    const DevMech& M = *rb(a.M, co);
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     int64_t tot = (int64_t)a.nx * a.ny * a.nz;
     if (t >= tot) return;
@@ -424,12 +456,12 @@ __global__ void __launch_bounds__(256) k_joule(JouleArgs a) {
             int lc[3] = {face[0], face[1], face[2]};
             lc[idim] -= 1;
             const int ol = a.sb.off(lc[0], lc[1], lc[2]), orr = a.sb.off(face[0], face[1], face[2]);
-            const double* S = a.sb.p;
+            const double* S = rb(a.sb.p, co);
             const int64_t N = a.sb.ns;
             double etemp = 0.5 * (S[VD_ETEMP(ns) * N + ol] + S[VD_ETEMP(ns) * N + orr]);
             double ne = 0.5 * (S[(int64_t)eidx * N + ol] + S[(int64_t)eidx * N + orr]);
-            double efield_face = EF(face[0], face[1], face[2]);
-            double gradne_face = GN(face[0], face[1], face[2]);
+            double efield_face = at(EF, co, face[0], face[1], face[2]);
+            double gradne_face = at(GN, co, face[0], face[1], face[2]);
             double mu, dcoeff;
             if (a.const_ele_trans || STATE) {
                 double ndens = 0.0;
@@ -459,9 +491,9 @@ __global__ void __launch_bounds__(256) k_joule(JouleArgs a) {
     }
     jheat *= 0.5;
     if (a.floor_jh && jheat < 0.0) jheat = 0.0;
-    a.dsdt(i, j, k) += jheat;
-    a.phi(i, j, k, VD_EJH(ns)) = jheat;
-    a.phi(i, j, k, VD_EIH(ns)) = a.rxn(i, j, k, ns);
+    at(a.dsdt, co, i, j, k) += jheat;
+    at(a.phi, co, i, j, k, VD_EJH(ns)) = jheat;
+    at(a.phi, co, i, j, k, VD_EIH(ns)) = at(a.rxn, co, i, j, k, ns);
 }
 
 int elecenergy_source(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, const vd_array4* rxn,
@@ -493,8 +525,8 @@ int elecenergy_source(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, const 
     a.ny = box->hi[1] - box->lo[1] + 1;
     a.nz = box->hi[2] - box->lo[2] + 1;
     int64_t tot = (int64_t)a.nx * a.ny * a.nz;
-    if (a.state_trans) hipLaunchKernelGGL(k_joule<true>, grid1d(tot, 256), dim3(256), 0, ctx->stream, a);
-    else hipLaunchKernelGGL(k_joule<false>, grid1d(tot, 256), dim3(256), 0, ctx->stream, a);
+    if (a.state_trans) hipLaunchKernelGGL(k_joule<true>, bz(ctx, grid1d(tot, 256)), dim3(256), 0, ctx->stream, a, ctx->bt);
+    else hipLaunchKernelGGL(k_joule<false>, bz(ctx, grid1d(tot, 256)), dim3(256), 0, ctx->stream, a, ctx->bt);
     VD_CHECK_LAUNCH();
     return VD_OK;
 }

@@ -33,13 +33,25 @@ namespace vd {
     const double* tc;                                                           \
     long long sbn, rxn;                                                         \
     int sbo, rxo, sbj, sbk, rxj, rxk, nx, ny, nz, pad;                          \
-    double Tg;
+    double Tg;                                                                  \
+    long long cstride;                                                          \
+    unsigned char cidx[16];
 #define VD_STRX(...) #__VA_ARGS__
 #define VD_STR(...) VD_STRX(__VA_ARGS__)
 
 struct GenArgs {
     VD_GEN_ARGS_BODY
 };
+static_assert(VD_MAX_BATCH == 16, "GenArgs::cidx holds one entry per batched case");
+
+// the generated kernel's first lines: its case's arrays (the batch
+// offset of blockIdx.z, as case_off gives it)
+static const char* GEN_PROLOGUE =
+    "  const long long co_ = a0.cstride * (long long)a0.cidx[blockIdx.z];\n"
+    "  GenArgs a = a0;\n"
+    "  a.sb = (const double*)((const char*)a.sb + co_);\n"
+    "  a.rx = (double*)((char*)a.rx + co_);\n"
+    "  if (a.tc) a.tc = (const double*)((const char*)a.tc + co_);\n";
 
 constexpr int GEN_BLOCK = 256;
 
@@ -59,7 +71,8 @@ static bool chem_source(const DevMech& h, std::string& s) {
     const int ns = h.nspec;
     s.reserve(64 * 1024);
     s += "struct GenArgs { " VD_STR(VD_GEN_ARGS_BODY) " };\n";
-    s += "extern \"C\" __global__ void __launch_bounds__(256) vd_chem_spec(GenArgs a) {\n";
+    s += "extern \"C\" __global__ void __launch_bounds__(256) vd_chem_spec(GenArgs a0) {\n";
+    s += GEN_PROLOGUE;
     // XCD-aware block order (same mapping as xcd_remap)
     s += "  const unsigned nb = gridDim.x, b = blockIdx.x;\n"
          "  const unsigned q8 = nb >> 3, r8 = nb & 7u, x8 = b & 7u;\n"
@@ -153,7 +166,8 @@ static bool chem_source_gen(const DevMech& h, std::string& s) {
             return false;
     s.reserve(256 * 1024);
     s += "struct GenArgs { " VD_STR(VD_GEN_ARGS_BODY) " };\n";
-    s += "extern \"C\" __global__ void __launch_bounds__(256) vd_chem_spec(GenArgs a) {\n";
+    s += "extern \"C\" __global__ void __launch_bounds__(256) vd_chem_spec(GenArgs a0) {\n";
+    s += GEN_PROLOGUE;
     s += "  const unsigned nb = gridDim.x, b = blockIdx.x;\n"
          "  const unsigned q8 = nb >> 3, r8 = nb & 7u, x8 = b & 7u;\n"
          "  const unsigned lb = x8 * q8 + (x8 < r8 ? x8 : r8) + (b >> 3);\n"
@@ -377,6 +391,8 @@ int chem_jit_launch(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array
     a.nz = box->hi[2] - box->lo[2] + 1;
     a.pad = 0;
     a.Tg = Tg;
+    a.cstride = ctx->bt.stride;
+    memcpy(a.cidx, ctx->bt.idx, sizeof(a.cidx));
     const int64_t tot = (int64_t)a.nx * a.ny * a.nz;
     if (tot <= 0) return VD_OK;
     if (tot > 0xffffffffLL - GEN_BLOCK) {
@@ -384,8 +400,8 @@ int chem_jit_launch(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array
         return VD_EINVAL;
     }
     void* params[] = {&a};
-    VD_HIP(hipModuleLaunchKernel((hipFunction_t)m->fn, (unsigned)((tot + GEN_BLOCK - 1) / GEN_BLOCK), 1, 1,
-                                 GEN_BLOCK, 1, 1, 0, ctx->stream, params, nullptr));
+    VD_HIP(hipModuleLaunchKernel((hipFunction_t)m->fn, (unsigned)((tot + GEN_BLOCK - 1) / GEN_BLOCK), 1,
+                                 (unsigned)ctx->bn, GEN_BLOCK, 1, 1, 0, ctx->stream, params, nullptr));
     return VD_OK;
 }
 

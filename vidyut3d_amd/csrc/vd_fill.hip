@@ -5,13 +5,15 @@
 
 namespace vd {
 
-__global__ void k_fill_value(double* __restrict__ p, int64_t n, double v) {
+__global__ void k_fill_value(double* __restrict__ p, int64_t n, double v, Batch bt) {
+    p = rb(p, case_off(bt));
     int64_t i = (int64_t)lblock() * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = v;
 }
 
-__global__ void k_fill_value_f(float* __restrict__ p, int64_t n, float v) {
+__global__ void k_fill_value_f(float* __restrict__ p, int64_t n, float v, Batch bt) {
+    p = rb(p, case_off(bt));
     int64_t i = (int64_t)lblock() * blockDim.x + threadIdx.x;
     int64_t stride = (int64_t)gridDim.x * blockDim.x;
     for (; i < n; i += stride) p[i] = v;
@@ -20,7 +22,8 @@ __global__ void k_fill_value_f(float* __restrict__ p, int64_t n, float v) {
 // zeroes n bytes at p: the bytes before the first 16-byte boundary and
 // after the last one one per lane, the aligned middle in 16-byte stores
 // (the grid has at least 16 lanes, so every head and tail byte has one)
-__global__ void k_fill_bytes(unsigned char* __restrict__ p, int64_t n) {
+__global__ void k_fill_bytes(unsigned char* __restrict__ p, int64_t n, Batch bt) {
+    p = rb(p, case_off(bt));
     const int64_t i = (int64_t)lblock() * blockDim.x + threadIdx.x;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     int64_t head = (int64_t)((16u - ((uintptr_t)p & 15u)) & 15u);
@@ -42,7 +45,7 @@ static inline dim3 grid_stride(int64_t n, int block) {
 
 int fill_value(vd_ctx* ctx, double* p, int64_t n, double v) {
     if (n <= 0) return VD_OK;
-    hipLaunchKernelGGL(k_fill_value, grid_stride(n, 256), dim3(256), 0, ctx->stream, p, n, v);
+    hipLaunchKernelGGL(k_fill_value, bz(ctx, grid_stride(n, 256)), dim3(256), 0, ctx->stream, p, n, v, ctx->bt);
     VD_CHECK_LAUNCH();
     return VD_OK;
 }
@@ -60,13 +63,13 @@ struct FillBatch {
     int count;
 };
 
-__global__ void __launch_bounds__(256) k_fill_batch(FillBatch b) {
+__global__ void __launch_bounds__(256) k_fill_batch(FillBatch b, Batch bt) {
     const unsigned blk = blockIdx.x;
     int q = 0;
     while (q + 1 < b.count && blk >= b.first[q + 1]) q++;
     const int64_t base = (int64_t)(blk - b.first[q]) * FILL_CHUNK;
     const int64_t end = base + FILL_CHUNK < b.n[q] ? base + FILL_CHUNK : b.n[q];
-    double* __restrict__ p = b.p[q];
+    double* __restrict__ p = rb(b.p[q], case_off(bt));
     const double v = b.v[q];
     for (int64_t i = base + threadIdx.x; i < end; i += 256) p[i] = v;
 }
@@ -92,7 +95,7 @@ int fill_values(vd_ctx* ctx, double* const* p, const int64_t* n, const double* v
             return VD_EINVAL;
         }
         b.first[b.count] = (unsigned)nb;
-        hipLaunchKernelGGL(k_fill_batch, dim3((unsigned)nb), dim3(256), 0, ctx->stream, b);
+        hipLaunchKernelGGL(k_fill_batch, bz(ctx, dim3((unsigned)nb)), dim3(256), 0, ctx->stream, b, ctx->bt);
     }
     VD_CHECK_LAUNCH();
     return VD_OK;
@@ -100,7 +103,7 @@ int fill_values(vd_ctx* ctx, double* const* p, const int64_t* n, const double* v
 
 int fill_value_f(vd_ctx* ctx, float* p, int64_t n, float v) {
     if (n <= 0) return VD_OK;
-    hipLaunchKernelGGL(k_fill_value_f, grid_stride(n, 256), dim3(256), 0, ctx->stream, p, n, v);
+    hipLaunchKernelGGL(k_fill_value_f, bz(ctx, grid_stride(n, 256)), dim3(256), 0, ctx->stream, p, n, v, ctx->bt);
     VD_CHECK_LAUNCH();
     return VD_OK;
 }
@@ -110,7 +113,10 @@ int fill_value_f(vd_ctx* ctx, float* p, int64_t n, float v) {
 // indices and clamping the others into the domain.  One thread per
 // destination cell, blockIdx.y = component.
 __global__ void k_fill_patch(Arr<double> src, Arr<double> dst, int dim, int3 lo, int3 n,
-                             int3 per, int nx, int ny, int nz, int scomp) {
+                             int3 per, int nx, int ny, int nz, int scomp, Batch bt) {
+    const int64_t co = case_off(bt);
+    rebase(src, co);
+    rebase(dst, co);
     int q = blockIdx.y + scomp;
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     int64_t tot = (int64_t)nx * ny * nz;
@@ -147,7 +153,8 @@ __global__ void k_fill_patch(Arr<double> src, Arr<double> dst, int dim, int3 lo,
 // (z ghost planes, then y ghost rows of the valid planes, then x ghost
 // cells of the valid rows); blockIdx.y = component.
 __global__ void k_fill_ghosts(Arr<double> a, int dim, int3 lo, int3 n, int3 per, int gx, int gy, int gz,
-                              int scomp) {
+                              int scomp, Batch bt) {
+    rebase(a, case_off(bt));
     const int q = blockIdx.y + scomp;
     const int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     const int gxl = lo.x - a.bx, gyl = lo.y - a.by, gzl = lo.z - a.bz;
@@ -220,15 +227,15 @@ int fill_patch(vd_ctx* ctx, const vd_array4* src, vd_array4* dst, const vd_box* 
         const int64_t ng = tot - nv;
         if (ng <= 0) return VD_OK;
         kt_begin(ctx, VD_KT_FILL);
-        hipLaunchKernelGGL(k_fill_ghosts, dim3((unsigned)((ng + 255) / 256), ncomp), dim3(256), 0, ctx->stream,
-                           arr(dst), g->dim, lo, n, per, nx, ny, nz, scomp);
+        hipLaunchKernelGGL(k_fill_ghosts, bz(ctx, dim3((unsigned)((ng + 255) / 256), ncomp)), dim3(256), 0,
+                           ctx->stream, arr(dst), g->dim, lo, n, per, nx, ny, nz, scomp, ctx->bt);
         kt_end(ctx, VD_KT_FILL, 16.0 * (double)ng * ncomp);
         VD_CHECK_LAUNCH();
         return VD_OK;
     }
     kt_begin(ctx, VD_KT_FILL);
-    hipLaunchKernelGGL(k_fill_patch, dim3((unsigned)((tot + 255) / 256), ncomp), dim3(256), 0,
-                       ctx->stream, arr(src), arr(dst), g->dim, lo, n, per, nx, ny, nz, scomp);
+    hipLaunchKernelGGL(k_fill_patch, bz(ctx, dim3((unsigned)((tot + 255) / 256), ncomp)), dim3(256), 0,
+                       ctx->stream, arr(src), arr(dst), g->dim, lo, n, per, nx, ny, nz, scomp, ctx->bt);
     kt_end(ctx, VD_KT_FILL, 16.0 * (double)tot * ncomp);
     VD_CHECK_LAUNCH();
     return VD_OK;
@@ -502,6 +509,7 @@ extern "C" int vd_fill_patch_boxes(vd_ctx* ctx, int nbox, const vd_array4* src, 
                                    const vd_box* boxes, const vd_geom* geom, int scomp, int ncomp) {
     if (!ctx || !src || !dst || !boxes || !geom) return VD_EINVAL;
     for (int b = 0; b < nbox; b++) VD_TRY(vd::check_extents("fill_patch_boxes", &boxes[b], {&src[b], &dst[b]}));
+    VD_TRY(vd::require_plain(ctx, "fill_patch_boxes"));
     return vd::fill_patch_boxes(ctx, nbox, src, dst, boxes, geom, scomp, ncomp);
 }
 
@@ -510,6 +518,7 @@ extern "C" int vd_level_gather(vd_ctx* ctx, int nbox, const vd_array4* parts, co
     if (!ctx || !parts || !boxes || !whole || !geom) return VD_EINVAL;
     VD_TRY(vd::check_extents("level gather/scatter", nullptr, {whole}));
     for (int b = 0; b < nbox; b++) VD_TRY(vd::check_extents("level gather/scatter", &boxes[b], {&parts[b]}));
+    VD_TRY(vd::require_plain(ctx, "level gather/scatter"));
     return vd::level_copy(ctx, true, nbox, parts, boxes, whole, geom, scomp, ncomp);
 }
 
@@ -518,6 +527,7 @@ extern "C" int vd_level_scatter(vd_ctx* ctx, int nbox, const vd_array4* whole, v
     if (!ctx || !parts || !boxes || !whole || !geom) return VD_EINVAL;
     VD_TRY(vd::check_extents("level gather/scatter", nullptr, {whole}));
     for (int b = 0; b < nbox; b++) VD_TRY(vd::check_extents("level gather/scatter", &boxes[b], {&parts[b]}));
+    VD_TRY(vd::require_plain(ctx, "level gather/scatter"));
     return vd::level_copy(ctx, false, nbox, parts, boxes, whole, geom, scomp, ncomp);
 }
 
@@ -530,8 +540,8 @@ extern "C" int vd_fill_patch(vd_ctx* ctx, const vd_array4* src, vd_array4* dst, 
 
 extern "C" int vd_memset_zero_kernel(vd_ctx* ctx, void* dst, size_t bytes) {
     if (!bytes) return VD_OK;
-    hipLaunchKernelGGL(vd::k_fill_bytes, vd::grid_stride((int64_t)(bytes >> 4) + 1, 256), dim3(256), 0,
-                       ctx->stream, (unsigned char*)dst, (int64_t)bytes);
+    hipLaunchKernelGGL(vd::k_fill_bytes, vd::bz(ctx, vd::grid_stride((int64_t)(bytes >> 4) + 1, 256)), dim3(256),
+                       0, ctx->stream, (unsigned char*)dst, (int64_t)bytes, ctx->bt);
     VD_CHECK_LAUNCH();
     return VD_OK;
 }

@@ -68,6 +68,38 @@ struct Arr {
     }
 };
 
+// Same-shape cases in one launch.  A batched context places case c's copy
+// of every device allocation at case 0's address plus c * stride (the
+// allocations are carved out of identical per-case regions of one arena),
+// so a kernel reaches its case's data by moving each of its pointers by
+// one byte offset.  gridDim.z is the number of cases of a launch and idx
+// names the case of each z-block; an unbatched launch has gridDim.z = 1
+// and stride 0.
+struct Batch {
+    int64_t stride;
+    unsigned char idx[VD_MAX_BATCH];
+};
+// doubles of the host-mapped norm read-back buffer (values of up to
+// VD_MAX_BATCH cases x 5 slot groups, then a sequence number in the last)
+constexpr int HNORM_DOUBLES = 256;
+__device__ __forceinline__ int64_t case_off(const Batch& b) { return b.stride * (int64_t)b.idx[blockIdx.z]; }
+// p moved by o bytes (a null pointer stays null: it marks absent data)
+template <class T>
+__host__ __device__ __forceinline__ T* rb(T* p, int64_t o) {
+    return p ? (T*)((char*)p + o) : p;
+}
+template <class T>
+__host__ __device__ __forceinline__ void rebase(Arr<T>& a, int64_t o) {
+    a.p = rb(a.p, o);
+}
+// element (i, j, k, n) of a's copy o bytes on, for kernels that keep their
+// argument structs unmodified (a struct modified in a kernel and then
+// selected or indexed at run time goes to private memory)
+template <class T>
+__device__ __forceinline__ T& at(const Arr<T>& a, int64_t o, int i, int j, int k, int n = 0) {
+    return rb(a.p, o)[(int64_t)n * a.ns + a.off(i, j, k)];
+}
+
 inline Arr<double> arr(const vd_array4* a) {
     Arr<double> r;
     r.p = a->p;
@@ -330,9 +362,46 @@ struct vd_ctx {
     double kt_bytes[32] = {0.0};
     long kt_open[32] = {0};  // index of the family's open pair + 1 (0: none)
     size_t kt_next = 0;
+    // batching (vd_ctx_create_batched): ncase regions of arena_case bytes;
+    // the context's allocations are carved out of region 0 (bump pointer,
+    // released with the context).  bt / bn: the cases the next launches
+    // run on (bn z-blocks); act / nact: the active cases of the caller
+    int ncase = 1;
+    char* arena = nullptr;
+    size_t arena_case = 0, arena_used = 0;
+    vd::Batch bt{};
+    int bn = 1;
+    int act[VD_MAX_BATCH] = {0};
+    int nact = 1;
+    // per-case overrides and results
+    int case_volt_set[VD_MAX_BATCH] = {0};
+    double case_volt[VD_MAX_BATCH][2] = {{0.0}};
+    vd_solve_info case_info[VD_MAX_BATCH] = {};
+    double case_ts[VD_MAX_BATCH][3] = {{0.0}};
 };
 
 namespace vd {
+
+// the grid of a launch on the context's current cases (one z-block each)
+inline dim3 bz(const vd_ctx* c, dim3 g) {
+    g.z = (unsigned)c->bn;
+    return g;
+}
+// device memory of the context: hipMalloc, or in a batched context the
+// next piece of case 0's arena region (freed with the context)
+int dev_alloc(vd_ctx* ctx, size_t bytes, void** out);
+void dev_free(vd_ctx* ctx, void* p);
+// the launch batch narrowed to `n` of the active cases for a scope
+struct BatchScope {
+    vd_ctx* c;
+    Batch bt;
+    int bn;
+    BatchScope(vd_ctx* ctx, const int* cases, int n);
+    ~BatchScope();
+};
+// entry points outside the batched step path: VD_EINVAL in a batched
+// context (their pointers need not lie in the per-case regions)
+int require_plain(vd_ctx* ctx, const char* what);
 
 // returns a device buffer of at least `bytes` registered under `name`
 int scratch(vd_ctx* ctx, const std::string& name, size_t bytes, void** out);

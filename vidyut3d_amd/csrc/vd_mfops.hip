@@ -34,7 +34,8 @@ static BoxIter box_iter(const vd_box* b, const vd_array4* a) {
     return it;
 }
 
-__global__ void k_setval(Arr<double> a, BoxIter it, int scomp, double v) {
+__global__ void k_setval(Arr<double> a, BoxIter it, int scomp, double v, Batch bt) {
+    rebase(a, case_off(bt));
     const int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)it.nx * it.ny * it.nz) return;
     int i, j, k;
@@ -44,7 +45,11 @@ __global__ void k_setval(Arr<double> a, BoxIter it, int scomp, double v) {
 
 // dst = a*x + b*y  (b == 0 and y == x gives a scaled copy)
 __global__ void k_lincomb(Arr<double> dst, Arr<double> x, Arr<double> y, BoxIter it, int xc, int yc,
-                          int dc, double a, double b, int copy) {
+                          int dc, double a, double b, int copy, Batch bt) {
+    const int64_t co = case_off(bt);
+    rebase(dst, co);
+    rebase(x, co);
+    rebase(y, co);
     const int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)it.nx * it.ny * it.nz) return;
     int i, j, k;
@@ -67,8 +72,8 @@ extern "C" int vd_setval(vd_ctx* ctx, vd_array4* a, const vd_box* box, int scomp
     BoxIter it = box_iter(box, a);
     int64_t tot = (int64_t)it.nx * it.ny * it.nz;
     if (tot >= ((int64_t)1 << 31)) return box_too_large(tot);
-    hipLaunchKernelGGL(k_setval, dim3((unsigned)((tot + 255) / 256), ncomp), dim3(256), 0, ctx->stream,
-                       arr(a), it, scomp, v);
+    hipLaunchKernelGGL(k_setval, bz(ctx, dim3((unsigned)((tot + 255) / 256), ncomp)), dim3(256), 0, ctx->stream,
+                       arr(a), it, scomp, v, ctx->bt);
     VD_CHECK_LAUNCH();
     return VD_OK;
 }
@@ -82,8 +87,8 @@ extern "C" int vd_copy(vd_ctx* ctx, const vd_array4* src, vd_array4* dst, const 
     BoxIter it = box_iter(box, dst);
     int64_t tot = (int64_t)it.nx * it.ny * it.nz;
     if (tot >= ((int64_t)1 << 31)) return box_too_large(tot);
-    hipLaunchKernelGGL(k_lincomb, dim3((unsigned)((tot + 255) / 256), ncomp), dim3(256), 0, ctx->stream,
-                       arr(dst), arr(src), arr(src), it, scomp, scomp, dcomp, 1.0, 0.0, 1);
+    hipLaunchKernelGGL(k_lincomb, bz(ctx, dim3((unsigned)((tot + 255) / 256), ncomp)), dim3(256), 0, ctx->stream,
+                       arr(dst), arr(src), arr(src), it, scomp, scomp, dcomp, 1.0, 0.0, 1, ctx->bt);
     VD_CHECK_LAUNCH();
     return VD_OK;
 }
@@ -96,14 +101,15 @@ extern "C" int vd_lincomb(vd_ctx* ctx, vd_array4* dst, double a, const vd_array4
     BoxIter it = box_iter(box, dst);
     int64_t tot = (int64_t)it.nx * it.ny * it.nz;
     if (tot >= ((int64_t)1 << 31)) return box_too_large(tot);
-    hipLaunchKernelGGL(k_lincomb, dim3((unsigned)((tot + 255) / 256), ncomp), dim3(256), 0, ctx->stream,
-                       arr(dst), arr(x), arr(y), it, xcomp, ycomp, dcomp, a, b, 0);
+    hipLaunchKernelGGL(k_lincomb, bz(ctx, dim3((unsigned)((tot + 255) / 256), ncomp)), dim3(256), 0, ctx->stream,
+                       arr(dst), arr(x), arr(y), it, xcomp, ycomp, dcomp, a, b, 0, ctx->bt);
     VD_CHECK_LAUNCH();
     return VD_OK;
 }
 
 namespace vd {
-__global__ void k_clamp_min(Arr<double> a, BoxIter it, int comp, double lo) {
+__global__ void k_clamp_min(Arr<double> a, BoxIter it, int comp, double lo, Batch bt) {
+    rebase(a, case_off(bt));
     const int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)it.nx * it.ny * it.nz) return;
     int i, j, k;
@@ -120,8 +126,8 @@ extern "C" int vd_clamp_min(vd_ctx* ctx, vd_array4* a, const vd_box* box, int co
     BoxIter it = box_iter(box, a);
     int64_t tot = (int64_t)it.nx * it.ny * it.nz;
     if (tot >= ((int64_t)1 << 31)) return box_too_large(tot);
-    hipLaunchKernelGGL(k_clamp_min, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, ctx->stream, arr(a),
-                       it, comp, lo);
+    hipLaunchKernelGGL(k_clamp_min, bz(ctx, dim3((unsigned)((tot + 255) / 256))), dim3(256), 0, ctx->stream, arr(a),
+                       it, comp, lo, ctx->bt);
     VD_CHECK_LAUNCH();
     return VD_OK;
 }

@@ -50,6 +50,35 @@ struct LevV {  // kernel view of one level
     T c0u[3][2];         // c0 of sides whose c0 is uniform (no Robin side)
     int c0uni;           // 1: c0u holds every boundary c0 (no c0 loads)
     int per0, per1, per2;
+    // the coefficient arrays moved by o bytes: only those the coefficient
+    // mode BM reads (0 faces, 2 the cell coefficient; constant b none) and
+    // the boundary c0 arrays when C0 (they are not read when every c0 is
+    // uniform), so that no other pointer takes registers
+    template <int BM, bool C0>
+    __device__ __forceinline__ void rebase_used(int64_t o) {
+        if constexpr (BM == 0) {
+            bf0 = vd::rb(bf0, o);
+            bf1 = vd::rb(bf1, o);
+            bf2 = vd::rb(bf2, o);
+        }
+        if constexpr (BM == 2) dcell = vd::rb(dcell, o);
+        if constexpr (C0) {
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+#pragma unroll
+                for (int s = 0; s < 2; s++) c0[d][s] = vd::rb(c0[d][s], o);
+        }
+    }
+    __device__ __forceinline__ void rebase(int64_t o) {
+        bf0 = vd::rb(bf0, o);
+        bf1 = vd::rb(bf1, o);
+        bf2 = vd::rb(bf2, o);
+        dcell = vd::rb(dcell, o);
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int s = 0; s < 2; s++) c0[d][s] = vd::rb(c0[d][s], o);
+    }
 };
 
 template <class T>
@@ -68,6 +97,7 @@ struct Lev {
 };
 
 struct MGWork {
+    vd_ctx* ctx = nullptr;  // the context whose memory holds the arrays
     int dim = 3, nlev = 0, prec = 0, constb = 0;
     int n[3] = {1, 1, 1};
     int per[3] = {0, 0, 0};
@@ -86,7 +116,7 @@ struct MGWork {
     double* dcell = nullptr;              // cell coefficient on the grown box
     double* part = nullptr;               // per-block partial norms
     int c0_types[3][2] = {{-1, -1}, {-1, -1}, {-1, -1}};
-    bool c0_valid = false;                // coarse c0 match c0_types
+    unsigned c0_mask = 0;                 // cases whose coarse c0 match c0_types
     // no Robin side: c0 is -1 (Dirichlet) or 1 (Neumann) on every boundary
     // face of every level, and the fused smoother takes it as a constant
     int c0uni = 0;
@@ -100,20 +130,22 @@ struct MGWork {
     // nlev-1 coarsened in FP64 (level 0 is bf0 / c00), and FP64 CG arrays
     // of the bottom level, for the FP64 bottom solve
     std::vector<Lev<double>> C64;
+    // batched contexts: each case's copy of the coarse tail's arguments
+    // (k_tail_args), the arguments they were made from, and the cases
+    // whose copies are current
+    mutable void* tail_dev = nullptr;
+    mutable std::vector<unsigned char> tail_key;
+    mutable unsigned tail_mask = 0;
 };
 
 void mgwork_free(MGWork* w) {
     if (!w) return;
-    for (void* p : w->allocs) (void)hipFree(p);
+    for (void* p : w->allocs) dev_free(w->ctx, p);
     delete w;
 }
 
 static int dalloc(MGWork* w, size_t bytes, void** out) {
-    hipError_t e = hipMalloc(out, bytes < 64 ? 64 : bytes);
-    if (e != hipSuccess) {
-        set_error("multigrid workspace: hipMalloc %zu: %s", bytes, hipGetErrorString(e));
-        return VD_ENOMEM;
-    }
+    if (dev_alloc(w->ctx, bytes < 64 ? 64 : bytes, out) != VD_OK) return VD_ENOMEM;
     w->allocs.push_back(*out);
     return VD_OK;
 }
@@ -272,7 +304,11 @@ __device__ __forceinline__ void stencil(const LevV<T>& L, const T* __restrict__ 
 // ordered (j, k) and XCD-remapped
 template <class T, int BM, int DIM>
 __global__ void __launch_bounds__(128) k_gsrb(LevV<T> L, T* __restrict__ x, const T* __restrict__ rhs,
-                                              int color, int gx) {
+                                              int color, int gx, Batch bt) {
+    const int64_t co = case_off(bt);
+    L.template rebase_used<BM, true>(co);
+    x = rb(x, co);
+    rhs = rb(rhs, co);
     const unsigned lb = lblock();
     const unsigned row = lb / gx;
     const int j = (int)(row % L.ny), k = (int)(row / L.ny);
@@ -362,12 +398,24 @@ __device__ __forceinline__ void fused_unroll(F& f, int t, int tend) {
 
 // ACC: instead of storing the smoothed x, add it to the FP64 array xacc
 // (the correction of the last post-smooth of a V-cycle, x += e)
-template <class T, int BM, int S, int RES, int ZI, int ACC = 0>
+// BT: a batched launch (each z-block moves the arrays by its case's
+// offset); unbatched launches take the BT = false kernel, whose registers
+// the offsets do not take
+template <class T, int BM, int S, int RES, int ZI, int ACC = 0, bool BT = false>
 __global__ void __launch_bounds__(FzTile<BM>::THREADS) k_gsrb_fused(LevV<T> L, const T* __restrict__ xin,
                                                            T* __restrict__ xout, const T* __restrict__ rhs,
                                                            int zc, int ntx, int nty,
                                                            double* __restrict__ part,
-                                                           double* __restrict__ xacc) {
+                                                           double* __restrict__ xacc, Batch bt) {
+    if constexpr (BT) {
+        const int64_t co = case_off(bt);
+        L.template rebase_used<BM, false>(co);
+        xin = rb(xin, co);
+        xout = rb(xout, co);
+        rhs = rb(rhs, co);
+        part = rb(part, co);
+        xacc = rb(xacc, co);
+    }
     // Register ring indexed by plane: plane p lives in slot (p - zlo) mod NV
     // (NV even), and the step loop is unrolled NV times, so every slot index
     // is a compile-time constant and no values move between registers.
@@ -866,7 +914,17 @@ __global__ void __launch_bounds__(256) k_resid0(LevV<double> L, const double* __
                                                 const double* __restrict__ f, double* __restrict__ r,
                                                 double* __restrict__ part, int gx,
                                                 unsigned long long* __restrict__ bslots,
-                                                unsigned long long* __restrict__ fslots) {
+                                                unsigned long long* __restrict__ fslots, Batch bt) {
+    {
+        const int64_t co = case_off(bt);
+        L.template rebase_used<BM, true>(co);
+        x = rb(x, co);
+        f = rb(f, co);
+        r = rb(r, co);
+        part = rb(part, co);
+        bslots = rb(bslots, co);
+        fslots = rb(fslots, co);
+    }
     constexpr int NV = 1 + (BN ? 1 : 0) + (FM ? DIM : 0);  // maxima reduced per block
     __shared__ double red[4][NV];
     const unsigned lb = lblock();
@@ -930,7 +988,15 @@ template <int BM, bool BN, bool FM, bool C0U>
 __global__ void __launch_bounds__(RZ_X * RZ_Y) k_resid_z(LevV<double> L, const double* __restrict__ x,
                                                          const double* __restrict__ f, double* __restrict__ r,
                                                          int zc, int ntx, int nty,
-                                                         unsigned long long* __restrict__ slots) {
+                                                         unsigned long long* __restrict__ slots, Batch bt) {
+    {
+        const int64_t co = case_off(bt);
+        L.template rebase_used<BM, !C0U>(co);
+        x = rb(x, co);
+        f = rb(f, co);
+        r = rb(r, co);
+        slots = rb(slots, co);
+    }
     constexpr int NV = 1 + (BN ? 1 : 0) + (FM ? 3 : 0);
     __shared__ double red[RZ_X * RZ_Y / 64][NV];
     const unsigned lb = lblock();
@@ -1040,7 +1106,15 @@ template <bool BN, bool C0U, class RT = double>
 __global__ void __launch_bounds__(RZ2_X * RZ2_Y) k_resid_z2(LevV<double> L, const double* __restrict__ x,
                                                             const double* __restrict__ f, RT* __restrict__ r,
                                                             int zc, int ntx, int nty,
-                                                            unsigned long long* __restrict__ slots) {
+                                                            unsigned long long* __restrict__ slots, Batch bt) {
+    {
+        const int64_t co = case_off(bt);
+        L.template rebase_used<1, !C0U>(co);
+        x = rb(x, co);
+        f = rb(f, co);
+        r = rb(r, co);
+        slots = rb(slots, co);
+    }
     constexpr int NV = 1 + (BN ? 1 : 0);
     __shared__ double red[RZ2_X * RZ2_Y / 64][NV];
     const unsigned lb = lblock();
@@ -1130,9 +1204,15 @@ __global__ void __launch_bounds__(RZ2_X * RZ2_Y) k_resid_z2(LevV<double> L, cons
 // ZI: the first colour of a sweep from x = 0 (no zeroing pass): every
 // neighbour is 0, and the thread also writes the 0 of the other cell.
 constexpr int GZ_X = 32, GZ_Y = 8;  // pairs x rows per block
-template <class T, int BM, bool C0U, bool ZI = false>
+template <class T, int BM, bool C0U, bool ZI = false, bool BT = false>
 __global__ void __launch_bounds__(GZ_X * GZ_Y) k_gsrb_z(LevV<T> L, T* __restrict__ x, const T* __restrict__ rhs,
-                                                        int color, int zc, int ntx, int nty) {
+                                                        int color, int zc, int ntx, int nty, Batch bt) {
+    if constexpr (BT) {
+        const int64_t co = case_off(bt);
+        L.template rebase_used<BM, !C0U>(co);
+        x = rb(x, co);
+        rhs = rb(rhs, co);
+    }
     const unsigned lb = lblock();
     const int ntile = ntx * nty;
     const int tile = (int)(lb % (unsigned)ntile), zb = (int)(lb / (unsigned)ntile);
@@ -1269,7 +1349,9 @@ static int stream_zc(vd_ctx* ctx, const void* kern, int block, const int n[3], i
 
 // max of n per-block partials into slot 0 of a slot group (one workgroup)
 __global__ void __launch_bounds__(1024) k_reduce_partials(const double* __restrict__ part, int n,
-                                                          unsigned long long* __restrict__ slots) {
+                                                          unsigned long long* __restrict__ slots, Batch bt) {
+    part = rb(part, case_off(bt));
+    slots = rb(slots, case_off(bt));
     __shared__ double red[16];
     double a = 0.0;
     for (int t = threadIdx.x; t < n; t += blockDim.x) a = fmax(a, part[t]);
@@ -1284,7 +1366,9 @@ __global__ void __launch_bounds__(1024) k_reduce_partials(const double* __restri
 }
 
 __global__ void __launch_bounds__(256) k_maxabs(const double* __restrict__ f, unsigned n,
-                                                unsigned long long* __restrict__ slots) {
+                                                unsigned long long* __restrict__ slots, Batch bt) {
+    f = rb(f, case_off(bt));
+    slots = rb(slots, case_off(bt));
     double a = 0.0;
     for (unsigned t = blockIdx.x * blockDim.x + threadIdx.x; t < n; t += gridDim.x * blockDim.x) {
         double b = fabs(f[t]);
@@ -1314,10 +1398,17 @@ __device__ __forceinline__ T resid_restrict_cell(const LevV<T>& F, const T* __re
 }
 
 // 1-D grid of gx blocks per coarse row, XCD-remapped
-template <class T, int BM, int DIM>
+template <class T, int BM, int DIM, bool BT = false>
 __global__ void __launch_bounds__(128) k_resid_restrict(LevV<T> F, const T* __restrict__ e,
                                                         const T* __restrict__ rhs, int cnx, int cny,
-                                                        int cnz, T* __restrict__ rc, int gx) {
+                                                        int cnz, T* __restrict__ rc, int gx, Batch bt) {
+    if constexpr (BT) {
+        const int64_t co = case_off(bt);
+        F.template rebase_used<BM, true>(co);
+        e = rb(e, co);
+        rhs = rb(rhs, co);
+        rc = rb(rc, co);
+    }
     const unsigned lb = lblock();
     const unsigned row = lb / gx;
     const int J = (int)(row % cny), K = (int)(row / cny);
@@ -1329,7 +1420,8 @@ __global__ void __launch_bounds__(128) k_resid_restrict(LevV<T> F, const T* __re
 // e_f += P e_c (linear, tensor 3/4-1/4; ghosts of the coarse correction
 // are c0 * interior, periodic wraps): the increment of fine cell (i,j,k)
 template <class T, int DIM>
-__device__ __forceinline__ T prolong_cell(const LevV<T>& C, const T* __restrict__ ec, int i, int j, int k) {
+__device__ __forceinline__ T prolong_cell(const LevV<T>& C, const T* __restrict__ ec, int i, int j, int k,
+                                          int64_t co = 0) {
     int f[3] = {i, j, k};
     int cn[3] = {C.nx, C.ny, C.nz};
     int pr[3] = {C.per0, C.per1, C.per2};
@@ -1367,7 +1459,7 @@ __device__ __forceinline__ T prolong_cell(const LevV<T>& C, const T* __restrict_
                 for (int d = 0; d < DIM; d++)
                     if (sel[d] == 1 && ghost[d]) {
                         int side = f[d] & 1;
-                        v = v * (C.c0uni ? C.c0u[d][side] : C.c0[d][side][tidx(C, d, cc0, cc1, cc2)]);
+                        v = v * (C.c0uni ? C.c0u[d][side] : rb(C.c0[d][side], co)[tidx(C, d, cc0, cc1, cc2)]);
                     }
                 T w = (ix == 0 ? T(0.75) : T(0.25)) * (jy == 0 ? T(0.75) : T(0.25)) *
                       (DIM == 3 ? (kz == 0 ? T(0.75) : T(0.25)) : T(1));
@@ -1377,15 +1469,21 @@ __device__ __forceinline__ T prolong_cell(const LevV<T>& C, const T* __restrict_
 }
 
 // 1-D grid of gx blocks per fine row
-template <class T, int DIM>
+template <class T, int DIM, bool BT = false>
 __global__ void __launch_bounds__(128) k_prolong(LevV<T> C, const T* __restrict__ ec, int fnx, int fny,
-                                                 int fnz, T* __restrict__ ef, int gx) {
+                                                 int fnz, T* __restrict__ ef, int gx, Batch bt) {
+    int64_t cco = 0;  // C's c0 arrays: moved where read
+    if constexpr (BT) {
+        cco = case_off(bt);
+        ec = rb(ec, cco);
+        ef = rb(ef, cco);
+    }
     const unsigned lb = lblock();
     const unsigned row = lb / gx;
     const int j = (int)(row % fny), k = (int)(row / fny);
     const int i = (int)(lb % gx) * blockDim.x + threadIdx.x;
     if (i >= fnx) return;
-    ef[i + fnx * (j + fny * k)] += prolong_cell<T, DIM>(C, ec, i, j, k);
+    ef[i + fnx * (j + fny * k)] += prolong_cell<T, DIM>(C, ec, i, j, k, cco);
 }
 
 // Residual-restriction of a 3-D level, streamed along z: a thread owns one
@@ -1394,11 +1492,18 @@ __global__ void __launch_bounds__(128) k_prolong(LevV<T> C, const T* __restrict_
 // in-plane neighbours outside the 2 x 2 block and rhs are loaded per
 // plane.  Arithmetic (terms and their order) as resid_restrict_cell.
 constexpr int RR_X = 64, RR_Y = 4;
-template <class T, int BM, bool C0U>
+template <class T, int BM, bool C0U, bool BT = false>
 __global__ void __launch_bounds__(RR_X * RR_Y) k_resid_restrict_z(LevV<T> F, const T* __restrict__ e,
                                                                   const T* __restrict__ rhs, int cnx, int cny,
                                                                   int cnz, T* __restrict__ rc, int zc, int ntx,
-                                                                  int nty) {
+                                                                  int nty, Batch bt) {
+    if constexpr (BT) {
+        const int64_t co = case_off(bt);
+        F.template rebase_used<BM, !C0U>(co);
+        e = rb(e, co);
+        rhs = rb(rhs, co);
+        rc = rb(rc, co);
+    }
     const unsigned lb = lblock();
     const int ntile = ntx * nty;
     const int tile = (int)(lb % (unsigned)ntile), zb = (int)(lb / (unsigned)ntile);
@@ -1486,7 +1591,13 @@ __global__ void __launch_bounds__(RR_X * RR_Y) k_resid_restrict_z(LevV<T> F, con
 constexpr int PZ_X = 64, PZ_Y = 4;
 template <class T>
 __global__ void __launch_bounds__(PZ_X * PZ_Y) k_prolong_z(LevV<T> C, const T* __restrict__ ec, int fnx, int fny,
-                                                           T* __restrict__ ef, int zc, int ntx, int nty) {
+                                                           T* __restrict__ ef, int zc, int ntx, int nty, Batch bt) {
+    {
+        const int64_t co = case_off(bt);
+        // C's coefficient arrays are not read (uniform c0 only)
+        ec = rb(ec, co);
+        ef = rb(ef, co);
+    }
     const unsigned lb = lblock();
     const int ntile = ntx * nty;
     const int tile = (int)(lb % (unsigned)ntile), zb = (int)(lb / (unsigned)ntile);
@@ -1653,6 +1764,22 @@ struct VTail {
     // VC_WAVE_BOTTOM cells
     LevV<double> b64;
     double *be, *br, *bp, *bq;
+    // every array moved by o bytes (case copies of the arguments)
+    __device__ void rebase(int64_t o) {
+        for (int l = 0; l < VC_MAXL; l++) {
+            v[l].rebase(o);
+            e[l] = vd::rb(e[l], o);
+            r[l] = vd::rb(r[l], o);
+        }
+        cr = vd::rb(cr, o);
+        cp = vd::rb(cp, o);
+        cq = vd::rb(cq, o);
+        b64.rebase(o);
+        be = vd::rb(be, o);
+        br = vd::rb(br, o);
+        bp = vd::rb(bp, o);
+        bq = vd::rb(bq, o);
+    }
 };
 
 // f(i, j, k) for i < ni and every (j, k) row of a level, by one workgroup:
@@ -1773,7 +1900,7 @@ constexpr int VC_LDS_CELLS = 5464;
 // so when they coarsen; a bottom level that cannot coarsen may be larger,
 // and then every level stays in memory)
 template <class T, int BM, int DIM, bool LDS, bool B64>
-__global__ void __launch_bounds__(1024) k_vcycle_tail(VTail<T> a) {
+__device__ __forceinline__ void vcycle_tail_body(const VTail<T>& a) {
     // with LDS, every sweep, transfer and the bottom solve of the tail read
     // and write LDS; only r of the first level comes from memory and only
     // its e goes back (the next finer level prolongs from it)
@@ -1841,10 +1968,36 @@ __global__ void __launch_bounds__(1024) k_vcycle_tail(VTail<T> a) {
     }
 }
 
+// one case: the arguments from the kernel argument block
+template <class T, int BM, int DIM, bool LDS, bool B64>
+__global__ void __launch_bounds__(1024) k_vcycle_tail(VTail<T> a) {
+    vcycle_tail_body<T, BM, DIM, LDS, B64>(a);
+}
+
+// a batch of cases: each z-block reads its case's copy of the arguments
+// (k_tail_args), whose arrays are already the case's -- the levels are
+// indexed at run time, so the views are not moved inside this kernel
+template <class T, int BM, int DIM, bool LDS, bool B64>
+__global__ void __launch_bounds__(1024) k_vcycle_tail_mem(const VTail<T>* __restrict__ ap, Batch bt) {
+    vcycle_tail_body<T, BM, DIM, LDS, B64>(*rb(ap, case_off(bt)));
+}
+
+// case c's copy of the tail arguments (case 0's moved by c's offset) into
+// its copy of dst
+template <class T>
+__global__ void k_tail_args(VTail<T> a, VTail<T>* __restrict__ dst, Batch bt) {
+    if (threadIdx.x != 0) return;
+    const int64_t co = case_off(bt);
+    a.rebase(co);
+    *rb(dst, co) = a;
+}
+
 // coarse face b = average of the covering fine faces
 template <class T>
 __global__ void k_coarsen_faces(const T* __restrict__ fb, int fnx, int fny, int d, int dim, int cnx,
-                                int cny, int cnz, T* __restrict__ cb) {
+                                int cny, int cnz, T* __restrict__ cb, Batch bt) {
+    fb = rb(fb, case_off(bt));
+    cb = rb(cb, case_off(bt));
     int n0 = cnx + (d == 0), n1 = cny + (d == 1), n2 = cnz + (d == 2);
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)n0 * n1 * n2) return;
@@ -1868,7 +2021,9 @@ __global__ void k_coarsen_faces(const T* __restrict__ fb, int fnx, int fny, int 
 
 template <class T>
 __global__ void k_coarsen_c0(const T* __restrict__ fc, int fnx, int fny, int d, int dim, int cnx,
-                             int cny, int cnz, T* __restrict__ cc) {
+                             int cny, int cnz, T* __restrict__ cc, Batch bt) {
+    fc = rb(fc, case_off(bt));
+    cc = rb(cc, case_off(bt));
     int t1 = (d == 0) ? 1 : 0, t2 = (d == 2) ? 1 : 2;
     int cn[3] = {cnx, cny, cnz}, fn[3] = {fnx, fny, 0};
     int nt1 = cn[t1], nt2 = dim == 3 ? cn[t2] : 1;
@@ -1893,19 +2048,24 @@ __global__ void k_coarsen_c0(const T* __restrict__ fc, int fnx, int fny, int d, 
 }
 
 template <class A, class B>
-__global__ void k_convert(const A* __restrict__ a, B* __restrict__ b, int64_t n) {
+__global__ void k_convert(const A* __restrict__ a, B* __restrict__ b, int64_t n, Batch bt) {
+    a = rb(a, case_off(bt));
+    b = rb(b, case_off(bt));
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t < n) b[t] = (B)a[t];
 }
 
 template <class T>
-__global__ void k_zero(T* __restrict__ p, int64_t n) {
+__global__ void k_zero(T* __restrict__ p, int64_t n, Batch bt) {
+    p = rb(p, case_off(bt));
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t < n) p[t] = T(0);
 }
 
 template <class T>
-__global__ void k_axpy_to_double(double* __restrict__ x, const T* __restrict__ e, int64_t n) {
+__global__ void k_axpy_to_double(double* __restrict__ x, const T* __restrict__ e, int64_t n, Batch bt) {
+    x = rb(x, case_off(bt));
+    e = rb(e, case_off(bt));
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t < n) x[t] += (double)e[t];
 }
@@ -1943,13 +2103,24 @@ struct SideArgs {
     double bconst;
     double* c0;
     double* f;         // rhs to shift
+    __device__ void rebase(int64_t o) {
+        bcv = vd::rb(bcv, o);
+        ra = vd::rb(ra, o);
+        rb = vd::rb(rb, o);
+        rf = vd::rb(rf, o);
+        bf = vd::rb(bf, o);
+        dcell = vd::rb(dcell, o);
+        c0 = vd::rb(c0, o);
+        f = vd::rb(f, o);
+    }
 };
 
 // finest-level c0 and rhs shift s*b*cinh for one boundary side
 // both sides of one direction in one launch (blockIdx.y = side): they
 // touch different cells, so their f updates do not meet
-__global__ void k_side_setup(SideArgs a0, SideArgs a1) {
-    const SideArgs& a = blockIdx.y ? a1 : a0;
+__global__ void k_side_setup(SideArgs a0, SideArgs a1, Batch bt) {
+    SideArgs a = blockIdx.y ? a1 : a0;
+    a.rebase(case_off(bt));
     int t1 = a.d == 0 ? 1 : 0, t2 = a.d == 2 ? 1 : 2;
     int n[3] = {a.nx, a.ny, a.nz};
     int nt1 = n[t1], nt2 = a.dim == 3 ? n[t2] : 1;
@@ -2100,19 +2271,27 @@ static void launch_fused_k(vd_ctx* ctx, const LevV<T>& v, int zc, const T* xin, 
     const int nzc = (v.nz + zc - 1) / zc;
     const int nb = ntx * nty * nzc;
     if (nblk) *nblk = nb;
+    const bool bt = ctx->ncase > 1;
+#define VD_FUSED_LAUNCH(ZIV, ACCV, XACC)                                                                          \
+    {                                                                                                             \
+        if (bt)                                                                                                   \
+            hipLaunchKernelGGL((k_gsrb_fused<T, BM, S, RES, ZIV, ACCV, true>), bz(ctx, dim3((unsigned)nb)),       \
+                               dim3(TT), 0, ctx->stream, v, xin, xout, rhs, zc, ntx, nty, part, XACC, ctx->bt);    \
+        else                                                                                                      \
+            hipLaunchKernelGGL((k_gsrb_fused<T, BM, S, RES, ZIV, ACCV, false>), dim3((unsigned)nb), dim3(TT), 0,  \
+                               ctx->stream, v, xin, xout, rhs, zc, ntx, nty, part, XACC, ctx->bt);                \
+    }
     if constexpr (RES == 0) {
         if (xin && xacc) {
-            hipLaunchKernelGGL((k_gsrb_fused<T, BM, S, RES, 0, 1>), dim3((unsigned)nb), dim3(TT), 0,
-                               ctx->stream, v, xin, xout, rhs, zc, ntx, nty, part, xacc);
+            VD_FUSED_LAUNCH(0, 1, xacc)
             return;
         }
     }
     if (xin)
-        hipLaunchKernelGGL((k_gsrb_fused<T, BM, S, RES, 0>), dim3((unsigned)nb), dim3(TT), 0, ctx->stream,
-                           v, xin, xout, rhs, zc, ntx, nty, part, (double*)nullptr);
+        VD_FUSED_LAUNCH(0, 0, (double*)nullptr)
     else if constexpr (RES == 0)
-        hipLaunchKernelGGL((k_gsrb_fused<T, BM, S, RES, 1>), dim3((unsigned)nb), dim3(TT), 0, ctx->stream,
-                           v, xin, xout, rhs, zc, ntx, nty, part, (double*)nullptr);
+        VD_FUSED_LAUNCH(1, 0, (double*)nullptr)
+#undef VD_FUSED_LAUNCH
 }
 
 // nsweep (1 or 2) red-black sweeps of x (zeros when xin is null) into
@@ -2274,6 +2453,7 @@ static int get_work(vd_ctx* ctx, const vd_geom* g, const vd_box* box, int prec, 
         return VD_OK;
     }
     MGWork* W = new MGWork();
+    W->ctx = ctx;
     W->dim = g->dim;
     W->prec = prec;
     W->constb = constb;
@@ -2314,9 +2494,15 @@ static int get_work(vd_ctx* ctx, const vd_geom* g, const vd_box* box, int prec, 
     W->res = (double*)p;
     if ((st = dalloc(W, 8 * NSLOT * SLOT_STRIDE * sizeof(unsigned long long), &p))) goto fail;
     W->dnorm = (unsigned long long*)p;
-    // the norm slots start at zero; k_norm_out clears what it reads
-    hipLaunchKernelGGL(k_zero<unsigned long long>, grid1d(8 * NSLOT * SLOT_STRIDE, 256), dim3(256), 0,
-                       ctx->stream, W->dnorm, (int64_t)8 * NSLOT * SLOT_STRIDE);
+    {
+        // the norm slots start at zero, every case's (not only the active
+        // ones'); k_norm_out clears what it reads
+        int all[VD_MAX_BATCH];
+        for (int c = 0; c < ctx->ncase; c++) all[c] = c;
+        BatchScope every(ctx, all, ctx->ncase);
+        hipLaunchKernelGGL(k_zero<unsigned long long>, bz(ctx, grid1d(8 * NSLOT * SLOT_STRIDE, 256)), dim3(256), 0,
+                           ctx->stream, W->dnorm, (int64_t)8 * NSLOT * SLOT_STRIDE, ctx->bt);
+    }
     {
         // per-block partial norms of k_resid0 and of the fused smoother
         int64_t np = ((n[0] + 255) / 256) * (int64_t)n[1] * n[2];
@@ -2347,6 +2533,9 @@ static int get_work(vd_ctx* ctx, const vd_geom* g, const vd_box* box, int prec, 
                 }
             }
     }
+    if (ctx->ncase > 1 &&
+        (st = dalloc(W, std::max(sizeof(VTail<double>), sizeof(VTail<float>)), &W->tail_dev)))
+        goto fail;
     if (prec == 0) {
         if ((st = alloc_levels<double>(W, W->L64, true))) goto fail;
         W->L64[0].r = W->res;
@@ -2381,8 +2570,12 @@ static void launch_gsrb_bm(vd_ctx* ctx, const MGWork& W, const LevV<T>& v, T* x,
     do {                                                                                                      \
         const int zc = stream_zc(ctx, (const void*)&k_gsrb_z<T, BM, C0UV, ZIV>, GZ_X * GZ_Y, n3, ntx * nty);   \
         const dim3 g((unsigned)(ntx * nty * ((v.nz + zc - 1) / zc)));                                        \
-        hipLaunchKernelGGL((k_gsrb_z<T, BM, C0UV, ZIV>), g, dim3(GZ_X * GZ_Y), 0, ctx->stream, v, x, rhs, color, \
-                           zc, ntx, nty);                                                                     \
+        if (ctx->ncase > 1)                                                                                   \
+            hipLaunchKernelGGL((k_gsrb_z<T, BM, C0UV, ZIV, true>), bz(ctx, g), dim3(GZ_X * GZ_Y), 0, ctx->stream, v, \
+                               x, rhs, color, zc, ntx, nty, ctx->bt);                                          \
+        else                                                                                                  \
+            hipLaunchKernelGGL((k_gsrb_z<T, BM, C0UV, ZIV, false>), g, dim3(GZ_X * GZ_Y), 0, ctx->stream, v, x, rhs, \
+                               color, zc, ntx, nty, ctx->bt);                                                  \
     } while (0)
         if (v.c0uni) {
             if (zi) VD_GZ(true, true); else VD_GZ(true, false);
@@ -2396,8 +2589,8 @@ static void launch_gsrb_bm(vd_ctx* ctx, const MGWork& W, const LevV<T>& v, T* x,
     const int bx = nh > 64 ? 128 : 64;
     const int gx = (nh + bx - 1) / bx;
     dim3 g((unsigned)gx * v.ny * v.nz), b(bx);
-    if (dim == 3) hipLaunchKernelGGL((k_gsrb<T, BM, 3>), g, b, 0, ctx->stream, v, x, rhs, color, gx);
-    else hipLaunchKernelGGL((k_gsrb<T, BM, 2>), g, b, 0, ctx->stream, v, x, rhs, color, gx);
+    if (dim == 3) hipLaunchKernelGGL((k_gsrb<T, BM, 3>), bz(ctx, g), b, 0, ctx->stream, v, x, rhs, color, gx, ctx->bt);
+    else hipLaunchKernelGGL((k_gsrb<T, BM, 2>), bz(ctx, g), b, 0, ctx->stream, v, x, rhs, color, gx, ctx->bt);
 }
 
 // bm: 0 face arrays, 1 constant, 2 cell coefficient (harmonic on the fly)
@@ -2424,23 +2617,33 @@ static void launch_resid_restrict(vd_ctx* ctx, const MGWork& W, const LevV<T>& v
                                   : (const void*)&k_resid_restrict_z<T, BM, false>;
         const int zc = stream_zc(ctx, kfn, RR_X * RR_Y, Cc.n, ntx * nty);
         const dim3 g((unsigned)(ntx * nty * ((Cc.n[2] + zc - 1) / zc)));
-        if (v.c0uni)
-            hipLaunchKernelGGL((k_resid_restrict_z<T, BM, true>), g, dim3(RR_X * RR_Y), 0, ctx->stream, v, F.e, F.r,
-                               Cc.n[0], Cc.n[1], Cc.n[2], Cc.r, zc, ntx, nty);
-        else
-            hipLaunchKernelGGL((k_resid_restrict_z<T, BM, false>), g, dim3(RR_X * RR_Y), 0, ctx->stream, v, F.e,
-                               F.r, Cc.n[0], Cc.n[1], Cc.n[2], Cc.r, zc, ntx, nty);
+#define VD_RRZ(C0UV, BTV)                                                                                          \
+    hipLaunchKernelGGL((k_resid_restrict_z<T, BM, C0UV, BTV>), bz(ctx, g), dim3(RR_X * RR_Y), 0, ctx->stream, v, F.e, \
+                       F.r, Cc.n[0], Cc.n[1], Cc.n[2], Cc.r, zc, ntx, nty, ctx->bt)
+        if (ctx->ncase > 1) {
+            if (v.c0uni) VD_RRZ(true, true);
+            else VD_RRZ(false, true);
+        } else {
+            if (v.c0uni) VD_RRZ(true, false);
+            else VD_RRZ(false, false);
+        }
+#undef VD_RRZ
         return;
     }
     const int bx = Cc.n[0] > 64 ? 128 : 64;
     const int gx = (Cc.n[0] + bx - 1) / bx;
     dim3 g((unsigned)gx * Cc.n[1] * Cc.n[2]), b(bx);
-    if (dim == 3)
-        hipLaunchKernelGGL((k_resid_restrict<T, BM, 3>), g, b, 0, ctx->stream, v, F.e, F.r, Cc.n[0],
-                           Cc.n[1], Cc.n[2], Cc.r, gx);
-    else
-        hipLaunchKernelGGL((k_resid_restrict<T, BM, 2>), g, b, 0, ctx->stream, v, F.e, F.r, Cc.n[0],
-                           Cc.n[1], Cc.n[2], Cc.r, gx);
+#define VD_RR(DIMV, BTV)                                                                                        \
+    hipLaunchKernelGGL((k_resid_restrict<T, BM, DIMV, BTV>), bz(ctx, g), b, 0, ctx->stream, v, F.e, F.r, Cc.n[0], \
+                       Cc.n[1], Cc.n[2], Cc.r, gx, ctx->bt)
+    if (ctx->ncase > 1) {
+        if (dim == 3) VD_RR(3, true);
+        else VD_RR(2, true);
+    } else {
+        if (dim == 3) VD_RR(3, false);
+        else VD_RR(2, false);
+    }
+#undef VD_RR
 }
 
 
@@ -2474,11 +2677,11 @@ static void vc_down(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L
             }
             s += k;
         }
-        if (nu1 == 0) hipLaunchKernelGGL(k_zero<T>, grid1d(nc, 256), dim3(256), 0, ctx->stream, F.e, nc);
+        if (nu1 == 0) hipLaunchKernelGGL(k_zero<T>, bz(ctx, grid1d(nc, 256)), dim3(256), 0, ctx->stream, F.e, nc, ctx->bt);
     } else {
         // from a zero correction: the first colour writes e without reading it
         const bool zi = nu1 > 0 && streamed_sweep(W, F.n[0]);
-        if (!zi) hipLaunchKernelGGL(k_zero<T>, grid1d(nc, 256), dim3(256), 0, ctx->stream, F.e, nc);
+        if (!zi) hipLaunchKernelGGL(k_zero<T>, bz(ctx, grid1d(nc, 256)), dim3(256), 0, ctx->stream, F.e, nc, ctx->bt);
         for (int s = 0; s < nu1; s++) {
             launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 0, l == 0, zi && s == 0);
             launch_gsrb<T>(ctx, W, bm, v, F.e, F.r, 1, l == 0);
@@ -2495,6 +2698,7 @@ static VTail<T> tail_args(const MGWork& W, std::vector<Lev<T>>& L, double alpha,
                           int nu1, int nu2, int l0, bool* lds) {
     const int nl = W.nlev;
     VTail<T> a;
+    memset((void*)&a, 0, sizeof(a));  // defined bytes: the batched path compares them
     for (int l = l0; l < nl; l++) {
         a.v[l - l0] = view<T>(W, L[l], alpha, beta, bconst);
         a.e[l - l0] = L[l].e;
@@ -2533,12 +2737,38 @@ static void vc_tail(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L
     const VTail<T> a = tail_args<T>(W, L, alpha, beta, bconst, nu1, nu2, l0, &lds);
     // FP32 V-cycles solve their bottom level in FP64
     constexpr bool b64 = std::is_same<T, float>::value;
+    const VTail<T>* ap = nullptr;
+    if (ctx->ncase > 1) {
+        // a batched context: the cases' own copies of the arguments, made
+        // where they are missing or were made from other arguments
+        const unsigned char* kb = (const unsigned char*)&a;
+        if (W.tail_key.size() != sizeof(a) || memcmp(W.tail_key.data(), kb, sizeof(a)) != 0) {
+            W.tail_key.assign(kb, kb + sizeof(a));
+            W.tail_mask = 0;
+        }
+        int miss[VD_MAX_BATCH], nmiss = 0;
+        for (int z = 0; z < ctx->bn; z++)
+            if (!((W.tail_mask >> ctx->bt.idx[z]) & 1u)) miss[nmiss++] = ctx->bt.idx[z];
+        if (nmiss > 0) {
+            BatchScope sc(ctx, miss, nmiss);
+            hipLaunchKernelGGL(k_tail_args<T>, bz(ctx, dim3(1)), dim3(64), 0, ctx->stream, a, (VTail<T>*)W.tail_dev,
+                               ctx->bt);
+            for (int q = 0; q < nmiss; q++) W.tail_mask |= 1u << miss[q];
+        }
+        ap = (const VTail<T>*)W.tail_dev;
+    }
 #define VD_TAIL(BMV, DIMV)                                                                                        \
     {                                                                                                             \
-        if (lds)                                                                                                  \
-            hipLaunchKernelGGL((k_vcycle_tail<T, BMV, DIMV, true, b64>), dim3(1), dim3(1024), 0, ctx->stream, a);  \
+        if (ap && lds)                                                                                            \
+            hipLaunchKernelGGL((k_vcycle_tail_mem<T, BMV, DIMV, true, b64>), bz(ctx, dim3(1)), dim3(1024), 0,    \
+                               ctx->stream, ap, ctx->bt);                                                         \
+        else if (ap)                                                                                              \
+            hipLaunchKernelGGL((k_vcycle_tail_mem<T, BMV, DIMV, false, b64>), bz(ctx, dim3(1)), dim3(1024), 0,   \
+                               ctx->stream, ap, ctx->bt);                                                         \
+        else if (lds)                                                                                             \
+            hipLaunchKernelGGL((k_vcycle_tail<T, BMV, DIMV, true, b64>), dim3(1), dim3(1024), 0, ctx->stream, a); \
         else                                                                                                      \
-            hipLaunchKernelGGL((k_vcycle_tail<T, BMV, DIMV, false, b64>), dim3(1), dim3(1024), 0, ctx->stream, a); \
+            hipLaunchKernelGGL((k_vcycle_tail<T, BMV, DIMV, false, b64>), dim3(1), dim3(1024), 0, ctx->stream, a);\
     }
     if (bm == 1) {
         if (W.dim == 3) VD_TAIL(1, 3) else VD_TAIL(1, 2)
@@ -2563,18 +2793,23 @@ static bool vc_up(vd_ctx* ctx, const MGWork& W, int bm, std::vector<Lev<T>>& L, 
         const int ntx = (F.n[0] + PZ_X - 1) / PZ_X, nty = (F.n[1] + PZ_Y - 1) / PZ_Y;
         const int zc = stream_zc(ctx, (const void*)&k_prolong_z<T>, PZ_X * PZ_Y, Cc.n, ntx * nty);
         const dim3 g((unsigned)(ntx * nty * ((Cc.n[2] + zc - 1) / zc)));
-        hipLaunchKernelGGL((k_prolong_z<T>), g, dim3(PZ_X * PZ_Y), 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.e,
-                           zc, ntx, nty);
+        hipLaunchKernelGGL((k_prolong_z<T>), bz(ctx, g), dim3(PZ_X * PZ_Y), 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.e,
+                           zc, ntx, nty, ctx->bt);
     } else {
         const int bx = F.n[0] > 64 ? 128 : 64;
         const int gx = (F.n[0] + bx - 1) / bx;
         dim3 g((unsigned)gx * F.n[1] * F.n[2]), b(bx);
-        if (W.dim == 3)
-            hipLaunchKernelGGL((k_prolong<T, 3>), g, b, 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.n[2], F.e,
-                               gx);
-        else
-            hipLaunchKernelGGL((k_prolong<T, 2>), g, b, 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.n[2], F.e,
-                               gx);
+#define VD_PR(DIMV, BTV)                                                                                         \
+    hipLaunchKernelGGL((k_prolong<T, DIMV, BTV>), bz(ctx, g), b, 0, ctx->stream, vc, Cc.e, F.n[0], F.n[1], F.n[2], F.e, \
+                       gx, ctx->bt)
+        if (ctx->ncase > 1) {
+            if (W.dim == 3) VD_PR(3, true);
+            else VD_PR(2, true);
+        } else {
+            if (W.dim == 3) VD_PR(3, false);
+            else VD_PR(2, false);
+        }
+#undef VD_PR
     }
     if (F.e2 && W.c0uni && bm != 2) {
         for (int s = 0; s < nu2;) {
@@ -2635,15 +2870,15 @@ static void coarsen_hierarchy(vd_ctx* ctx, MGWork& W, std::vector<Lev<T>>& L, bo
         for (int d = 0; d < W.dim; d++) {
             if (faces && !W.constb) {
                 int64_t nf = nfacef(Cc.n, d);
-                hipLaunchKernelGGL(k_coarsen_faces<T>, grid1d(nf, 256), dim3(256), 0, ctx->stream,
-                                   F.bf[d], F.n[0], F.n[1], d, W.dim, Cc.n[0], Cc.n[1], Cc.n[2], Cc.bf[d]);
+                hipLaunchKernelGGL(k_coarsen_faces<T>, bz(ctx, grid1d(nf, 256)), dim3(256), 0, ctx->stream,
+                                   F.bf[d], F.n[0], F.n[1], d, W.dim, Cc.n[0], Cc.n[1], Cc.n[2], Cc.bf[d], ctx->bt);
             }
             if (c0 && !W.per[d])
                 for (int s = 0; s < 2; s++) {
                     int64_t nt = ntanf(Cc.n, d);
-                    hipLaunchKernelGGL(k_coarsen_c0<T>, grid1d(nt, 256), dim3(256), 0, ctx->stream,
+                    hipLaunchKernelGGL(k_coarsen_c0<T>, bz(ctx, grid1d(nt, 256)), dim3(256), 0, ctx->stream,
                                        F.c0[d][s], F.n[0], F.n[1], d, W.dim, Cc.n[0], Cc.n[1], Cc.n[2],
-                                       Cc.c0[d][s]);
+                                       Cc.c0[d][s], ctx->bt);
                 }
         }
     }
@@ -2663,19 +2898,27 @@ static inline unsigned long long* slot_group(MGWork& W, int q) { return W.dnorm 
 // maximum as doubles is exact.  Each slot is cleared by the lane that read
 // it, so the groups start at zero for the next norm (every accumulation
 // into slot groups is read back by exactly one read_groups call).
-constexpr int HNORM_SEQ = 63;
-__global__ void __launch_bounds__(64) k_norm_out(unsigned long long* __restrict__ slots, int ngroups,
-                                                 double* __restrict__ hout, unsigned long long seq) {
+// In a batched context the wave does this for each case of the launch
+// batch in turn (case z's groups at hout[z * ngroups ..]), so that the
+// sequence number follows every case's values.
+constexpr int HNORM_SEQ = HNORM_DOUBLES - 1;
+static_assert(VD_MAX_BATCH * 5 < HNORM_SEQ, "norm read-back buffer too small");
+__global__ void __launch_bounds__(64) k_norm_out(unsigned long long* __restrict__ slots0, int ngroups,
+                                                 double* __restrict__ hout, unsigned long long seq, Batch bt,
+                                                 int ncase) {
     const int s = (int)threadIdx.x;
-    for (int q = 0; q < ngroups; q++) {
-        double v = 0.0;
-        if (s < NSLOT) {
-            const size_t o = ((size_t)q * NSLOT + s) * SLOT_STRIDE;
-            v = __longlong_as_double((long long)slots[o]);
-            slots[o] = 0ull;
+    for (int z = 0; z < ncase; z++) {
+        unsigned long long* __restrict__ slots = rb(slots0, bt.stride * (int64_t)bt.idx[z]);
+        for (int q = 0; q < ngroups; q++) {
+            double v = 0.0;
+            if (s < NSLOT) {
+                const size_t o = ((size_t)q * NSLOT + s) * SLOT_STRIDE;
+                v = __longlong_as_double((long long)slots[o]);
+                slots[o] = 0ull;
+            }
+            for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+            if (s == 0) hout[z * ngroups + q] = v;
         }
-        for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
-        if (s == 0) hout[q] = v;
     }
     if (s == 0) {
         __threadfence_system();
@@ -2686,7 +2929,9 @@ __global__ void __launch_bounds__(64) k_norm_out(unsigned long long* __restrict_
 // max |a| over n values into a slot group (grid-stride, 2 values per
 // thread and step)
 __global__ void __launch_bounds__(256) k_absmax(const double* __restrict__ a, int64_t n,
-                                                unsigned long long* __restrict__ slots) {
+                                                unsigned long long* __restrict__ slots, Batch bt) {
+    a = rb(a, case_off(bt));
+    slots = rb(slots, case_off(bt));
     double m = 0.0;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x * 2;
     for (int64_t i = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * 2; i < n; i += stride) {
@@ -2696,12 +2941,14 @@ __global__ void __launch_bounds__(256) k_absmax(const double* __restrict__ a, in
     block_max_atomic(m, slots);
 }
 
-// the norms of up to 5 slot groups on the host: the host waits for the
-// sequence number k_norm_out writes last, rather than for a copy and a
-// stream synchronisation, and checks the stream for errors while it waits
+// the norms of up to 5 slot groups on the host, for each case of the
+// launch batch (out[z * ngroups + q]): the host waits for the sequence
+// number k_norm_out writes last, rather than for a copy and a stream
+// synchronisation, and checks the stream for errors while it waits
 static int read_groups(vd_ctx* ctx, MGWork& W, int ngroups, double* out) {
     const unsigned long long seq = ++ctx->norm_seq;
-    hipLaunchKernelGGL(k_norm_out, dim3(1), dim3(64), 0, ctx->stream, W.dnorm, ngroups, ctx->hnorm_d, seq);
+    hipLaunchKernelGGL(k_norm_out, dim3(1), dim3(64), 0, ctx->stream, W.dnorm, ngroups, ctx->hnorm_d, seq, ctx->bt,
+                       ctx->bn);
     VD_CHECK_LAUNCH();
     volatile unsigned long long* flag = (volatile unsigned long long*)(ctx->hnorm + HNORM_SEQ);
     for (unsigned long it = 1; *flag != seq; it++) {
@@ -2724,14 +2971,15 @@ static int read_groups(vd_ctx* ctx, MGWork& W, int ngroups, double* out) {
         __builtin_ia32_pause();
     }
     std::atomic_thread_fence(std::memory_order_acquire);
-    for (int q = 0; q < ngroups; q++) out[q] = ((volatile double*)ctx->hnorm)[q];
+    for (int q = 0; q < ngroups * ctx->bn; q++) out[q] = ((volatile double*)ctx->hnorm)[q];
     return VD_OK;
 }
 
 // residual of the finest level into W.res (unless write_res is false) and
 // its max-norm; with bnorm also the max-norm of f (the rhs with boundary
 // data), with fmax (cell coefficient, bm 2) the max face coefficient per
-// direction -- all from the one pass over the level
+// direction -- all from the one pass over the level.  Per case of the
+// launch batch: nrm_out[z], bnorm[z], fmax[3 z + d].
 // With res32 the residual goes straight into res32 in FP32 instead, where
 // the kernel allows (*in32 says whether it did).
 static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double* nrm_out,
@@ -2755,12 +3003,12 @@ static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double
         const int zc = stream_zc(ctx, (const void*)&k_resid_z2<BNV, C0UV>, RZ2_X * RZ2_Y, W.n, ntx * nty);    \
         const dim3 gz((unsigned)(ntx * nty * ((W.n[2] + zc - 1) / zc)));                                      \
         if (res && res32) {                                                                                   \
-            hipLaunchKernelGGL((k_resid_z2<BNV, C0UV, float>), gz, dim3(RZ2_X * RZ2_Y), 0, ctx->stream, v0,   \
-                               W.x, W.f, res32, zc, ntx, nty, sl);                                            \
+            hipLaunchKernelGGL((k_resid_z2<BNV, C0UV, float>), bz(ctx, gz), dim3(RZ2_X * RZ2_Y), 0, ctx->stream, v0,   \
+                               W.x, W.f, res32, zc, ntx, nty, sl, ctx->bt);                                            \
             if (in32) *in32 = true;                                                                           \
         } else {                                                                                              \
-            hipLaunchKernelGGL((k_resid_z2<BNV, C0UV>), gz, dim3(RZ2_X * RZ2_Y), 0, ctx->stream, v0, W.x,     \
-                               W.f, res, zc, ntx, nty, sl);                                                   \
+            hipLaunchKernelGGL((k_resid_z2<BNV, C0UV>), bz(ctx, gz), dim3(RZ2_X * RZ2_Y), 0, ctx->stream, v0, W.x,     \
+                               W.f, res, zc, ntx, nty, sl, ctx->bt);                                                   \
         }                                                                                                     \
     }
         if (bnorm) {
@@ -2779,11 +3027,11 @@ static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double
                                  RZ_X * RZ_Y, W.n, ntx * nty);                                                \
         const dim3 gz((unsigned)(ntx * nty * ((W.n[2] + zc - 1) / zc)));                                      \
         if (v0.c0uni)                                                                                         \
-            hipLaunchKernelGGL((k_resid_z<BMV, BNV, FMV, true>), gz, dim3(RZ_X * RZ_Y), 0, ctx->stream, v0,   \
-                               W.x, W.f, res, zc, ntx, nty, sl);                                              \
+            hipLaunchKernelGGL((k_resid_z<BMV, BNV, FMV, true>), bz(ctx, gz), dim3(RZ_X * RZ_Y), 0, ctx->stream, v0,   \
+                               W.x, W.f, res, zc, ntx, nty, sl, ctx->bt);                                              \
         else                                                                                                  \
-            hipLaunchKernelGGL((k_resid_z<BMV, BNV, FMV, false>), gz, dim3(RZ_X * RZ_Y), 0, ctx->stream, v0,  \
-                               W.x, W.f, res, zc, ntx, nty, sl);                                              \
+            hipLaunchKernelGGL((k_resid_z<BMV, BNV, FMV, false>), bz(ctx, gz), dim3(RZ_X * RZ_Y), 0, ctx->stream, v0,  \
+                               W.x, W.f, res, zc, ntx, nty, sl, ctx->bt);                                              \
     }
 #define VD_RESIDZ(BMV)                                                                                        \
     {                                                                                                         \
@@ -2806,14 +3054,14 @@ static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double
 #define VD_RESID(BMV, DIMV)                                                                                   \
     {                                                                                                         \
         if (fmax && bnorm)                                                                                    \
-            hipLaunchKernelGGL((k_resid0<BMV, DIMV, true, true>), g, b, 0, ctx->stream, v0, W.x, W.f, res,     \
-                               part, gx, bs, fs);                                                             \
+            hipLaunchKernelGGL((k_resid0<BMV, DIMV, true, true>), bz(ctx, g), b, 0, ctx->stream, v0, W.x, W.f, res,     \
+                               part, gx, bs, fs, ctx->bt);                                                             \
         else if (bnorm)                                                                                       \
-            hipLaunchKernelGGL((k_resid0<BMV, DIMV, true, false>), g, b, 0, ctx->stream, v0, W.x, W.f, res,    \
-                               part, gx, bs, fs);                                                             \
+            hipLaunchKernelGGL((k_resid0<BMV, DIMV, true, false>), bz(ctx, g), b, 0, ctx->stream, v0, W.x, W.f, res,    \
+                               part, gx, bs, fs, ctx->bt);                                                             \
         else                                                                                                  \
-            hipLaunchKernelGGL((k_resid0<BMV, DIMV, false, false>), g, b, 0, ctx->stream, v0, W.x, W.f, res,   \
-                               part, gx, bs, fs);                                                             \
+            hipLaunchKernelGGL((k_resid0<BMV, DIMV, false, false>), bz(ctx, g), b, 0, ctx->stream, v0, W.x, W.f, res,   \
+                               part, gx, bs, fs, ctx->bt);                                                             \
     }
         if (bm == 1) {
             if (W.dim == 3) VD_RESID(1, 3) else VD_RESID(1, 2)
@@ -2823,18 +3071,20 @@ static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double
             if (W.dim == 3) VD_RESID(2, 3) else VD_RESID(2, 2)
         }
 #undef VD_RESID
-        hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, ctx->stream, part, nblk, sl);
+        hipLaunchKernelGGL(k_reduce_partials, bz(ctx, dim3(1)), dim3(1024), 0, ctx->stream, part, nblk, sl, ctx->bt);
     }
     const int nb = bm == 1 ? 0 : (bm == 0 ? W.dim : 1);
     kt_end(ctx, VD_KT_RESID, 8.0 * (double)nc * ((write_res ? 4 : 3) + nb));
     VD_CHECK_LAUNCH();
-    double m[5];
+    double m[VD_MAX_BATCH * 5];
     const int ng = fmax ? 2 + W.dim : (bnorm ? 2 : 1);
     VD_TRY(read_groups(ctx, W, ng, m));
-    *nrm_out = m[0];
-    if (bnorm) *bnorm = m[1];
-    if (fmax)
-        for (int d = 0; d < W.dim; d++) fmax[d] = m[2 + d];
+    for (int z = 0; z < ctx->bn; z++) {
+        nrm_out[z] = m[z * ng];
+        if (bnorm) bnorm[z] = m[z * ng + 1];
+        if (fmax)
+            for (int d = 0; d < W.dim; d++) fmax[3 * z + d] = m[z * ng + 2 + d];
+    }
     return VD_OK;
 }
 
@@ -2843,7 +3093,10 @@ static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double
 // a norm slot group (Jacobi bound of the implicit operator)
 __global__ void __launch_bounds__(256) k_harmonic_faces(const double* __restrict__ dcell, int nx, int ny,
                                                         int nz, int d, int dim, double* __restrict__ bf,
-                                                        unsigned long long* __restrict__ slots) {
+                                                        unsigned long long* __restrict__ slots, Batch bt) {
+    dcell = rb(dcell, case_off(bt));
+    bf = rb(bf, case_off(bt));
+    slots = rb(slots, case_off(bt));
     int n0 = nx + (d == 0), n1 = ny + (d == 1), n2 = nz + (d == 2);
     unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
     double m = 0.0;
@@ -2877,8 +3130,8 @@ static int jacobi_rho(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
     } else {
         for (int d = 0; d < W.dim; d++) {
             int64_t nf = nfacef(W.n, d);
-            hipLaunchKernelGGL(k_maxabs, norm_grid(nf), dim3(256), 0, ctx->stream, W.bf0[d], (unsigned)nf,
-                               slot_group(W, d));
+            hipLaunchKernelGGL(k_maxabs, bz(ctx, norm_grid(nf)), dim3(256), 0, ctx->stream, W.bf0[d], (unsigned)nf,
+                               slot_group(W, d), ctx->bt);
         }
         VD_CHECK_LAUNCH();
         VD_TRY(read_groups(ctx, W, W.dim, bmax));
@@ -2920,137 +3173,235 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
     }
     LevV<double> v0 = view<double>(W, fine, alpha, beta, bconst);
     v0.dcell = dcell;
-    double r0, rn, bnorm;
-    bool smooth_only = smooth_mode == 1;
+    // Every case of the launch batch (ctx->bt, ctx->bn) takes the
+    // operations it would take alone: its own initial norms, its own
+    // choice of finest-level sweeps or V-cycles, its own stopping
+    // iteration.  Launches run on the cases that are still iterating.
+    const int ncs = ctx->bn;
+    int cid[VD_MAX_BATCH];
+    for (int z = 0; z < ncs; z++) cid[z] = ctx->bt.idx[z];
+    double r0[VD_MAX_BATCH], rn[VD_MAX_BATCH], bnorm[VD_MAX_BATCH], target[VD_MAX_BATCH];
+    bool sonly[VD_MAX_BATCH];
+    int iters[VD_MAX_BATCH];
+    for (int z = 0; z < ncs; z++) {
+        sonly[z] = smooth_mode == 1;
+        iters[z] = 0;
+    }
+    bool initial_done = false;
     if (smooth_mode == 2 && bm == 1) {
         // constant coefficient: every face is b, so the bound needs no pass,
         // and the residual is stored only when V-cycles follow
         double rho = 0.0;
         for (int d = 0; d < W.dim; d++) rho += 2.0 * fabs(beta / (W.dx[d] * W.dx[d])) * fabs(bconst);
-        smooth_only = alpha != 0.0 && rho / fabs(alpha) <= SMOOTH_ONLY_RHO;
-        VD_TRY(resid0(ctx, W, bm, v0, &r0, &bnorm, !smooth_only));
+        const bool so = alpha != 0.0 && rho / fabs(alpha) <= SMOOTH_ONLY_RHO;
+        for (int z = 0; z < ncs; z++) sonly[z] = so;
+        VD_TRY(resid0(ctx, W, bm, v0, r0, bnorm, !so));
+        initial_done = true;
     } else if (smooth_mode == 2) {
-        double bmax[3] = {0.0, 0.0, 0.0};
-        VD_TRY(resid0(ctx, W, bm, v0, &r0, &bnorm, true, bmax));
-        if (alpha != 0.0) {
-            double rho = 0.0;
-            for (int d = 0; d < W.dim; d++) rho += 2.0 * fabs(beta / (W.dx[d] * W.dx[d])) * bmax[d];
-            smooth_only = rho / fabs(alpha) <= SMOOTH_ONLY_RHO;
-        }
-        if (!smooth_only && bm == 2) {
-            for (int d = 0; d < W.dim; d++)
-                hipLaunchKernelGGL(k_harmonic_faces, grid1d(nfacef(W.n, d), 256), dim3(256), 0, ctx->stream, dcell,
-                                   W.n[0], W.n[1], W.n[2], d, W.dim, W.bf0[d], (unsigned long long*)nullptr);
-            bm = 0;
-        }
-    }
-    if (!smooth_only) {
-        // coarse c0 depends only on the BC types unless a side is Robin
-        bool robin = false, same = W.c0_valid;
-        for (int d = 0; d < W.dim; d++) {
-            if (W.per[d]) continue;
-            robin |= tlo[d] == VD_ROBINBC || thi[d] == VD_ROBINBC;
-            same &= W.c0_types[d][0] == tlo[d] && W.c0_types[d][1] == thi[d];
-        }
-        const bool c0 = robin || !same;
-        if (W.prec == 0) {
-            coarsen_hierarchy<double>(ctx, W, W.L64, true, c0);
-        } else {
-            Lev<float>& L0 = W.L32[0];
-            for (int d = 0; d < W.dim; d++) {
-                if (!W.constb) {
-                    int64_t nf = nfacef(W.n, d);
-                    hipLaunchKernelGGL((k_convert<double, float>), grid1d(nf, 256), dim3(256), 0, ctx->stream,
-                                       W.bf0[d], L0.bf[d], nf);
-                }
-                if (c0 && !W.per[d])
-                    for (int s = 0; s < 2; s++) {
-                        int64_t nt = ntanf(W.n, d);
-                        hipLaunchKernelGGL((k_convert<double, float>), grid1d(nt, 256), dim3(256), 0,
-                                           ctx->stream, W.c00[d][s], L0.c0[d][s], nt);
-                    }
+        double bmax[3 * VD_MAX_BATCH] = {0.0};
+        VD_TRY(resid0(ctx, W, bm, v0, r0, bnorm, true, bmax));
+        for (int z = 0; z < ncs; z++) {
+            sonly[z] = false;
+            if (alpha != 0.0) {
+                double rho = 0.0;
+                for (int d = 0; d < W.dim; d++) rho += 2.0 * fabs(beta / (W.dx[d] * W.dx[d])) * bmax[3 * z + d];
+                sonly[z] = rho / fabs(alpha) <= SMOOTH_ONLY_RHO;
             }
-            coarsen_hierarchy<float>(ctx, W, W.L32, true, c0);
-            // the FP64 bottom solve's operator, coarsened in FP64
-            coarsen_hierarchy<double>(ctx, W, W.C64, true, c0);
         }
-        for (int d = 0; d < 3; d++) {
-            W.c0_types[d][0] = tlo[d];
-            W.c0_types[d][1] = thi[d];
-        }
-        W.c0_valid = !robin;
-        VD_CHECK_LAUNCH();
+        initial_done = true;
     }
-    // the first V-cycle's finest rhs: the stored residual, or f when x is 0;
-    // for FP32 V-cycles the residual pass may store it in FP32 directly
-    bool rhs_is_f = false, r_in32 = false;
-    float* r32p = W.prec == 1 && !W.L32.empty() ? W.L32[0].r : nullptr;
-    if (smooth_mode != 2) {
-        if (x_zero && !smooth_only) {
-            // r = f - L 0 = f bit for bit (every stencil term is 0)
-            hipLaunchKernelGGL(k_absmax, dim3(NORM_BLOCKS), dim3(256), 0, ctx->stream, W.f, nc, slot_group(W, 0));
+    // the V-cycle cases first (their solution stays in W.x), then the
+    // sweep-only ones (whose fused path alternates W.x and W.x2)
+    int st = VD_OK;
+    for (int pass = 0; pass < 2; pass++) {
+        const bool smooth_only = pass == 1;
+        int grp[VD_MAX_BATCH], gz[VD_MAX_BATCH], ng = 0;  // case ids and their z in the caller's batch
+        for (int z = 0; z < ncs; z++)
+            if (sonly[z] == smooth_only) {
+                gz[ng] = z;
+                grp[ng++] = cid[z];
+            }
+        if (ng == 0) continue;
+        BatchScope scope(ctx, grp, ng);
+        int bmg = bm;
+        if (!smooth_only && smooth_mode == 2 && bm == 2) {
+            // V-cycles on the harmonic face arrays of the cell coefficient
+            for (int d = 0; d < W.dim; d++)
+                hipLaunchKernelGGL(k_harmonic_faces, bz(ctx, grid1d(nfacef(W.n, d), 256)), dim3(256), 0, ctx->stream,
+                                   dcell, W.n[0], W.n[1], W.n[2], d, W.dim, W.bf0[d], (unsigned long long*)nullptr,
+                                   ctx->bt);
+            bmg = 0;
+        }
+        if (!smooth_only) {
+            // coarse c0 depends only on the BC types unless a side is Robin:
+            // rebuilt when a side is Robin, the types changed, or a case of
+            // this group has not had it built for these types
+            bool robin = false, same = true;
+            for (int d = 0; d < W.dim; d++) {
+                if (W.per[d]) continue;
+                robin |= tlo[d] == VD_ROBINBC || thi[d] == VD_ROBINBC;
+                same &= W.c0_types[d][0] == tlo[d] && W.c0_types[d][1] == thi[d];
+            }
+            unsigned gmask = 0;
+            for (int q = 0; q < ng; q++) gmask |= 1u << grp[q];
+            const bool c0 = robin || !same || (W.c0_mask & gmask) != gmask;
+            if (W.prec == 0) {
+                coarsen_hierarchy<double>(ctx, W, W.L64, true, c0);
+            } else {
+                Lev<float>& L0 = W.L32[0];
+                for (int d = 0; d < W.dim; d++) {
+                    if (!W.constb) {
+                        int64_t nf = nfacef(W.n, d);
+                        hipLaunchKernelGGL((k_convert<double, float>), bz(ctx, grid1d(nf, 256)), dim3(256), 0,
+                                           ctx->stream, W.bf0[d], L0.bf[d], nf, ctx->bt);
+                    }
+                    if (c0 && !W.per[d])
+                        for (int s = 0; s < 2; s++) {
+                            int64_t nt = ntanf(W.n, d);
+                            hipLaunchKernelGGL((k_convert<double, float>), bz(ctx, grid1d(nt, 256)), dim3(256), 0,
+                                               ctx->stream, W.c00[d][s], L0.c0[d][s], nt, ctx->bt);
+                        }
+                }
+                coarsen_hierarchy<float>(ctx, W, W.L32, true, c0);
+                // the FP64 bottom solve's operator, coarsened in FP64
+                coarsen_hierarchy<double>(ctx, W, W.C64, true, c0);
+            }
+            if (c0) {
+                if (!same) W.c0_mask = 0;
+                if (robin) W.c0_mask &= ~gmask;
+                else W.c0_mask |= gmask;
+            }
+            for (int d = 0; d < 3; d++) {
+                W.c0_types[d][0] = tlo[d];
+                W.c0_types[d][1] = thi[d];
+            }
             VD_CHECK_LAUNCH();
-            VD_TRY(read_groups(ctx, W, 1, &bnorm));
-            r0 = bnorm;
-            rhs_is_f = true;
-        } else {
-            VD_TRY(resid0(ctx, W, bm, v0, &r0, &bnorm, !smooth_only, nullptr, r32p, &r_in32));
         }
-    }
-    double target = std::max(abstol, reltol * std::max(bnorm, r0));
-    rn = r0;
-    int it = 0, st = VD_OK;
-    if (!(r0 <= target)) {
-        for (it = 1; it <= maxiter; it++) {
-            if (smooth_only && W.x2 && W.c0uni && (bm != 2 || fused_cellcoef(W))) {
+        // the first V-cycle's finest rhs: the stored residual, or f when x is 0;
+        // for FP32 V-cycles the residual pass may store it in FP32 directly
+        bool rhs_is_f = false;
+        float* r32p = W.prec == 1 && !W.L32.empty() ? W.L32[0].r : nullptr;
+        double gr0[VD_MAX_BATCH], gb[VD_MAX_BATCH];
+        bool r_in32 = false;
+        if (!initial_done) {
+            if (x_zero && !smooth_only) {
+                // r = f - L 0 = f bit for bit (every stencil term is 0)
+                hipLaunchKernelGGL(k_absmax, bz(ctx, dim3(NORM_BLOCKS)), dim3(256), 0, ctx->stream, W.f, nc,
+                                   slot_group(W, 0), ctx->bt);
+                VD_CHECK_LAUNCH();
+                VD_TRY(read_groups(ctx, W, 1, gb));
+                for (int q = 0; q < ng; q++) gr0[q] = gb[q];
+                rhs_is_f = true;
+            } else {
+                VD_TRY(resid0(ctx, W, bmg, v0, gr0, gb, !smooth_only, nullptr, r32p, &r_in32));
+            }
+            for (int q = 0; q < ng; q++) {
+                r0[gz[q]] = gr0[q];
+                bnorm[gz[q]] = gb[q];
+            }
+        }
+        // the cases still iterating (ids, and their z in the caller's batch)
+        int act[VD_MAX_BATCH], az[VD_MAX_BATCH], na = 0;
+        for (int q = 0; q < ng; q++) {
+            const int z = gz[q];
+            target[z] = std::max(abstol, reltol * std::max(bnorm[z], r0[z]));
+            rn[z] = r0[z];
+            if (!(r0[z] <= target[z])) {
+                act[na] = grp[q];
+                az[na++] = z;
+            }
+        }
+        // fused sweep-only path: where each case's solution ended up
+        double* xfinal[VD_MAX_BATCH];
+        for (int q = 0; q < ng; q++) xfinal[gz[q]] = W.x;
+        const bool fused_so =
+            smooth_only && W.x2 && W.c0uni && (bmg != 2 || fused_cellcoef(W));
+        for (int it = 1; it <= maxiter && na > 0; it++) {
+            BatchScope iscope(ctx, act, na);
+            double m[VD_MAX_BATCH];
+            if (fused_so) {
                 // one red-black sweep and the residual norm of its result in one pass
                 int nblk = 0;
-                launch_fused<double>(ctx, W, bm, v0, bm == 2 ? fused_zc_cc(W.n) : std::max(1, fused_zc(W.n)), W.x,
-                                     W.x2, W.f, 1, 1, W.part,
-                                     &nblk);
+                launch_fused<double>(ctx, W, bmg, v0, bmg == 2 ? fused_zc_cc(W.n) : std::max(1, fused_zc(W.n)), W.x,
+                                     W.x2, W.f, 1, 1, W.part, &nblk);
                 std::swap(W.x, W.x2);
-                hipLaunchKernelGGL(k_reduce_partials, dim3(1), dim3(1024), 0, ctx->stream, W.part, nblk,
-                                   slot_group(W, 0));
+                hipLaunchKernelGGL(k_reduce_partials, bz(ctx, dim3(1)), dim3(1024), 0, ctx->stream, W.part, nblk,
+                                   slot_group(W, 0), ctx->bt);
                 VD_CHECK_LAUNCH();
-                VD_TRY(read_groups(ctx, W, 1, &rn));
-                if (rn <= target) break;
-                continue;
-            }
-            if (smooth_only) {
-                launch_gsrb<double>(ctx, W, bm, v0, W.x, W.f, 0, true);
-                launch_gsrb<double>(ctx, W, bm, v0, W.x, W.f, 1, true);
-            } else if (W.prec == 0) {
-                if (rhs_is_f) W.L64[0].r = W.f;  // read only by the V-cycle
-                const bool acc = vcycle<double>(ctx, W, bm, W.L64, alpha, beta, bconst, nu1, nu2, W.x);
-                W.L64[0].r = W.res;
-                if (!acc)
-                    hipLaunchKernelGGL(k_axpy_to_double<double>, grid1d(nc, 256), dim3(256), 0, ctx->stream,
-                                       W.x, W.L64[0].e, nc);
+                VD_TRY(read_groups(ctx, W, 1, m));
             } else {
-                if (!r_in32)
-                    hipLaunchKernelGGL((k_convert<double, float>), grid1d(nc, 256), dim3(256), 0, ctx->stream,
-                                       rhs_is_f ? W.f : W.res, W.L32[0].r, nc);
-                if (!vcycle<float>(ctx, W, bm, W.L32, alpha, beta, bconst, nu1, nu2, W.x))
-                    hipLaunchKernelGGL(k_axpy_to_double<float>, grid1d(nc, 256), dim3(256), 0, ctx->stream,
-                                       W.x, W.L32[0].e, nc);
+                if (smooth_only) {
+                    launch_gsrb<double>(ctx, W, bmg, v0, W.x, W.f, 0, true);
+                    launch_gsrb<double>(ctx, W, bmg, v0, W.x, W.f, 1, true);
+                } else if (W.prec == 0) {
+                    if (rhs_is_f) W.L64[0].r = W.f;  // read only by the V-cycle
+                    const bool acc = vcycle<double>(ctx, W, bmg, W.L64, alpha, beta, bconst, nu1, nu2, W.x);
+                    W.L64[0].r = W.res;
+                    if (!acc)
+                        hipLaunchKernelGGL(k_axpy_to_double<double>, bz(ctx, grid1d(nc, 256)), dim3(256), 0,
+                                           ctx->stream, W.x, W.L64[0].e, nc, ctx->bt);
+                } else {
+                    if (!r_in32)
+                        hipLaunchKernelGGL((k_convert<double, float>), bz(ctx, grid1d(nc, 256)), dim3(256), 0,
+                                           ctx->stream, rhs_is_f ? W.f : W.res, W.L32[0].r, nc, ctx->bt);
+                    if (!vcycle<float>(ctx, W, bmg, W.L32, alpha, beta, bconst, nu1, nu2, W.x))
+                        hipLaunchKernelGGL(k_axpy_to_double<float>, bz(ctx, grid1d(nc, 256)), dim3(256), 0,
+                                           ctx->stream, W.x, W.L32[0].e, nc, ctx->bt);
+                }
+                VD_CHECK_LAUNCH();
+                rhs_is_f = false;
+                VD_TRY(resid0(ctx, W, bmg, v0, m, nullptr, !smooth_only, nullptr, r32p, &r_in32));
             }
-            VD_CHECK_LAUNCH();
-            rhs_is_f = false;
-            VD_TRY(resid0(ctx, W, bm, v0, &rn, nullptr, !smooth_only, nullptr, r32p, &r_in32));
-            if (rn <= target) break;
+            int nna = 0;
+            for (int q = 0; q < na; q++) {
+                const int z = az[q];
+                rn[z] = m[q];
+                xfinal[z] = W.x;
+                if (rn[z] <= target[z]) {
+                    iters[z] = it;
+                } else {
+                    act[nna] = act[q];
+                    az[nna++] = z;
+                }
+            }
+            na = nna;
         }
-        if (it > maxiter) {
-            it = maxiter;
+        for (int q = 0; q < na; q++) {
+            const int z = az[q];
+            iters[z] = maxiter;
             st = VD_ENOCONV;
-            set_error("multigrid: no convergence in %d iterations (|r|=%g target=%g)", maxiter, rn,
-                      target);
+            set_error("multigrid: no convergence in %d iterations (|r|=%g target=%g)", maxiter, rn[z], target[z]);
+        }
+        if (fused_so) {
+            // the cases that stopped with their solution in the other
+            // buffer copy it into W.x, where every case's solution is read
+            int cp[VD_MAX_BATCH], ncp = 0;
+            for (int q = 0; q < ng; q++)
+                if (xfinal[gz[q]] != W.x) cp[ncp++] = grp[q];
+            if (ncp > 0) {
+                BatchScope cscope(ctx, cp, ncp);
+                hipLaunchKernelGGL((k_convert<double, double>), bz(ctx, grid1d(nc, 256)), dim3(256), 0, ctx->stream,
+                                   W.x2, W.x, nc, ctx->bt);
+                VD_CHECK_LAUNCH();
+            }
         }
     }
+    // per case, and for the call: the first case's figures with the most
+    // iterations of any case
+    int imax = 0;
+    for (int z = 0; z < ncs; z++) {
+        vd_solve_info& ci = ctx->case_info[cid[z]];
+        ci.iterations = iters[z];
+        ci.resnorm0 = r0[z];
+        ci.resnorm = rn[z];
+        ci.target = target[z];
+        imax = std::max(imax, iters[z]);
+    }
     if (info) {
-        info->iterations = it;
-        info->resnorm0 = r0;
-        info->resnorm = rn;
-        info->target = target;
+        info->iterations = imax;
+        info->resnorm0 = r0[0];
+        info->resnorm = rn[0];
+        info->target = target[0];
     }
     return st;
 }
@@ -3122,6 +3473,17 @@ struct GradArgs {
     const double* ra[2];
     const double* rb[2];
     const double* rf[2];
+    __device__ void rebase(int64_t o) {
+        x = vd::rb(x, o);
+        vd::rebase(out, o);
+#pragma unroll
+        for (int s = 0; s < 2; s++) {
+            bcv[s] = vd::rb(bcv[s], o);
+            ra[s] = vd::rb(ra[s], o);
+            rb[s] = vd::rb(rb[s], o);
+            rf[s] = vd::rb(rf[s], o);
+        }
+    }
 };
 
 // the gradient on face (i, j, k) of direction a.d (face f[d] lies between
@@ -3172,7 +3534,8 @@ __device__ __forceinline__ double grad_face(const GradArgs& a, int i, int j, int
     return gv;
 }
 
-__global__ void k_grad_faces(GradArgs a) {
+__global__ void k_grad_faces(GradArgs a, Batch bt) {
+    a.rebase(case_off(bt));
     int n0 = a.nx + (a.d == 0), n1 = a.ny + (a.d == 1), n2 = a.nz + (a.d == 2);
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)n0 * n1 * n2) return;
@@ -3213,7 +3576,7 @@ static int grad_faces(vd_ctx* ctx, MGWork& W, const int tlo[3], const int thi[3]
     for (int d = 0; d < W.dim; d++) {
         GradArgs a = grad_args(W, tlo, thi, &out[d], d, box, sign);
         int64_t nf = nfacef(W.n, d);
-        hipLaunchKernelGGL(k_grad_faces, grid1d(nf, 256), dim3(256), 0, ctx->stream, a);
+        hipLaunchKernelGGL(k_grad_faces, bz(ctx, grid1d(nf, 256)), dim3(256), 0, ctx->stream, a, ctx->bt);
     }
     VD_CHECK_LAUNCH();
     return VD_OK;
@@ -3253,10 +3616,10 @@ static int side_setup(vd_ctx* ctx, MGWork& W, const int tlo[3], const int thi[3]
         dim3 gs = grid1d(ntanf(W.n, d), 256);
         if (W.n[d] > 1) {
             gs.y = 2;
-            hipLaunchKernelGGL(k_side_setup, gs, dim3(256), 0, ctx->stream, sa[0], sa[1]);
+            hipLaunchKernelGGL(k_side_setup, bz(ctx, gs), dim3(256), 0, ctx->stream, sa[0], sa[1], ctx->bt);
         } else {
-            hipLaunchKernelGGL(k_side_setup, gs, dim3(256), 0, ctx->stream, sa[0], sa[0]);
-            hipLaunchKernelGGL(k_side_setup, gs, dim3(256), 0, ctx->stream, sa[1], sa[1]);
+            hipLaunchKernelGGL(k_side_setup, bz(ctx, gs), dim3(256), 0, ctx->stream, sa[0], sa[0], ctx->bt);
+            hipLaunchKernelGGL(k_side_setup, bz(ctx, gs), dim3(256), 0, ctx->stream, sa[1], sa[1], ctx->bt);
         }
     }
     VD_CHECK_LAUNCH();
@@ -3281,10 +3644,16 @@ struct PotRhsArgs {
     Arr<double> sb;
     int3 lo;
     int nx, ny, nz, spacechrg;
+    __device__ void rebase(int64_t o) {
+        M = vd::rb(M, o);
+        vd::rebase(sb, o);
+    }
 };
 
 // rhs = -e/eps0 * sum z n (PotentialSolve.cpp:203-223)
-__global__ void k_pot_rhs(PotRhsArgs a, double* __restrict__ f) {
+__global__ void k_pot_rhs(PotRhsArgs a, double* __restrict__ f, Batch bt) {
+    a.rebase(case_off(bt));
+    f = rb(f, case_off(bt));
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.nx * a.ny * a.nz) return;
     double r = 0.0;
@@ -3308,6 +3677,12 @@ struct PotPostArgs {
     int ns, dim;
     int3 lo;
     int nx, ny, nz;
+    __device__ void rebase(int64_t o) {
+        x = vd::rb(x, o);
+#pragma unroll
+        for (int d = 0; d < 3; d++) g[d].rebase(o);
+        vd::rebase(phi, o);
+    }
 };
 
 // E on the faces (MLMG::getGradSolution times -1), the POT copy, E cell
@@ -3326,7 +3701,9 @@ __device__ __forceinline__ double pot_face_pair(const GradArgs& g, const int c[3
     return 0.5 * (flo + fhi);
 }
 
-__global__ void k_pot_post(PotPostArgs a) {
+template <bool BT>
+__global__ void k_pot_post(PotPostArgs a, Batch bt) {
+    if constexpr (BT) a.rebase(case_off(bt));
     // the box holds fewer than 2^31 cells (32-bit cell offsets)
     const int t = (int)((int64_t)lblock() * blockDim.x + threadIdx.x);
     if (t >= a.nx * a.ny * a.nz) return;
@@ -3349,8 +3726,9 @@ __global__ void k_pot_post(PotPostArgs a) {
 }
 
 // applied voltage (Source/HelperFuncs.H:75-92)
-static double applied_potential(double t, int domend, const vd_params* p) {
+static double applied_potential(double t, int domend, const vd_params* p, const double* amp_case = nullptr) {
     double amp = domend == -1 ? p->voltage_amp_1 : p->voltage_amp_2;
+    if (amp_case) amp = domend == -1 ? amp_case[0] : amp_case[1];
     if (p->voltage_profile == 1) return sin(2.0 * VD_PI * p->voltage_freq * t) * amp;
     if (p->voltage_profile == 2) {
         double vc = p->voltage_center, vd = p->voltage_dur;
@@ -3386,10 +3764,15 @@ int solve_potential(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array
     ra.ny = n[1];
     ra.nz = n[2];
     ra.spacechrg = prm->do_spacechrg;
-    hipLaunchKernelGGL(k_pot_rhs, grid1d(nc, 256), dim3(256), 0, ctx->stream, ra, W->f);
+    hipLaunchKernelGGL(k_pot_rhs, bz(ctx, grid1d(nc, 256)), dim3(256), 0, ctx->stream, ra, W->f, ctx->bt);
     // potential_bc (PlasmaChem.H:18-72): Dirichlet / Robin(1,0,V) value V,
-    // every side's arrays in one launch
-    {
+    // every side's arrays in one launch -- one per case when the cases of a
+    // batched context have voltages of their own
+    bool per_case = false;
+    for (int z = 0; z < ctx->bn; z++) per_case |= ctx->case_volt_set[ctx->bt.idx[z]] != 0;
+    for (int z = 0; z < (per_case ? ctx->bn : 1); z++) {
+        const int c = ctx->bt.idx[z];
+        const double* amp = per_case && ctx->case_volt_set[c] ? ctx->case_volt[c] : nullptr;
         double* fp[FILL_SIDES];
         int64_t fn[FILL_SIDES];
         double fv[FILL_SIDES];
@@ -3397,7 +3780,7 @@ int solve_potential(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array
         for (int d = 0; d < g->dim; d++) {
             if (W->per[d]) continue;
             for (int s = 0; s < 2; s++) {
-                const double V = applied_potential(time, s ? 1 : -1, prm);
+                const double V = applied_potential(time, s ? 1 : -1, prm, amp);
                 const int64_t nt = ntanf(n, d);
                 double* arrs[4] = {W->bcv[d][s], W->rob[d][s][0], W->rob[d][s][1], W->rob[d][s][2]};
                 const double vals[4] = {V, 1.0, 0.0, V};
@@ -3414,7 +3797,12 @@ int solve_potential(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array
         fn[nf] = nc;
         fv[nf] = 0.0;
         nf++;
-        VD_TRY(fill_values(ctx, fp, fn, fv, nf));
+        if (per_case) {
+            BatchScope one(ctx, &c, 1);
+            VD_TRY(fill_values(ctx, fp, fn, fv, nf));
+        } else {
+            VD_TRY(fill_values(ctx, fp, fn, fv, nf));
+        }
     }
     const double bconst = -1.0, alpha = 0.0, beta = 1.0;
     VD_TRY(side_setup(ctx, *W, prm->pot_bc_lo, prm->pot_bc_hi, beta, bconst));
@@ -3438,7 +3826,10 @@ int solve_potential(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array
     // reads the solution and the species (E/N), writes the potential, E
     // cell and face components and E/N
     kt_begin(ctx, VD_KT_POTPOST);
-    hipLaunchKernelGGL(k_pot_post, grid1d(nc, 256), dim3(256), 0, ctx->stream, pa);
+    if (ctx->ncase > 1)
+        hipLaunchKernelGGL(k_pot_post<true>, bz(ctx, grid1d(nc, 256)), dim3(256), 0, ctx->stream, pa, ctx->bt);
+    else
+        hipLaunchKernelGGL(k_pot_post<false>, grid1d(nc, 256), dim3(256), 0, ctx->stream, pa, ctx->bt);
     kt_end(ctx, VD_KT_POTPOST, 8.0 * (double)nc * (1 + m->hdev.nspec + 2 + 2 * g->dim));
     VD_CHECK_LAUNCH();
     return st;
@@ -3453,11 +3844,17 @@ struct CellDArgs {
     int3 lo;
     int gnx, gny, gnz;  // grown box extent
     double Tg;
+    __device__ void rebase(int64_t o) {
+        M = vd::rb(M, o);
+        vd::rebase(sb, o);
+    }
 };
 
 // cell specDiff on the box grown by one (ScalarSolve.cpp:483-497)
 template <bool STATE>
-__global__ void __launch_bounds__(256) k_cell_diff(CellDArgs a, double* __restrict__ dcell) {
+__global__ void __launch_bounds__(256) k_cell_diff(CellDArgs a, double* __restrict__ dcell, Batch bt) {
+    a.rebase(case_off(bt));
+    dcell = rb(dcell, case_off(bt));
     const int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.gnx * a.gny * a.gnz) return;
     const int i = (int)(t % a.gnx);
@@ -3493,6 +3890,11 @@ struct SpecBcArgs {
     int nx, ny, nz;
     double Tg;
     double* out;
+    __device__ void rebase(int64_t o) {
+        M = vd::rb(M, o);
+        vd::rebase(sb, o);
+        out = vd::rb(out, o);
+    }
 };
 
 // the non-periodic sides of one solve, one launch (blockIdx.y = side)
@@ -3506,10 +3908,11 @@ struct SpecBcSides {
 // STATE: the species' coefficients are fits of the state (then ndens and
 // |E| of the boundary cell are formed)
 template <bool STATE>
-__global__ void k_species_bc(SpecBcArgs a, SpecBcSides sides) {
+__global__ void k_species_bc(SpecBcArgs a, SpecBcSides sides, Batch bt) {
     a.d = sides.d[blockIdx.y];
     a.side = sides.side[blockIdx.y];
     a.out = sides.out[blockIdx.y];
+    a.rebase(case_off(bt));
     int n[3] = {a.nx, a.ny, a.nz};
     int t1 = a.d == 0 ? 1 : 0, t2 = a.d == 2 ? 1 : 2;
     int nt1 = n[t1], nt2 = a.dim == 3 ? n[t2] : 1;
@@ -3577,10 +3980,18 @@ struct SpecRhsArgs {
     int3 lo;
     int nx, ny, nz;
     double idt;
+    __device__ void rebase(int64_t o) {
+        vd::rebase(old, o);
+        vd::rebase(ds, o);
+        vd::rebase(sb, o);
+    }
 };
 
 // rhs = (1/dt) n_old + dsdt; initial guess = Sborder (ScalarSolve.cpp:443-465)
-__global__ void k_spec_rhs(SpecRhsArgs a, double* __restrict__ f, double* __restrict__ x) {
+__global__ void k_spec_rhs(SpecRhsArgs a, double* __restrict__ f, double* __restrict__ x, Batch bt) {
+    a.rebase(case_off(bt));
+    f = rb(f, case_off(bt));
+    x = rb(x, case_off(bt));
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.nx * a.ny * a.nz) return;
     int i = (int)(t % a.nx);
@@ -3600,9 +4011,15 @@ struct SpecPostArgs {
     double floor_val, min_te;
     int3 lo;
     int nx, ny, nz;
+    __device__ void rebase(int64_t o) {
+        x = vd::rb(x, o);
+        vd::rebase(phi, o);
+        vd::rebase(sb, o);
+    }
 };
 
-__global__ void k_spec_floor(SpecPostArgs a) {
+__global__ void k_spec_floor(SpecPostArgs a, Batch bt) {
+    a.rebase(case_off(bt));
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.nx * a.ny * a.nz) return;
     if (a.floor_on && a.x[t] < a.floor_val) a.x[t] = a.floor_val;
@@ -3612,7 +4029,8 @@ __global__ void k_spec_floor(SpecPostArgs a) {
 // density floor of k_spec_floor applied on the way (when nothing reads the
 // floored solution in between)
 template <bool FLOOR>
-__global__ void k_spec_post(SpecPostArgs a) {
+__global__ void k_spec_post(SpecPostArgs a, Batch bt) {
+    a.rebase(case_off(bt));
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.nx * a.ny * a.nz) return;
     int i = (int)(t % a.nx);
@@ -3710,8 +4128,8 @@ static int implicit_solve_body(vd_ctx* ctx, const vd_mech* m, int spec, const vd
         cd.Tg = Tg;
         cd.state = cd_state;
         const dim3 gcd = grid1d((int64_t)cd.gnx * cd.gny * cd.gnz, 256);
-        if (cd.state) hipLaunchKernelGGL(k_cell_diff<true>, gcd, dim3(256), 0, ctx->stream, cd, W->dcell);
-        else hipLaunchKernelGGL(k_cell_diff<false>, gcd, dim3(256), 0, ctx->stream, cd, W->dcell);
+        if (cd.state) hipLaunchKernelGGL(k_cell_diff<true>, bz(ctx, gcd), dim3(256), 0, ctx->stream, cd, W->dcell, ctx->bt);
+        else hipLaunchKernelGGL(k_cell_diff<false>, bz(ctx, gcd), dim3(256), 0, ctx->stream, cd, W->dcell, ctx->bt);
     }
     {
         SpecBcArgs b;
@@ -3743,8 +4161,8 @@ static int implicit_solve_body(vd_ctx* ctx, const vd_mech* m, int spec, const vd
         if (sides.nside > 0) {
             dim3 gb = grid1d(ntmax, 256);
             gb.y = (unsigned)sides.nside;
-            if (cd_state) hipLaunchKernelGGL(k_species_bc<true>, gb, dim3(256), 0, ctx->stream, b, sides);
-            else hipLaunchKernelGGL(k_species_bc<false>, gb, dim3(256), 0, ctx->stream, b, sides);
+            if (cd_state) hipLaunchKernelGGL(k_species_bc<true>, bz(ctx, gb), dim3(256), 0, ctx->stream, b, sides, ctx->bt);
+            else hipLaunchKernelGGL(k_species_bc<false>, bz(ctx, gb), dim3(256), 0, ctx->stream, b, sides, ctx->bt);
         }
     }
     // Robin species sides: the case's user_transport::species_bc data, or
@@ -3770,7 +4188,7 @@ static int implicit_solve_body(vd_ctx* ctx, const vd_mech* m, int spec, const vd
     r.ny = n[1];
     r.nz = n[2];
     r.idt = 1.0 / dt;
-    hipLaunchKernelGGL(k_spec_rhs, grid1d(nc, 256), dim3(256), 0, ctx->stream, r, W->f, W->x);
+    hipLaunchKernelGGL(k_spec_rhs, bz(ctx, grid1d(nc, 256)), dim3(256), 0, ctx->stream, r, W->f, W->x, ctx->bt);
     VD_CHECK_LAUNCH();
     const double alpha = 1.0 / dt, beta = 1.0;
     const double* dcell = constd ? nullptr : W->dcell;
@@ -3798,11 +4216,11 @@ static int implicit_solve_body(vd_ctx* ctx, const vd_mech* m, int spec, const vd
     p.nz = n[2];
     if (spec == E && grad) {
         // the electron gradient faces see the floored density
-        hipLaunchKernelGGL(k_spec_floor, grid1d(nc, 256), dim3(256), 0, ctx->stream, p);
+        hipLaunchKernelGGL(k_spec_floor, bz(ctx, grid1d(nc, 256)), dim3(256), 0, ctx->stream, p, ctx->bt);
         VD_TRY(grad_faces(ctx, *W, bclo, bchi, grad, box, 1.0));
-        hipLaunchKernelGGL(k_spec_post<false>, grid1d(nc, 256), dim3(256), 0, ctx->stream, p);
+        hipLaunchKernelGGL(k_spec_post<false>, bz(ctx, grid1d(nc, 256)), dim3(256), 0, ctx->stream, p, ctx->bt);
     } else {
-        hipLaunchKernelGGL(k_spec_post<true>, grid1d(nc, 256), dim3(256), 0, ctx->stream, p);
+        hipLaunchKernelGGL(k_spec_post<true>, bz(ctx, grid1d(nc, 256)), dim3(256), 0, ctx->stream, p, ctx->bt);
     }
     VD_CHECK_LAUNCH();
     return st;
@@ -3910,6 +4328,7 @@ extern "C" int vd_mlmg_solve(vd_ctx* ctx, const vd_mlmg_problem* prob, vd_array4
     if (!ctx || !prob || !phi || !rhs || !box || !geom) return VD_EINVAL;
     VD_TRY(check_extents("mlmg_solve", box, {phi, rhs, prob->bcval, prob->robin_a, prob->robin_b, prob->robin_f}));
     for (int d = 0; d < geom->dim && d < 3; d++) VD_TRY(check_extents("mlmg_solve", nullptr, {prob->bface[d]}));
+    VD_TRY(require_plain(ctx, "mlmg_solve"));
     if (prob->precision != 0 && prob->precision != 1) return VD_EINVAL;
     return mlmg_solve(ctx, prob, phi, rhs, box, geom, info);
 }

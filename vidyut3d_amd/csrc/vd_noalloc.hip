@@ -74,6 +74,7 @@ extern "C" int vd_transport_rhs_noalloc(vd_ctx* ctx, const vd_mech* m, const vd_
         return VD_EINVAL;
     VD_TRY(check_extents("transport_rhs_noalloc", box, {sborder, rxn_src, dsdt_out}));
     VD_TRY(check_transport_args(prm, specid, m->hdev.nspec));
+    VD_TRY(require_plain(ctx, "transport_rhs_noalloc"));
     for (int d = 0; d < 3; d++)
         if (box->hi[d] < box->lo[d]) {
             set_error("transport (no allocation): empty box");

@@ -16,6 +16,11 @@ struct CflArgs {
     int3 lo;
     int nx, ny, nz;
     unsigned long long* out;  // 3 accumulators (bit patterns of non-negative doubles)
+    __device__ void rebase(int64_t o) {
+        M = rb(M, o);
+        vd::rebase(phi, o);
+        out = rb(out, o);
+    }
 };
 
 __device__ __forceinline__ double wave_max(double a) {
@@ -30,7 +35,8 @@ __device__ __forceinline__ unsigned long long bits(double v) {
 
 // STATE: the electron coefficients are fits of the state (a.state_trans)
 template <bool STATE>
-__global__ void __launch_bounds__(256) k_cfl(CflArgs a) {
+__global__ void __launch_bounds__(256) k_cfl(CflArgs a, Batch bt) {
+    a.rebase(case_off(bt));
     __shared__ double red[3][4];
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     int64_t tot = (int64_t)a.nx * a.ny * a.nz;
@@ -88,7 +94,8 @@ __global__ void __launch_bounds__(256) k_cfl(CflArgs a) {
     }
 }
 
-__global__ void k_zero3(unsigned long long* p) {
+__global__ void k_zero3(unsigned long long* p, Batch bt) {
+    p = rb(p, case_off(bt));
     if (threadIdx.x < 3) p[threadIdx.x] = 0ull;
 }
 
@@ -113,23 +120,31 @@ int find_time_scales(vd_ctx* ctx, const vd_mech* m, const vd_array4* phi, const 
     int64_t tot = (int64_t)a.nx * a.ny * a.nz;
     int64_t blocks = (tot + 255) / 256;
     if (blocks > 2048) blocks = 2048;  // grid-stride: 8 blocks per CU
-    hipLaunchKernelGGL(k_zero3, dim3(1), dim3(64), 0, ctx->stream, a.out);
-    if (a.state_trans) hipLaunchKernelGGL(k_cfl<true>, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a);
-    else hipLaunchKernelGGL(k_cfl<false>, dim3((unsigned)blocks), dim3(256), 0, ctx->stream, a);
+    hipLaunchKernelGGL(k_zero3, bz(ctx, dim3(1)), dim3(64), 0, ctx->stream, a.out, ctx->bt);
+    if (a.state_trans)
+        hipLaunchKernelGGL(k_cfl<true>, bz(ctx, dim3((unsigned)blocks)), dim3(256), 0, ctx->stream, a, ctx->bt);
+    else
+        hipLaunchKernelGGL(k_cfl<false>, bz(ctx, dim3((unsigned)blocks)), dim3(256), 0, ctx->stream, a, ctx->bt);
     VD_CHECK_LAUNCH();
-    VD_HIP(hipMemcpyAsync(ctx->pinned, acc, 3 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+    // each case's three maxima (at its own copy of the accumulators)
+    for (int z = 0; z < ctx->bn; z++)
+        VD_HIP(hipMemcpyAsync(ctx->pinned + 3 * z, (char*)acc + ctx->bt.stride * ctx->bt.idx[z], 3 * sizeof(double),
+                              hipMemcpyDeviceToHost, ctx->stream));
     VD_HIP(hipStreamSynchronize(ctx->stream));
-    double mx[3];
-    memcpy(mx, ctx->pinned, sizeof(mx));
-    double dt_edrift = DBL_MAX, dt_ediff = DBL_MAX, dt_diel = DBL_MAX;
-    if (mx[0] > 0)
-        for (int d = 0; d < g->dim; d++) dt_edrift = fmin(dt_edrift, g->dx[d] / mx[0]);
-    if (mx[1] > 0)
-        for (int d = 0; d < g->dim; d++) dt_ediff = fmin(dt_ediff, 0.5 * g->dx[d] * g->dx[d] / mx[1] / g->dim);
-    if (mx[2] > 0) dt_diel = VD_EPS0 / VD_ECHARGE / mx[2];
-    out[0] = dt_edrift;
-    out[1] = dt_ediff;
-    out[2] = dt_diel;
+    for (int z = ctx->bn - 1; z >= 0; z--) {
+        double mx[3];
+        memcpy(mx, ctx->pinned + 3 * z, sizeof(mx));
+        double dt_edrift = DBL_MAX, dt_ediff = DBL_MAX, dt_diel = DBL_MAX;
+        if (mx[0] > 0)
+            for (int d = 0; d < g->dim; d++) dt_edrift = fmin(dt_edrift, g->dx[d] / mx[0]);
+        if (mx[1] > 0)
+            for (int d = 0; d < g->dim; d++) dt_ediff = fmin(dt_ediff, 0.5 * g->dx[d] * g->dx[d] / mx[1] / g->dim);
+        if (mx[2] > 0) dt_diel = VD_EPS0 / VD_ECHARGE / mx[2];
+        // out: the first active case's (every case's in vd_ctx_case_time_scales)
+        out[0] = ctx->case_ts[ctx->bt.idx[z]][0] = dt_edrift;
+        out[1] = ctx->case_ts[ctx->bt.idx[z]][1] = dt_ediff;
+        out[2] = ctx->case_ts[ctx->bt.idx[z]][2] = dt_diel;
+    }
     return VD_OK;
 }
 
@@ -251,5 +266,6 @@ extern "C" int vd_error_est(vd_ctx* ctx, const vd_array4* state, vd_tagarray4* t
         return VD_EINVAL;
     VD_TRY(check_extents("error_est", box, {state}));
     VD_TRY(check_array_extent("error_est", tags->begin, tags->end, tags->jstride, tags->kstride));
+    VD_TRY(require_plain(ctx, "error_est"));
     return error_est(ctx, state, tags, box, ntagvars, comps, refine_phi, refine_phigrad, tagval);
 }

@@ -168,11 +168,17 @@ struct FluxArgs {
     int3 flo;            // first face index
     int fnx, fny, fnz;   // face box extent
     double dxmin, dt;
+    __device__ void rebase(int64_t o) {
+        vd::rebase(sb, o);
+        vd::rebase(flux, o);
+        ev = rb(ev, o);
+    }
 };
 
 // one thread per face of direction a.dir
 template <int HYP, int S>
-__global__ void __launch_bounds__(256) k_face_flux(FluxArgs a) {
+__global__ void __launch_bounds__(256) k_face_flux(FluxArgs a, Batch bt) {
+    a.rebase(case_off(bt));
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     int64_t tot = (int64_t)a.fnx * a.fny * a.fnz;
     if (t >= tot) return;
@@ -245,10 +251,22 @@ struct PencilArgs {
     int fuse;
     Arr<double> fx, fy, rxn, ds;
     double dx0, dx1, dx2;
+    __device__ void rebase(int64_t o) {
+        vd::rebase(sb, o);
+        vd::rebase(flux, o);
+        ev = rb(ev, o);
+        if (fuse) {
+            vd::rebase(fx, o);
+            vd::rebase(fy, o);
+            vd::rebase(rxn, o);
+            vd::rebase(ds, o);
+        }
+    }
 };
 
 template <int S, int D>
-__global__ void __launch_bounds__(256) k_pencil_flux(PencilArgs a) {
+__global__ void __launch_bounds__(256) k_pencil_flux(PencilArgs a, Batch bt) {
+    a.rebase(case_off(bt));
     const unsigned t = lblock() * blockDim.x + threadIdx.x;
     if (t >= (unsigned)a.nx * a.no * a.nchunk) return;
     const int i = (int)(t % a.nx);
@@ -334,10 +352,16 @@ struct XLineArgs {
     int3 flo;
     int nx, nrow, nseg;  // cells along x, rows (y, z); nseg unused
     int ny;
+    __device__ void rebase(int64_t o) {
+        vd::rebase(sb, o);
+        vd::rebase(flux, o);
+        ev = rb(ev, o);
+    }
 };
 
 template <int S>
-__global__ void __launch_bounds__(256) k_xline_flux(XLineArgs a) {
+__global__ void __launch_bounds__(256) k_xline_flux(XLineArgs a, Batch bt) {
+    a.rebase(case_off(bt));
     const unsigned wv = (lblock() * blockDim.x + threadIdx.x) >> 6;
     const int lane = threadIdx.x & 63;
     // the rows' reconstructed cells -1 .. nx laid end to end: wave wv
@@ -387,10 +411,18 @@ struct DsdtArgs {
     double dx0, dx1, dx2;
     int3 lo;
     int nx, ny, nz;
+    __device__ void rebase(int64_t o) {
+        vd::rebase(f0, o);
+        vd::rebase(f1, o);
+        vd::rebase(f2, o);
+        vd::rebase(rxn, o);
+        vd::rebase(dsdt, o);
+    }
 };
 
 // compute_dsdt (Source/ScalarSolve.cpp:44-53)
-__global__ void __launch_bounds__(256) k_dsdt(DsdtArgs a) {
+__global__ void __launch_bounds__(256) k_dsdt(DsdtArgs a, Batch bt) {
+    a.rebase(case_off(bt));
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     int64_t tot = (int64_t)a.nx * a.ny * a.nz;
     if (t >= tot) return;
@@ -410,7 +442,7 @@ __global__ void __launch_bounds__(256) k_dsdt(DsdtArgs a) {
 template <int HYP, int S>
 static void launch_flux(vd_ctx* ctx, const FluxArgs& a) {
     int64_t tot = (int64_t)a.fnx * a.fny * a.fnz;
-    hipLaunchKernelGGL((k_face_flux<HYP, S>), grid1d(tot, 256), dim3(256), 0, ctx->stream, a);
+    hipLaunchKernelGGL((k_face_flux<HYP, S>), bz(ctx, grid1d(tot, 256)), dim3(256), 0, ctx->stream, a, ctx->bt);
 }
 
 // chunk of faces per pencil thread along D: long enough to amortise the
@@ -461,13 +493,13 @@ static int launch_pencil(vd_ctx* ctx, const FluxArgs& f, int d, int scheme, cons
     }
     dim3 g = grid1d(tot, 256), b(256);
     if (d == 1) {
-        if (scheme == 1) hipLaunchKernelGGL((k_pencil_flux<1, 1>), g, b, 0, ctx->stream, a);
-        else if (scheme == 2) hipLaunchKernelGGL((k_pencil_flux<2, 1>), g, b, 0, ctx->stream, a);
-        else hipLaunchKernelGGL((k_pencil_flux<3, 1>), g, b, 0, ctx->stream, a);
+        if (scheme == 1) hipLaunchKernelGGL((k_pencil_flux<1, 1>), bz(ctx, g), b, 0, ctx->stream, a, ctx->bt);
+        else if (scheme == 2) hipLaunchKernelGGL((k_pencil_flux<2, 1>), bz(ctx, g), b, 0, ctx->stream, a, ctx->bt);
+        else hipLaunchKernelGGL((k_pencil_flux<3, 1>), bz(ctx, g), b, 0, ctx->stream, a, ctx->bt);
     } else {
-        if (scheme == 1) hipLaunchKernelGGL((k_pencil_flux<1, 2>), g, b, 0, ctx->stream, a);
-        else if (scheme == 2) hipLaunchKernelGGL((k_pencil_flux<2, 2>), g, b, 0, ctx->stream, a);
-        else hipLaunchKernelGGL((k_pencil_flux<3, 2>), g, b, 0, ctx->stream, a);
+        if (scheme == 1) hipLaunchKernelGGL((k_pencil_flux<1, 2>), bz(ctx, g), b, 0, ctx->stream, a, ctx->bt);
+        else if (scheme == 2) hipLaunchKernelGGL((k_pencil_flux<2, 2>), bz(ctx, g), b, 0, ctx->stream, a, ctx->bt);
+        else hipLaunchKernelGGL((k_pencil_flux<3, 2>), bz(ctx, g), b, 0, ctx->stream, a, ctx->bt);
     }
     VD_CHECK_LAUNCH();
     return VD_OK;
@@ -494,9 +526,9 @@ static int launch_xline(vd_ctx* ctx, const FluxArgs& f, int scheme, const vd_box
         return VD_EINVAL;
     }
     dim3 g = grid1d(threads, 256), b(256);
-    if (scheme == 1) hipLaunchKernelGGL((k_xline_flux<1>), g, b, 0, ctx->stream, a);
-    else if (scheme == 2) hipLaunchKernelGGL((k_xline_flux<2>), g, b, 0, ctx->stream, a);
-    else hipLaunchKernelGGL((k_xline_flux<3>), g, b, 0, ctx->stream, a);
+    if (scheme == 1) hipLaunchKernelGGL((k_xline_flux<1>), bz(ctx, g), b, 0, ctx->stream, a, ctx->bt);
+    else if (scheme == 2) hipLaunchKernelGGL((k_xline_flux<2>), bz(ctx, g), b, 0, ctx->stream, a, ctx->bt);
+    else hipLaunchKernelGGL((k_xline_flux<3>), bz(ctx, g), b, 0, ctx->stream, a, ctx->bt);
     VD_CHECK_LAUNCH();
     return VD_OK;
 }
@@ -513,9 +545,15 @@ struct DriftArgs {
     int nx, ny, nz;  // sb's allocated extent
     double Tg;
     double* vel;
+    __device__ void rebase(int64_t o) {
+        M = rb(M, o);
+        vd::rebase(sb, o);
+        vel = rb(vel, o);
+    }
 };
 
-__global__ void __launch_bounds__(256) k_drift_vel(DriftArgs a) {
+__global__ void __launch_bounds__(256) k_drift_vel(DriftArgs a, Batch bt) {
+    a.rebase(case_off(bt));
     const int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.nx * a.ny * a.nz) return;
     const int i = (int)(t % a.nx);
@@ -594,7 +632,8 @@ int transport_flux(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array4
         v.nz = sb->end[2] - sb->begin[2];
         v.Tg = prm->gas_temperature;
         v.vel = vbuf;
-        hipLaunchKernelGGL(k_drift_vel, grid1d((int64_t)v.nx * v.ny * v.nz, 256), dim3(256), 0, ctx->stream, v);
+        hipLaunchKernelGGL(k_drift_vel, bz(ctx, grid1d((int64_t)v.nx * v.ny * v.nz, 256)), dim3(256), 0, ctx->stream, v,
+                           ctx->bt);
         VD_CHECK_LAUNCH();
         ev = vbuf;
         mob = 1.0;
@@ -657,7 +696,7 @@ int dsdt(vd_ctx* ctx, const vd_array4 flux[3], const vd_array4* rxn, vd_array4* 
     a.ny = box->hi[1] - box->lo[1] + 1;
     a.nz = box->hi[2] - box->lo[2] + 1;
     int64_t tot = (int64_t)a.nx * a.ny * a.nz;
-    hipLaunchKernelGGL(k_dsdt, grid1d(tot, 256), dim3(256), 0, ctx->stream, a);
+    hipLaunchKernelGGL(k_dsdt, bz(ctx, grid1d(tot, 256)), dim3(256), 0, ctx->stream, a, ctx->bt);
     VD_CHECK_LAUNCH();
     return VD_OK;
 }

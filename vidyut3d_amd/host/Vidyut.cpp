@@ -906,6 +906,9 @@ int vd_sim_create(vd_ctx* ctx, const vd_mech_desc* mech, const vd_geom* geom, co
                   vd_sim** out) {
     if (!ctx || !mech || !geom || !prm || !out) return VD_EINVAL;
     if (geom->dim != 2 && geom->dim != 3) return VD_EINVAL;
+    // the cases of a batched context share one step sequence: each case's
+    // adaptive dt would differ
+    if (vd_ctx_ncase(ctx) > 1 && prm->adaptive_dt) return VD_EINVAL;
     Vidyut* v = new Vidyut(ctx, *mech, *geom, *prm);
     if (v->status() != VD_OK) {
         int st = v->status();
@@ -932,6 +935,20 @@ int vd_sim_set_state(vd_sim* s, const double* host, double time) {
 int vd_sim_get_state(vd_sim* s, double* host) {
     auto& mf = s->v->phi_new();
     return vd_memcpy_d2h(s->ctx, host, mf.data(), mf.bytes());
+}
+
+int vd_sim_set_state_case(vd_sim* s, int c, const double* host) {
+    if (!s || !host || c < 0 || c >= vd_ctx_ncase(s->ctx)) return VD_EINVAL;
+    auto& mf = s->v->phi_new();
+    char* p = (char*)mf.data() + (size_t)c * (size_t)vd_ctx_case_stride(s->ctx);
+    return vd_memcpy_h2d(s->ctx, p, host, mf.bytes());
+}
+
+int vd_sim_get_state_case(vd_sim* s, int c, double* host) {
+    if (!s || !host || c < 0 || c >= vd_ctx_ncase(s->ctx)) return VD_EINVAL;
+    auto& mf = s->v->phi_new();
+    const char* p = (const char*)mf.data() + (size_t)c * (size_t)vd_ctx_case_stride(s->ctx);
+    return vd_memcpy_d2h(s->ctx, host, p, mf.bytes());
 }
 
 int vd_sim_state_view(vd_sim* s, vd_array4* out) {

@@ -46,6 +46,9 @@ def lib():
         L.vd_sim_mg_iterations.restype = C.c_longlong
         L.vd_sim_mg_iterations.argtypes = [C.c_void_p]
         L.vd_mech_find_id.argtypes = [C.c_void_p, C.c_char_p]
+        L.vd_ctx_case_stride.restype = C.c_longlong
+        L.vd_ctx_case_stride.argtypes = [C.c_void_p]
+        L.vd_ctx_ncase.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 
@@ -68,6 +71,7 @@ class Device:
         self.ptr = C.c_void_p()
         self._deps = weakref.WeakSet()
         check(lib().vd_ctx_create(int(device), C.byref(self.ptr)), "vd_ctx_create")
+        self.ncase = 1
 
     def adopt(self, obj):
         self._deps.add(obj)
@@ -131,6 +135,42 @@ class Device:
         check(lib().vd_ctx_kernel_timing(self.ptr, C.byref(ms), C.byref(n), C.byref(cells)),
               "vd_ctx_kernel_timing")
         return ms.value, n.value, cells.value
+
+
+class BatchDevice(Device):
+    """A batched vd_ctx (vd_ctx_create_batched): ncase same-shape cases,
+    each in its own region of case_bytes device memory, stepped together."""
+
+    def __init__(self, device, ncase, case_bytes):
+        self.ptr = C.c_void_p()
+        self._deps = weakref.WeakSet()
+        check(lib().vd_ctx_create_batched(int(device), int(ncase), C.c_size_t(int(case_bytes)), C.byref(self.ptr)),
+              "vd_ctx_create_batched")
+        self.ncase = int(ncase)
+
+    def set_active(self, cases):
+        cases = list(cases)
+        check(lib().vd_ctx_set_active(self.ptr, len(cases), (C.c_int * max(len(cases), 1))(*cases)),
+              "vd_ctx_set_active")
+
+    def set_case_voltage(self, c, amp_1, amp_2, use=True):
+        check(lib().vd_ctx_set_case_voltage(self.ptr, int(c), int(bool(use)), C.c_double(amp_1),
+                                            C.c_double(amp_2)), "vd_ctx_set_case_voltage")
+
+    def arena_used(self):
+        n = C.c_size_t()
+        check(lib().vd_ctx_arena_used(self.ptr, C.byref(n)), "vd_ctx_arena_used")
+        return n.value
+
+    def case_solve_info(self, c):
+        info = abi.vd_solve_info()
+        check(lib().vd_ctx_case_solve_info(self.ptr, int(c), C.byref(info)), "vd_ctx_case_solve_info")
+        return info
+
+    def case_time_scales(self, c):
+        out = (C.c_double * 3)()
+        check(lib().vd_ctx_case_time_scales(self.ptr, int(c), out), "vd_ctx_case_time_scales")
+        return list(out)
 
 
 class DeviceArray:
@@ -414,6 +454,19 @@ class Sim:
         dt = self.case.dt if dt is None else dt
         check(lib().vd_sim_evolve(self.ptr, int(nsteps), C.c_double(dt), int(ncorr)),
               "vd_sim_evolve", (abi.VD_OK, abi.VD_ENOCONV))
+
+    # ---- batched contexts: one state per case
+    def set_state_case(self, c, st):
+        st = np.ascontiguousarray(st, dtype=np.float64)
+        assert st.shape == self.shape
+        check(lib().vd_sim_set_state_case(self.ptr, int(c), st.ctypes.data_as(C.c_void_p)),
+              "vd_sim_set_state_case")
+
+    def get_state_case(self, c):
+        out = np.empty(self.shape)
+        check(lib().vd_sim_get_state_case(self.ptr, int(c), out.ctypes.data_as(C.c_void_p)),
+              "vd_sim_get_state_case")
+        return out
 
     @property
     def time(self):

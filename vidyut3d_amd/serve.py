@@ -81,6 +81,33 @@ def draw_requests(rate_hz, duration_s=None, seed=0, nsteps=500, min_cells=64 ** 
     return out
 
 
+_REGION_BYTES = {}
+
+
+def case_region_bytes(dev_index, case):
+    """Device bytes one case of `case`'s shape and options takes in a
+    batched context: measured once per (dim, n, nspec, precision) by a
+    one-case batched context that runs one step (every buffer and solver
+    workspace of a step is allocated by then), plus a margin."""
+    g = case.geom
+    key = (g.dim, tuple(g.dom_hi[d] - g.dom_lo[d] + 1 for d in range(g.dim)), case.mech.nspec,
+           case.params.mg_precision, case.params.hyp_order)
+    if key not in _REGION_BYTES:
+        nspec = case.mech.nspec
+        n = key[1][0]
+        guess = 3 * footprint_bytes(g.dim, n, nspec) + (64 << 20)
+        dev = native.BatchDevice(dev_index, 1, guess)
+        try:
+            sim = native.Sim(dev, case)
+            sim.evolve(1)
+            used = dev.arena_used()
+            sim.close()
+        finally:
+            dev.close()
+        _REGION_BYTES[key] = int(used * 1.02) + (4 << 20)
+    return _REGION_BYTES[key]
+
+
 class Server:
     """Worker slots with their own contexts; admission by footprint."""
 
