@@ -389,8 +389,10 @@ def main():
     ap.add_argument("--only", default="", help="comma-separated subset of: mixed,discharge,poisson,chemistry,generated,serving")
     ap.add_argument("--poisson-n", type=int, default=512)
     ap.add_argument("--chem-n", type=int, default=256)
-    ap.add_argument("--serve-requests", type=int, default=32)
-    ap.add_argument("--serve-rate", type=float, default=2.0, help="Poisson arrival rate of the serving leg (1/s)")
+    ap.add_argument("--serve-requests", type=int, default=128)
+    ap.add_argument("--serve-rate", type=float, default=4.0,
+                    help="Poisson arrival rate of the serving leg (1/s; the default keeps the device saturated)")
+    ap.add_argument("--serve-max-batch", type=int, default=16, help="cases per same-shape batch (1: no batching)")
     ap.add_argument("--serve-sequential", default="sampled", choices=["sampled", "run", "none"],
                     help="FP64 one-at-a-time baseline of the serving leg: per-shape samples, or a full run")
     ap.add_argument("--copy-probe", action="store_true",
@@ -484,7 +486,7 @@ def main():
         from vidyut3d_amd import serve
         log("serving loop: %d requests, Poisson arrivals at %.2f/s" % (args.serve_requests, args.serve_rate))
         configs["serving"], served = serve.bench(dev_index=0, requests=args.serve_requests, rate_hz=args.serve_rate,
-                                                 sequential=args.serve_sequential)
+                                                 sequential=args.serve_sequential, max_batch=args.serve_max_batch)
         log("serving: p50 %.2f s, p99 %.2f s, aggregate %.3g cell-steps/s"
             % (configs["serving"]["p50_time_to_solution_s"], configs["serving"]["p99_time_to_solution_s"],
                configs["serving"]["aggregate_cell_steps_per_s"]))

@@ -32,36 +32,44 @@ def test_footprint_grows_with_size():
 
 @pytest.mark.gpu
 def test_results_are_the_same_alone_and_in_the_mix():
-    reqs = [serve.Request(0, 0.00, 2, 64, 800.0, 30, 11), serve.Request(1, 0.01, 2, 96, 3000.0, 30, 12),
-            serve.Request(2, 0.02, 3, 32, 5000.0, 20, 13), serve.Request(3, 0.03, 2, 64, 9000.0, 30, 14),
-            serve.Request(4, 0.04, 3, 48, 2000.0, 20, 15)]
-    srv = serve.Server(0, slots=3, keep_state=True)
+    # four 2-D 64^2 requests that batch (one group, joining at chunk
+    # boundaries), with requests of other shapes running beside them
+    reqs = [serve.Request(0, 0.000, 2, 64, 800.0, 40, 11), serve.Request(1, 0.004, 2, 64, 3000.0, 30, 12),
+            serve.Request(2, 0.008, 3, 32, 5000.0, 20, 13), serve.Request(3, 0.012, 2, 64, 9000.0, 35, 14),
+            serve.Request(4, 0.016, 3, 48, 2000.0, 20, 15), serve.Request(5, 0.020, 2, 64, 6500.0, 25, 16),
+            serve.Request(6, 0.024, 2, 96, 4100.0, 30, 17)]
+    srv = serve.Server(0, slots=3, keep_state=True, chunk=5)
     try:
         srv.run(reqs)
     finally:
         srv.close()
     assert all(r.error is None for r in reqs), [r.error for r in reqs]
-    # requests overlapped on the device
+    # requests overlapped on the device, and same-shape requests shared launches
     assert any(reqs[q + 1].start < reqs[q].done for q in range(len(reqs) - 1))
+    assert max(r.batch_peak for r in reqs if r.shape == (2, 64)) >= 2
     for r in reqs:
-        alone = serve.Request(r.rid, 0.0, r.dim, r.n, r.voltage, r.nsteps, r.seed)
-        s1 = serve.Server(0, slots=1, keep_state=True)
-        try:
-            s1.run([alone])
-        finally:
-            s1.close()
+        alone = serve.run_alone(0, r, 1)
         assert np.isfinite(r.state).all()
-        assert np.array_equal(alone.state, r.state), r.rid
+        assert np.array_equal(alone, r.state), r.rid
         # the served (FP32 V-cycle) result against the all-FP64 oracle
         c = r.make_case(0)
         ref, _ = O.evolve(c, r.nsteps)
         errs = compare.state_errors(ref, r.state, c.mech.nspec, c.geom.dim)
         worst = max(errs.values())
-        print("request %d (%dD %d^%d): worst field rel L2 vs FP64 oracle %.2e"
-              % (r.rid, r.dim, r.n, r.dim, worst), flush=True)
+        print("request %d (%dD %d^%d, batch of up to %d): worst field rel L2 vs FP64 oracle %.2e"
+              % (r.rid, r.dim, r.n, r.dim, r.batch_peak, worst), flush=True)
         assert worst <= 1e-8, (r.rid, errs)
     out = serve.summarize(reqs)
     assert out["p99_time_to_solution_s"] >= out["p50_time_to_solution_s"] > 0
+
+
+def test_sizes_snap_to_the_shape_buckets():
+    assert serve.snap(64 ** 2) == (2, 64) and serve.snap(256 ** 2) == (2, 256)
+    assert serve.snap(256 ** 3) == (3, 256) and serve.snap(70000) == (3, 48)
+    assert serve.snap(100 ** 3) == (3, 96) and serve.snap(150 ** 2) == (2, 160)
+    reqs = serve.draw_requests(rate_hz=10.0, seed=2, nsteps=500, count=400)
+    shapes = set(r.shape for r in reqs)
+    assert shapes <= set((2, n) for n in serve.BUCKETS_2D) | set((3, n) for n in serve.BUCKETS_3D)
 
 
 def test_draw_a_fixed_number_of_requests():
