@@ -157,6 +157,53 @@ def cpu_single_thread(case_fn, sample_n, sample_steps):
 
 
 # ------------------------------------------------------------ sub-benchmarks
+def bench_batching(native, dev_index=0, steps=10, warmup=2, shapes=None):
+    """Same-shape batching per shape bucket: ms per step of one case on a
+    plain context, and of B cases of that shape (different seeds and
+    voltages) stepped together in a batched context, in FP32 V-cycles as
+    the serving loop runs them; speedup = B x (ms alone) / (ms batched)."""
+    import time
+    from vidyut3d_amd import serve
+    out = {}
+    for dim, n in shapes or ((2, 64), (2, 128), (2, 256), (3, 48), (3, 64), (3, 96), (3, 128)):
+        base = serve.Request(0, 0.0, dim, n, 5000.0, steps, 7).make_case(1)
+        d = native.Device(dev_index)
+        sim = native.Sim(d, base)
+        sim.evolve(warmup)
+        d.sync()
+        t0 = time.perf_counter()
+        sim.evolve(steps)
+        d.sync()
+        alone = (time.perf_counter() - t0) / steps
+        sim.close()
+        d.close()
+        row = {"cells": n ** dim, "alone_ms_per_step": round(alone * 1e3, 3)}
+        region = serve.case_region_bytes(dev_index, base)
+        for B in (4, 16):
+            d = native.BatchDevice(dev_index, B, region)
+            sim = native.Sim(d, base)
+            for c in range(B):
+                cc = serve.Request(c, 0.0, dim, n, 600.0 + 550.0 * c, steps, 100 + c).make_case(1)
+                sim.set_state_case(c, cc.state)
+                d.set_case_voltage(c, cc.params.voltage_amp_1, cc.params.voltage_amp_2)
+            sim.evolve(warmup)
+            d.sync()
+            t0 = time.perf_counter()
+            sim.evolve(steps)
+            d.sync()
+            tb = (time.perf_counter() - t0) / steps
+            sim.close()
+            d.close()
+            row["batch%d_ms_per_step" % B] = round(tb * 1e3, 3)
+            row["batch%d_cell_steps_per_s" % B] = B * n ** dim / tb
+            row["batch%d_speedup" % B] = round(B * alone / tb, 2)
+        out["%dD %d" % (dim, n)] = row
+        log("batching %dD %d: alone %.3f ms/step, 4 cases %.3f (%.2fx), 16 cases %.3f (%.2fx)"
+            % (dim, n, row["alone_ms_per_step"], row["batch4_ms_per_step"], row["batch4_speedup"],
+               row["batch16_ms_per_step"], row["batch16_speedup"]))
+    return out
+
+
 def bench_discharge_2d(native, abi, cases, compare, dev, with_cpu):
     """Config 0: 2-D 128^2, 100 steps of dt = 1e-11 s, FP64, against the
     oracle run on this host (the CPU reference path)."""
@@ -386,13 +433,14 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--headline-only", action="store_true", help="skip the other configurations")
-    ap.add_argument("--only", default="", help="comma-separated subset of: mixed,discharge,poisson,chemistry,generated,serving")
+    ap.add_argument("--only", default="", help="comma-separated subset of: mixed,discharge,poisson,chemistry,generated,batching,serving")
     ap.add_argument("--poisson-n", type=int, default=512)
     ap.add_argument("--chem-n", type=int, default=256)
     ap.add_argument("--serve-requests", type=int, default=128)
     ap.add_argument("--serve-rate", type=float, default=4.0,
                     help="Poisson arrival rate of the serving leg (1/s; the default keeps the device saturated)")
     ap.add_argument("--serve-max-batch", type=int, default=16, help="cases per same-shape batch (1: no batching)")
+    ap.add_argument("--batch-shape", default="", help="batching leg on one shape only, as DIM:N (e.g. 3:64)")
     ap.add_argument("--serve-sequential", default="sampled", choices=["sampled", "run", "none"],
                     help="FP64 one-at-a-time baseline of the serving leg: per-shape samples, or a full run")
     ap.add_argument("--copy-probe", action="store_true",
@@ -481,6 +529,10 @@ def main():
                                                                          not args.no_cpu)
         log("generated mechanism: %.3f ms per launch, %s" % (configs["chemistry_co2_h2_ar_44sp"]["launch_ms"],
                                                             configs["chemistry_co2_h2_ar_44sp"].get("vs_oracle")))
+    if want("batching"):
+        log("same-shape batching per shape bucket")
+        shp = [tuple(int(v) for v in args.batch_shape.split(":"))] if args.batch_shape else None
+        configs["batching"] = bench_batching(native, shapes=shp)
     served = []
     if want("serving") and args.serve_requests > 0:
         from vidyut3d_amd import serve

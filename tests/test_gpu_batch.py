@@ -77,5 +77,9 @@ def test_batched_context_rejects_what_it_does_not_batch():
         c.params.adaptive_dt = 1
         with pytest.raises(native.VidyutError):
             native.Sim(dev, c)
+        c.params.adaptive_dt = 0
+        c.params.voltage_profile = 1
+        with pytest.raises(native.VidyutError):
+            native.Sim(dev, c)
     finally:
         dev.close()

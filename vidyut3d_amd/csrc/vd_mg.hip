@@ -2893,21 +2893,21 @@ static inline dim3 norm_grid(int64_t n) {
 static inline unsigned long long* slot_group(MGWork& W, int q) { return W.dnorm + (size_t)q * NSLOT * SLOT_STRIDE; }
 
 // the max of each slot group into the host-mapped norm buffer, then the
-// call's sequence number into its last slot (one wave).  Slots hold the bit
+// call's sequence number into its last slot (one workgroup).  Slots hold the bit
 // patterns of non-negative doubles (NaN is stored as +inf), so their
 // maximum as doubles is exact.  Each slot is cleared by the lane that read
 // it, so the groups start at zero for the next norm (every accumulation
 // into slot groups is read back by exactly one read_groups call).
-// In a batched context the wave does this for each case of the launch
-// batch in turn (case z's groups at hout[z * ngroups ..]), so that the
-// sequence number follows every case's values.
+// In a batched context wave z of the one workgroup does this for case z of
+// the launch batch (its groups at hout[z * ngroups ..]), and the sequence
+// number follows every case's values (after the workgroup barrier).
 constexpr int HNORM_SEQ = HNORM_DOUBLES - 1;
 static_assert(VD_MAX_BATCH * 5 < HNORM_SEQ, "norm read-back buffer too small");
-__global__ void __launch_bounds__(64) k_norm_out(unsigned long long* __restrict__ slots0, int ngroups,
-                                                 double* __restrict__ hout, unsigned long long seq, Batch bt,
-                                                 int ncase) {
-    const int s = (int)threadIdx.x;
-    for (int z = 0; z < ncase; z++) {
+__global__ void __launch_bounds__(64 * VD_MAX_BATCH) k_norm_out(unsigned long long* __restrict__ slots0, int ngroups,
+                                                                double* __restrict__ hout, unsigned long long seq,
+                                                                Batch bt, int ncase) {
+    const int s = (int)threadIdx.x & 63, z = (int)threadIdx.x >> 6;
+    if (z < ncase) {
         unsigned long long* __restrict__ slots = rb(slots0, bt.stride * (int64_t)bt.idx[z]);
         for (int q = 0; q < ngroups; q++) {
             double v = 0.0;
@@ -2919,8 +2919,10 @@ __global__ void __launch_bounds__(64) k_norm_out(unsigned long long* __restrict_
             for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
             if (s == 0) hout[z * ngroups + q] = v;
         }
+        __threadfence_system();
     }
-    if (s == 0) {
+    __syncthreads();
+    if (threadIdx.x == 0) {
         __threadfence_system();
         ((volatile unsigned long long*)hout)[HNORM_SEQ] = seq;
     }
@@ -2947,8 +2949,8 @@ __global__ void __launch_bounds__(256) k_absmax(const double* __restrict__ a, in
 // synchronisation, and checks the stream for errors while it waits
 static int read_groups(vd_ctx* ctx, MGWork& W, int ngroups, double* out) {
     const unsigned long long seq = ++ctx->norm_seq;
-    hipLaunchKernelGGL(k_norm_out, dim3(1), dim3(64), 0, ctx->stream, W.dnorm, ngroups, ctx->hnorm_d, seq, ctx->bt,
-                       ctx->bn);
+    hipLaunchKernelGGL(k_norm_out, dim3(1), dim3(64 * ctx->bn), 0, ctx->stream, W.dnorm, ngroups, ctx->hnorm_d, seq,
+                       ctx->bt, ctx->bn);
     VD_CHECK_LAUNCH();
     volatile unsigned long long* flag = (volatile unsigned long long*)(ctx->hnorm + HNORM_SEQ);
     for (unsigned long it = 1; *flag != seq; it++) {

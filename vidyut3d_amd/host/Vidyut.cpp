@@ -906,9 +906,10 @@ int vd_sim_create(vd_ctx* ctx, const vd_mech_desc* mech, const vd_geom* geom, co
                   vd_sim** out) {
     if (!ctx || !mech || !geom || !prm || !out) return VD_EINVAL;
     if (geom->dim != 2 && geom->dim != 3) return VD_EINVAL;
-    // the cases of a batched context share one step sequence: each case's
-    // adaptive dt would differ
-    if (vd_ctx_ncase(ctx) > 1 && prm->adaptive_dt) return VD_EINVAL;
+    // the cases of a batched context share one step sequence and one
+    // simulation time: each case's adaptive dt would differ, and a case that
+    // joins later would see the batch's time in a time-dependent voltage
+    if (vd_ctx_ncase(ctx) > 1 && (prm->adaptive_dt || prm->voltage_profile != 0)) return VD_EINVAL;
     Vidyut* v = new Vidyut(ctx, *mech, *geom, *prm);
     if (v->status() != VD_OK) {
         int st = v->status();
