@@ -38,10 +38,8 @@ __global__ void __launch_bounds__(CHEM_BLOCK) k_rxn_table(ChemArgs a, Batch bt) 
     int64_t t = (int64_t)blockIdx.x * CHEM_BLOCK + tid;
     int64_t tot = (int64_t)a.nx * a.ny * a.nz;
     if (t >= tot) return;
-    int i = (int)(t % a.nx);
-    int64_t r0 = t / a.nx;
-    int j = (int)(r0 % a.ny);
-    int k = (int)(r0 / a.ny);
+    int i, j, k;
+    cell_of((unsigned)t, a.nx, a.ny, i, j, k);
     i += a.lo.x;
     j += a.lo.y;
     k += a.lo.z;
@@ -81,10 +79,8 @@ __global__ void __launch_bounds__(CHEM_BLOCK) k_rxn_ckwc(ChemArgs a, Batch bt) {
     int64_t t = (int64_t)blockIdx.x * CHEM_BLOCK + tid;
     int64_t tot = (int64_t)a.nx * a.ny * a.nz;
     if (t >= tot) return;
-    int i = (int)(t % a.nx);
-    int64_t r0 = t / a.nx;
-    int j = (int)(r0 % a.ny);
-    int k = (int)(r0 / a.ny);
+    int i, j, k;
+    cell_of((unsigned)t, a.nx, a.ny, i, j, k);
     i += a.lo.x;
     j += a.lo.y;
     k += a.lo.z;
@@ -225,10 +221,8 @@ __global__ void __launch_bounds__(CHEM_BLOCK) k_rxn_gen(ChemArgs a, const double
     int64_t t = (int64_t)blockIdx.x * CHEM_BLOCK + tid;
     int64_t tot = (int64_t)a.nx * a.ny * a.nz;
     if (t >= tot) return;
-    int i = (int)(t % a.nx);
-    int64_t r0 = t / a.nx;
-    int j = (int)(r0 % a.ny);
-    int k = (int)(r0 / a.ny);
+    int i, j, k;
+    cell_of((unsigned)t, a.nx, a.ny, i, j, k);
     i += a.lo.x;
     j += a.lo.y;
     k += a.lo.z;
@@ -330,9 +324,8 @@ __global__ void __launch_bounds__(256) k_user_src(ChemArgs a, Batch bt) {
     a.rebase(case_off(bt));
     const int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.nx * a.ny * a.nz) return;
-    const int i = (int)(t % a.nx);
-    const int64_t q = t / a.nx;
-    const int j = (int)(q % a.ny), k = (int)(q / a.ny);
+    int i, j, k;
+    cell_of((unsigned)t, a.nx, a.ny, i, j, k);
     const DevMech& M = *a.M;
     const int o = a.rxn.off(a.lo.x + i, a.lo.y + j, a.lo.z + k);
     for (int sp = 0; sp <= M.nspec; sp++)
@@ -424,30 +417,33 @@ struct JouleArgs {
 
 // Vidyut::compute_elecenergy_source (Source/ElecEnergySource.cpp:65-171)
 // STATE: the electron coefficients are fits of the face state (else
-// constants / Te only; a.state_trans mirrors it)
-template <bool STATE>
+// constants / Te only; a.state_trans mirrors it).  DIM: the face loop is
+// unrolled, so every face's loads are issued before the first is used;
+// the sum runs over the faces in the same order as a loop over a.dim
+template <bool STATE, int DIM>
 __global__ void __launch_bounds__(256) k_joule(JouleArgs a, Batch bt) {
-    // a stays unmodified (its face arrays are selected by direction at run
-    // time); the case's offset is applied where each array is read
+    // a stays unmodified; the case's offset is applied where each array is read
     const int64_t co = case_off(bt);
     const DevMech& M = *rb(a.M, co);
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     int64_t tot = (int64_t)a.nx * a.ny * a.nz;
     if (t >= tot) return;
-    int i = (int)(t % a.nx);
-    int64_t r0 = t / a.nx;
-    int j = (int)(r0 % a.ny);
-    int k = (int)(r0 / a.ny);
+    int i, j, k;
+    cell_of((unsigned)t, a.nx, a.ny, i, j, k);
     i += a.lo.x;
     j += a.lo.y;
     k += a.lo.z;
     const int ns = M.nspec, eidx = M.e_idx;
     const double charge = M.charge[eidx] * VD_ECHARGE;
     const int dlo[3] = {a.dlo.x, a.dlo.y, a.dlo.z}, dhi[3] = {a.dhi.x, a.dhi.y, a.dhi.z};
+    const double* S = rb(a.sb.p, co);
+    const int64_t N = a.sb.ns;
     double jheat = 0.0;
-    for (int idim = 0; idim < a.dim; idim++) {
+#pragma unroll
+    for (int idim = 0; idim < DIM; idim++) {
         const Arr<double>& EF = idim == 0 ? a.ef0 : (idim == 1 ? a.ef1 : a.ef2);
         const Arr<double>& GN = idim == 0 ? a.gn0 : (idim == 1 ? a.gn1 : a.gn2);
+#pragma unroll
         for (int f = 0; f < 2; f++) {
             int face[3] = {i, j, k};
             face[idim] += f;
@@ -456,8 +452,6 @@ __global__ void __launch_bounds__(256) k_joule(JouleArgs a, Batch bt) {
             int lc[3] = {face[0], face[1], face[2]};
             lc[idim] -= 1;
             const int ol = a.sb.off(lc[0], lc[1], lc[2]), orr = a.sb.off(face[0], face[1], face[2]);
-            const double* S = rb(a.sb.p, co);
-            const int64_t N = a.sb.ns;
             double etemp = 0.5 * (S[VD_ETEMP(ns) * N + ol] + S[VD_ETEMP(ns) * N + orr]);
             double ne = 0.5 * (S[(int64_t)eidx * N + ol] + S[(int64_t)eidx * N + orr]);
             double efield_face = at(EF, co, face[0], face[1], face[2]);
@@ -473,7 +467,7 @@ __global__ void __launch_bounds__(256) k_joule(JouleArgs a, Batch bt) {
                     // specMob / specDiff of the face-averaged state
                     // (ElecEnergySource.cpp:102-131)
                     double es = 0.0;
-                    for (int q = 0; q < a.dim; q++) {
+                    for (int q = 0; q < DIM; q++) {
                         const double e = 0.5 * (S[(int64_t)(VD_EFX(ns) + q) * N + ol] +
                                                 S[(int64_t)(VD_EFX(ns) + q) * N + orr]);
                         es += e * e;
@@ -525,8 +519,17 @@ int elecenergy_source(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, const 
     a.ny = box->hi[1] - box->lo[1] + 1;
     a.nz = box->hi[2] - box->lo[2] + 1;
     int64_t tot = (int64_t)a.nx * a.ny * a.nz;
-    if (a.state_trans) hipLaunchKernelGGL(k_joule<true>, bz(ctx, grid1d(tot, 256)), dim3(256), 0, ctx->stream, a, ctx->bt);
-    else hipLaunchKernelGGL(k_joule<false>, bz(ctx, grid1d(tot, 256)), dim3(256), 0, ctx->stream, a, ctx->bt);
+#define VD_JOULE(ST, D) hipLaunchKernelGGL((k_joule<ST, D>), bz(ctx, grid1d(tot, 256)), dim3(256), 0, ctx->stream, a, ctx->bt)
+    if (a.state_trans) {
+        if (a.dim == 3) VD_JOULE(true, 3);
+        else if (a.dim == 2) VD_JOULE(true, 2);
+        else VD_JOULE(true, 1);
+    } else {
+        if (a.dim == 3) VD_JOULE(false, 3);
+        else if (a.dim == 2) VD_JOULE(false, 2);
+        else VD_JOULE(false, 1);
+    }
+#undef VD_JOULE
     VD_CHECK_LAUNCH();
     return VD_OK;
 }

@@ -35,19 +35,20 @@ namespace vd {
     int sbo, rxo, sbj, sbk, rxj, rxk, nx, ny, nz, pad;                          \
     double Tg;                                                                  \
     long long cstride;                                                          \
-    unsigned char cidx[16];
+    unsigned cidxw[4];
 #define VD_STRX(...) #__VA_ARGS__
 #define VD_STR(...) VD_STRX(__VA_ARGS__)
 
 struct GenArgs {
     VD_GEN_ARGS_BODY
 };
-static_assert(VD_MAX_BATCH == 16, "GenArgs::cidx holds one entry per batched case");
+static_assert(VD_MAX_BATCH == 16, "GenArgs::cidxw holds one byte per batched case");
 
 // the generated kernel's first lines: its case's arrays (the batch
 // offset of blockIdx.z, as case_off gives it)
 static const char* GEN_PROLOGUE =
-    "  const long long co_ = a0.cstride * (long long)a0.cidx[blockIdx.z];\n"
+    "  const unsigned z_ = blockIdx.z;\n"
+    "  const long long co_ = a0.cstride * (long long)((a0.cidxw[z_ >> 2] >> ((z_ & 3u) * 8u)) & 0xffu);\n"
     "  GenArgs a = a0;\n"
     "  a.sb = (const double*)((const char*)a.sb + co_);\n"
     "  a.rx = (double*)((char*)a.rx + co_);\n"
@@ -392,7 +393,7 @@ int chem_jit_launch(vd_ctx* ctx, const vd_mech* m, const vd_array4* sb, vd_array
     a.pad = 0;
     a.Tg = Tg;
     a.cstride = ctx->bt.stride;
-    memcpy(a.cidx, ctx->bt.idx, sizeof(a.cidx));
+    memcpy(a.cidxw, ctx->bt.idx, sizeof(a.cidxw));
     const int64_t tot = (int64_t)a.nx * a.ny * a.nz;
     if (tot <= 0) return VD_OK;
     if (tot > 0xffffffffLL - GEN_BLOCK) {

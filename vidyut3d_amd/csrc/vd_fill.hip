@@ -121,10 +121,8 @@ __global__ void k_fill_patch(Arr<double> src, Arr<double> dst, int dim, int3 lo,
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     int64_t tot = (int64_t)nx * ny * nz;
     if (t >= tot) return;
-    int i = (int)(t % nx);
-    int64_t r = t / nx;
-    int j = (int)(r % ny);
-    int k = (int)(r / ny);
+    int i, j, k;
+    cell_of((unsigned)t, nx, ny, i, j, k);
     i += dst.bx;
     j += dst.by;
     k += dst.bz;

@@ -77,12 +77,23 @@ struct Arr {
 // and stride 0.
 struct Batch {
     int64_t stride;
-    unsigned char idx[VD_MAX_BATCH];
+    union {
+        unsigned char idx[VD_MAX_BATCH];
+        unsigned idxw[VD_MAX_BATCH / 4];  // the same bytes as four words
+    };
 };
+static_assert(VD_MAX_BATCH % 4 == 0, "Batch::idxw covers idx");
 // doubles of the host-mapped norm read-back buffer (values of up to
 // VD_MAX_BATCH cases x 5 slot groups, then a sequence number in the last)
 constexpr int HNORM_DOUBLES = 256;
-__device__ __forceinline__ int64_t case_off(const Batch& b) { return b.stride * (int64_t)b.idx[blockIdx.z]; }
+// the byte offset of this z-block's case.  The index is taken from its
+// aligned word: a byte load from the kernel arguments at a block-uniform
+// position has no scalar form and would be a vector load that every wave
+// waits for before its first data load
+__device__ __forceinline__ int64_t case_off(const Batch& b) {
+    const unsigned z = blockIdx.z;
+    return b.stride * (int64_t)((b.idxw[z >> 2] >> ((z & 3u) * 8u)) & 0xffu);
+}
 // p moved by o bytes (a null pointer stays null: it marks absent data)
 template <class T>
 __host__ __device__ __forceinline__ T* rb(T* p, int64_t o) {
@@ -435,6 +446,16 @@ __device__ __forceinline__ unsigned xcd_remap(unsigned b, unsigned nb) {
 }
 // logical (remapped) block index of a 1-D grid
 __device__ __forceinline__ unsigned lblock() { return xcd_remap(blockIdx.x, gridDim.x); }
+// cell (i, j, k), counted from 0, of linear index t in a box n0 cells wide
+// and n1 deep.  Boxes and arrays hold fewer than 2^31 cells (every entry
+// point checks), so the division is 32-bit unsigned, not the much longer
+// 64-bit sequence
+__device__ __forceinline__ void cell_of(unsigned t, int n0, int n1, int& i, int& j, int& k) {
+    const unsigned q = t / (unsigned)n0;
+    i = (int)(t - q * (unsigned)n0);
+    j = (int)(q % (unsigned)n1);
+    k = (int)(q / (unsigned)n1);
+}
 
 // a/b from the hardware reciprocal with two Newton steps and a final
 // residual correction: within an ulp of the rounded quotient for normal,

@@ -3419,18 +3419,16 @@ struct FlatArgs {
 __global__ void k_a4_to_flat(FlatArgs f, double* __restrict__ out) {
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)f.nx * f.ny * f.nz) return;
-    int i = (int)(t % f.nx);
-    int64_t r = t / f.nx;
-    int j = (int)(r % f.ny), k = (int)(r / f.ny);
+    int i, j, k;
+    cell_of((unsigned)t, f.nx, f.ny, i, j, k);
     out[t] = f.a(i + f.lo.x, j + f.lo.y, k + f.lo.z, f.comp);
 }
 
 __global__ void k_flat_to_a4(FlatArgs f, const double* __restrict__ in) {
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)f.nx * f.ny * f.nz) return;
-    int i = (int)(t % f.nx);
-    int64_t r = t / f.nx;
-    int j = (int)(r % f.ny), k = (int)(r / f.ny);
+    int i, j, k;
+    cell_of((unsigned)t, f.nx, f.ny, i, j, k);
     f.a(i + f.lo.x, j + f.lo.y, k + f.lo.z, f.comp) = in[t];
 }
 
@@ -3541,9 +3539,8 @@ __global__ void k_grad_faces(GradArgs a, Batch bt) {
     int n0 = a.nx + (a.d == 0), n1 = a.ny + (a.d == 1), n2 = a.nz + (a.d == 2);
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)n0 * n1 * n2) return;
-    int i = (int)(t % n0);
-    int64_t r = t / n0;
-    int j = (int)(r % n1), k = (int)(r / n1);
+    int i, j, k;
+    cell_of((unsigned)t, n0, n1, i, j, k);
     a.out(i + a.lo.x, j + a.lo.y, k + a.lo.z) = grad_face(a, i, j, k);
 }
 
@@ -3660,9 +3657,8 @@ __global__ void k_pot_rhs(PotRhsArgs a, double* __restrict__ f, Batch bt) {
     if (t >= (int64_t)a.nx * a.ny * a.nz) return;
     double r = 0.0;
     if (a.spacechrg) {
-        int i = (int)(t % a.nx);
-        int64_t q = t / a.nx;
-        int j = (int)(q % a.ny), k = (int)(q / a.ny);
+        int i, j, k;
+        cell_of((unsigned)t, a.nx, a.ny, i, j, k);
         int o = a.sb.off(i + a.lo.x, j + a.lo.y, k + a.lo.z);
         const DevMech& M = *a.M;
         for (int sp = 0; sp < M.nspec; sp++)
@@ -3703,6 +3699,22 @@ __device__ __forceinline__ double pot_face_pair(const GradArgs& g, const int c[3
     return 0.5 * (flo + fhi);
 }
 
+// both faces of direction D of a cell inside the level (no wall, no
+// periodic wrap): grad_face's interior branch for each, from the three
+// values of the line
+template <int D>
+__device__ __forceinline__ double pot_face_pair_in(const GradArgs& g, double xl, double xc, double xh, int gi, int gj,
+                                                   int gk) {
+    double flo = (xc - xl) / g.dx;
+    double fhi = (xh - xc) / g.dx;
+    if (g.sign != 1.0) {
+        flo *= g.sign;
+        fhi *= g.sign;
+    }
+    g.out(gi, gj, gk) = flo;
+    return 0.5 * (flo + fhi);
+}
+
 template <bool BT>
 __global__ void k_pot_post(PotPostArgs a, Batch bt) {
     if constexpr (BT) a.rebase(case_off(bt));
@@ -3714,14 +3726,46 @@ __global__ void k_pot_post(PotPostArgs a, Batch bt) {
     const int n[3] = {a.nx, a.ny, a.nz};
     const int i = c[0] + a.lo.x, j = c[1] + a.lo.y, k = c[2] + a.lo.z;
     const int ns = a.ns;
-    a.phi(i, j, k, VD_POT(ns)) = a.x[t];
-    double e[3] = {0.0, 0.0, 0.0};
-    e[0] = pot_face_pair<0>(a.g[0], c, n, i, j, k);
-    if (a.dim > 1) e[1] = pot_face_pair<1>(a.g[1], c, n, i, j, k);
-    if (a.dim > 2) e[2] = pot_face_pair<2>(a.g[2], c, n, i, j, k);
-    for (int d = 0; d < a.dim; d++) a.phi(i, j, k, VD_EFX(ns) + d) = e[d];
+    // the number density first (the species are not written here), its
+    // loads four at a time, summed in species order
     double nd = 0.0;
-    for (int sp = 0; sp < ns; sp++) nd += a.phi(i, j, k, sp);
+    {
+        const double* P = &a.phi(i, j, k, 0);
+        const int64_t N = a.phi.ns;
+        int sp = 0;
+        for (; sp + 4 <= ns; sp += 4) {
+            const double v0 = P[sp * N], v1 = P[(sp + 1) * N], v2 = P[(sp + 2) * N], v3 = P[(sp + 3) * N];
+            nd += v0;
+            nd += v1;
+            nd += v2;
+            nd += v3;
+        }
+        for (; sp < ns; sp++) nd += P[sp * N];
+    }
+    double e[3] = {0.0, 0.0, 0.0};
+    const bool in = c[0] > 0 && c[0] < n[0] - 1 && (a.dim < 2 || (c[1] > 0 && c[1] < n[1] - 1)) &&
+                    (a.dim < 3 || (c[2] > 0 && c[2] < n[2] - 1));
+    double xc;
+    if (in) {
+        // a cell with every face inside the level: the seven values of its
+        // stencil loaded together (a.x is the solution the gradients read)
+        const double* X = a.x;
+        const int sy = a.nx, sz = a.nx * a.ny;
+        xc = X[t];
+        const double xl = X[t - 1], xh = X[t + 1];
+        const double yl = a.dim > 1 ? X[t - sy] : 0.0, yh = a.dim > 1 ? X[t + sy] : 0.0;
+        const double zl = a.dim > 2 ? X[t - sz] : 0.0, zh = a.dim > 2 ? X[t + sz] : 0.0;
+        e[0] = pot_face_pair_in<0>(a.g[0], xl, xc, xh, i, j, k);
+        if (a.dim > 1) e[1] = pot_face_pair_in<1>(a.g[1], yl, xc, yh, i, j, k);
+        if (a.dim > 2) e[2] = pot_face_pair_in<2>(a.g[2], zl, xc, zh, i, j, k);
+    } else {
+        xc = a.x[t];
+        e[0] = pot_face_pair<0>(a.g[0], c, n, i, j, k);
+        if (a.dim > 1) e[1] = pot_face_pair<1>(a.g[1], c, n, i, j, k);
+        if (a.dim > 2) e[2] = pot_face_pair<2>(a.g[2], c, n, i, j, k);
+    }
+    a.phi(i, j, k, VD_POT(ns)) = xc;
+    for (int d = 0; d < a.dim; d++) a.phi(i, j, k, VD_EFX(ns) + d) = e[d];
     double es = 0.0;
     for (int d = 0; d < a.dim; d++) es += e[d] * e[d];
     a.phi(i, j, k, VD_REF(ns)) = (sqrt(es) / nd) / 1.0e-21;
@@ -3859,9 +3903,8 @@ __global__ void __launch_bounds__(256) k_cell_diff(CellDArgs a, double* __restri
     dcell = rb(dcell, case_off(bt));
     const int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.gnx * a.gny * a.gnz) return;
-    const int i = (int)(t % a.gnx);
-    const int64_t q = t / a.gnx;
-    const int j = (int)(q % a.gny), k = (int)(q / a.gny);
+    int i, j, k;
+    cell_of((unsigned)t, a.gnx, a.gny, i, j, k);
     const int gz = a.dim == 3 ? 1 : 0;
     const int o = a.sb.off(a.lo.x + i - 1, a.lo.y + j - 1, a.lo.z + k - gz);
     const double* S = a.sb.p;
@@ -3996,9 +4039,8 @@ __global__ void k_spec_rhs(SpecRhsArgs a, double* __restrict__ f, double* __rest
     x = rb(x, case_off(bt));
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.nx * a.ny * a.nz) return;
-    int i = (int)(t % a.nx);
-    int64_t q = t / a.nx;
-    int j = (int)(q % a.ny), k = (int)(q / a.ny);
+    int i, j, k;
+    cell_of((unsigned)t, a.nx, a.ny, i, j, k);
     i += a.lo.x;
     j += a.lo.y;
     k += a.lo.z;
@@ -4035,9 +4077,8 @@ __global__ void k_spec_post(SpecPostArgs a, Batch bt) {
     a.rebase(case_off(bt));
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.nx * a.ny * a.nz) return;
-    int i = (int)(t % a.nx);
-    int64_t q = t / a.nx;
-    int j = (int)(q % a.ny), k = (int)(q / a.ny);
+    int i, j, k;
+    cell_of((unsigned)t, a.nx, a.ny, i, j, k);
     i += a.lo.x;
     j += a.lo.y;
     k += a.lo.z;

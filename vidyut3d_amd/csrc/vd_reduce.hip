@@ -43,9 +43,8 @@ __global__ void __launch_bounds__(256) k_cfl(CflArgs a, Batch bt) {
     double m0 = 0.0, m1 = 0.0, m2 = 0.0;
     const DevMech& M = *a.M;
     for (; t < tot; t += (int64_t)gridDim.x * blockDim.x) {
-        int i = (int)(t % a.nx);
-        int64_t q = t / a.nx;
-        int j = (int)(q % a.ny), k = (int)(q / a.ny);
+        int i, j, k;
+        cell_of((unsigned)t, a.nx, a.ny, i, j, k);
         int o = a.phi.off(i + a.lo.x, j + a.lo.y, k + a.lo.z);
         const double* P = a.phi.p;
         const int64_t N = a.phi.ns;
@@ -165,9 +164,8 @@ struct TagArgs {
 __global__ void __launch_bounds__(256) k_tags(TagArgs a) {
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.nx * a.ny * a.nz) return;
-    int i = (int)(t % a.nx);
-    int64_t q = t / a.nx;
-    int j = (int)(q % a.ny), k = (int)(q / a.ny);
+    int i, j, k;
+    cell_of((unsigned)t, a.nx, a.ny, i, j, k);
     i += a.lo.x;
     j += a.lo.y;
     k += a.lo.z;

@@ -182,10 +182,8 @@ __global__ void __launch_bounds__(256) k_face_flux(FluxArgs a, Batch bt) {
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     int64_t tot = (int64_t)a.fnx * a.fny * a.fnz;
     if (t >= tot) return;
-    int i = (int)(t % a.fnx);
-    int64_t r = t / a.fnx;
-    int j = (int)(r % a.fny);
-    int k = (int)(r / a.fny);
+    int i, j, k;
+    cell_of((unsigned)t, a.fnx, a.fny, i, j, k);
     i += a.flo.x;
     j += a.flo.y;
     k += a.flo.z;
@@ -426,10 +424,8 @@ __global__ void __launch_bounds__(256) k_dsdt(DsdtArgs a, Batch bt) {
     int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     int64_t tot = (int64_t)a.nx * a.ny * a.nz;
     if (t >= tot) return;
-    int i = (int)(t % a.nx);
-    int64_t r = t / a.nx;
-    int j = (int)(r % a.ny);
-    int k = (int)(r / a.ny);
+    int i, j, k;
+    cell_of((unsigned)t, a.nx, a.ny, i, j, k);
     i += a.lo.x;
     j += a.lo.y;
     k += a.lo.z;
@@ -556,9 +552,8 @@ __global__ void __launch_bounds__(256) k_drift_vel(DriftArgs a, Batch bt) {
     a.rebase(case_off(bt));
     const int64_t t = (int64_t)lblock() * blockDim.x + threadIdx.x;
     if (t >= (int64_t)a.nx * a.ny * a.nz) return;
-    const int i = (int)(t % a.nx);
-    const int64_t q = t / a.nx;
-    const int j = (int)(q % a.ny), k = (int)(q / a.ny);
+    int i, j, k;
+    cell_of((unsigned)t, a.nx, a.ny, i, j, k);
     const DevMech& M = *a.M;
     const int ns = M.nspec;
     const int o = a.sb.off(a.sb.bx + i, a.sb.by + j, a.sb.bz + k);
