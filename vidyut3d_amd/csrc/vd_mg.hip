@@ -96,6 +96,20 @@ struct Lev {
     T* w = nullptr;
 };
 
+// The implicit species solve's copy-back (ScalarSolve.cpp:632-647: the
+// solution into the state's component, with the density floor), written by
+// a sweep-plus-norm pass for every iterate it produces, so that the state
+// holds the final iterate when the iterations stop and no copy pass
+// follows.  p: the component at the level's cell (0,0,0); bytes: its
+// extent from p (below 2^31, the pass's byte offsets).
+struct SweepOut {
+    double* p;
+    int js, ks;
+    unsigned bytes;
+    int floor_on;
+    double floor_val;
+};
+
 struct MGWork {
     vd_ctx* ctx = nullptr;  // the context whose memory holds the arrays
     int dim = 3, nlev = 0, prec = 0, constb = 0;
@@ -111,6 +125,11 @@ struct MGWork {
     double* bf0[3] = {nullptr, nullptr, nullptr};
     double* c00[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
     double* bcv[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    // the inhomogeneous ghost part of each side's faces, when the rhs pass
+    // adds the boundary terms (beta/dx^2 * b * it) itself
+    double* sh[3][2] = {{nullptr, nullptr}, {nullptr, nullptr}, {nullptr, nullptr}};
+    // the initial residual and rhs norms are already in slot groups 0 and 1
+    bool r0_done = false;
     double* rob[3][2][3];  // robin a, b, f per side
     unsigned long long* dnorm = nullptr;  // 8 groups of NSLOT norm slots
     double* dcell = nullptr;              // cell coefficient on the grown box
@@ -136,6 +155,10 @@ struct MGWork {
     mutable void* tail_dev = nullptr;
     mutable std::vector<unsigned char> tail_key;
     mutable unsigned tail_mask = 0;
+    // an implicit species solve's copy-back, written by its fused sweeps
+    // (post.p null: none), and the cases whose final iterate they wrote
+    SweepOut post{};
+    unsigned post_mask = 0;
 };
 
 void mgwork_free(MGWork* w) {
@@ -401,12 +424,16 @@ __device__ __forceinline__ void fused_unroll(F& f, int t, int tend) {
 // BT: a batched launch (each z-block moves the arrays by its case's
 // offset); unbatched launches take the BT = false kernel, whose registers
 // the offsets do not take
-template <class T, int BM, int S, int RES, int ZI, int ACC = 0, bool BT = false>
+// POST (FP64, BM 1, one sweep and its norm): the iterate also written to
+// the state as SweepOut describes
+template <class T, int BM, int S, int RES, int ZI, int ACC = 0, bool BT = false, int POST = 0>
 __global__ void __launch_bounds__(FzTile<BM>::THREADS) k_gsrb_fused(LevV<T> L, const T* __restrict__ xin,
                                                            T* __restrict__ xout, const T* __restrict__ rhs,
                                                            int zc, int ntx, int nty,
                                                            double* __restrict__ part,
-                                                           double* __restrict__ xacc, Batch bt) {
+                                                           double* __restrict__ xacc, SweepOut so, Batch bt) {
+    static_assert(!POST || (std::is_same<T, double>::value && BM == 1 && S == 2 && RES == 1 && !ZI && !ACC),
+                  "POST: the FP64 constant-coefficient sweep and its norm");
     if constexpr (BT) {
         const int64_t co = case_off(bt);
         L.template rebase_used<BM, false>(co);
@@ -415,6 +442,7 @@ __global__ void __launch_bounds__(FzTile<BM>::THREADS) k_gsrb_fused(LevV<T> L, c
         rhs = rb(rhs, co);
         part = rb(part, co);
         xacc = rb(xacc, co);
+        if constexpr (POST) so.p = rb(so.p, co);
     }
     // Register ring indexed by plane: plane p lives in slot (p - zlo) mod NV
     // (NV even), and the step loop is unrolled NV times, so every slot index
@@ -496,6 +524,10 @@ __global__ void __launch_bounds__(FzTile<BM>::THREADS) k_gsrb_fused(LevV<T> L, c
     };
     const __amdgpu_buffer_rsrc_t ra = buf_rsrc(xacc, (unsigned)((int64_t)pstr * L.nz * 8));
     auto offd = [&](int pc, bool ok, int p) -> unsigned { return ok ? (unsigned)(pc + pstr * p) * 8u : OOB; };
+    // POST: the state component (cells written are inside the level, so
+    // their unwrapped and wrapped indices agree)
+    const __amdgpu_buffer_rsrc_t rpo = buf_rsrc(POST ? (const void*)so.p : nullptr, POST ? so.bytes : 0u);
+    const int po0 = POST ? ic0c + so.js * jcc : 0;  // cell 1 of a written pair is at po0 + 1
 
     T vr[NV], vb[NV], fr[NV], fb[NV];
 #pragma unroll
@@ -853,6 +885,12 @@ __global__ void __launch_bounds__(FzTile<BM>::THREADS) k_gsrb_fused(LevV<T> L, c
                 buf_st<T>(ro, offs(pc0, outp && oc0, po), o0);
                 buf_st<T>(ro, offs(pc1, outp && oc1, po), o1);
             }
+            if constexpr (POST) {
+                const T w0 = (so.floor_on && o0 < so.floor_val) ? T(so.floor_val) : o0;
+                const T w1 = (so.floor_on && o1 < so.floor_val) ? T(so.floor_val) : o1;
+                buf_st<T>(rpo, outp && oc0 ? (unsigned)(po0 + so.ks * po) * 8u : OOB, w0);
+                buf_st<T>(rpo, outp && oc1 ? (unsigned)(po0 + 1 + so.ks * po) * 8u : OOB, w1);
+            }
         }
         load(q, t + 3);
         if constexpr (BM == 2) load_d(pd, t + 2);  // one step ahead
@@ -1191,6 +1229,144 @@ __global__ void __launch_bounds__(RZ2_X * RZ2_Y) k_resid_z2(LevV<double> L, cons
         double mm = red[0][q];
         for (int w = 1; w < RZ2_X * RZ2_Y / 64; w++) mm = fmax(mm, red[w][q]);
         atomic_max_nonneg(slots + ((size_t)q * NSLOT + lb % NSLOT) * SLOT_STRIDE, mm);
+    }
+}
+
+// The rhs of an implicit species solve on the constant-coefficient stencil
+// and the norms of its initial residual in one pass (3-D, even nx, the
+// streamed layout of k_resid_z2), so that neither f nor x is read back:
+//   f = (1/dt) n_old + dsdt (ScalarSolve.cpp:443-465), then on a wall cell
+//       + beta/dx_d^2 * b * cinh of each wall face, directions in order
+//       (the terms side_setup adds to f; cinh from side_setup's W.sh),
+//   x = Sborder's component (the initial guess),
+// and max |f - L x|, max |f| into slot groups 0 and 1, as
+// k_resid_z2<true, C0U> leaves them.  Per cell the arithmetic of
+// k_spec_rhs, side_setup and k_resid_z2, the contractions written out.
+struct SpecR0Args {
+    Arr<double> old, ds, sb;  // the species component of old and sb; ds component 0
+    int spec;
+    int3 lo;
+    double idt;
+    const double* sh[3][2];  // cinh by face of each wall side (null on periodic sides)
+    double sbd[3];           // beta/dx_d^2 * b
+    double* f;
+    double* x;
+    unsigned long long* slots;
+    __device__ void rebase(int64_t o) {
+        vd::rebase(old, o);
+        vd::rebase(ds, o);
+        vd::rebase(sb, o);
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int s = 0; s < 2; s++) sh[d][s] = vd::rb(sh[d][s], o);
+        f = vd::rb(f, o);
+        x = vd::rb(x, o);
+        slots = vd::rb(slots, o);
+    }
+};
+
+template <bool C0U>
+__global__ void __launch_bounds__(RZ2_X * RZ2_Y) k_spec_rhs_r0(LevV<double> L, SpecR0Args a, int zc, int ntx,
+                                                               int nty, Batch bt) {
+    {
+        const int64_t co = case_off(bt);
+        L.template rebase_used<1, !C0U>(co);
+        a.rebase(co);
+    }
+    __shared__ double red[RZ2_X * RZ2_Y / 64][2];
+    const unsigned lb = lblock();
+    const int ntile = ntx * nty;
+    const int tile = (int)(lb % (unsigned)ntile), zb = (int)(lb / (unsigned)ntile);
+    const int i0 = 2 * ((tile % ntx) * RZ2_X + (int)threadIdx.x % RZ2_X);
+    const int j = (tile / ntx) * RZ2_Y + (int)threadIdx.x / RZ2_X;
+    const int z0 = zb * zc, z1 = min(z0 + zc, L.nz);
+    const bool own = i0 < L.nx && j < L.ny;  // both cells of the pair (nx even)
+    const int ic = min(i0, L.nx - 2), jc = min(j, L.ny - 1);
+    const int pstr = L.nx * L.ny, c2 = ic + L.nx * jc;
+    // x from Sborder in its own layout: the pair on level plane 0, the
+    // strides, and the neighbours as k_resid_z2 takes them (wrapped on a
+    // periodic side, a cell of the pair on a wall)
+    const double* S = a.sb.comp(a.spec);
+    const int sj = a.sb.js, sk = a.sb.ks;
+    const int s2 = a.sb.off(a.lo.x + ic, a.lo.y + jc, a.lo.z);
+    const int oxl = ic > 0 ? -1 : (L.per0 ? L.nx - 1 : 0);
+    const int oxh = ic + 1 < L.nx - 1 ? 2 : (L.per0 ? 2 - L.nx : 1);
+    const int oyl = jc > 0 ? -sj : (L.per1 ? (L.ny - 1) * sj : 0);
+    const int oyh = jc < L.ny - 1 ? sj : (L.per1 ? (1 - L.ny) * sj : 0);
+    auto zw = [&](int k) {
+        return k < 0 ? (L.per2 ? k + L.nz : 0) : (k >= L.nz ? (L.per2 ? k - L.nz : L.nz - 1) : k);
+    };
+    auto ld2 = [&](int o) { return make_double2(S[o], S[o + 1]); };
+    const double* O = a.old.comp(a.spec);
+    const int o2 = a.old.off(a.lo.x + ic, a.lo.y + jc, a.lo.z), ok = a.old.ks;
+    const double* D = a.ds.p;
+    const int d2 = a.ds.off(a.lo.x + ic, a.lo.y + jc, a.lo.z), dk = a.ds.ks;
+    double2 xm = ld2(s2 + sk * zw(z0 - 1)), xc = ld2(s2 + sk * z0), xp = ld2(s2 + sk * zw(z0 + 1));
+    double m[2] = {0.0, 0.0};
+    for (int k = z0; k < z1; k++) {
+        const int ck = c2 + pstr * k, sc = s2 + sk * k;
+        const double2 xn = ld2(s2 + sk * zw(k + 2));  // two planes ahead
+        const double xl = S[sc + oxl], xh = S[sc + oxh];
+        const double2 yl = ld2(sc + oyl), yh = ld2(sc + oyh);
+        double fv[2];
+#pragma unroll
+        for (int c = 0; c < 2; c++) fv[c] = fma(a.idt, O[o2 + ok * k + c], D[d2 + dk * k + c]);
+        // the wall terms, direction by direction and low side first
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            const int i = ic + c;
+            if (!L.per0) {
+                if (i == 0) fv[c] = fma(a.sbd[0], a.sh[0][0][jc + L.ny * k], fv[c]);
+                if (i == L.nx - 1) fv[c] = fma(a.sbd[0], a.sh[0][1][jc + L.ny * k], fv[c]);
+            }
+            if (!L.per1) {
+                if (jc == 0) fv[c] = fma(a.sbd[1], a.sh[1][0][i + L.nx * k], fv[c]);
+                if (jc == L.ny - 1) fv[c] = fma(a.sbd[1], a.sh[1][1][i + L.nx * k], fv[c]);
+            }
+            if (!L.per2) {
+                if (k == 0) fv[c] = fma(a.sbd[2], a.sh[2][0][i + L.nx * jc], fv[c]);
+                if (k == L.nz - 1) fv[c] = fma(a.sbd[2], a.sh[2][1][i + L.nx * jc], fv[c]);
+            }
+        }
+        double v[2];
+#pragma unroll
+        for (int c = 0; c < 2; c++) {
+            auto nb = [&](int d, int side) -> double {
+                if (d == 0) return c == 0 ? (side ? xc.y : xl) : (side ? xh : xc.x);
+                if (d == 1) return side ? (c ? yh.y : yh.x) : (c ? yl.y : yl.x);
+                return side ? (c ? xp.y : xp.x) : (c ? xm.y : xm.x);
+            };
+            auto fc = [&](int d, double& blo, double& bhi) { face_coef<double, 1, 3>(L, d, ic + c, jc, k, blo, bhi); };
+            double dg, of;
+            stencil_nbf<double, 3, decltype(nb), decltype(fc), C0U>(L, ic + c, jc, k, nb, fc, dg, of);
+            v[c] = fv[c] - (dg * (c ? xc.y : xc.x) - of);
+        }
+        if (own) {
+            *(double2*)(a.f + ck) = make_double2(fv[0], fv[1]);
+            *(double2*)(a.x + ck) = xc;
+#pragma unroll
+            for (int c = 0; c < 2; c++) {
+                const double r = fabs(v[c]), b = fabs(fv[c]);
+                m[0] = fmax(m[0], r != r ? __builtin_huge_val() : r);
+                m[1] = fmax(m[1], b != b ? __builtin_huge_val() : b);
+            }
+        }
+        xm = xc;
+        xc = xp;
+        xp = xn;
+    }
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+        for (int o = 32; o > 0; o >>= 1) m[q] = fmax(m[q], __shfl_xor(m[q], o, 64));
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6][q] = m[q];
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        const int q = (int)threadIdx.x;
+        double mm = red[0][q];
+        for (int w = 1; w < RZ2_X * RZ2_Y / 64; w++) mm = fmax(mm, red[w][q]);
+        atomic_max_nonneg(a.slots + ((size_t)q * NSLOT + lb % NSLOT) * SLOT_STRIDE, mm);
     }
 }
 
@@ -2103,7 +2279,9 @@ struct SideArgs {
     double bconst;
     double* c0;
     double* f;         // rhs to shift
+    double* sh;        // or, when not null, the side's cinh stored by face
     __device__ void rebase(int64_t o) {
+        sh = vd::rb(sh, o);
         bcv = vd::rb(bcv, o);
         ra = vd::rb(ra, o);
         rb = vd::rb(rb, o);
@@ -2156,8 +2334,13 @@ __global__ void k_side_setup(SideArgs a0, SideArgs a1, Batch bt) {
     } else {
         b = a.bconst;
     }
-    int cell = c[0] + a.nx * (c[1] + a.ny * c[2]);
-    a.f[cell] += a.s * b * cinh;
+    if (a.sh) {
+        // the rhs pass adds s * b * cinh itself (b is the constant bconst)
+        a.sh[ti] = cinh;
+    } else {
+        const int cell = c[0] + a.nx * (c[1] + a.ny * c[2]);
+        a.f[cell] = fma(a.s * b, cinh, a.f[cell]);
+    }
 }
 
 // ------------------------------------------------------------ host side
@@ -2264,7 +2447,7 @@ static int fused_zc_cc(const int n[3]) {
 
 template <class T, int BM, int S, int RES>
 static void launch_fused_k(vd_ctx* ctx, const LevV<T>& v, int zc, const T* xin, T* xout, const T* rhs,
-                           double* part, int* nblk, double* xacc = nullptr) {
+                           double* part, int* nblk, double* xacc = nullptr, const SweepOut* so = nullptr) {
     // the kernel reads xin unless it starts from zero (ZI)
     constexpr int TIX = FzTile<BM>::IX, TT = FzTile<BM>::THREADS;
     const int ntx = (v.nx + TIX - 1) / TIX, nty = (v.ny + FZ_IY - 1) / FZ_IY;
@@ -2272,14 +2455,22 @@ static void launch_fused_k(vd_ctx* ctx, const LevV<T>& v, int zc, const T* xin, 
     const int nb = ntx * nty * nzc;
     if (nblk) *nblk = nb;
     const bool bt = ctx->ncase > 1;
-#define VD_FUSED_LAUNCH(ZIV, ACCV, XACC)                                                                          \
-    {                                                                                                             \
-        if (bt)                                                                                                   \
-            hipLaunchKernelGGL((k_gsrb_fused<T, BM, S, RES, ZIV, ACCV, true>), bz(ctx, dim3((unsigned)nb)),       \
-                               dim3(TT), 0, ctx->stream, v, xin, xout, rhs, zc, ntx, nty, part, XACC, ctx->bt);    \
-        else                                                                                                      \
-            hipLaunchKernelGGL((k_gsrb_fused<T, BM, S, RES, ZIV, ACCV, false>), dim3((unsigned)nb), dim3(TT), 0,  \
-                               ctx->stream, v, xin, xout, rhs, zc, ntx, nty, part, XACC, ctx->bt);                \
+    const SweepOut sw = so ? *so : SweepOut{};
+#define VD_FUSED_LAUNCH_P(ZIV, ACCV, XACC, POSTV)                                                                     \
+    {                                                                                                                 \
+        if (bt)                                                                                                       \
+            hipLaunchKernelGGL((k_gsrb_fused<T, BM, S, RES, ZIV, ACCV, true, POSTV>), bz(ctx, dim3((unsigned)nb)),    \
+                               dim3(TT), 0, ctx->stream, v, xin, xout, rhs, zc, ntx, nty, part, XACC, sw, ctx->bt);    \
+        else                                                                                                          \
+            hipLaunchKernelGGL((k_gsrb_fused<T, BM, S, RES, ZIV, ACCV, false, POSTV>), dim3((unsigned)nb), dim3(TT),  \
+                               0, ctx->stream, v, xin, xout, rhs, zc, ntx, nty, part, XACC, sw, ctx->bt);             \
+    }
+#define VD_FUSED_LAUNCH(ZIV, ACCV, XACC) VD_FUSED_LAUNCH_P(ZIV, ACCV, XACC, 0)
+    if constexpr (std::is_same<T, double>::value && BM == 1 && S == 2 && RES == 1) {
+        if (xin && so && so->p) {
+            VD_FUSED_LAUNCH_P(0, 0, (double*)nullptr, 1)
+            return;
+        }
     }
     if constexpr (RES == 0) {
         if (xin && xacc) {
@@ -2292,6 +2483,7 @@ static void launch_fused_k(vd_ctx* ctx, const LevV<T>& v, int zc, const T* xin, 
     else if constexpr (RES == 0)
         VD_FUSED_LAUNCH(1, 0, (double*)nullptr)
 #undef VD_FUSED_LAUNCH
+#undef VD_FUSED_LAUNCH_P
 }
 
 // nsweep (1 or 2) red-black sweeps of x (zeros when xin is null) into
@@ -2302,13 +2494,14 @@ static void launch_fused_k(vd_ctx* ctx, const LevV<T>& v, int zc, const T* xin, 
 // timed as the VD_KT_FUSED family
 template <class T>
 static void launch_fused(vd_ctx* ctx, const MGWork& W, int bm, const LevV<T>& v, int zc, const T* xin, T* xout,
-                         const T* rhs, int nsweep, int res, double* part, int* nblk, double* xacc = nullptr) {
+                         const T* rhs, int nsweep, int res, double* part, int* nblk, double* xacc = nullptr,
+                         const SweepOut* so = nullptr) {
     const bool fine = v.nx == W.n[0] && v.ny == W.n[1] && v.nz == W.n[2];
     if (fine) kt_begin(ctx, VD_KT_FUSED);
 #define VD_FUSED_BM(BMV)                                                                          \
     if (nsweep == 2) launch_fused_k<T, BMV, 4, 0>(ctx, v, zc, xin, xout, rhs, part, nblk, xacc);     \
     else if constexpr (std::is_same<T, double>::value) {                                          \
-        if (res) launch_fused_k<T, BMV, 2, 1>(ctx, v, zc, xin, xout, rhs, part, nblk);          \
+        if (res) launch_fused_k<T, BMV, 2, 1>(ctx, v, zc, xin, xout, rhs, part, nblk, nullptr, so); \
         else launch_fused_k<T, BMV, 2, 0>(ctx, v, zc, xin, xout, rhs, part, nblk, xacc);        \
     } else launch_fused_k<T, BMV, 2, 0>(ctx, v, zc, xin, xout, rhs, part, nblk, xacc);
     if (bm == 1) {
@@ -2327,7 +2520,8 @@ static void launch_fused(vd_ctx* ctx, const MGWork& W, int bm, const LevV<T>& v,
     const double N = (double)v.nx * v.ny * v.nz;
     if (fine)
         kt_end(ctx, VD_KT_FUSED,
-               sizeof(T) * N * ((xin ? 1 : 0) + 1 + nb) + (xacc ? 16.0 * N : (double)sizeof(T) * N));
+               sizeof(T) * N * ((xin ? 1 : 0) + 1 + nb) + (xacc ? 16.0 * N : (double)sizeof(T) * N) +
+                   (so && so->p && bm == 1 && res ? 8.0 * N : 0.0));
 }
 
 template <class T>
@@ -2527,6 +2721,8 @@ static int get_work(vd_ctx* ctx, const vd_geom* g, const vd_box* box, int prec, 
                 W->c00[d][s] = (double*)p;
                 if ((st = dalloc(W, ntanf(n, d) * 8, &p))) goto fail;
                 W->bcv[d][s] = (double*)p;
+                if ((st = dalloc(W, ntanf(n, d) * 8, &p))) goto fail;
+                W->sh[d][s] = (double*)p;
                 for (int q = 0; q < 3; q++) {
                     if ((st = dalloc(W, ntanf(n, d) * 8, &p))) goto fail;
                     W->rob[d][s][q] = (double*)p;
@@ -2986,7 +3182,7 @@ static int read_groups(vd_ctx* ctx, MGWork& W, int ngroups, double* out) {
 // the kernel allows (*in32 says whether it did).
 static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double* nrm_out,
                   double* bnorm = nullptr, bool write_res = true, double* fmax = nullptr,
-                  float* res32 = nullptr, bool* in32 = nullptr) {
+                  float* res32 = nullptr, bool* in32 = nullptr, bool read = true) {
     if (in32) *in32 = false;
     int64_t nc = ncell(W.n);
     const int gx = (W.n[0] + 255) / 256;
@@ -3078,6 +3274,7 @@ static int resid0(vd_ctx* ctx, MGWork& W, int bm, const LevV<double>& v0, double
     const int nb = bm == 1 ? 0 : (bm == 0 ? W.dim : 1);
     kt_end(ctx, VD_KT_RESID, 8.0 * (double)nc * ((write_res ? 4 : 3) + nb));
     VD_CHECK_LAUNCH();
+    if (!read) return VD_OK;  // the caller reads the slot groups
     double m[VD_MAX_BATCH * 5];
     const int ng = fmax ? 2 + W.dim : (bnorm ? 2 : 1);
     VD_TRY(read_groups(ctx, W, ng, m));
@@ -3154,11 +3351,10 @@ static int jacobi_rho(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
 // x_zero: the solution array holds zeros on entry (a solve from a zero
 // initial guess), so the initial residual is the rhs itself: its norm needs
 // one read of f, and the first V-cycle takes f as its finest rhs
-static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta, double bconst,
-                      const double* dcell, const int tlo[3], const int thi[3], int smooth_mode,
-                      double reltol, double abstol, int maxiter, int nu1, int nu2, vd_solve_info* info,
-                      bool x_zero = false) {
-    int64_t nc = ncell(W.n);
+// the finest level's operator view for these boundary types (sets the
+// uniform-c0 flags of W)
+static LevV<double> fine_view(MGWork& W, const int tlo[3], const int thi[3], double alpha, double beta,
+                              double bconst, const double* dcell) {
     W.c0uni = 1;
     for (int d = 0; d < 3; d++)
         for (int sd = 0; sd < 2; sd++) {
@@ -3175,6 +3371,18 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
     }
     LevV<double> v0 = view<double>(W, fine, alpha, beta, bconst);
     v0.dcell = dcell;
+    return v0;
+}
+
+static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta, double bconst,
+                      const double* dcell, const int tlo[3], const int thi[3], int smooth_mode,
+                      double reltol, double abstol, int maxiter, int nu1, int nu2, vd_solve_info* info,
+                      bool x_zero = false) {
+    int64_t nc = ncell(W.n);
+    const LevV<double> v0 = fine_view(W, tlo, thi, alpha, beta, bconst, dcell);
+    // the caller's rhs pass already left the initial norms in slot groups 0, 1
+    const bool r0_done = W.r0_done;
+    W.r0_done = false;
     // Every case of the launch batch (ctx->bt, ctx->bn) takes the
     // operations it would take alone: its own initial norms, its own
     // choice of finest-level sweeps or V-cycles, its own stopping
@@ -3190,6 +3398,15 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
         iters[z] = 0;
     }
     bool initial_done = false;
+    W.post_mask = 0;
+    const SweepOut* post = W.post.p && bm == 1 ? &W.post : nullptr;
+    // constant coefficient, sweep-only iterations on the fused pass: the
+    // first sweep (and its norm, into slot group 2) is launched right after
+    // the initial residual, and one read-back returns both norms; a case
+    // whose initial residual already meets its target keeps its initial
+    // guess, and the sweep's result in the other buffer is not used
+    bool spec1 = false;
+    double r1[VD_MAX_BATCH];
     if (smooth_mode == 2 && bm == 1) {
         // constant coefficient: every face is b, so the bound needs no pass,
         // and the residual is stored only when V-cycles follow
@@ -3197,7 +3414,25 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
         for (int d = 0; d < W.dim; d++) rho += 2.0 * fabs(beta / (W.dx[d] * W.dx[d])) * fabs(bconst);
         const bool so = alpha != 0.0 && rho / fabs(alpha) <= SMOOTH_ONLY_RHO;
         for (int z = 0; z < ncs; z++) sonly[z] = so;
-        VD_TRY(resid0(ctx, W, bm, v0, r0, bnorm, !so));
+        if (so && W.x2 && W.c0uni && maxiter >= 1) {
+            if (!r0_done) VD_TRY(resid0(ctx, W, bm, v0, r0, bnorm, false, nullptr, nullptr, nullptr, false));
+            int nblk = 0;
+            launch_fused<double>(ctx, W, bm, v0, std::max(1, fused_zc(W.n)), W.x, W.x2, W.f, 1, 1, W.part, &nblk,
+                                 nullptr, post);
+            hipLaunchKernelGGL(k_reduce_partials, bz(ctx, dim3(1)), dim3(1024), 0, ctx->stream, W.part, nblk,
+                               slot_group(W, 2), ctx->bt);
+            VD_CHECK_LAUNCH();
+            double m[VD_MAX_BATCH * 5];
+            VD_TRY(read_groups(ctx, W, 3, m));
+            for (int z = 0; z < ncs; z++) {
+                r0[z] = m[3 * z];
+                bnorm[z] = m[3 * z + 1];
+                r1[z] = m[3 * z + 2];
+            }
+            spec1 = true;
+        } else {
+            VD_TRY(resid0(ctx, W, bm, v0, r0, bnorm, !so));
+        }
         initial_done = true;
     } else if (smooth_mode == 2) {
         double bmax[3 * VD_MAX_BATCH] = {0.0};
@@ -3321,11 +3556,15 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
         for (int it = 1; it <= maxiter && na > 0; it++) {
             BatchScope iscope(ctx, act, na);
             double m[VD_MAX_BATCH];
-            if (fused_so) {
+            if (fused_so && it == 1 && spec1) {
+                // the sweep launched with the initial residual
+                std::swap(W.x, W.x2);
+                for (int q = 0; q < na; q++) m[q] = r1[az[q]];
+            } else if (fused_so) {
                 // one red-black sweep and the residual norm of its result in one pass
                 int nblk = 0;
                 launch_fused<double>(ctx, W, bmg, v0, bmg == 2 ? fused_zc_cc(W.n) : std::max(1, fused_zc(W.n)), W.x,
-                                     W.x2, W.f, 1, 1, W.part, &nblk);
+                                     W.x2, W.f, 1, 1, W.part, &nblk, nullptr, bmg == 1 ? post : nullptr);
                 std::swap(W.x, W.x2);
                 hipLaunchKernelGGL(k_reduce_partials, bz(ctx, dim3(1)), dim3(1024), 0, ctx->stream, W.part, nblk,
                                    slot_group(W, 0), ctx->bt);
@@ -3375,6 +3614,10 @@ static int mg_iterate(vd_ctx* ctx, MGWork& W, int bm, double alpha, double beta,
             set_error("multigrid: no convergence in %d iterations (|r|=%g target=%g)", maxiter, rn[z], target[z]);
         }
         if (fused_so) {
+            // the cases whose last iterate the sweeps also wrote to the state
+            if (post && bmg == 1)
+                for (int q = 0; q < ng; q++)
+                    if (iters[gz[q]] >= 1) W.post_mask |= 1u << grp[q];
             // the cases that stopped with their solution in the other
             // buffer copy it into W.x, where every case's solution is read
             int cp[VD_MAX_BATCH], ncp = 0;
@@ -3583,8 +3826,10 @@ static int grad_faces(vd_ctx* ctx, MGWork& W, const int tlo[3], const int thi[3]
 
 // c0 and rhs shift for every non-periodic side (sequential sides: no races
 // on corner cells).  bcv / rob must be filled.
+// to_sh: each face's cinh goes to W.sh (for a rhs pass that adds the
+// terms, constant b only) instead of its term into f
 static int side_setup(vd_ctx* ctx, MGWork& W, const int tlo[3], const int thi[3], double beta,
-                      double bconst, const double* dcell = nullptr, bool use_const = false) {
+                      double bconst, const double* dcell = nullptr, bool use_const = false, bool to_sh = false) {
     for (int d = 0; d < W.dim; d++) {
         if (W.per[d]) continue;
         SideArgs sa[2];
@@ -3608,7 +3853,8 @@ static int side_setup(vd_ctx* ctx, MGWork& W, const int tlo[3], const int thi[3]
             a.dcell = dcell;
             a.bconst = bconst;
             a.c0 = W.c00[d][s];
-            a.f = W.f;
+            a.f = to_sh ? nullptr : W.f;
+            a.sh = to_sh ? W.sh[d][s] : nullptr;
         }
         // a direction one cell thick has both sides on the same cells:
         // keep them in order there
@@ -4221,26 +4467,81 @@ static int implicit_solve_body(vd_ctx* ctx, const vd_mech* m, int spec, const vd
                                       user ? m->host.robin[spec][d][s][q] : (q == 1 ? 1.0 : 0.0)));
             }
     }
-    SpecRhsArgs r;
-    r.old = arr(old);
-    r.ds = arr(ds);
-    r.sb = arr(sb);
-    r.spec = spec;
-    r.lo = make_int3(lo[0], lo[1], lo[2]);
-    r.nx = n[0];
-    r.ny = n[1];
-    r.nz = n[2];
-    r.idt = 1.0 / dt;
-    hipLaunchKernelGGL(k_spec_rhs, bz(ctx, grid1d(nc, 256)), dim3(256), 0, ctx->stream, r, W->f, W->x, ctx->bt);
-    VD_CHECK_LAUNCH();
     const double alpha = 1.0 / dt, beta = 1.0;
     const double* dcell = constd ? nullptr : W->dcell;
-    VD_TRY(side_setup(ctx, *W, bclo, bchi, beta, bc, dcell, constd));
+    // constant coefficient in 3-D on the streamed layout: the rhs, its
+    // boundary terms and the initial residual's norms in one pass (the
+    // boundary setup first, keeping each face's cinh); else the rhs pass
+    // and then the boundary terms added to it
+    if (constd && g->dim == 3 && W->stream && n[0] % 2 == 0) {
+        VD_TRY(side_setup(ctx, *W, bclo, bchi, beta, bc, dcell, constd, true));
+        const LevV<double> v0 = fine_view(*W, bclo, bchi, alpha, beta, bc, dcell);
+        SpecR0Args r;
+        r.old = arr(old);
+        r.ds = arr(ds);
+        r.sb = arr(sb);
+        r.spec = spec;
+        r.lo = make_int3(lo[0], lo[1], lo[2]);
+        r.idt = 1.0 / dt;
+        for (int d = 0; d < 3; d++) {
+            r.sbd[d] = d < g->dim ? beta / (W->dx[d] * W->dx[d]) * bc : 0.0;
+            for (int sd = 0; sd < 2; sd++) r.sh[d][sd] = W->per[d] || d >= g->dim ? nullptr : W->sh[d][sd];
+        }
+        r.f = W->f;
+        r.x = W->x;
+        r.slots = slot_group(*W, 0);
+        const int ntx = (n[0] / 2 + RZ2_X - 1) / RZ2_X, nty = (n[1] + RZ2_Y - 1) / RZ2_Y;
+        const void* kp = v0.c0uni ? (const void*)&k_spec_rhs_r0<true> : (const void*)&k_spec_rhs_r0<false>;
+        const int zc = stream_zc(ctx, kp, RZ2_X * RZ2_Y, n, ntx * nty);
+        const dim3 gz((unsigned)(ntx * nty * ((n[2] + zc - 1) / zc)));
+        if (v0.c0uni)
+            hipLaunchKernelGGL(k_spec_rhs_r0<true>, bz(ctx, gz), dim3(RZ2_X * RZ2_Y), 0, ctx->stream, v0, r, zc, ntx,
+                               nty, ctx->bt);
+        else
+            hipLaunchKernelGGL(k_spec_rhs_r0<false>, bz(ctx, gz), dim3(RZ2_X * RZ2_Y), 0, ctx->stream, v0, r, zc, ntx,
+                               nty, ctx->bt);
+        VD_CHECK_LAUNCH();
+        W->r0_done = true;
+    } else {
+        SpecRhsArgs r;
+        r.old = arr(old);
+        r.ds = arr(ds);
+        r.sb = arr(sb);
+        r.spec = spec;
+        r.lo = make_int3(lo[0], lo[1], lo[2]);
+        r.nx = n[0];
+        r.ny = n[1];
+        r.nz = n[2];
+        r.idt = 1.0 / dt;
+        hipLaunchKernelGGL(k_spec_rhs, bz(ctx, grid1d(nc, 256)), dim3(256), 0, ctx->stream, r, W->f, W->x, ctx->bt);
+        VD_CHECK_LAUNCH();
+        VD_TRY(side_setup(ctx, *W, bclo, bchi, beta, bc, dcell, constd));
+    }
+    const bool floor_on = prm->bound_specden && spec != ns;
+    const double floor_val = spec == E ? prm->min_electron_density : prm->min_species_density;
+    // a constant-coefficient solve whose solution nothing reads before the
+    // copy-back: its fused sweeps write each iterate to the state as well
+    // (when the component's extent has 32-bit byte offsets)
+    SweepOut so{};
+    if (constd && spec != ns && !(spec == E && grad)) {
+        const Arr<double> P = arr(phi);
+        const int64_t ext = (int64_t)(n[0] - 1) + (int64_t)P.js * (n[1] - 1) + (int64_t)P.ks * (n[2] - 1) + 1;
+        if (ext * 8 < ((int64_t)1 << 31)) {
+            so.p = &P(lo[0], lo[1], lo[2], spec);
+            so.js = P.js;
+            so.ks = P.ks;
+            so.bytes = (unsigned)(ext * 8);
+            so.floor_on = floor_on;
+            so.floor_val = floor_val;
+        }
+    }
+    W->post = so;
     // smoother-only iterations when the operator is diagonally dominant
     // enough, else V-cycles (on harmonic face arrays of the cell coefficient)
     int st = mg_iterate(ctx, *W, constd ? 1 : 2, alpha, beta, bc, dcell, bclo, bchi, 2, prm->linsolve_reltol,
                         prm->linsolve_abstol, prm->linsolve_maxiter, prm->mg_pre_smooth, prm->mg_post_smooth,
                         info);
+    W->post = SweepOut{};
     if (st != VD_OK && st != VD_ENOCONV) return st;
     SpecPostArgs p;
     p.x = W->x;
@@ -4249,8 +4550,8 @@ static int implicit_solve_body(vd_ctx* ctx, const vd_mech* m, int spec, const vd
     p.spec = spec;
     p.ns = ns;
     p.E = E;
-    p.floor_on = prm->bound_specden && spec != ns;
-    p.floor_val = spec == E ? prm->min_electron_density : prm->min_species_density;
+    p.floor_on = floor_on;
+    p.floor_val = floor_val;
     p.energy = spec == ns;
     p.min_te = prm->min_electron_temp;
     p.lo = make_int3(lo[0], lo[1], lo[2]);
@@ -4263,7 +4564,16 @@ static int implicit_solve_body(vd_ctx* ctx, const vd_mech* m, int spec, const vd
         VD_TRY(grad_faces(ctx, *W, bclo, bchi, grad, box, 1.0));
         hipLaunchKernelGGL(k_spec_post<false>, bz(ctx, grid1d(nc, 256)), dim3(256), 0, ctx->stream, p, ctx->bt);
     } else {
-        hipLaunchKernelGGL(k_spec_post<true>, bz(ctx, grid1d(nc, 256)), dim3(256), 0, ctx->stream, p, ctx->bt);
+        // the cases whose final iterate the fused sweeps did not write
+        int pc[VD_MAX_BATCH], npc = 0;
+        for (int z = 0; z < ctx->bn; z++)
+            if (!((W->post_mask >> ctx->bt.idx[z]) & 1u)) pc[npc++] = ctx->bt.idx[z];
+        if (npc == ctx->bn) {
+            hipLaunchKernelGGL(k_spec_post<true>, bz(ctx, grid1d(nc, 256)), dim3(256), 0, ctx->stream, p, ctx->bt);
+        } else if (npc > 0) {
+            BatchScope some(ctx, pc, npc);
+            hipLaunchKernelGGL(k_spec_post<true>, bz(ctx, grid1d(nc, 256)), dim3(256), 0, ctx->stream, p, ctx->bt);
+        }
     }
     VD_CHECK_LAUNCH();
     return st;
