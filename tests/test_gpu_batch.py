@@ -29,6 +29,20 @@ def _alone(case, nsteps):
 
 @pytest.mark.parametrize("dim,n,prec", [(2, 32, 0), (2, 64, 1), (3, 32, 0), (3, 32, 1), (3, 64, 1)])
 def test_batched_cases_equal_their_runs_alone(dim, n, prec):
+    _batched_vs_alone(dim, n, prec)
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_batched_fused_sweeps_equal_their_runs_alone(prec, monkeypatch):
+    """The fused finest-level passes at 64^3 (16-plane chunks, which the
+    default chooses from 192^3 up): the species solves' first sweep launched
+    with the rhs pass, and the sweeps' copy-back into the state, per case."""
+    monkeypatch.setenv("VD_FUSED_ZC", "16")
+    monkeypatch.setattr(serve, "_REGION_BYTES", {})  # the fused passes' second solution buffer
+    _batched_vs_alone(3, 64, prec)
+
+
+def _batched_vs_alone(dim, n, prec):
     reqs, cs = _cases(dim, n, prec, 8)
     print("batch: %dD %d^%d, mg_precision %d, 3 cases" % (dim, n, dim, prec), flush=True)
     nbytes = serve.case_region_bytes(0, cs[0])
