@@ -86,3 +86,4 @@ def test_streamed_kernels_whole_steps_match_row_kernels():
         return st
 
     assert np.array_equal(_with_stream(True, run), _with_stream(False, run))
+

@@ -3,6 +3,9 @@
 // exponentials included).  Per-thread source accumulators live in LDS
 // columns (species-major, thread-minor: conflict-free, indexable by the
 // table's species ids); densities are re-read through L1/L2.
+#include <cstdlib>
+#include <cstring>
+
 #include "vd_internal.hpp"
 
 namespace vd {
@@ -415,11 +418,68 @@ struct JouleArgs {
     }
 };
 
+// the current density of the electrons on one face times e (charge *
+// (mu n_e E - D grad n_e)) from the face's two cells ol, orr (offsets in
+// the state) and its E and grad n_e (Source/ElecEnergySource.cpp:84-142);
+// Te and n_e of the two cells come in as values
+template <bool STATE, int DIM>
+__device__ __forceinline__ double joule_cur(const JouleArgs& a, const DevMech& M, const double* S, int64_t N,
+                                            double tl, double tr, double nl, double nr, double efield_face,
+                                            double gradne_face, int ol, int orr, double charge) {
+    const int ns = M.nspec, eidx = M.e_idx;
+    double etemp = 0.5 * (tl + tr);
+    double ne = 0.5 * (nl + nr);
+    double mu, dcoeff;
+    if (a.const_ele_trans || STATE) {
+        double ndens = 0.0;
+        for (int sp = 0; sp < ns; sp++) ndens += 0.5 * (S[(int64_t)sp * N + ol] + S[(int64_t)sp * N + orr]);
+        if (!STATE || a.const_ele_trans) {
+            mu = a.ele_mob / ndens;
+            dcoeff = a.ele_diff / ndens;
+        } else {
+            // specMob / specDiff of the face-averaged state
+            // (ElecEnergySource.cpp:102-131)
+            double es = 0.0;
+            for (int q = 0; q < DIM; q++) {
+                const double e = 0.5 * (S[(int64_t)(VD_EFX(ns) + q) * N + ol] +
+                                        S[(int64_t)(VD_EFX(ns) + q) * N + orr]);
+                es += e * e;
+            }
+            mu = spec_mob_st(M, eidx, etemp, a.Tg, ndens, sqrt(es));
+            dcoeff = spec_diff_st(M, eidx, etemp, a.Tg, ndens, sqrt(es));
+        }
+    } else {
+        mu = spec_mob(M, eidx);
+        dcoeff = spec_diff(M, eidx, etemp, a.Tg);
+    }
+    return charge * (mu * ne * efield_face - dcoeff * gradne_face);
+}
+
+// the face of direction idim and side f of cell (i, j, k) whose E and grad
+// n_e a cell's Joule heating takes -- on a domain wall the face next to it
+// (ElecEnergySource.cpp:74-83) -- and its left and right cells
+__device__ __forceinline__ void joule_face_cells(const JouleArgs& a, int idim, int f, int i, int j, int k,
+                                                 int face[3], int& ol, int& orr) {
+    const int dlo[3] = {a.dlo.x, a.dlo.y, a.dlo.z}, dhi[3] = {a.dhi.x, a.dhi.y, a.dhi.z};
+    face[0] = i;
+    face[1] = j;
+    face[2] = k;
+    face[idim] += f;
+    if (face[idim] == dlo[idim]) face[idim] += 1;
+    if (face[idim] == dhi[idim] + 1) face[idim] -= 1;
+    int lc[3] = {face[0], face[1], face[2]};
+    lc[idim] -= 1;
+    ol = a.sb.off(lc[0], lc[1], lc[2]);
+    orr = a.sb.off(face[0], face[1], face[2]);
+}
+
 // Vidyut::compute_elecenergy_source (Source/ElecEnergySource.cpp:65-171)
 // STATE: the electron coefficients are fits of the face state (else
-// constants / Te only; a.state_trans mirrors it).  DIM: the face loop is
-// unrolled, so every face's loads are issued before the first is used;
-// the sum runs over the faces in the same order as a loop over a.dim
+// constants / Te only; a.state_trans mirrors it).  DIM: the dimension.
+// The faces are taken one direction at a time (an unrolled face loop holds
+// every face's addresses at once and runs at lower occupancy, no faster;
+// a z-streamed form that keeps the column in registers ran slower still,
+// its chain of dependent loads per thread exposed)
 template <bool STATE, int DIM>
 __global__ void __launch_bounds__(256) k_joule(JouleArgs a, Batch bt) {
     // a stays unmodified; the case's offset is applied where each array is read
@@ -435,51 +495,22 @@ __global__ void __launch_bounds__(256) k_joule(JouleArgs a, Batch bt) {
     k += a.lo.z;
     const int ns = M.nspec, eidx = M.e_idx;
     const double charge = M.charge[eidx] * VD_ECHARGE;
-    const int dlo[3] = {a.dlo.x, a.dlo.y, a.dlo.z}, dhi[3] = {a.dhi.x, a.dhi.y, a.dhi.z};
     const double* S = rb(a.sb.p, co);
     const int64_t N = a.sb.ns;
     double jheat = 0.0;
-#pragma unroll
+#pragma unroll 1
     for (int idim = 0; idim < DIM; idim++) {
         const Arr<double>& EF = idim == 0 ? a.ef0 : (idim == 1 ? a.ef1 : a.ef2);
         const Arr<double>& GN = idim == 0 ? a.gn0 : (idim == 1 ? a.gn1 : a.gn2);
 #pragma unroll
         for (int f = 0; f < 2; f++) {
-            int face[3] = {i, j, k};
-            face[idim] += f;
-            if (face[idim] == dlo[idim]) face[idim] += 1;
-            if (face[idim] == dhi[idim] + 1) face[idim] -= 1;
-            int lc[3] = {face[0], face[1], face[2]};
-            lc[idim] -= 1;
-            const int ol = a.sb.off(lc[0], lc[1], lc[2]), orr = a.sb.off(face[0], face[1], face[2]);
-            double etemp = 0.5 * (S[VD_ETEMP(ns) * N + ol] + S[VD_ETEMP(ns) * N + orr]);
-            double ne = 0.5 * (S[(int64_t)eidx * N + ol] + S[(int64_t)eidx * N + orr]);
-            double efield_face = at(EF, co, face[0], face[1], face[2]);
-            double gradne_face = at(GN, co, face[0], face[1], face[2]);
-            double mu, dcoeff;
-            if (a.const_ele_trans || STATE) {
-                double ndens = 0.0;
-                for (int sp = 0; sp < ns; sp++) ndens += 0.5 * (S[(int64_t)sp * N + ol] + S[(int64_t)sp * N + orr]);
-                if (!STATE || a.const_ele_trans) {
-                    mu = a.ele_mob / ndens;
-                    dcoeff = a.ele_diff / ndens;
-                } else {
-                    // specMob / specDiff of the face-averaged state
-                    // (ElecEnergySource.cpp:102-131)
-                    double es = 0.0;
-                    for (int q = 0; q < DIM; q++) {
-                        const double e = 0.5 * (S[(int64_t)(VD_EFX(ns) + q) * N + ol] +
-                                                S[(int64_t)(VD_EFX(ns) + q) * N + orr]);
-                        es += e * e;
-                    }
-                    mu = spec_mob_st(M, eidx, etemp, a.Tg, ndens, sqrt(es));
-                    dcoeff = spec_diff_st(M, eidx, etemp, a.Tg, ndens, sqrt(es));
-                }
-            } else {
-                mu = spec_mob(M, eidx);
-                dcoeff = spec_diff(M, eidx, etemp, a.Tg);
-            }
-            double cur = charge * (mu * ne * efield_face - dcoeff * gradne_face);
+            int face[3], ol, orr;
+            joule_face_cells(a, idim, f, i, j, k, face, ol, orr);
+            const double efield_face = at(EF, co, face[0], face[1], face[2]);
+            const double gradne_face = at(GN, co, face[0], face[1], face[2]);
+            const double cur = joule_cur<STATE, DIM>(a, M, S, N, S[VD_ETEMP(ns) * N + ol], S[VD_ETEMP(ns) * N + orr],
+                                                     S[(int64_t)eidx * N + ol], S[(int64_t)eidx * N + orr],
+                                                     efield_face, gradne_face, ol, orr, charge);
             jheat += cur * efield_face;
         }
     }
