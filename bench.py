@@ -436,7 +436,7 @@ def main():
     ap.add_argument("--only", default="", help="comma-separated subset of: mixed,discharge,poisson,chemistry,generated,batching,serving")
     ap.add_argument("--poisson-n", type=int, default=512)
     ap.add_argument("--chem-n", type=int, default=256)
-    ap.add_argument("--serve-requests", type=int, default=128)
+    ap.add_argument("--serve-requests", type=int, default=200)
     ap.add_argument("--serve-rate", type=float, default=4.0,
                     help="Poisson arrival rate of the serving leg (1/s; the default keeps the device saturated)")
     ap.add_argument("--serve-max-batch", type=int, default=16, help="cases per same-shape batch (1: no batching)")
@@ -537,8 +537,11 @@ def main():
     if want("serving") and args.serve_requests > 0:
         from vidyut3d_amd import serve
         log("serving loop: %d requests, Poisson arrivals at %.2f/s" % (args.serve_requests, args.serve_rate))
-        configs["serving"], served = serve.bench(dev_index=0, requests=args.serve_requests, rate_hz=args.serve_rate,
-                                                 sequential=args.serve_sequential, max_batch=args.serve_max_batch)
+        configs["serving"], served = serve.bench(
+            dev_index=0, requests=args.serve_requests, rate_hz=args.serve_rate, sequential=args.serve_sequential,
+            max_batch=args.serve_max_batch,
+            progress=lambda done, waiting, t: log("serving: %d of %d requests done, %d waiting, %.0f s"
+                                                  % (done, args.serve_requests, waiting, t)))
         log("serving: p50 %.2f s, p99 %.2f s, aggregate %.3g cell-steps/s"
             % (configs["serving"]["p50_time_to_solution_s"], configs["serving"]["p99_time_to_solution_s"],
                configs["serving"]["aggregate_cell_steps_per_s"]))
@@ -579,11 +582,26 @@ def main():
                     e = compare.state_errors(ref, st, c.mech.nspec, c.geom.dim)
                     q = max(e, key=e.get)
                     row[key] = {"worst": e[q], "field": names[q]}
+                if max(row["served_mixed"]["worst"], row["alone_fp64"]["worst"]) > 1e-8:
+                    # how far the oracle itself moves when its initial state
+                    # moves by 1e-14 (relative): the case's own sensitivity
+                    cp = r.make_case(0)
+                    st0 = np.array(cp.state, copy=True)
+                    st0 *= 1.0 + 1e-14 * np.random.default_rng(r.rid).standard_normal(st0.shape)
+                    cp.state = st0
+                    ref_p, _ = O.evolve(cp, r.nsteps)
+                    e = compare.state_errors(ref, ref_p, c.mech.nspec, c.geom.dim)
+                    q = max(e, key=e.get)
+                    row["oracle_perturbed_1e-14"] = {"worst": e[q], "field": names[q]}
+                    del ref_p
                 rows.append(row)
                 r.state = r.state_fp64 = None
-                log("serving request %d (%s) vs oracle: served %.2e (%s), FP64 alone %.2e (%s)"
+                log("serving request %d (%s) vs oracle: served %.2e (%s), FP64 alone %.2e (%s)%s"
                     % (r.rid, row["shape"], row["served_mixed"]["worst"], row["served_mixed"]["field"],
-                       row["alone_fp64"]["worst"], row["alone_fp64"]["field"]))
+                       row["alone_fp64"]["worst"], row["alone_fp64"]["field"],
+                       "; oracle vs oracle from a 1e-14 perturbed start %.2e (%s)"
+                       % (row["oracle_perturbed_1e-14"]["worst"], row["oracle_perturbed_1e-14"]["field"])
+                       if "oracle_perturbed_1e-14" in row else ""))
             configs["serving"]["rel_l2_vs_fp64_oracle"] = {
                 "worst_served_mixed": max(x["served_mixed"]["worst"] for x in rows),
                 "worst_alone_fp64": max(x["alone_fp64"]["worst"] for x in rows),

@@ -313,13 +313,18 @@ class Server:
         self._start(g)
         return g
 
-    def run(self, requests, t0=None):
+    def run(self, requests, t0=None, progress=None, every_s=30.0):
         """Serves `requests` (arrival times relative to t0); returns them
-        with start/done filled."""
+        with start/done filled.  progress(finished, waiting, elapsed) is
+        called about every every_s seconds."""
         pending = sorted(requests, key=lambda r: r.arrival)
         t0 = time.perf_counter() if t0 is None else t0
         waiting, finished = [], []
+        t_report = time.perf_counter()
         while len(finished) < len(requests):
+            if progress is not None and time.perf_counter() - t_report >= every_s:
+                t_report = time.perf_counter()
+                progress(len(finished), len(waiting), t_report - t0)
             now = time.perf_counter() - t0
             while pending and pending[0].arrival <= now:
                 waiting.append(pending.pop(0))
@@ -416,23 +421,26 @@ def run_sequential(dev_index, reqs, mg_precision=0):
 
 
 def bench(dev_index=0, requests=48, rate_hz=1.0, nsteps=500, slots=4, seed=7, sequential="sampled",
-          keep_cells_steps=2.0e7, max_batch=16):
+          keep_cells_steps=2.0e7, max_batch=16, max_kept=24, progress=None):
     """Serves `requests` Poisson arrivals with FP32 V-cycles (mixed
     precision) and same-shape batching; the baseline is the same request
     list in FP64, one at a time without batching (sequential="run":
     actually run; "sampled": the sum over requests of nsteps times each
     shape's measured FP64 ms/step).  Requests whose cells * steps are at
-    most keep_cells_steps keep their final state, and are run again alone
-    in FP64 (state_fp64), for a check of both against the FP64 oracle."""
+    most keep_cells_steps keep their final state (the first max_kept of
+    them in arrival order), and are run again alone in FP64 (state_fp64),
+    for a check of both against the FP64 oracle."""
     reqs = draw_requests(rate_hz, seed=seed, nsteps=nsteps, count=requests)
+    kept = 0
     for r in reqs:
-        r.keep = r.cells * r.nsteps <= keep_cells_steps
+        r.keep = r.cells * r.nsteps <= keep_cells_steps and kept < max_kept
+        kept += r.keep
     srv = Server(dev_index, slots=slots, max_batch=max_batch)
     tw = time.perf_counter()
     srv.warm(r.shape for r in reqs)
     warm_s = time.perf_counter() - tw
     try:
-        srv.run(reqs)
+        srv.run(reqs, progress=progress)
     finally:
         srv.close()
     out = summarize(reqs)
