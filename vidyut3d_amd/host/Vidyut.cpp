@@ -458,11 +458,18 @@ int Vidyut::step_body(std::vector<MultiFab>& sb, std::vector<MultiFab>& sb_old, 
             // Source/Evolve.cpp:267-287 (also reached for the electron
             // index, whose solved value it then updates): n_new = n + dt *
             // rxn, a background species' n taken from Sborder, whose species
-            // equal those of the state
+            // equal those of the state.  A run of consecutive background
+            // species is one launch (each component's update reads only its
+            // own n and rxn, so the order among them does not matter)
+            int run = 1;
+            if (ind != E)
+                while (ind + run < ns_ && desc_.is_bg[ind + run] && ind + run != E) run++;
             const vd_array4& n_old = ind == E ? out().array() : sb[0].array();  // E: the solved value
-            TRY(vd_lincomb(ctx_, &out().array(), 1.0, &n_old, ind, dt, &rxn_src_[0].array(), ind, ind, 1, &dom_));
+            TRY(vd_lincomb(ctx_, &out().array(), 1.0, &n_old, ind, dt, &rxn_src_[0].array(), ind, ind, run, &dom_));
             if (prm_.bound_specden)
-                TRY(vd_clamp_min(ctx_, &out().array(), &dom_, ind, prm_.min_species_density));
+                for (int q = ind; q < ind + run; q++)
+                    TRY(vd_clamp_min(ctx_, &out().array(), &dom_, q, prm_.min_species_density));
+            ind += run - 1;
         }
     }
     return VD_OK;
