@@ -591,6 +591,12 @@ enum {
     VD_KT_SPECIES = 8,    /* one whole implicit species solve */
     VD_KT_POTPOST = 9     /* the potential solve's E faces, cell E and E/N */
 };
+/* Where vd_find_time_scales reads E: the cell E components (EFX ..) of
+ * `efield` instead of the state's, until called again with NULL.  No
+ * counterpart in the reference (whose state always holds E): for a driver
+ * that leaves the newest E in the Sborder of the last step instead of
+ * copying it into the state between the steps of one call. */
+int vd_ctx_set_cfl_efield(vd_ctx* ctx, const vd_array4* efield);
 /* times one family (VD_KT_NONE: none); resets the totals */
 int vd_ctx_set_kernel_timing(vd_ctx* ctx, int family);
 /* times every family whose bit (1 << family) is set; resets the totals */

@@ -691,6 +691,13 @@ int vd_ctx_set_kernel_timing_mask(vd_ctx* ctx, unsigned mask) {
     return VD_OK;
 }
 
+int vd_ctx_set_cfl_efield(vd_ctx* ctx, const vd_array4* efield) {
+    if (!ctx) return VD_EINVAL;
+    ctx->cfl_ef_set = efield != nullptr;
+    if (efield) ctx->cfl_ef = *efield;
+    return VD_OK;
+}
+
 int vd_ctx_set_kernel_timing(vd_ctx* ctx, int family) {
     if (!ctx || family < 0 || family >= 32) return VD_EINVAL;
     return vd_ctx_set_kernel_timing_mask(ctx, family == VD_KT_NONE ? 0u : 1u << family);

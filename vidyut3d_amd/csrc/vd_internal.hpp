@@ -361,6 +361,9 @@ struct vd_ctx {
     double* hnorm = nullptr;
     double* hnorm_d = nullptr;  // its device address
     unsigned long long norm_seq = 0;
+    // vd_ctx_set_cfl_efield: the array the CFL reduction reads E from
+    vd_array4 cfl_ef{};
+    bool cfl_ef_set = false;
     // kernel timing: families selected by bit, one event pair per launch
     // (families may nest: a species solve contains sweep launches)
     struct KtEvent {

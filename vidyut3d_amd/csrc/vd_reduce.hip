@@ -10,6 +10,7 @@ namespace vd {
 struct CflArgs {
     const DevMech* __restrict__ M;
     Arr<double> phi;
+    Arr<double> ef;  // holds E (EFX ..): phi, or the array vd_ctx_set_cfl_efield names
     int dim, const_ele_trans;
     int state_trans;  // the electron mobility or diffusivity is a fit of the state
     double Tg, ele_mob, ele_diff;
@@ -19,6 +20,7 @@ struct CflArgs {
     __device__ void rebase(int64_t o) {
         M = rb(M, o);
         vd::rebase(phi, o);
+        vd::rebase(ef, o);
         out = rb(out, o);
     }
 };
@@ -52,7 +54,7 @@ __global__ void __launch_bounds__(256) k_cfl(CflArgs a, Batch bt) {
         double etemp = P[VD_ETEMP(ns) * N + o];
         double es = 0.0;
         for (int d = 0; d < a.dim; d++) {
-            double v = P[(VD_EFX(ns) + d) * N + o];
+            double v = a.ef(i + a.lo.x, j + a.lo.y, k + a.lo.z, VD_EFX(ns) + d);
             es += v * v;
         }
         double emag = sqrt(es);
@@ -105,6 +107,7 @@ int find_time_scales(vd_ctx* ctx, const vd_mech* m, const vd_array4* phi, const 
     CflArgs a;
     a.M = m->dev;
     a.phi = arr(phi);
+    a.ef = ctx->cfl_ef_set ? arr(&ctx->cfl_ef) : a.phi;
     a.dim = g->dim;
     a.const_ele_trans = prm->const_ele_trans;
     a.state_trans = m->hdev.mfit_kind[m->hdev.e_idx] == VD_TR_FIT || m->hdev.dfit_kind[m->hdev.e_idx] == VD_TR_FIT;
@@ -254,6 +257,7 @@ extern "C" int vd_find_time_scales(vd_ctx* ctx, const vd_mech* m, const vd_array
                                    double out[3]) {
     if (!ctx || !m || !phi_new || !box || !geom || !prm || !out) return VD_EINVAL;
     VD_TRY(check_extents("find_time_scales", box, {phi_new}));
+    if (ctx->cfl_ef_set) VD_TRY(check_extents("find_time_scales (E)", box, {&ctx->cfl_ef}));
     return find_time_scales(ctx, m, phi_new, box, geom, prm, out);
 }
 

@@ -487,6 +487,7 @@ int Vidyut::evolve_one_corrector(int nsteps) {
     MultiFab* P = &phi_new_[0];
     std::vector<MultiFab>* G[2] = {&Sborder_, &Swork_};
     MultiFab* cur = P;
+    MultiFab* e_src = nullptr;  // the previous step's Sborder, which holds the state's E
     for (int step = 0; step < nsteps; step++) {
         const bool first = step == 0, last = step == nsteps - 1;
         std::vector<MultiFab>& sb = first ? *G[0] : (cur == &(*G[0])[0] ? *G[0] : *G[1]);
@@ -496,21 +497,28 @@ int Vidyut::evolve_one_corrector(int nsteps) {
         out_ = out;
         phi_old_stale_ = false;
         const Real cur_time = t_new_;
-        TRY(ComputeDt(cur_time, dt_));
+        // the time scales read E where the last step left it
+        if (e_src) TRY(vd_ctx_set_cfl_efield(ctx_, &e_src->array()));
+        const int dst = ComputeDt(cur_time, dt_);
+        if (e_src) TRY(vd_ctx_set_cfl_efield(ctx_, nullptr));
+        TRY(dst);
         const Real dt = dt_;
         // species, EEN, ETEMP into Sborder (a copy in the first step, the
         // ghost cells only after)
         TRY(vd_fill_patch(ctx_, &cur->array(), &sb[0].array(), &dom_, &geom_, 0, ns_ + 2));
         TRY(step_body(sb, sb, cur_time, dt, true));
-        // what the step wrote into Sborder: E, which the next step's time
-        // scales read from the state, and -- for phi_new only, since no step
-        // reads them -- the potential and E/N
+        // what the step wrote into Sborder: the potential, E and E/N go to
+        // phi_new in the last step; within the call no step reads them from
+        // the state but the next step's time scales, which read E from this
+        // Sborder (it is the next step's output buffer, written only after
+        // those time scales)
         const int nef = dim_;
         if (last) {
             TRY(out->copy_from(sb[0], VD_POT(ns_), VD_POT(ns_), 1 + nef));
             TRY(out->copy_from(sb[0], VD_REF(ns_), VD_REF(ns_), 1));
+            e_src = nullptr;
         } else {
-            TRY(out->copy_from(sb[0], VD_EFX(ns_), VD_EFX(ns_), nef));
+            e_src = &sb[0];
         }
         // the components no step writes keep their values, so each buffer
         // needs them once per call: the two ghosted buffers receive them in
